@@ -1,0 +1,233 @@
+/*
+ * mdffgpu.h -- C ABI of the MI355X (gfx950) force/field library.
+ *
+ * This library replaces the body of MDFF's force/field evaluation: the
+ * subroutine `engforce_driver` (src/field.f90:1111-1166) and everything it
+ * calls -- n-m Lennard-Jones (engforce_nmlj_pbc, src/field.f90:1181-1345),
+ * BMHFTD with Tang-Toennies damping (engforce_bmhftd_pbc, :2876-3026), Morse
+ * (engforce_morse_pbc, :3039-3185), the polarizable-ion SCF
+ * (get_moments :3686, moment_from_pola_scf :3200-3458, induced_moment
+ * :1716-1803, extrapolate_dipole_poly/aspc :3807-3992) and the Ewald
+ * multipole sum (multipole_ES :1817-2054, multipole_ES_dir :2057-2643,
+ * multipole_ES_rec :2646-2873) with per-site electric fields and EFG tensors.
+ *
+ * The reference has no FFI: engforce_driver takes no arguments and reads and
+ * writes Fortran module globals.  Every entry point below takes those globals
+ * as plain pointers and sizes instead, with Fortran (column-major) layouts so
+ * a Fortran host passes its arrays through ISO_C_BINDING without copies.  No
+ * entry point throws; each returns an mdffgpu_status, and the message of the
+ * last failure is available from mdffgpu_last_error().
+ *
+ * Threading: one context per device, calls on a context are synchronous with
+ * respect to the host (the *_device entry point is asynchronous on the stream
+ * it is given) and are not re-entrant, like the reference's single-threaded
+ * MPI ranks.
+ */
+#ifndef MDFFGPU_H
+#define MDFFGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDFFGPU_NTYPEMAX 16 /* ntypemax, src/config.f90 */
+#define MDFFGPU_TT_MAX_ORDER 8 /* maximum_of_TT_expansion, src/tt_damp.f90:8 */
+#define MDFFGPU_ABI_VERSION 1
+
+typedef enum mdffgpu_status {
+  MDFFGPU_OK = 0,
+  MDFFGPU_ERR_ARG = 1,          /* invalid argument or parameter table */
+  MDFFGPU_ERR_DEVICE = 2,       /* HIP runtime failure */
+  MDFFGPU_ERR_ALLOC = 3,        /* device allocation failure */
+  MDFFGPU_ERR_NLIST = 4,        /* neighbour list capacity exceeded
+                                   (src/tools.f90:339-342) */
+  MDFFGPU_ERR_TT_ORDER = 5,     /* Tang-Toennies order > 8
+                                   (src/field.f90:355-358) */
+  MDFFGPU_ERR_UNSUPPORTED = 6,  /* quadrupole tasks, Wannier centres, ... */
+  MDFFGPU_ERR_NO_LIBRARY = 7    /* reported by host wrappers only */
+} mdffgpu_status;
+
+typedef enum mdffgpu_precision {
+  MDFFGPU_PREC_FP64 = 0,  /* plain FP64 path: parity with the reference */
+  MDFFGPU_PREC_MIXED = 1  /* FP32 pair / erfc / k-space arithmetic, FP64
+                             energy, virial and SCF-residual reductions */
+} mdffgpu_precision;
+
+typedef enum mdffgpu_ext_algo {
+  MDFFGPU_EXT_POLY = 0, /* algo_ext_dipole = 'poly' */
+  MDFFGPU_EXT_ASPC = 1  /* algo_ext_dipole = 'aspc' */
+} mdffgpu_ext_algo;
+
+/*
+ * Force-field parameters as engforce_driver sees them, i.e. AFTER
+ * field_check_tag (src/field.f90:282-433: tables symmetrised, isotropic
+ * polarizabilities expanded, task_coul set) and ewald_param
+ * (src/field.f90:860-908: cutlongrange clamped to half the smallest cell
+ * width, alphaES/kES chosen when lautoES).
+ *
+ * Index conventions (0-based, Fortran column-major, so a Fortran host may
+ * pass its module arrays as they are):
+ *   pair table      t(it,jt)        -> [it + 16*jt]
+ *   dip(it,a)                       -> [it + 16*a]
+ *   poldip(it,a,b)                  -> [it + 16*(a + 3*b)]
+ *   damping (a,b,c) e.g. ldip_damping(ita,ita,jta)
+ *                                   -> [a + 16*(b + 16*c)]
+ * Fortran logicals are passed as int32 0/1.
+ */
+typedef struct mdffgpu_params {
+  int32_t natm;  /* number of atoms */
+  int32_t ntype; /* number of species, 1..16 */
+
+  /* &controltag (src/control.f90:46-66) */
+  int32_t lnmlj, lmorse, lbmhft, lbmhftd, lcoulomb, lsurf;
+  /* calc: 0 any other mode, 1 'md', 2 'opt'.  In 'md' and 'opt' the first
+     SCF iteration takes the extrapolated dipoles (src/field.f90:3321); the
+     ASPC corrector runs in 'md' only (src/field.f90:3370). */
+  int32_t calc_mode;
+  int32_t trunc; /* ctrunc: 0 notrunc, 1 linear, 2 quadratic */
+  double cutshortrange;
+  double cutlongrange; /* after the clamp of src/field.f90:879-881 */
+  double skindiff;     /* Verlet skin (src/control.f90:225) */
+  double coul_unit;    /* 14.3996441494161, or 1 with lreduced */
+  double press_unit;   /* 0.00624150964712042, or 1 with lreduced */
+
+  /* n-m Lennard-Jones (src/field.f90:84-95) */
+  double plj[256], qlj[256], epslj[256], sigmalj[256];
+  /* Morse (src/field.f90:103-107) */
+  double epsmor[256], sigmamor[256], rhomor[256];
+  /* BMHFTD (src/field.f90:118-122) */
+  double Abmhftd[256], Bbmhftd[256], Cbmhftd[256], Dbmhftd[256],
+      BDbmhftd[256];
+
+  /* per-type moments (src/field.f90:128-136) */
+  double qch[16];
+  double dip[48];
+  double poldip[144];
+  int32_t ldip_polar[16];
+
+  /* Tang-Toennies dipole damping (src/field.f90:144-148) */
+  int32_t ldip_damping[4096];
+  double dip_pol_damp_b[4096];
+  double dip_pol_damp_c[4096];
+  int32_t dip_pol_damp_k[4096];
+
+  /* Ewald (src/field.f90:163-167) */
+  double alphaES;
+  int32_t kES[3];
+  /* task_coul(1:3): charge-charge, charge-dipole, dipole-dipole.  The
+     quadrupole tasks 4..6 are not supported. */
+  int32_t task_coul[3];
+
+  /* polarizable-ion SCF (src/field.f90:139-143, 150-151) */
+  double conv_tol_ind;
+  int32_t min_scf_pol_iter, max_scf_pol_iter;
+  int32_t extrapolate_order; /* 0..1 for 'poly', 0..5 for 'aspc' */
+  int32_t algo_ext_dipole;   /* mdffgpu_ext_algo */
+  int32_t doefield, doefg;
+
+  /* this library */
+  int32_t precision; /* mdffgpu_precision */
+  int32_t reserved[7];
+} mdffgpu_params;
+
+/* Scalar outputs of one evaluation: the thermodynamic module variables
+   written by the hot path (src/thermo.f90:41-76), the stress tensors
+   tau_nonb/tau_coul (src/config.f90:51-52, Fortran (3,3) order) and the SCF
+   statistics. */
+typedef struct mdffgpu_energies {
+  double u_lj, vir_lj;
+  double u_morse, vir_morse;
+  double u_bmhft, vir_bmhft;
+  double u_coul;       /* includes u_pol*coul_unit (src/field.f90:1983) */
+  double u_pol;        /* internal units, without coul_unit */
+  double pvirial_coul; /* tr(tau_coul)/3*press_unit (src/field.f90:2050) */
+  double tau_nonb[9];
+  double tau_coul[9];
+  int32_t n_scf;       /* SCF iterations of the last evaluation */
+  int32_t n_es_calls;  /* Ewald evaluations of the last evaluation */
+  double rmsd;         /* final SCF residual (src/field.f90:3378-3400) */
+} mdffgpu_energies;
+
+typedef struct mdffgpu_ctx mdffgpu_ctx;
+
+/* ---------------------------------------------------------------------
+ * Set-up.  Replaces the allocations of initialize_coulomb
+ * (src/field.f90:1596-1662: k-vector mesh kpoint_sum_init, ef_t, efg_t,
+ * mu_t, theta_t, dipia_ind_t) and the per-atom tables of typeinfo_init
+ * (src/read_pos.f90:162-218).  itype is 1-based like config%itype; A is
+ * simu_cell%A (column j = lattice vector j).
+ * ------------------------------------------------------------------- */
+int mdffgpu_create(const mdffgpu_params* params, const int32_t* itype,
+                   const double* A, mdffgpu_ctx** out);
+
+/* Replaces finalize_coulomb (src/field.f90:1668-1706). */
+void mdffgpu_destroy(mdffgpu_ctx* ctx);
+
+/* New cell (e.g. NPT): recomputes B, Omega and the k-vector mesh as
+   lattice (src/lattice.f90:60-168) and kpoint_sum_init
+   (src/kspace.f90:142-207) do. */
+int mdffgpu_set_cell(mdffgpu_ctx* ctx, const double* A);
+
+/* ---------------------------------------------------------------------
+ * engforce_driver (src/field.f90:1111-1166) on host arrays.
+ *   rx,ry,rz  in      Cartesian positions (natm)
+ *   itime     in      MD step, used by the dipole extrapolation
+ *   fx,fy,fz  in/out  forces; overwritten when LJ or BMHFTD is on, and the
+ *                     Coulomb forces are added to them (src/field.f90:1220,
+ *                     2904, 1966-1968)
+ *   mu_t      out     total dipoles (natm,3)       may be NULL
+ *   theta_t   out     total quadrupoles (natm,3,3) may be NULL (zeros)
+ *   ef_t      out     electric field (natm,3), without coul_unit
+ *                     (src/field.f90:1968-1972)      may be NULL
+ *   efg_t     out     EFG (natm,3,3) * coul_unit    may be NULL
+ *   en        out     energies, virials, stress, SCF statistics
+ * ------------------------------------------------------------------- */
+int mdffgpu_engforce(mdffgpu_ctx* ctx, const double* rx, const double* ry,
+                     const double* rz, int32_t itime, double* fx, double* fy,
+                     double* fz, double* mu_t, double* theta_t, double* ef_t,
+                     double* efg_t, mdffgpu_energies* en);
+
+/* The same evaluation on device-resident SoA buffers: d_pos = [x|y|z]
+   (3*natm doubles), d_force = [fx|fy|fz] in/out, d_mu = [mx|my|mz],
+   d_ef (3*natm), d_efg (9*natm, Fortran (natm,3,3) order); any output may
+   be NULL.  stream is a hipStream_t (NULL = default stream).  Returns after
+   the scalar outputs are on the host. */
+int mdffgpu_engforce_device(mdffgpu_ctx* ctx, const double* d_pos,
+                            int32_t itime, double* d_force, double* d_mu,
+                            double* d_ef, double* d_efg,
+                            mdffgpu_energies* en, void* stream);
+
+/* Induced-dipole history dipia_ind_t(extrapolate_order+1, natm, 3)
+   (src/field.f90:182), Fortran order, for restart and parity runs. */
+int mdffgpu_get_dipole_history(mdffgpu_ctx* ctx, double* hist);
+int mdffgpu_set_dipole_history(mdffgpu_ctx* ctx, const double* hist);
+
+/* Number of half-space k vectors (src/field.f90:1648). */
+int32_t mdffgpu_num_kvectors(const mdffgpu_ctx* ctx);
+
+/* Message of the last failure on ctx (or of the last failed create when
+   ctx is NULL).  Never NULL. */
+const char* mdffgpu_last_error(const mdffgpu_ctx* ctx);
+
+/* ---------------------------------------------------------------------
+ * Kernel instrumentation (no counterpart in the reference, which only has
+ * MPI_WTIME accumulators, src/time.f90:35-73).  When enabled, every launch
+ * on the context's stream is bracketed by HIP events; the elapsed times are
+ * folded in at the next stream synchronisation of an evaluation.
+ * ------------------------------------------------------------------- */
+int mdffgpu_profile_enable(mdffgpu_ctx* ctx, int32_t on); /* also resets */
+int mdffgpu_profile_read(const mdffgpu_ctx* ctx, int32_t kernel,
+                         double* total_ms, int64_t* launches);
+const char* mdffgpu_kernel_name(int32_t kernel);
+int32_t mdffgpu_kernel_count(void);
+
+/* Number of visible HIP devices (0 without a GPU); never fails. */
+int32_t mdffgpu_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDFFGPU_H */

@@ -1,0 +1,149 @@
+// kernels.h -- kernel argument blocks and launch declarations shared by the
+// kernel files and the host orchestration (mdffgpu.hip).
+#pragma once
+
+#include "common.h"
+
+struct mdffgpu_ctx;
+
+namespace mdff {
+
+constexpr int KC_MAX = 16;      // k vectors per chunk (consecutive n3)
+constexpr int RHO_TILE = 32;    // atoms per LDS tile in the structure factor
+constexpr int APPLY_TILE = 64;  // k chunks per LDS tile in the apply
+
+enum { ES_STATIC = 0, ES_IND = 1, ES_FINAL = 2 };
+enum { SR_LJ = 1, SR_BM = 2, SR_BMD = 4, SR_MO = 8, SR_TRUNC1 = 16, SR_TRUNC2 = 32 };
+enum { TASK_QQ = 1, TASK_QD = 2, TASK_DD = 4 };
+
+// Kernel identifiers of the launch instrumentation (mdffgpu_profile_*).
+enum KernelId {
+  K_PREPARE, K_NLIST, K_SR, K_DIR_STATIC, K_DIR_IND, K_DIR_FINAL, K_RHO_CHG, K_RHO_DIP,
+  K_RHO_BOTH, K_RHO_RED, K_APPLY_F, K_APPLY_EFG, K_MOMENTS, K_COMBINE, K_INDUCED,
+  K_RESID, K_EXTRAP, K_REDUCE, K_ADD, K_MX_SR, K_MX_DIR, K_MX_PHASE, K_MX_RHO,
+  K_MX_RHO_RED, K_MX_APPLY, K_MX_COMBINE, K_BATCH, K_ADJ, K_COUNT
+};
+extern const char* const kKernelNames[K_COUNT];
+
+// RAII bracket of one launch with HIP events on the context's stream when
+// instrumentation is enabled (implemented in mdffgpu.hip).
+struct ProfScope {
+  mdffgpu_ctx* c;
+  int id;
+  hipEvent_t a, b;
+  ProfScope(mdffgpu_ctx* ctx, int kid);
+  ~ProfScope() noexcept(false);
+};
+
+// A chunk of KC consecutive k vectors (n1, n2, n3s .. n3s+KC-1) of the
+// half-space mesh of kpoint_sum_init (src/kspace.f90:162-191).
+struct KChunk {
+  int n1, n2, n3s, pad;
+};
+
+struct EsDirArgs {
+  int n;
+  const double* s;
+  const int* typ;
+  const int* cnt;
+  const int* nbr;
+  Cell cell;
+  const PairTables* T;
+  TTTable tt;
+  const double* q;
+  const double* mu;
+  double cut2, alpha;
+  int task, do_efield, do_efg;
+  double* ef;
+  double* efg;
+  double* f;
+  double* part;
+};
+
+struct RhoArgs {
+  int n, nchunk, KC, kmax[3];
+  const KChunk* chunks;
+  const double2* eph;
+  const double* q;
+  const double* mu;
+  Cell cell;
+  int atoms_per_split, kpad;
+  double2* part;
+};
+
+struct RhoRedArgs {
+  int kpad, nsplit, do_energy;
+  const double2* part;
+  const double* Ak;
+  const double* kcoe;
+  const double* kvec;
+  double2* g;
+  double* pout;
+};
+
+struct ApplyArgs {
+  int n, nchunk, KC, chunks_per_split;
+  const KChunk* chunks;
+  const double2* g;
+  const double2* eph;
+  const double* s;
+  double* part;
+};
+
+struct CombineArgs {
+  int n, nsplit, lsurf, do_efield, do_efg, dip_force;
+  const double* apart;
+  Cell cell;
+  double fpi_V, fpi_3V, selfa2, coul_unit;
+  const double* mu;
+  const double* q;
+  const double* ef_dir;
+  const double* efg_dir;
+  const double* f_dir;
+  const double* qm;  // device: qtot[3], mutot[3]
+  double* E_stat;
+  double* E;
+  double* ef_out;
+  double* efg_out;
+  double* f;
+};
+
+struct ExtrapArgs {
+  int n, nh, op, ord;
+  double b[6];
+  double w;
+  double* hist;
+  double* mu;
+  double* mup;
+  const double* E;
+  const int* typ;
+  const TypeTables* T;
+  double* part;
+};
+
+__global__ void k_prepare(int n, const double* pos, Cell cell, double* s, double2* eph);
+__global__ void k_nlist(int n, const double* s, Cell cell, double cut2, int maxnb, int* cnt,
+                        int* nbr, int* overflow);
+__global__ void k_shortrange(int n, const double* s, const int* typ, const int* cnt,
+                             const int* nbr, Cell cell, const PairTables* T, TTTable tt,
+                             double rc2, int flags, double* f, double* part);
+template <int MODE>
+__global__ void k_es_dir(EsDirArgs a);
+template <bool CHG, bool DIP>
+__global__ void k_rho64(RhoArgs a);
+__global__ void k_rho_reduce64(RhoRedArgs a);
+template <bool EFG>
+__global__ void k_apply64(ApplyArgs a);
+__global__ void k_moments(int n, const double* pos, const double* q, const double* mu,
+                          double* part);
+template <int MODE>
+__global__ void k_combine64(CombineArgs a);
+__global__ void k_induced(int n, const int* typ, const TypeTables* T, const double* E,
+                          double* mu, double* part);
+__global__ void k_scf_resid(int n, const int* typ, const TypeTables* T, const double* mu,
+                            const double* E, double* part);
+__global__ void k_extrap(ExtrapArgs a);
+__global__ void k_reduce_rows(const double* part, int nb, double* out);
+__global__ void k_add(int m, const double* a, const double* b, double* out);
+
+}  // namespace mdff

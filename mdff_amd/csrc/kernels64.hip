@@ -1,0 +1,886 @@
+// kernels64.hip -- FP64 kernels of the force/field evaluation (gfx950).
+//
+// Every kernel maps one output row to one thread, so nothing is written by
+// two threads and no atomics are needed; block partial sums go to a partial
+// buffer that k_reduce_rows sums in a fixed order, which makes every result
+// bitwise reproducible from run to run.
+#include "kernels.h"
+
+namespace mdff {
+
+// ---------------------------------------------------------------------------
+// Cartesian -> fractional (kardir with BASIS = B, src/lattice.f90:183-213)
+// and the per-atom phase bases e^{2 pi i s_a} used by the k-space kernels.
+__global__ __launch_bounds__(MDFF_BLOCK) void k_prepare(
+    int n, const double* __restrict__ pos, Cell cell, double* __restrict__ s,
+    double2* __restrict__ eph) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = pos[i], y = pos[i + n], z = pos[i + 2 * n];
+  const double* B = cell.B;
+  double s1 = x * B[0] + y * B[1] + z * B[2];
+  double s2 = x * B[3] + y * B[4] + z * B[5];
+  double s3 = x * B[6] + y * B[7] + z * B[8];
+  s[i] = s1;
+  s[i + n] = s2;
+  s[i + 2 * n] = s3;
+  double sn, cs;
+  sincospi(2.0 * s1, &sn, &cs);
+  eph[i] = make_double2(cs, sn);
+  sincospi(2.0 * s2, &sn, &cs);
+  eph[i + n] = make_double2(cs, sn);
+  sincospi(2.0 * s3, &sn, &cs);
+  eph[i + 2 * n] = make_double2(cs, sn);
+}
+
+// ---------------------------------------------------------------------------
+// Full neighbour list (replaces vnlist_pbc, src/tools.f90:274-370): every j
+// whose minimum image lies within cut (list cutoff incl. skin).  Brute force
+// over LDS tiles of fractional positions; j ascending.
+__global__ __launch_bounds__(MDFF_BLOCK) void k_nlist(
+    int n, const double* __restrict__ s, Cell cell, double cut2, int maxnb,
+    int* __restrict__ cnt, int* __restrict__ nbr, int* __restrict__ overflow) {
+  __shared__ double ts[3][MDFF_BLOCK];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool act = i < n;
+  double si1 = 0, si2 = 0, si3 = 0;
+  if (act) {
+    si1 = s[i];
+    si2 = s[i + n];
+    si3 = s[i + 2 * n];
+  }
+  int c = 0;
+  for (int base = 0; base < n; base += MDFF_BLOCK) {
+    int j = base + threadIdx.x;
+    __syncthreads();
+    if (j < n) {
+      ts[0][threadIdx.x] = s[j];
+      ts[1][threadIdx.x] = s[j + n];
+      ts[2][threadIdx.x] = s[j + 2 * n];
+    }
+    __syncthreads();
+    int m = min(MDFF_BLOCK, n - base);
+    if (act) {
+      for (int jj = 0; jj < m; ++jj) {
+        int jg = base + jj;
+        double rx, ry, rz;
+        double d2 = min_image(cell, si1 - ts[0][jj], si2 - ts[1][jj],
+                              si3 - ts[2][jj], rx, ry, rz);
+        if (jg != i && d2 <= cut2) {
+          if (c < maxnb) nbr[(size_t)c * n + i] = jg;
+          ++c;
+        }
+      }
+    }
+  }
+  if (act) {
+    cnt[i] = c;
+    if (c > maxnb) atomicMax(overflow, c);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Short-range pair forces: n-m LJ (engforce_nmlj_pbc, src/field.f90:1181),
+// BMHFTD (engforce_bmhftd_pbc, :2876) and Morse, per centre atom over its
+// full list, energies halved.  Partial rows (per block):
+//   0 u_lj 1 vir_lj 2 u_bm 3 vir_bm 4 u_mo 5 vir_mo 6..11 tau (xx yy zz xy
+//   xz yz) of the terms that contribute forces.
+__global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(
+    int n, const double* __restrict__ s, const int* __restrict__ typ,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, Cell cell,
+    const PairTables* __restrict__ T, TTTable tt, double rc2, int flags,
+    double* __restrict__ f, double* __restrict__ part) {
+  __shared__ double red[4 * 12];
+  const bool LJ = flags & SR_LJ, BM = flags & SR_BM, BMD = flags & SR_BMD,
+             MO = flags & SR_MO;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+  if (i < n) {
+    double si1 = s[i], si2 = s[i + n], si3 = s[i + 2 * n];
+    int ti = typ[i];
+    double fx = 0, fy = 0, fz = 0;
+    int c = cnt[i];
+    for (int k = 0; k < c; ++k) {
+      int j = nbr[(size_t)k * n + i];
+      double rx, ry, rz;
+      double rsq = min_image(cell, si1 - s[j], si2 - s[j + n], si3 - s[j + 2 * n],
+                             rx, ry, rz);
+      if (!(rsq < rc2)) continue;
+      int t = ti + MDFF_NT * typ[j];
+      double w = 0.0;  // force / r of the terms that contribute forces
+      if (LJ) {
+        double sr2 = T->lj_sigsq[t] / rsq;
+        double srp = pow(sr2, T->lj_ptwo[t]);
+        double srq = pow(sr2, T->lj_qtwo[t]);
+        double u = T->lj_epsp[t] * (T->lj_p[t] * srq - T->lj_q[t] * srp);
+        if (flags & SR_TRUNC1) u = u - T->lj_uc[t];
+        if (flags & SR_TRUNC2) u = u + T->lj_uc1[t] * rsq - T->lj_uc2[t];
+        double wl = T->lj_fc[t] * (srq - srp) * sr2;
+        acc[0] += 0.5 * u;
+        acc[1] += 0.5 * wl * rsq;
+        if (!BM) w += wl;
+      }
+      if (BM && T->bm_A[t] != 0.0) {
+        double ir2 = 1.0 / rsq;
+        double d = sqrt(rsq);
+        double erh = T->bm_A[t] * exp(-T->bm_B[t] * d);
+        double ir6 = ir2 * ir2 * ir2;
+        double ir8 = ir6 * ir2;
+        ir6 = ir6 * T->bm_C[t];
+        ir8 = ir8 * T->bm_D[t];
+        double ir6d = ir6, ir8d = ir8, fd6 = 0.0, fd8 = 0.0;
+        if (BMD) {
+          double f6, f8;
+          tt_damp(tt, T->bm_BD[t], 1.0, d, 6, f6, fd6);
+          tt_damp(tt, T->bm_BD[t], 1.0, d, 8, f8, fd8);
+          ir6d = ir6 * f6;
+          ir8d = ir8 * f8;
+        }
+        double ir7 = 6.0 * ir6d / d;
+        double ir9 = 8.0 * ir8d / d;
+        acc[2] += 0.5 * (erh - ir6d - ir8d);
+        double wb = (T->bm_B[t] * erh - ir7 - ir9 + ir6 * fd6 + ir8 * fd8) / d;
+        acc[3] += 0.5 * wb * rsq;
+        w += wb;
+      }
+      if (MO) {
+        double d = sqrt(rsq);
+        double X = exp(-d * T->mo_rho[t]) * T->mo_rs[t];
+        acc[4] += 0.5 * T->mo_eps[t] * ((X - 1.0) * (X - 1.0) - 1.0);
+        double wm = 2.0 * T->mo_eps[t] * T->mo_rho[t] * X * (X - 1.0) / d;
+        acc[5] += 0.5 * wm * rsq;
+        w += wm;
+      }
+      double gx = w * rx, gy = w * ry, gz = w * rz;
+      fx += gx;
+      fy += gy;
+      fz += gz;
+      acc[6] += 0.5 * rx * gx;
+      acc[7] += 0.5 * ry * gy;
+      acc[8] += 0.5 * rz * gz;
+      acc[9] += 0.5 * rx * gy;
+      acc[10] += 0.5 * rx * gz;
+      acc[11] += 0.5 * ry * gz;
+    }
+    f[i] = fx;
+    f[i + n] = fy;
+    f[i + 2 * n] = fz;
+  }
+  block_sum<12>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 12; ++k) part[(size_t)k * gridDim.x + blockIdx.x] = acc[k];
+}
+
+// ---------------------------------------------------------------------------
+// Ewald real space, multipole_ES_dir (src/field.f90:2057-2643), charges and
+// dipoles, per centre atom i over its full list.  The reference visits each
+// pair once as (ia<ja) and updates both atoms; here each directed pair
+// (i,j) adds i's share, with r = r_i - r_j:
+//   field      qq  q_j r (F1 - F1d(p_i)) / d^3        dd  T2 mu_j
+//   EFG        qq  -q_j T2                            dd  T3 mu_j
+//   force      qq  q_i q_j r F1 / d^3                 dd  mu_i T3 mu_j
+//              qd  q_i (T2 - T2d(p_j)) mu_j - q_j (T2 - T2d(p_i)) mu_i
+//   energy     qq  q_i q_j T0 / 2                     dd  -mu_i T2 mu_j / 2
+//              qd  q_j (T1 - T1d(p_i)) . mu_i
+// where p_i are the damping parameters the reference uses for atom i of the
+// pair (dmp_* slot 0 if i < j, slot 1 otherwise).
+// Partial rows: 0 u, 1..6 tau (xx yy zz xy xz yz), before the 1/(2 Omega).
+template <int MODE>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
+  __shared__ double red[4 * 7];
+  const int n = a.n;
+  const bool QQ = a.task & 1, QD = a.task & 2, DD = a.task & 4;
+  const bool EFIELD = (MODE != ES_FINAL) || a.do_efield;
+  const bool EFG = (MODE == ES_FINAL) && a.do_efg;
+  const bool FORCE = (MODE == ES_FINAL);
+  const bool DAMP = (MODE != ES_IND);
+  const double alpha = a.alpha, alpha2 = alpha * alpha, alpha3 = alpha2 * alpha,
+               alpha5 = alpha3 * alpha2;
+  const double piroot = sqrt(MDFF_PI);
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) acc[k] = 0.0;
+  if (i < n) {
+    const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
+    const int ti = a.typ[i];
+    const double qi = a.q[i];
+    const double mi0 = a.mu[i], mi1 = a.mu[i + n], mi2 = a.mu[i + 2 * n];
+    const bool dip_i = (mi0 != 0.0 || mi1 != 0.0 || mi2 != 0.0);
+    double e0 = 0, e1 = 0, e2 = 0;
+    double g00 = 0, g11 = 0, g22 = 0, g01 = 0, g02 = 0, g12 = 0;
+    double f0 = 0, f1 = 0, f2 = 0;
+    const int c = a.cnt[i];
+    for (int k = 0; k < c; ++k) {
+      const int j = a.nbr[(size_t)k * n + i];
+      double rx, ry, rz;
+      const double d2 = min_image(a.cell, si1 - a.s[j], si2 - a.s[j + n],
+                                  si3 - a.s[j + 2 * n], rx, ry, rz);
+      if (d2 > a.cut2) continue;
+      const double qj = a.q[j];
+      const double mj0 = a.mu[j], mj1 = a.mu[j + n], mj2 = a.mu[j + 2 * n];
+      const bool dip_j = (mj0 != 0.0 || mj1 != 0.0 || mj2 != 0.0);
+      const bool dd = DD && (dip_i || dip_j);
+      const bool qd = QD && (dip_i || dip_j);
+      const int tj = a.typ[j];
+      const int o = (i < j) ? 0 : 1;
+      const int pij = ti + MDFF_NT * tj;
+      const int pji = tj + MDFF_NT * ti;
+      bool ldamp = DAMP && a.T->dmp_on[o][pij];
+      const double d = sqrt(d2);
+      const double d3 = d2 * d;
+      const double d5 = d3 * d2;
+      const double dm1 = 1.0 / d;
+      const double dm3 = dm1 / d2;
+      const double dm5 = dm3 / d2;
+      const double dm7 = dm5 / d2;
+      const double expon = exp(-alpha2 * d2) / piroot;
+      const double F0 = errfc_nr(alpha * d);
+      const double F1 = F0 + 2.0 * alpha * d * expon;
+      const double F2 = F1 + 4.0 * alpha3 * d3 * expon / 3.0;
+      const double F1_dm3 = F1 * dm3;
+      const double F2_dm5 = 3.0 * F2 * dm5;
+      // damping of i's own terms (p_i) and of j's terms (p_j)
+      double F1d_i = 0.0, F2d_i = 0.0, F1d_j = 0.0, F2d_j = 0.0;
+      if (ldamp) {
+        double fdmp, fdd;
+        tt_damp(a.tt, a.T->dmp_b[o][pij], a.T->dmp_c[o][pij], d,
+                a.T->dmp_k[o][pij], fdmp, fdd);
+        F1d_i = -fdmp + 1.0;
+        F2d_i = F1d_i + (d / 3.0) * fdd;
+        if (FORCE && qd) {
+          tt_damp(a.tt, a.T->dmp_b[1 - o][pji], a.T->dmp_c[1 - o][pji], d,
+                  a.T->dmp_k[1 - o][pji], fdmp, fdd);
+          F1d_j = -fdmp + 1.0;
+          F2d_j = F1d_j + (d / 3.0) * fdd;
+        }
+      }
+      const double rmi = rx * mi0 + ry * mi1 + rz * mi2;
+      const double rmj = rx * mj0 + ry * mj1 + rz * mj2;
+      if (QQ && EFIELD) {
+        const double w = qj * (F1 - F1d_i) * dm3;
+        e0 += w * rx;
+        e1 += w * ry;
+        e2 += w * rz;
+      }
+      if (dd && EFIELD) {
+        // T2 mu_j = r (r.mu_j) 3F2/d^5 - mu_j F1/d^3
+        e0 += rx * rmj * F2_dm5 - mj0 * F1_dm3;
+        e1 += ry * rmj * F2_dm5 - mj1 * F1_dm3;
+        e2 += rz * rmj * F2_dm5 - mj2 * F1_dm3;
+      }
+      if (EFG || FORCE) {
+        const double F3 = F2 + 8.0 * alpha5 * d5 * expon / 15.0;
+        const double F3_dm7 = F3 * dm7 * 15.0;
+        if (EFG) {
+          if (QQ) {
+            // -q_j T2
+            const double w = -qj * F2_dm5;
+            g00 += w * rx * rx + qj * F1_dm3;
+            g11 += w * ry * ry + qj * F1_dm3;
+            g22 += w * rz * rz + qj * F1_dm3;
+            g01 += w * rx * ry;
+            g02 += w * rx * rz;
+            g12 += w * ry * rz;
+          }
+          if (dd) {
+            // (T3 mu_j)_ab = -r_a r_b (r.mu_j) 15F3/d^7
+            //               + (delta_ab r.mu_j + r_a mu_jb + r_b mu_ja) 3F2/d^5
+            const double w3 = -rmj * F3_dm7;
+            g00 += rx * rx * w3 + (rmj + 2.0 * rx * mj0) * F2_dm5;
+            g11 += ry * ry * w3 + (rmj + 2.0 * ry * mj1) * F2_dm5;
+            g22 += rz * rz * w3 + (rmj + 2.0 * rz * mj2) * F2_dm5;
+            g01 += rx * ry * w3 + (rx * mj1 + ry * mj0) * F2_dm5;
+            g02 += rx * rz * w3 + (rx * mj2 + rz * mj0) * F2_dm5;
+            g12 += ry * rz * w3 + (ry * mj2 + rz * mj1) * F2_dm5;
+          }
+        }
+        if (FORCE) {
+          double px = 0, py = 0, pz = 0, u = 0;
+          if (QQ) {
+            const double w = qi * qj * F1_dm3;
+            px += w * rx;
+            py += w * ry;
+            pz += w * rz;
+            u += 0.5 * qi * qj * dm1 * F0;
+          }
+          if (dd) {
+            const double mm = mi0 * mj0 + mi1 * mj1 + mi2 * mj2;
+            const double w3 = -rmi * rmj * F3_dm7;
+            px += rx * w3 + (mi0 * rmj + rmi * mj0 + rx * mm) * F2_dm5;
+            py += ry * w3 + (mi1 * rmj + rmi * mj1 + ry * mm) * F2_dm5;
+            pz += rz * w3 + (mi2 * rmj + rmi * mj2 + rz * mm) * F2_dm5;
+            u -= 0.5 * (rmi * rmj * F2_dm5 - mm * F1_dm3);
+          }
+          if (qd) {
+            // q_i (T2 - T2d_j) mu_j - q_j (T2 - T2d_i) mu_i
+            const double a2j = qi * (F2_dm5 - 3.0 * F2d_j * dm5);
+            const double a1j = qi * (F1_dm3 - F1d_j * dm3);
+            const double a2i = qj * (F2_dm5 - 3.0 * F2d_i * dm5);
+            const double a1i = qj * (F1_dm3 - F1d_i * dm3);
+            px += rx * rmj * a2j - mj0 * a1j - (rx * rmi * a2i - mi0 * a1i);
+            py += ry * rmj * a2j - mj1 * a1j - (ry * rmi * a2i - mi1 * a1i);
+            pz += rz * rmj * a2j - mj2 * a1j - (rz * rmi * a2i - mi2 * a1i);
+            // q_j (T1 - T1d_i).mu_i, T1 = -r F1/d^3
+            u += -qj * rmi * (F1 - F1d_i) * dm3;
+          }
+          f0 += px;
+          f1 += py;
+          f2 += pz;
+          acc[0] += u;
+          acc[1] += rx * px;
+          acc[2] += ry * py;
+          acc[3] += rz * pz;
+          acc[4] += 0.5 * (rx * py + ry * px);
+          acc[5] += 0.5 * (rx * pz + rz * px);
+          acc[6] += 0.5 * (ry * pz + rz * py);
+        }
+      }
+    }
+    if (EFIELD) {
+      a.ef[i] = e0;
+      a.ef[i + n] = e1;
+      a.ef[i + 2 * n] = e2;
+    }
+    if (EFG) {
+      a.efg[i] = g00;
+      a.efg[i + n] = g01;
+      a.efg[i + 2 * n] = g02;
+      a.efg[i + 3 * n] = g01;
+      a.efg[i + 4 * n] = g11;
+      a.efg[i + 5 * n] = g12;
+      a.efg[i + 6 * n] = g02;
+      a.efg[i + 7 * n] = g12;
+      a.efg[i + 8 * n] = g22;
+    }
+    if (FORCE) {
+      a.f[i] = f0;
+      a.f[i + n] = f1;
+      a.f[i + 2 * n] = f2;
+    }
+  }
+  if (FORCE) {
+    block_sum<7>(acc, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 7; ++k) a.part[(size_t)k * gridDim.x + blockIdx.x] = acc[k];
+  }
+}
+
+template __global__ void k_es_dir<ES_STATIC>(EsDirArgs);
+template __global__ void k_es_dir<ES_IND>(EsDirArgs);
+template __global__ void k_es_dir<ES_FINAL>(EsDirArgs);
+
+// ---------------------------------------------------------------------------
+// Structure factor (multipole_ES_rec, src/field.f90:2697-2742) for all
+// half-space k vectors, FP64.  Thread = one k chunk (fixed n1, n2, KC
+// consecutive n3); blockIdx.y = atom split.  Per LDS atom tile the block
+// builds E_a[t][m] = e^{2 pi i m s_a}, m = 0..kmax_a, by recurrence from the
+// per-atom bases; e^{-i x} = conj(e^{i x}) gives the negative m.  Along the
+// chunk the phase advances by e3.
+//   rho(k) = sum_j (q_j + i k.mu_j) e^{i k.r_j},  k.mu_j = 2 pi n.(B^T mu_j)
+template <bool CHG, bool DIP>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_rho64(RhoArgs a) {
+  extern __shared__ double2 sh[];
+  const int TA = RHO_TILE;
+  const int L1 = a.kmax[0] + 1, L2 = a.kmax[1] + 1, L3 = a.kmax[2] + 1;
+  double2* E1 = sh;                 // [TA][L1]
+  double2* E2 = E1 + TA * L1;       // [TA][L2]
+  double2* E3 = E2 + TA * L2;       // [TA][L3]
+  double* aq = (double*)(E3 + TA * L3);  // [TA]
+  double* am = aq + TA;                  // [3][TA]  2 pi B^T mu
+  const int ck = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = ck < a.nchunk;
+  KChunk C = act ? a.chunks[ck] : a.chunks[0];
+  const int KC = a.KC;
+  double accR[KC_MAX], accI[KC_MAX];
+#pragma unroll
+  for (int c = 0; c < KC_MAX; ++c) accR[c] = accI[c] = 0.0;
+  const int n = a.n;
+  const int j0 = blockIdx.y * a.atoms_per_split;
+  const int j1 = min(n, j0 + a.atoms_per_split);
+  const double tpi = 2.0 * MDFF_PI;
+  for (int tb = j0; tb < j1; tb += TA) {
+    const int m = min(TA, j1 - tb);
+    __syncthreads();
+    // tables by recurrence: thread -> (atom, axis)
+    for (int w = threadIdx.x; w < 3 * TA; w += blockDim.x) {
+      const int t = w / 3, ax = w % 3;
+      if (t < m) {
+        const double2 e = a.eph[tb + t + ax * n];
+        const int L = (ax == 0) ? L1 : (ax == 1 ? L2 : L3);
+        double2* E = (ax == 0) ? E1 + t * L1 : (ax == 1 ? E2 + t * L2 : E3 + t * L3);
+        double2 p = make_double2(1.0, 0.0);
+        E[0] = p;
+        for (int l = 1; l < L; ++l) {
+          p = make_double2(p.x * e.x - p.y * e.y, p.x * e.y + p.y * e.x);
+          E[l] = p;
+        }
+      }
+    }
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+      const int j = tb + t;
+      aq[t] = CHG ? a.q[j] : 0.0;
+      if (DIP) {
+        const double mx = a.mu[j], my = a.mu[j + n], mz = a.mu[j + 2 * n];
+        const double* B = a.cell.B;
+        am[t] = tpi * (B[0] * mx + B[1] * my + B[2] * mz);
+        am[TA + t] = tpi * (B[3] * mx + B[4] * my + B[5] * mz);
+        am[2 * TA + t] = tpi * (B[6] * mx + B[7] * my + B[8] * mz);
+      }
+    }
+    __syncthreads();
+    if (act) {
+      for (int t = 0; t < m; ++t) {
+        double2 e1 = E1[t * L1 + C.n1];
+        double2 e2 = E2[t * L2 + abs(C.n2)];
+        if (C.n2 < 0) e2.y = -e2.y;
+        double2 e3 = E3[t * L3 + abs(C.n3s)];
+        if (C.n3s < 0) e3.y = -e3.y;
+        const double2 st = E3[t * L3 + 1];
+        double2 p12 = make_double2(e1.x * e2.x - e1.y * e2.y, e1.x * e2.y + e1.y * e2.x);
+        double pr = p12.x * e3.x - p12.y * e3.y;
+        double pi = p12.x * e3.y + p12.y * e3.x;
+        const double qj = aq[t];
+        double tt = 0.0, dt = 0.0;
+        if (DIP) {
+          tt = C.n1 * am[t] + C.n2 * am[TA + t] + C.n3s * am[2 * TA + t];
+          dt = am[2 * TA + t];
+        }
+#pragma unroll
+        for (int c = 0; c < KC_MAX; ++c) {
+          if (c < KC) {
+            if (CHG) {
+              accR[c] = fma(pr, qj, accR[c]);
+              accI[c] = fma(pi, qj, accI[c]);
+            }
+            if (DIP) {
+              accR[c] = fma(-pi, tt, accR[c]);
+              accI[c] = fma(pr, tt, accI[c]);
+              tt += dt;
+            }
+            const double nr = pr * st.x - pi * st.y;
+            pi = pr * st.y + pi * st.x;
+            pr = nr;
+          }
+        }
+      }
+    }
+  }
+  if (act) {
+    double2* out = a.part + (size_t)blockIdx.y * a.kpad + (size_t)ck * KC;
+    for (int c = 0; c < KC; ++c) out[c] = make_double2(accR[c], accI[c]);
+  }
+}
+
+template __global__ void k_rho64<true, false>(RhoArgs);
+template __global__ void k_rho64<false, true>(RhoArgs);
+template __global__ void k_rho64<true, true>(RhoArgs);
+
+// Sum of the atom-split partials, g = A_k rho, energy and stress per k
+// (src/field.f90:2742-2745, 2803-2815).  Partial rows: 0 u, 1..6 tau.
+__global__ __launch_bounds__(MDFF_BLOCK) void k_rho_reduce64(RhoRedArgs a) {
+  __shared__ double red[4 * 7];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) acc[c] = 0.0;
+  if (k < a.kpad) {
+    double rR = 0.0, rI = 0.0;
+    for (int sp = 0; sp < a.nsplit; ++sp) {
+      const double2 v = a.part[(size_t)sp * a.kpad + k];
+      rR += v.x;
+      rI += v.y;
+    }
+    const double Ak = a.Ak[k];
+    a.g[k] = make_double2(Ak * rR, Ak * rI);
+    if (a.do_energy) {
+      const double str = (rR * rR + rI * rI) * Ak;
+      const double kx = a.kvec[k], ky = a.kvec[k + a.kpad], kz = a.kvec[k + 2 * a.kpad];
+      const double kc = a.kcoe[k];
+      acc[0] = str;
+      acc[1] = (1.0 - kc * kx * kx) * str;
+      acc[2] = (1.0 - kc * ky * ky) * str;
+      acc[3] = (1.0 - kc * kz * kz) * str;
+      acc[4] = -kc * kx * ky * str;
+      acc[5] = -kc * kx * kz * str;
+      acc[6] = -kc * ky * kz * str;
+    }
+  }
+  if (a.do_energy) {
+    block_sum<7>(acc, red);
+    if (threadIdx.x == 0)
+      for (int c = 0; c < 7; ++c) a.pout[(size_t)c * gridDim.x + blockIdx.x] = acc[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k-space field / EFG apply per atom (multipole_ES_rec, src/field.f90:
+// 2746-2801), FP64.  Thread = atom i; blockIdx.y = k split (a contiguous
+// range of chunks).  With P = e^{i k.r_i} and g = A_k rho(k):
+//   recarg  = Im(g conj(P))... = g_I P_R - g_R P_I   (field, charge force)
+//   recarg2 = g_R P_R + g_I P_I                      (EFG, dipole force)
+// accumulated in the fractional basis: U_c = sum n_c recarg,
+// V_cd = sum n_c n_d recarg2, c,d = 1..3.  The phase walks each (n1,n2) run
+// by recurrence; runs and n1 changes advance by e2 and e1.
+// Output rows (split-major): [split][comp][i], comp 0..2 U, 3..8 V (11 22 33
+// 12 13 23).
+template <bool EFG>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_apply64(ApplyArgs a) {
+  __shared__ KChunk sc[APPLY_TILE];
+  __shared__ double2 sg[APPLY_TILE * KC_MAX];
+  const int n = a.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < n;
+  const int KC = a.KC;
+  const int c0 = blockIdx.y * a.chunks_per_split;
+  const int c1 = min(a.nchunk, c0 + a.chunks_per_split);
+  double U1 = 0, U2 = 0, U3 = 0;
+  double V11 = 0, V22 = 0, V33 = 0, V12 = 0, V13 = 0, V23 = 0;
+  double2 e1 = make_double2(1, 0), e2 = make_double2(1, 0), e3 = make_double2(1, 0);
+  double s1 = 0, s2 = 0, s3 = 0;
+  if (act) {
+    e1 = a.eph[i];
+    e2 = a.eph[i + n];
+    e3 = a.eph[i + 2 * n];
+    s1 = a.s[i];
+    s2 = a.s[i + n];
+    s3 = a.s[i + 2 * n];
+  }
+  // running phases: P12 = e1^n1 e2^n2, ph = P12 e3^n3
+  double2 P12 = make_double2(1, 0), ph = make_double2(1, 0);
+  int cur_n1 = -1000000, cur_n2 = -1000000, next_n3 = -1000000;
+  for (int cb = c0; cb < c1; cb += APPLY_TILE) {
+    const int m = min(APPLY_TILE, c1 - cb);
+    __syncthreads();
+    for (int w = threadIdx.x; w < m; w += blockDim.x) sc[w] = a.chunks[cb + w];
+    for (int w = threadIdx.x; w < m * KC; w += blockDim.x)
+      sg[w] = a.g[(size_t)cb * KC + w];
+    __syncthreads();
+    if (!act) continue;
+    for (int t = 0; t < m; ++t) {
+      const KChunk C = sc[t];
+      if (C.n1 != cur_n1) {
+        double sn, cs;
+        sincospi(2.0 * (C.n1 * s1 + C.n2 * s2), &sn, &cs);
+        P12 = make_double2(cs, sn);
+        cur_n1 = C.n1;
+        cur_n2 = C.n2;
+        sincospi(2.0 * C.n3s * s3, &sn, &cs);
+        ph = make_double2(P12.x * cs - P12.y * sn, P12.x * sn + P12.y * cs);
+      } else if (C.n2 != cur_n2) {
+        // next run of the same n1: advance P12 by e2^(dn2) (dn2 = 1)
+        P12 = make_double2(P12.x * e2.x - P12.y * e2.y, P12.x * e2.y + P12.y * e2.x);
+        cur_n2 = C.n2;
+        double sn, cs;
+        sincospi(2.0 * C.n3s * s3, &sn, &cs);
+        ph = make_double2(P12.x * cs - P12.y * sn, P12.x * sn + P12.y * cs);
+      } else if (C.n3s != next_n3) {
+        double sn, cs;
+        sincospi(2.0 * C.n3s * s3, &sn, &cs);
+        ph = make_double2(P12.x * cs - P12.y * sn, P12.x * sn + P12.y * cs);
+      }
+      double pr = ph.x, pi = ph.y;
+      double R0 = 0, R3 = 0, Q0 = 0, Q3 = 0, Q33 = 0;
+      double n3 = (double)C.n3s;
+      const double2* g = sg + t * KC;
+#pragma unroll
+      for (int c = 0; c < KC_MAX; ++c) {
+        if (c < KC) {
+          const double2 gk = g[c];
+          const double ra = gk.y * pr - gk.x * pi;
+          R0 += ra;
+          R3 = fma(n3, ra, R3);
+          if (EFG) {
+            const double rb = gk.x * pr + gk.y * pi;
+            Q0 += rb;
+            Q3 = fma(n3, rb, Q3);
+            Q33 = fma(n3 * n3, rb, Q33);
+          }
+          const double nr = pr * e3.x - pi * e3.y;
+          pi = pr * e3.y + pi * e3.x;
+          pr = nr;
+          n3 += 1.0;
+        }
+      }
+      ph = make_double2(pr, pi);
+      next_n3 = C.n3s + KC;
+      const double dn1 = C.n1, dn2 = C.n2;
+      U1 = fma(dn1, R0, U1);
+      U2 = fma(dn2, R0, U2);
+      U3 += R3;
+      if (EFG) {
+        V11 = fma(dn1 * dn1, Q0, V11);
+        V22 = fma(dn2 * dn2, Q0, V22);
+        V33 += Q33;
+        V12 = fma(dn1 * dn2, Q0, V12);
+        V13 = fma(dn1, Q3, V13);
+        V23 = fma(dn2, Q3, V23);
+      }
+    }
+  }
+  if (act) {
+    double* out = a.part + (size_t)blockIdx.y * (EFG ? 9 : 3) * n;
+    out[i] = U1;
+    out[i + n] = U2;
+    out[i + 2 * n] = U3;
+    if (EFG) {
+      out[i + 3 * n] = V11;
+      out[i + 4 * n] = V22;
+      out[i + 5 * n] = V33;
+      out[i + 6 * n] = V12;
+      out[i + 7 * n] = V13;
+      out[i + 8 * n] = V23;
+    }
+  }
+}
+
+template __global__ void k_apply64<false>(ApplyArgs);
+template __global__ void k_apply64<true>(ApplyArgs);
+
+// ---------------------------------------------------------------------------
+// Per-call moment sums of multipole_ES (src/field.f90:1864-1880): rows
+// 0..2 qtot = sum q r, 3..5 mutot, 6 qsq, 7 musq.
+__global__ __launch_bounds__(MDFF_BLOCK) void k_moments(
+    int n, const double* __restrict__ pos, const double* __restrict__ q,
+    const double* __restrict__ mu, double* __restrict__ part) {
+  __shared__ double red[4 * 8];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+  if (i < n) {
+    const double qi = q[i];
+    const double m0 = mu[i], m1 = mu[i + n], m2 = mu[i + 2 * n];
+    acc[0] = qi * pos[i];
+    acc[1] = qi * pos[i + n];
+    acc[2] = qi * pos[i + 2 * n];
+    acc[3] = m0;
+    acc[4] = m1;
+    acc[5] = m2;
+    acc[6] = qi * qi;
+    acc[7] = m0 * m0 + m1 * m1 + m2 * m2;
+  }
+  block_sum<8>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 8; ++k) part[(size_t)k * gridDim.x + blockIdx.x] = acc[k];
+}
+
+// ---------------------------------------------------------------------------
+// Per-atom total of one multipole_ES call (src/field.f90:1933-1990): sums
+// the k-split partials, maps U, V back to Cartesian with k = 2 pi B n, adds
+// the self (and surface) terms.  In ES_IND mode the result is the induced
+// field and E = E_stat + E_ind is formed; in ES_FINAL mode the forces are
+// added with coul_unit and ef_t / efg_t are written.
+template <int MODE>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_combine64(CombineArgs a) {
+  const int n = a.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool EFG = (MODE == ES_FINAL) && a.do_efg;
+  const bool EFIELD = (MODE != ES_FINAL) || a.do_efield;
+  const int ncomp = (MODE == ES_FINAL && (a.do_efg || a.dip_force)) ? 9 : 3;
+  double U[3] = {0, 0, 0}, V[6] = {0, 0, 0, 0, 0, 0};
+  for (int sp = 0; sp < a.nsplit; ++sp) {
+    const double* p = a.apart + (size_t)sp * ncomp * n;
+    U[0] += p[i];
+    U[1] += p[i + n];
+    U[2] += p[i + 2 * n];
+    if (ncomp == 9)
+      for (int c = 0; c < 6; ++c) V[c] += p[i + (3 + c) * n];
+  }
+  const double* B = a.cell.B;
+  const double tpi = 2.0 * MDFF_PI;
+  // E_rec = -2 (4 pi / V) sum_k k recarg
+  const double fe = -2.0 * a.fpi_V * tpi;
+  double er[3];
+  for (int r = 0; r < 3; ++r)
+    er[r] = fe * (B[r] * U[0] + B[r + 3] * U[1] + B[r + 6] * U[2]);
+  // raw k-space EFG, 2 (4 pi/V) (2 pi)^2 B V B^T (full symmetric)
+  double G[3][3] = {{0}};
+  if (ncomp == 9) {
+    const double Vm[3][3] = {{V[0], V[3], V[4]}, {V[3], V[1], V[5]}, {V[4], V[5], V[2]}};
+    const double fg = 2.0 * a.fpi_V * tpi * tpi;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double acc = 0.0;
+        for (int x = 0; x < 3; ++x)
+          for (int y = 0; y < 3; ++y) acc += B[r + 3 * x] * Vm[x][y] * B[c + 3 * y];
+        G[r][c] = fg * acc;
+      }
+  }
+  const double m0 = a.mu[i], m1 = a.mu[i + n], m2 = a.mu[i + 2 * n];
+  const double mu3[3] = {m0, m1, m2};
+  double ef[3];
+  for (int r = 0; r < 3; ++r) {
+    double v = 0.0;
+    if (EFIELD) v = a.ef_dir[i + r * n] + er[r];
+    if (a.lsurf) v = v + (a.qm[r] + a.qm[3 + r]);
+    ef[r] = v + 2.0 * a.selfa2 * mu3[r];
+  }
+  if (MODE == ES_STATIC) {
+    for (int r = 0; r < 3; ++r) a.E_stat[i + r * n] = ef[r];
+  } else if (MODE == ES_IND) {
+    for (int r = 0; r < 3; ++r) a.E[i + r * n] = a.E_stat[i + r * n] + ef[r];
+  } else {
+    if (a.ef_out)
+      for (int r = 0; r < 3; ++r) a.ef_out[i + r * n] = ef[r];
+    if (a.efg_out) {
+      const double qi = a.q[i];
+      for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+          double v = 0.0;
+          if (EFG) {
+            v = a.efg_dir[i + n * (r + 3 * c)];
+            if (r <= c) v += G[r][c];  // k-space EFG has the upper 6 only
+          }
+          if (r == c) v += -2.0 * a.selfa2 * qi;
+          a.efg_out[i + n * (r + 3 * c)] = v * a.coul_unit;
+        }
+    }
+    // forces: f += (f_rec + f_dir [+ f_surf]) coul_unit
+    const double qi = a.q[i];
+    for (int r = 0; r < 3; ++r) {
+      double fr = qi * er[r];
+      if (a.dip_force) fr += G[r][0] * m0 + G[r][1] * m1 + G[r][2] * m2;
+      double v = fr + a.f_dir[i + r * n];
+      if (a.lsurf) v = v + (-(qi * (a.qm[r] + a.qm[3 + r])) * a.fpi_3V);
+      a.f[i + r * n] = a.f[i + r * n] + v * a.coul_unit;
+    }
+  }
+}
+
+template __global__ void k_combine64<ES_STATIC>(CombineArgs);
+template __global__ void k_combine64<ES_IND>(CombineArgs);
+template __global__ void k_combine64<ES_FINAL>(CombineArgs);
+
+// ---------------------------------------------------------------------------
+// induced_moment (src/field.f90:1716-1803): mu = alpha E on polarizable
+// atoms; partial row 0 = sum mu alpha^-1 mu (u_pol before the 1/2).
+__global__ __launch_bounds__(MDFF_BLOCK) void k_induced(
+    int n, const int* __restrict__ typ, const TypeTables* __restrict__ T,
+    const double* __restrict__ E, double* __restrict__ mu, double* __restrict__ part) {
+  __shared__ double red[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < n) {
+    const int t = typ[i];
+    double m[3] = {0, 0, 0};
+    if (T->polar[t]) {
+      const double e[3] = {E[i], E[i + n], E[i + 2 * n]};
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) m[r] = m[r] + T->pol[t][r + 3 * c] * e[c];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) acc[0] = acc[0] + m[r] * T->invpol[t][r + 3 * c] * m[c];
+    }
+    mu[i] = m[0];
+    mu[i + n] = m[1];
+    mu[i + 2 * n] = m[2];
+  }
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// SCF stopping criterion (src/field.f90:3378-3400): row 0 = sum of squared
+// residuals mu/alpha_aa - E_a, row 1 = number of polarizable atoms.
+__global__ __launch_bounds__(MDFF_BLOCK) void k_scf_resid(
+    int n, const int* __restrict__ typ, const TypeTables* __restrict__ T,
+    const double* __restrict__ mu, const double* __restrict__ E,
+    double* __restrict__ part) {
+  __shared__ double red[4 * 2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (i < n) {
+    const int t = typ[i];
+    if (T->polar[t]) {
+      acc[1] = 1.0;
+      for (int r = 0; r < 3; ++r) {
+        const double pa = T->pol[t][r + 3 * r];
+        if (pa == 0.0) continue;
+        const double d = mu[i + r * n] / pa - E[i + r * n];
+        acc[0] = acc[0] + d * d;
+      }
+    }
+  }
+  block_sum<2>(acc, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = acc[0];
+    part[gridDim.x + blockIdx.x] = acc[1];
+  }
+}
+
+// Dipole extrapolation and history updates (src/field.f90:3807-3992).
+//   op 0: mu = h0                         (poly order 0)
+//   op 1: mu = 2 h0 - h1, h1 = h0         (poly order 1, itime > 1)
+//   op 2: mu = h0, h1 = h0                (poly order 1, itime <= 1)
+//   op 3: ASPC predictor, mu = sum_k b_k h_k, shift h_k <- h_{k-1}
+//   op 4: h0 = mu                         (store after the SCF)
+//   op 5: ASPC corrector, mu = w alpha E + (1 - w) mu_p (+ u_pol row)
+// hist layout: Fortran dipia_ind_t(nh, n, 3): [t + nh*(i + n*a)].
+__global__ __launch_bounds__(MDFF_BLOCK) void k_extrap(ExtrapArgs a) {
+  __shared__ double red[4];
+  const int n = a.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < n) {
+    const int nh = a.nh;
+    for (int r = 0; r < 3; ++r) {
+      double* h = a.hist + (size_t)nh * (i + (size_t)n * r);
+      double* m = a.mu + i + (size_t)r * n;
+      switch (a.op) {
+        case 0: *m = h[0]; break;
+        case 1: *m = 2.0 * h[0] - h[1]; h[1] = h[0]; break;
+        case 2: *m = h[0]; h[1] = h[0]; break;
+        case 3: {
+          double v = 0.0;
+          for (int k = 0; k <= a.ord; ++k) v = v + a.b[k] * h[k];
+          *m = v;
+          for (int k = a.ord; k >= 1; --k) h[k] = h[k - 1];
+          a.mup[i + (size_t)r * n] = v;
+          break;
+        }
+        case 4: h[0] = *m; break;
+        default: break;
+      }
+    }
+    if (a.op == 5) {
+      const int t = a.typ[i];
+      double mm[3] = {0, 0, 0};
+      if (a.T->polar[t]) {
+        const double e[3] = {a.E[i], a.E[i + n], a.E[i + 2 * n]};
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) mm[r] = mm[r] + a.T->pol[t][r + 3 * c] * e[c];
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) acc[0] = acc[0] + mm[r] * a.T->invpol[t][r + 3 * c] * mm[c];
+      }
+      for (int r = 0; r < 3; ++r)
+        a.mu[i + r * n] = a.w * mm[r] + (1.0 - a.w) * a.mup[i + r * n];
+    }
+  }
+  if (a.op == 5) {
+    block_sum<1>(acc, red);
+    if (threadIdx.x == 0) a.part[blockIdx.x] = acc[0];
+  }
+}
+
+// Row sums of block partials in a fixed order: out[r] = sum_b part[r*nb+b].
+__global__ __launch_bounds__(MDFF_BLOCK) void k_reduce_rows(
+    const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double red[4];
+  const int r = blockIdx.x;
+  double acc[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc[0] += part[(size_t)r * nb + b];
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) out[r] = acc[0];
+}
+
+// out = a + b elementwise (total dipoles mu = dipia + mu_ind)
+__global__ __launch_bounds__(MDFF_BLOCK) void k_add(int m, const double* __restrict__ a,
+                                                    const double* __restrict__ b,
+                                                    double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = a[i] + b[i];
+}
+
+}  // namespace mdff

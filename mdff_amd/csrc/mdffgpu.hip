@@ -1,0 +1,1179 @@
+// mdffgpu.hip -- C ABI (include/mdffgpu.h) and host orchestration of the
+// force/field evaluation: engforce_driver (src/field.f90:1111-1166) with
+// the polarizable-ion SCF (moment_from_pola_scf, :3200-3458) and the Ewald
+// calls (multipole_ES, :1817-2054) on one MI355X.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mdffgpu.h"
+#include "kernels.h"
+#include "mixed.h"
+
+using namespace mdff;
+
+namespace {
+
+// scalar slots of ctx->d_sc
+enum {
+  SC_SR = 0,     // 12: u_lj vir_lj u_bm vir_bm u_mo vir_mo tau6
+  SC_DIR = 12,   // 7: u tau6
+  SC_REC = 19,   // 7: u tau6
+  SC_MOM = 26,   // 8: qtot3 mutot3 qsq musq
+  SC_UPOL = 34,  // 1
+  SC_RMSD = 35,  // 2: sum, npol
+  SC_OVF = 37,   // unused slot kept for layout
+  SC_N = 40
+};
+
+thread_local std::string g_create_error;
+
+const double B_ASPC[6][6] = {
+    {1.0, 0, 0, 0, 0, 0},
+    {2.0, -1.0, 0, 0, 0, 0},
+    {2.5, -2.0, 0.5, 0, 0, 0},
+    {2.8, -2.8, 1.2, -0.2, 0, 0},
+    {3.0, -24.0 / 7.0, 27.0 / 14.0, -4.0 / 7.0, 1.0 / 14.0, 0},
+    {22.0 / 7.0, -55.0 / 14.0, 55.0 / 21.0, -22.0 / 21.0, 5.0 / 21.0, -1.0 / 42.0}};
+const double W_ASPC[6] = {0.0, 2.0 / 3.0, 0.6, 4.0 / 7.0, 5.0 / 9.0, 6.0 / 11.0};
+
+struct Fail {
+  int code;
+};
+
+}  // namespace
+
+struct mdffgpu_ctx {
+  mdffgpu_params P;
+  int n = 0;
+  int device = 0;
+  std::string err;
+  std::vector<int> typ0;
+  Cell cell;
+  double W[3];
+  // k-space mesh
+  int kmax[3] = {0, 0, 0};
+  int KC = 1, nchunk = 0, kpad = 0, nk = 0;
+  std::vector<KChunk> chunks_h;
+  int rho_split = 1, rho_atoms = 1, app_split = 1, app_chunks = 1;
+  // neighbour list
+  int maxnb = 0;
+  double list_cut = 0.0;
+  // device buffers
+  double *d_pos = nullptr, *d_s = nullptr;
+  double2* d_eph = nullptr;
+  int *d_typ = nullptr, *d_cnt = nullptr, *d_nbr = nullptr, *d_ovf = nullptr;
+  double *d_q = nullptr, *d_qzero = nullptr, *d_dip = nullptr, *d_mu = nullptr,
+         *d_muind = nullptr, *d_mup = nullptr;
+  double *d_Estat = nullptr, *d_E = nullptr, *d_efdir = nullptr, *d_efgdir = nullptr,
+         *d_fdir = nullptr, *d_force = nullptr, *d_ef = nullptr, *d_efg = nullptr;
+  double* d_apart = nullptr;
+  double2* d_rpart = nullptr;
+  double2* d_g = nullptr;
+  double *d_Ak = nullptr, *d_kcoe = nullptr, *d_kvec = nullptr;
+  KChunk* d_chunks = nullptr;
+  double *d_part = nullptr, *d_sc = nullptr, *d_hist = nullptr;
+  size_t part_cap = 0;
+  PairTables* d_pt = nullptr;
+  TypeTables* d_tt = nullptr;
+  PairTables pt_h;
+  TypeTables tt_h;
+  TTTable tt;
+  double u_pol = 0.0;
+  double* h_sc = nullptr;  // pinned
+  int* h_ovf = nullptr;    // pinned
+  hipStream_t own_stream = nullptr;
+  hipStream_t st = nullptr;
+  int linduced = 0, ldip_static = 0;
+  mixed::MixedState* mx = nullptr;
+  // kernel instrumentation: HIP events around each launch of the own stream
+  int prof = 0;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_pending;
+  std::vector<hipEvent_t> ev_pool;
+  double prof_ms[mdff::K_COUNT] = {0};
+  long long prof_n[mdff::K_COUNT] = {0};
+};
+
+namespace mdff {
+const char* const kKernelNames[K_COUNT] = {
+    "k_prepare", "k_nlist", "k_shortrange", "k_es_dir<static>", "k_es_dir<ind>",
+    "k_es_dir<final>", "k_rho64<chg>", "k_rho64<dip>", "k_rho64<chg+dip>", "k_rho_reduce64",
+    "k_apply64<field>", "k_apply64<efg>", "k_moments", "k_combine64", "k_induced",
+    "k_scf_resid", "k_extrap", "k_reduce_rows", "k_add", "mx_shortrange", "mx_es_dir",
+    "mx_phase", "mx_rho", "mx_rho_reduce", "mx_apply", "mx_combine", "batch", "adjoint"};
+
+ProfScope::ProfScope(mdffgpu_ctx* ctx, int kid) : c(ctx), id(kid), a(nullptr), b(nullptr) {
+  if (!c->prof) return;
+  if (c->ev_pool.size() < 2) {
+    for (int k = 0; k < 2; ++k) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) == hipSuccess) c->ev_pool.push_back(e);
+    }
+  }
+  if (c->ev_pool.size() < 2) return;
+  a = c->ev_pool.back();
+  c->ev_pool.pop_back();
+  b = c->ev_pool.back();
+  c->ev_pool.pop_back();
+  (void)hipEventRecord(a, c->st);
+}
+
+ProfScope::~ProfScope() noexcept(false) {
+  if (!a) return;
+  (void)hipEventRecord(b, c->st);
+  c->prof_pending.push_back({id, {a, b}});
+}
+}  // namespace mdff
+
+namespace {
+
+void fail(mdffgpu_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c)
+    c->err = buf;
+  else
+    g_create_error = buf;
+  throw Fail{code};
+}
+
+void hipck(mdffgpu_ctx* c, hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    fail(c, e == hipErrorOutOfMemory ? MDFFGPU_ERR_ALLOC : MDFFGPU_ERR_DEVICE,
+         "%s: %s", what, hipGetErrorString(e));
+  }
+}
+
+#define HCK(c, x) hipck((c), (x), #x)
+
+template <typename T>
+void dalloc(mdffgpu_ctx* c, T** p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) count = 1;
+  hipck(c, hipMalloc((void**)p, count * sizeof(T)), "hipMalloc");
+}
+
+inline int nblk(int n) { return (n + MDFF_BLOCK - 1) / MDFF_BLOCK; }
+
+void check_launch(mdffgpu_ctx* c, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) fail(c, MDFFGPU_ERR_DEVICE, "launch %s: %s", what, hipGetErrorString(e));
+}
+
+// After a stream synchronisation: fold the recorded intervals in.
+void prof_collect(mdffgpu_ctx* c) {
+  for (auto& p : c->prof_pending) {
+    float ms = 0.f;
+    HCK(c, hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    c->prof_ms[p.first] += ms;
+    c->prof_n[p.first] += 1;
+    c->ev_pool.push_back(p.second.first);
+    c->ev_pool.push_back(p.second.second);
+  }
+  c->prof_pending.clear();
+}
+
+// 3x3 inverse, Gauss-Jordan with partial pivoting (DGETRF/DGETRI of
+// src/read_pos.f90:201-210 for the polarizability); singular -> zeros.
+void inv3(const double* m, double* out) {
+  double a[3][6];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      a[i][j] = m[i + 3 * j];
+      a[i][j + 3] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int col = 0; col < 3; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < 3; ++r)
+      if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+    if (a[piv][col] == 0.0) {
+      for (int k = 0; k < 9; ++k) out[k] = 0.0;
+      return;
+    }
+    if (piv != col)
+      for (int k = 0; k < 6; ++k) std::swap(a[col][k], a[piv][k]);
+    double d = a[col][col];
+    for (int k = 0; k < 6; ++k) a[col][k] = a[col][k] / d;
+    for (int r = 0; r < 3; ++r)
+      if (r != col) {
+        double f = a[r][col];
+        for (int k = 0; k < 6; ++k) a[r][k] = a[r][k] - f * a[col][k];
+      }
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[i + 3 * j] = a[i][j + 3];
+}
+
+// lattice (src/lattice.f90:60-168)
+void set_lattice(mdffgpu_ctx* c, const double* A) {
+  Cell& cl = c->cell;
+  std::memcpy(cl.A, A, 9 * sizeof(double));
+  const double* a1 = A;
+  const double* a2 = A + 3;
+  const double* a3 = A + 6;
+  double* B = cl.B;
+  B[0] = a2[1] * a3[2] - a2[2] * a3[1];
+  B[1] = a2[2] * a3[0] - a2[0] * a3[2];
+  B[2] = a2[0] * a3[1] - a2[1] * a3[0];
+  B[3] = a3[1] * a1[2] - a3[2] * a1[1];
+  B[4] = a3[2] * a1[0] - a3[0] * a1[2];
+  B[5] = a3[0] * a1[1] - a3[1] * a1[0];
+  B[6] = a1[1] * a2[2] - a1[2] * a2[1];
+  B[7] = a1[2] * a2[0] - a1[0] * a2[2];
+  B[8] = a1[0] * a2[1] - a1[1] * a2[0];
+  double omega = B[0] * A[0] + B[1] * A[1] + B[2] * A[2];
+  if (!(omega > 0.0)) fail(c, MDFFGPU_ERR_ARG, "cell volume %g is not positive", omega);
+  cl.omega = omega;
+  for (int i = 0; i < 3; ++i) {
+    const double* b = B + 3 * i;
+    c->W[i] = omega / std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  }
+  for (int i = 0; i < 9; ++i) B[i] = B[i] / omega;
+}
+
+// Half-space k mesh of kpoint_sum_init (src/kspace.f90:142-207) arranged
+// in chunks of KC consecutive n3; padding slots carry A_k = 0.
+void build_kmesh(mdffgpu_ctx* c) {
+  const mdffgpu_params& P = c->P;
+  for (int a = 0; a < 3; ++a) {
+    c->kmax[a] = P.kES[a];
+    if (P.kES[a] < 0 || P.kES[a] > 512) fail(c, MDFFGPU_ERR_ARG, "kES out of range");
+  }
+  const int kx = c->kmax[0], ky = c->kmax[1], kz = c->kmax[2];
+  const int L3 = 2 * kz + 1;
+  const int nch3 = (L3 + KC_MAX - 1) / KC_MAX;
+  c->KC = (L3 + nch3 - 1) / nch3;
+  const int KC = c->KC;
+  std::vector<KChunk>& ch = c->chunks_h;
+  ch.clear();
+  std::vector<double> Ak, kcoe, kv[3];
+  const double tpi = 2.0 * MDFF_PI;
+  const double alpha2 = P.alphaES * P.alphaES;
+  const double* B = c->cell.B;
+  int nk = 0;
+  for (int n1 = 0; n1 <= kx; ++n1) {
+    int n2min = (n1 == 0) ? 0 : -ky;
+    for (int n2 = n2min; n2 <= ky; ++n2) {
+      int lo = (n1 == 0 && n2 == 0) ? 1 : -kz;
+      if (lo > kz) continue;
+      for (int s = lo; s <= kz; s += KC) {
+        KChunk k;
+        k.n1 = n1;
+        k.n2 = n2;
+        k.n3s = s;
+        k.pad = 0;
+        ch.push_back(k);
+        for (int cc = 0; cc < KC; ++cc) {
+          int n3 = s + cc;
+          if (n3 <= kz) {
+            double x = n1, y = n2, z = n3;
+            double vx = tpi * (x * B[0] + y * B[3] + z * B[6]);
+            double vy = tpi * (x * B[1] + y * B[4] + z * B[7]);
+            double vz = tpi * (x * B[2] + y * B[5] + z * B[8]);
+            double kk = vx * vx + vy * vy + vz * vz;
+            Ak.push_back(std::exp(-kk * 0.25 / alpha2) / kk);
+            kcoe.push_back(2.0 * (1.0 / kk + 1.0 / alpha2 / 4.0));
+            kv[0].push_back(vx);
+            kv[1].push_back(vy);
+            kv[2].push_back(vz);
+            ++nk;
+          } else {
+            Ak.push_back(0.0);
+            kcoe.push_back(0.0);
+            kv[0].push_back(0.0);
+            kv[1].push_back(0.0);
+            kv[2].push_back(0.0);
+          }
+        }
+      }
+    }
+  }
+  c->nk = nk;
+  c->nchunk = (int)ch.size();
+  c->kpad = c->nchunk * KC;
+  if (c->nchunk == 0) return;
+  std::vector<double> kvec(3 * (size_t)c->kpad);
+  for (int a = 0; a < 3; ++a)
+    std::memcpy(kvec.data() + (size_t)a * c->kpad, kv[a].data(), c->kpad * sizeof(double));
+  dalloc(c, &c->d_chunks, c->nchunk);
+  dalloc(c, &c->d_Ak, c->kpad);
+  dalloc(c, &c->d_kcoe, c->kpad);
+  dalloc(c, &c->d_kvec, 3 * (size_t)c->kpad);
+  dalloc(c, &c->d_g, c->kpad);
+  HCK(c, hipMemcpy(c->d_chunks, ch.data(), ch.size() * sizeof(KChunk), hipMemcpyHostToDevice));
+  HCK(c, hipMemcpy(c->d_Ak, Ak.data(), Ak.size() * sizeof(double), hipMemcpyHostToDevice));
+  HCK(c, hipMemcpy(c->d_kcoe, kcoe.data(), kcoe.size() * sizeof(double), hipMemcpyHostToDevice));
+  HCK(c, hipMemcpy(c->d_kvec, kvec.data(), kvec.size() * sizeof(double), hipMemcpyHostToDevice));
+  // work split so that a launch has about 1024 blocks
+  const int n = c->n;
+  int gx = (c->nchunk + MDFF_BLOCK - 1) / MDFF_BLOCK;
+  c->rho_split = std::max(1, std::min((1024 + gx - 1) / gx, (n + RHO_TILE - 1) / RHO_TILE));
+  c->rho_atoms = (n + c->rho_split - 1) / c->rho_split;
+  c->rho_split = (n + c->rho_atoms - 1) / c->rho_atoms;
+  int ax = nblk(n);
+  c->app_split = std::max(1, std::min((1024 + ax - 1) / ax, c->nchunk));
+  c->app_chunks = (c->nchunk + c->app_split - 1) / c->app_split;
+  c->app_split = (c->nchunk + c->app_chunks - 1) / c->app_chunks;
+  dalloc(c, &c->d_rpart, (size_t)c->rho_split * c->kpad);
+  dalloc(c, &c->d_apart, (size_t)c->app_split * 9 * n);
+}
+
+void build_tables(mdffgpu_ctx* c) {
+  const mdffgpu_params& P = c->P;
+  PairTables& T = c->pt_h;
+  std::memset(&T, 0, sizeof(T));
+  const double two16 = std::pow(2.0, 1.0 / 6.0);
+  const double rc = P.cutshortrange, rcsq = rc * rc;
+  const int nt = P.ntype;
+  auto L = [&](int a, int b, int cc) { return P.ldip_damping[a + 16 * (b + 16 * cc)] != 0; };
+  auto D = [&](const double* tab, int a, int b, int cc) { return tab[a + 16 * (b + 16 * cc)]; };
+  for (int ti = 0; ti < nt; ++ti)
+    for (int tj = 0; tj < nt; ++tj) {
+      const int t = ti + 16 * tj;
+      // initialize_param_non_bonded, src/field.f90:1021-1064
+      const double pp = P.plj[t], qq = P.qlj[t];
+      const double epsp = P.epslj[t] / (qq - pp);
+      const double sigsq = two16 * two16 * P.sigmalj[t] * P.sigmalj[t];
+      const double sr2 = sigsq / rcsq;
+      const double sr = std::sqrt(sr2);
+      const double srp = std::pow(sr, pp), srq = std::pow(sr, qq);
+      const double ppqq = pp * qq;
+      T.lj_epsp[t] = epsp;
+      T.lj_sigsq[t] = sigsq;
+      T.lj_fc[t] = ppqq * epsp / sigsq;
+      T.lj_ptwo[t] = pp * 0.5;
+      T.lj_qtwo[t] = qq * 0.5;
+      T.lj_p[t] = pp;
+      T.lj_q[t] = qq;
+      T.lj_uc[t] = epsp * (pp * srq - qq * srp);
+      T.lj_uc1[t] = epsp * ppqq / (2.0 * rcsq) * (srq - srp);
+      T.lj_uc2[t] = 0.5 * epsp * ((2.0 * pp + ppqq) * srq - (2.0 * qq + ppqq) * srp);
+      T.bm_A[t] = P.Abmhftd[t];
+      T.bm_B[t] = P.Bbmhftd[t];
+      T.bm_C[t] = P.Cbmhftd[t];
+      T.bm_D[t] = P.Dbmhftd[t];
+      T.bm_BD[t] = P.BDbmhftd[t];
+      T.mo_eps[t] = P.epsmor[t];
+      T.mo_rho[t] = P.rhomor[t];
+      T.mo_rs[t] = std::exp(P.rhomor[t] * P.sigmamor[t]);
+      // damping as seen from ti (src/field.f90:2207-2211)
+      T.dmp_on[0][t] = L(ti, ti, tj) || L(tj, ti, tj);
+      T.dmp_on[1][t] = L(tj, tj, ti) || L(ti, tj, ti);
+      T.dmp_b[0][t] = D(P.dip_pol_damp_b, ti, ti, tj);
+      T.dmp_c[0][t] = D(P.dip_pol_damp_c, ti, ti, tj);
+      T.dmp_k[0][t] = P.dip_pol_damp_k[ti + 16 * (ti + 16 * tj)];
+      T.dmp_b[1][t] = D(P.dip_pol_damp_b, ti, tj, ti);
+      T.dmp_c[1][t] = D(P.dip_pol_damp_c, ti, tj, ti);
+      T.dmp_k[1][t] = P.dip_pol_damp_k[ti + 16 * (tj + 16 * ti)];
+      for (int o = 0; o < 2; ++o)
+        if (T.dmp_k[o][t] < 0 || T.dmp_k[o][t] > MDFFGPU_TT_MAX_ORDER)
+          fail(c, MDFFGPU_ERR_TT_ORDER, "Tang-Toennies order %d > %d", T.dmp_k[o][t],
+               MDFFGPU_TT_MAX_ORDER);
+    }
+  TypeTables& Y = c->tt_h;
+  std::memset(&Y, 0, sizeof(Y));
+  c->linduced = 0;
+  c->ldip_static = 0;
+  for (int k = 0; k < 48; ++k)
+    if (P.dip[k] != 0.0) c->ldip_static = 1;
+  for (int t = 0; t < 16; ++t) {
+    Y.qch[t] = P.qch[t];
+    for (int a = 0; a < 3; ++a) Y.dip[t][a] = P.dip[t + 16 * a];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) Y.pol[t][a + 3 * b] = P.poldip[t + 16 * (a + 3 * b)];
+    Y.polar[t] = P.ldip_polar[t] ? 1 : 0;
+    if (Y.polar[t]) inv3(Y.pol[t], Y.invpol[t]);
+    if (t < nt && Y.polar[t]) c->linduced = 1;
+  }
+  c->tt.e[0] = 1.0;
+  for (int k = 1; k <= 8; ++k) c->tt.e[k] = c->tt.e[k - 1] / (double)k;
+  HCK(c, hipMemcpy(c->d_pt, &T, sizeof(T), hipMemcpyHostToDevice));
+  HCK(c, hipMemcpy(c->d_tt, &Y, sizeof(Y), hipMemcpyHostToDevice));
+}
+
+void validate(mdffgpu_ctx* c, const mdffgpu_params* p) {
+  if (!p) fail(c, MDFFGPU_ERR_ARG, "params is NULL");
+  if (p->natm < 1) fail(c, MDFFGPU_ERR_ARG, "natm must be positive");
+  if (p->ntype < 1 || p->ntype > MDFFGPU_NTYPEMAX) fail(c, MDFFGPU_ERR_ARG, "ntype out of range");
+  if ((p->lnmlj || p->lbmhft || p->lbmhftd || p->lmorse) && !(p->cutshortrange > 0.0))
+    fail(c, MDFFGPU_ERR_ARG, "cutshortrange is null");
+  if (p->lcoulomb && !(p->cutlongrange > 0.0)) fail(c, MDFFGPU_ERR_ARG, "cutlongrange is null");
+  if (p->lcoulomb && !(p->alphaES > 0.0)) fail(c, MDFFGPU_ERR_ARG, "alphaES must be positive");
+  if (p->lcoulomb && (p->kES[0] < 1 || p->kES[1] < 1 || p->kES[2] < 1))
+    fail(c, MDFFGPU_ERR_ARG, "kES must be at least 1 along every axis");
+  if (p->algo_ext_dipole == MDFFGPU_EXT_POLY && (p->extrapolate_order < 0 || p->extrapolate_order > 1))
+    fail(c, MDFFGPU_ERR_UNSUPPORTED,
+         "polynomial dipole extrapolation of order %d is not supported (0 or 1)",
+         p->extrapolate_order);
+  if (p->algo_ext_dipole == MDFFGPU_EXT_ASPC && (p->extrapolate_order < 0 || p->extrapolate_order > 5))
+    fail(c, MDFFGPU_ERR_ARG, "ASPC extrapolation order must be 0..5");
+  if (p->algo_ext_dipole != MDFFGPU_EXT_POLY && p->algo_ext_dipole != MDFFGPU_EXT_ASPC)
+    fail(c, MDFFGPU_ERR_ARG, "unknown algo_ext_dipole");
+  if (p->precision != MDFFGPU_PREC_FP64 && p->precision != MDFFGPU_PREC_MIXED)
+    fail(c, MDFFGPU_ERR_ARG, "unknown precision");
+  if (p->max_scf_pol_iter < 0 || p->min_scf_pol_iter < 0)
+    fail(c, MDFFGPU_ERR_ARG, "SCF iteration bounds must be non-negative");
+}
+
+void ensure_part(mdffgpu_ctx* c, size_t count) {
+  if (count > c->part_cap) {
+    dalloc(c, &c->d_part, count);
+    c->part_cap = count;
+  }
+}
+
+void reduce(mdffgpu_ctx* c, int rows, int nb, int slot) {
+  {
+    ProfScope _ps(c, K_REDUCE);
+    k_reduce_rows<<<rows, MDFF_BLOCK, 0, c->st>>>(c->d_part, nb, c->d_sc + slot);
+  }
+  check_launch(c, "k_reduce_rows");
+}
+
+void alloc_atoms(mdffgpu_ctx* c) {
+  const size_t n = c->n;
+  dalloc(c, &c->d_pos, 3 * n);
+  dalloc(c, &c->d_s, 3 * n);
+  dalloc(c, &c->d_eph, 3 * n);
+  dalloc(c, &c->d_typ, n);
+  dalloc(c, &c->d_cnt, n);
+  dalloc(c, &c->d_ovf, 1);
+  dalloc(c, &c->d_q, n);
+  dalloc(c, &c->d_qzero, n);
+  dalloc(c, &c->d_dip, 3 * n);
+  dalloc(c, &c->d_mu, 3 * n);
+  dalloc(c, &c->d_muind, 3 * n);
+  dalloc(c, &c->d_mup, 3 * n);
+  dalloc(c, &c->d_Estat, 3 * n);
+  dalloc(c, &c->d_E, 3 * n);
+  dalloc(c, &c->d_efdir, 3 * n);
+  dalloc(c, &c->d_efgdir, 9 * n);
+  dalloc(c, &c->d_fdir, 3 * n);
+  dalloc(c, &c->d_force, 3 * n);
+  dalloc(c, &c->d_ef, 3 * n);
+  dalloc(c, &c->d_efg, 9 * n);
+  dalloc(c, &c->d_sc, SC_N);
+  dalloc(c, &c->d_hist, (size_t)(c->P.extrapolate_order + 1) * 3 * n);
+  dalloc(c, &c->d_pt, 1);
+  dalloc(c, &c->d_tt, 1);
+  HCK(c, hipMemset(c->d_qzero, 0, n * sizeof(double)));
+  HCK(c, hipMemset(c->d_hist, 0, (size_t)(c->P.extrapolate_order + 1) * 3 * n * sizeof(double)));
+  HCK(c, hipMemset(c->d_sc, 0, SC_N * sizeof(double)));
+  ensure_part(c, 64 * (size_t)nblk(c->n) + 4096);
+}
+
+void set_list_capacity(mdffgpu_ctx* c, int maxnb) {
+  c->maxnb = std::max(1, std::min(maxnb, c->n));
+  dalloc(c, &c->d_nbr, (size_t)c->maxnb * c->n);
+}
+
+int initial_capacity(mdffgpu_ctx* c) {
+  // 1.5 x the mean number of neighbours inside the list cutoff, + 64
+  const double rc = c->list_cut;
+  const double nmean = 4.0 / 3.0 * MDFF_PI * rc * rc * rc * c->n / c->cell.omega;
+  return (int)std::min<double>(c->n, 1.5 * nmean + 64.0);
+}
+
+void update_cutoffs(mdffgpu_ctx* c) {
+  const mdffgpu_params& P = c->P;
+  double rc = 0.0;
+  if (P.lnmlj || P.lbmhft || P.lbmhftd || P.lmorse) rc = std::max(rc, P.cutshortrange);
+  if (P.lcoulomb) rc = std::max(rc, P.cutlongrange);
+  c->list_cut = rc;
+}
+
+// ---------------------------------------------------------------------------
+// One multipole_ES call on device buffers.
+void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int task,
+             int do_efield, int do_efg) {
+  const int n = c->n;
+  hipStream_t st = c->st;
+  const mdffgpu_params& P = c->P;
+  const int gb = nblk(n);
+  // real space
+  EsDirArgs da;
+  da.n = n;
+  da.s = c->d_s;
+  da.typ = c->d_typ;
+  da.cnt = c->d_cnt;
+  da.nbr = c->d_nbr;
+  da.cell = c->cell;
+  da.T = c->d_pt;
+  da.tt = c->tt;
+  da.q = q;
+  da.mu = mu;
+  da.cut2 = P.cutlongrange * P.cutlongrange;
+  da.alpha = P.alphaES;
+  da.task = task;
+  da.do_efield = do_efield;
+  da.do_efg = do_efg;
+  da.ef = c->d_efdir;
+  da.efg = c->d_efgdir;
+  da.f = c->d_fdir;
+  da.part = c->d_part;
+  if (mode == ES_STATIC)
+    {
+      ProfScope _ps(c, K_DIR_STATIC);
+      k_es_dir<ES_STATIC><<<gb, MDFF_BLOCK, 0, st>>>(da);
+    }
+  else if (mode == ES_IND)
+    {
+      ProfScope _ps(c, K_DIR_IND);
+      k_es_dir<ES_IND><<<gb, MDFF_BLOCK, 0, st>>>(da);
+    }
+  else
+    {
+      ProfScope _ps(c, K_DIR_FINAL);
+      k_es_dir<ES_FINAL><<<gb, MDFF_BLOCK, 0, st>>>(da);
+    }
+  check_launch(c, "k_es_dir");
+  if (mode == ES_FINAL) reduce(c, 7, gb, SC_DIR);
+  // reciprocal space
+  const bool chg = (mode != ES_IND);
+  const bool dip = (task & (TASK_QD | TASK_DD)) != 0;
+  const bool efgv = (mode == ES_FINAL) && (do_efg || dip);
+  if (c->nchunk > 0) {
+    RhoArgs ra;
+    ra.n = n;
+    ra.nchunk = c->nchunk;
+    ra.KC = c->KC;
+    for (int a = 0; a < 3; ++a) ra.kmax[a] = c->kmax[a];
+    ra.chunks = c->d_chunks;
+    ra.eph = c->d_eph;
+    ra.q = q;
+    ra.mu = mu;
+    ra.cell = c->cell;
+    ra.atoms_per_split = c->rho_atoms;
+    ra.kpad = c->kpad;
+    ra.part = c->d_rpart;
+    dim3 g((c->nchunk + MDFF_BLOCK - 1) / MDFF_BLOCK, c->rho_split);
+    size_t lds = (size_t)RHO_TILE * (c->kmax[0] + c->kmax[1] + c->kmax[2] + 3) * sizeof(double2) +
+                 4 * RHO_TILE * sizeof(double);
+    if (chg && dip)
+      {
+        ProfScope _ps(c, K_RHO_BOTH);
+        k_rho64<true, true><<<g, MDFF_BLOCK, lds, st>>>(ra);
+      }
+    else if (dip)
+      {
+        ProfScope _ps(c, K_RHO_DIP);
+        k_rho64<false, true><<<g, MDFF_BLOCK, lds, st>>>(ra);
+      }
+    else
+      {
+        ProfScope _ps(c, K_RHO_CHG);
+        k_rho64<true, false><<<g, MDFF_BLOCK, lds, st>>>(ra);
+      }
+    check_launch(c, "k_rho64");
+    RhoRedArgs rr;
+    rr.kpad = c->kpad;
+    rr.nsplit = c->rho_split;
+    rr.do_energy = (mode == ES_FINAL);
+    rr.part = c->d_rpart;
+    rr.Ak = c->d_Ak;
+    rr.kcoe = c->d_kcoe;
+    rr.kvec = c->d_kvec;
+    rr.g = c->d_g;
+    rr.pout = c->d_part;
+    const int gk = nblk(c->kpad);
+    {
+      ProfScope _ps(c, K_RHO_RED);
+      k_rho_reduce64<<<gk, MDFF_BLOCK, 0, st>>>(rr);
+    }
+    check_launch(c, "k_rho_reduce64");
+    if (mode == ES_FINAL) reduce(c, 7, gk, SC_REC);
+    ApplyArgs aa;
+    aa.n = n;
+    aa.nchunk = c->nchunk;
+    aa.KC = c->KC;
+    aa.chunks_per_split = c->app_chunks;
+    aa.chunks = c->d_chunks;
+    aa.g = c->d_g;
+    aa.eph = c->d_eph;
+    aa.s = c->d_s;
+    aa.part = c->d_apart;
+    dim3 ga(gb, c->app_split);
+    if (efgv)
+      {
+        ProfScope _ps(c, K_APPLY_EFG);
+        k_apply64<true><<<ga, MDFF_BLOCK, 0, st>>>(aa);
+      }
+    else
+      {
+        ProfScope _ps(c, K_APPLY_F);
+        k_apply64<false><<<ga, MDFF_BLOCK, 0, st>>>(aa);
+      }
+    check_launch(c, "k_apply64");
+  } else {
+    HCK(c, hipMemsetAsync(c->d_apart, 0, (size_t)c->app_split * 9 * n * sizeof(double), st));
+    if (mode == ES_FINAL) HCK(c, hipMemsetAsync(c->d_sc + SC_REC, 0, 7 * sizeof(double), st));
+  }
+  // moment sums (self and surface terms)
+  if (mode == ES_FINAL || P.lsurf) {
+    {
+      ProfScope _ps(c, K_MOMENTS);
+      k_moments<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, q, mu, c->d_part);
+    }
+    check_launch(c, "k_moments");
+    reduce(c, 8, gb, SC_MOM);
+  }
+  CombineArgs ca;
+  ca.n = n;
+  ca.nsplit = (c->nchunk > 0) ? c->app_split : 1;
+  ca.lsurf = P.lsurf;
+  ca.do_efield = do_efield;
+  ca.do_efg = do_efg;
+  ca.dip_force = dip;
+  ca.apart = c->d_apart;
+  ca.cell = c->cell;
+  ca.fpi_V = 2.0 * (2.0 * MDFF_PI / c->cell.omega);
+  ca.fpi_3V = 2.0 * (2.0 * MDFF_PI / c->cell.omega / 3.0);
+  const double selfa = P.alphaES / std::sqrt(MDFF_PI);
+  ca.selfa2 = 2.0 * selfa * P.alphaES * P.alphaES / 3.0;
+  ca.coul_unit = P.coul_unit;
+  ca.mu = mu;
+  ca.q = q;
+  ca.ef_dir = c->d_efdir;
+  ca.efg_dir = c->d_efgdir;
+  ca.f_dir = c->d_fdir;
+  ca.qm = c->d_sc + SC_MOM;
+  ca.E_stat = c->d_Estat;
+  ca.E = c->d_E;
+  ca.ef_out = c->d_ef;
+  ca.efg_out = c->d_efg;
+  ca.f = c->d_force;
+  if (mode == ES_STATIC)
+    {
+      ProfScope _ps(c, K_COMBINE);
+      k_combine64<ES_STATIC><<<gb, MDFF_BLOCK, 0, st>>>(ca);
+    }
+  else if (mode == ES_IND)
+    {
+      ProfScope _ps(c, K_COMBINE);
+      k_combine64<ES_IND><<<gb, MDFF_BLOCK, 0, st>>>(ca);
+    }
+  else
+    {
+      ProfScope _ps(c, K_COMBINE);
+      k_combine64<ES_FINAL><<<gb, MDFF_BLOCK, 0, st>>>(ca);
+    }
+  check_launch(c, "k_combine64");
+}
+
+void sync_scalars(mdffgpu_ctx* c) {
+  HCK(c, hipMemcpyAsync(c->h_sc, c->d_sc, SC_N * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HCK(c, hipStreamSynchronize(c->st));
+  prof_collect(c);
+}
+
+void extrap(mdffgpu_ctx* c, int op, int ord, double w) {
+  ExtrapArgs ea;
+  ea.n = c->n;
+  ea.nh = c->P.extrapolate_order + 1;
+  ea.op = op;
+  ea.ord = ord;
+  for (int k = 0; k < 6; ++k) ea.b[k] = (ord >= 0 && ord < 6) ? B_ASPC[ord][k] : 0.0;
+  ea.w = w;
+  ea.hist = c->d_hist;
+  ea.mu = c->d_muind;
+  ea.mup = c->d_mup;
+  ea.E = c->d_E;
+  ea.typ = c->d_typ;
+  ea.T = c->d_tt;
+  ea.part = c->d_part;
+  {
+    ProfScope _ps(c, K_EXTRAP);
+    k_extrap<<<nblk(c->n), MDFF_BLOCK, 0, c->st>>>(ea);
+  }
+  check_launch(c, "k_extrap");
+}
+
+int aspc_order(const mdffgpu_params& P, int itime) {
+  if (itime >= P.extrapolate_order + 1) return P.extrapolate_order;
+  if (itime == 1 || itime == 0) return 0;
+  return itime - 1;
+}
+
+// moment_from_pola_scf (src/field.f90:3200-3458) on device; returns the
+// iteration count and final rmsd.
+void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
+  const mdffgpu_params& P = c->P;
+  const int n = c->n;
+  const int gb = nblk(n);
+  hipStream_t st = c->st;
+  HCK(c, hipMemsetAsync(c->d_muind, 0, 3 * (size_t)n * sizeof(double), st));
+  *n_scf = 0;
+  *rmsd_out = 0.0;
+  if (!c->linduced) return;
+  int task_static = TASK_QQ | (c->ldip_static ? (TASK_QD | TASK_DD) : 0);
+  if (P.precision == MDFFGPU_PREC_MIXED)
+    mixed::es_call(c->mx, ES_STATIC, c->d_q, c->d_dip, task_static, 1, 0);
+  else
+    es_call(c, ES_STATIC, c->d_q, c->d_dip, task_static, 1, 0);
+  ++*n_es;
+  HCK(c, hipMemcpyAsync(c->d_E, c->d_Estat, 3 * (size_t)n * sizeof(double),
+                        hipMemcpyDeviceToDevice, st));
+  int iscf = 0;
+  double rmsd = 1.79769313486231570e308;
+  bool upol_new = false;
+  while ((iscf < P.max_scf_pol_iter && rmsd > P.conv_tol_ind) || iscf < P.min_scf_pol_iter) {
+    ++iscf;
+    if (iscf != 1 || P.calc_mode == 0) {
+      {
+        ProfScope _ps(c, K_INDUCED);
+        k_induced<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, c->d_tt, c->d_E, c->d_muind, c->d_part);
+      }
+      check_launch(c, "k_induced");
+      reduce(c, 1, gb, SC_UPOL);
+      upol_new = true;
+    } else if (P.algo_ext_dipole == MDFFGPU_EXT_POLY) {
+      int op = (P.extrapolate_order == 0) ? 0 : (itime > P.extrapolate_order ? 1 : 2);
+      extrap(c, op, 0, 0.0);
+    } else {
+      extrap(c, 3, aspc_order(P, itime), 0.0);
+    }
+    if (P.precision == MDFFGPU_PREC_MIXED)
+      mixed::es_call(c->mx, ES_IND, c->d_qzero, c->d_muind, TASK_DD, 1, 0);
+    else
+      es_call(c, ES_IND, c->d_qzero, c->d_muind, TASK_DD, 1, 0);
+    ++*n_es;
+    if (iscf == 1 && P.calc_mode == 1 && P.algo_ext_dipole == MDFFGPU_EXT_ASPC) {
+      const int eo = aspc_order(P, itime);
+      extrap(c, 5, eo, W_ASPC[eo]);
+      reduce(c, 1, gb, SC_UPOL);
+      upol_new = true;
+    }
+    {
+      ProfScope _ps(c, K_RESID);
+      k_scf_resid<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, c->d_tt, c->d_muind, c->d_E, c->d_part);
+    }
+    check_launch(c, "k_scf_resid");
+    reduce(c, 2, gb, SC_RMSD);
+    sync_scalars(c);
+    rmsd = std::sqrt(c->h_sc[SC_RMSD] / c->h_sc[SC_RMSD + 1]);
+  }
+  extrap(c, 4, 0, 0.0);
+  if (upol_new) c->u_pol = 0.5 * c->h_sc[SC_UPOL];
+  *n_scf = iscf;
+  *rmsd_out = rmsd;
+}
+
+void sym9(const double* t6, double scale, double* out) {
+  // t6: xx yy zz xy xz yz -> Fortran (3,3)
+  out[0] = t6[0] * scale;
+  out[4] = t6[1] * scale;
+  out[8] = t6[2] * scale;
+  out[1] = out[3] = t6[3] * scale;
+  out[2] = out[6] = t6[4] * scale;
+  out[5] = out[7] = t6[5] * scale;
+}
+
+// engforce_driver on the device buffers of ctx; d_pos and d_force are
+// loaded by the caller.
+void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
+  const mdffgpu_params& P = c->P;
+  const int n = c->n;
+  const int gb = nblk(n);
+  hipStream_t st = c->st;
+  std::memset(en, 0, sizeof(*en));
+  {
+    ProfScope _ps(c, K_PREPARE);
+    k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->d_s, c->d_eph);
+  }
+  check_launch(c, "k_prepare");
+  if (c->list_cut > 0.0) {
+    for (;;) {
+      HCK(c, hipMemsetAsync(c->d_ovf, 0, sizeof(int), st));
+      {
+        ProfScope _ps(c, K_NLIST);
+        k_nlist<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->cell, c->list_cut * c->list_cut, c->maxnb,
+                                            c->d_cnt, c->d_nbr, c->d_ovf);
+      }
+      check_launch(c, "k_nlist");
+      HCK(c, hipMemcpyAsync(c->h_ovf, c->d_ovf, sizeof(int), hipMemcpyDeviceToHost, st));
+      HCK(c, hipStreamSynchronize(st));
+      prof_collect(c);
+      if (*c->h_ovf <= c->maxnb) break;
+      set_list_capacity(c, (int)(*c->h_ovf * 1.2) + 16);
+    }
+  }
+  const bool sr = P.lnmlj || P.lbmhft || P.lbmhftd || P.lmorse;
+  if (sr) {
+    int flags = (P.lnmlj ? SR_LJ : 0) | ((P.lbmhft || P.lbmhftd) ? SR_BM : 0) |
+                (P.lbmhftd ? SR_BMD : 0) | (P.lmorse ? SR_MO : 0) |
+                (P.trunc == 1 ? SR_TRUNC1 : 0) | (P.trunc == 2 ? SR_TRUNC2 : 0);
+    if (P.precision == MDFFGPU_PREC_MIXED)
+      mixed::shortrange(c->mx, flags);
+    else
+      {
+        ProfScope _ps(c, K_SR);
+        k_shortrange<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->d_typ, c->d_cnt, c->d_nbr, c->cell,
+                                                 c->d_pt, c->tt,
+                                                 P.cutshortrange * P.cutshortrange, flags,
+                                                 c->d_force, c->d_part);
+      }
+    check_launch(c, "k_shortrange");
+    reduce(c, 12, gb, SC_SR);
+  }
+  int n_es = 0;
+  if (P.lcoulomb) {
+    scf(c, itime, &en->n_scf, &en->rmsd, &n_es);
+    {
+      ProfScope _ps(c, K_ADD);
+      k_add<<<nblk(3 * n), MDFF_BLOCK, 0, st>>>(3 * n, c->d_dip, c->d_muind, c->d_mu);
+    }
+    check_launch(c, "k_add");
+    int task = (P.task_coul[0] ? TASK_QQ : 0) | (P.task_coul[1] ? TASK_QD : 0) |
+               (P.task_coul[2] ? TASK_DD : 0);
+    if (P.precision == MDFFGPU_PREC_MIXED)
+      mixed::es_call(c->mx, ES_FINAL, c->d_q, c->d_mu, task, P.doefield, P.doefg);
+    else
+      es_call(c, ES_FINAL, c->d_q, c->d_mu, task, P.doefield, P.doefg);
+    ++n_es;
+  }
+  sync_scalars(c);
+  const double* h = c->h_sc;
+  const double omega = c->cell.omega;
+  if (sr) {
+    if (P.lnmlj) {
+      en->u_lj = h[SC_SR + 0];
+      en->vir_lj = h[SC_SR + 1] / 3.0;
+    }
+    if (P.lbmhft || P.lbmhftd) {
+      en->u_bmhft = h[SC_SR + 2];
+      en->vir_bmhft = h[SC_SR + 3] / 3.0;
+    }
+    if (P.lmorse) {
+      en->u_morse = h[SC_SR + 4];
+      en->vir_morse = h[SC_SR + 5] / 3.0;
+    }
+    sym9(h + SC_SR + 6, 1.0 / omega / P.press_unit, en->tau_nonb);
+  }
+  if (P.lcoulomb) {
+    const double tpi_V = 2.0 * MDFF_PI / omega;
+    const double alpha = P.alphaES, alpha2 = alpha * alpha;
+    const double selfa = alpha / std::sqrt(MDFF_PI);
+    const double selfa2 = 2.0 * selfa * alpha2 / 3.0;
+    const double u_dir = h[SC_DIR];
+    const double u_rec = h[SC_REC] * 2.0 * tpi_V;
+    const double* m = h + SC_MOM;
+    const double u_self = -selfa * m[6] - selfa2 * m[7];
+    double u_surf = (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) +
+                    2.0 * (m[0] * m[3] + m[1] * m[4] + m[2] * m[5]) +
+                    (m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
+    u_surf = u_surf * (tpi_V / 3.0);
+    const double cu = P.coul_unit;
+    if (P.lsurf)
+      en->u_coul = (u_dir + u_rec + u_surf + u_self + c->u_pol) * cu;
+    else
+      en->u_coul = (u_dir + u_rec + u_self + c->u_pol) * cu;
+    en->u_pol = c->u_pol;
+    double td[9], tr[9];
+    sym9(h + SC_DIR + 1, 0.5 / omega, td);
+    sym9(h + SC_REC + 1, 2.0 * tpi_V / omega, tr);
+    for (int k = 0; k < 9; ++k) en->tau_coul[k] = (td[k] + tr[k]) * cu / P.press_unit;
+    en->pvirial_coul =
+        1.0 / 3.0 * (en->tau_coul[0] + en->tau_coul[4] + en->tau_coul[8]) * P.press_unit;
+  }
+  en->n_es_calls = n_es;
+}
+
+void create_impl(const mdffgpu_params* params, const int32_t* itype, const double* A,
+                 mdffgpu_ctx** out) {
+  validate(nullptr, params);
+  if (!itype || !A || !out) fail(nullptr, MDFFGPU_ERR_ARG, "NULL argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    fail(nullptr, MDFFGPU_ERR_DEVICE, "no HIP device visible");
+  mdffgpu_ctx* c = new mdffgpu_ctx();
+  try {
+    c->P = *params;
+    c->n = params->natm;
+    HCK(c, hipGetDevice(&c->device));
+    c->typ0.resize(c->n);
+    for (int i = 0; i < c->n; ++i) {
+      if (itype[i] < 1 || itype[i] > params->ntype)
+        fail(c, MDFFGPU_ERR_ARG, "itype(%d) = %d outside 1..ntype", i + 1, itype[i]);
+      c->typ0[i] = itype[i] - 1;
+    }
+    HCK(c, hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->st = c->own_stream;
+    HCK(c, hipHostMalloc((void**)&c->h_sc, SC_N * sizeof(double), hipHostMallocDefault));
+    HCK(c, hipHostMalloc((void**)&c->h_ovf, sizeof(int), hipHostMallocDefault));
+    set_lattice(c, A);
+    alloc_atoms(c);
+    build_tables(c);
+    HCK(c, hipMemcpy(c->d_typ, c->typ0.data(), c->n * sizeof(int), hipMemcpyHostToDevice));
+    // per-atom moments of typeinfo_init (src/read_pos.f90:175-199)
+    std::vector<double> q(c->n), dip(3 * (size_t)c->n);
+    for (int i = 0; i < c->n; ++i) {
+      const int t = c->typ0[i];
+      q[i] = c->P.qch[t];
+      for (int a = 0; a < 3; ++a) dip[i + (size_t)c->n * a] = c->P.dip[t + 16 * a];
+    }
+    HCK(c, hipMemcpy(c->d_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+    HCK(c, hipMemcpy(c->d_dip, dip.data(), dip.size() * sizeof(double), hipMemcpyHostToDevice));
+    update_cutoffs(c);
+    if (c->list_cut > 0.0) set_list_capacity(c, initial_capacity(c));
+    if (c->P.lcoulomb) build_kmesh(c);
+    size_t need = 12 * (size_t)std::max(nblk(c->kpad), nblk(c->n)) + 4096;
+    ensure_part(c, need);
+    if (c->P.precision == MDFFGPU_PREC_MIXED) {
+      c->mx = mixed::create(c);
+      if (!c->mx) fail(c, MDFFGPU_ERR_UNSUPPORTED, "mixed-precision path unavailable");
+    }
+    *out = c;
+  } catch (Fail&) {
+    g_create_error = c->err;
+    mdffgpu_destroy(c);
+    throw;
+  }
+}
+
+void copy_in_host(mdffgpu_ctx* c, const double* rx, const double* ry, const double* rz,
+                  const double* fx, const double* fy, const double* fz) {
+  const size_t n = c->n, b = n * sizeof(double);
+  HCK(c, hipMemcpyAsync(c->d_pos, rx, b, hipMemcpyHostToDevice, c->st));
+  HCK(c, hipMemcpyAsync(c->d_pos + n, ry, b, hipMemcpyHostToDevice, c->st));
+  HCK(c, hipMemcpyAsync(c->d_pos + 2 * n, rz, b, hipMemcpyHostToDevice, c->st));
+  HCK(c, hipMemcpyAsync(c->d_force, fx, b, hipMemcpyHostToDevice, c->st));
+  HCK(c, hipMemcpyAsync(c->d_force + n, fy, b, hipMemcpyHostToDevice, c->st));
+  HCK(c, hipMemcpyAsync(c->d_force + 2 * n, fz, b, hipMemcpyHostToDevice, c->st));
+}
+
+}  // namespace
+
+// ===========================================================================
+// mixed-precision hooks need access to the context internals
+namespace mdff {
+namespace mixed {
+CtxView view(mdffgpu_ctx* c) {
+  CtxView v;
+  v.n = c->n;
+  v.st = c->st;
+  v.cell = c->cell;
+  v.P = &c->P;
+  v.kmax[0] = c->kmax[0];
+  v.kmax[1] = c->kmax[1];
+  v.kmax[2] = c->kmax[2];
+  v.KC = c->KC;
+  v.nchunk = c->nchunk;
+  v.kpad = c->kpad;
+  v.chunks = c->d_chunks;
+  v.Ak = c->d_Ak;
+  v.kcoe = c->d_kcoe;
+  v.kvec = c->d_kvec;
+  v.pos = c->d_pos;
+  v.s = c->d_s;
+  v.eph = c->d_eph;
+  v.typ = c->d_typ;
+  v.cnt = c->d_cnt;
+  v.nbr = c->d_nbr;
+  v.maxnb = c->maxnb;
+  v.pt = c->d_pt;
+  v.tt = c->tt;
+  v.ttab = c->d_tt;
+  v.efdir = c->d_efdir;
+  v.efgdir = c->d_efgdir;
+  v.fdir = c->d_fdir;
+  v.force = c->d_force;
+  v.Estat = c->d_Estat;
+  v.E = c->d_E;
+  v.ef = c->d_ef;
+  v.efg = c->d_efg;
+  v.part = c->d_part;
+  v.sc = c->d_sc;
+  v.sc_sr = SC_SR;
+  v.sc_dir = SC_DIR;
+  v.sc_rec = SC_REC;
+  v.sc_mom = SC_MOM;
+  return v;
+}
+}  // namespace mixed
+}  // namespace mdff
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+int mdffgpu_create(const mdffgpu_params* params, const int32_t* itype, const double* A,
+                   mdffgpu_ctx** out) {
+  if (out) *out = nullptr;
+  try {
+    create_impl(params, itype, A, out);
+    return MDFFGPU_OK;
+  } catch (Fail& f) {
+    return f.code;
+  } catch (...) {
+    g_create_error = "unexpected host exception";
+    return MDFFGPU_ERR_DEVICE;
+  }
+}
+
+void mdffgpu_destroy(mdffgpu_ctx* c) {
+  if (!c) return;
+  if (c->mx) mixed::destroy(c->mx);
+  double* dp[] = {c->d_pos, c->d_s, c->d_q, c->d_qzero, c->d_dip, c->d_mu, c->d_muind,
+                  c->d_mup, c->d_Estat, c->d_E, c->d_efdir, c->d_efgdir, c->d_fdir,
+                  c->d_force, c->d_ef, c->d_efg, c->d_apart, c->d_Ak, c->d_kcoe, c->d_kvec,
+                  c->d_part, c->d_sc, c->d_hist};
+  for (double* p : dp)
+    if (p) (void)hipFree(p);
+  void* vp[] = {c->d_eph, c->d_typ, c->d_cnt, c->d_nbr, c->d_ovf, c->d_rpart, c->d_g,
+                c->d_chunks, c->d_pt, c->d_tt};
+  for (void* p : vp)
+    if (p) (void)hipFree(p);
+  for (auto& p : c->prof_pending) {
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->h_sc) (void)hipHostFree(c->h_sc);
+  if (c->h_ovf) (void)hipHostFree(c->h_ovf);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+int mdffgpu_set_cell(mdffgpu_ctx* c, const double* A) {
+  if (!c || !A) return MDFFGPU_ERR_ARG;
+  try {
+    set_lattice(c, A);
+    update_cutoffs(c);
+    if (c->list_cut > 0.0) set_list_capacity(c, std::max(c->maxnb, initial_capacity(c)));
+    if (c->P.lcoulomb) build_kmesh(c);
+    if (c->mx) mixed::on_cell(c->mx);
+    return MDFFGPU_OK;
+  } catch (Fail& f) {
+    return f.code;
+  }
+}
+
+int mdffgpu_engforce(mdffgpu_ctx* c, const double* rx, const double* ry, const double* rz,
+                     int32_t itime, double* fx, double* fy, double* fz, double* mu_t,
+                     double* theta_t, double* ef_t, double* efg_t, mdffgpu_energies* en) {
+  if (!c) return MDFFGPU_ERR_ARG;
+  if (!rx || !ry || !rz || !fx || !fy || !fz || !en) {
+    c->err = "NULL position, force or energy argument";
+    return MDFFGPU_ERR_ARG;
+  }
+  try {
+    c->st = c->own_stream;
+    copy_in_host(c, rx, ry, rz, fx, fy, fz);
+    run(c, itime, en);
+    const size_t n = c->n, b = n * sizeof(double);
+    HCK(c, hipMemcpyAsync(fx, c->d_force, b, hipMemcpyDeviceToHost, c->st));
+    HCK(c, hipMemcpyAsync(fy, c->d_force + n, b, hipMemcpyDeviceToHost, c->st));
+    HCK(c, hipMemcpyAsync(fz, c->d_force + 2 * n, b, hipMemcpyDeviceToHost, c->st));
+    if (c->P.lcoulomb) {
+      if (mu_t) HCK(c, hipMemcpyAsync(mu_t, c->d_mu, 3 * b, hipMemcpyDeviceToHost, c->st));
+      if (ef_t) HCK(c, hipMemcpyAsync(ef_t, c->d_ef, 3 * b, hipMemcpyDeviceToHost, c->st));
+      if (efg_t) HCK(c, hipMemcpyAsync(efg_t, c->d_efg, 9 * b, hipMemcpyDeviceToHost, c->st));
+    }
+    HCK(c, hipStreamSynchronize(c->st));
+    if (theta_t) std::memset(theta_t, 0, 9 * b);
+    return MDFFGPU_OK;
+  } catch (Fail& f) {
+    (void)hipStreamSynchronize(c->st);
+    return f.code;
+  }
+}
+
+int mdffgpu_engforce_device(mdffgpu_ctx* c, const double* d_pos, int32_t itime,
+                            double* d_force, double* d_mu, double* d_ef, double* d_efg,
+                            mdffgpu_energies* en, void* stream) {
+  if (!c) return MDFFGPU_ERR_ARG;
+  if (!d_pos || !d_force || !en) {
+    c->err = "NULL position, force or energy argument";
+    return MDFFGPU_ERR_ARG;
+  }
+  try {
+    c->st = stream ? (hipStream_t)stream : c->own_stream;
+    const size_t n = c->n, b = n * sizeof(double);
+    HCK(c, hipMemcpyAsync(c->d_pos, d_pos, 3 * b, hipMemcpyDeviceToDevice, c->st));
+    HCK(c, hipMemcpyAsync(c->d_force, d_force, 3 * b, hipMemcpyDeviceToDevice, c->st));
+    run(c, itime, en);
+    HCK(c, hipMemcpyAsync(d_force, c->d_force, 3 * b, hipMemcpyDeviceToDevice, c->st));
+    if (c->P.lcoulomb) {
+      if (d_mu) HCK(c, hipMemcpyAsync(d_mu, c->d_mu, 3 * b, hipMemcpyDeviceToDevice, c->st));
+      if (d_ef) HCK(c, hipMemcpyAsync(d_ef, c->d_ef, 3 * b, hipMemcpyDeviceToDevice, c->st));
+      if (d_efg) HCK(c, hipMemcpyAsync(d_efg, c->d_efg, 9 * b, hipMemcpyDeviceToDevice, c->st));
+    }
+    c->st = c->own_stream;
+    return MDFFGPU_OK;
+  } catch (Fail& f) {
+    c->st = c->own_stream;
+    return f.code;
+  }
+}
+
+int mdffgpu_get_dipole_history(mdffgpu_ctx* c, double* hist) {
+  if (!c || !hist) return MDFFGPU_ERR_ARG;
+  size_t cnt = (size_t)(c->P.extrapolate_order + 1) * 3 * c->n;
+  if (hipMemcpy(hist, c->d_hist, cnt * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+    c->err = "hipMemcpy of the dipole history failed";
+    return MDFFGPU_ERR_DEVICE;
+  }
+  return MDFFGPU_OK;
+}
+
+int mdffgpu_set_dipole_history(mdffgpu_ctx* c, const double* hist) {
+  if (!c || !hist) return MDFFGPU_ERR_ARG;
+  size_t cnt = (size_t)(c->P.extrapolate_order + 1) * 3 * c->n;
+  if (hipMemcpy(c->d_hist, hist, cnt * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    c->err = "hipMemcpy of the dipole history failed";
+    return MDFFGPU_ERR_DEVICE;
+  }
+  return MDFFGPU_OK;
+}
+
+int32_t mdffgpu_num_kvectors(const mdffgpu_ctx* c) { return c ? c->nk : 0; }
+
+const char* mdffgpu_last_error(const mdffgpu_ctx* c) {
+  if (!c) return g_create_error.c_str();
+  return c->err.c_str();
+}
+
+int mdffgpu_profile_enable(mdffgpu_ctx* c, int32_t on) {
+  if (!c) return MDFFGPU_ERR_ARG;
+  c->prof = on ? 1 : 0;
+  for (int k = 0; k < K_COUNT; ++k) {
+    c->prof_ms[k] = 0.0;
+    c->prof_n[k] = 0;
+  }
+  return MDFFGPU_OK;
+}
+
+int mdffgpu_profile_read(const mdffgpu_ctx* c, int32_t kernel, double* total_ms,
+                         int64_t* launches) {
+  if (!c || kernel < 0 || kernel >= K_COUNT || !total_ms || !launches) return MDFFGPU_ERR_ARG;
+  *total_ms = c->prof_ms[kernel];
+  *launches = c->prof_n[kernel];
+  return MDFFGPU_OK;
+}
+
+const char* mdffgpu_kernel_name(int32_t kernel) {
+  if (kernel < 0 || kernel >= K_COUNT) return "";
+  return kKernelNames[kernel];
+}
+
+int32_t mdffgpu_kernel_count(void) { return K_COUNT; }
+
+int32_t mdffgpu_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+}  // extern "C"

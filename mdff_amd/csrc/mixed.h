@@ -1,0 +1,60 @@
+// mixed.h -- mixed-precision evaluation path (FP32 pair, erfc and k-space
+// arithmetic; FP64 energy / virial reductions and SCF residual).
+#pragma once
+
+#include "../../include/mdffgpu.h"
+#include "kernels.h"
+
+namespace mdff {
+namespace mixed {
+
+// Read-only view of the context buffers the mixed path shares with the
+// FP64 path (neighbour list, k mesh, per-atom arrays, scalar slots).
+struct CtxView {
+  int n;
+  hipStream_t st;
+  Cell cell;
+  const mdffgpu_params* P;
+  int kmax[3];
+  int KC, nchunk, kpad;
+  const KChunk* chunks;
+  const double* Ak;
+  const double* kcoe;
+  const double* kvec;
+  const double* pos;
+  const double* s;
+  const double2* eph;
+  const int* typ;
+  const int* cnt;
+  const int* nbr;
+  int maxnb;
+  const PairTables* pt;
+  TTTable tt;
+  const TypeTables* ttab;
+  double* efdir;
+  double* efgdir;
+  double* fdir;
+  double* force;
+  double* Estat;
+  double* E;
+  double* ef;
+  double* efg;
+  double* part;
+  double* sc;
+  int sc_sr, sc_dir, sc_rec, sc_mom;
+};
+
+struct MixedState;
+
+// defined in mdffgpu.hip
+CtxView view(mdffgpu_ctx* c);
+
+MixedState* create(mdffgpu_ctx* c);
+void destroy(MixedState* m);
+void on_cell(MixedState* m);
+void shortrange(MixedState* m, int flags);
+void es_call(MixedState* m, int mode, const double* q, const double* mu, int task,
+             int do_efield, int do_efg);
+
+}  // namespace mixed
+}  // namespace mdff

@@ -1,0 +1,145 @@
+"""Host front-end of the force/field evaluation: the Python mirror of
+MDFF's engforce_driver (src/field.f90:1111-1166) over the C ABI.
+
+``ForceField`` owns one library context (one configuration on one GPU), the
+equivalent of the state the reference keeps in the field and config modules
+between calls (k mesh, dipole history dipia_ind_t, stale u_pol).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .params import MdffEnergies, MdffParams
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class ForceField:
+    """engforce_driver on one MI355X.
+
+    params: MdffParams after field_check_tag / ewald_param
+    itype:  (natm,) 1-based species indices
+    A:      (3,3) cell, lattice vectors as columns
+    """
+
+    def __init__(self, params: MdffParams, itype, A):
+        L = _lib.lib()
+        self.params = params
+        self.n = params.natm
+        self.itype = np.ascontiguousarray(itype, dtype=np.int32)
+        if self.itype.shape != (self.n,):
+            raise ValueError("itype must have natm entries")
+        Af = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(3, 3).reshape(-1, order="F"))
+        h = ctypes.c_void_p()
+        st = L.mdffgpu_create(ctypes.byref(params), self.itype.ctypes.data_as(_ip), _ptr(Af),
+                              ctypes.byref(h))
+        _lib.check(st, None)
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().mdffgpu_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_cell(self, A) -> None:
+        Af = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(3, 3).reshape(-1, order="F"))
+        _lib.check(_lib.lib().mdffgpu_set_cell(self._h, _ptr(Af)), self._h)
+
+    def num_kvectors(self) -> int:
+        return int(_lib.lib().mdffgpu_num_kvectors(self._h))
+
+    def engforce(self, r, itime: int = 0, f_in=None) -> dict:
+        """One engforce_driver call on host arrays; r is (natm,3) Cartesian.
+        Returns f, mu, ef (natm,3), efg, theta (natm,3,3) and the energies."""
+        n = self.n
+        r = np.asarray(r, dtype=np.float64)
+        rx, ry, rz = (np.ascontiguousarray(r[:, k]) for k in range(3))
+        f = np.zeros((3, n)) if f_in is None else np.ascontiguousarray(np.asarray(f_in, dtype=np.float64).T)
+        mu = np.zeros(3 * n)
+        theta = np.zeros(9 * n)
+        ef = np.zeros(3 * n)
+        efg = np.zeros(9 * n)
+        en = MdffEnergies()
+        st = _lib.lib().mdffgpu_engforce(self._h, _ptr(rx), _ptr(ry), _ptr(rz), int(itime),
+                                         _ptr(f[0]), _ptr(f[1]), _ptr(f[2]), _ptr(mu),
+                                         _ptr(theta), _ptr(ef), _ptr(efg), ctypes.byref(en))
+        _lib.check(st, self._h)
+        out = en.as_dict()
+        out["f"] = f.T.copy()
+        out["mu"] = mu.reshape(3, n).T.copy()
+        out["ef"] = ef.reshape(3, n).T.copy()
+        out["efg"] = efg.reshape(3, 3, n).transpose(2, 1, 0).copy()
+        out["theta"] = theta.reshape(3, 3, n).transpose(2, 1, 0).copy()
+        return out
+
+    def engforce_device(self, pos, force, mu=None, ef=None, efg=None, itime: int = 0,
+                        stream=None) -> MdffEnergies:
+        """Evaluation on device-resident torch tensors (float64, contiguous):
+        pos and force are (3, natm) SoA, mu / ef (3, natm), efg (3, 3, natm)
+        in Fortran (natm,3,3) order, i.e. efg[b, a, i].  force is in/out."""
+        for t in (pos, force):
+            if t.dtype.itemsize != 8 or not t.is_contiguous() or t.numel() != 3 * self.n:
+                raise ValueError("pos and force must be contiguous float64 (3, natm) tensors")
+        en = MdffEnergies()
+        sp = None if stream is None else ctypes.c_void_p(stream)
+        st = _lib.lib().mdffgpu_engforce_device(
+            self._h, ctypes.c_void_p(pos.data_ptr()), int(itime), ctypes.c_void_p(force.data_ptr()),
+            None if mu is None else ctypes.c_void_p(mu.data_ptr()),
+            None if ef is None else ctypes.c_void_p(ef.data_ptr()),
+            None if efg is None else ctypes.c_void_p(efg.data_ptr()), ctypes.byref(en), sp)
+        _lib.check(st, self._h)
+        return en
+
+    def profile(self, on: bool = True) -> None:
+        """Enable (and reset) the per-kernel HIP-event instrumentation."""
+        _lib.check(_lib.lib().mdffgpu_profile_enable(self._h, int(on)), self._h)
+
+    def profile_stats(self) -> dict:
+        """{kernel name: (total ms, launches)} of the instrumented launches."""
+        L = _lib.lib()
+        out = {}
+        for k in range(L.mdffgpu_kernel_count()):
+            ms = ctypes.c_double()
+            cnt = ctypes.c_int64()
+            _lib.check(L.mdffgpu_profile_read(self._h, k, ctypes.byref(ms), ctypes.byref(cnt)), self._h)
+            if cnt.value:
+                out[L.mdffgpu_kernel_name(k).decode()] = (ms.value, cnt.value)
+        return out
+
+    def dipole_history(self) -> np.ndarray:
+        p = self.params
+        h = np.zeros((p.extrapolate_order + 1) * self.n * 3)
+        _lib.check(_lib.lib().mdffgpu_get_dipole_history(self._h, _ptr(h)), self._h)
+        return h
+
+    def set_dipole_history(self, h) -> None:
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        _lib.check(_lib.lib().mdffgpu_set_dipole_history(self._h, _ptr(h)), self._h)
+
+
+def device_count() -> int:
+    return int(_lib.lib().mdffgpu_device_count())

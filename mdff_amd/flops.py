@@ -1,0 +1,40 @@
+"""Algorithmic work per unit of the hot kernels, fixed once here and used by
+bench.py's roofline and by the documentation.
+
+Convention: add/sub/mul = 1 flop, FMA = 2 flops.  The k-space kernels are
+written with separable phases e^{i k.r} = e1^n1 e2^n2 e3^n3, so the
+per-(atom, k) work is a complex multiply by the step phase plus the
+accumulations; no transcendental is evaluated per (atom, k).
+
+unit of the k-space kernels: one (atom, k) pair, k over the half-space mesh
+(src/field.f90:1648: K = kx + ky(2kz+1) + kx(2ky+1)(2kz+1)).
+unit of the real-space kernels: one directed neighbour pair inside the
+cutoff (each pair is evaluated from both atoms).
+"""
+
+# structure factor rho(k) = sum_j w_j e^{ik.r_j}
+RHO_CHG = 10      # phase step 6 + 2 FMA (q weight)
+RHO_DIP = 11      # phase step 6 + t update 1 + 2 FMA (i k.mu weight)
+RHO_BOTH = 15     # phase step 6 + t update 1 + 4 FMA
+# apply: field sums U_c (13) and EFG / dipole-force sums V_cd (22)
+APPLY_FIELD = 13  # phase step 6, recarg 3, R0 1, R3 2, n3 1
+APPLY_EFG = 22    # + recarg2 3, Q0 1, Q3 2, Q33 3
+
+# bytes per unit moved by the hot k-space kernels from HBM/L2 are O(1/K) and
+# O(1/N): both are compute bound (arithmetic intensity > 1e3 flop/B).
+
+KERNEL_FLOPS_PER_UNIT = {
+    "k_rho64<chg>": RHO_CHG,
+    "k_rho64<dip>": RHO_DIP,
+    "k_rho64<chg+dip>": RHO_BOTH,
+    "k_apply64<field>": APPLY_FIELD,
+    "k_apply64<efg>": APPLY_EFG,
+    "mx_rho": None,     # MFMA path, flops from the GEMM shape
+    "mx_apply": None,
+}
+
+# MI355X published peaks
+PEAK_FP64_TFLOPS = 78.6     # FP64 vector and FP64 matrix
+PEAK_FP32_TFLOPS = 157.3    # FP32 vector (packed)
+PEAK_FP16_MFMA_TFLOPS = 2500.0  # dense FP16/BF16 matrix
+PEAK_HBM_GBS = 8000.0

@@ -1,0 +1,1152 @@
+/*
+ * mdff_oracle.c -- CPU FP64 restatement of MDFF's force/field evaluation.
+ *
+ * TEST INFRASTRUCTURE ONLY (see mdff_oracle.h).  Plain C99, double
+ * precision, no vectorisation tricks: each function follows the reference
+ * routine cited above it.  Citations are relative to the MDFF source tree.
+ *
+ * Parity-unpinned parts (no golden vector or closed form in the reference's
+ * own tests reaches them; they are pinned here only by the invariants in
+ * tests/test_oracle_*.py):
+ *   - Morse: the reference routine is never dispatched and reads
+ *     uninitialised erh2/u2 (src/field.f90:3124,3132); this file evaluates
+ *     the reference's energy expression and its exact derivative.
+ *   - BMHFT without damping (lbmhft): the reference reads uninitialised
+ *     fdiff6/fdiff8 (src/field.f90:2900,2975); this file uses 0.
+ */
+#include "mdff_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NT MDFFGPU_NTYPEMAX
+#define IX2(i, j) ((i) + NT * (j))
+#define IX3(a, b, c) ((a) + NT * ((b) + NT * (c)))
+
+static const double PI = 3.14159265358979323846264338327950;
+
+/* src/functions.f90:28-40 */
+double orc_errfc(double x) {
+  double z = fabs(x);
+  double t = 1.0 / (1.0 + 0.5 * z);
+  double r =
+      t * exp(-z * z - 1.26551223 +
+              t * (1.00002368 +
+                   t * (.37409196 +
+                        t * (.09678418 +
+                             t * (-.18628806 +
+                                  t * (.27886807 +
+                                       t * (-1.13520398 +
+                                            t * (1.48851587 +
+                                                 t * (-.82215223 +
+                                                      t * .17087277)))))))));
+  if (x < 0.0) r = 2.0 - r;
+  return r;
+}
+
+/* E_TT(k) = 1/k!, src/tt_damp.f90:13-29 */
+static double e_tt(int k) {
+  double e = 1.0;
+  for (int i = 1; i <= k; ++i) e = e / (double)i;
+  return e;
+}
+
+/* src/field.f90:3775-3805 */
+void orc_tt_damp(double b, double c, double r, int order, double* f,
+                 double* fd) {
+  double br = b * r;
+  double expbdr = exp(-br) * c;
+  double ff = e_tt(order);
+  for (int k = order - 1; k >= 1; --k) ff = ff * br + e_tt(k);
+  ff = ff * br + e_tt(0);
+  *f = 1.0 - ff * expbdr;
+  double brn = 1.0;
+  for (int k = 0; k < order; ++k) brn *= br;
+  *fd = e_tt(order) * brn * expbdr * b;
+}
+
+/* src/lattice.f90:60-168 (EXPRO cross products, volume, widths) */
+void orc_lattice(const double* A, orc_cell* c) {
+  memcpy(c->A, A, 9 * sizeof(double));
+  const double* a1 = A;
+  const double* a2 = A + 3;
+  const double* a3 = A + 6;
+  double* B = c->B;
+  /* B(:,1) = a2 x a3, B(:,2) = a3 x a1, B(:,3) = a1 x a2 */
+  B[0] = a2[1] * a3[2] - a2[2] * a3[1];
+  B[1] = a2[2] * a3[0] - a2[0] * a3[2];
+  B[2] = a2[0] * a3[1] - a2[1] * a3[0];
+  B[3] = a3[1] * a1[2] - a3[2] * a1[1];
+  B[4] = a3[2] * a1[0] - a3[0] * a1[2];
+  B[5] = a3[0] * a1[1] - a3[1] * a1[0];
+  B[6] = a1[1] * a2[2] - a1[2] * a2[1];
+  B[7] = a1[2] * a2[0] - a1[0] * a2[2];
+  B[8] = a1[0] * a2[1] - a1[1] * a2[0];
+  double omega = B[0] * A[0] + B[1] * A[1] + B[2] * A[2];
+  c->omega = omega;
+  for (int i = 0; i < 3; ++i) {
+    const double* b = B + 3 * i;
+    c->W[i] = omega / sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  }
+  for (int i = 0; i < 9; ++i) B[i] = B[i] / omega;
+  for (int i = 0; i < 3; ++i) {
+    const double* a = A + 3 * i;
+    c->anorm[i] = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  }
+}
+
+/* src/field.f90:1648 */
+int orc_kmesh_count(const int* k) {
+  return k[2] + k[1] * (2 * k[2] + 1) + k[0] * (2 * k[1] + 1) * (2 * k[2] + 1);
+}
+
+/* stable merge sort of idx by key (src/tools.f90:691-734 keeps equal keys
+   in input order) */
+static void msort(const double* key, int* idx, int* tmp, int n) {
+  if (n < 2) return;
+  int na = (n + 1) / 2;
+  msort(key, idx, tmp, na);
+  msort(key, idx + na, tmp, n - na);
+  int i = 0, j = na, k = 0;
+  while (i < na && j < n) {
+    if (key[idx[i]] <= key[idx[j]])
+      tmp[k++] = idx[i++];
+    else
+      tmp[k++] = idx[j++];
+  }
+  while (i < na) tmp[k++] = idx[i++];
+  while (j < n) tmp[k++] = idx[j++];
+  memcpy(idx, tmp, n * sizeof(int));
+}
+
+/* src/kspace.f90:142-207, reorder_kpt :287-345 */
+int orc_kmesh(const orc_cell* c, const int* kmax, double alpha, double* kxo,
+              double* kyo, double* kzo, double* kko, double* Ako,
+              double* kcoeo, int* nvec) {
+  int nk = orc_kmesh_count(kmax);
+  double* buf = (double*)malloc(6 * (size_t)nk * sizeof(double));
+  int* nv = (int*)malloc(3 * (size_t)nk * sizeof(int));
+  int* idx = (int*)malloc(2 * (size_t)nk * sizeof(int));
+  if (!buf || !nv || !idx) {
+    free(buf);
+    free(nv);
+    free(idx);
+    return -1;
+  }
+  double *kx = buf, *ky = buf + nk, *kz = buf + 2 * nk, *kk = buf + 3 * nk,
+         *Ak = buf + 4 * nk, *kc = buf + 5 * nk;
+  const double tpi = 2.0 * PI;
+  const double* B = c->B;
+  double alpha2 = alpha * alpha;
+  int n = 0;
+  for (int nx = 0; nx <= kmax[0]; ++nx) {
+    int nymin = (nx == 0) ? 0 : -kmax[1];
+    for (int ny = nymin; ny <= kmax[1]; ++ny) {
+      int nzmin = (nx == 0 && ny == 0) ? 1 : -kmax[2];
+      for (int nz = nzmin; nz <= kmax[2]; ++nz) {
+        double x = (double)nx, y = (double)ny, z = (double)nz;
+        double vx = tpi * (x * B[0] + y * B[3] + z * B[6]);
+        double vy = tpi * (x * B[1] + y * B[4] + z * B[7]);
+        double vz = tpi * (x * B[2] + y * B[5] + z * B[8]);
+        double k2 = vx * vx + vy * vy + vz * vz;
+        kx[n] = vx;
+        ky[n] = vy;
+        kz[n] = vz;
+        kk[n] = k2;
+        Ak[n] = exp(-k2 * 0.25 / alpha2) / k2;
+        kc[n] = 2.0 * (1.0 / k2 + 1.0 / alpha2 / 4.0);
+        nv[3 * n] = nx;
+        nv[3 * n + 1] = ny;
+        nv[3 * n + 2] = nz;
+        ++n;
+      }
+    }
+  }
+  for (int i = 0; i < nk; ++i) idx[i] = i;
+  msort(kk, idx, idx + nk, nk);
+  for (int i = 0; i < nk; ++i) {
+    int l = idx[i];
+    kxo[i] = kx[l];
+    kyo[i] = ky[l];
+    kzo[i] = kz[l];
+    kko[i] = kk[l];
+    Ako[i] = Ak[l];
+    kcoeo[i] = kc[l];
+    if (nvec) {
+      nvec[3 * i] = nv[3 * l];
+      nvec[3 * i + 1] = nv[3 * l + 1];
+      nvec[3 * i + 2] = nv[3 * l + 2];
+    }
+  }
+  free(buf);
+  free(nv);
+  free(idx);
+  return nk;
+}
+
+/* src/field.f90:860-908 (dl_poly-like branch used with lautoES) */
+void orc_ewald_param(const orc_cell* c, double epsw, double* cutlongrange,
+                     double* alpha, int* kmax) {
+  double rcut = 0.5 * fmin(c->W[0], fmin(c->W[1], c->W[2]));
+  *cutlongrange = fmin(*cutlongrange, rcut);
+  double eps = fmin(fabs(epsw), 0.5);
+  double tol = sqrt(fabs(log(eps * (*cutlongrange))));
+  double a = sqrt(fabs(log(eps * (*cutlongrange) * tol))) / rcut;
+  double tol1 =
+      sqrt(-log(eps * (*cutlongrange) * (2.0 * tol * a) * (2.0 * tol * a)));
+  for (int i = 0; i < 3; ++i)
+    kmax[i] = (int)lround(0.25 + c->anorm[i] * a * tol1 / PI);
+  *alpha = a;
+}
+
+/* cartesian -> fractional, kardir with BASIS=B (src/lattice.f90:183-213) */
+static void kardir(const orc_cell* c, int n, const double* rx,
+                   const double* ry, const double* rz, double* s) {
+  const double* B = c->B;
+  for (int i = 0; i < n; ++i) {
+    s[3 * i + 0] = rx[i] * B[0] + ry[i] * B[1] + rz[i] * B[2];
+    s[3 * i + 1] = rx[i] * B[3] + ry[i] * B[4] + rz[i] * B[5];
+    s[3 * i + 2] = rx[i] * B[6] + ry[i] * B[7] + rz[i] * B[8];
+  }
+}
+
+/* minimum image in fractional coordinates, then back to Cartesian
+   (src/field.f90:1268-1273) */
+static double min_image(const orc_cell* c, const double* si, const double* sj,
+                        double* r) {
+  const double* A = c->A;
+  double sx = si[0] - sj[0], sy = si[1] - sj[1], sz = si[2] - sj[2];
+  sx = sx - round(sx);
+  sy = sy - round(sy);
+  sz = sz - round(sz);
+  r[0] = sx * A[0] + sy * A[3] + sz * A[6];
+  r[1] = sx * A[1] + sy * A[4] + sz * A[7];
+  r[2] = sx * A[2] + sy * A[5] + sz * A[8];
+  return r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+}
+
+typedef struct sr_out {
+  double u, vir, tau[9];
+} sr_out;
+
+/* engforce_nmlj_pbc, src/field.f90:1181-1345, tables of
+   initialize_param_non_bonded :914-1101 */
+static void lj(const mdffgpu_params* p, const int* it, const orc_cell* c,
+               const double* s, double* fx, double* fy, double* fz,
+               sr_out* o) {
+  int n = p->natm;
+  double rcutsq = p->cutshortrange * p->cutshortrange;
+  double two16 = pow(2.0, 1.0 / 6.0);
+  memset(o, 0, sizeof(*o));
+  for (int i = 0; i < n; ++i) fx[i] = fy[i] = fz[i] = 0.0;
+  for (int ia = 0; ia < n; ++ia) {
+    for (int ja = ia + 1; ja < n; ++ja) {
+      double r[3];
+      double rijsq = min_image(c, s + 3 * ia, s + 3 * ja, r);
+      int t = IX2(it[ia], it[ja]);
+      if (!(rijsq < rcutsq)) continue;
+      double pp = p->plj[t], qq = p->qlj[t];
+      double epsp = p->epslj[t] / (qq - pp);
+      double sigsq = two16 * two16 * p->sigmalj[t] * p->sigmalj[t];
+      double fc = pp * qq * epsp / sigsq;
+      double sr2 = sigsq / rijsq;
+      double srp = pow(sr2, pp * 0.5);
+      double srq = pow(sr2, qq * 0.5);
+      double uu = epsp * (pp * srq - qq * srp);
+      if (p->trunc == 0) {
+        o->u = o->u + uu;
+      } else {
+        double sr2c = sigsq / rcutsq;
+        double src = sqrt(sr2c);
+        double srpc = pow(src, pp), srqc = pow(src, qq);
+        if (p->trunc == 1) {
+          double uc = epsp * (pp * srqc - qq * srpc);
+          o->u = o->u + uu - uc;
+        } else {
+          double uc1 = epsp * (pp * qq) / (2.0 * rcutsq) * (srqc - srpc);
+          double uc2 = 0.5 * epsp *
+                       ((2.0 * pp + pp * qq) * srqc - (2.0 * qq + pp * qq) * srpc);
+          o->u = o->u + uu + uc1 * rijsq - uc2;
+        }
+      }
+      double wij = fc * (srq - srp) * sr2;
+      double f[3] = {wij * r[0], wij * r[1], wij * r[2]};
+      o->vir = o->vir + wij * rijsq;
+      fx[ia] += f[0];
+      fy[ia] += f[1];
+      fz[ia] += f[2];
+      fx[ja] -= f[0];
+      fy[ja] -= f[1];
+      fz[ja] -= f[2];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          o->tau[a + 3 * b] += (r[a] * f[b] + r[b] * f[a]) * 0.5;
+    }
+  }
+  for (int k = 0; k < 9; ++k) o->tau[k] = o->tau[k] / c->omega / p->press_unit;
+  o->vir = o->vir / 3.0;
+}
+
+/* engforce_bmhftd_pbc, src/field.f90:2876-3026 */
+static void bmhftd(const mdffgpu_params* p, const int* it, const orc_cell* c,
+                   const double* s, double* fx, double* fy, double* fz,
+                   sr_out* o) {
+  int n = p->natm;
+  double rcutsq = p->cutshortrange * p->cutshortrange;
+  memset(o, 0, sizeof(*o));
+  for (int i = 0; i < n; ++i) fx[i] = fy[i] = fz[i] = 0.0;
+  for (int ia = 0; ia < n; ++ia) {
+    for (int ja = ia + 1; ja < n; ++ja) {
+      double r[3];
+      double rijsq = min_image(c, s + 3 * ia, s + 3 * ja, r);
+      int t = IX2(it[ia], it[ja]);
+      if (p->Abmhftd[t] == 0.0) continue;
+      if (!(rijsq < rcutsq)) continue;
+      double ir2 = 1.0 / rijsq;
+      double d = sqrt(rijsq);
+      double erh = p->Abmhftd[t] * exp(-p->Bbmhftd[t] * d);
+      double ir6 = ir2 * ir2 * ir2;
+      double ir8 = ir6 * ir2;
+      ir6 = ir6 * p->Cbmhftd[t];
+      ir8 = ir8 * p->Dbmhftd[t];
+      double f6 = 1.0, f8 = 1.0, fd6 = 0.0, fd8 = 0.0, ir6d = ir6, ir8d = ir8;
+      if (p->lbmhftd) {
+        orc_tt_damp(p->BDbmhftd[t], 1.0, d, 6, &f6, &fd6);
+        orc_tt_damp(p->BDbmhftd[t], 1.0, d, 8, &f8, &fd8);
+        ir6d = ir6 * f6;
+        ir8d = ir8 * f8;
+      }
+      double ir7 = 6.0 * ir6d / d;
+      double ir9 = 8.0 * ir8d / d;
+      o->u = o->u + erh - ir6d - ir8d;
+      double wij = p->Bbmhftd[t] * erh - ir7 - ir9 + ir6 * fd6 + ir8 * fd8;
+      wij = wij / d;
+      double f[3] = {wij * r[0], wij * r[1], wij * r[2]};
+      o->vir = o->vir + (r[0] * f[0] + r[1] * f[1] + r[2] * f[2]);
+      fx[ia] += f[0];
+      fy[ia] += f[1];
+      fz[ia] += f[2];
+      fx[ja] -= f[0];
+      fy[ja] -= f[1];
+      fz[ja] -= f[2];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) o->tau[a + 3 * b] += (r[a] * f[b] + r[b] * f[a]);
+    }
+  }
+  o->vir = o->vir / 3.0;
+  for (int k = 0; k < 9; ++k)
+    o->tau[k] = o->tau[k] / c->omega / p->press_unit * 0.5;
+}
+
+/* Morse: energy expression of engforce_morse_pbc (src/field.f90:3120-3122),
+   V = eps*((exp(-rho*(r-sigma)) - 1)^2 - 1), with its exact derivative.
+   The reference's force line (:3132) reads uninitialised erh2 and omits
+   1/r; it is not reproduced.  Forces and stress are added (accumulate). */
+static void morse(const mdffgpu_params* p, const int* it, const orc_cell* c,
+                  const double* s, double* fx, double* fy, double* fz,
+                  sr_out* o) {
+  int n = p->natm;
+  double rcutsq = p->cutshortrange * p->cutshortrange;
+  memset(o, 0, sizeof(*o));
+  for (int ia = 0; ia < n; ++ia) {
+    for (int ja = ia + 1; ja < n; ++ja) {
+      double r[3];
+      double rijsq = min_image(c, s + 3 * ia, s + 3 * ja, r);
+      int t = IX2(it[ia], it[ja]);
+      if (!(rijsq < rcutsq)) continue;
+      double rho = p->rhomor[t], eps = p->epsmor[t], sig = p->sigmamor[t];
+      double d = sqrt(rijsq);
+      double rs = exp(rho * sig);
+      double erh = exp(-d * rho);
+      double X = erh * rs;
+      double expon = (X - 1.0) * (X - 1.0) - 1.0;
+      o->u = o->u + eps * expon;
+      double wij = 2.0 * eps * rho * X * (X - 1.0) / d;
+      double f[3] = {wij * r[0], wij * r[1], wij * r[2]};
+      o->vir = o->vir + wij * rijsq;
+      fx[ia] += f[0];
+      fy[ia] += f[1];
+      fz[ia] += f[2];
+      fx[ja] -= f[0];
+      fy[ja] -= f[1];
+      fz[ja] -= f[2];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) o->tau[a + 3 * b] += r[a] * f[b];
+    }
+  }
+  o->vir = o->vir / 3.0;
+  for (int k = 0; k < 9; ++k) o->tau[k] = o->tau[k] / c->omega / p->press_unit;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Ewald multipole sum                                                     */
+
+typedef struct es_ctx {
+  const mdffgpu_params* p;
+  const int* it;
+  orc_cell cell;
+  int nk;
+  double *kx, *ky, *kz, *kk, *Ak, *kcoe;
+  double* s;
+} es_ctx;
+
+static int es_init(es_ctx* e, const mdffgpu_params* p, const int* it,
+                   const double* A, const double* rx, const double* ry,
+                   const double* rz) {
+  memset(e, 0, sizeof(*e));
+  e->p = p;
+  e->it = it;
+  orc_lattice(A, &e->cell);
+  e->nk = orc_kmesh_count(p->kES);
+  e->kx = (double*)malloc(6 * (size_t)e->nk * sizeof(double));
+  e->s = (double*)malloc(3 * (size_t)p->natm * sizeof(double));
+  if (!e->kx || !e->s) return MDFFGPU_ERR_ALLOC;
+  e->ky = e->kx + e->nk;
+  e->kz = e->kx + 2 * e->nk;
+  e->kk = e->kx + 3 * e->nk;
+  e->Ak = e->kx + 4 * e->nk;
+  e->kcoe = e->kx + 5 * e->nk;
+  orc_kmesh(&e->cell, p->kES, p->alphaES, e->kx, e->ky, e->kz, e->kk, e->Ak,
+            e->kcoe, NULL);
+  kardir(&e->cell, p->natm, rx, ry, rz, e->s);
+  return 0;
+}
+
+static void es_free(es_ctx* e) {
+  free(e->kx);
+  free(e->s);
+}
+
+/* multipole_ES_dir, src/field.f90:2057-2643 (charges and dipoles; the
+   quadrupole tasks are not supported and their terms are zero here).
+   mu is (natm,3) Fortran order; ef (natm,3), efg (natm,3,3), f (natm) are
+   accumulated; tau (9) accumulated before the final scaling. */
+static void es_dir(const es_ctx* e, const double* q, const double* mu,
+                   const int* task, int damp_ind, int do_efield, int do_efg,
+                   int do_forces, int do_stress, double* u_dir, double* ef,
+                   double* efg, double* fx, double* fy, double* fz,
+                   double* tau) {
+  const mdffgpu_params* p = e->p;
+  int n = p->natm;
+  const int* it = e->it;
+  double alpha = p->alphaES;
+  double cutsq = p->cutlongrange * p->cutlongrange;
+  double alpha2 = alpha * alpha, alpha3 = alpha2 * alpha, alpha5 = alpha3 * alpha2;
+  double piroot = sqrt(PI);
+  double u = 0.0;
+  double taud[9] = {0};
+#define EF(i, a) ef[(i) + n * (a)]
+#define EFG(i, a, b) efg[(i) + n * ((a) + 3 * (b))]
+  for (int ia = 0; ia < n; ++ia) {
+    int ita = it[ia];
+    double qi = q[ia];
+    double mui[3] = {mu[ia], mu[ia + n], mu[ia + 2 * n]};
+    int dip_i = (mui[0] != 0.0 || mui[1] != 0.0 || mui[2] != 0.0);
+    for (int ja = ia + 1; ja < n; ++ja) {
+      int jta = it[ja];
+      double qj = q[ja];
+      double muj[3] = {mu[ja], mu[ja + n], mu[ja + 2 * n]};
+      int dip_j = (muj[0] != 0.0 || muj[1] != 0.0 || muj[2] != 0.0);
+      double qij = qi * qj;
+      double rij[3];
+      double d2 = min_image(&e->cell, e->s + 3 * ia, e->s + 3 * ja, rij);
+      if (d2 > cutsq) continue;
+      double d = sqrt(d2);
+      double d3 = d2 * d;
+      double d5 = d3 * d2;
+      double dm1 = 1.0 / d;
+      double dm3 = dm1 / d2;
+      double dm5 = dm3 / d2;
+      double dm7 = dm5 / d2;
+      int ldamp = p->ldip_damping[IX3(ita, ita, jta)] ||
+                  p->ldip_damping[IX3(jta, ita, jta)];
+      if (!damp_ind) ldamp = 0;
+      double fdamp = 1.0, fdamp2 = 1.0, fdd = 0.0, fdd2 = 0.0;
+      if (ldamp) {
+        int k1 = IX3(ita, ita, jta), k2 = IX3(jta, ita, jta);
+        orc_tt_damp(p->dip_pol_damp_b[k1], p->dip_pol_damp_c[k1], d,
+                    p->dip_pol_damp_k[k1], &fdamp, &fdd);
+        orc_tt_damp(p->dip_pol_damp_b[k2], p->dip_pol_damp_c[k2], d,
+                    p->dip_pol_damp_k[k2], &fdamp2, &fdd2);
+      }
+      double expon = exp(-alpha2 * d2) / piroot;
+      double F0 = orc_errfc(alpha * d);
+      double F1 = F0 + 2.0 * alpha * d * expon;
+      double F2 = F1 + 4.0 * alpha3 * d3 * expon / 3.0;
+      double F3 = F2 + 8.0 * alpha5 * d5 * expon / 15.0;
+      double F1d = -fdamp + 1.0;
+      double F2d = F1d + (d / 3.0) * fdd;
+      double F1d2 = -fdamp2 + 1.0;
+      double F2d2 = F1d2 + (d / 3.0) * fdd2;
+      double T0 = dm1 * F0;
+      double T1[3], T1d[3] = {0}, T1d2[3] = {0};
+      for (int a = 0; a < 3; ++a) {
+        T1[a] = -rij[a] * dm3;
+        if (ldamp) {
+          T1d[a] = T1[a] * F1d;
+          T1d2[a] = T1[a] * F1d2;
+        }
+        T1[a] = T1[a] * F1;
+      }
+      double F2_dm5 = 3.0 * F2 * dm5, F2d_dm5 = 3.0 * F2d * dm5,
+             F2d2_dm5 = 3.0 * F2d2 * dm5;
+      double F1_dm3 = F1 * dm3, F1d_dm3 = F1d * dm3, F1d2_dm3 = F1d2 * dm3;
+      double T2[3][3], T2d[3][3] = {{0}}, T2d2[3][3] = {{0}};
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          T2[a][b] = rij[a] * rij[b] * F2_dm5;
+          if (a == b) T2[a][a] = T2[a][a] - F1_dm3;
+          if (ldamp) {
+            T2d[a][b] = rij[a] * rij[b] * F2d_dm5;
+            T2d2[a][b] = rij[a] * rij[b] * F2d2_dm5;
+            if (a == b) {
+              T2d[a][a] = T2d[a][a] - F1d_dm3;
+              T2d2[a][a] = T2d2[a][a] - F1d2_dm3;
+            }
+          }
+        }
+      double F3_dm7 = F3 * dm7 * 15.0;
+      double T3[3][3][3];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          for (int g = 0; g < 3; ++g) {
+            double v = -rij[a] * rij[b] * rij[g] * F3_dm7;
+            if (a == b) v = v + rij[g] * F2_dm5;
+            if (a == g) v = v + rij[b] * F2_dm5;
+            if (b == g) v = v + rij[a] * F2_dm5;
+            T3[a][b][g] = v;
+          }
+      double fij[3] = {0, 0, 0};
+      /* charge-charge, :2363-2403 */
+      if (task[0]) {
+        u = u + qij * T0;
+        if (do_efield) {
+          for (int a = 0; a < 3; ++a) {
+            EF(ia, a) = EF(ia, a) - qj * T1[a];
+            EF(ja, a) = EF(ja, a) + qi * T1[a];
+          }
+          if (ldamp)
+            for (int a = 0; a < 3; ++a) {
+              EF(ia, a) = EF(ia, a) + qj * T1d[a];
+              EF(ja, a) = EF(ja, a) - qi * T1d2[a];
+            }
+        }
+        if (do_efg)
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+              EFG(ia, a, b) = EFG(ia, a, b) - qj * T2[a][b];
+              EFG(ja, a, b) = EFG(ja, a, b) - qi * T2[a][b];
+            }
+        if (do_forces) {
+          for (int a = 0; a < 3; ++a) fij[a] = qij * T1[a];
+          fx[ia] -= fij[0];
+          fy[ia] -= fij[1];
+          fz[ia] -= fij[2];
+          fx[ja] += fij[0];
+          fy[ja] += fij[1];
+          fz[ja] += fij[2];
+        }
+        if (do_stress)
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+              taud[a + 3 * b] = taud[a + 3 * b] - (rij[a] * fij[b] + rij[b] * fij[a]);
+      }
+      /* dipole-dipole, :2407-2462 */
+      if (task[2] && (dip_i || dip_j)) {
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) u = u - mui[a] * T2[a][b] * muj[b];
+        if (do_efield)
+          for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a) {
+              EF(ia, a) = EF(ia, a) + T2[a][b] * muj[b];
+              EF(ja, a) = EF(ja, a) + T2[a][b] * mui[b];
+            }
+        if (do_efg)
+          for (int g = 0; g < 3; ++g)
+            for (int b = 0; b < 3; ++b)
+              for (int a = 0; a < 3; ++a) {
+                EFG(ia, a, b) = EFG(ia, a, b) + T3[a][b][g] * muj[g];
+                EFG(ja, a, b) = EFG(ja, a, b) - T3[a][b][g] * mui[g];
+              }
+        if (do_forces) {
+          fij[0] = fij[1] = fij[2] = 0.0;
+          for (int a = 0; a < 3; ++a)
+            for (int g = 0; g < 3; ++g) {
+              fij[0] = fij[0] + mui[a] * T3[a][0][g] * muj[g];
+              fij[1] = fij[1] + mui[a] * T3[a][1][g] * muj[g];
+              fij[2] = fij[2] + mui[a] * T3[a][2][g] * muj[g];
+            }
+          fx[ia] += fij[0];
+          fy[ia] += fij[1];
+          fz[ia] += fij[2];
+          fx[ja] -= fij[0];
+          fy[ja] -= fij[1];
+          fz[ja] -= fij[2];
+        }
+        if (do_stress)
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+              taud[a + 3 * b] = taud[a + 3 * b] + (rij[a] * fij[b] + rij[b] * fij[a]);
+      }
+      /* charge-dipole, :2468-2538 */
+      if (task[1] && (dip_i || dip_j)) {
+        for (int g = 0; g < 3; ++g) {
+          u = u - qi * T1[g] * muj[g];
+          u = u + qj * T1[g] * mui[g];
+          if (ldamp) {
+            u = u + qi * T1d2[g] * muj[g];
+            u = u - qj * T1d[g] * mui[g];
+          }
+        }
+        if (do_forces) {
+          fij[0] = fij[1] = fij[2] = 0.0;
+          for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a) {
+              fij[a] = fij[a] + qi * T2[a][b] * muj[b];
+              fij[a] = fij[a] - qj * T2[a][b] * mui[b];
+            }
+          fx[ia] += fij[0];
+          fy[ia] += fij[1];
+          fz[ia] += fij[2];
+          fx[ja] -= fij[0];
+          fy[ja] -= fij[1];
+          fz[ja] -= fij[2];
+        }
+        if (do_stress)
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+              taud[a + 3 * b] = taud[a + 3 * b] + (rij[a] * fij[b] + rij[b] * fij[a]);
+        if (ldamp) {
+          if (do_forces) {
+            fij[0] = fij[1] = fij[2] = 0.0;
+            for (int b = 0; b < 3; ++b)
+              for (int a = 0; a < 3; ++a) {
+                fij[a] = fij[a] + qj * T2d[a][b] * mui[b];
+                fij[a] = fij[a] - qi * T2d2[a][b] * muj[b];
+              }
+            fx[ia] += fij[0];
+            fy[ia] += fij[1];
+            fz[ia] += fij[2];
+            fx[ja] -= fij[0];
+            fy[ja] -= fij[1];
+            fz[ja] -= fij[2];
+          }
+          if (do_stress)
+            for (int a = 0; a < 3; ++a)
+              for (int b = 0; b < 3; ++b)
+                taud[a + 3 * b] =
+                    taud[a + 3 * b] + (rij[a] * fij[b] + rij[b] * fij[a]);
+        }
+      }
+    }
+  }
+  if (do_stress)
+    for (int k = 0; k < 9; ++k) tau[k] = taud[k] / e->cell.omega * 0.5;
+  *u_dir = u;
+#undef EF
+#undef EFG
+}
+
+/* multipole_ES_rec, src/field.f90:2646-2873 (Cartesian positions) */
+static void es_rec(const es_ctx* e, const double* rx, const double* ry,
+                   const double* rz, const double* q, const double* mu,
+                   const int* task, int do_efield, int do_efg, int do_forces,
+                   int do_stress, double* u_rec, double* ef, double* efg,
+                   double* fx, double* fy, double* fz, double* tau) {
+  const mdffgpu_params* p = e->p;
+  int n = p->natm;
+  double tpi_V = 2.0 * PI / e->cell.omega;
+  double fpi_V = tpi_V * 2.0;
+  int ldip = task[1] || task[2];
+  double* ckr = (double*)malloc(2 * (size_t)n * sizeof(double));
+  double* skr = ckr + n;
+  double u = 0.0;
+  double t[9] = {0};
+#define EF(i, a) ef[(i) + n * (a)]
+#define EFG(i, a, b) efg[(i) + n * ((a) + 3 * (b))]
+  for (int ik = 0; ik < e->nk; ++ik) {
+    if (e->kk[ik] == 0.0) continue;
+    double kx = e->kx[ik], ky = e->ky[ik], kz = e->kz[ik];
+    double Ak = e->Ak[ik], kcoe = e->kcoe[ik];
+    double rR = 0.0, rI = 0.0;
+    for (int ia = 0; ia < n; ++ia) {
+      double qi = q[ia];
+      double kr = (kx * rx[ia] + ky * ry[ia] + kz * rz[ia]);
+      ckr[ia] = cos(kr);
+      skr[ia] = sin(kr);
+      rR = rR + qi * ckr[ia];
+      rI = rI + qi * skr[ia];
+      if (!ldip) continue;
+      double kmu = (mu[ia] * kx + mu[ia + n] * ky + mu[ia + 2 * n] * kz);
+      rR = rR - kmu * skr[ia];
+      rI = rI + kmu * ckr[ia];
+    }
+    double str = (rR * rR + rI * rI) * Ak;
+    u = u + str;
+    for (int ia = 0; ia < n; ++ia) {
+      double fxij = 0, fyij = 0, fzij = 0;
+      if (do_efield || do_forces) {
+        double recarg = Ak * (rI * ckr[ia] - rR * skr[ia]);
+        fxij = kx * recarg;
+        fyij = ky * recarg;
+        fzij = kz * recarg;
+      }
+      if (do_efield) {
+        EF(ia, 0) = EF(ia, 0) - fxij;
+        EF(ia, 1) = EF(ia, 1) - fyij;
+        EF(ia, 2) = EF(ia, 2) - fzij;
+      }
+      if (do_efg) {
+        double recarg2 = Ak * (rR * ckr[ia] + rI * skr[ia]);
+        EFG(ia, 0, 0) = EFG(ia, 0, 0) + kx * kx * recarg2;
+        EFG(ia, 1, 1) = EFG(ia, 1, 1) + ky * ky * recarg2;
+        EFG(ia, 2, 2) = EFG(ia, 2, 2) + kz * kz * recarg2;
+        EFG(ia, 0, 1) = EFG(ia, 0, 1) + kx * ky * recarg2;
+        EFG(ia, 0, 2) = EFG(ia, 0, 2) + kx * kz * recarg2;
+        EFG(ia, 1, 2) = EFG(ia, 1, 2) + ky * kz * recarg2;
+      }
+      if (do_forces) {
+        double qi = q[ia];
+        fx[ia] = fx[ia] - qi * fxij;
+        fy[ia] = fy[ia] - qi * fyij;
+        fz[ia] = fz[ia] - qi * fzij;
+        if (!ldip) continue;
+        double recarg2 = Ak * (rR * ckr[ia] + rI * skr[ia]);
+        double kmu = (mu[ia] * kx + mu[ia + n] * ky + mu[ia + 2 * n] * kz) * recarg2;
+        fx[ia] = fx[ia] + kx * kmu;
+        fy[ia] = fy[ia] + ky * kmu;
+        fz[ia] = fz[ia] + kz * kmu;
+      }
+    }
+    if (do_stress) {
+      double k3[3] = {kx, ky, kz};
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          if (a == b)
+            t[a + 3 * b] = t[a + 3 * b] + (1.0 - kcoe * k3[a] * k3[b]) * str;
+          else
+            t[a + 3 * b] = t[a + 3 * b] - kcoe * k3[a] * k3[b] * str;
+        }
+    }
+  }
+  /* half mesh and units, :2820-2867 */
+  u = u * 2.0;
+  if (do_efield)
+    for (int i = 0; i < 3 * n; ++i) ef[i] = ef[i] * 2.0 * fpi_V;
+  if (do_efg)
+    for (int i = 0; i < 9 * n; ++i) efg[i] = efg[i] * 2.0 * fpi_V;
+  if (do_forces)
+    for (int i = 0; i < n; ++i) {
+      fx[i] = fx[i] * 2.0 * fpi_V;
+      fy[i] = fy[i] * 2.0 * fpi_V;
+      fz[i] = fz[i] * 2.0 * fpi_V;
+    }
+  if (do_stress)
+    for (int k = 0; k < 9; ++k) tau[k] = t[k] * 2.0 * tpi_V / e->cell.omega;
+  *u_rec = u * tpi_V;
+  free(ckr);
+#undef EF
+#undef EFG
+}
+
+/* multipole_ES, src/field.f90:1817-2054.  ef/efg overwritten, f added. */
+static int es_call(const es_ctx* e, const double* rx, const double* ry,
+                   const double* rz, const double* q, const double* mu,
+                   const int* task, int damp_ind, int do_efield, int do_efg,
+                   int do_forces, int do_stress, double u_pol, double* ef,
+                   double* efg, double* fx, double* fy, double* fz,
+                   double* u_coul, double* tau_coul, double* pvirial,
+                   double* parts) {
+  const mdffgpu_params* p = e->p;
+  int n = p->natm;
+  size_t n3 = 3 * (size_t)n, n9 = 9 * (size_t)n;
+  double* buf = (double*)calloc(2 * n3 + 2 * n9 + 6 * (size_t)n, sizeof(double));
+  if (!buf) return MDFFGPU_ERR_ALLOC;
+  double *ef_dir = buf, *ef_rec = ef_dir + n3, *efg_dir = ef_rec + n3,
+         *efg_rec = efg_dir + n9, *fdx = efg_rec + n9, *fdy = fdx + n,
+         *fdz = fdy + n, *frx = fdz + n, *fry = frx + n, *frz = fry + n;
+  double qtot[3] = {0, 0, 0}, mutot[3] = {0, 0, 0}, musq = 0.0, qsq = 0.0;
+  for (int ia = 0; ia < n; ++ia) {
+    double m0 = mu[ia], m1 = mu[ia + n], m2 = mu[ia + 2 * n];
+    mutot[0] += m0;
+    mutot[1] += m1;
+    mutot[2] += m2;
+    musq = musq + (m0 * m0 + m1 * m1 + m2 * m2);
+    qtot[0] = qtot[0] + q[ia] * rx[ia];
+    qtot[1] = qtot[1] + q[ia] * ry[ia];
+    qtot[2] = qtot[2] + q[ia] * rz[ia];
+    qsq = qsq + q[ia] * q[ia];
+  }
+  double qmu[3] = {qtot[0] + mutot[0], qtot[1] + mutot[1], qtot[2] + mutot[2]};
+  double tpi_V = 2.0 * PI / e->cell.omega;
+  double tpi_3V = tpi_V / 3.0;
+  double fpi_3V = tpi_3V * 2.0;
+  double alpha2 = p->alphaES * p->alphaES;
+  double selfa = p->alphaES / sqrt(PI);
+  double selfa2 = 2.0 * selfa * alpha2 / 3.0;
+  double u_dir = 0.0, u_rec = 0.0, tau_dir[9] = {0}, tau_rec[9] = {0};
+  es_dir(e, q, mu, task, damp_ind, do_efield, do_efg, do_forces, do_stress,
+         &u_dir, ef_dir, efg_dir, fdx, fdy, fdz, tau_dir);
+  es_rec(e, rx, ry, rz, q, mu, task, do_efield, do_efg, do_forces, do_stress,
+         &u_rec, ef_rec, efg_rec, frx, fry, frz, tau_rec);
+  double u_surf = (qtot[0] * qtot[0] + qtot[1] * qtot[1] + qtot[2] * qtot[2]) +
+                  2.0 * (qtot[0] * mutot[0] + qtot[1] * mutot[1] + qtot[2] * mutot[2]) +
+                  (mutot[0] * mutot[0] + mutot[1] * mutot[1] + mutot[2] * mutot[2]);
+  u_surf = u_surf * tpi_3V;
+  double u_self = -selfa * qsq - selfa2 * musq;
+  double cu = p->coul_unit;
+  if (p->lsurf)
+    *u_coul = (u_dir + u_rec + u_surf + u_self + u_pol) * cu;
+  else
+    *u_coul = (u_dir + u_rec + u_self + u_pol) * cu;
+  for (int a = 0; a < 3; ++a)
+    for (int ia = 0; ia < n; ++ia) {
+      size_t k = ia + (size_t)n * a;
+      double self = 2.0 * selfa2 * mu[k];
+      double v = ef_dir[k] + ef_rec[k];
+      if (p->lsurf) v = v + qmu[a];
+      ef[k] = v + self;
+    }
+  for (int b = 0; b < 3; ++b)
+    for (int a = 0; a < 3; ++a)
+      for (int ia = 0; ia < n; ++ia) {
+        size_t k = ia + (size_t)n * (a + 3 * b);
+        double self = (a == b) ? -2.0 * selfa2 * q[ia] : 0.0;
+        efg[k] = (efg_dir[k] + efg_rec[k] + self) * cu;
+      }
+  for (int k = 0; k < 9; ++k) tau_coul[k] = (tau_dir[k] + tau_rec[k]) * cu / p->press_unit;
+  for (int ia = 0; ia < n; ++ia) {
+    if (p->lsurf) {
+      fx[ia] = fx[ia] + (frx[ia] + fdx[ia] + (-(q[ia] * qmu[0]) * fpi_3V)) * cu;
+      fy[ia] = fy[ia] + (fry[ia] + fdy[ia] + (-(q[ia] * qmu[1]) * fpi_3V)) * cu;
+      fz[ia] = fz[ia] + (frz[ia] + fdz[ia] + (-(q[ia] * qmu[2]) * fpi_3V)) * cu;
+    } else {
+      fx[ia] = fx[ia] + (frx[ia] + fdx[ia]) * cu;
+      fy[ia] = fy[ia] + (fry[ia] + fdy[ia]) * cu;
+      fz[ia] = fz[ia] + (frz[ia] + fdz[ia]) * cu;
+    }
+  }
+  *pvirial = 1.0 / 3.0 * (tau_coul[0] + tau_coul[4] + tau_coul[8]) * p->press_unit;
+  if (parts) {
+    parts[0] = u_dir;
+    parts[1] = u_rec;
+    parts[2] = u_self;
+    parts[3] = u_surf;
+  }
+  free(buf);
+  return 0;
+}
+
+int orc_multipole_es(const mdffgpu_params* p, const int* itype,
+                     const double* A, const double* rx, const double* ry,
+                     const double* rz, const double* q, const double* mu,
+                     const int* task, int damp_ind, int do_efield, int do_efg,
+                     int do_forces, int do_stress, double u_pol, double* ef,
+                     double* efg, double* fx, double* fy, double* fz,
+                     double* u_coul, double* tau_coul, double* pvirial,
+                     double* u_parts) {
+  int n = p->natm;
+  int* it0 = (int*)malloc((size_t)n * sizeof(int));
+  if (!it0) return MDFFGPU_ERR_ALLOC;
+  for (int i = 0; i < n; ++i) it0[i] = itype[i] - 1;
+  es_ctx e;
+  int rc = es_init(&e, p, it0, A, rx, ry, rz);
+  if (!rc)
+    rc = es_call(&e, rx, ry, rz, q, mu, task, damp_ind, do_efield, do_efg,
+                 do_forces, do_stress, u_pol, ef, efg, fx, fy, fz, u_coul,
+                 tau_coul, pvirial, u_parts);
+  es_free(&e);
+  free(it0);
+  return rc;
+}
+
+/* 3x3 inverse by Gauss-Jordan with partial pivoting (the reference uses
+   LAPACK DGETRF/DGETRI, src/read_pos.f90:201-210).  Singular -> zeros. */
+static void inv3(const double* m, double* out) {
+  double a[3][6];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      a[i][j] = m[i + 3 * j];
+      a[i][j + 3] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int c = 0; c < 3; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 3; ++r)
+      if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
+    if (a[piv][c] == 0.0) {
+      for (int k = 0; k < 9; ++k) out[k] = 0.0;
+      return;
+    }
+    if (piv != c)
+      for (int k = 0; k < 6; ++k) {
+        double t = a[c][k];
+        a[c][k] = a[piv][k];
+        a[piv][k] = t;
+      }
+    double d = a[c][c];
+    for (int k = 0; k < 6; ++k) a[c][k] = a[c][k] / d;
+    for (int r = 0; r < 3; ++r)
+      if (r != c) {
+        double f = a[r][c];
+        for (int k = 0; k < 6; ++k) a[r][k] = a[r][k] - f * a[c][k];
+      }
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[i + 3 * j] = a[i][j + 3];
+}
+
+/* induced_moment, src/field.f90:1716-1803 (dipoles; no quadrupole
+   polarizability).  E and mu are (natm,3). */
+static void induced_moment(const mdffgpu_params* p, const int* it,
+                           const double* E, double* mu, double* u_pol) {
+  int n = p->natm;
+  double u = 0.0;
+  for (int ia = 0; ia < n; ++ia) {
+    int t = it[ia];
+    for (int a = 0; a < 3; ++a) mu[ia + n * a] = 0.0;
+    if (!p->ldip_polar[t]) continue;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        mu[ia + n * a] =
+            mu[ia + n * a] + p->poldip[t + NT * (a + 3 * b)] * E[ia + n * b];
+  }
+  for (int ia = 0; ia < n; ++ia) {
+    int t = it[ia];
+    if (!p->ldip_polar[t]) continue;
+    double pol[9], inv[9];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) pol[a + 3 * b] = p->poldip[t + NT * (a + 3 * b)];
+    inv3(pol, inv);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        u = u + mu[ia + n * a] * inv[a + 3 * b] * mu[ia + n * b];
+  }
+  *u_pol = u * 0.5;
+}
+
+static const double B_ASPC[6][6] = {
+    {1.0, 0, 0, 0, 0, 0},
+    {2.0, -1.0, 0, 0, 0, 0},
+    {2.5, -2.0, 0.5, 0, 0, 0},
+    {2.8, -2.8, 1.2, -0.2, 0, 0},
+    {3.0, -24.0 / 7.0, 27.0 / 14.0, -4.0 / 7.0, 1.0 / 14.0, 0},
+    {22.0 / 7.0, -55.0 / 14.0, 55.0 / 21.0, -22.0 / 21.0, 5.0 / 21.0, -1.0 / 42.0}};
+static const double W_ASPC[6] = {0.0, 2.0 / 3.0, 0.6, 4.0 / 7.0, 5.0 / 9.0, 6.0 / 11.0};
+
+static int aspc_order(const mdffgpu_params* p, int itime) {
+  if (itime >= p->extrapolate_order + 1) return p->extrapolate_order;
+  if (itime == 1 || itime == 0) return 0;
+  return itime - 1;
+}
+
+/* moment_from_pola_scf, src/field.f90:3200-3458 */
+static int scf(const es_ctx* e, const double* rx, const double* ry,
+               const double* rz, const double* qia, const double* dipia,
+               int itime, orc_state* st, double* mu_ind, double* fx,
+               double* fy, double* fz, int* n_scf, double* rmsd_out,
+               int* n_es) {
+  const mdffgpu_params* p = e->p;
+  int n = p->natm;
+  const int* it = e->it;
+  size_t n3 = 3 * (size_t)n;
+  int linduced = 0;
+  for (int t = 0; t < p->ntype; ++t)
+    if (p->ldip_polar[t]) linduced = 1;
+  for (size_t k = 0; k < n3; ++k) mu_ind[k] = 0.0;
+  *n_scf = 0;
+  *rmsd_out = 0.0;
+  if (!linduced) return 0;
+  int ldip = 0;
+  for (int k = 0; k < 48; ++k)
+    if (p->dip[k] != 0.0) ldip = 1;
+  int task_static[3] = {1, ldip, ldip};
+  int task_ind[3] = {0, 0, 1};
+  double* buf = (double*)calloc(4 * n3 + 9 * (size_t)n + (size_t)n + n3, sizeof(double));
+  if (!buf) return MDFFGPU_ERR_ALLOC;
+  double *Estat = buf, *Eind = Estat + n3, *E = Eind + n3, *mup = E + n3,
+         *efg = mup + n3, *qzero = efg + 9 * (size_t)n, *muzero = qzero + n;
+  (void)muzero;
+  double u_coul, tau[9], pv;
+  for (int i = 0; i < n; ++i) fx[i] = fy[i] = fz[i] = 0.0;
+  int rc = es_call(e, rx, ry, rz, qia, dipia, task_static, 1, 1, 0, 0, 0,
+                   st->u_pol, Estat, efg, fx, fy, fz, &u_coul, tau, &pv, NULL);
+  if (rc) {
+    free(buf);
+    return rc;
+  }
+  ++*n_es;
+  for (int i = 0; i < n; ++i) fx[i] = fy[i] = fz[i] = 0.0;
+  memcpy(E, Estat, n3 * sizeof(double));
+  int ord = p->extrapolate_order;
+  int nh = ord + 1;
+  double* h = st->hist;
+#define H(t, i, a) h[(t) + nh * ((i) + (size_t)n * (a))]
+  int iscf = 0;
+  double rmsd = 1.79769313486231570e308;
+  while ((iscf < p->max_scf_pol_iter && rmsd > p->conv_tol_ind) ||
+         iscf < p->min_scf_pol_iter) {
+    ++iscf;
+    if (iscf != 1 || p->calc_mode == 0) {
+      induced_moment(p, it, E, mu_ind, &st->u_pol);
+    } else if (p->algo_ext_dipole == MDFFGPU_EXT_POLY) {
+      /* extrapolate_dipole_poly, :3807-3893 (orders 0 and 1) */
+      if (ord == 0) {
+        for (int i = 0; i < n; ++i)
+          for (int a = 0; a < 3; ++a) mu_ind[i + n * a] = H(0, i, a);
+      } else if (itime > ord) {
+        for (int i = 0; i < n; ++i)
+          for (int a = 0; a < 3; ++a) {
+            mu_ind[i + n * a] = 2.0 * H(0, i, a) - H(1, i, a);
+            H(1, i, a) = H(0, i, a);
+          }
+      } else {
+        for (int i = 0; i < n; ++i)
+          for (int a = 0; a < 3; ++a) {
+            mu_ind[i + n * a] = H(0, i, a);
+            H(1, i, a) = H(0, i, a);
+          }
+      }
+    } else {
+      /* extrapolate_dipole_aspc predictor, :3897-3992 */
+      int eo = aspc_order(p, itime);
+      for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+          double v = 0.0;
+          for (int k = 0; k <= eo; ++k) v = v + B_ASPC[eo][k] * H(k, i, a);
+          mu_ind[i + n * a] = v;
+        }
+      for (int k = eo; k >= 1; --k)
+        for (int i = 0; i < n; ++i)
+          for (int a = 0; a < 3; ++a) H(k, i, a) = H(k - 1, i, a);
+    }
+    /* induced field: charges and static dipoles zeroed, :3290-3301 */
+    rc = es_call(e, rx, ry, rz, qzero, mu_ind, task_ind, 0, 1, 0, 0, 0,
+                 st->u_pol, Eind, efg, fx, fy, fz, &u_coul, tau, &pv, NULL);
+    if (rc) break;
+    ++*n_es;
+    for (size_t k = 0; k < n3; ++k) E[k] = Estat[k] + Eind[k];
+    for (int i = 0; i < n; ++i) fx[i] = fy[i] = fz[i] = 0.0;
+    if (iscf == 1 && p->calc_mode == 1 && p->algo_ext_dipole == MDFFGPU_EXT_ASPC) {
+      int eo = aspc_order(p, itime);
+      memcpy(mup, mu_ind, n3 * sizeof(double));
+      induced_moment(p, it, E, mu_ind, &st->u_pol);
+      for (size_t k = 0; k < n3; ++k)
+        mu_ind[k] = W_ASPC[eo] * mu_ind[k] + (1.0 - W_ASPC[eo]) * mup[k];
+    }
+    /* stopping criterion, :3378-3400 */
+    rmsd = 0.0;
+    int npol = 0;
+    for (int ia = 0; ia < n; ++ia) {
+      int t = it[ia];
+      if (!p->ldip_polar[t]) continue;
+      ++npol;
+      for (int a = 0; a < 3; ++a) {
+        double pa = p->poldip[t + NT * (a + 3 * a)];
+        if (pa == 0.0) continue;
+        double r = mu_ind[ia + n * a] / pa - E[ia + n * a];
+        rmsd = rmsd + r * r;
+      }
+    }
+    rmsd = sqrt(rmsd / (double)npol);
+  }
+  if (!rc)
+    for (int i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) H(0, i, a) = mu_ind[i + n * a];
+#undef H
+  *n_scf = iscf;
+  *rmsd_out = rmsd;
+  free(buf);
+  return rc;
+}
+
+int orc_engforce(const mdffgpu_params* p, const int* itype, const double* A,
+                 const double* rx, const double* ry, const double* rz,
+                 int itime, orc_state* st, double* fx, double* fy, double* fz,
+                 double* mu_t, double* theta_t, double* ef_t, double* efg_t,
+                 mdffgpu_energies* en) {
+  int n = p->natm;
+  if (n <= 0 || p->ntype < 1 || p->ntype > NT) return MDFFGPU_ERR_ARG;
+  if (p->algo_ext_dipole == MDFFGPU_EXT_POLY && p->extrapolate_order > 1)
+    return MDFFGPU_ERR_UNSUPPORTED;
+  if (p->extrapolate_order < 0 || p->extrapolate_order > 5) return MDFFGPU_ERR_ARG;
+  memset(en, 0, sizeof(*en));
+  int* it = (int*)malloc((size_t)n * sizeof(int));
+  if (!it) return MDFFGPU_ERR_ALLOC;
+  for (int i = 0; i < n; ++i) it[i] = itype[i] - 1;
+  es_ctx e;
+  int rc = es_init(&e, p, it, A, rx, ry, rz);
+  if (rc) goto done;
+  size_t n3 = 3 * (size_t)n, n9 = 9 * (size_t)n;
+  sr_out o;
+  int any_sr = 0;
+  if (p->lnmlj) {
+    lj(p, it, &e.cell, e.s, fx, fy, fz, &o);
+    en->u_lj = o.u;
+    en->vir_lj = o.vir;
+    memcpy(en->tau_nonb, o.tau, sizeof(o.tau));
+    any_sr = 1;
+  }
+  if (p->lbmhft || p->lbmhftd) {
+    bmhftd(p, it, &e.cell, e.s, fx, fy, fz, &o);
+    en->u_bmhft = o.u;
+    en->vir_bmhft = o.vir;
+    memcpy(en->tau_nonb, o.tau, sizeof(o.tau));
+    any_sr = 1;
+  }
+  if (p->lmorse) {
+    if (!any_sr) {
+      for (int i = 0; i < n; ++i) fx[i] = fy[i] = fz[i] = 0.0;
+      memset(en->tau_nonb, 0, sizeof(en->tau_nonb));
+    }
+    morse(p, it, &e.cell, e.s, fx, fy, fz, &o);
+    en->u_morse = o.u;
+    en->vir_morse = o.vir;
+    for (int k = 0; k < 9; ++k) en->tau_nonb[k] += o.tau[k];
+  }
+  if (p->lcoulomb) {
+    double* buf = (double*)calloc(5 * n3 + 2 * n9 + (size_t)n, sizeof(double));
+    if (!buf) {
+      rc = MDFFGPU_ERR_ALLOC;
+      goto done;
+    }
+    double *qia = buf, *dipia = qia + n, *muind = dipia + n3, *mu = muind + n3,
+           *ef = mu + n3, *fsave = ef + n3, *efg = fsave + n3, *efgs = efg + n9;
+    (void)efgs;
+    for (int i = 0; i < n; ++i) {
+      qia[i] = p->qch[it[i]];
+      for (int a = 0; a < 3; ++a) dipia[i + n * a] = p->dip[it[i] + NT * a];
+    }
+    /* get_moments, :3686-3765: forces saved around the SCF */
+    memcpy(fsave, fx, n * sizeof(double));
+    memcpy(fsave + n, fy, n * sizeof(double));
+    memcpy(fsave + 2 * n, fz, n * sizeof(double));
+    int n_es = 0;
+    rc = scf(&e, rx, ry, rz, qia, dipia, itime, st, muind, fx, fy, fz,
+             &en->n_scf, &en->rmsd, &n_es);
+    memcpy(fx, fsave, n * sizeof(double));
+    memcpy(fy, fsave + n, n * sizeof(double));
+    memcpy(fz, fsave + 2 * n, n * sizeof(double));
+    if (!rc) {
+      for (size_t k = 0; k < n3; ++k) mu[k] = dipia[k] + muind[k];
+      int task[3] = {p->task_coul[0], p->task_coul[1], p->task_coul[2]};
+      double pv;
+      rc = es_call(&e, rx, ry, rz, qia, mu, task, 1, p->doefield, p->doefg, 1,
+                   1, st->u_pol, ef, efg, fx, fy, fz, &en->u_coul,
+                   en->tau_coul, &pv, NULL);
+      ++n_es;
+      en->pvirial_coul = pv;
+      en->u_pol = st->u_pol;
+      en->n_es_calls = n_es;
+      if (mu_t) memcpy(mu_t, mu, n3 * sizeof(double));
+      if (theta_t) memset(theta_t, 0, n9 * sizeof(double));
+      if (ef_t) memcpy(ef_t, ef, n3 * sizeof(double));
+      if (efg_t) memcpy(efg_t, efg, n9 * sizeof(double));
+    }
+    free(buf);
+  }
+done:
+  es_free(&e);
+  free(it);
+  return rc;
+}

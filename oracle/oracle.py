@@ -1,0 +1,147 @@
+"""ctypes front-end of the CPU FP64 oracle (oracle/mdff_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The library is built by
+oracle/Makefile into oracle/_build/liboracle.so.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from mdff_amd.params import MdffEnergies, MdffParams
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class _State(ctypes.Structure):
+    _fields_ = [("hist", _dp), ("u_pol", ctypes.c_double)]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_errfc.restype = ctypes.c_double
+        L.orc_errfc.argtypes = [ctypes.c_double]
+        L.orc_tt_damp.restype = None
+        L.orc_tt_damp.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _dp, _dp]
+        L.orc_kmesh_count.restype = ctypes.c_int
+        L.orc_kmesh_count.argtypes = [_ip]
+        L.orc_engforce.restype = ctypes.c_int
+        L.orc_engforce.argtypes = [ctypes.POINTER(MdffParams), _ip, _dp, _dp, _dp, _dp,
+                                   ctypes.c_int, ctypes.POINTER(_State), _dp, _dp, _dp,
+                                   _dp, _dp, _dp, _dp, ctypes.POINTER(MdffEnergies)]
+        L.orc_multipole_es.restype = ctypes.c_int
+        L.orc_multipole_es.argtypes = [ctypes.POINTER(MdffParams), _ip, _dp, _dp, _dp, _dp,
+                                       _dp, _dp, _ip] + [ctypes.c_int] * 5 + [
+                                           ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp,
+                                           _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def errfc(x: float) -> float:
+    """errfc of src/functions.f90:28-40."""
+    return lib().orc_errfc(float(x))
+
+
+def tt_damp(b: float, c: float, r: float, order: int):
+    """(f, f') of TT_damping_functions, src/field.f90:3775-3805."""
+    f = ctypes.c_double()
+    fd = ctypes.c_double()
+    lib().orc_tt_damp(b, c, r, order, ctypes.byref(f), ctypes.byref(fd))
+    return f.value, fd.value
+
+
+class Oracle:
+    """Stateful oracle mirroring one MDFF process: keeps the induced-dipole
+    history dipia_ind_t and the stale u_pol between calls."""
+
+    def __init__(self, params: MdffParams, itype, A):
+        self.p = params
+        self.itype = np.ascontiguousarray(itype, dtype=np.int32)
+        self.A = np.asfortranarray(np.asarray(A, dtype=np.float64).reshape(3, 3))
+        n = params.natm
+        self.hist = np.zeros((params.extrapolate_order + 1) * n * 3, dtype=np.float64)
+        self.state = _State(_p(self.hist), 0.0)
+
+    def engforce(self, r, itime: int = 0, f_in=None):
+        """One engforce_driver call.  r is (natm, 3) Cartesian; returns a dict
+        with f (natm,3), mu, ef (natm,3), efg (natm,3,3), theta and the
+        energies."""
+        n = self.p.natm
+        r = np.asarray(r, dtype=np.float64)
+        rx, ry, rz = (np.ascontiguousarray(r[:, k]) for k in range(3))
+        f = np.zeros((3, n)) if f_in is None else np.ascontiguousarray(np.asarray(f_in, dtype=np.float64).T)
+        mu = np.zeros(3 * n)
+        theta = np.zeros(9 * n)
+        ef = np.zeros(3 * n)
+        efg = np.zeros(9 * n)
+        en = MdffEnergies()
+        Aflat = np.ascontiguousarray(self.A.reshape(-1, order="F"))
+        rc = lib().orc_engforce(ctypes.byref(self.p), self.itype.ctypes.data_as(_ip), _p(Aflat),
+                                _p(rx), _p(ry), _p(rz), int(itime), ctypes.byref(self.state),
+                                _p(f[0]), _p(f[1]), _p(f[2]), _p(mu), _p(theta), _p(ef),
+                                _p(efg), ctypes.byref(en))
+        if rc != 0:
+            raise RuntimeError(f"oracle engforce failed with status {rc}")
+        out = en.as_dict()
+        out["f"] = f.T.copy()
+        out["mu"] = mu.reshape(3, n).T.copy()
+        out["ef"] = ef.reshape(3, n).T.copy()
+        out["efg"] = efg.reshape(3, 3, n).transpose(2, 1, 0).copy()
+        out["theta"] = theta.reshape(3, 3, n).transpose(2, 1, 0).copy()
+        return out
+
+
+def multipole_es(params: MdffParams, itype, A, r, q, mu, task, damp_ind=True,
+                 do_efield=True, do_efg=True, do_forces=True, do_stress=True, u_pol=0.0):
+    """One multipole_ES call (src/field.f90:1817-2054) with explicit charges
+    q (natm) and dipoles mu (natm,3)."""
+    n = params.natm
+    r = np.asarray(r, dtype=np.float64)
+    rx, ry, rz = (np.ascontiguousarray(r[:, k]) for k in range(3))
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    muf = np.ascontiguousarray(np.asarray(mu, dtype=np.float64).T.reshape(-1))
+    it = np.ascontiguousarray(itype, dtype=np.int32)
+    t = (ctypes.c_int32 * 3)(*[int(x) for x in task])
+    ef = np.zeros(3 * n)
+    efg = np.zeros(9 * n)
+    f = np.zeros((3, n))
+    u = ctypes.c_double()
+    pv = ctypes.c_double()
+    tau = np.zeros(9)
+    parts = np.zeros(4)
+    Aflat = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(3, 3).reshape(-1, order="F"))
+    rc = lib().orc_multipole_es(ctypes.byref(params), it.ctypes.data_as(_ip), _p(Aflat), _p(rx),
+                                _p(ry), _p(rz), _p(q), _p(muf), t, int(damp_ind),
+                                int(do_efield), int(do_efg), int(do_forces), int(do_stress),
+                                float(u_pol), _p(ef), _p(efg), _p(f[0]), _p(f[1]), _p(f[2]),
+                                ctypes.byref(u), _p(tau), ctypes.byref(pv), _p(parts))
+    if rc != 0:
+        raise RuntimeError(f"oracle multipole_ES failed with status {rc}")
+    return {"u_coul": u.value, "pvirial": pv.value, "tau": tau.reshape(3, 3, order="F"),
+            "ef": ef.reshape(3, n).T.copy(), "efg": efg.reshape(3, 3, n).transpose(2, 1, 0).copy(),
+            "f": f.T.copy(), "u_dir": parts[0], "u_rec": parts[1], "u_self": parts[2],
+            "u_surf": parts[3]}
