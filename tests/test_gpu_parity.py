@@ -159,3 +159,25 @@ def test_device_entry_matches_host_entry(oracle_lib):
     assert_close(mu.cpu().numpy().T, g["mu"], 1e-13, "mu")
     assert_close(efg.cpu().numpy().transpose(2, 1, 0), g["efg"], 1e-13, "efg")
     assert abs(en.u_coul - g["u_coul"]) <= 1e-12 * abs(g["u_coul"])
+
+
+def test_ex9_test11_reference_energy_on_gpu(oracle_lib):
+    """examples/ex9 test 11 (BMHFTD + damped PIM + Ewald, 200 atoms):
+    the GPU reproduces the reference run's Utot and SCF iteration count."""
+    import golden_cases as G
+    from mdff_amd.field import ForceField
+    p, itype, A, r, meta = G.ex9_test11()
+    g = ForceField(p, itype, A).engforce(r, itime=0)
+    o = oracle_lib.Oracle(p, itype, A).engforce(r, itime=0)
+    compare_engforce(g, o)
+    assert g["n_scf"] == meta["reference_nscf"]
+    assert abs(g["u_bmhft"] + g["u_coul"] - meta["reference_utot"]) < 1e-7
+
+
+def test_ex4_cluster_efg_on_gpu(oracle_lib):
+    import golden_cases as G
+    from mdff_amd.field import ForceField
+    p, itype, A, r, gulp = G.ex4_cluster()
+    g = ForceField(p, itype, A).engforce(r)
+    o = oracle_lib.Oracle(p, itype, A).engforce(r)
+    compare_engforce(g, o)
