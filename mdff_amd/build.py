@@ -38,6 +38,27 @@ def _run(cmd):
         sys.stderr.write(out + "\n")
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FC = os.environ.get("MDFF_FC", "/opt/rocm/bin/amdflang")
+DRIVER = os.path.join(FORTRAN, "_build", "engforce_drive")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The Fortran host: the ISO_C_BINDING module and the driver program,
+    linked against libmdffgpu.so (rpath relative to the executable)."""
+    bdir = os.path.join(FORTRAN, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    mod = os.path.join(FORTRAN, "mdffgpu_mod.f90")
+    prog = os.path.join(FORTRAN, "engforce_drive.f90")
+    cmd = [FC, "-O2", "-J", bdir, "-o", DRIVER + ".tmp", mod, prog, "-L", HERE, "-lmdffgpu",
+           "-Wl,-rpath,$ORIGIN/../.."]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build(verbose: bool = False) -> str:
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with tempfile.TemporaryDirectory() as td:
@@ -57,3 +78,5 @@ def build(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    if os.path.exists(FC):
+        print(build_fortran(verbose="-v" in sys.argv))
