@@ -87,17 +87,27 @@ def time_path(args, sys_, precision, frames_dev):
     ff.close()
     # dominant kernel and its roofline
     name, (ms, cnt) = max(stats.items(), key=lambda kv: kv[1][0])
-    per_unit = FL.KERNEL_FLOPS_PER_UNIT.get(name)
+    avg_s = ms / cnt * 1e-3
     roof = None
+    per_unit = FL.KERNEL_FLOPS_PER_UNIT.get(name)
     if per_unit:
-        avg_s = ms / cnt * 1e-3
         flop = per_unit * n * nk
+        peak, pipe = FL.PEAK_FP64_TFLOPS, "fp64 vector ALU"
+    elif name in ("mx_rho", "mx_apply"):
+        # executed MFMA work of all launches of the timed region / launches
+        kmax = list(p.kES)
+        nsc = sum(nscf)
+        per_step = FL.mixed_kspace_flops(name, n, kmax, K, nsc, p.doefg)
+        flop = per_step / cnt
+        peak, pipe = FL.PEAK_FP16_MFMA_TFLOPS, "fp16 matrix core (3 hi/lo products)"
+    else:
+        flop = None
+    if flop:
         achieved = flop / avg_s / 1e12
-        peak = FL.PEAK_FP64_TFLOPS
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None, "kernel": name,
                 "avg_launch_us": avg_s * 1e6, "launches": cnt,
-                "flops_per_launch": flop, "pipe": "fp64 vector ALU"}
+                "flops_per_launch": flop, "pipe": pipe}
     return {"ms_per_step": dt * 1e3, "atom_steps_per_s": n / dt, "n_scf": nscf,
             "kernels": {k: {"total_ms": v[0], "launches": v[1]} for k, v in stats.items()},
             "roofline": roof, "force": f_last}

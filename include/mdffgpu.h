@@ -55,6 +55,15 @@ typedef enum mdffgpu_precision {
                              energy, virial and SCF-residual reductions */
 } mdffgpu_precision;
 
+/* Bits of mdffgpu_params.mixed_fp64_parts: parts of the mixed path that are
+   evaluated with the FP64 kernels instead (accuracy studies). */
+typedef enum mdffgpu_mixed_part {
+  MDFFGPU_MIXED_FP64_SHORTRANGE = 1, /* BMHFTD / LJ / Morse pair forces */
+  MDFFGPU_MIXED_FP64_DIRECT = 2,     /* real-space Ewald (static, final) */
+  MDFFGPU_MIXED_FP64_SCF_FIELD = 4,  /* real-space dipole field of the SCF */
+  MDFFGPU_MIXED_FP64_KSPACE = 8      /* reciprocal-space sums */
+} mdffgpu_mixed_part;
+
 typedef enum mdffgpu_ext_algo {
   MDFFGPU_EXT_POLY = 0, /* algo_ext_dipole = 'poly' */
   MDFFGPU_EXT_ASPC = 1  /* algo_ext_dipole = 'aspc' */
@@ -128,8 +137,9 @@ typedef struct mdffgpu_params {
   int32_t doefield, doefg;
 
   /* this library */
-  int32_t precision; /* mdffgpu_precision */
-  int32_t reserved[7];
+  int32_t precision;        /* mdffgpu_precision */
+  int32_t mixed_fp64_parts; /* mdffgpu_mixed_part bits, 0 = all mixed */
+  int32_t reserved[6];
 } mdffgpu_params;
 
 /* Scalar outputs of one evaluation: the thermodynamic module variables

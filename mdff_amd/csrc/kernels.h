@@ -20,7 +20,7 @@ enum { TASK_QQ = 1, TASK_QD = 2, TASK_DD = 4 };
 enum KernelId {
   K_PREPARE, K_NLIST, K_SR, K_DIR_STATIC, K_DIR_IND, K_DIR_FINAL, K_RHO_CHG, K_RHO_DIP,
   K_RHO_BOTH, K_RHO_RED, K_APPLY_F, K_APPLY_EFG, K_MOMENTS, K_COMBINE, K_INDUCED,
-  K_RESID, K_EXTRAP, K_REDUCE, K_ADD, K_MX_SR, K_MX_DIR, K_MX_PHASE, K_MX_RHO,
+  K_RESID, K_EXTRAP, K_REDUCE, K_ADD, K_PAIRS_T2, K_DD_FIELD, K_MX_SR, K_MX_DIR, K_MX_PHASE, K_MX_RHO,
   K_MX_RHO_RED, K_MX_APPLY, K_MX_COMBINE, K_BATCH, K_ADJ, K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
@@ -41,9 +41,26 @@ struct KChunk {
   int n1, n2, n3s, pad;
 };
 
-struct EsDirArgs {
-  int n;
+struct SrArgs {
+  int n, maxnb;
   const double* s;
+  const double4* s4;  // the same, one (s1, s2, s3, 0) per atom
+  const int* typ;
+  const int* cnt;
+  const int* nbr;
+  Cell cell;
+  const PairTables* T;
+  TTTable tt;
+  double rc2;
+  int flags;
+  double* f;
+  double* part;
+};
+
+struct EsDirArgs {
+  int n, maxnb;
+  const double* s;
+  const double4* s4;  // the same, one (s1, s2, s3, 0) per atom
   const int* typ;
   const int* cnt;
   const int* nbr;
@@ -87,6 +104,7 @@ struct ApplyArgs {
   const double2* g;
   const double2* eph;
   const double* s;
+  const double4* s4;  // the same, one (s1, s2, s3, 0) per atom
   double* part;
 };
 
@@ -121,14 +139,20 @@ struct ExtrapArgs {
   double* part;
 };
 
-__global__ void k_prepare(int n, const double* pos, Cell cell, double* s, double2* eph);
+__global__ void k_prepare(int n, const double* pos, Cell cell, double* s, double4* s4,
+                          double2* eph);
 __global__ void k_nlist(int n, const double* s, Cell cell, double cut2, int maxnb, int* cnt,
                         int* nbr, int* overflow);
-__global__ void k_shortrange(int n, const double* s, const int* typ, const int* cnt,
-                             const int* nbr, Cell cell, const PairTables* T, TTTable tt,
-                             double rc2, int flags, double* f, double* part);
-template <int MODE>
+template <typename R>
+__global__ void k_shortrange(SrArgs a);
+template <int MODE, typename R>
 __global__ void k_es_dir(EsDirArgs a);
+template <typename R>
+__global__ void k_pairs_t2(EsDirArgs a, R* tab);
+template <typename R, typename R4>
+__global__ void k_dd_field(EsDirArgs a, const R* tab, const R4* mu4);
+template <typename R4>
+__global__ void k_pack_mu(int n, const double* mu, R4* mu4);
 template <bool CHG, bool DIP>
 __global__ void k_rho64(RhoArgs a);
 __global__ void k_rho_reduce64(RhoRedArgs a);
@@ -144,6 +168,11 @@ __global__ void k_scf_resid(int n, const int* typ, const TypeTables* T, const do
                             const double* E, double* part);
 __global__ void k_extrap(ExtrapArgs a);
 __global__ void k_reduce_rows(const double* part, int nb, double* out);
+__global__ void k_disp(int n, const double* pos, const double* ref, double* part);
+__global__ void k_reduce_max(const double* part, int nb, double* out);
+
+// lanes that share one centre atom in the real-space kernels
+constexpr int LPA = 16;
 __global__ void k_add(int m, const double* a, const double* b, double* out);
 
 }  // namespace mdff

@@ -13,7 +13,7 @@ namespace mdff {
 // and the per-atom phase bases e^{2 pi i s_a} used by the k-space kernels.
 __global__ __launch_bounds__(MDFF_BLOCK) void k_prepare(
     int n, const double* __restrict__ pos, Cell cell, double* __restrict__ s,
-    double2* __restrict__ eph) {
+    double4* __restrict__ s4, double2* __restrict__ eph) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double x = pos[i], y = pos[i + n], z = pos[i + 2 * n];
@@ -24,6 +24,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_prepare(
   s[i] = s1;
   s[i + n] = s2;
   s[i + 2 * n] = s3;
+  s4[i] = make_double4(s1, s2, s3, 0.0);
   double sn, cs;
   sincospi(2.0 * s1, &sn, &cs);
   eph[i] = make_double2(cs, sn);
@@ -36,7 +37,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_prepare(
 // ---------------------------------------------------------------------------
 // Full neighbour list (replaces vnlist_pbc, src/tools.f90:274-370): every j
 // whose minimum image lies within cut (list cutoff incl. skin).  Brute force
-// over LDS tiles of fractional positions; j ascending.
+// over LDS tiles of fractional positions; j ascending; row-major rows
+// nbr[i*maxnb + k] so that the lanes sharing an atom read contiguously.
 __global__ __launch_bounds__(MDFF_BLOCK) void k_nlist(
     int n, const double* __restrict__ s, Cell cell, double cut2, int maxnb,
     int* __restrict__ cnt, int* __restrict__ nbr, int* __restrict__ overflow) {
@@ -67,7 +69,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_nlist(
         double d2 = min_image(cell, si1 - ts[0][jj], si2 - ts[1][jj],
                               si3 - ts[2][jj], rx, ry, rz);
         if (jg != i && d2 <= cut2) {
-          if (c < maxnb) nbr[(size_t)c * n + i] = jg;
+          if (c < maxnb) nbr[(size_t)i * maxnb + c] = jg;
           ++c;
         }
       }
@@ -78,299 +80,6 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_nlist(
     if (c > maxnb) atomicMax(overflow, c);
   }
 }
-
-// ---------------------------------------------------------------------------
-// Short-range pair forces: n-m LJ (engforce_nmlj_pbc, src/field.f90:1181),
-// BMHFTD (engforce_bmhftd_pbc, :2876) and Morse, per centre atom over its
-// full list, energies halved.  Partial rows (per block):
-//   0 u_lj 1 vir_lj 2 u_bm 3 vir_bm 4 u_mo 5 vir_mo 6..11 tau (xx yy zz xy
-//   xz yz) of the terms that contribute forces.
-__global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(
-    int n, const double* __restrict__ s, const int* __restrict__ typ,
-    const int* __restrict__ cnt, const int* __restrict__ nbr, Cell cell,
-    const PairTables* __restrict__ T, TTTable tt, double rc2, int flags,
-    double* __restrict__ f, double* __restrict__ part) {
-  __shared__ double red[4 * 12];
-  const bool LJ = flags & SR_LJ, BM = flags & SR_BM, BMD = flags & SR_BMD,
-             MO = flags & SR_MO;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
-  if (i < n) {
-    double si1 = s[i], si2 = s[i + n], si3 = s[i + 2 * n];
-    int ti = typ[i];
-    double fx = 0, fy = 0, fz = 0;
-    int c = cnt[i];
-    for (int k = 0; k < c; ++k) {
-      int j = nbr[(size_t)k * n + i];
-      double rx, ry, rz;
-      double rsq = min_image(cell, si1 - s[j], si2 - s[j + n], si3 - s[j + 2 * n],
-                             rx, ry, rz);
-      if (!(rsq < rc2)) continue;
-      int t = ti + MDFF_NT * typ[j];
-      double w = 0.0;  // force / r of the terms that contribute forces
-      if (LJ) {
-        double sr2 = T->lj_sigsq[t] / rsq;
-        double srp = pow(sr2, T->lj_ptwo[t]);
-        double srq = pow(sr2, T->lj_qtwo[t]);
-        double u = T->lj_epsp[t] * (T->lj_p[t] * srq - T->lj_q[t] * srp);
-        if (flags & SR_TRUNC1) u = u - T->lj_uc[t];
-        if (flags & SR_TRUNC2) u = u + T->lj_uc1[t] * rsq - T->lj_uc2[t];
-        double wl = T->lj_fc[t] * (srq - srp) * sr2;
-        acc[0] += 0.5 * u;
-        acc[1] += 0.5 * wl * rsq;
-        if (!BM) w += wl;
-      }
-      if (BM && T->bm_A[t] != 0.0) {
-        double ir2 = 1.0 / rsq;
-        double d = sqrt(rsq);
-        double erh = T->bm_A[t] * exp(-T->bm_B[t] * d);
-        double ir6 = ir2 * ir2 * ir2;
-        double ir8 = ir6 * ir2;
-        ir6 = ir6 * T->bm_C[t];
-        ir8 = ir8 * T->bm_D[t];
-        double ir6d = ir6, ir8d = ir8, fd6 = 0.0, fd8 = 0.0;
-        if (BMD) {
-          double f6, f8;
-          tt_damp(tt, T->bm_BD[t], 1.0, d, 6, f6, fd6);
-          tt_damp(tt, T->bm_BD[t], 1.0, d, 8, f8, fd8);
-          ir6d = ir6 * f6;
-          ir8d = ir8 * f8;
-        }
-        double ir7 = 6.0 * ir6d / d;
-        double ir9 = 8.0 * ir8d / d;
-        acc[2] += 0.5 * (erh - ir6d - ir8d);
-        double wb = (T->bm_B[t] * erh - ir7 - ir9 + ir6 * fd6 + ir8 * fd8) / d;
-        acc[3] += 0.5 * wb * rsq;
-        w += wb;
-      }
-      if (MO) {
-        double d = sqrt(rsq);
-        double X = exp(-d * T->mo_rho[t]) * T->mo_rs[t];
-        acc[4] += 0.5 * T->mo_eps[t] * ((X - 1.0) * (X - 1.0) - 1.0);
-        double wm = 2.0 * T->mo_eps[t] * T->mo_rho[t] * X * (X - 1.0) / d;
-        acc[5] += 0.5 * wm * rsq;
-        w += wm;
-      }
-      double gx = w * rx, gy = w * ry, gz = w * rz;
-      fx += gx;
-      fy += gy;
-      fz += gz;
-      acc[6] += 0.5 * rx * gx;
-      acc[7] += 0.5 * ry * gy;
-      acc[8] += 0.5 * rz * gz;
-      acc[9] += 0.5 * rx * gy;
-      acc[10] += 0.5 * rx * gz;
-      acc[11] += 0.5 * ry * gz;
-    }
-    f[i] = fx;
-    f[i + n] = fy;
-    f[i + 2 * n] = fz;
-  }
-  block_sum<12>(acc, red);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 12; ++k) part[(size_t)k * gridDim.x + blockIdx.x] = acc[k];
-}
-
-// ---------------------------------------------------------------------------
-// Ewald real space, multipole_ES_dir (src/field.f90:2057-2643), charges and
-// dipoles, per centre atom i over its full list.  The reference visits each
-// pair once as (ia<ja) and updates both atoms; here each directed pair
-// (i,j) adds i's share, with r = r_i - r_j:
-//   field      qq  q_j r (F1 - F1d(p_i)) / d^3        dd  T2 mu_j
-//   EFG        qq  -q_j T2                            dd  T3 mu_j
-//   force      qq  q_i q_j r F1 / d^3                 dd  mu_i T3 mu_j
-//              qd  q_i (T2 - T2d(p_j)) mu_j - q_j (T2 - T2d(p_i)) mu_i
-//   energy     qq  q_i q_j T0 / 2                     dd  -mu_i T2 mu_j / 2
-//              qd  q_j (T1 - T1d(p_i)) . mu_i
-// where p_i are the damping parameters the reference uses for atom i of the
-// pair (dmp_* slot 0 if i < j, slot 1 otherwise).
-// Partial rows: 0 u, 1..6 tau (xx yy zz xy xz yz), before the 1/(2 Omega).
-template <int MODE>
-__global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
-  __shared__ double red[4 * 7];
-  const int n = a.n;
-  const bool QQ = a.task & 1, QD = a.task & 2, DD = a.task & 4;
-  const bool EFIELD = (MODE != ES_FINAL) || a.do_efield;
-  const bool EFG = (MODE == ES_FINAL) && a.do_efg;
-  const bool FORCE = (MODE == ES_FINAL);
-  const bool DAMP = (MODE != ES_IND);
-  const double alpha = a.alpha, alpha2 = alpha * alpha, alpha3 = alpha2 * alpha,
-               alpha5 = alpha3 * alpha2;
-  const double piroot = sqrt(MDFF_PI);
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[7];
-#pragma unroll
-  for (int k = 0; k < 7; ++k) acc[k] = 0.0;
-  if (i < n) {
-    const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
-    const int ti = a.typ[i];
-    const double qi = a.q[i];
-    const double mi0 = a.mu[i], mi1 = a.mu[i + n], mi2 = a.mu[i + 2 * n];
-    const bool dip_i = (mi0 != 0.0 || mi1 != 0.0 || mi2 != 0.0);
-    double e0 = 0, e1 = 0, e2 = 0;
-    double g00 = 0, g11 = 0, g22 = 0, g01 = 0, g02 = 0, g12 = 0;
-    double f0 = 0, f1 = 0, f2 = 0;
-    const int c = a.cnt[i];
-    for (int k = 0; k < c; ++k) {
-      const int j = a.nbr[(size_t)k * n + i];
-      double rx, ry, rz;
-      const double d2 = min_image(a.cell, si1 - a.s[j], si2 - a.s[j + n],
-                                  si3 - a.s[j + 2 * n], rx, ry, rz);
-      if (d2 > a.cut2) continue;
-      const double qj = a.q[j];
-      const double mj0 = a.mu[j], mj1 = a.mu[j + n], mj2 = a.mu[j + 2 * n];
-      const bool dip_j = (mj0 != 0.0 || mj1 != 0.0 || mj2 != 0.0);
-      const bool dd = DD && (dip_i || dip_j);
-      const bool qd = QD && (dip_i || dip_j);
-      const int tj = a.typ[j];
-      const int o = (i < j) ? 0 : 1;
-      const int pij = ti + MDFF_NT * tj;
-      const int pji = tj + MDFF_NT * ti;
-      bool ldamp = DAMP && a.T->dmp_on[o][pij];
-      const double d = sqrt(d2);
-      const double d3 = d2 * d;
-      const double d5 = d3 * d2;
-      const double dm1 = 1.0 / d;
-      const double dm3 = dm1 / d2;
-      const double dm5 = dm3 / d2;
-      const double dm7 = dm5 / d2;
-      const double expon = exp(-alpha2 * d2) / piroot;
-      const double F0 = errfc_nr(alpha * d);
-      const double F1 = F0 + 2.0 * alpha * d * expon;
-      const double F2 = F1 + 4.0 * alpha3 * d3 * expon / 3.0;
-      const double F1_dm3 = F1 * dm3;
-      const double F2_dm5 = 3.0 * F2 * dm5;
-      // damping of i's own terms (p_i) and of j's terms (p_j)
-      double F1d_i = 0.0, F2d_i = 0.0, F1d_j = 0.0, F2d_j = 0.0;
-      if (ldamp) {
-        double fdmp, fdd;
-        tt_damp(a.tt, a.T->dmp_b[o][pij], a.T->dmp_c[o][pij], d,
-                a.T->dmp_k[o][pij], fdmp, fdd);
-        F1d_i = -fdmp + 1.0;
-        F2d_i = F1d_i + (d / 3.0) * fdd;
-        if (FORCE && qd) {
-          tt_damp(a.tt, a.T->dmp_b[1 - o][pji], a.T->dmp_c[1 - o][pji], d,
-                  a.T->dmp_k[1 - o][pji], fdmp, fdd);
-          F1d_j = -fdmp + 1.0;
-          F2d_j = F1d_j + (d / 3.0) * fdd;
-        }
-      }
-      const double rmi = rx * mi0 + ry * mi1 + rz * mi2;
-      const double rmj = rx * mj0 + ry * mj1 + rz * mj2;
-      if (QQ && EFIELD) {
-        const double w = qj * (F1 - F1d_i) * dm3;
-        e0 += w * rx;
-        e1 += w * ry;
-        e2 += w * rz;
-      }
-      if (dd && EFIELD) {
-        // T2 mu_j = r (r.mu_j) 3F2/d^5 - mu_j F1/d^3
-        e0 += rx * rmj * F2_dm5 - mj0 * F1_dm3;
-        e1 += ry * rmj * F2_dm5 - mj1 * F1_dm3;
-        e2 += rz * rmj * F2_dm5 - mj2 * F1_dm3;
-      }
-      if (EFG || FORCE) {
-        const double F3 = F2 + 8.0 * alpha5 * d5 * expon / 15.0;
-        const double F3_dm7 = F3 * dm7 * 15.0;
-        if (EFG) {
-          if (QQ) {
-            // -q_j T2
-            const double w = -qj * F2_dm5;
-            g00 += w * rx * rx + qj * F1_dm3;
-            g11 += w * ry * ry + qj * F1_dm3;
-            g22 += w * rz * rz + qj * F1_dm3;
-            g01 += w * rx * ry;
-            g02 += w * rx * rz;
-            g12 += w * ry * rz;
-          }
-          if (dd) {
-            // (T3 mu_j)_ab = -r_a r_b (r.mu_j) 15F3/d^7
-            //               + (delta_ab r.mu_j + r_a mu_jb + r_b mu_ja) 3F2/d^5
-            const double w3 = -rmj * F3_dm7;
-            g00 += rx * rx * w3 + (rmj + 2.0 * rx * mj0) * F2_dm5;
-            g11 += ry * ry * w3 + (rmj + 2.0 * ry * mj1) * F2_dm5;
-            g22 += rz * rz * w3 + (rmj + 2.0 * rz * mj2) * F2_dm5;
-            g01 += rx * ry * w3 + (rx * mj1 + ry * mj0) * F2_dm5;
-            g02 += rx * rz * w3 + (rx * mj2 + rz * mj0) * F2_dm5;
-            g12 += ry * rz * w3 + (ry * mj2 + rz * mj1) * F2_dm5;
-          }
-        }
-        if (FORCE) {
-          double px = 0, py = 0, pz = 0, u = 0;
-          if (QQ) {
-            const double w = qi * qj * F1_dm3;
-            px += w * rx;
-            py += w * ry;
-            pz += w * rz;
-            u += 0.5 * qi * qj * dm1 * F0;
-          }
-          if (dd) {
-            const double mm = mi0 * mj0 + mi1 * mj1 + mi2 * mj2;
-            const double w3 = -rmi * rmj * F3_dm7;
-            px += rx * w3 + (mi0 * rmj + rmi * mj0 + rx * mm) * F2_dm5;
-            py += ry * w3 + (mi1 * rmj + rmi * mj1 + ry * mm) * F2_dm5;
-            pz += rz * w3 + (mi2 * rmj + rmi * mj2 + rz * mm) * F2_dm5;
-            u -= 0.5 * (rmi * rmj * F2_dm5 - mm * F1_dm3);
-          }
-          if (qd) {
-            // q_i (T2 - T2d_j) mu_j - q_j (T2 - T2d_i) mu_i
-            const double a2j = qi * (F2_dm5 - 3.0 * F2d_j * dm5);
-            const double a1j = qi * (F1_dm3 - F1d_j * dm3);
-            const double a2i = qj * (F2_dm5 - 3.0 * F2d_i * dm5);
-            const double a1i = qj * (F1_dm3 - F1d_i * dm3);
-            px += rx * rmj * a2j - mj0 * a1j - (rx * rmi * a2i - mi0 * a1i);
-            py += ry * rmj * a2j - mj1 * a1j - (ry * rmi * a2i - mi1 * a1i);
-            pz += rz * rmj * a2j - mj2 * a1j - (rz * rmi * a2i - mi2 * a1i);
-            // q_j (T1 - T1d_i).mu_i, T1 = -r F1/d^3
-            u += -qj * rmi * (F1 - F1d_i) * dm3;
-          }
-          f0 += px;
-          f1 += py;
-          f2 += pz;
-          acc[0] += u;
-          acc[1] += rx * px;
-          acc[2] += ry * py;
-          acc[3] += rz * pz;
-          acc[4] += 0.5 * (rx * py + ry * px);
-          acc[5] += 0.5 * (rx * pz + rz * px);
-          acc[6] += 0.5 * (ry * pz + rz * py);
-        }
-      }
-    }
-    if (EFIELD) {
-      a.ef[i] = e0;
-      a.ef[i + n] = e1;
-      a.ef[i + 2 * n] = e2;
-    }
-    if (EFG) {
-      a.efg[i] = g00;
-      a.efg[i + n] = g01;
-      a.efg[i + 2 * n] = g02;
-      a.efg[i + 3 * n] = g01;
-      a.efg[i + 4 * n] = g11;
-      a.efg[i + 5 * n] = g12;
-      a.efg[i + 6 * n] = g02;
-      a.efg[i + 7 * n] = g12;
-      a.efg[i + 8 * n] = g22;
-    }
-    if (FORCE) {
-      a.f[i] = f0;
-      a.f[i + n] = f1;
-      a.f[i + 2 * n] = f2;
-    }
-  }
-  if (FORCE) {
-    block_sum<7>(acc, red);
-    if (threadIdx.x == 0)
-      for (int k = 0; k < 7; ++k) a.part[(size_t)k * gridDim.x + blockIdx.x] = acc[k];
-  }
-}
-
-template __global__ void k_es_dir<ES_STATIC>(EsDirArgs);
-template __global__ void k_es_dir<ES_IND>(EsDirArgs);
-template __global__ void k_es_dir<ES_FINAL>(EsDirArgs);
 
 // ---------------------------------------------------------------------------
 // Structure factor (multipole_ES_rec, src/field.f90:2697-2742) for all
@@ -861,6 +570,43 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_extrap(ExtrapArgs a) {
   if (a.op == 5) {
     block_sum<1>(acc, red);
     if (threadIdx.x == 0) a.part[blockIdx.x] = acc[0];
+  }
+}
+
+// Largest squared displacement since the list build: block maxima.
+__global__ __launch_bounds__(MDFF_BLOCK) void k_disp(int n, const double* __restrict__ pos,
+                                                     const double* __restrict__ ref,
+                                                     double* __restrict__ part) {
+  __shared__ double red[MDFF_BLOCK / 64];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d = 0.0;
+  if (i < n) {
+    const double dx = pos[i] - ref[i], dy = pos[i + n] - ref[i + n],
+                 dz = pos[i + 2 * n] - ref[i + 2 * n];
+    d = dx * dx + dy * dy + dz * dz;
+  }
+  for (int off = 32; off > 0; off >>= 1) d = fmax(d, __shfl_down(d, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < MDFF_BLOCK / 64; ++w) m = fmax(m, red[w]);
+    part[blockIdx.x] = m;
+  }
+}
+
+__global__ __launch_bounds__(MDFF_BLOCK) void k_reduce_max(const double* __restrict__ part,
+                                                           int nb, double* __restrict__ out) {
+  __shared__ double red[MDFF_BLOCK / 64];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) m = fmax(m, part[b]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < MDFF_BLOCK / 64; ++w) r = fmax(r, red[w]);
+    *out = r;
   }
 }
 

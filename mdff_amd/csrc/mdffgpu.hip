@@ -28,7 +28,7 @@ enum {
   SC_MOM = 26,   // 8: qtot3 mutot3 qsq musq
   SC_UPOL = 34,  // 1
   SC_RMSD = 35,  // 2: sum, npol
-  SC_OVF = 37,   // unused slot kept for layout
+  SC_DISP = 37,  // max squared displacement since the list build
   SC_N = 40
 };
 
@@ -71,6 +71,8 @@ struct mdffgpu_ctx {
   int *d_typ = nullptr, *d_cnt = nullptr, *d_nbr = nullptr, *d_ovf = nullptr;
   double *d_q = nullptr, *d_qzero = nullptr, *d_dip = nullptr, *d_mu = nullptr,
          *d_muind = nullptr, *d_mup = nullptr;
+  double4* d_s4 = nullptr;   // fractional coordinates, one double4 per atom
+  double4* d_mu4 = nullptr;  // packed dipoles of the SCF field (double4 or float4)
   double *d_Estat = nullptr, *d_E = nullptr, *d_efdir = nullptr, *d_efgdir = nullptr,
          *d_fdir = nullptr, *d_force = nullptr, *d_ef = nullptr, *d_efg = nullptr;
   double* d_apart = nullptr;
@@ -80,6 +82,11 @@ struct mdffgpu_ctx {
   KChunk* d_chunks = nullptr;
   double *d_part = nullptr, *d_sc = nullptr, *d_hist = nullptr;
   size_t part_cap = 0;
+  void* d_t2 = nullptr;  // SCF pair tensor table (double or float)
+  size_t t2_cap = 0;
+  double* d_rref = nullptr;  // positions at the last list build
+  bool list_valid = false;
+  int list_builds = 0;
   PairTables* d_pt = nullptr;
   TypeTables* d_tt = nullptr;
   PairTables pt_h;
@@ -105,7 +112,8 @@ const char* const kKernelNames[K_COUNT] = {
     "k_prepare", "k_nlist", "k_shortrange", "k_es_dir<static>", "k_es_dir<ind>",
     "k_es_dir<final>", "k_rho64<chg>", "k_rho64<dip>", "k_rho64<chg+dip>", "k_rho_reduce64",
     "k_apply64<field>", "k_apply64<efg>", "k_moments", "k_combine64", "k_induced",
-    "k_scf_resid", "k_extrap", "k_reduce_rows", "k_add", "mx_shortrange", "mx_es_dir",
+    "k_scf_resid", "k_extrap", "k_reduce_rows", "k_add", "k_pairs_t2", "k_dd_field",
+    "mx_shortrange", "mx_es_dir",
     "mx_phase", "mx_rho", "mx_rho_reduce", "mx_apply", "mx_combine", "batch", "adjoint"};
 
 ProfScope::ProfScope(mdffgpu_ctx* ctx, int kid) : c(ctx), id(kid), a(nullptr), b(nullptr) {
@@ -455,6 +463,8 @@ void alloc_atoms(mdffgpu_ctx* c) {
   dalloc(c, &c->d_dip, 3 * n);
   dalloc(c, &c->d_mu, 3 * n);
   dalloc(c, &c->d_muind, 3 * n);
+  dalloc(c, &c->d_mu4, n);
+  dalloc(c, &c->d_s4, n);
   dalloc(c, &c->d_mup, 3 * n);
   dalloc(c, &c->d_Estat, 3 * n);
   dalloc(c, &c->d_E, 3 * n);
@@ -465,6 +475,7 @@ void alloc_atoms(mdffgpu_ctx* c) {
   dalloc(c, &c->d_ef, 3 * n);
   dalloc(c, &c->d_efg, 9 * n);
   dalloc(c, &c->d_sc, SC_N);
+  dalloc(c, &c->d_rref, 3 * n);
   dalloc(c, &c->d_hist, (size_t)(c->P.extrapolate_order + 1) * 3 * n);
   dalloc(c, &c->d_pt, 1);
   dalloc(c, &c->d_tt, 1);
@@ -481,7 +492,7 @@ void set_list_capacity(mdffgpu_ctx* c, int maxnb) {
 
 int initial_capacity(mdffgpu_ctx* c) {
   // 1.5 x the mean number of neighbours inside the list cutoff, + 64
-  const double rc = c->list_cut;
+  const double rc = c->list_cut + c->P.skindiff;
   const double nmean = 4.0 / 3.0 * MDFF_PI * rc * rc * rc * c->n / c->cell.omega;
   return (int)std::min<double>(c->n, 1.5 * nmean + 64.0);
 }
@@ -502,10 +513,13 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   hipStream_t st = c->st;
   const mdffgpu_params& P = c->P;
   const int gb = nblk(n);
+  const int gbl = nblk(n * LPA);
   // real space
   EsDirArgs da;
   da.n = n;
+  da.maxnb = c->maxnb;
   da.s = c->d_s;
+  da.s4 = c->d_s4;
   da.typ = c->d_typ;
   da.cnt = c->d_cnt;
   da.nbr = c->d_nbr;
@@ -523,28 +537,45 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   da.efg = c->d_efgdir;
   da.f = c->d_fdir;
   da.part = c->d_part;
-  if (mode == ES_STATIC)
-    {
-      ProfScope _ps(c, K_DIR_STATIC);
-      k_es_dir<ES_STATIC><<<gb, MDFF_BLOCK, 0, st>>>(da);
+  const bool mixed = (P.precision == MDFFGPU_PREC_MIXED);
+  const bool mx = mixed && !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_DIRECT);
+  const bool mxf = mixed && !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_SCF_FIELD);
+  const bool mxk = mixed && !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_KSPACE);
+  if (mode == ES_STATIC) {
+    ProfScope _ps(c, mx ? K_MX_DIR : K_DIR_STATIC);
+    if (mx)
+      k_es_dir<ES_STATIC, float><<<gbl, MDFF_BLOCK, 0, st>>>(da);
+    else
+      k_es_dir<ES_STATIC, double><<<gbl, MDFF_BLOCK, 0, st>>>(da);
+  } else if (mode == ES_IND) {
+    ProfScope _ps(c, K_DD_FIELD);
+    if (mxf) {
+      float4* m4 = reinterpret_cast<float4*>(c->d_mu4);
+      k_pack_mu<float4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, m4);
+      k_dd_field<float, float4><<<gbl, MDFF_BLOCK, 0, st>>>(da, (const float*)c->d_t2, m4);
+    } else {
+      k_pack_mu<double4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_mu4);
+      k_dd_field<double, double4><<<gbl, MDFF_BLOCK, 0, st>>>(da, (const double*)c->d_t2,
+                                                              c->d_mu4);
     }
-  else if (mode == ES_IND)
-    {
-      ProfScope _ps(c, K_DIR_IND);
-      k_es_dir<ES_IND><<<gb, MDFF_BLOCK, 0, st>>>(da);
-    }
-  else
-    {
-      ProfScope _ps(c, K_DIR_FINAL);
-      k_es_dir<ES_FINAL><<<gb, MDFF_BLOCK, 0, st>>>(da);
-    }
+  } else {
+    ProfScope _ps(c, mx ? K_MX_DIR : K_DIR_FINAL);
+    if (mx)
+      k_es_dir<ES_FINAL, float><<<gbl, MDFF_BLOCK, 0, st>>>(da);
+    else
+      k_es_dir<ES_FINAL, double><<<gbl, MDFF_BLOCK, 0, st>>>(da);
+  }
   check_launch(c, "k_es_dir");
-  if (mode == ES_FINAL) reduce(c, 7, gb, SC_DIR);
+  if (mode == ES_FINAL) reduce(c, 7, gbl, SC_DIR);
   // reciprocal space
   const bool chg = (mode != ES_IND);
   const bool dip = (task & (TASK_QD | TASK_DD)) != 0;
   const bool efgv = (mode == ES_FINAL) && (do_efg || dip);
-  if (c->nchunk > 0) {
+  int nsplit_apply = c->app_split;
+  if (mxk && c->nchunk > 0) {
+    mixed::kspace(c->mx, mode, q, mu, chg, dip, efgv, c->d_apart);
+    nsplit_apply = 1;
+  } else if (c->nchunk > 0) {
     RhoArgs ra;
     ra.n = n;
     ra.nchunk = c->nchunk;
@@ -603,6 +634,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     aa.g = c->d_g;
     aa.eph = c->d_eph;
     aa.s = c->d_s;
+    aa.s4 = c->d_s4;
     aa.part = c->d_apart;
     dim3 ga(gb, c->app_split);
     if (efgv)
@@ -631,7 +663,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   }
   CombineArgs ca;
   ca.n = n;
-  ca.nsplit = (c->nchunk > 0) ? c->app_split : 1;
+  ca.nsplit = (c->nchunk > 0) ? nsplit_apply : 1;
   ca.lsurf = P.lsurf;
   ca.do_efield = do_efield;
   ca.do_efg = do_efg;
@@ -670,6 +702,39 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
       k_combine64<ES_FINAL><<<gb, MDFF_BLOCK, 0, st>>>(ca);
     }
   check_launch(c, "k_combine64");
+}
+
+// Pair table of the SCF dipole field (k_pairs_t2): once per evaluation,
+// positions are fixed during the SCF.
+void build_t2(mdffgpu_ctx* c) {
+  const int n = c->n;
+  const bool mx = (c->P.precision == MDFFGPU_PREC_MIXED) &&
+                  !(c->P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_SCF_FIELD);
+  const size_t es = mx ? sizeof(float) : sizeof(double);
+  const size_t need = 5 * (size_t)c->maxnb * n * es;
+  if (need > c->t2_cap) {
+    if (c->d_t2) (void)hipFree(c->d_t2);
+    c->d_t2 = nullptr;
+    HCK(c, hipMalloc(&c->d_t2, need));
+    c->t2_cap = need;
+  }
+  EsDirArgs da;
+  std::memset(&da, 0, sizeof(da));
+  da.n = n;
+  da.maxnb = c->maxnb;
+  da.s = c->d_s;
+  da.s4 = c->d_s4;
+  da.cnt = c->d_cnt;
+  da.nbr = c->d_nbr;
+  da.cell = c->cell;
+  da.cut2 = c->P.cutlongrange * c->P.cutlongrange;
+  da.alpha = c->P.alphaES;
+  ProfScope _ps(c, K_PAIRS_T2);
+  if (mx)
+    k_pairs_t2<float><<<nblk(n * LPA), MDFF_BLOCK, 0, c->st>>>(da, (float*)c->d_t2);
+  else
+    k_pairs_t2<double><<<nblk(n * LPA), MDFF_BLOCK, 0, c->st>>>(da, (double*)c->d_t2);
+  check_launch(c, "k_pairs_t2");
 }
 
 void sync_scalars(mdffgpu_ctx* c) {
@@ -718,17 +783,19 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
   *rmsd_out = 0.0;
   if (!c->linduced) return;
   int task_static = TASK_QQ | (c->ldip_static ? (TASK_QD | TASK_DD) : 0);
-  if (P.precision == MDFFGPU_PREC_MIXED)
-    mixed::es_call(c->mx, ES_STATIC, c->d_q, c->d_dip, task_static, 1, 0);
-  else
-    es_call(c, ES_STATIC, c->d_q, c->d_dip, task_static, 1, 0);
+  es_call(c, ES_STATIC, c->d_q, c->d_dip, task_static, 1, 0);
   ++*n_es;
   HCK(c, hipMemcpyAsync(c->d_E, c->d_Estat, 3 * (size_t)n * sizeof(double),
                         hipMemcpyDeviceToDevice, st));
   int iscf = 0;
   double rmsd = 1.79769313486231570e308;
   bool upol_new = false;
-  while ((iscf < P.max_scf_pol_iter && rmsd > P.conv_tol_ind) || iscf < P.min_scf_pol_iter) {
+  const bool mixed_prec = (P.precision == MDFFGPU_PREC_MIXED);
+  bool stalled = false;
+  double rmsd_best = rmsd, rmsd_first = 0.0;
+  int no_progress = 0;
+  while (((iscf < P.max_scf_pol_iter && rmsd > P.conv_tol_ind) || iscf < P.min_scf_pol_iter) &&
+         !stalled) {
     ++iscf;
     if (iscf != 1 || P.calc_mode == 0) {
       {
@@ -744,10 +811,7 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
     } else {
       extrap(c, 3, aspc_order(P, itime), 0.0);
     }
-    if (P.precision == MDFFGPU_PREC_MIXED)
-      mixed::es_call(c->mx, ES_IND, c->d_qzero, c->d_muind, TASK_DD, 1, 0);
-    else
-      es_call(c, ES_IND, c->d_qzero, c->d_muind, TASK_DD, 1, 0);
+    es_call(c, ES_IND, c->d_qzero, c->d_muind, TASK_DD, 1, 0);
     ++*n_es;
     if (iscf == 1 && P.calc_mode == 1 && P.algo_ext_dipole == MDFFGPU_EXT_ASPC) {
       const int eo = aspc_order(P, itime);
@@ -763,6 +827,22 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
     reduce(c, 2, gb, SC_RMSD);
     sync_scalars(c);
     rmsd = std::sqrt(c->h_sc[SC_RMSD] / c->h_sc[SC_RMSD + 1]);
+    // mixed precision: the FP64 residual of FP32 fields has a noise floor
+    // that may lie above the tolerance.  The floor is reached when two
+    // successive iterations fail to improve on the best residual so far
+    // (the residual of the Jacobi iteration need not fall monotonically),
+    // while the residual is well below that of the first iteration (a
+    // diverging iteration stays above it).
+    if (iscf == 1) rmsd_first = rmsd;
+    if (rmsd < rmsd_best) {
+      rmsd_best = rmsd;
+      no_progress = 0;
+    } else {
+      ++no_progress;
+    }
+    if (mixed_prec && iscf >= std::max(P.min_scf_pol_iter, 3) && no_progress >= 2 &&
+        rmsd < 0.1 * rmsd_first)
+      stalled = true;
   }
   extrap(c, 4, 0, 0.0);
   if (upol_new) c->u_pol = 0.5 * c->h_sc[SC_UPOL];
@@ -790,15 +870,37 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
   std::memset(en, 0, sizeof(*en));
   {
     ProfScope _ps(c, K_PREPARE);
-    k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->d_s, c->d_eph);
+    k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->d_s, c->d_s4, c->d_eph);
   }
   check_launch(c, "k_prepare");
-  if (c->list_cut > 0.0) {
+  bool rebuild = !c->list_valid;
+  if (c->list_cut > 0.0 && !rebuild) {
+    // Verlet skin (vnlistcheck, src/tools.f90:446-535): the list holds every
+    // pair within cutoff + skin at the last build, so it stays complete
+    // while no atom has moved more than skin/2
+    {
+      ProfScope _ps(c, K_NLIST);
+      k_disp<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->d_rref, c->d_part);
+      k_reduce_max<<<1, MDFF_BLOCK, 0, st>>>(c->d_part, gb, c->d_sc + SC_DISP);
+    }
+    check_launch(c, "k_disp");
+    HCK(c, hipMemcpyAsync(c->h_sc + SC_DISP, c->d_sc + SC_DISP, sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+    HCK(c, hipStreamSynchronize(st));
+    prof_collect(c);
+    rebuild = 2.0 * std::sqrt(c->h_sc[SC_DISP]) > P.skindiff;
+  }
+  if (c->list_cut > 0.0 && rebuild) {
+    HCK(c, hipMemcpyAsync(c->d_rref, c->d_pos, 3 * (size_t)n * sizeof(double),
+                          hipMemcpyDeviceToDevice, st));
+    c->list_valid = true;
+    ++c->list_builds;
     for (;;) {
       HCK(c, hipMemsetAsync(c->d_ovf, 0, sizeof(int), st));
       {
         ProfScope _ps(c, K_NLIST);
-        k_nlist<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->cell, c->list_cut * c->list_cut, c->maxnb,
+        const double lc = c->list_cut + P.skindiff;
+      k_nlist<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->cell, lc * lc, c->maxnb,
                                             c->d_cnt, c->d_nbr, c->d_ovf);
       }
       check_launch(c, "k_nlist");
@@ -809,23 +911,38 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
       set_list_capacity(c, (int)(*c->h_ovf * 1.2) + 16);
     }
   }
+  if (P.lcoulomb && c->linduced) build_t2(c);
   const bool sr = P.lnmlj || P.lbmhft || P.lbmhftd || P.lmorse;
+  const bool mxs = (P.precision == MDFFGPU_PREC_MIXED) &&
+                   !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_SHORTRANGE);
   if (sr) {
     int flags = (P.lnmlj ? SR_LJ : 0) | ((P.lbmhft || P.lbmhftd) ? SR_BM : 0) |
                 (P.lbmhftd ? SR_BMD : 0) | (P.lmorse ? SR_MO : 0) |
                 (P.trunc == 1 ? SR_TRUNC1 : 0) | (P.trunc == 2 ? SR_TRUNC2 : 0);
-    if (P.precision == MDFFGPU_PREC_MIXED)
-      mixed::shortrange(c->mx, flags);
-    else
-      {
-        ProfScope _ps(c, K_SR);
-        k_shortrange<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->d_typ, c->d_cnt, c->d_nbr, c->cell,
-                                                 c->d_pt, c->tt,
-                                                 P.cutshortrange * P.cutshortrange, flags,
-                                                 c->d_force, c->d_part);
-      }
+    {
+        ProfScope _ps(c, mxs ? K_MX_SR : K_SR);
+        SrArgs sa;
+        sa.n = n;
+        sa.maxnb = c->maxnb;
+        sa.s = c->d_s;
+        sa.s4 = c->d_s4;
+        sa.typ = c->d_typ;
+        sa.cnt = c->d_cnt;
+        sa.nbr = c->d_nbr;
+        sa.cell = c->cell;
+        sa.T = c->d_pt;
+        sa.tt = c->tt;
+        sa.rc2 = P.cutshortrange * P.cutshortrange;
+        sa.flags = flags;
+        sa.f = c->d_force;
+        sa.part = c->d_part;
+        if (mxs)
+          k_shortrange<float><<<nblk(n * LPA), MDFF_BLOCK, 0, st>>>(sa);
+        else
+          k_shortrange<double><<<nblk(n * LPA), MDFF_BLOCK, 0, st>>>(sa);
+    }
     check_launch(c, "k_shortrange");
-    reduce(c, 12, gb, SC_SR);
+    reduce(c, 12, nblk(n * LPA), SC_SR);
   }
   int n_es = 0;
   if (P.lcoulomb) {
@@ -837,10 +954,7 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
     check_launch(c, "k_add");
     int task = (P.task_coul[0] ? TASK_QQ : 0) | (P.task_coul[1] ? TASK_QD : 0) |
                (P.task_coul[2] ? TASK_DD : 0);
-    if (P.precision == MDFFGPU_PREC_MIXED)
-      mixed::es_call(c->mx, ES_FINAL, c->d_q, c->d_mu, task, P.doefield, P.doefg);
-    else
-      es_call(c, ES_FINAL, c->d_q, c->d_mu, task, P.doefield, P.doefg);
+    es_call(c, ES_FINAL, c->d_q, c->d_mu, task, P.doefield, P.doefg);
     ++n_es;
   }
   sync_scalars(c);
@@ -928,7 +1042,7 @@ void create_impl(const mdffgpu_params* params, const int32_t* itype, const doubl
     update_cutoffs(c);
     if (c->list_cut > 0.0) set_list_capacity(c, initial_capacity(c));
     if (c->P.lcoulomb) build_kmesh(c);
-    size_t need = 12 * (size_t)std::max(nblk(c->kpad), nblk(c->n)) + 4096;
+    size_t need = 12 * (size_t)std::max(nblk(c->kpad), nblk(c->n * LPA)) + 4096;
     ensure_part(c, need);
     if (c->P.precision == MDFFGPU_PREC_MIXED) {
       c->mx = mixed::create(c);
@@ -1025,14 +1139,15 @@ int mdffgpu_create(const mdffgpu_params* params, const int32_t* itype, const dou
 void mdffgpu_destroy(mdffgpu_ctx* c) {
   if (!c) return;
   if (c->mx) mixed::destroy(c->mx);
-  double* dp[] = {c->d_pos, c->d_s, c->d_q, c->d_qzero, c->d_dip, c->d_mu, c->d_muind,
+  double* dp[] = {c->d_rref, c->d_pos, c->d_s, c->d_q, c->d_qzero, c->d_dip, c->d_mu, c->d_muind,
                   c->d_mup, c->d_Estat, c->d_E, c->d_efdir, c->d_efgdir, c->d_fdir,
                   c->d_force, c->d_ef, c->d_efg, c->d_apart, c->d_Ak, c->d_kcoe, c->d_kvec,
                   c->d_part, c->d_sc, c->d_hist};
   for (double* p : dp)
     if (p) (void)hipFree(p);
+  if (c->d_t2) (void)hipFree(c->d_t2);
   void* vp[] = {c->d_eph, c->d_typ, c->d_cnt, c->d_nbr, c->d_ovf, c->d_rpart, c->d_g,
-                c->d_chunks, c->d_pt, c->d_tt};
+                c->d_chunks, c->d_pt, c->d_tt, c->d_mu4, c->d_s4};
   for (void* p : vp)
     if (p) (void)hipFree(p);
   for (auto& p : c->prof_pending) {
@@ -1051,6 +1166,7 @@ int mdffgpu_set_cell(mdffgpu_ctx* c, const double* A) {
   try {
     set_lattice(c, A);
     update_cutoffs(c);
+    c->list_valid = false;
     if (c->list_cut > 0.0) set_list_capacity(c, std::max(c->maxnb, initial_capacity(c)));
     if (c->P.lcoulomb) build_kmesh(c);
     if (c->mx) mixed::on_cell(c->mx);

@@ -52,9 +52,11 @@ CtxView view(mdffgpu_ctx* c);
 MixedState* create(mdffgpu_ctx* c);
 void destroy(MixedState* m);
 void on_cell(MixedState* m);
-void shortrange(MixedState* m, int flags);
-void es_call(MixedState* m, int mode, const double* q, const double* mu, int task,
-             int do_efield, int do_efg);
+// Reciprocal-space part of one multipole_ES call: writes the per-atom U, V
+// sums (apply-partials layout of k_apply64 with one split) to apart and the
+// energy / stress rows to the scalar slots.
+void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg, bool dip,
+            bool efg, double* apart);
 
 }  // namespace mixed
 }  // namespace mdff

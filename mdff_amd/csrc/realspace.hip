@@ -1,0 +1,552 @@
+// realspace.hip -- real-space kernels (short-range pairs and the Ewald
+// real-space multipole sum), templated on the arithmetic type R: double for
+// the plain FP64 path, float for the mixed path.  Block partial sums of
+// energies, virials and stress are always reduced in double.
+#include "kernels.h"
+
+namespace mdff {
+
+template <typename R>
+__device__ __forceinline__ R rexp(R x);
+template <>
+__device__ __forceinline__ double rexp<double>(double x) {
+  return exp(x);
+}
+template <>
+__device__ __forceinline__ float rexp<float>(float x) {
+  return __expf(x);
+}
+
+template <typename R>
+__device__ __forceinline__ R rpow(R x, R y);
+template <>
+__device__ __forceinline__ double rpow<double>(double x, double y) {
+  return pow(x, y);
+}
+template <>
+__device__ __forceinline__ float rpow<float>(float x, float y) {
+  return powf(x, y);
+}
+
+// errfc of src/functions.f90:28-40 in type R
+template <typename R>
+__device__ __forceinline__ R errfc_t(R x) {
+  const R z = fabs(x);
+  const R t = R(1) / (R(1) + R(0.5) * z);
+  const R p =
+      t * (R(1.00002368) +
+           t * (R(.37409196) +
+                t * (R(.09678418) +
+                     t * (R(-.18628806) +
+                          t * (R(.27886807) +
+                               t * (R(-1.13520398) +
+                                    t * (R(1.48851587) + t * (R(-.82215223) + t * R(.17087277)))))))));
+  const R r = t * rexp<R>(-z * z - R(1.26551223) + p);
+  return (x < R(0)) ? R(2) - r : r;
+}
+
+// Tang-Toennies f and f' (src/field.f90:3775-3805) in type R
+template <typename R>
+__device__ __forceinline__ void tt_damp_t(const TTTable& E, R b, R c, R r, int order, R& f,
+                                          R& fd) {
+  const R br = b * r;
+  const R ex = rexp<R>(-br) * c;
+  R ff = R(E.e[order]);
+  for (int k = order - 1; k >= 1; --k) ff = ff * br + R(E.e[k]);
+  ff = ff * br + R(E.e[0]);
+  f = R(1) - ff * ex;
+  R brn = R(1);
+  for (int k = 0; k < order; ++k) brn *= br;
+  fd = R(E.e[order]) * brn * ex * b;
+}
+
+// Minimum image in type R: the fractional difference is formed in double
+// (positions are stored in double), the rest in R.
+template <typename R>
+__device__ __forceinline__ R min_image_t(const Cell& c, double dsx, double dsy, double dsz,
+                                         R& rx, R& ry, R& rz) {
+  const R sx = R(dsx - nint_d(dsx));
+  const R sy = R(dsy - nint_d(dsy));
+  const R sz = R(dsz - nint_d(dsz));
+  rx = sx * R(c.A[0]) + sy * R(c.A[3]) + sz * R(c.A[6]);
+  ry = sx * R(c.A[1]) + sy * R(c.A[4]) + sz * R(c.A[7]);
+  rz = sx * R(c.A[2]) + sy * R(c.A[5]) + sz * R(c.A[8]);
+  return rx * rx + ry * ry + rz * rz;
+}
+
+// ---------------------------------------------------------------------------
+// Short-range pair forces: n-m LJ (engforce_nmlj_pbc, src/field.f90:1181),
+// BMHFTD (engforce_bmhftd_pbc, :2876) and Morse, per centre atom over its
+// full list, energies halved.  The first short-range term zeroes the
+// forces; with BMHFTD on, the LJ forces and stress are dropped as the
+// reference's routine overwrites them (:2902-2907).  Partial rows:
+//   0 u_lj 1 vir_lj 2 u_bm 3 vir_bm 4 u_mo 5 vir_mo 6..11 tau (xx yy zz xy
+//   xz yz) of the terms that contribute forces.
+template <typename R>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(SrArgs a) {
+  __shared__ double red[4 * 12];
+  const int n = a.n;
+  const bool LJ = a.flags & SR_LJ, BM = a.flags & SR_BM, BMD = a.flags & SR_BMD,
+             MO = a.flags & SR_MO;
+  const PairTables* T = a.T;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = tid / LPA, sub = tid % LPA;
+  R acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = R(0);
+  R fx = 0, fy = 0, fz = 0;
+  if (i < n) {
+    const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
+    const int ti = a.typ[i];
+    const int c = a.cnt[i];
+    const R rc2 = R(a.rc2);
+    const int* nb = a.nbr + (size_t)i * a.maxnb;
+    for (int k = sub; k < c; k += LPA) {
+      const int j = nb[k];
+      R rx, ry, rz;
+      const R rsq = min_image_t<R>(a.cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
+                                   rx, ry, rz);
+      if (!(rsq < rc2)) continue;
+      const int t = ti + MDFF_NT * a.typ[j];
+      R w = R(0);
+      if (LJ) {
+        const R sr2 = R(T->lj_sigsq[t]) / rsq;
+        const R srp = rpow<R>(sr2, R(T->lj_ptwo[t]));
+        const R srq = rpow<R>(sr2, R(T->lj_qtwo[t]));
+        R u = R(T->lj_epsp[t]) * (R(T->lj_p[t]) * srq - R(T->lj_q[t]) * srp);
+        if (a.flags & SR_TRUNC1) u = u - R(T->lj_uc[t]);
+        if (a.flags & SR_TRUNC2) u = u + R(T->lj_uc1[t]) * rsq - R(T->lj_uc2[t]);
+        const R wl = R(T->lj_fc[t]) * (srq - srp) * sr2;
+        acc[0] += R(0.5) * u;
+        acc[1] += R(0.5) * wl * rsq;
+        if (!BM) w += wl;
+      }
+      if (BM && T->bm_A[t] != 0.0) {
+        const R ir2 = R(1) / rsq;
+        const R d = sqrt(rsq);
+        const R erh = R(T->bm_A[t]) * rexp<R>(-R(T->bm_B[t]) * d);
+        R ir6 = ir2 * ir2 * ir2;
+        R ir8 = ir6 * ir2;
+        ir6 = ir6 * R(T->bm_C[t]);
+        ir8 = ir8 * R(T->bm_D[t]);
+        R ir6d = ir6, ir8d = ir8, fd6 = 0, fd8 = 0;
+        if (BMD) {
+          R f6, f8;
+          tt_damp_t<R>(a.tt, R(T->bm_BD[t]), R(1), d, 6, f6, fd6);
+          tt_damp_t<R>(a.tt, R(T->bm_BD[t]), R(1), d, 8, f8, fd8);
+          ir6d = ir6 * f6;
+          ir8d = ir8 * f8;
+        }
+        const R ir7 = R(6) * ir6d / d;
+        const R ir9 = R(8) * ir8d / d;
+        acc[2] += R(0.5) * (erh - ir6d - ir8d);
+        const R wb = (R(T->bm_B[t]) * erh - ir7 - ir9 + ir6 * fd6 + ir8 * fd8) / d;
+        acc[3] += R(0.5) * wb * rsq;
+        w += wb;
+      }
+      if (MO) {
+        const R d = sqrt(rsq);
+        const R X = rexp<R>(-d * R(T->mo_rho[t])) * R(T->mo_rs[t]);
+        acc[4] += R(0.5) * R(T->mo_eps[t]) * ((X - R(1)) * (X - R(1)) - R(1));
+        const R wm = R(2) * R(T->mo_eps[t]) * R(T->mo_rho[t]) * X * (X - R(1)) / d;
+        acc[5] += R(0.5) * wm * rsq;
+        w += wm;
+      }
+      const R gx = w * rx, gy = w * ry, gz = w * rz;
+      fx += gx;
+      fy += gy;
+      fz += gz;
+      acc[6] += R(0.5) * rx * gx;
+      acc[7] += R(0.5) * ry * gy;
+      acc[8] += R(0.5) * rz * gz;
+      acc[9] += R(0.5) * rx * gy;
+      acc[10] += R(0.5) * rx * gz;
+      acc[11] += R(0.5) * ry * gz;
+    }
+  }
+  for (int off = LPA / 2; off > 0; off >>= 1) {
+    fx += __shfl_xor(fx, off, 64);
+    fy += __shfl_xor(fy, off, 64);
+    fz += __shfl_xor(fz, off, 64);
+  }
+  if (i < n && sub == 0) {
+    a.f[i] = fx;
+    a.f[i + n] = fy;
+    a.f[i + 2 * n] = fz;
+  }
+  double dacc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) dacc[k] = (double)acc[k];
+  block_sum<12>(dacc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 12; ++k) a.part[(size_t)k * gridDim.x + blockIdx.x] = dacc[k];
+}
+
+template __global__ void k_shortrange<double>(SrArgs);
+template __global__ void k_shortrange<float>(SrArgs);
+
+// ---------------------------------------------------------------------------
+// Ewald real space, multipole_ES_dir (src/field.f90:2057-2643), charges and
+// dipoles, per centre atom i over its full list.  The reference visits each
+// pair once as (ia<ja) and updates both atoms; here each directed pair
+// (i,j) adds i's share, with r = r_i - r_j:
+//   field      qq  q_j r (F1 - F1d(p_i)) / d^3        dd  T2 mu_j
+//   EFG        qq  -q_j T2                            dd  T3 mu_j
+//   force      qq  q_i q_j r F1 / d^3                 dd  mu_i T3 mu_j
+//              qd  q_i (T2 - T2d(p_j)) mu_j - q_j (T2 - T2d(p_i)) mu_i
+//   energy     qq  q_i q_j T0 / 2                     dd  -mu_i T2 mu_j / 2
+//              qd  q_j (T1 - T1d(p_i)) . mu_i
+// where p_i are the damping parameters the reference uses for atom i of the
+// pair (dmp_* slot 0 if i < j, slot 1 otherwise).
+// Partial rows: 0 u, 1..6 tau (xx yy zz xy xz yz), before the 1/(2 Omega).
+template <int MODE, typename R>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
+  __shared__ double red[4 * 7];
+  const int n = a.n;
+  const bool QQ = a.task & 1, QD = a.task & 2, DD = a.task & 4;
+  const bool EFIELD = (MODE != ES_FINAL) || a.do_efield;
+  const bool EFG = (MODE == ES_FINAL) && a.do_efg;
+  const bool FORCE = (MODE == ES_FINAL);
+  const bool DAMP = (MODE != ES_IND);
+  const R alpha = R(a.alpha), alpha2 = alpha * alpha, alpha3 = alpha2 * alpha,
+          alpha5 = alpha3 * alpha2;
+  const R rpiroot = R(1.0 / sqrt(MDFF_PI));
+  const R cut2 = R(a.cut2);
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = tid / LPA, sub = tid % LPA;
+  R acc[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) acc[k] = R(0);
+  R e0 = 0, e1 = 0, e2 = 0;
+  R g00 = 0, g11 = 0, g22 = 0, g01 = 0, g02 = 0, g12 = 0;
+  R f0 = 0, f1 = 0, f2 = 0;
+  if (i < n) {
+    const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
+    const int ti = a.typ[i];
+    const R qi = R(a.q[i]);
+    const R mi0 = R(a.mu[i]), mi1 = R(a.mu[i + n]), mi2 = R(a.mu[i + 2 * n]);
+    const bool dip_i = (mi0 != R(0) || mi1 != R(0) || mi2 != R(0));
+    const int c = a.cnt[i];
+    const int* nb = a.nbr + (size_t)i * a.maxnb;
+    for (int k = sub; k < c; k += LPA) {
+      const int j = nb[k];
+      R rx, ry, rz;
+      const R d2 = min_image_t<R>(a.cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
+                                  rx, ry, rz);
+      if (d2 > cut2) continue;
+      const R qj = R(a.q[j]);
+      const R mj0 = R(a.mu[j]), mj1 = R(a.mu[j + n]), mj2 = R(a.mu[j + 2 * n]);
+      const bool dip_j = (mj0 != R(0) || mj1 != R(0) || mj2 != R(0));
+      const bool dd = DD && (dip_i || dip_j);
+      const bool qd = QD && (dip_i || dip_j);
+      const int tj = a.typ[j];
+      const int o = (i < j) ? 0 : 1;
+      const int pij = ti + MDFF_NT * tj;
+      const int pji = tj + MDFF_NT * ti;
+      const bool ldamp = DAMP && a.T->dmp_on[o][pij];
+      const R d = sqrt(d2);
+      const R d3 = d2 * d;
+      const R d5 = d3 * d2;
+      const R dm1 = R(1) / d;
+      const R dm3 = dm1 / d2;
+      const R dm5 = dm3 / d2;
+      const R dm7 = dm5 / d2;
+      const R expon = rexp<R>(-alpha2 * d2) * rpiroot;
+      const R F0 = errfc_t<R>(alpha * d);
+      const R F1 = F0 + R(2) * alpha * d * expon;
+      const R F2 = F1 + R(4) * alpha3 * d3 * expon / R(3);
+      const R F1_dm3 = F1 * dm3;
+      const R F2_dm5 = R(3) * F2 * dm5;
+      R F1d_i = 0, F2d_i = 0, F1d_j = 0, F2d_j = 0;
+      if (ldamp) {
+        R fdmp, fdd;
+        tt_damp_t<R>(a.tt, R(a.T->dmp_b[o][pij]), R(a.T->dmp_c[o][pij]), d,
+                     a.T->dmp_k[o][pij], fdmp, fdd);
+        F1d_i = -fdmp + R(1);
+        F2d_i = F1d_i + (d / R(3)) * fdd;
+        if (FORCE && qd) {
+          tt_damp_t<R>(a.tt, R(a.T->dmp_b[1 - o][pji]), R(a.T->dmp_c[1 - o][pji]), d,
+                       a.T->dmp_k[1 - o][pji], fdmp, fdd);
+          F1d_j = -fdmp + R(1);
+          F2d_j = F1d_j + (d / R(3)) * fdd;
+        }
+      }
+      const R rmi = rx * mi0 + ry * mi1 + rz * mi2;
+      const R rmj = rx * mj0 + ry * mj1 + rz * mj2;
+      if (QQ && EFIELD) {
+        const R w = qj * (F1 - F1d_i) * dm3;
+        e0 += w * rx;
+        e1 += w * ry;
+        e2 += w * rz;
+      }
+      if (dd && EFIELD) {
+        // T2 mu_j = r (r.mu_j) 3F2/d^5 - mu_j F1/d^3
+        e0 += rx * rmj * F2_dm5 - mj0 * F1_dm3;
+        e1 += ry * rmj * F2_dm5 - mj1 * F1_dm3;
+        e2 += rz * rmj * F2_dm5 - mj2 * F1_dm3;
+      }
+      if (EFG || FORCE) {
+        const R F3 = F2 + R(8) * alpha5 * d5 * expon / R(15);
+        const R F3_dm7 = F3 * dm7 * R(15);
+        if (EFG) {
+          if (QQ) {
+            const R w = -qj * F2_dm5;
+            g00 += w * rx * rx + qj * F1_dm3;
+            g11 += w * ry * ry + qj * F1_dm3;
+            g22 += w * rz * rz + qj * F1_dm3;
+            g01 += w * rx * ry;
+            g02 += w * rx * rz;
+            g12 += w * ry * rz;
+          }
+          if (dd) {
+            // (T3 mu_j)_ab = -r_a r_b (r.mu_j) 15F3/d^7
+            //               + (delta_ab r.mu_j + r_a mu_jb + r_b mu_ja) 3F2/d^5
+            const R w3 = -rmj * F3_dm7;
+            g00 += rx * rx * w3 + (rmj + R(2) * rx * mj0) * F2_dm5;
+            g11 += ry * ry * w3 + (rmj + R(2) * ry * mj1) * F2_dm5;
+            g22 += rz * rz * w3 + (rmj + R(2) * rz * mj2) * F2_dm5;
+            g01 += rx * ry * w3 + (rx * mj1 + ry * mj0) * F2_dm5;
+            g02 += rx * rz * w3 + (rx * mj2 + rz * mj0) * F2_dm5;
+            g12 += ry * rz * w3 + (ry * mj2 + rz * mj1) * F2_dm5;
+          }
+        }
+        if (FORCE) {
+          R px = 0, py = 0, pz = 0, u = 0;
+          if (QQ) {
+            const R w = qi * qj * F1_dm3;
+            px += w * rx;
+            py += w * ry;
+            pz += w * rz;
+            u += R(0.5) * qi * qj * dm1 * F0;
+          }
+          if (dd) {
+            const R mm = mi0 * mj0 + mi1 * mj1 + mi2 * mj2;
+            const R w3 = -rmi * rmj * F3_dm7;
+            px += rx * w3 + (mi0 * rmj + rmi * mj0 + rx * mm) * F2_dm5;
+            py += ry * w3 + (mi1 * rmj + rmi * mj1 + ry * mm) * F2_dm5;
+            pz += rz * w3 + (mi2 * rmj + rmi * mj2 + rz * mm) * F2_dm5;
+            u -= R(0.5) * (rmi * rmj * F2_dm5 - mm * F1_dm3);
+          }
+          if (qd) {
+            // q_i (T2 - T2d_j) mu_j - q_j (T2 - T2d_i) mu_i
+            const R a2j = qi * (F2_dm5 - R(3) * F2d_j * dm5);
+            const R a1j = qi * (F1_dm3 - F1d_j * dm3);
+            const R a2i = qj * (F2_dm5 - R(3) * F2d_i * dm5);
+            const R a1i = qj * (F1_dm3 - F1d_i * dm3);
+            px += rx * rmj * a2j - mj0 * a1j - (rx * rmi * a2i - mi0 * a1i);
+            py += ry * rmj * a2j - mj1 * a1j - (ry * rmi * a2i - mi1 * a1i);
+            pz += rz * rmj * a2j - mj2 * a1j - (rz * rmi * a2i - mi2 * a1i);
+            // q_j (T1 - T1d_i).mu_i, T1 = -r F1/d^3
+            u += -qj * rmi * (F1 - F1d_i) * dm3;
+          }
+          f0 += px;
+          f1 += py;
+          f2 += pz;
+          acc[0] += u;
+          acc[1] += rx * px;
+          acc[2] += ry * py;
+          acc[3] += rz * pz;
+          acc[4] += R(0.5) * (rx * py + ry * px);
+          acc[5] += R(0.5) * (rx * pz + rz * px);
+          acc[6] += R(0.5) * (ry * pz + rz * py);
+        }
+      }
+    }
+  }
+  for (int off = LPA / 2; off > 0; off >>= 1) {
+    if (EFIELD) {
+      e0 += __shfl_xor(e0, off, 64);
+      e1 += __shfl_xor(e1, off, 64);
+      e2 += __shfl_xor(e2, off, 64);
+    }
+    if (EFG) {
+      g00 += __shfl_xor(g00, off, 64);
+      g11 += __shfl_xor(g11, off, 64);
+      g22 += __shfl_xor(g22, off, 64);
+      g01 += __shfl_xor(g01, off, 64);
+      g02 += __shfl_xor(g02, off, 64);
+      g12 += __shfl_xor(g12, off, 64);
+    }
+    if (FORCE) {
+      f0 += __shfl_xor(f0, off, 64);
+      f1 += __shfl_xor(f1, off, 64);
+      f2 += __shfl_xor(f2, off, 64);
+    }
+  }
+  if (i < n && sub == 0) {
+    if (EFIELD) {
+      a.ef[i] = e0;
+      a.ef[i + n] = e1;
+      a.ef[i + 2 * n] = e2;
+    }
+    if (EFG) {
+      a.efg[i] = g00;
+      a.efg[i + n] = g01;
+      a.efg[i + 2 * n] = g02;
+      a.efg[i + 3 * n] = g01;
+      a.efg[i + 4 * n] = g11;
+      a.efg[i + 5 * n] = g12;
+      a.efg[i + 6 * n] = g02;
+      a.efg[i + 7 * n] = g12;
+      a.efg[i + 8 * n] = g22;
+    }
+    if (FORCE) {
+      a.f[i] = f0;
+      a.f[i + n] = f1;
+      a.f[i + 2 * n] = f2;
+    }
+  }
+  if (FORCE) {
+    double dacc[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) dacc[k] = (double)acc[k];
+    block_sum<7>(dacc, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 7; ++k) a.part[(size_t)k * gridDim.x + blockIdx.x] = dacc[k];
+  }
+}
+
+template __global__ void k_es_dir<ES_STATIC, double>(EsDirArgs);
+template __global__ void k_es_dir<ES_IND, double>(EsDirArgs);
+template __global__ void k_es_dir<ES_FINAL, double>(EsDirArgs);
+template __global__ void k_es_dir<ES_STATIC, float>(EsDirArgs);
+template __global__ void k_es_dir<ES_FINAL, float>(EsDirArgs);
+
+// ---------------------------------------------------------------------------
+// Dipole field tensor of the SCF (multipole_ES_dir with task d-d and no
+// damping, :2407-2421).  The pair geometry is fixed during the SCF, so
+// k_pairs_t2 stores for every directed pair inside the cutoff r (3) and the
+// two radial factors 3F2/d^5, F1/d^3 in type R, in the row layout of nbr
+// (5 planes); pairs outside the cutoff get zero factors.  k_dd_field applies
+// E_i = sum_j [r (r.mu_j) 3F2/d^5 - mu_j F1/d^3] from that table.
+template <typename R>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, R* __restrict__ tab) {
+  const int n = a.n;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = tid / LPA, sub = tid % LPA;
+  if (i >= n) return;
+  const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
+  const int c = a.cnt[i];
+  const size_t plane = (size_t)a.maxnb * n;
+  for (int k = sub; k < c; k += LPA) {
+    const size_t e = (size_t)i * a.maxnb + k;
+    const int j = a.nbr[e];
+    R rx, ry, rz, A2 = R(0), A1 = R(0);
+    if constexpr (sizeof(R) == sizeof(double)) {
+      const double alpha = a.alpha, alpha2 = alpha * alpha, alpha3 = alpha2 * alpha;
+      const double piroot = sqrt(MDFF_PI);
+      const double d2 = min_image(a.cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
+                                  rx, ry, rz);
+      if (!(d2 > a.cut2)) {
+        const double d = sqrt(d2);
+        const double d3 = d2 * d;
+        const double dm1 = 1.0 / d;
+        const double dm3 = dm1 / d2;
+        const double dm5 = dm3 / d2;
+        const double expon = exp(-alpha2 * d2) / piroot;
+        const double F0 = errfc_nr(alpha * d);
+        const double F1 = F0 + 2.0 * alpha * d * expon;
+        const double F2 = F1 + 4.0 * alpha3 * d3 * expon / 3.0;
+        A2 = 3.0 * F2 * dm5;
+        A1 = F1 * dm3;
+      }
+    } else {
+      // FP32 pair arithmetic; the fractional difference is formed in FP64
+      const R alpha = R(a.alpha);
+      const R d2 = min_image_t<R>(a.cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
+                                  rx, ry, rz);
+      if (!(d2 > R(a.cut2))) {
+        const R d = sqrt(d2);
+        const R dm1 = R(1) / d;
+        const R dm3 = dm1 * dm1 * dm1;
+        const R expon = rexp<R>(-alpha * alpha * d2) * R(0.56418958354775628);  // 1/sqrt(pi)
+        const R F1 = errfc_t<R>(alpha * d) + R(2) * alpha * d * expon;
+        const R F2 = F1 + R(4.0 / 3.0) * alpha * alpha * alpha * d2 * d * expon;
+        A2 = R(3) * F2 * dm3 * dm1 * dm1;
+        A1 = F1 * dm3;
+      }
+    }
+    tab[e] = rx;
+    tab[e + plane] = ry;
+    tab[e + 2 * plane] = rz;
+    tab[e + 3 * plane] = A2;
+    tab[e + 4 * plane] = A1;
+  }
+}
+
+// Dipoles of the SCF iteration packed as (x, y, z, 0) per atom, so that a
+// neighbour's dipole is one gather.
+template <typename R4>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_pack_mu(int n, const double* __restrict__ mu,
+                                                        R4* __restrict__ mu4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R4 v;
+  v.x = mu[i];
+  v.y = mu[i + n];
+  v.z = mu[i + 2 * n];
+  v.w = 0;
+  mu4[i] = v;
+}
+
+// Pairs of one lane are taken 4 at a time (loads in flight together) and
+// accumulated in list order.  A neighbour without a dipole contributes
+// exact zeros.
+template <typename R, typename R4>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a, const R* __restrict__ tab,
+                                                         const R4* __restrict__ mu4) {
+  const int n = a.n;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = tid / LPA, sub = tid % LPA;
+  R e0 = 0, e1 = 0, e2 = 0;
+  const size_t plane = (size_t)a.maxnb * n;
+  const bool act = i < n;
+  const int c = act ? a.cnt[i] : 0;
+  const size_t row = (size_t)(act ? i : 0) * a.maxnb;
+  constexpr int U = 4;
+  for (int k0 = sub; k0 < c; k0 += U * LPA) {
+    int jj[U];
+    R rx[U], ry[U], rz[U], A2[U], A1[U];
+    R4 m[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * LPA;
+      const size_t e = row + (k < c ? k : 0);
+      jj[u] = a.nbr[e];
+      rx[u] = tab[e];
+      ry[u] = tab[e + plane];
+      rz[u] = tab[e + 2 * plane];
+      A2[u] = tab[e + 3 * plane];
+      A1[u] = tab[e + 4 * plane];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) m[u] = mu4[jj[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (k0 + u * LPA >= c) break;
+      const R rmj = rx[u] * m[u].x + ry[u] * m[u].y + rz[u] * m[u].z;
+      e0 += rx[u] * rmj * A2[u] - m[u].x * A1[u];
+      e1 += ry[u] * rmj * A2[u] - m[u].y * A1[u];
+      e2 += rz[u] * rmj * A2[u] - m[u].z * A1[u];
+    }
+  }
+  for (int off = LPA / 2; off > 0; off >>= 1) {
+    e0 += __shfl_xor(e0, off, 64);
+    e1 += __shfl_xor(e1, off, 64);
+    e2 += __shfl_xor(e2, off, 64);
+  }
+  if (act && sub == 0) {
+    a.ef[i] = e0;
+    a.ef[i + n] = e1;
+    a.ef[i + 2 * n] = e2;
+  }
+}
+
+template __global__ void k_pairs_t2<double>(EsDirArgs, double*);
+template __global__ void k_pairs_t2<float>(EsDirArgs, float*);
+template __global__ void k_dd_field<double, double4>(EsDirArgs, const double*, const double4*);
+template __global__ void k_dd_field<float, float4>(EsDirArgs, const float*, const float4*);
+template __global__ void k_pack_mu<double4>(int, const double*, double4*);
+template __global__ void k_pack_mu<float4>(int, const double*, float4*);
+
+}  // namespace mdff

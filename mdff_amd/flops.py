@@ -38,3 +38,21 @@ PEAK_FP64_TFLOPS = 78.6     # FP64 vector and FP64 matrix
 PEAK_FP32_TFLOPS = 157.3    # FP32 vector (packed)
 PEAK_FP16_MFMA_TFLOPS = 2500.0  # dense FP16/BF16 matrix
 PEAK_HBM_GBS = 8000.0
+
+
+def mixed_kspace_flops(kernel: str, natm: int, kmax, steps: int, n_scf_total: int,
+                       efg: bool) -> float:
+    """Executed matrix-core flops of the mixed k-space GEMMs over a run of
+    `steps` evaluations with `n_scf_total` SCF iterations in all: per call
+    3 fp16 products of a real GEMM.
+      mx_rho:   rows 2*NW*(kx+1), cols (2ky+1)(2kz+1), K 2*natm
+      mx_apply: rows natm, K 2*(2ky+1)(2kz+1), cols 2*W*(kx+1)
+    NW = 1 (static call), 3 (SCF calls), 4 (final call); W = 3, or 6 in the
+    final call when EFGs or dipole forces are needed."""
+    kx, ky, kz = kmax
+    n1, c23 = kx + 1, (2 * ky + 1) * (2 * kz + 1)
+    if kernel == "mx_rho":
+        rows = [2 * 1 * n1] * steps + [2 * 3 * n1] * n_scf_total + [2 * 4 * n1] * steps
+        return sum(3 * 2.0 * r * c23 * 2 * natm for r in rows)
+    cols = [2 * 3 * n1] * (steps + n_scf_total) + [2 * 6 * n1] * steps
+    return sum(3 * 2.0 * natm * 2 * c23 * c for c in cols)

@@ -39,7 +39,8 @@ module mdffgpu_mod
     integer(c_int32_t) :: min_scf_pol_iter, max_scf_pol_iter, extrapolate_order
     integer(c_int32_t) :: algo_ext_dipole, doefield, doefg
     integer(c_int32_t) :: precision
-    integer(c_int32_t) :: reserved(7)
+    integer(c_int32_t) :: mixed_fp64_parts
+    integer(c_int32_t) :: reserved(6)
   end type
 
   type, bind(C), public :: mdffgpu_energies

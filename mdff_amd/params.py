@@ -29,6 +29,11 @@ TT_MAX_ORDER = 8
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+# mixed_fp64_parts bits (parts of the mixed path evaluated in FP64)
+MIXED_FP64_SHORTRANGE = 1
+MIXED_FP64_DIRECT = 2
+MIXED_FP64_SCF_FIELD = 4
+MIXED_FP64_KSPACE = 8
 EXT_POLY = 0
 EXT_ASPC = 1
 CALC_OTHER = 0
@@ -95,7 +100,8 @@ class MdffParams(ctypes.Structure):
         ("doefield", ctypes.c_int32),
         ("doefg", ctypes.c_int32),
         ("precision", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("mixed_fp64_parts", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 

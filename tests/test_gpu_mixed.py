@@ -1,0 +1,60 @@
+"""Mixed-precision path (FP32 real space, fp16-split MFMA k-space, FP64
+reductions) against the FP64 GPU path on the same inputs.  Bar: relative
+force RMS error <= 1e-5 (the stated target), dipoles / EFG / energies to
+1e-5."""
+import copy
+
+import numpy as np
+import pytest
+
+from mdff_amd import configs as C
+from mdff_amd import params as P
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
+
+
+def _pair(s, steps=1, perturb=0.0):
+    from mdff_amd.field import ForceField
+    p64 = copy.deepcopy(s.params)
+    p64.precision = P.PREC_FP64
+    pmx = copy.deepcopy(s.params)
+    pmx.precision = P.PREC_MIXED
+    f64 = ForceField(p64, s.itype, s.A)
+    fmx = ForceField(pmx, s.itype, s.A)
+    rng = np.random.default_rng(3)
+    r = s.r.copy()
+    out = []
+    for it in range(1, steps + 1):
+        out.append((fmx.engforce(r, itime=it), f64.engforce(r, itime=it)))
+        r = r + perturb * rng.standard_normal(r.shape)
+    f64.close()
+    fmx.close()
+    return out
+
+
+@pytest.mark.parametrize("pim", [False, True])
+def test_mixed_vs_fp64_silica(pim):
+    s = C.silica(200, 400, seed=31, rc=9.0, kmax=8, pim=pim, efg=True)
+    for g, o in _pair(s, steps=2, perturb=0.01):
+        assert rel_rms(g["f"], o["f"]) <= 1e-5, rel_rms(g["f"], o["f"])
+        assert abs(g["u_coul"] - o["u_coul"]) <= 1e-6 * abs(o["u_coul"])
+        assert abs(g["u_bmhft"] - o["u_bmhft"]) <= 1e-6 * abs(o["u_bmhft"])
+        assert rel_rms(g["efg"], o["efg"]) <= 1e-5
+        if pim:
+            assert rel_rms(g["mu"], o["mu"]) <= 1e-5
+            # the mixed SCF ends at tolerance or when the FP32 noise floor stalls
+            # it; either way well before the iteration cap
+            assert g["n_scf"] < s.params.max_scf_pol_iter
+        assert np.abs(g["tau_coul"] - o["tau_coul"]).max() <= 1e-5 * np.abs(o["tau_coul"]).max()
+
+
+def test_mixed_ex9_reference_energy():
+    import golden_cases as G
+    from mdff_amd.field import ForceField
+    p, itype, A, r, meta = G.ex9_test11(precision=P.PREC_MIXED)
+    g = ForceField(p, itype, A).engforce(r, itime=0)
+    assert abs(g["u_bmhft"] + g["u_coul"] - meta["reference_utot"]) <= 1e-6 * abs(meta["reference_utot"])
