@@ -233,6 +233,73 @@ int mdffgpu_profile_read(const mdffgpu_ctx* ctx, int32_t kernel,
 const char* mdffgpu_kernel_name(int32_t kernel);
 int32_t mdffgpu_kernel_count(void);
 
+/* ---------------------------------------------------------------------
+ * Batched evaluation and force-matching adjoint (no counterpart in the
+ * reference, which evaluates one configuration per process; used from
+ * the Python tooling for parameter fitting).  Many configurations of
+ * uneven atom counts share one model (mdffgpu_params: potential, charges,
+ * polarizabilities, damping, precision); each has its own cell, Ewald
+ * parameter, k mesh and cutoffs.  Atom arrays are concatenated over the
+ * configurations; device arrays are SoA [x|y|z] over all atoms.
+ * Supported model: one or two species, BMHFTD/LJ/Morse pair forces,
+ * point charges, isotropic induced dipoles with Tang-Toennies damping,
+ * no permanent dipoles, no surface term.
+ * ------------------------------------------------------------------- */
+typedef struct mdffgpu_batch mdffgpu_batch;
+
+typedef struct mdffgpu_batch_config {
+  int32_t natm;
+  int32_t kmax[3];      /* 1..12 */
+  double A[9];          /* lattice vectors as columns */
+  double alphaES;       /* Ewald alpha */
+  double cutlongrange;  /* real-space Ewald cutoff, at most half a width */
+  double cutshortrange; /* short-range cutoff, at most half a width */
+} mdffgpu_batch_config;
+
+int mdffgpu_batch_create(const mdffgpu_params* model, int32_t nconf,
+                         const mdffgpu_batch_config* cfgs,
+                         const int32_t* itype, mdffgpu_batch** out);
+void mdffgpu_batch_destroy(mdffgpu_batch* b);
+
+/* New parameter values (same species, flags and precision). */
+int mdffgpu_batch_set_model(mdffgpu_batch* b, const mdffgpu_params* model);
+
+/* Positions of all atoms (device SoA); builds the neighbour lists, phase
+   tables and the SCF pair table.  stream: hipStream_t the inputs were
+   produced on (NULL = none). */
+int mdffgpu_batch_set_positions(mdffgpu_batch* b, const double* d_pos,
+                                void* stream);
+
+/* Forces (device SoA, overwritten) of every atom.  scf_iters > 0: exactly
+   that many dipole iterations; 0: until conv_tol_ind (residual over all
+   polarizable atoms of the batch). */
+int mdffgpu_batch_forces(mdffgpu_batch* b, double* d_force, int32_t scf_iters,
+                         int32_t* n_scf, double* rmsd, void* stream);
+
+/* Force-matching loss L = sum |F - F_target|^2 / (3 N) over all atoms and
+   its gradient with respect to the parameter vector
+     [A, B, C6, C8, BD] per unordered species pair (11, 12, 22),
+     qch per species, polarizability per polarizable species
+   (mdffgpu_batch_num_params entries).  d_force (may be NULL) receives the
+   forces.  The SCF iteration count is as in mdffgpu_batch_forces. */
+int32_t mdffgpu_batch_num_params(const mdffgpu_batch* b);
+int64_t mdffgpu_batch_num_atoms(const mdffgpu_batch* b);
+int mdffgpu_batch_loss_grad(mdffgpu_batch* b, const double* d_target,
+                            double* d_force, double* loss, double* grad,
+                            int32_t scf_iters, int32_t* n_scf, double* rmsd,
+                            void* stream);
+
+/* enable != 0: mdffgpu_batch_loss_grad records its launches in a hipGraph
+   on the first call (fixed scf_iters > 0 required) and replays it while the
+   iteration count, positions and tuning stay the same. */
+int mdffgpu_batch_graph(mdffgpu_batch* b, int32_t enable);
+
+/* Launch shape chosen by the autotuner: lanes per atom of the
+   reciprocal-space field/force kernel (2, 4, 8 or 16). */
+int mdffgpu_batch_set_tuning(mdffgpu_batch* b, int32_t apply_lanes);
+
+const char* mdffgpu_batch_last_error(const mdffgpu_batch* b);
+
 /* Number of visible HIP devices (0 without a GPU); never fails. */
 int32_t mdffgpu_device_count(void);
 

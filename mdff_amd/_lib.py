@@ -10,7 +10,7 @@ import ctypes
 import importlib.util
 import os
 
-from .params import MdffEnergies, MdffParams
+from .params import MdffBatchConfig, MdffEnergies, MdffParams
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmdffgpu.so")
 
@@ -37,6 +37,20 @@ PROTOTYPES = {
                                             ctypes.POINTER(ctypes.c_int64)]),
     "mdffgpu_kernel_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "mdffgpu_kernel_count": (ctypes.c_int32, []),
+    "mdffgpu_batch_create": (ctypes.c_int, [ctypes.POINTER(MdffParams), ctypes.c_int32,
+                                            ctypes.POINTER(MdffBatchConfig), _ip,
+                                            ctypes.POINTER(_vp)]),
+    "mdffgpu_batch_destroy": (None, [_vp]),
+    "mdffgpu_batch_set_model": (ctypes.c_int, [_vp, ctypes.POINTER(MdffParams)]),
+    "mdffgpu_batch_set_positions": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mdffgpu_batch_forces": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _ip, _dp, _vp]),
+    "mdffgpu_batch_num_params": (ctypes.c_int32, [_vp]),
+    "mdffgpu_batch_num_atoms": (ctypes.c_int64, [_vp]),
+    "mdffgpu_batch_loss_grad": (ctypes.c_int, [_vp, _vp, _vp, _dp, _dp, ctypes.c_int32, _ip,
+                                               _dp, _vp]),
+    "mdffgpu_batch_graph": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mdffgpu_batch_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mdffgpu_batch_last_error": (ctypes.c_char_p, [_vp]),
 }
 
 
@@ -84,6 +98,12 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def check_batch(status: int, b=None) -> None:
+    if status != 0:
+        msg = lib().mdffgpu_batch_last_error(b)
+        raise MdffError(status, msg.decode() if msg else "")
 
 
 def check(status: int, ctx=None) -> None:

@@ -35,6 +35,22 @@ struct ProfScope {
   ~ProfScope() noexcept(false);
 };
 
+// Per-configuration constants of a batch of configurations (the batched
+// path evaluates many cells at once; kernels instantiated with BATCH read
+// the cell, Ewald parameter and cutoffs of atom i's configuration here).
+struct BatchCfg {
+  Cell cell;
+  double alpha;   // Ewald alpha
+  double cut2;    // real-space Ewald cutoff^2
+  double rc2;     // short-range cutoff^2
+  double fpi_V;   // 4 pi / Omega
+  double selfa2;  // 2 alpha^3 / (3 sqrt(pi))
+  int n, off;     // atoms, first atom in the batch arrays
+  int koff, nk;   // half-space k vectors of this configuration
+  int kmax[3];
+  int pad;
+};
+
 // A chunk of KC consecutive k vectors (n1, n2, n3s .. n3s+KC-1) of the
 // half-space mesh of kpoint_sum_init (src/kspace.f90:162-191).
 struct KChunk {
@@ -55,6 +71,8 @@ struct SrArgs {
   int flags;
   double* f;
   double* part;
+  const BatchCfg* bc;  // batched path: per-configuration constants
+  const int* cfg;      // batched path: configuration of each atom
 };
 
 struct EsDirArgs {
@@ -75,6 +93,8 @@ struct EsDirArgs {
   double* efg;
   double* f;
   double* part;
+  const BatchCfg* bc;  // batched path: per-configuration constants
+  const int* cfg;      // batched path: configuration of each atom
 };
 
 struct RhoArgs {
@@ -143,11 +163,11 @@ __global__ void k_prepare(int n, const double* pos, Cell cell, double* s, double
                           double2* eph);
 __global__ void k_nlist(int n, const double* s, Cell cell, double cut2, int maxnb, int* cnt,
                         int* nbr, int* overflow);
-template <typename R>
+template <typename R, bool BATCH = false>
 __global__ void k_shortrange(SrArgs a);
-template <int MODE, typename R>
+template <int MODE, typename R, bool BATCH = false>
 __global__ void k_es_dir(EsDirArgs a);
-template <typename R>
+template <typename R, bool BATCH = false>
 __global__ void k_pairs_t2(EsDirArgs a, R* tab);
 template <typename R, typename R4>
 __global__ void k_dd_field(EsDirArgs a, const R* tab, const R4* mu4);

@@ -15,6 +15,7 @@
 #include "../../include/mdffgpu.h"
 #include "kernels.h"
 #include "mixed.h"
+#include "tables.h"
 
 using namespace mdff;
 
@@ -195,60 +196,10 @@ void prof_collect(mdffgpu_ctx* c) {
 
 // 3x3 inverse, Gauss-Jordan with partial pivoting (DGETRF/DGETRI of
 // src/read_pos.f90:201-210 for the polarizability); singular -> zeros.
-void inv3(const double* m, double* out) {
-  double a[3][6];
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) {
-      a[i][j] = m[i + 3 * j];
-      a[i][j + 3] = (i == j) ? 1.0 : 0.0;
-    }
-  for (int col = 0; col < 3; ++col) {
-    int piv = col;
-    for (int r = col + 1; r < 3; ++r)
-      if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
-    if (a[piv][col] == 0.0) {
-      for (int k = 0; k < 9; ++k) out[k] = 0.0;
-      return;
-    }
-    if (piv != col)
-      for (int k = 0; k < 6; ++k) std::swap(a[col][k], a[piv][k]);
-    double d = a[col][col];
-    for (int k = 0; k < 6; ++k) a[col][k] = a[col][k] / d;
-    for (int r = 0; r < 3; ++r)
-      if (r != col) {
-        double f = a[r][col];
-        for (int k = 0; k < 6; ++k) a[r][k] = a[r][k] - f * a[col][k];
-      }
-  }
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) out[i + 3 * j] = a[i][j + 3];
-}
 
 // lattice (src/lattice.f90:60-168)
 void set_lattice(mdffgpu_ctx* c, const double* A) {
-  Cell& cl = c->cell;
-  std::memcpy(cl.A, A, 9 * sizeof(double));
-  const double* a1 = A;
-  const double* a2 = A + 3;
-  const double* a3 = A + 6;
-  double* B = cl.B;
-  B[0] = a2[1] * a3[2] - a2[2] * a3[1];
-  B[1] = a2[2] * a3[0] - a2[0] * a3[2];
-  B[2] = a2[0] * a3[1] - a2[1] * a3[0];
-  B[3] = a3[1] * a1[2] - a3[2] * a1[1];
-  B[4] = a3[2] * a1[0] - a3[0] * a1[2];
-  B[5] = a3[0] * a1[1] - a3[1] * a1[0];
-  B[6] = a1[1] * a2[2] - a1[2] * a2[1];
-  B[7] = a1[2] * a2[0] - a1[0] * a2[2];
-  B[8] = a1[0] * a2[1] - a1[1] * a2[0];
-  double omega = B[0] * A[0] + B[1] * A[1] + B[2] * A[2];
-  if (!(omega > 0.0)) fail(c, MDFFGPU_ERR_ARG, "cell volume %g is not positive", omega);
-  cl.omega = omega;
-  for (int i = 0; i < 3; ++i) {
-    const double* b = B + 3 * i;
-    c->W[i] = omega / std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-  }
-  for (int i = 0; i < 9; ++i) B[i] = B[i] / omega;
+  if (!make_cell(A, c->cell, c->W)) fail(c, MDFFGPU_ERR_ARG, "cell volume is not positive");
 }
 
 // Half-space k mesh of kpoint_sum_init (src/kspace.f90:142-207) arranged
@@ -339,76 +290,17 @@ void build_kmesh(mdffgpu_ctx* c) {
 }
 
 void build_tables(mdffgpu_ctx* c) {
-  const mdffgpu_params& P = c->P;
-  PairTables& T = c->pt_h;
-  std::memset(&T, 0, sizeof(T));
-  const double two16 = std::pow(2.0, 1.0 / 6.0);
-  const double rc = P.cutshortrange, rcsq = rc * rc;
-  const int nt = P.ntype;
-  auto L = [&](int a, int b, int cc) { return P.ldip_damping[a + 16 * (b + 16 * cc)] != 0; };
-  auto D = [&](const double* tab, int a, int b, int cc) { return tab[a + 16 * (b + 16 * cc)]; };
-  for (int ti = 0; ti < nt; ++ti)
-    for (int tj = 0; tj < nt; ++tj) {
-      const int t = ti + 16 * tj;
-      // initialize_param_non_bonded, src/field.f90:1021-1064
-      const double pp = P.plj[t], qq = P.qlj[t];
-      const double epsp = P.epslj[t] / (qq - pp);
-      const double sigsq = two16 * two16 * P.sigmalj[t] * P.sigmalj[t];
-      const double sr2 = sigsq / rcsq;
-      const double sr = std::sqrt(sr2);
-      const double srp = std::pow(sr, pp), srq = std::pow(sr, qq);
-      const double ppqq = pp * qq;
-      T.lj_epsp[t] = epsp;
-      T.lj_sigsq[t] = sigsq;
-      T.lj_fc[t] = ppqq * epsp / sigsq;
-      T.lj_ptwo[t] = pp * 0.5;
-      T.lj_qtwo[t] = qq * 0.5;
-      T.lj_p[t] = pp;
-      T.lj_q[t] = qq;
-      T.lj_uc[t] = epsp * (pp * srq - qq * srp);
-      T.lj_uc1[t] = epsp * ppqq / (2.0 * rcsq) * (srq - srp);
-      T.lj_uc2[t] = 0.5 * epsp * ((2.0 * pp + ppqq) * srq - (2.0 * qq + ppqq) * srp);
-      T.bm_A[t] = P.Abmhftd[t];
-      T.bm_B[t] = P.Bbmhftd[t];
-      T.bm_C[t] = P.Cbmhftd[t];
-      T.bm_D[t] = P.Dbmhftd[t];
-      T.bm_BD[t] = P.BDbmhftd[t];
-      T.mo_eps[t] = P.epsmor[t];
-      T.mo_rho[t] = P.rhomor[t];
-      T.mo_rs[t] = std::exp(P.rhomor[t] * P.sigmamor[t]);
-      // damping as seen from ti (src/field.f90:2207-2211)
-      T.dmp_on[0][t] = L(ti, ti, tj) || L(tj, ti, tj);
-      T.dmp_on[1][t] = L(tj, tj, ti) || L(ti, tj, ti);
-      T.dmp_b[0][t] = D(P.dip_pol_damp_b, ti, ti, tj);
-      T.dmp_c[0][t] = D(P.dip_pol_damp_c, ti, ti, tj);
-      T.dmp_k[0][t] = P.dip_pol_damp_k[ti + 16 * (ti + 16 * tj)];
-      T.dmp_b[1][t] = D(P.dip_pol_damp_b, ti, tj, ti);
-      T.dmp_c[1][t] = D(P.dip_pol_damp_c, ti, tj, ti);
-      T.dmp_k[1][t] = P.dip_pol_damp_k[ti + 16 * (tj + 16 * ti)];
-      for (int o = 0; o < 2; ++o)
-        if (T.dmp_k[o][t] < 0 || T.dmp_k[o][t] > MDFFGPU_TT_MAX_ORDER)
-          fail(c, MDFFGPU_ERR_TT_ORDER, "Tang-Toennies order %d > %d", T.dmp_k[o][t],
-               MDFFGPU_TT_MAX_ORDER);
-    }
-  TypeTables& Y = c->tt_h;
-  std::memset(&Y, 0, sizeof(Y));
+  const int bad = make_tables(c->P, c->pt_h, c->tt_h, c->tt);
+  if (bad >= 0)
+    fail(c, MDFFGPU_ERR_TT_ORDER, "Tang-Toennies order %d > %d", bad, MDFFGPU_TT_MAX_ORDER);
   c->linduced = 0;
   c->ldip_static = 0;
   for (int k = 0; k < 48; ++k)
-    if (P.dip[k] != 0.0) c->ldip_static = 1;
-  for (int t = 0; t < 16; ++t) {
-    Y.qch[t] = P.qch[t];
-    for (int a = 0; a < 3; ++a) Y.dip[t][a] = P.dip[t + 16 * a];
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) Y.pol[t][a + 3 * b] = P.poldip[t + 16 * (a + 3 * b)];
-    Y.polar[t] = P.ldip_polar[t] ? 1 : 0;
-    if (Y.polar[t]) inv3(Y.pol[t], Y.invpol[t]);
-    if (t < nt && Y.polar[t]) c->linduced = 1;
-  }
-  c->tt.e[0] = 1.0;
-  for (int k = 1; k <= 8; ++k) c->tt.e[k] = c->tt.e[k - 1] / (double)k;
-  HCK(c, hipMemcpy(c->d_pt, &T, sizeof(T), hipMemcpyHostToDevice));
-  HCK(c, hipMemcpy(c->d_tt, &Y, sizeof(Y), hipMemcpyHostToDevice));
+    if (c->P.dip[k] != 0.0) c->ldip_static = 1;
+  for (int t = 0; t < c->P.ntype; ++t)
+    if (c->tt_h.polar[t]) c->linduced = 1;
+  HCK(c, hipMemcpy(c->d_pt, &c->pt_h, sizeof(c->pt_h), hipMemcpyHostToDevice));
+  HCK(c, hipMemcpy(c->d_tt, &c->tt_h, sizeof(c->tt_h), hipMemcpyHostToDevice));
 }
 
 void validate(mdffgpu_ctx* c, const mdffgpu_params* p) {
@@ -1071,6 +963,129 @@ void copy_in_host(mdffgpu_ctx* c, const double* rx, const double* ry, const doub
 
 // ===========================================================================
 // mixed-precision hooks need access to the context internals
+namespace mdff {
+
+void inv3(const double* m, double* out) {
+  double a[3][6];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      a[i][j] = m[i + 3 * j];
+      a[i][j + 3] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int col = 0; col < 3; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < 3; ++r)
+      if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+    if (a[piv][col] == 0.0) {
+      for (int k = 0; k < 9; ++k) out[k] = 0.0;
+      return;
+    }
+    if (piv != col)
+      for (int k = 0; k < 6; ++k) std::swap(a[col][k], a[piv][k]);
+    double d = a[col][col];
+    for (int k = 0; k < 6; ++k) a[col][k] = a[col][k] / d;
+    for (int r = 0; r < 3; ++r)
+      if (r != col) {
+        double f = a[r][col];
+        for (int k = 0; k < 6; ++k) a[r][k] = a[r][k] - f * a[col][k];
+      }
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[i + 3 * j] = a[i][j + 3];
+}
+
+bool make_cell(const double* A, Cell& cl, double W[3]) {
+  std::memcpy(cl.A, A, 9 * sizeof(double));
+  const double* a1 = A;
+  const double* a2 = A + 3;
+  const double* a3 = A + 6;
+  double* B = cl.B;
+  B[0] = a2[1] * a3[2] - a2[2] * a3[1];
+  B[1] = a2[2] * a3[0] - a2[0] * a3[2];
+  B[2] = a2[0] * a3[1] - a2[1] * a3[0];
+  B[3] = a3[1] * a1[2] - a3[2] * a1[1];
+  B[4] = a3[2] * a1[0] - a3[0] * a1[2];
+  B[5] = a3[0] * a1[1] - a3[1] * a1[0];
+  B[6] = a1[1] * a2[2] - a1[2] * a2[1];
+  B[7] = a1[2] * a2[0] - a1[0] * a2[2];
+  B[8] = a1[0] * a2[1] - a1[1] * a2[0];
+  const double omega = B[0] * A[0] + B[1] * A[1] + B[2] * A[2];
+  if (!(omega > 0.0)) return false;
+  cl.omega = omega;
+  for (int i = 0; i < 3; ++i) {
+    const double* b = B + 3 * i;
+    W[i] = omega / std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  }
+  for (int i = 0; i < 9; ++i) B[i] = B[i] / omega;
+  return true;
+}
+
+int make_tables(const mdffgpu_params& P, PairTables& T, TypeTables& Y, TTTable& tt) {
+  int bad = -1;
+  std::memset(&T, 0, sizeof(T));
+  const double two16 = std::pow(2.0, 1.0 / 6.0);
+  const double rc = P.cutshortrange, rcsq = rc * rc;
+  const int nt = P.ntype;
+  auto L = [&](int a, int b, int cc) { return P.ldip_damping[a + 16 * (b + 16 * cc)] != 0; };
+  auto D = [&](const double* tab, int a, int b, int cc) { return tab[a + 16 * (b + 16 * cc)]; };
+  for (int ti = 0; ti < nt; ++ti)
+    for (int tj = 0; tj < nt; ++tj) {
+      const int t = ti + 16 * tj;
+      // initialize_param_non_bonded, src/field.f90:1021-1064
+      const double pp = P.plj[t], qq = P.qlj[t];
+      const double epsp = P.epslj[t] / (qq - pp);
+      const double sigsq = two16 * two16 * P.sigmalj[t] * P.sigmalj[t];
+      const double sr2 = sigsq / rcsq;
+      const double sr = std::sqrt(sr2);
+      const double srp = std::pow(sr, pp), srq = std::pow(sr, qq);
+      const double ppqq = pp * qq;
+      T.lj_epsp[t] = epsp;
+      T.lj_sigsq[t] = sigsq;
+      T.lj_fc[t] = ppqq * epsp / sigsq;
+      T.lj_ptwo[t] = pp * 0.5;
+      T.lj_qtwo[t] = qq * 0.5;
+      T.lj_p[t] = pp;
+      T.lj_q[t] = qq;
+      T.lj_uc[t] = epsp * (pp * srq - qq * srp);
+      T.lj_uc1[t] = epsp * ppqq / (2.0 * rcsq) * (srq - srp);
+      T.lj_uc2[t] = 0.5 * epsp * ((2.0 * pp + ppqq) * srq - (2.0 * qq + ppqq) * srp);
+      T.bm_A[t] = P.Abmhftd[t];
+      T.bm_B[t] = P.Bbmhftd[t];
+      T.bm_C[t] = P.Cbmhftd[t];
+      T.bm_D[t] = P.Dbmhftd[t];
+      T.bm_BD[t] = P.BDbmhftd[t];
+      T.mo_eps[t] = P.epsmor[t];
+      T.mo_rho[t] = P.rhomor[t];
+      T.mo_rs[t] = std::exp(P.rhomor[t] * P.sigmamor[t]);
+      // damping as seen from ti (src/field.f90:2207-2211)
+      T.dmp_on[0][t] = L(ti, ti, tj) || L(tj, ti, tj);
+      T.dmp_on[1][t] = L(tj, tj, ti) || L(ti, tj, ti);
+      T.dmp_b[0][t] = D(P.dip_pol_damp_b, ti, ti, tj);
+      T.dmp_c[0][t] = D(P.dip_pol_damp_c, ti, ti, tj);
+      T.dmp_k[0][t] = P.dip_pol_damp_k[ti + 16 * (ti + 16 * tj)];
+      T.dmp_b[1][t] = D(P.dip_pol_damp_b, ti, tj, ti);
+      T.dmp_c[1][t] = D(P.dip_pol_damp_c, ti, tj, ti);
+      T.dmp_k[1][t] = P.dip_pol_damp_k[ti + 16 * (tj + 16 * ti)];
+      for (int o = 0; o < 2; ++o)
+        if ((T.dmp_k[o][t] < 0 || T.dmp_k[o][t] > MDFFGPU_TT_MAX_ORDER) && bad < 0)
+          bad = T.dmp_k[o][t];
+    }
+  std::memset(&Y, 0, sizeof(Y));
+  for (int t = 0; t < 16; ++t) {
+    Y.qch[t] = P.qch[t];
+    for (int a = 0; a < 3; ++a) Y.dip[t][a] = P.dip[t + 16 * a];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) Y.pol[t][a + 3 * b] = P.poldip[t + 16 * (a + 3 * b)];
+    Y.polar[t] = P.ldip_polar[t] ? 1 : 0;
+    if (Y.polar[t]) inv3(Y.pol[t], Y.invpol[t]);
+  }
+  tt.e[0] = 1.0;
+  for (int k = 1; k <= 8; ++k) tt.e[k] = tt.e[k - 1] / (double)k;
+  return bad;
+}
+
+}  // namespace mdff
+
 namespace mdff {
 namespace mixed {
 CtxView view(mdffgpu_ctx* c) {

@@ -3,76 +3,9 @@
 // the plain FP64 path, float for the mixed path.  Block partial sums of
 // energies, virials and stress are always reduced in double.
 #include "kernels.h"
+#include "rtypes.h"
 
 namespace mdff {
-
-template <typename R>
-__device__ __forceinline__ R rexp(R x);
-template <>
-__device__ __forceinline__ double rexp<double>(double x) {
-  return exp(x);
-}
-template <>
-__device__ __forceinline__ float rexp<float>(float x) {
-  return __expf(x);
-}
-
-template <typename R>
-__device__ __forceinline__ R rpow(R x, R y);
-template <>
-__device__ __forceinline__ double rpow<double>(double x, double y) {
-  return pow(x, y);
-}
-template <>
-__device__ __forceinline__ float rpow<float>(float x, float y) {
-  return powf(x, y);
-}
-
-// errfc of src/functions.f90:28-40 in type R
-template <typename R>
-__device__ __forceinline__ R errfc_t(R x) {
-  const R z = fabs(x);
-  const R t = R(1) / (R(1) + R(0.5) * z);
-  const R p =
-      t * (R(1.00002368) +
-           t * (R(.37409196) +
-                t * (R(.09678418) +
-                     t * (R(-.18628806) +
-                          t * (R(.27886807) +
-                               t * (R(-1.13520398) +
-                                    t * (R(1.48851587) + t * (R(-.82215223) + t * R(.17087277)))))))));
-  const R r = t * rexp<R>(-z * z - R(1.26551223) + p);
-  return (x < R(0)) ? R(2) - r : r;
-}
-
-// Tang-Toennies f and f' (src/field.f90:3775-3805) in type R
-template <typename R>
-__device__ __forceinline__ void tt_damp_t(const TTTable& E, R b, R c, R r, int order, R& f,
-                                          R& fd) {
-  const R br = b * r;
-  const R ex = rexp<R>(-br) * c;
-  R ff = R(E.e[order]);
-  for (int k = order - 1; k >= 1; --k) ff = ff * br + R(E.e[k]);
-  ff = ff * br + R(E.e[0]);
-  f = R(1) - ff * ex;
-  R brn = R(1);
-  for (int k = 0; k < order; ++k) brn *= br;
-  fd = R(E.e[order]) * brn * ex * b;
-}
-
-// Minimum image in type R: the fractional difference is formed in double
-// (positions are stored in double), the rest in R.
-template <typename R>
-__device__ __forceinline__ R min_image_t(const Cell& c, double dsx, double dsy, double dsz,
-                                         R& rx, R& ry, R& rz) {
-  const R sx = R(dsx - nint_d(dsx));
-  const R sy = R(dsy - nint_d(dsy));
-  const R sz = R(dsz - nint_d(dsz));
-  rx = sx * R(c.A[0]) + sy * R(c.A[3]) + sz * R(c.A[6]);
-  ry = sx * R(c.A[1]) + sy * R(c.A[4]) + sz * R(c.A[7]);
-  rz = sx * R(c.A[2]) + sy * R(c.A[5]) + sz * R(c.A[8]);
-  return rx * rx + ry * ry + rz * rz;
-}
 
 // ---------------------------------------------------------------------------
 // Short-range pair forces: n-m LJ (engforce_nmlj_pbc, src/field.f90:1181),
@@ -82,7 +15,7 @@ __device__ __forceinline__ R min_image_t(const Cell& c, double dsx, double dsy, 
 // reference's routine overwrites them (:2902-2907).  Partial rows:
 //   0 u_lj 1 vir_lj 2 u_bm 3 vir_bm 4 u_mo 5 vir_mo 6..11 tau (xx yy zz xy
 //   xz yz) of the terms that contribute forces.
-template <typename R>
+template <typename R, bool BATCH>
 __global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(SrArgs a) {
   __shared__ double red[4 * 12];
   const int n = a.n;
@@ -99,12 +32,14 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(SrArgs a) {
     const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
     const int ti = a.typ[i];
     const int c = a.cnt[i];
-    const R rc2 = R(a.rc2);
+    const int cf = BATCH ? a.cfg[i] : 0;
+    const Cell& cell = BATCH ? a.bc[cf].cell : a.cell;
+    const R rc2 = R(BATCH ? a.bc[cf].rc2 : a.rc2);
     const int* nb = a.nbr + (size_t)i * a.maxnb;
     for (int k = sub; k < c; k += LPA) {
       const int j = nb[k];
       R rx, ry, rz;
-      const R rsq = min_image_t<R>(a.cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
+      const R rsq = min_image_t<R>(cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
                                    rx, ry, rz);
       if (!(rsq < rc2)) continue;
       const int t = ti + MDFF_NT * a.typ[j];
@@ -182,8 +117,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(SrArgs a) {
     for (int k = 0; k < 12; ++k) a.part[(size_t)k * gridDim.x + blockIdx.x] = dacc[k];
 }
 
-template __global__ void k_shortrange<double>(SrArgs);
-template __global__ void k_shortrange<float>(SrArgs);
+template __global__ void k_shortrange<double, false>(SrArgs);
+template __global__ void k_shortrange<float, false>(SrArgs);
+template __global__ void k_shortrange<double, true>(SrArgs);
+template __global__ void k_shortrange<float, true>(SrArgs);
 
 // ---------------------------------------------------------------------------
 // Ewald real space, multipole_ES_dir (src/field.f90:2057-2643), charges and
@@ -199,7 +136,7 @@ template __global__ void k_shortrange<float>(SrArgs);
 // where p_i are the damping parameters the reference uses for atom i of the
 // pair (dmp_* slot 0 if i < j, slot 1 otherwise).
 // Partial rows: 0 u, 1..6 tau (xx yy zz xy xz yz), before the 1/(2 Omega).
-template <int MODE, typename R>
+template <int MODE, typename R, bool BATCH>
 __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
   __shared__ double red[4 * 7];
   const int n = a.n;
@@ -208,12 +145,14 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
   const bool EFG = (MODE == ES_FINAL) && a.do_efg;
   const bool FORCE = (MODE == ES_FINAL);
   const bool DAMP = (MODE != ES_IND);
-  const R alpha = R(a.alpha), alpha2 = alpha * alpha, alpha3 = alpha2 * alpha,
-          alpha5 = alpha3 * alpha2;
-  const R rpiroot = R(1.0 / sqrt(MDFF_PI));
-  const R cut2 = R(a.cut2);
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = tid / LPA, sub = tid % LPA;
+  const int cf = (BATCH && i < n) ? a.cfg[i] : 0;
+  const Cell& cell = BATCH ? a.bc[cf].cell : a.cell;
+  const R alpha = R(BATCH ? a.bc[cf].alpha : a.alpha), alpha2 = alpha * alpha,
+          alpha3 = alpha2 * alpha, alpha5 = alpha3 * alpha2;
+  const R rpiroot = R(1.0 / sqrt(MDFF_PI));
+  const R cut2 = R(BATCH ? a.bc[cf].cut2 : a.cut2);
   R acc[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) acc[k] = R(0);
@@ -231,7 +170,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
     for (int k = sub; k < c; k += LPA) {
       const int j = nb[k];
       R rx, ry, rz;
-      const R d2 = min_image_t<R>(a.cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
+      const R d2 = min_image_t<R>(cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
                                   rx, ry, rz);
       if (d2 > cut2) continue;
       const R qj = R(a.q[j]);
@@ -406,11 +345,15 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
   }
 }
 
-template __global__ void k_es_dir<ES_STATIC, double>(EsDirArgs);
-template __global__ void k_es_dir<ES_IND, double>(EsDirArgs);
-template __global__ void k_es_dir<ES_FINAL, double>(EsDirArgs);
-template __global__ void k_es_dir<ES_STATIC, float>(EsDirArgs);
-template __global__ void k_es_dir<ES_FINAL, float>(EsDirArgs);
+template __global__ void k_es_dir<ES_STATIC, double, false>(EsDirArgs);
+template __global__ void k_es_dir<ES_IND, double, false>(EsDirArgs);
+template __global__ void k_es_dir<ES_FINAL, double, false>(EsDirArgs);
+template __global__ void k_es_dir<ES_STATIC, float, false>(EsDirArgs);
+template __global__ void k_es_dir<ES_FINAL, float, false>(EsDirArgs);
+template __global__ void k_es_dir<ES_STATIC, double, true>(EsDirArgs);
+template __global__ void k_es_dir<ES_FINAL, double, true>(EsDirArgs);
+template __global__ void k_es_dir<ES_STATIC, float, true>(EsDirArgs);
+template __global__ void k_es_dir<ES_FINAL, float, true>(EsDirArgs);
 
 // ---------------------------------------------------------------------------
 // Dipole field tensor of the SCF (multipole_ES_dir with task d-d and no
@@ -419,12 +362,16 @@ template __global__ void k_es_dir<ES_FINAL, float>(EsDirArgs);
 // two radial factors 3F2/d^5, F1/d^3 in type R, in the row layout of nbr
 // (5 planes); pairs outside the cutoff get zero factors.  k_dd_field applies
 // E_i = sum_j [r (r.mu_j) 3F2/d^5 - mu_j F1/d^3] from that table.
-template <typename R>
+template <typename R, bool BATCH>
 __global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, R* __restrict__ tab) {
   const int n = a.n;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = tid / LPA, sub = tid % LPA;
   if (i >= n) return;
+  const int cf = BATCH ? a.cfg[i] : 0;
+  const Cell& cell = BATCH ? a.bc[cf].cell : a.cell;
+  const double cut2 = BATCH ? a.bc[cf].cut2 : a.cut2;
+  const double alpha_ = BATCH ? a.bc[cf].alpha : a.alpha;
   const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
   const int c = a.cnt[i];
   const size_t plane = (size_t)a.maxnb * n;
@@ -433,11 +380,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, R* __restr
     const int j = a.nbr[e];
     R rx, ry, rz, A2 = R(0), A1 = R(0);
     if constexpr (sizeof(R) == sizeof(double)) {
-      const double alpha = a.alpha, alpha2 = alpha * alpha, alpha3 = alpha2 * alpha;
+      const double alpha = alpha_, alpha2 = alpha * alpha, alpha3 = alpha2 * alpha;
       const double piroot = sqrt(MDFF_PI);
-      const double d2 = min_image(a.cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
+      const double d2 = min_image(cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
                                   rx, ry, rz);
-      if (!(d2 > a.cut2)) {
+      if (!(d2 > cut2)) {
         const double d = sqrt(d2);
         const double d3 = d2 * d;
         const double dm1 = 1.0 / d;
@@ -452,10 +399,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, R* __restr
       }
     } else {
       // FP32 pair arithmetic; the fractional difference is formed in FP64
-      const R alpha = R(a.alpha);
-      const R d2 = min_image_t<R>(a.cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
+      const R alpha = R(alpha_);
+      const R d2 = min_image_t<R>(cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
                                   rx, ry, rz);
-      if (!(d2 > R(a.cut2))) {
+      if (!(d2 > R(cut2))) {
         const R d = sqrt(d2);
         const R dm1 = R(1) / d;
         const R dm3 = dm1 * dm1 * dm1;
@@ -542,8 +489,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a, const R* _
   }
 }
 
-template __global__ void k_pairs_t2<double>(EsDirArgs, double*);
-template __global__ void k_pairs_t2<float>(EsDirArgs, float*);
+template __global__ void k_pairs_t2<double, false>(EsDirArgs, double*);
+template __global__ void k_pairs_t2<float, false>(EsDirArgs, float*);
+template __global__ void k_pairs_t2<double, true>(EsDirArgs, double*);
+template __global__ void k_pairs_t2<float, true>(EsDirArgs, float*);
 template __global__ void k_dd_field<double, double4>(EsDirArgs, const double*, const double4*);
 template __global__ void k_dd_field<float, float4>(EsDirArgs, const float*, const float4*);
 template __global__ void k_pack_mu<double4>(int, const double*, double4*);

@@ -264,3 +264,16 @@ def num_kvectors(kmax) -> int:
     """src/field.f90:1648."""
     kx, ky, kz = kmax
     return kz + ky * (2 * kz + 1) + kx * (2 * ky + 1) * (2 * kz + 1)
+
+
+class MdffBatchConfig(ctypes.Structure):
+    """Mirror of mdffgpu_batch_config (include/mdffgpu.h)."""
+
+    _fields_ = [
+        ("natm", ctypes.c_int32),
+        ("kmax", ctypes.c_int32 * 3),
+        ("A", ctypes.c_double * 9),
+        ("alphaES", ctypes.c_double),
+        ("cutlongrange", ctypes.c_double),
+        ("cutshortrange", ctypes.c_double),
+    ]
