@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="run one plain device copy of this size first (calibrates HBM "
+                         "byte counters of a profiler against a known transfer)")
     return ap.parse_args()
 
 
@@ -169,6 +172,14 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_mb:
+        nbytes = args.copy_mb * (1 << 20)
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(args, f"[bench] calibration copy of {args.copy_mb} MiB done")
+        del src, dst
     t0 = time.perf_counter()
     sys_ = C.config(args.workload)
     log(args, f"[bench] workload {sys_.name} n={sys_.natm} generated in "

@@ -691,7 +691,7 @@ void es_call(mdffgpu_batch* b, int mode, const double* q, const double* mu, cons
     da.ef = b->d_dir;
     R4* m4 = reinterpret_cast<R4*>(b->d_mu4);
     k_pack_mu<R4><<<nblk(N), MDFF_BLOCK, 0, b->st>>>(N, mu, m4);
-    k_dd_field<R, R4><<<gbl, MDFF_BLOCK, 0, b->st>>>(da, (const R*)b->d_t2, m4);
+    k_dd_field<R, R4><<<nblk((size_t)N * DD_LANES), MDFF_BLOCK, 0, b->st>>>(da, (const R*)b->d_t2, m4);
   } else {
     da.task = TASK_QQ | (b->pim ? (TASK_QD | TASK_DD) : 0);
     da.f = b->d_dir;

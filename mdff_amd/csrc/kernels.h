@@ -11,6 +11,7 @@ namespace mdff {
 constexpr int KC_MAX = 16;      // k vectors per chunk (consecutive n3)
 constexpr int RHO_TILE = 32;    // atoms per LDS tile in the structure factor
 constexpr int APPLY_TILE = 64;  // k chunks per LDS tile in the apply
+constexpr int DD_LANES = 64;    // lanes per atom of the SCF dipole field
 
 enum { ES_STATIC = 0, ES_IND = 1, ES_FINAL = 2 };
 enum { SR_LJ = 1, SR_BM = 2, SR_BMD = 4, SR_MO = 8, SR_TRUNC1 = 16, SR_TRUNC2 = 32 };
@@ -21,7 +22,7 @@ enum KernelId {
   K_PREPARE, K_NLIST, K_SR, K_DIR_STATIC, K_DIR_IND, K_DIR_FINAL, K_RHO_CHG, K_RHO_DIP,
   K_RHO_BOTH, K_RHO_RED, K_APPLY_F, K_APPLY_EFG, K_MOMENTS, K_COMBINE, K_INDUCED,
   K_RESID, K_EXTRAP, K_REDUCE, K_ADD, K_PAIRS_T2, K_DD_FIELD, K_MX_SR, K_MX_DIR, K_MX_PHASE, K_MX_RHO,
-  K_MX_RHO_RED, K_MX_APPLY, K_MX_COMBINE, K_BATCH, K_ADJ, K_COUNT
+  K_MX_RHO_RED, K_MX_APPLY, K_MX_COMBINE, K_MX_WEIGHTS, K_MX_GPREP, K_BATCH, K_ADJ, K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
 

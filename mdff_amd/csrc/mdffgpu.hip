@@ -73,6 +73,7 @@ struct mdffgpu_ctx {
   double *d_q = nullptr, *d_qzero = nullptr, *d_dip = nullptr, *d_mu = nullptr,
          *d_muind = nullptr, *d_mup = nullptr;
   double4* d_s4 = nullptr;   // fractional coordinates, one double4 per atom
+  double mx_floor = 0.0;     // SCF residual floor of the mixed path (0 = unknown)
   double4* d_mu4 = nullptr;  // packed dipoles of the SCF field (double4 or float4)
   double *d_Estat = nullptr, *d_E = nullptr, *d_efdir = nullptr, *d_efgdir = nullptr,
          *d_fdir = nullptr, *d_force = nullptr, *d_ef = nullptr, *d_efg = nullptr;
@@ -115,7 +116,8 @@ const char* const kKernelNames[K_COUNT] = {
     "k_apply64<field>", "k_apply64<efg>", "k_moments", "k_combine64", "k_induced",
     "k_scf_resid", "k_extrap", "k_reduce_rows", "k_add", "k_pairs_t2", "k_dd_field",
     "mx_shortrange", "mx_es_dir",
-    "mx_phase", "mx_rho", "mx_rho_reduce", "mx_apply", "mx_combine", "batch", "adjoint"};
+    "mx_phase", "mx_rho", "mx_rho_reduce", "mx_apply", "mx_combine", "mx_weights", "mx_gprep",
+    "batch", "adjoint"};
 
 ProfScope::ProfScope(mdffgpu_ctx* ctx, int kid) : c(ctx), id(kid), a(nullptr), b(nullptr) {
   if (!c->prof) return;
@@ -444,10 +446,10 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     if (mxf) {
       float4* m4 = reinterpret_cast<float4*>(c->d_mu4);
       k_pack_mu<float4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, m4);
-      k_dd_field<float, float4><<<gbl, MDFF_BLOCK, 0, st>>>(da, (const float*)c->d_t2, m4);
+      k_dd_field<float, float4><<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, (const float*)c->d_t2, m4);
     } else {
       k_pack_mu<double4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_mu4);
-      k_dd_field<double, double4><<<gbl, MDFF_BLOCK, 0, st>>>(da, (const double*)c->d_t2,
+      k_dd_field<double, double4><<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, (const double*)c->d_t2,
                                                               c->d_mu4);
     }
   } else {
@@ -683,11 +685,19 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
   double rmsd = 1.79769313486231570e308;
   bool upol_new = false;
   const bool mixed_prec = (P.precision == MDFFGPU_PREC_MIXED);
+  // Mixed precision: the FP64 residual of FP32 fields has a noise floor
+  // that may lie above the tolerance.  The context keeps an estimate of
+  // that floor (the best residual of the last stalled solve) and stops at
+  // 1.5x the estimate; without an estimate, or well above it, a solve is
+  // taken as stalled when two successive iterations fail to improve on the
+  // best residual while the residual is below 0.1 of the first one (the
+  // Jacobi residual need not fall monotonically; a diverging iteration
+  // stays above the first residual).
+  const double tol = mixed_prec ? std::max(P.conv_tol_ind, 1.5 * c->mx_floor) : P.conv_tol_ind;
   bool stalled = false;
   double rmsd_best = rmsd, rmsd_first = 0.0;
   int no_progress = 0;
-  while (((iscf < P.max_scf_pol_iter && rmsd > P.conv_tol_ind) || iscf < P.min_scf_pol_iter) &&
-         !stalled) {
+  while (((iscf < P.max_scf_pol_iter && rmsd > tol) || iscf < P.min_scf_pol_iter) && !stalled) {
     ++iscf;
     if (iscf != 1 || P.calc_mode == 0) {
       {
@@ -719,12 +729,6 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
     reduce(c, 2, gb, SC_RMSD);
     sync_scalars(c);
     rmsd = std::sqrt(c->h_sc[SC_RMSD] / c->h_sc[SC_RMSD + 1]);
-    // mixed precision: the FP64 residual of FP32 fields has a noise floor
-    // that may lie above the tolerance.  The floor is reached when two
-    // successive iterations fail to improve on the best residual so far
-    // (the residual of the Jacobi iteration need not fall monotonically),
-    // while the residual is well below that of the first iteration (a
-    // diverging iteration stays above it).
     if (iscf == 1) rmsd_first = rmsd;
     if (rmsd < rmsd_best) {
       rmsd_best = rmsd;
@@ -733,9 +737,10 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
       ++no_progress;
     }
     if (mixed_prec && iscf >= std::max(P.min_scf_pol_iter, 3) && no_progress >= 2 &&
-        rmsd < 0.1 * rmsd_first)
+        rmsd < 0.1 * rmsd_first && (c->mx_floor == 0.0 || rmsd < 5.0 * c->mx_floor))
       stalled = true;
   }
+  if (mixed_prec && (stalled || rmsd > tol)) c->mx_floor = rmsd_best;
   extrap(c, 4, 0, 0.0);
   if (upol_new) c->u_pol = 0.5 * c->h_sc[SC_UPOL];
   *n_scf = iscf;

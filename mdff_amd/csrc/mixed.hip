@@ -12,9 +12,9 @@
 // with the atoms as rows.  Complex products are real GEMMs over interleaved
 // (re, im) K; every operand is split into fp16 hi + lo parts and three
 // products (hi.hi + hi.lo + lo.hi) are accumulated in FP32 (22-bit
-// operands).  Structure factors are accumulated in FP32 over blocks of
-// atoms and the blocks are summed in FP64; energy, stress and g = A_k rho
-// are formed in FP64.
+// operands).  Structure factors are accumulated in FP32 over 32-atom
+// stages and compensated in FP64 across stages and atom splits; energy,
+// stress and g = A_k rho are formed in FP64.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -29,8 +29,14 @@ namespace mixed {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int RHO_ATOMS_PER_SPLIT = 1024;
+constexpr int RHO_ATOMS_PER_SPLIT = 2048;
 constexpr float DIP_SCALE = 256.0f;     // dipole weights are scaled by 2^8
+// The apply operand w g of each n1 plane is normalised to |.| <= APPLY_SCALE
+// (a power of two): most elements of a plane are far below its maximum, and
+// this keeps their fp16 lo parts above the subnormal range (fp16 normals
+// start at 2^-14; |lo| <= 2^-11 |x|).  2^12 leaves the largest hi part at
+// 4096, well inside the fp16 range.
+constexpr float APPLY_SCALE = 4096.0f;
 
 struct MixedState {
   mdffgpu_ctx* ctx;
@@ -41,7 +47,7 @@ struct MixedState {
   float2* tj1 = nullptr;  // [n1][j]   (N1 x npad)
   float2* tj2 = nullptr;  // [n2+ky][j]
   float2* tj3 = nullptr;  // [n3+kz][j]
-  float* rpart = nullptr;       // [split][row][col]  FP32 partials
+  double* rpart = nullptr;      // [split][row][col]  FP64 partials
   size_t rpart_cap = 0;
   double2* g = nullptr;         // [n1][c]
   unsigned int* gmax = nullptr; // max |g| per n1 plane, as float bits
@@ -78,7 +84,13 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_tables(int n, int npad, const d
   }
 }
 
+// hi + lo = x to 22 bits.  x is first pinned as a rounded FP32 value:
+// otherwise the compiler may fuse the producing multiply into both
+// conversions (v_fma_mix*_f16 from the exact product for lo, v_cvt_pk_f16_f32
+// of the FP32-rounded product for hi), and at an fp16 rounding tie the two
+// roundings of "hi" differ by one ulp, so hi + lo misses x by an ulp.
 __device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
+  __asm__ volatile("" : "+v"(x));
   hi = (_Float16)x;
   lo = (_Float16)(x - (float)hi);
 }
@@ -130,7 +142,7 @@ template <int NRT, int CT, int NWV>
 __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
     int npad, int N1, int N3, int C23, int C23p, int NW, const float* __restrict__ Wt,
     const float2* __restrict__ tj1, const float2* __restrict__ tj2,
-    const float2* __restrict__ tj3, int atoms_per_split, float* __restrict__ out) {
+    const float2* __restrict__ tj3, int atoms_per_split, double* __restrict__ out) {
   __shared__ half8 sA[2][2][NRT][2][64];  // [buf][kstep][rt][hi|lo][lane]
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -147,13 +159,22 @@ __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
     n2i[t] = colok[t] ? col[t] / N3 : 0;
     n3i[t] = colok[t] ? col[t] % N3 : 0;
   }
+  // FP32 MFMA accumulators of one stage, flushed into FP64 after every
+  // stage: the structure factors of small |k| are far smaller than the
+  // partial sums of a long FP32 chain (charge neutrality), so the chain is
+  // kept 32 atoms short and the sum over stages is compensated in FP64.
   floatx4 acc[CT][2][NRT];
+  double accd[CT][2][NRT][4];
 #pragma unroll
   for (int t = 0; t < CT; ++t)
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
-      for (int r = 0; r < NRT; ++r) acc[t][p][r] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int r = 0; r < NRT; ++r) {
+        acc[t][p][r] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) accd[t][p][r][e] = 0.0;
+      }
   const int j0 = blockIdx.y * atoms_per_split;
   const int j1 = min(npad, j0 + atoms_per_split);
 
@@ -209,45 +230,50 @@ __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
     }
   };
 
-  if (j0 < j1) make_a(j0, 0);
+  // B fragments of both planes from the loaded phases
+  half8 bre_h[CT], bre_l[CT], bim_h[CT], bim_l[CT];
+  auto make_b = [&]() {
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const float e2r[4] = {p2a[t].x, p2a[t].z, p2b[t].x, p2b[t].z};
+      const float e2i[4] = {p2a[t].y, p2a[t].w, p2b[t].y, p2b[t].w};
+      const float e3r[4] = {p3a[t].x, p3a[t].z, p3b[t].x, p3b[t].z};
+      const float e3i[4] = {p3a[t].y, p3a[t].w, p3b[t].y, p3b[t].w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float pr = e2r[e] * e3r[e] - e2i[e] * e3i[e];
+        float pi = e2r[e] * e3i[e] + e2i[e] * e3r[e];
+        if (!colok[t]) {
+          pr = 0.f;
+          pi = 0.f;
+        }
+        _Float16 rh, rl, ih, il;
+        split16(pr, rh, rl);
+        split16(pi, ih, il);
+        bre_h[t][2 * e] = rh;
+        bre_h[t][2 * e + 1] = -ih;
+        bre_l[t][2 * e] = rl;
+        bre_l[t][2 * e + 1] = -il;
+        bim_h[t][2 * e] = ih;
+        bim_h[t][2 * e + 1] = rh;
+        bim_l[t][2 * e] = il;
+        bim_l[t][2 * e + 1] = rl;
+      }
+    }
+  };
+
   int buf = 0;
-  load_p(j0 + 4 * kgrp);
+  if (j0 < j1) {
+    make_a(j0, 0);
+    load_p(j0 + 4 * kgrp);
+    make_b();
+    if (j0 + 16 < j1) load_p(j0 + 16 + 4 * kgrp);
+  }
   __syncthreads();
   for (int jb = j0; jb < j1; jb += RHO_STAGE) {
     if (jb + RHO_STAGE < j1) make_a(jb + RHO_STAGE, buf ^ 1);
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
-      half8 bre_h[CT], bre_l[CT], bim_h[CT], bim_l[CT];
-#pragma unroll
-      for (int t = 0; t < CT; ++t) {
-        const float e2r[4] = {p2a[t].x, p2a[t].z, p2b[t].x, p2b[t].z};
-        const float e2i[4] = {p2a[t].y, p2a[t].w, p2b[t].y, p2b[t].w};
-        const float e3r[4] = {p3a[t].x, p3a[t].z, p3b[t].x, p3b[t].z};
-        const float e3i[4] = {p3a[t].y, p3a[t].w, p3b[t].y, p3b[t].w};
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          float pr = e2r[e] * e3r[e] - e2i[e] * e3i[e];
-          float pi = e2r[e] * e3i[e] + e2i[e] * e3r[e];
-          if (!colok[t]) {
-            pr = 0.f;
-            pi = 0.f;
-          }
-          _Float16 rh, rl, ih, il;
-          split16(pr, rh, rl);
-          split16(pi, ih, il);
-          bre_h[t][2 * e] = rh;
-          bre_h[t][2 * e + 1] = -ih;
-          bre_l[t][2 * e] = rl;
-          bre_l[t][2 * e + 1] = -il;
-          bim_h[t][2 * e] = ih;
-          bim_h[t][2 * e + 1] = rh;
-          bim_l[t][2 * e] = il;
-          bim_l[t][2 * e + 1] = rl;
-        }
-      }
-      // next K step's phases in flight during the MFMAs
-      const int jn = jb + (ks + 1) * 16 + 4 * kgrp;
-      if (jn < j1) load_p(jn);
 #pragma unroll
       for (int rt = 0; rt < NRT; ++rt) {
         const half8 ah = sA[buf][ks][rt][0][lane];
@@ -263,12 +289,29 @@ __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
           acc[t][1][rt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bim_h[t], acc[t][1][rt], 0, 0, 0);
         }
       }
+      // the next K step's B fragments (phases already loaded), and the loads
+      // of the one after
+      const int jn = jb + (ks + 1) * 16;
+      if (jn < j1) {
+        make_b();
+        if (jn + 16 < j1) load_p(jn + 16 + 4 * kgrp);
+      }
     }
+#pragma unroll
+    for (int t = 0; t < CT; ++t)
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int r = 0; r < NRT; ++r) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) accd[t][p][r][e] += (double)acc[t][p][r][e];
+          acc[t][p][r] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
     __syncthreads();
     buf ^= 1;
   }
   // D layout: acc[..][rt][e] = D[rt*16 + 4*kgrp + e][col]
-  float* o = out + (size_t)blockIdx.y * (NRT * 16) * (2 * C23p);
+  double* o = out + (size_t)blockIdx.y * (NRT * 16) * (2 * C23p);
 #pragma unroll
   for (int t = 0; t < CT; ++t) {
     if (!grpok[t]) continue;
@@ -278,7 +321,7 @@ __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
       for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-          o[(size_t)(rt * 16 + 4 * kgrp + e) * (2 * C23p) + p * C23p + col[t]] = acc[t][p][rt][e];
+          o[(size_t)(rt * 16 + 4 * kgrp + e) * (2 * C23p) + p * C23p + col[t]] = accd[t][p][rt][e];
   }
 }
 
@@ -289,7 +332,7 @@ __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
 // Grid (column blocks, N1): one n1 plane per block row.
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
     int N1, int N2, int N3, int C23, int C23p, int NW, int chg, int dip, int nsplit, int NRT16,
-    const float* __restrict__ part, Cell cell, double alpha, int do_energy, double2* __restrict__ g,
+    const double* __restrict__ part, Cell cell, double alpha, int do_energy, double2* __restrict__ g,
     unsigned int* __restrict__ gmax, double* __restrict__ pout) {
   __shared__ double red[4 * 7];
   __shared__ float wmax[MDFF_BLOCK / 64];
@@ -303,10 +346,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
     const int n2 = c / N3 - ky, n3 = c % N3 - kz;
     double Mr[4] = {0, 0, 0, 0}, Mi[4] = {0, 0, 0, 0};
     const size_t sstr = (size_t)NRT16 * (2 * C23p);
-    const float* p0 = part + (size_t)n1 * (2 * C23p) + c;
+    const double* p0 = part + (size_t)n1 * (2 * C23p) + c;
     for (int w = 0; w < NW; ++w) {
-      const float* p = p0 + (size_t)w * N1 * (2 * C23p);
-      float vr[4], vi[4];
+      const double* p = p0 + (size_t)w * N1 * (2 * C23p);
+      double vr[4], vi[4];
       int sp = 0;
       for (; sp + 4 <= nsplit; sp += 4) {
 #pragma unroll
@@ -316,13 +359,13 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          Mr[w] += (double)vr[u];
-          Mi[w] += (double)vi[u];
+          Mr[w] += vr[u];
+          Mi[w] += vi[u];
         }
       }
       for (; sp < nsplit; ++sp) {
-        Mr[w] += (double)p[sp * sstr];
-        Mi[w] += (double)p[sp * sstr + C23p];
+        Mr[w] += p[sp * sstr];
+        Mi[w] += p[sp * sstr + C23p];
       }
     }
     double rR = 0.0, rI = 0.0;
@@ -384,14 +427,14 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
   }
 }
 
-// Scale exponent of apply column (w, n1): values of w(n) g(n1, n) are at
-// most 1 after multiplying by 2^-e; gmaxbits = max |g| over the n1 plane,
-// wmax = largest |w| (1, kmax2, kmax3, ...).  A power of two keeps the
-// scaling exact.
+// Scale of apply column (w, n1): values of w(n) g(n1, n) are at most
+// APPLY_SCALE after multiplying by APPLY_SCALE 2^-e; gmaxbits = max |g| over
+// the n1 plane, wmax = largest |w| (1, kmax2, kmax3, ...).  A power of two
+// keeps the scaling exact.
 __device__ __forceinline__ float class_scale(unsigned int gmaxbits, float wmax) {
   const float gm = __uint_as_float(gmaxbits) * wmax;
-  if (!(gm > 0.f)) return 1.f;
-  return exp2f(-ceilf(log2f(gm)));
+  if (!(gm > 0.f)) return APPLY_SCALE;
+  return exp2f(-ceilf(log2f(gm))) * APPLY_SCALE;
 }
 
 // weights of the apply: 0 -> 1, 1 -> n2, 2 -> n3, 3 -> n2^2, 4 -> n2 n3,
@@ -422,7 +465,7 @@ __device__ __forceinline__ float apply_wmax(int w, int ky, int kz) {
 // B fragments of the apply GEMM.  cols = (part, w, n1): col = part*(W*N1) +
 // w*N1 + n1, NCT tiles of 16; K: k' = 2c + t over C23p columns c.
 //   Re col: B[2c] = Gr, B[2c+1] = Gi;  Im col: B[2c] = Gi, B[2c+1] = -Gr
-// with G = 2^-e(w,n1) w(n) g(n).  Layout [ks][hi|lo][ct][lane] of half8, so
+// with G = APPLY_SCALE 2^-e(w,n1) w(n) g(n).  Layout [ks][hi|lo][ct][lane] of half8, so
 // that the fragments of one K step are contiguous.
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, int C23p, int W,
                                                         int NCT, const double2* __restrict__ g,
@@ -540,7 +583,30 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
       }
     }
   };
-  if (ks0 < ks1) load_e(ks0);
+  // A fragments: A[i][2c] = Re P_i[c], A[i][2c+1] = Im P_i[c]
+  auto make_frags = [&](half8 (&h8)[RT], half8 (&l8)[RT]) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int r = 0; r < RT; ++r) {
+        const float2 e2 = pe2[r][e], e3 = pe3[r][e];
+        const float pr = e2.x * e3.x - e2.y * e3.y;
+        const float pi = e2.x * e3.y + e2.y * e3.x;
+        _Float16 h, l;
+        split16(pr, h, l);
+        h8[r][2 * e] = h;
+        l8[r][2 * e] = l;
+        split16(pi, h, l);
+        h8[r][2 * e + 1] = h;
+        l8[r][2 * e + 1] = l;
+      }
+  };
+  half8 ah[RT], al[RT];
+  if (ks0 < ks1) {
+    load_e(ks0);
+    make_frags(ah, al);
+    if (ks0 + 1 < ks1) load_e(ks0 + 1);
+  }
   int cur = 0;
   for (int ks = ks0; ks < ks1; ++ks) {
     const bool more = ks + 1 < ks1;
@@ -551,24 +617,6 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
         if (idx < NB) pre[u] = bsrc[(size_t)(ks + 1) * NB + idx];
       }
     }
-    // A fragments: A[i][2c] = Re P_i[c], A[i][2c+1] = Im P_i[c]
-    half8 ah[RT], al[RT];
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-#pragma unroll
-      for (int r = 0; r < RT; ++r) {
-        const float2 e2 = pe2[r][e], e3 = pe3[r][e];
-        const float pr = e2.x * e3.x - e2.y * e3.y;
-        const float pi = e2.x * e3.y + e2.y * e3.x;
-        _Float16 h, l;
-        split16(pr, h, l);
-        ah[r][2 * e] = h;
-        al[r][2 * e] = l;
-        split16(pi, h, l);
-        ah[r][2 * e + 1] = h;
-        al[r][2 * e + 1] = l;
-      }
-    if (more) load_e(ks + 1);
     const half8* B = sB[cur];
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct) {
@@ -581,7 +629,11 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
         acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], b1, acc[r][ct], 0, 0, 0);
       }
     }
+    // the next K step's fragments (independent of the MFMAs above) and the
+    // phases of the one after
     if (more) {
+      make_frags(ah, al);
+      if (ks + 2 < ks1) load_e(ks + 2);
 #pragma unroll
       for (int u = 0; u < PER_T; ++u) {
         const int idx = threadIdx.x + u * 64 * APP_WAVES;
@@ -681,7 +733,7 @@ constexpr int APP_NCT_E = 19, APP_RT_E = 1;  // final call with EFG (W = 6)
 constexpr int APP_ROWS_MAX = APP_WAVES * 2 * 16;
 constexpr int APP_SLICES = 2;
 constexpr int RHO_NRT_MAX = 7;  // 4 weights x 25 n1 rows
-constexpr int RHO_CT = 2, RHO_NWV = 4;
+constexpr int RHO_CT = 1, RHO_NWV = 4;
 
 static bool sizes(MixedState* m) {
   CtxView v = view(m->ctx);
@@ -776,9 +828,12 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
   const int NRT = (NW * m->N1 + 15) / 16;
   const int NRTl = (NRT <= 2) ? 2 : (NRT <= 4) ? 4 : (NRT <= 5) ? 5 : 7;
   {
-    ProfScope _ps(m->ctx, K_MX_RHO);
+    ProfScope _ps(m->ctx, K_MX_WEIGHTS);
     mx_weights<<<(m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK, 0, st>>>(
         n, m->npad, chg, dip, q, mu, v.cell, m->Wt, m->gmax, m->N1);
+  }
+  {
+    ProfScope _ps(m->ctx, K_MX_RHO);
     switch (NRTl) {
       case 2: launch_rho<2>(m, v, NW); break;
       case 4: launch_rho<4>(m, v, NW); break;
@@ -803,10 +858,13 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
   const int NCTl = (NCT <= APP_NCT_F) ? APP_NCT_F : APP_NCT_E;
   const int KS = 2 * m->C23p / 32;
   {
-    ProfScope _ps(m->ctx, K_MX_APPLY);
+    ProfScope _ps(m->ctx, K_MX_GPREP);
     const size_t total = (size_t)NCTl * KS * 64;
     mx_gprep<<<(unsigned)((total + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0, st>>>(
         m->N1, m->N3, m->C23, m->C23p, W, NCTl, m->g, m->gmax, m->ky, m->kz, m->bfrag);
+  }
+  {
+    ProfScope _ps(m->ctx, K_MX_APPLY);
     if (NCTl == APP_NCT_F)
       launch_apply<APP_NCT_F, APP_RT_F>(m, v);
     else

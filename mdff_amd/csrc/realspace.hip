@@ -358,10 +358,10 @@ template __global__ void k_es_dir<ES_FINAL, float, true>(EsDirArgs);
 // ---------------------------------------------------------------------------
 // Dipole field tensor of the SCF (multipole_ES_dir with task d-d and no
 // damping, :2407-2421).  The pair geometry is fixed during the SCF, so
-// k_pairs_t2 stores for every directed pair inside the cutoff r (3) and the
-// two radial factors 3F2/d^5, F1/d^3 in type R, in the row layout of nbr
-// (5 planes); pairs outside the cutoff get zero factors.  k_dd_field applies
-// E_i = sum_j [r (r.mu_j) 3F2/d^5 - mu_j F1/d^3] from that table.
+// k_pairs_t2 stores for every directed pair inside the cutoff, in the row
+// layout of nbr, the four-vector (r, 3F2/d^5) and then, in a plane of its
+// own, F1/d^3, in type R; pairs outside the cutoff get zero factors.
+// k_dd_field applies E_i = sum_j [r (r.mu_j) 3F2/d^5 - mu_j F1/d^3].
 template <typename R, bool BATCH>
 __global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, R* __restrict__ tab) {
   const int n = a.n;
@@ -413,11 +413,14 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, R* __restr
         A1 = F1 * dm3;
       }
     }
-    tab[e] = rx;
-    tab[e + plane] = ry;
-    tab[e + 2 * plane] = rz;
-    tab[e + 3 * plane] = A2;
-    tab[e + 4 * plane] = A1;
+    R4t<R>* t4 = reinterpret_cast<R4t<R>*>(tab);
+    R4t<R> v;
+    v.x = rx;
+    v.y = ry;
+    v.z = rz;
+    v.w = A2;
+    t4[e] = v;
+    tab[4 * plane + e] = A1;
   }
 }
 
@@ -436,48 +439,47 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pack_mu(int n, const double* __r
   mu4[i] = v;
 }
 
-// Pairs of one lane are taken 4 at a time (loads in flight together) and
-// accumulated in list order.  A neighbour without a dipole contributes
-// exact zeros.
+// DD_LANES lanes per atom (a whole wave), pairs taken 2 at a time per lane
+// (loads in flight together), accumulated in list order per lane.  A
+// neighbour without a dipole contributes exact zeros.
 template <typename R, typename R4>
 __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a, const R* __restrict__ tab,
                                                          const R4* __restrict__ mu4) {
   const int n = a.n;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = tid / LPA, sub = tid % LPA;
+  const int i = tid / DD_LANES, sub = tid % DD_LANES;
   R e0 = 0, e1 = 0, e2 = 0;
   const size_t plane = (size_t)a.maxnb * n;
+  const R4* t4 = reinterpret_cast<const R4*>(tab);
+  const R* t1 = tab + 4 * plane;
   const bool act = i < n;
   const int c = act ? a.cnt[i] : 0;
   const size_t row = (size_t)(act ? i : 0) * a.maxnb;
-  constexpr int U = 4;
-  for (int k0 = sub; k0 < c; k0 += U * LPA) {
+  constexpr int U = 2;
+  for (int k0 = sub; k0 < c; k0 += U * DD_LANES) {
     int jj[U];
-    R rx[U], ry[U], rz[U], A2[U], A1[U];
-    R4 m[U];
+    R4 g[U], m[U];
+    R A1[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int k = k0 + u * LPA;
+      const int k = k0 + u * DD_LANES;
       const size_t e = row + (k < c ? k : 0);
       jj[u] = a.nbr[e];
-      rx[u] = tab[e];
-      ry[u] = tab[e + plane];
-      rz[u] = tab[e + 2 * plane];
-      A2[u] = tab[e + 3 * plane];
-      A1[u] = tab[e + 4 * plane];
+      g[u] = t4[e];
+      A1[u] = t1[e];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) m[u] = mu4[jj[u]];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (k0 + u * LPA >= c) break;
-      const R rmj = rx[u] * m[u].x + ry[u] * m[u].y + rz[u] * m[u].z;
-      e0 += rx[u] * rmj * A2[u] - m[u].x * A1[u];
-      e1 += ry[u] * rmj * A2[u] - m[u].y * A1[u];
-      e2 += rz[u] * rmj * A2[u] - m[u].z * A1[u];
+      if (k0 + u * DD_LANES >= c) break;
+      const R rmj = g[u].x * m[u].x + g[u].y * m[u].y + g[u].z * m[u].z;
+      e0 += g[u].x * rmj * g[u].w - m[u].x * A1[u];
+      e1 += g[u].y * rmj * g[u].w - m[u].y * A1[u];
+      e2 += g[u].z * rmj * g[u].w - m[u].z * A1[u];
     }
   }
-  for (int off = LPA / 2; off > 0; off >>= 1) {
+  for (int off = DD_LANES / 2; off > 0; off >>= 1) {
     e0 += __shfl_xor(e0, off, 64);
     e1 += __shfl_xor(e1, off, 64);
     e2 += __shfl_xor(e2, off, 64);

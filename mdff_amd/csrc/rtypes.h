@@ -6,6 +6,20 @@
 
 namespace mdff {
 
+// four-vector of type R (rx, ry, rz, radial factor of the SCF pair table)
+template <typename R>
+struct R4Sel;
+template <>
+struct R4Sel<double> {
+  typedef double4 type;
+};
+template <>
+struct R4Sel<float> {
+  typedef float4 type;
+};
+template <typename R>
+using R4t = typename R4Sel<R>::type;
+
 template <typename R>
 __device__ __forceinline__ R rexp(R x);
 template <>
