@@ -174,6 +174,7 @@ template <typename R, typename R4>
 __global__ void k_dd_field(EsDirArgs a, const R* tab, const R4* mu4);
 template <typename R4>
 __global__ void k_pack_mu(int n, const double* mu, R4* mu4);
+__global__ void k_dd_direct(EsDirArgs a, const float4* mu4);
 template <bool CHG, bool DIP>
 __global__ void k_rho64(RhoArgs a);
 __global__ void k_rho_reduce64(RhoRedArgs a);

@@ -446,7 +446,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     if (mxf) {
       float4* m4 = reinterpret_cast<float4*>(c->d_mu4);
       k_pack_mu<float4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, m4);
-      k_dd_field<float, float4><<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, (const float*)c->d_t2, m4);
+      k_dd_direct<<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, m4);
     } else {
       k_pack_mu<double4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_mu4);
       k_dd_field<double, double4><<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, (const double*)c->d_t2,
@@ -604,7 +604,8 @@ void build_t2(mdffgpu_ctx* c) {
   const int n = c->n;
   const bool mx = (c->P.precision == MDFFGPU_PREC_MIXED) &&
                   !(c->P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_SCF_FIELD);
-  const size_t es = mx ? sizeof(float) : sizeof(double);
+  if (mx) return;  // the mixed SCF field recomputes its pair terms (k_dd_direct)
+  const size_t es = sizeof(double);
   const size_t need = 5 * (size_t)c->maxnb * n * es;
   if (need > c->t2_cap) {
     if (c->d_t2) (void)hipFree(c->d_t2);
@@ -624,10 +625,7 @@ void build_t2(mdffgpu_ctx* c) {
   da.cut2 = c->P.cutlongrange * c->P.cutlongrange;
   da.alpha = c->P.alphaES;
   ProfScope _ps(c, K_PAIRS_T2);
-  if (mx)
-    k_pairs_t2<float><<<nblk(n * LPA), MDFF_BLOCK, 0, c->st>>>(da, (float*)c->d_t2);
-  else
-    k_pairs_t2<double><<<nblk(n * LPA), MDFF_BLOCK, 0, c->st>>>(da, (double*)c->d_t2);
+  k_pairs_t2<double><<<nblk(n * LPA), MDFF_BLOCK, 0, c->st>>>(da, (double*)c->d_t2);
   check_launch(c, "k_pairs_t2");
 }
 

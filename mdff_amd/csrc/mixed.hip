@@ -28,6 +28,7 @@ namespace mixed {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // 16 raw bytes
 
 constexpr int RHO_ATOMS_PER_SPLIT = 2048;
 constexpr float DIP_SCALE = 256.0f;     // dipole weights are scaled by 2^8
@@ -55,6 +56,7 @@ struct MixedState {
   size_t bfrag_cap = 0;
   float* X = nullptr;           // [slice][i][col]
   float* Wt = nullptr;          // [w][j] structure-factor weights
+  u32x4* afrag = nullptr;       // structure-factor A fragments of one call
   size_t X_cap = 0;
 };
 
@@ -124,6 +126,56 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_weights(int n, int npad, int ch
 }
 
 // ---------------------------------------------------------------------------
+// A fragments of the structure-factor GEMM for the whole call, in the
+// fragment order the GEMM stages through LDS: per K step g (16 atoms), row
+// tile rt, hi|lo, lane l: atoms 16 g + 4 (l / 16) + (0..3), row
+// r = rt*16 + l%16 = w*N1 + n1, values (Re, Im) of W_w[j] E1_j[n1] split
+// into fp16 hi + lo.  Rows past NW*N1 are zero.
+template <int NRT>
+__global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_afrag(int npad, int N1, int NW,
+                                                            const float* __restrict__ Wt,
+                                                            const float2* __restrict__ tj1,
+                                                            u32x4* __restrict__ afrag) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)(npad / 16) * NRT * 64;
+  if (t >= total) return;
+  const int l = (int)(t & 63);
+  const int rt = (int)((t >> 6) % NRT);
+  const int g = (int)((t >> 6) / NRT);
+  const int r = rt * 16 + (l & 15);
+  const int j = g * 16 + 4 * (l >> 4);
+  half8 h, lo;
+  if (r < NW * N1) {
+    const int w = r / N1, n1 = r % N1;
+    const float4 wv = *(const float4*)(Wt + (size_t)w * npad + j);
+    const float4 e01 = *(const float4*)(tj1 + (size_t)n1 * npad + j);
+    const float4 e23 = *(const float4*)(tj1 + (size_t)n1 * npad + j + 2);
+    const float ww[4] = {wv.x, wv.y, wv.z, wv.w};
+    const float er[4] = {e01.x, e01.z, e23.x, e23.z};
+    const float ei[4] = {e01.y, e01.w, e23.y, e23.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      _Float16 a, bb;
+      split16(ww[e] * er[e], a, bb);
+      h[2 * e] = a;
+      lo[2 * e] = bb;
+      split16(ww[e] * ei[e], a, bb);
+      h[2 * e + 1] = a;
+      lo[2 * e + 1] = bb;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      h[e] = (_Float16)0.f;
+      lo[e] = (_Float16)0.f;
+    }
+  }
+  u32x4* o = afrag + (((size_t)g * NRT + rt) * 2) * 64 + l;
+  o[0] = __builtin_bit_cast(u32x4, h);
+  o[64] = __builtin_bit_cast(u32x4, lo);
+}
+
+// ---------------------------------------------------------------------------
 // Structure-factor GEMM.  Rows r = w*N1 + n1 (weight, n1), NRT tiles;
 // columns: Re and Im planes of the (n2, n3) mesh, col = part*C23p + c;
 // K = 2 x atoms (re, im interleaved).
@@ -140,14 +192,13 @@ constexpr int RHO_STAGE = 32;  // atoms per LDS stage (two K steps)
 
 template <int NRT, int CT, int NWV>
 __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
-    int npad, int N1, int N3, int C23, int C23p, int NW, const float* __restrict__ Wt,
-    const float2* __restrict__ tj1, const float2* __restrict__ tj2,
-    const float2* __restrict__ tj3, int atoms_per_split, double* __restrict__ out) {
-  __shared__ half8 sA[2][2][NRT][2][64];  // [buf][kstep][rt][hi|lo][lane]
+    int npad, int N3, int C23, int C23p, const u32x4* __restrict__ afrag,
+    const float2* __restrict__ tj2, const float2* __restrict__ tj3, int atoms_per_split,
+    double* __restrict__ out) {
+  __shared__ u32x4 sA[2][2][NRT][2][64];  // [buf][kstep][rt][hi|lo][lane], fp16 x8
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int kgrp = lane >> 4;
-  const int NR = NW * N1;
   int col[CT], n2i[CT], n3i[CT];
   bool colok[CT], grpok[CT];
 #pragma unroll
@@ -159,10 +210,10 @@ __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
     n2i[t] = colok[t] ? col[t] / N3 : 0;
     n3i[t] = colok[t] ? col[t] % N3 : 0;
   }
-  // FP32 MFMA accumulators of one stage, flushed into FP64 after every
-  // stage: the structure factors of small |k| are far smaller than the
-  // partial sums of a long FP32 chain (charge neutrality), so the chain is
-  // kept 32 atoms short and the sum over stages is compensated in FP64.
+  // FP32 MFMA accumulators flushed into FP64 every 64 atoms: the structure
+  // factors of small |k| are far smaller than the partial sums of a long
+  // FP32 chain (charge neutrality), so the chain is kept short and the sum
+  // over chains is compensated in FP64.
   floatx4 acc[CT][2][NRT];
   double accd[CT][2][NRT][4];
 #pragma unroll
@@ -178,44 +229,28 @@ __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
   const int j0 = blockIdx.y * atoms_per_split;
   const int j1 = min(npad, j0 + atoms_per_split);
 
-  // A fragments of the stage starting at atom jb into buffer b
-  auto make_a = [&](int jb, int b) {
-    for (int slot = threadIdx.x; slot < 2 * NRT * 64; slot += 64 * NWV) {
-      const int l = slot & 63;
-      const int rt = (slot >> 6) % NRT;
-      const int ks = (slot >> 6) / NRT;
-      const int r = rt * 16 + (l & 15);
-      const int j = jb + ks * 16 + 4 * (l >> 4);
-      half8 h, lo;
-      if (r < NR) {
-        const int w = r / N1, n1 = r % N1;
-        const float4 wv = *(const float4*)(Wt + (size_t)w * npad + j);
-        const float4 e01 = *(const float4*)(tj1 + (size_t)n1 * npad + j);
-        const float4 e23 = *(const float4*)(tj1 + (size_t)n1 * npad + j + 2);
-        const float ww[4] = {wv.x, wv.y, wv.z, wv.w};
-        const float er[4] = {e01.x, e01.z, e23.x, e23.z};
-        const float ei[4] = {e01.y, e01.w, e23.y, e23.w};
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          _Float16 a, bb;
-          split16(ww[e] * er[e], a, bb);
-          h[2 * e] = a;
-          lo[2 * e] = bb;
-          split16(ww[e] * ei[e], a, bb);
-          h[2 * e + 1] = a;
-          lo[2 * e + 1] = bb;
-        }
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          h[e] = (_Float16)0.f;
-          lo[e] = (_Float16)0.f;
-        }
-      }
-      sA[b][ks][rt][0][l] = h;
-      sA[b][ks][rt][1][l] = lo;
-    }
-  };
+  // A fragments of a stage (32 atoms): a contiguous block of ABLK 16-byte
+  // words of afrag, copied through registers into LDS
+  constexpr int ABLK = 2 * NRT * 2 * 64;
+  constexpr int NTH = 64 * NWV;
+  constexpr int APER = (ABLK + NTH - 1) / NTH;
+  u32x4 apre[APER];
+#define MX_FETCH_A(jb)                                                                   \
+  {                                                                                      \
+    const u32x4* src_ = afrag + (size_t)((jb) / 16) * NRT * 2 * 64;                      \
+    _Pragma("unroll") for (int u = 0; u < APER; ++u) {                                   \
+      const int idx_ = threadIdx.x + u * NTH;                                            \
+      if (ABLK % NTH == 0 || idx_ < ABLK) apre[u] = src_[idx_];                          \
+    }                                                                                    \
+  }
+#define MX_STAGE_A(b)                                                                    \
+  {                                                                                      \
+    u32x4* dst_ = &sA[b][0][0][0][0];                                                    \
+    _Pragma("unroll") for (int u = 0; u < APER; ++u) {                                   \
+      const int idx_ = threadIdx.x + u * NTH;                                            \
+      if (ABLK % NTH == 0 || idx_ < ABLK) dst_[idx_] = apre[u];                          \
+    }                                                                                    \
+  }
   // phase products of 4 atoms for column group t
   float4 p2a[CT], p2b[CT], p3a[CT], p3b[CT];
   auto load_p = [&](int j) {
@@ -264,30 +299,68 @@ __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
 
   int buf = 0;
   if (j0 < j1) {
-    make_a(j0, 0);
+    MX_FETCH_A(j0)
+    MX_STAGE_A(0)
     load_p(j0 + 4 * kgrp);
     make_b();
     if (j0 + 16 < j1) load_p(j0 + 16 + 4 * kgrp);
   }
   __syncthreads();
-  for (int jb = j0; jb < j1; jb += RHO_STAGE) {
-    if (jb + RHO_STAGE < j1) make_a(jb + RHO_STAGE, buf ^ 1);
+  int stage = 0;
+  for (int jb = j0; jb < j1; jb += RHO_STAGE, ++stage) {
+    const bool more = jb + RHO_STAGE < j1;
+    if (more) MX_FETCH_A(jb + RHO_STAGE)
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
+      // row tiles two at a time: four independent accumulators (2 tiles x
+      // Re/Im planes) between dependent MFMAs; the next pair's A fragments
+      // are read from LDS ahead
+      u32x4 a0h = sA[buf][ks][0][0][lane], a0l = sA[buf][ks][0][1][lane];
+      u32x4 a1h = a0h, a1l = a0l;
+      if (NRT > 1) {
+        a1h = sA[buf][ks][1][0][lane];
+        a1l = sA[buf][ks][1][1][lane];
+      }
 #pragma unroll
-      for (int rt = 0; rt < NRT; ++rt) {
-        const half8 ah = sA[buf][ks][rt][0][lane];
-        const half8 al = sA[buf][ks][rt][1][lane];
+      for (int r0 = 0; r0 < NRT; r0 += 2) {
+        u32x4 n0h = a0h, n0l = a0l, n1h = a1h, n1l = a1l;
+        if (r0 + 2 < NRT) {
+          n0h = sA[buf][ks][r0 + 2][0][lane];
+          n0l = sA[buf][ks][r0 + 2][1][lane];
+        }
+        if (r0 + 3 < NRT) {
+          n1h = sA[buf][ks][r0 + 3][0][lane];
+          n1l = sA[buf][ks][r0 + 3][1][lane];
+        }
+        const bool two = r0 + 1 < NRT;
+        const half8 h0 = __builtin_bit_cast(half8, a0h), l0 = __builtin_bit_cast(half8, a0l);
+        const half8 h1 = __builtin_bit_cast(half8, a1h), l1 = __builtin_bit_cast(half8, a1l);
 #pragma unroll
         for (int t = 0; t < CT; ++t) {
           if (!grpok[t]) continue;
-          acc[t][0][rt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bre_h[t], acc[t][0][rt], 0, 0, 0);
-          acc[t][0][rt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bre_l[t], acc[t][0][rt], 0, 0, 0);
-          acc[t][0][rt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bre_h[t], acc[t][0][rt], 0, 0, 0);
-          acc[t][1][rt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bim_h[t], acc[t][1][rt], 0, 0, 0);
-          acc[t][1][rt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bim_l[t], acc[t][1][rt], 0, 0, 0);
-          acc[t][1][rt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bim_h[t], acc[t][1][rt], 0, 0, 0);
+          acc[t][0][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h0, bre_h[t], acc[t][0][r0], 0, 0, 0);
+          acc[t][1][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h0, bim_h[t], acc[t][1][r0], 0, 0, 0);
+          if (two) {
+            acc[t][0][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h1, bre_h[t], acc[t][0][r0 + 1], 0, 0, 0);
+            acc[t][1][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h1, bim_h[t], acc[t][1][r0 + 1], 0, 0, 0);
+          }
+          acc[t][0][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h0, bre_l[t], acc[t][0][r0], 0, 0, 0);
+          acc[t][1][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h0, bim_l[t], acc[t][1][r0], 0, 0, 0);
+          if (two) {
+            acc[t][0][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h1, bre_l[t], acc[t][0][r0 + 1], 0, 0, 0);
+            acc[t][1][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h1, bim_l[t], acc[t][1][r0 + 1], 0, 0, 0);
+          }
+          acc[t][0][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(l0, bre_h[t], acc[t][0][r0], 0, 0, 0);
+          acc[t][1][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(l0, bim_h[t], acc[t][1][r0], 0, 0, 0);
+          if (two) {
+            acc[t][0][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(l1, bre_h[t], acc[t][0][r0 + 1], 0, 0, 0);
+            acc[t][1][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(l1, bim_h[t], acc[t][1][r0 + 1], 0, 0, 0);
+          }
         }
+        a0h = n0h;
+        a0l = n0l;
+        a1h = n1h;
+        a1l = n1l;
       }
       // the next K step's B fragments (phases already loaded), and the loads
       // of the one after
@@ -297,16 +370,20 @@ __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
         if (jn + 16 < j1) load_p(jn + 16 + 4 * kgrp);
       }
     }
+    // FP64 flush every second stage (64 atoms) and after the last
+    if ((stage & 1) || !more) {
 #pragma unroll
-    for (int t = 0; t < CT; ++t)
+      for (int t = 0; t < CT; ++t)
 #pragma unroll
-      for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int r = 0; r < NRT; ++r) {
+          for (int r = 0; r < NRT; ++r) {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) accd[t][p][r][e] += (double)acc[t][p][r][e];
-          acc[t][p][r] = floatx4{0.f, 0.f, 0.f, 0.f};
-        }
+            for (int e = 0; e < 4; ++e) accd[t][p][r][e] += (double)acc[t][p][r][e];
+            acc[t][p][r] = floatx4{0.f, 0.f, 0.f, 0.f};
+          }
+    }
+    if (more) MX_STAGE_A(buf ^ 1)
     __syncthreads();
     buf ^= 1;
   }
@@ -324,6 +401,8 @@ __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
           o[(size_t)(rt * 16 + 4 * kgrp + e) * (2 * C23p) + p * C23p + col[t]] = accd[t][p][rt][e];
   }
 }
+#undef MX_FETCH_A
+#undef MX_STAGE_A
 
 // ---------------------------------------------------------------------------
 // Sum of the splits in FP64, rho(k) = M_q + i sum_c n_c M_c, g = A_k rho on
@@ -534,7 +613,7 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
     int npad, int N3, int C23, int KS, int S, int ksper, int mblocks,
     const float2* __restrict__ tj2, const float2* __restrict__ tj3,
     const _Float16* __restrict__ bf, float* __restrict__ Xp) {
-  __shared__ half8 sB[2][2 * NCT * 64];
+  __shared__ u32x4 sB[2][2 * NCT * 64];
   const int per = NXCD / S;
   const int xcd = blockIdx.x % NXCD;
   const int sl = xcd / per;
@@ -551,31 +630,53 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
   for (int r = 0; r < RT; ++r)
 #pragma unroll
     for (int t = 0; t < NCT; ++t) acc[r][t] = floatx4{0.f, 0.f, 0.f, 0.f};
-  constexpr int NB = 2 * NCT * 64;                       // half8 per K step
-  constexpr int PER_T = (NB + 64 * APP_WAVES - 1) / (64 * APP_WAVES);
-  const half8* bsrc = (const half8*)bf;
-  half8 pre[PER_T];
+  // B fragments of one K step: NB 16-byte words, staged through LDS
+  constexpr int NB = 2 * NCT * 64;
+  constexpr int NT = 64 * APP_WAVES;
+  constexpr int PER_FULL = NB / NT;
+  constexpr int PER_T = (NB + NT - 1) / NT;
+  const u32x4* bsrc = reinterpret_cast<const u32x4*>(bf);
+  u32x4 pre[PER_T];
+#define MX_FETCH_B(ks)                                                                   \
+  {                                                                                      \
+    _Pragma("unroll") for (int u = 0; u < PER_FULL; ++u) pre[u] =                       \
+        bsrc[(size_t)(ks) * NB + threadIdx.x + u * NT];                                  \
+    if (PER_T > PER_FULL) {                                                              \
+      const int idx_ = threadIdx.x + PER_FULL * NT;                                      \
+      pre[PER_T - 1] = idx_ < NB ? bsrc[(size_t)(ks) * NB + idx_] : u32x4{0u, 0u, 0u, 0u}; \
+    }                                                                                    \
+  }
+#define MX_STAGE_B(buf)                                                                  \
+  {                                                                                      \
+    _Pragma("unroll") for (int u = 0; u < PER_FULL; ++u) sB[buf][threadIdx.x + u * NT] = \
+        pre[u];                                                                          \
+    if (PER_T > PER_FULL) {                                                              \
+      const int idx_ = threadIdx.x + PER_FULL * NT;                                      \
+      if (idx_ < NB) sB[buf][idx_] = pre[PER_T - 1];                                     \
+    }                                                                                    \
+  }
   if (ks0 < ks1) {
-#pragma unroll
-    for (int u = 0; u < PER_T; ++u) {
-      const int idx = threadIdx.x + u * 64 * APP_WAVES;
-      if (idx < NB) sB[0][idx] = bsrc[(size_t)ks0 * NB + idx];
-    }
+    MX_FETCH_B(ks0)
+    MX_STAGE_B(0)
   }
   __syncthreads();
-  // phases E2, E3 of the K step's columns c = 16 ks + 4 kgrp + e, for the
-  // RT atoms of this lane (zero past C23)
+  // phases E2, E3 of the K step's columns c = 16 ks + 4 kgrp + e for the RT
+  // atoms of this lane; columns past C23 read column 0 and are zeroed
   float2 pe2[RT][4], pe3[RT][4];
+  unsigned okm = 0;
   auto load_e = [&](int ks) {
     const int cb = ks * 16 + 4 * kgrp;
     int n2 = cb / N3, n3 = cb - (cb / N3) * N3;
+    okm = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const bool ok = cb + e < C23;
+      okm |= ok ? (1u << e) : 0u;
+      const int m2 = ok ? n2 : 0, m3 = ok ? n3 : 0;
 #pragma unroll
       for (int r = 0; r < RT; ++r) {
-        pe2[r][e] = ok ? tj2[(size_t)n2 * npad + ibase + 16 * r] : make_float2(0.f, 0.f);
-        pe3[r][e] = ok ? tj3[(size_t)n3 * npad + ibase + 16 * r] : make_float2(0.f, 0.f);
+        pe2[r][e] = tj2[(size_t)m2 * npad + ibase + 16 * r];
+        pe3[r][e] = tj3[(size_t)m3 * npad + ibase + 16 * r];
       }
       if (++n3 == N3) {
         n3 = 0;
@@ -590,8 +691,9 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
 #pragma unroll
       for (int r = 0; r < RT; ++r) {
         const float2 e2 = pe2[r][e], e3 = pe3[r][e];
-        const float pr = e2.x * e3.x - e2.y * e3.y;
-        const float pi = e2.x * e3.y + e2.y * e3.x;
+        const bool ok = (okm >> e) & 1u;
+        const float pr = ok ? e2.x * e3.x - e2.y * e3.y : 0.f;
+        const float pi = ok ? e2.x * e3.y + e2.y * e3.x : 0.f;
         _Float16 h, l;
         split16(pr, h, l);
         h8[r][2 * e] = h;
@@ -610,35 +712,59 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
   int cur = 0;
   for (int ks = ks0; ks < ks1; ++ks) {
     const bool more = ks + 1 < ks1;
-    if (more) {
-#pragma unroll
-      for (int u = 0; u < PER_T; ++u) {
-        const int idx = threadIdx.x + u * 64 * APP_WAVES;
-        if (idx < NB) pre[u] = bsrc[(size_t)(ks + 1) * NB + idx];
-      }
+    if (more) MX_FETCH_B(ks + 1)
+    const u32x4* B = sB[cur];
+    // column tiles two at a time: four independent accumulators between
+    // dependent MFMAs; the next pair's fragments are read from LDS ahead
+    u32x4 h0 = B[lane], l0 = B[NCT * 64 + lane];
+    u32x4 h1 = u32x4{0u, 0u, 0u, 0u}, l1 = u32x4{0u, 0u, 0u, 0u};
+    if (NCT > 1) {
+      h1 = B[64 + lane];
+      l1 = B[NCT * 64 + 64 + lane];
     }
-    const half8* B = sB[cur];
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) {
-      const half8 b1 = B[ct * 64 + lane];
-      const half8 b2 = B[NCT * 64 + ct * 64 + lane];
+    for (int c0 = 0; c0 < NCT; c0 += 2) {
+      u32x4 nh0 = h0, nl0 = l0, nh1 = h1, nl1 = l1;
+      if (c0 + 2 < NCT) {
+        nh0 = B[(c0 + 2) * 64 + lane];
+        nl0 = B[NCT * 64 + (c0 + 2) * 64 + lane];
+      }
+      if (c0 + 3 < NCT) {
+        nh1 = B[(c0 + 3) * 64 + lane];
+        nl1 = B[NCT * 64 + (c0 + 3) * 64 + lane];
+      }
+      const bool two = c0 + 1 < NCT;
+      const half8 b10 = __builtin_bit_cast(half8, h0), b20 = __builtin_bit_cast(half8, l0);
+      const half8 b11 = __builtin_bit_cast(half8, h1), b21 = __builtin_bit_cast(half8, l1);
 #pragma unroll
       for (int r = 0; r < RT; ++r) {
-        acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], b1, acc[r][ct], 0, 0, 0);
-        acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], b2, acc[r][ct], 0, 0, 0);
-        acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], b1, acc[r][ct], 0, 0, 0);
+        acc[r][c0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], b10, acc[r][c0], 0, 0, 0);
+        if (two)
+          acc[r][c0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], b11, acc[r][c0 + 1], 0, 0, 0);
       }
+#pragma unroll
+      for (int r = 0; r < RT; ++r) {
+        acc[r][c0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], b20, acc[r][c0], 0, 0, 0);
+        if (two)
+          acc[r][c0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], b21, acc[r][c0 + 1], 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < RT; ++r) {
+        acc[r][c0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], b10, acc[r][c0], 0, 0, 0);
+        if (two)
+          acc[r][c0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], b11, acc[r][c0 + 1], 0, 0, 0);
+      }
+      h0 = nh0;
+      l0 = nl0;
+      h1 = nh1;
+      l1 = nl1;
     }
     // the next K step's fragments (independent of the MFMAs above) and the
     // phases of the one after
     if (more) {
       make_frags(ah, al);
       if (ks + 2 < ks1) load_e(ks + 2);
-#pragma unroll
-      for (int u = 0; u < PER_T; ++u) {
-        const int idx = threadIdx.x + u * 64 * APP_WAVES;
-        if (idx < NB) sB[cur ^ 1][idx] = pre[u];
-      }
+      MX_STAGE_B(cur ^ 1)
     }
     __syncthreads();
     cur ^= 1;
@@ -655,6 +781,8 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
       *(floatx4*)(X + (size_t)colx * npad + i0 + 16 * r) = acc[r][ct];
     }
 }
+#undef MX_FETCH_B
+#undef MX_STAGE_B
 
 // ---------------------------------------------------------------------------
 // Sum of the K slices and fold over n1 with conj(E1_i[n1]) (FP64), into the
@@ -758,12 +886,14 @@ static bool sizes(MixedState* m) {
   m->rpart_cap = (size_t)m->nsplit * RHO_NRT_MAX * 16 * 2 * m->C23p;
   mx_alloc(&m->rpart, m->rpart_cap);
   mx_alloc(&m->Wt, (size_t)4 * np);
+  mx_alloc(&m->afrag, (np / 16) * RHO_NRT_MAX * 2 * 64);
   const int KS = 2 * m->C23p / 32;
   m->bfrag_cap = (size_t)2 * APP_NCT_E * KS * 64 * 8;
   mx_alloc(&m->bfrag, m->bfrag_cap);
   m->X_cap = (size_t)APP_SLICES * np * APP_NCT_E * 16;
   mx_alloc(&m->X, m->X_cap);
-  return m->tj1 && m->tj2 && m->tj3 && m->g && m->gmax && m->rpart && m->bfrag && m->X && m->Wt;
+  return m->tj1 && m->tj2 && m->tj3 && m->g && m->gmax && m->rpart && m->bfrag && m->X && m->Wt &&
+         m->afrag;
 }
 
 MixedState* create(mdffgpu_ctx* c) {
@@ -779,7 +909,7 @@ MixedState* create(mdffgpu_ctx* c) {
 
 void destroy(MixedState* m) {
   if (!m) return;
-  void* p[] = {m->tj1, m->tj2, m->tj3, m->rpart, m->g, m->gmax, m->bfrag, m->X, m->Wt};
+  void* p[] = {m->tj1, m->tj2, m->tj3, m->rpart, m->g, m->gmax, m->bfrag, m->X, m->Wt, m->afrag};
   for (void* x : p)
     if (x) (void)hipFree(x);
   delete m;
@@ -790,12 +920,11 @@ void on_cell(MixedState* m) {
 }
 
 template <int NRT>
-static void launch_rho(MixedState* m, const CtxView& v, int NW) {
+static void launch_rho(MixedState* m, const CtxView& v) {
   const int groups = m->C23p / 16;
   dim3 grid((groups + RHO_CT * RHO_NWV - 1) / (RHO_CT * RHO_NWV), m->nsplit);
   mx_rho_gemm<NRT, RHO_CT, RHO_NWV><<<grid, 64 * RHO_NWV, 0, v.st>>>(
-      m->npad, m->N1, m->N3, m->C23, m->C23p, NW, m->Wt, m->tj1, m->tj2, m->tj3,
-      RHO_ATOMS_PER_SPLIT, m->rpart);
+      m->npad, m->N3, m->C23, m->C23p, m->afrag, m->tj2, m->tj3, RHO_ATOMS_PER_SPLIT, m->rpart);
 }
 
 template <int NCT, int RT>
@@ -831,14 +960,22 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
     ProfScope _ps(m->ctx, K_MX_WEIGHTS);
     mx_weights<<<(m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK, 0, st>>>(
         n, m->npad, chg, dip, q, mu, v.cell, m->Wt, m->gmax, m->N1);
+    const size_t at = (size_t)(m->npad / 16) * NRTl * 64;
+    const unsigned ga = (unsigned)((at + MDFF_BLOCK - 1) / MDFF_BLOCK);
+    switch (NRTl) {
+      case 2: mx_rho_afrag<2><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag); break;
+      case 4: mx_rho_afrag<4><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag); break;
+      case 5: mx_rho_afrag<5><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag); break;
+      default: mx_rho_afrag<7><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag); break;
+    }
   }
   {
     ProfScope _ps(m->ctx, K_MX_RHO);
     switch (NRTl) {
-      case 2: launch_rho<2>(m, v, NW); break;
-      case 4: launch_rho<4>(m, v, NW); break;
-      case 5: launch_rho<5>(m, v, NW); break;
-      default: launch_rho<7>(m, v, NW); break;
+      case 2: launch_rho<2>(m, v); break;
+      case 4: launch_rho<4>(m, v); break;
+      case 5: launch_rho<5>(m, v); break;
+      default: launch_rho<7>(m, v); break;
     }
   }
   const dim3 gred((m->C23 + MDFF_BLOCK - 1) / MDFF_BLOCK, m->N1);

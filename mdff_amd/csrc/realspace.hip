@@ -491,6 +491,72 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a, const R* _
   }
 }
 
+// SCF dipole field of the mixed path with the pair terms recomputed in FP32
+// from the positions (no pair table): the positions and dipoles it gathers
+// stay in L2, so only the neighbour indices stream from HBM.  Same terms as
+// k_pairs_t2 + k_dd_field (undamped d-d, src/field.f90:2407-2421); pairs
+// beyond the cutoff contribute nothing.
+__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_direct(EsDirArgs a,
+                                                         const float4* __restrict__ mu4) {
+  const int n = a.n;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = tid / DD_LANES, sub = tid % DD_LANES;
+  const bool act = i < n;
+  const int c = act ? a.cnt[i] : 0;
+  const size_t row = (size_t)(act ? i : 0) * a.maxnb;
+  const double4 si = a.s4[act ? i : 0];
+  const float alpha = (float)a.alpha, alpha2 = alpha * alpha;
+  const float c43a3 = (float)(4.0 / 3.0 * a.alpha * a.alpha * a.alpha);
+  const float cut2 = (float)a.cut2;
+  float e0 = 0.f, e1 = 0.f, e2 = 0.f;
+  constexpr int U = 2;
+  for (int k0 = sub; k0 < c; k0 += U * DD_LANES) {
+    int jj[U];
+    double4 sj[U];
+    float4 m[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * DD_LANES;
+      jj[u] = a.nbr[row + (k < c ? k : 0)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      sj[u] = a.s4[jj[u]];
+      m[u] = mu4[jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (k0 + u * DD_LANES >= c) break;
+      float rx, ry, rz;
+      const float d2 =
+          min_image_t<float>(a.cell, si.x - sj[u].x, si.y - sj[u].y, si.z - sj[u].z, rx, ry, rz);
+      if (d2 > cut2) continue;
+      const float d = sqrtf(d2);
+      const float dm1 = 1.f / d;
+      const float dm3 = dm1 * dm1 * dm1;
+      const float expon = __expf(-alpha2 * d2) * 0.56418958354775628f;  // 1/sqrt(pi)
+      const float F1 = errfc_t<float>(alpha * d) + 2.f * alpha * d * expon;
+      const float F2 = F1 + c43a3 * d2 * d * expon;
+      const float A2 = 3.f * F2 * dm3 * dm1 * dm1;
+      const float A1 = F1 * dm3;
+      const float rmj = rx * m[u].x + ry * m[u].y + rz * m[u].z;
+      e0 += rx * rmj * A2 - m[u].x * A1;
+      e1 += ry * rmj * A2 - m[u].y * A1;
+      e2 += rz * rmj * A2 - m[u].z * A1;
+    }
+  }
+  for (int off = DD_LANES / 2; off > 0; off >>= 1) {
+    e0 += __shfl_xor(e0, off, 64);
+    e1 += __shfl_xor(e1, off, 64);
+    e2 += __shfl_xor(e2, off, 64);
+  }
+  if (act && sub == 0) {
+    a.ef[i] = e0;
+    a.ef[i + n] = e1;
+    a.ef[i + 2 * n] = e2;
+  }
+}
+
 template __global__ void k_pairs_t2<double, false>(EsDirArgs, double*);
 template __global__ void k_pairs_t2<float, false>(EsDirArgs, float*);
 template __global__ void k_pairs_t2<double, true>(EsDirArgs, double*);
