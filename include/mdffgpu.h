@@ -228,6 +228,9 @@ const char* mdffgpu_last_error(const mdffgpu_ctx* ctx);
  * folded in at the next stream synchronisation of an evaluation.
  * ------------------------------------------------------------------- */
 int mdffgpu_profile_enable(mdffgpu_ctx* ctx, int32_t on); /* also resets */
+/* While enabled, bracket only the launches of one kernel id (-1: all), so
+ * that a timed run carries the events of the kernel it reports alone. */
+int mdffgpu_profile_only(mdffgpu_ctx* ctx, int32_t kernel);
 int mdffgpu_profile_read(const mdffgpu_ctx* ctx, int32_t kernel,
                          double* total_ms, int64_t* launches);
 const char* mdffgpu_kernel_name(int32_t kernel);

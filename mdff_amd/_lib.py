@@ -33,6 +33,7 @@ PROTOTYPES = {
     "mdffgpu_last_error": (ctypes.c_char_p, [_vp]),
     "mdffgpu_device_count": (ctypes.c_int32, []),
     "mdffgpu_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mdffgpu_profile_only": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "mdffgpu_profile_read": (ctypes.c_int, [_vp, ctypes.c_int32, _dp,
                                             ctypes.POINTER(ctypes.c_int64)]),
     "mdffgpu_kernel_name": (ctypes.c_char_p, [ctypes.c_int32]),

@@ -691,7 +691,16 @@ void es_call(mdffgpu_batch* b, int mode, const double* q, const double* mu, cons
     da.ef = b->d_dir;
     R4* m4 = reinterpret_cast<R4*>(b->d_mu4);
     k_pack_mu<R4><<<nblk(N), MDFF_BLOCK, 0, b->st>>>(N, mu, m4);
-    k_dd_field<R, R4><<<nblk((size_t)N * DD_LANES), MDFF_BLOCK, 0, b->st>>>(da, (const R*)b->d_t2, m4);
+    if constexpr (sizeof(R) == sizeof(double)) {
+      k_dd_field<<<nblk((size_t)N * DD_LANES), MDFF_BLOCK, 0, b->st>>>(da, (const double*)b->d_t2,
+                                                                     m4);
+    } else {
+      const size_t cap = (size_t)b->maxnb * N;
+      const float4* tab = reinterpret_cast<const float4*>(b->d_t2);
+      const int* tj = reinterpret_cast<const int*>(tab + cap);
+      k_dd_tab<<<nblk((size_t)N * 64), MDFF_BLOCK, 0, b->st>>>(N, b->maxnb, tab, tj, tj + cap, m4,
+                                                              b->d_dir);
+    }
   } else {
     da.task = TASK_QQ | (b->pim ? (TASK_QD | TASK_DD) : 0);
     da.f = b->d_dir;
@@ -1036,7 +1045,12 @@ void create_impl(const mdffgpu_params* model, int nconf, const mdffgpu_batch_con
 template <typename R>
 void build_pairs(mdffgpu_batch* b) {
   EsDirArgs da = es_args(b, b->d_q, b->d_zero);
-  const size_t need = 5 * (size_t)b->maxnb * b->N * sizeof(R);
+  // FP64: five doubles per list entry; FP32: compacted (v, A1) float4 +
+  // index per entry, then the per-atom counts
+  const size_t cap = (size_t)b->maxnb * b->N;
+  const size_t need = sizeof(R) == sizeof(double)
+                          ? 5 * cap * sizeof(double)
+                          : cap * (sizeof(float4) + sizeof(int)) + (size_t)b->N * sizeof(int);
   if (need > b->t2_cap) {
     if (b->d_t2) {
       (void)hipFree(b->d_t2);
@@ -1047,7 +1061,13 @@ void build_pairs(mdffgpu_batch* b) {
     b->allocs.push_back(b->d_t2);
     b->t2_cap = need;
   }
-  k_pairs_t2<R, true><<<nblk((size_t)b->N * LPA), MDFF_BLOCK, 0, b->st>>>(da, (R*)b->d_t2);
+  if constexpr (sizeof(R) == sizeof(double)) {
+    k_pairs_t2<true><<<nblk((size_t)b->N * LPA), MDFF_BLOCK, 0, b->st>>>(da, (double*)b->d_t2);
+  } else {
+    float4* tab = reinterpret_cast<float4*>(b->d_t2);
+    int* tj = reinterpret_cast<int*>(tab + cap);
+    k_dd_table<true><<<nblk((size_t)b->N * 64), MDFF_BLOCK, 0, b->st>>>(da, tab, tj, tj + cap);
+  }
   launch_check(b, "batch pair table");
 }
 

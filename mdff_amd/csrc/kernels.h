@@ -168,13 +168,15 @@ template <typename R, bool BATCH = false>
 __global__ void k_shortrange(SrArgs a);
 template <int MODE, typename R, bool BATCH = false>
 __global__ void k_es_dir(EsDirArgs a);
-template <typename R, bool BATCH = false>
-__global__ void k_pairs_t2(EsDirArgs a, R* tab);
-template <typename R, typename R4>
-__global__ void k_dd_field(EsDirArgs a, const R* tab, const R4* mu4);
+template <bool BATCH = false>
+__global__ void k_pairs_t2(EsDirArgs a, double* tab);
+__global__ void k_dd_field(EsDirArgs a, const double* tab, const double4* mu4);
 template <typename R4>
 __global__ void k_pack_mu(int n, const double* mu, R4* mu4);
-__global__ void k_dd_direct(EsDirArgs a, const float4* mu4);
+template <bool BATCH = false>
+__global__ void k_dd_table(EsDirArgs a, float4* tab, int* tj, int* tcnt);
+__global__ void k_dd_tab(int n, int maxnb, const float4* tab, const int* tj, const int* tcnt,
+                         const float4* mu4, double* ef);
 template <bool CHG, bool DIP>
 __global__ void k_rho64(RhoArgs a);
 __global__ void k_rho_reduce64(RhoRedArgs a);

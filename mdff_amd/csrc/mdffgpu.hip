@@ -103,6 +103,7 @@ struct mdffgpu_ctx {
   mixed::MixedState* mx = nullptr;
   // kernel instrumentation: HIP events around each launch of the own stream
   int prof = 0;
+  int prof_only = -1;  // with prof: time only this kernel id (-1: all)
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_pending;
   std::vector<hipEvent_t> ev_pool;
   double prof_ms[mdff::K_COUNT] = {0};
@@ -120,7 +121,7 @@ const char* const kKernelNames[K_COUNT] = {
     "batch", "adjoint"};
 
 ProfScope::ProfScope(mdffgpu_ctx* ctx, int kid) : c(ctx), id(kid), a(nullptr), b(nullptr) {
-  if (!c->prof) return;
+  if (!c->prof || (c->prof_only >= 0 && c->prof_only != kid)) return;
   if (c->ev_pool.size() < 2) {
     for (int k = 0; k < 2; ++k) {
       hipEvent_t e;
@@ -445,12 +446,15 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     ProfScope _ps(c, K_DD_FIELD);
     if (mxf) {
       float4* m4 = reinterpret_cast<float4*>(c->d_mu4);
+      const size_t cap = (size_t)c->maxnb * n;
+      const float4* tab = reinterpret_cast<const float4*>(c->d_t2);
+      const int* tj = reinterpret_cast<const int*>(tab + cap);
       k_pack_mu<float4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, m4);
-      k_dd_direct<<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, m4);
+      k_dd_tab<<<nblk(n * 64), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4,
+                                                             c->d_efdir);
     } else {
       k_pack_mu<double4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_mu4);
-      k_dd_field<double, double4><<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, (const double*)c->d_t2,
-                                                              c->d_mu4);
+      k_dd_field<<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, (const double*)c->d_t2, c->d_mu4);
     }
   } else {
     ProfScope _ps(c, mx ? K_MX_DIR : K_DIR_FINAL);
@@ -604,9 +608,11 @@ void build_t2(mdffgpu_ctx* c) {
   const int n = c->n;
   const bool mx = (c->P.precision == MDFFGPU_PREC_MIXED) &&
                   !(c->P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_SCF_FIELD);
-  if (mx) return;  // the mixed SCF field recomputes its pair terms (k_dd_direct)
-  const size_t es = sizeof(double);
-  const size_t need = 5 * (size_t)c->maxnb * n * es;
+  // FP64: five doubles per list entry; mixed: (v, A1) float4 + index per
+  // entry, then the per-atom counts
+  const size_t cap = (size_t)c->maxnb * n;
+  const size_t need = mx ? cap * (sizeof(float4) + sizeof(int)) + (size_t)n * sizeof(int)
+                         : 5 * cap * sizeof(double);
   if (need > c->t2_cap) {
     if (c->d_t2) (void)hipFree(c->d_t2);
     c->d_t2 = nullptr;
@@ -625,7 +631,13 @@ void build_t2(mdffgpu_ctx* c) {
   da.cut2 = c->P.cutlongrange * c->P.cutlongrange;
   da.alpha = c->P.alphaES;
   ProfScope _ps(c, K_PAIRS_T2);
-  k_pairs_t2<double><<<nblk(n * LPA), MDFF_BLOCK, 0, c->st>>>(da, (double*)c->d_t2);
+  if (mx) {
+    float4* tab = reinterpret_cast<float4*>(c->d_t2);
+    int* tj = reinterpret_cast<int*>(tab + cap);
+    k_dd_table<<<nblk(n * 64), MDFF_BLOCK, 0, c->st>>>(da, tab, tj, tj + cap);
+  } else {
+    k_pairs_t2<<<nblk(n * LPA), MDFF_BLOCK, 0, c->st>>>(da, (double*)c->d_t2);
+  }
   check_launch(c, "k_pairs_t2");
 }
 
@@ -1282,10 +1294,17 @@ const char* mdffgpu_last_error(const mdffgpu_ctx* c) {
 int mdffgpu_profile_enable(mdffgpu_ctx* c, int32_t on) {
   if (!c) return MDFFGPU_ERR_ARG;
   c->prof = on ? 1 : 0;
+  c->prof_only = -1;
   for (int k = 0; k < K_COUNT; ++k) {
     c->prof_ms[k] = 0.0;
     c->prof_n[k] = 0;
   }
+  return MDFFGPU_OK;
+}
+
+int mdffgpu_profile_only(mdffgpu_ctx* c, int32_t kernel) {
+  if (!c || kernel < -1 || kernel >= K_COUNT) return MDFFGPU_ERR_ARG;
+  c->prof_only = kernel;
   return MDFFGPU_OK;
 }
 

@@ -357,13 +357,13 @@ template __global__ void k_es_dir<ES_FINAL, float, true>(EsDirArgs);
 
 // ---------------------------------------------------------------------------
 // Dipole field tensor of the SCF (multipole_ES_dir with task d-d and no
-// damping, :2407-2421).  The pair geometry is fixed during the SCF, so
-// k_pairs_t2 stores for every directed pair inside the cutoff, in the row
-// layout of nbr, the four-vector (r, 3F2/d^5) and then, in a plane of its
-// own, F1/d^3, in type R; pairs outside the cutoff get zero factors.
-// k_dd_field applies E_i = sum_j [r (r.mu_j) 3F2/d^5 - mu_j F1/d^3].
-template <typename R, bool BATCH>
-__global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, R* __restrict__ tab) {
+// damping, :2407-2421), FP64 path.  The pair geometry is fixed during the
+// SCF, so k_pairs_t2 stores for every directed pair inside the cutoff, in the
+// row layout of nbr, the four-vector (r, 3F2/d^5) and then, in a plane of
+// its own, F1/d^3; pairs outside the cutoff get zero factors.  k_dd_field
+// applies E_i = sum_j [r (r.mu_j) 3F2/d^5 - mu_j F1/d^3].
+template <bool BATCH>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, double* __restrict__ tab) {
   const int n = a.n;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = tid / LPA, sub = tid % LPA;
@@ -371,55 +371,32 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, R* __restr
   const int cf = BATCH ? a.cfg[i] : 0;
   const Cell& cell = BATCH ? a.bc[cf].cell : a.cell;
   const double cut2 = BATCH ? a.bc[cf].cut2 : a.cut2;
-  const double alpha_ = BATCH ? a.bc[cf].alpha : a.alpha;
+  const double alpha = BATCH ? a.bc[cf].alpha : a.alpha;
+  const double alpha2 = alpha * alpha, alpha3 = alpha2 * alpha;
+  const double piroot = sqrt(MDFF_PI);
   const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
   const int c = a.cnt[i];
   const size_t plane = (size_t)a.maxnb * n;
   for (int k = sub; k < c; k += LPA) {
     const size_t e = (size_t)i * a.maxnb + k;
     const int j = a.nbr[e];
-    R rx, ry, rz, A2 = R(0), A1 = R(0);
-    if constexpr (sizeof(R) == sizeof(double)) {
-      const double alpha = alpha_, alpha2 = alpha * alpha, alpha3 = alpha2 * alpha;
-      const double piroot = sqrt(MDFF_PI);
-      const double d2 = min_image(cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
-                                  rx, ry, rz);
-      if (!(d2 > cut2)) {
-        const double d = sqrt(d2);
-        const double d3 = d2 * d;
-        const double dm1 = 1.0 / d;
-        const double dm3 = dm1 / d2;
-        const double dm5 = dm3 / d2;
-        const double expon = exp(-alpha2 * d2) / piroot;
-        const double F0 = errfc_nr(alpha * d);
-        const double F1 = F0 + 2.0 * alpha * d * expon;
-        const double F2 = F1 + 4.0 * alpha3 * d3 * expon / 3.0;
-        A2 = 3.0 * F2 * dm5;
-        A1 = F1 * dm3;
-      }
-    } else {
-      // FP32 pair arithmetic; the fractional difference is formed in FP64
-      const R alpha = R(alpha_);
-      const R d2 = min_image_t<R>(cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
-                                  rx, ry, rz);
-      if (!(d2 > R(cut2))) {
-        const R d = sqrt(d2);
-        const R dm1 = R(1) / d;
-        const R dm3 = dm1 * dm1 * dm1;
-        const R expon = rexp<R>(-alpha * alpha * d2) * R(0.56418958354775628);  // 1/sqrt(pi)
-        const R F1 = errfc_t<R>(alpha * d) + R(2) * alpha * d * expon;
-        const R F2 = F1 + R(4.0 / 3.0) * alpha * alpha * alpha * d2 * d * expon;
-        A2 = R(3) * F2 * dm3 * dm1 * dm1;
-        A1 = F1 * dm3;
-      }
+    double rx, ry, rz, A2 = 0.0, A1 = 0.0;
+    const double d2 =
+        min_image(cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z, rx, ry, rz);
+    if (!(d2 > cut2)) {
+      const double d = sqrt(d2);
+      const double d3 = d2 * d;
+      const double dm1 = 1.0 / d;
+      const double dm3 = dm1 / d2;
+      const double dm5 = dm3 / d2;
+      const double expon = exp(-alpha2 * d2) / piroot;
+      const double F0 = errfc_nr(alpha * d);
+      const double F1 = F0 + 2.0 * alpha * d * expon;
+      const double F2 = F1 + 4.0 * alpha3 * d3 * expon / 3.0;
+      A2 = 3.0 * F2 * dm5;
+      A1 = F1 * dm3;
     }
-    R4t<R>* t4 = reinterpret_cast<R4t<R>*>(tab);
-    R4t<R> v;
-    v.x = rx;
-    v.y = ry;
-    v.z = rz;
-    v.w = A2;
-    t4[e] = v;
+    reinterpret_cast<double4*>(tab)[e] = make_double4(rx, ry, rz, A2);
     tab[4 * plane + e] = A1;
   }
 }
@@ -442,9 +419,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pack_mu(int n, const double* __r
 // DD_LANES lanes per atom (a whole wave), pairs taken 2 at a time per lane
 // (loads in flight together), accumulated in list order per lane.  A
 // neighbour without a dipole contributes exact zeros.
-template <typename R, typename R4>
-__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a, const R* __restrict__ tab,
-                                                         const R4* __restrict__ mu4) {
+__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a,
+                                                         const double* __restrict__ tab,
+                                                         const double4* __restrict__ mu4) {
+  using R = double;
+  using R4 = double4;
   const int n = a.n;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = tid / DD_LANES, sub = tid % DD_LANES;
@@ -491,78 +470,128 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a, const R* _
   }
 }
 
-// SCF dipole field of the mixed path with the pair terms recomputed in FP32
-// from the positions (no pair table): the positions and dipoles it gathers
-// stay in L2, so only the neighbour indices stream from HBM.  Same terms as
-// k_pairs_t2 + k_dd_field (undamped d-d, src/field.f90:2407-2421); pairs
-// beyond the cutoff contribute nothing.
-__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_direct(EsDirArgs a,
-                                                         const float4* __restrict__ mu4) {
+// ---------------------------------------------------------------------------
+// SCF dipole field of the mixed path, from a compacted FP32 pair table.  Once
+// per evaluation k_dd_table keeps, per atom and in list order, only the list
+// entries inside the cutoff, as (v, A1) with v = r sqrt(A2), A2 = 3F2/d^5,
+// A1 = F1/d^3 (the undamped d-d terms of src/field.f90:2407-2421, formed in
+// FP32 from an FP64 fractional difference), plus the neighbour index.  Each
+// SCF iteration k_dd_tab then applies E_i = sum_j [v (v.mu_j) - A1 mu_j]:
+// 20 bytes of table and one dipole gather per pair.  One wave per atom.
+template <bool BATCH>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __restrict__ tab,
+                                                         int* __restrict__ tj,
+                                                         int* __restrict__ tcnt) {
   const int n = a.n;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = tid / DD_LANES, sub = tid % DD_LANES;
-  const bool act = i < n;
-  const int c = act ? a.cnt[i] : 0;
-  const size_t row = (size_t)(act ? i : 0) * a.maxnb;
-  const double4 si = a.s4[act ? i : 0];
-  const float alpha = (float)a.alpha, alpha2 = alpha * alpha;
-  const float c43a3 = (float)(4.0 / 3.0 * a.alpha * a.alpha * a.alpha);
-  const float cut2 = (float)a.cut2;
-  float e0 = 0.f, e1 = 0.f, e2 = 0.f;
-  constexpr int U = 2;
-  for (int k0 = sub; k0 < c; k0 += U * DD_LANES) {
-    int jj[U];
-    double4 sj[U];
-    float4 m[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = k0 + u * DD_LANES;
-      jj[u] = a.nbr[row + (k < c ? k : 0)];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      sj[u] = a.s4[jj[u]];
-      m[u] = mu4[jj[u]];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (k0 + u * DD_LANES >= c) break;
+  const int i = tid >> 6, lane = tid & 63;
+  if (i >= n) return;  // whole waves
+  const int cf = BATCH ? a.cfg[i] : 0;
+  const Cell& cell = BATCH ? a.bc[cf].cell : a.cell;
+  const float cut2 = (float)(BATCH ? a.bc[cf].cut2 : a.cut2);
+  const double alphad = BATCH ? a.bc[cf].alpha : a.alpha;
+  const float alpha = (float)alphad, alpha2 = alpha * alpha;
+  const float c43a3 = (float)(4.0 / 3.0 * alphad * alphad * alphad);
+  const double4 si = a.s4[i];
+  const int c = a.cnt[i];
+  const size_t row = (size_t)i * a.maxnb;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int base = 0;
+  for (int k0 = 0; k0 < c; k0 += 64) {
+    const int k = k0 + lane;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    int j = 0;
+    bool in = false;
+    if (k < c) {
+      j = a.nbr[row + k];
+      const double4 sj = a.s4[j];
       float rx, ry, rz;
-      const float d2 =
-          min_image_t<float>(a.cell, si.x - sj[u].x, si.y - sj[u].y, si.z - sj[u].z, rx, ry, rz);
-      if (d2 > cut2) continue;
-      const float d = sqrtf(d2);
-      const float dm1 = 1.f / d;
-      const float dm3 = dm1 * dm1 * dm1;
-      const float expon = __expf(-alpha2 * d2) * 0.56418958354775628f;  // 1/sqrt(pi)
-      const float F1 = errfc_t<float>(alpha * d) + 2.f * alpha * d * expon;
-      const float F2 = F1 + c43a3 * d2 * d * expon;
-      const float A2 = 3.f * F2 * dm3 * dm1 * dm1;
-      const float A1 = F1 * dm3;
-      const float rmj = rx * m[u].x + ry * m[u].y + rz * m[u].z;
-      e0 += rx * rmj * A2 - m[u].x * A1;
-      e1 += ry * rmj * A2 - m[u].y * A1;
-      e2 += rz * rmj * A2 - m[u].z * A1;
+      const float d2 = min_image_t<float>(cell, si.x - sj.x, si.y - sj.y, si.z - sj.z, rx, ry, rz);
+      in = !(d2 > cut2);
+      if (in) {
+        const float d = sqrtf(d2);
+        const float dm1 = 1.f / d;
+        const float dm3 = dm1 * dm1 * dm1;
+        const float expon = __expf(-alpha2 * d2) * 0.56418958354775628f;  // 1/sqrt(pi)
+        const float F1 = errfc_t<float>(alpha * d) + 2.f * alpha * d * expon;
+        const float F2 = F1 + c43a3 * d2 * d * expon;
+        const float sA2 = sqrtf(3.f * F2) * dm1 * dm1 * sqrtf(dm1);
+        v = make_float4(rx * sA2, ry * sA2, rz * sA2, F1 * dm3);
+      }
+    }
+    const unsigned long long m = __ballot(in);
+    if (in) {
+      const size_t e = row + base + __popcll(m & below);
+      tab[e] = v;
+      tj[e] = j;
+    }
+    base += __popcll(m);
+  }
+  if (lane == 0) tcnt[i] = base;
+}
+
+__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_tab(int n, int maxnb,
+                                                       const float4* __restrict__ tab,
+                                                       const int* __restrict__ tj,
+                                                       const int* __restrict__ tcnt,
+                                                       const float4* __restrict__ mu4,
+                                                       double* __restrict__ ef) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = tid >> 6, lane = tid & 63;
+  if (i >= n) return;  // whole waves
+  const int c = tcnt[i];
+  const size_t row = (size_t)i * maxnb;
+  float e0 = 0.f, e1 = 0.f, e2 = 0.f;
+  // 8 pairs per lane in flight: one pass covers 512 neighbours
+  constexpr int U = 8;
+  for (int k0 = lane; k0 < c; k0 += U * 64) {
+    float4 g[U], m[U];
+    int jj[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * 64;
+      g[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      jj[u] = i;
+      if (k < c) {
+        g[u] = tab[row + k];
+        jj[u] = tj[row + k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) m[u] = mu4[jj[u]];
+    // all gathers in flight before the first use (the scheduler would
+    // otherwise trade them for registers and serialise them)
+    static_assert(U == 8, "operand list below");
+    asm volatile(""
+                 : "+v"(m[0].x), "+v"(m[0].y), "+v"(m[0].z), "+v"(m[1].x), "+v"(m[1].y),
+                   "+v"(m[1].z), "+v"(m[2].x), "+v"(m[2].y), "+v"(m[2].z), "+v"(m[3].x),
+                   "+v"(m[3].y), "+v"(m[3].z), "+v"(m[4].x), "+v"(m[4].y), "+v"(m[4].z),
+                   "+v"(m[5].x), "+v"(m[5].y), "+v"(m[5].z), "+v"(m[6].x), "+v"(m[6].y),
+                   "+v"(m[6].z), "+v"(m[7].x), "+v"(m[7].y), "+v"(m[7].z));
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float vm = g[u].x * m[u].x + g[u].y * m[u].y + g[u].z * m[u].z;
+      e0 += g[u].x * vm - g[u].w * m[u].x;
+      e1 += g[u].y * vm - g[u].w * m[u].y;
+      e2 += g[u].z * vm - g[u].w * m[u].z;
     }
   }
-  for (int off = DD_LANES / 2; off > 0; off >>= 1) {
+  for (int off = 32; off > 0; off >>= 1) {
     e0 += __shfl_xor(e0, off, 64);
     e1 += __shfl_xor(e1, off, 64);
     e2 += __shfl_xor(e2, off, 64);
   }
-  if (act && sub == 0) {
-    a.ef[i] = e0;
-    a.ef[i + n] = e1;
-    a.ef[i + 2 * n] = e2;
+  if (lane == 0) {
+    ef[i] = e0;
+    ef[i + n] = e1;
+    ef[i + 2 * n] = e2;
   }
 }
 
-template __global__ void k_pairs_t2<double, false>(EsDirArgs, double*);
-template __global__ void k_pairs_t2<float, false>(EsDirArgs, float*);
-template __global__ void k_pairs_t2<double, true>(EsDirArgs, double*);
-template __global__ void k_pairs_t2<float, true>(EsDirArgs, float*);
-template __global__ void k_dd_field<double, double4>(EsDirArgs, const double*, const double4*);
-template __global__ void k_dd_field<float, float4>(EsDirArgs, const float*, const float4*);
+template __global__ void k_pairs_t2<false>(EsDirArgs, double*);
+template __global__ void k_pairs_t2<true>(EsDirArgs, double*);
+template __global__ void k_dd_table<false>(EsDirArgs, float4*, int*, int*);
+template __global__ void k_dd_table<true>(EsDirArgs, float4*, int*, int*);
 template __global__ void k_pack_mu<double4>(int, const double*, double4*);
 template __global__ void k_pack_mu<float4>(int, const double*, float4*);
 

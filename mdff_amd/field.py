@@ -114,9 +114,14 @@ class ForceField:
         _lib.check(st, self._h)
         return en
 
-    def profile(self, on: bool = True) -> None:
-        """Enable (and reset) the per-kernel HIP-event instrumentation."""
-        _lib.check(_lib.lib().mdffgpu_profile_enable(self._h, int(on)), self._h)
+    def profile(self, on: bool = True, only: str = None) -> None:
+        """Enable (and reset) the per-kernel HIP-event instrumentation; with
+        ``only``, bracket just the launches of that kernel."""
+        L = _lib.lib()
+        _lib.check(L.mdffgpu_profile_enable(self._h, int(on)), self._h)
+        if on and only is not None:
+            names = [L.mdffgpu_kernel_name(k).decode() for k in range(L.mdffgpu_kernel_count())]
+            _lib.check(L.mdffgpu_profile_only(self._h, names.index(only)), self._h)
 
     def profile_stats(self) -> dict:
         """{kernel name: (total ms, launches)} of the instrumented launches."""
