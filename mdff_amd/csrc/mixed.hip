@@ -42,6 +42,7 @@ constexpr float APPLY_SCALE = 4096.0f;
 struct MixedState {
   mdffgpu_ctx* ctx;
   int N1 = 0, N2 = 0, N3 = 0, C23 = 0, C23p = 0, npad = 0;
+  int CH = 0, CHp = 0;  // conjugate-paired half of the (n2, n3) columns
   int kx = 0, ky = 0, kz = 0;
   int nsplit = 0;
   // phase tables, float2 (re, im)
@@ -127,41 +128,42 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_weights(int n, int npad, int ch
 
 // ---------------------------------------------------------------------------
 // A fragments of the structure-factor GEMM for the whole call, in the
-// fragment order the GEMM stages through LDS: per K step g (16 atoms), row
-// tile rt, hi|lo, lane l: atoms 16 g + 4 (l / 16) + (0..3), row
-// r = rt*16 + l%16 = w*N1 + n1, values (Re, Im) of W_w[j] E1_j[n1] split
-// into fp16 hi + lo.  Rows past NW*N1 are zero.
-template <int NRT>
+// fragment order the GEMM stages through LDS: per K step g (32 atoms), row
+// tile rt, hi|lo, lane l: row r = rt*16 + l%16 = 2 p + x for the pair
+// p = w*N1 + n1 and x = re|im, atoms 32 g + 8 (l / 16) + (0..7), values
+// x-part of W_w[j] E1_j[n1] split into fp16 hi + lo.  Rows past 2 NW N1 are
+// zero.
+template <int NRT2>
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_afrag(int npad, int N1, int NW,
                                                             const float* __restrict__ Wt,
                                                             const float2* __restrict__ tj1,
                                                             u32x4* __restrict__ afrag) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t total = (size_t)(npad / 16) * NRT * 64;
+  const size_t total = (size_t)(npad / 32) * NRT2 * 64;
   if (t >= total) return;
   const int l = (int)(t & 63);
-  const int rt = (int)((t >> 6) % NRT);
-  const int g = (int)((t >> 6) / NRT);
+  const int rt = (int)((t >> 6) % NRT2);
+  const int g = (int)((t >> 6) / NRT2);
   const int r = rt * 16 + (l & 15);
-  const int j = g * 16 + 4 * (l >> 4);
+  const int p = r >> 1, x = r & 1;
+  const int j = g * 32 + 8 * (l >> 4);
   half8 h, lo;
-  if (r < NW * N1) {
-    const int w = r / N1, n1 = r % N1;
-    const float4 wv = *(const float4*)(Wt + (size_t)w * npad + j);
-    const float4 e01 = *(const float4*)(tj1 + (size_t)n1 * npad + j);
-    const float4 e23 = *(const float4*)(tj1 + (size_t)n1 * npad + j + 2);
-    const float ww[4] = {wv.x, wv.y, wv.z, wv.w};
-    const float er[4] = {e01.x, e01.z, e23.x, e23.z};
-    const float ei[4] = {e01.y, e01.w, e23.y, e23.w};
+  if (p < NW * N1) {
+    const int w = p / N1, n1 = p % N1;
+    const float4* wp = reinterpret_cast<const float4*>(Wt + (size_t)w * npad + j);
+    const float4* ep = reinterpret_cast<const float4*>(tj1 + (size_t)n1 * npad + j);
+    const float4 w0 = wp[0], w1 = wp[1];
+    const float ww[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      _Float16 a, bb;
-      split16(ww[e] * er[e], a, bb);
-      h[2 * e] = a;
-      lo[2 * e] = bb;
-      split16(ww[e] * ei[e], a, bb);
-      h[2 * e + 1] = a;
-      lo[2 * e + 1] = bb;
+    for (int q = 0; q < 4; ++q) {
+      const float4 e = ep[q];  // atoms j + 2q, j + 2q + 1 as (re, im, re, im)
+      _Float16 a, b;
+      split16(ww[2 * q] * (x ? e.y : e.x), a, b);
+      h[2 * q] = a;
+      lo[2 * q] = b;
+      split16(ww[2 * q + 1] * (x ? e.w : e.z), a, b);
+      h[2 * q + 1] = a;
+      lo[2 * q + 1] = b;
     }
   } else {
 #pragma unroll
@@ -170,235 +172,210 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_afrag(int npad, int N1, int
       lo[e] = (_Float16)0.f;
     }
   }
-  u32x4* o = afrag + (((size_t)g * NRT + rt) * 2) * 64 + l;
+  u32x4* o = afrag + (((size_t)g * NRT2 + rt) * 2) * 64 + l;
   o[0] = __builtin_bit_cast(u32x4, h);
   o[64] = __builtin_bit_cast(u32x4, lo);
 }
 
 // ---------------------------------------------------------------------------
-// Structure-factor GEMM.  Rows r = w*N1 + n1 (weight, n1), NRT tiles;
-// columns: Re and Im planes of the (n2, n3) mesh, col = part*C23p + c;
-// K = 2 x atoms (re, im interleaved).
-//   A[r][2j] = Re(W_w[j] E1_j[n1]),  A[r][2j+1] = Im(W_w[j] E1_j[n1])
-//   Re plane: B[2j][c] = Re P_j[c], B[2j+1][c] = -Im P_j[c]
-//   Im plane: B[2j][c] = Im P_j[c], B[2j+1][c] =  Re P_j[c]     (P = E2 E3)
-// The A fragments of a 32-atom stage are formed once per workgroup into LDS
-// (fragment order, double buffered, one barrier per stage); each wave forms
-// the B fragments of its CT column groups (16 c each, both planes from one
-// set of products) in registers, the loads of the next K step in flight.
-// Grid (column blocks of NWV*CT groups, atom splits).  Output:
-// part[split][r][col] (rows NRT*16, 2*C23p columns).
-constexpr int RHO_STAGE = 32;  // atoms per LDS stage (two K steps)
+// Structure-factor GEMM with conjugate-paired columns.  The (n2, n3) mesh is
+// closed under negation: column c' = C23 - 1 - c has the phases
+// B_c' = E2[-n2] E3[-n3] = conj(B_c).  For the half c < CH, the four real
+// products
+//   P_xy[p][c] = sum_j A_x[p][j] B_y[j][c],  A = W_w E1[n1],  x, y = re|im
+// give both structure factors of the pair,
+//   S_c  = (P_rr - P_ii) + i (P_ri + P_ir),
+//   S_c' = (P_rr + P_ii) + i (P_ir - P_ri),
+// at half the MFMA work and half the B operand of complex products over all
+// columns.  GEMM rows (p, x) as row 2p + x (NRT2 tiles), columns (c, y) as
+// two tiles (y = re, im) per 16 columns c, K = atoms (32 per MFMA step).
+// A fragments (mx_rho_afrag) are staged through LDS one K step ahead and
+// shared by the four waves; a wave owns one column tile and row tiles
+// rg*RTW .. of NRT2, and forms the B fragments of its 16 columns in
+// registers from the phase tables (the loads of the next K step in flight).
+// FP32 MFMA chains are flushed into FP64 every 128 atoms.  Blocks map to
+// (atom split, column block) XCD-aware: the blocks of one XCD take
+// consecutive indices of the split-major order, so each XCD's L2 holds the
+// A fragments and phases of about one split.  Output: part[split][p][col]
+// with S at Re plane col and Im plane C23p + col, for both c and c'.
+constexpr int NXCD = 8;
 
-template <int NRT, int CT, int NWV>
-__global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
-    int npad, int N3, int C23, int C23p, const u32x4* __restrict__ afrag,
-    const float2* __restrict__ tj2, const float2* __restrict__ tj3, int atoms_per_split,
-    double* __restrict__ out) {
-  __shared__ u32x4 sA[2][2][NRT][2][64];  // [buf][kstep][rt][hi|lo][lane], fp16 x8
+template <int NRT2, int RTW>
+__global__ __launch_bounds__(256) void mx_rho_gemm(
+    int npad, int N3, int C23, int CH, int C23p, int nrt_used, int ncb, int nblk_used,
+    int atoms_per_split, const u32x4* __restrict__ afrag, const float2* __restrict__ tj2,
+    const float2* __restrict__ tj3, double* __restrict__ out) {
+  constexpr int RG = (NRT2 + RTW - 1) / RTW;  // row groups
+  constexpr int CPB = 4 / RG;                 // column tiles per block
+  constexpr int NRTP = RG * RTW;              // row tiles incl. padding
+  static_assert(RG * CPB == 4, "four waves per block");
+  __shared__ u32x4 sA[2][NRTP][2][64];  // [buf][rt][hi|lo][lane], fp16 x8
+  const int per = gridDim.x / NXCD;
+  const int gid = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+  if (gid >= nblk_used) return;  // whole block, before any barrier
+  const int split = gid / ncb, cblk = gid % ncb;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int kgrp = lane >> 4;
-  int col[CT], n2i[CT], n3i[CT];
-  bool colok[CT], grpok[CT];
-#pragma unroll
-  for (int t = 0; t < CT; ++t) {
-    const int grp = (blockIdx.x * NWV + wave) * CT + t;
-    grpok[t] = grp * 16 < C23p;
-    col[t] = grp * 16 + (lane & 15);
-    colok[t] = col[t] < C23;
-    n2i[t] = colok[t] ? col[t] / N3 : 0;
-    n3i[t] = colok[t] ? col[t] % N3 : 0;
+  const int rg = wave % RG;
+  const int ctile = cblk * CPB + wave / RG;
+  const int rt0 = rg * RTW;
+  const bool tile_ok = ctile * 16 < CH && rt0 < nrt_used;  // wave-uniform
+  const int c = ctile * 16 + (lane & 15);
+  const bool cok = c < CH;
+  const float2* e2row = tj2 + (size_t)(cok ? c / N3 : 0) * npad + 8 * kgrp;
+  const float2* e3row = tj3 + (size_t)(cok ? c % N3 : 0) * npad + 8 * kgrp;
+  // padding row tiles (past NRT2) multiply zeros
+  if (NRTP > NRT2) {
+    constexpr int PADW = (NRTP - NRT2) * 2 * 64;
+    for (int idx = threadIdx.x; idx < 2 * PADW; idx += 256)
+      (&sA[idx / PADW][NRT2][0][0])[idx % PADW] = u32x4{0u, 0u, 0u, 0u};
   }
-  // FP32 MFMA accumulators flushed into FP64 every 64 atoms: the structure
-  // factors of small |k| are far smaller than the partial sums of a long
-  // FP32 chain (charge neutrality), so the chain is kept short and the sum
-  // over chains is compensated in FP64.
-  floatx4 acc[CT][2][NRT];
-  double accd[CT][2][NRT][4];
+  floatx4 acc[2][RTW];
+  double accd[2][RTW][4];
 #pragma unroll
-  for (int t = 0; t < CT; ++t)
+  for (int y = 0; y < 2; ++y)
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int r = 0; r < RTW; ++r) {
+      acc[y][r] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int r = 0; r < NRT; ++r) {
-        acc[t][p][r] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int e = 0; e < 4; ++e) accd[t][p][r][e] = 0.0;
-      }
-  const int j0 = blockIdx.y * atoms_per_split;
-  const int j1 = min(npad, j0 + atoms_per_split);
+      for (int e = 0; e < 4; ++e) accd[y][r][e] = 0.0;
+    }
+  const int j0 = split * atoms_per_split;
+  const int j1 = min(npad, j0 + atoms_per_split);  // j1 - j0: a multiple of 128
 
-  // A fragments of a stage (32 atoms): a contiguous block of ABLK 16-byte
-  // words of afrag, copied through registers into LDS
-  constexpr int ABLK = 2 * NRT * 2 * 64;
-  constexpr int NTH = 64 * NWV;
-  constexpr int APER = (ABLK + NTH - 1) / NTH;
+  // A fragments of one K step: ABLK contiguous 16-byte words of afrag
+  constexpr int ABLK = NRT2 * 2 * 64;
+  constexpr int APER = (ABLK + 255) / 256;
   u32x4 apre[APER];
 #define MX_FETCH_A(jb)                                                                   \
   {                                                                                      \
-    const u32x4* src_ = afrag + (size_t)((jb) / 16) * NRT * 2 * 64;                      \
+    const u32x4* src_ = afrag + (size_t)((jb) / 32) * ABLK;                              \
     _Pragma("unroll") for (int u = 0; u < APER; ++u) {                                   \
-      const int idx_ = threadIdx.x + u * NTH;                                            \
-      if (ABLK % NTH == 0 || idx_ < ABLK) apre[u] = src_[idx_];                          \
+      const int idx_ = threadIdx.x + u * 256;                                            \
+      if (ABLK % 256 == 0 || idx_ < ABLK) apre[u] = src_[idx_];                          \
     }                                                                                    \
   }
 #define MX_STAGE_A(b)                                                                    \
   {                                                                                      \
-    u32x4* dst_ = &sA[b][0][0][0][0];                                                    \
+    u32x4* dst_ = &sA[b][0][0][0];                                                       \
     _Pragma("unroll") for (int u = 0; u < APER; ++u) {                                   \
-      const int idx_ = threadIdx.x + u * NTH;                                            \
-      if (ABLK % NTH == 0 || idx_ < ABLK) dst_[idx_] = apre[u];                          \
+      const int idx_ = threadIdx.x + u * 256;                                            \
+      if (ABLK % 256 == 0 || idx_ < ABLK) dst_[idx_] = apre[u];                          \
     }                                                                                    \
   }
-  // phase products of 4 atoms for column group t
-  float4 p2a[CT], p2b[CT], p3a[CT], p3b[CT];
-  auto load_p = [&](int j) {
+  // phases of the lane's 8 atoms (j .. j+7 of the K step) for column c
+  float4 q2[4], q3[4];
+  auto load_p = [&](int jb) {
 #pragma unroll
-    for (int t = 0; t < CT; ++t) {
-      const float2* a2 = tj2 + (size_t)n2i[t] * npad + j;
-      const float2* a3 = tj3 + (size_t)n3i[t] * npad + j;
-      p2a[t] = *(const float4*)a2;
-      p2b[t] = *(const float4*)(a2 + 2);
-      p3a[t] = *(const float4*)a3;
-      p3b[t] = *(const float4*)(a3 + 2);
+    for (int u = 0; u < 4; ++u) {
+      q2[u] = *reinterpret_cast<const float4*>(e2row + jb + 2 * u);
+      q3[u] = *reinterpret_cast<const float4*>(e3row + jb + 2 * u);
     }
   };
-
-  // B fragments of both planes from the loaded phases
-  half8 bre_h[CT], bre_l[CT], bim_h[CT], bim_l[CT];
+  // B fragments: Re and Im of B = E2 E3 for the 8 atoms, hi and lo
+  half8 brh, brl, bih, bil;
   auto make_b = [&]() {
 #pragma unroll
-    for (int t = 0; t < CT; ++t) {
-      const float e2r[4] = {p2a[t].x, p2a[t].z, p2b[t].x, p2b[t].z};
-      const float e2i[4] = {p2a[t].y, p2a[t].w, p2b[t].y, p2b[t].w};
-      const float e3r[4] = {p3a[t].x, p3a[t].z, p3b[t].x, p3b[t].z};
-      const float e3i[4] = {p3a[t].y, p3a[t].w, p3b[t].y, p3b[t].w};
+    for (int u = 0; u < 4; ++u) {
+      const float e2r[2] = {q2[u].x, q2[u].z}, e2i[2] = {q2[u].y, q2[u].w};
+      const float e3r[2] = {q3[u].x, q3[u].z}, e3i[2] = {q3[u].y, q3[u].w};
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        float pr = e2r[e] * e3r[e] - e2i[e] * e3i[e];
-        float pi = e2r[e] * e3i[e] + e2i[e] * e3r[e];
-        if (!colok[t]) {
-          pr = 0.f;
-          pi = 0.f;
-        }
-        _Float16 rh, rl, ih, il;
-        split16(pr, rh, rl);
-        split16(pi, ih, il);
-        bre_h[t][2 * e] = rh;
-        bre_h[t][2 * e + 1] = -ih;
-        bre_l[t][2 * e] = rl;
-        bre_l[t][2 * e + 1] = -il;
-        bim_h[t][2 * e] = ih;
-        bim_h[t][2 * e + 1] = rh;
-        bim_l[t][2 * e] = il;
-        bim_l[t][2 * e + 1] = rl;
+      for (int v = 0; v < 2; ++v) {
+        const float br = cok ? e2r[v] * e3r[v] - e2i[v] * e3i[v] : 0.f;
+        const float bi = cok ? e2r[v] * e3i[v] + e2i[v] * e3r[v] : 0.f;
+        _Float16 a, b;
+        split16(br, a, b);
+        brh[2 * u + v] = a;
+        brl[2 * u + v] = b;
+        split16(bi, a, b);
+        bih[2 * u + v] = a;
+        bil[2 * u + v] = b;
       }
     }
   };
 
-  int buf = 0;
   if (j0 < j1) {
     MX_FETCH_A(j0)
     MX_STAGE_A(0)
-    load_p(j0 + 4 * kgrp);
+    load_p(j0);
     make_b();
-    if (j0 + 16 < j1) load_p(j0 + 16 + 4 * kgrp);
+    load_p(j0 + 32);
   }
   __syncthreads();
-  int stage = 0;
-  for (int jb = j0; jb < j1; jb += RHO_STAGE, ++stage) {
-    const bool more = jb + RHO_STAGE < j1;
-    if (more) MX_FETCH_A(jb + RHO_STAGE)
+  const floatx4 zero = floatx4{0.f, 0.f, 0.f, 0.f};
+  // 128 atoms (4 K steps) per FP32 chain, then an FP64 flush; the first K
+  // step of a chain starts from zero
+  for (int jb = j0; jb < j1; jb += 128) {
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      // row tiles two at a time: four independent accumulators (2 tiles x
-      // Re/Im planes) between dependent MFMAs; the next pair's A fragments
-      // are read from LDS ahead
-      u32x4 a0h = sA[buf][ks][0][0][lane], a0l = sA[buf][ks][0][1][lane];
-      u32x4 a1h = a0h, a1l = a0l;
-      if (NRT > 1) {
-        a1h = sA[buf][ks][1][0][lane];
-        a1l = sA[buf][ks][1][1][lane];
+    for (int ks = 0; ks < 4; ++ks) {
+      const int js = jb + 32 * ks;
+      const bool more = js + 32 < j1;
+      if (more) MX_FETCH_A(js + 32)
+      if (tile_ok) {
+        half8 ah[RTW], al[RTW];
+#pragma unroll
+        for (int r = 0; r < RTW; ++r) {
+          ah[r] = __builtin_bit_cast(half8, sA[ks & 1][rt0 + r][0][lane]);
+          al[r] = __builtin_bit_cast(half8, sA[ks & 1][rt0 + r][1][lane]);
+        }
+        // three passes (hi.hi, hi.lo, lo.hi), 2 RTW independent
+        // accumulators between dependent MFMAs
+#pragma unroll
+        for (int r = 0; r < RTW; ++r) {
+          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], brh, ks ? acc[0][r] : zero, 0, 0, 0);
+          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bih, ks ? acc[1][r] : zero, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < RTW; ++r) {
+          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], brl, acc[0][r], 0, 0, 0);
+          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bil, acc[1][r], 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < RTW; ++r) {
+          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], brh, acc[0][r], 0, 0, 0);
+          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], bih, acc[1][r], 0, 0, 0);
+        }
       }
-#pragma unroll
-      for (int r0 = 0; r0 < NRT; r0 += 2) {
-        u32x4 n0h = a0h, n0l = a0l, n1h = a1h, n1l = a1l;
-        if (r0 + 2 < NRT) {
-          n0h = sA[buf][ks][r0 + 2][0][lane];
-          n0l = sA[buf][ks][r0 + 2][1][lane];
-        }
-        if (r0 + 3 < NRT) {
-          n1h = sA[buf][ks][r0 + 3][0][lane];
-          n1l = sA[buf][ks][r0 + 3][1][lane];
-        }
-        const bool two = r0 + 1 < NRT;
-        const half8 h0 = __builtin_bit_cast(half8, a0h), l0 = __builtin_bit_cast(half8, a0l);
-        const half8 h1 = __builtin_bit_cast(half8, a1h), l1 = __builtin_bit_cast(half8, a1l);
-#pragma unroll
-        for (int t = 0; t < CT; ++t) {
-          if (!grpok[t]) continue;
-          acc[t][0][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h0, bre_h[t], acc[t][0][r0], 0, 0, 0);
-          acc[t][1][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h0, bim_h[t], acc[t][1][r0], 0, 0, 0);
-          if (two) {
-            acc[t][0][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h1, bre_h[t], acc[t][0][r0 + 1], 0, 0, 0);
-            acc[t][1][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h1, bim_h[t], acc[t][1][r0 + 1], 0, 0, 0);
-          }
-          acc[t][0][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h0, bre_l[t], acc[t][0][r0], 0, 0, 0);
-          acc[t][1][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h0, bim_l[t], acc[t][1][r0], 0, 0, 0);
-          if (two) {
-            acc[t][0][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h1, bre_l[t], acc[t][0][r0 + 1], 0, 0, 0);
-            acc[t][1][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(h1, bim_l[t], acc[t][1][r0 + 1], 0, 0, 0);
-          }
-          acc[t][0][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(l0, bre_h[t], acc[t][0][r0], 0, 0, 0);
-          acc[t][1][r0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(l0, bim_h[t], acc[t][1][r0], 0, 0, 0);
-          if (two) {
-            acc[t][0][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(l1, bre_h[t], acc[t][0][r0 + 1], 0, 0, 0);
-            acc[t][1][r0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(l1, bim_h[t], acc[t][1][r0 + 1], 0, 0, 0);
-          }
-        }
-        a0h = n0h;
-        a0l = n0l;
-        a1h = n1h;
-        a1l = n1l;
-      }
-      // the next K step's B fragments (phases already loaded), and the loads
-      // of the one after
-      const int jn = jb + (ks + 1) * 16;
-      if (jn < j1) {
+      // the next K step's B fragments (phases already loaded), the loads of
+      // the one after, and its A fragments into the other LDS buffer
+      if (more) {
         make_b();
-        if (jn + 16 < j1) load_p(jn + 16 + 4 * kgrp);
+        if (js + 64 < j1) load_p(js + 64);
+        MX_STAGE_A((ks & 1) ^ 1)
+      }
+      __syncthreads();
+    }
+    if (tile_ok) {
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int r = 0; r < RTW; ++r)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) accd[y][r][e] += (double)acc[y][r][e];
+    }
+  }
+  if (!tile_ok || !cok) return;
+  // D layout: acc[y][r][e] = P_xy of row (rt0 + r)*16 + 4 kgrp + e: e = 0, 1
+  // are (re, im) of pair p0, e = 2, 3 of pair p0 + 1
+  double* o = out + (size_t)split * (NRT2 * 8) * (2 * C23p);
+  const int cp = C23 - 1 - c;
+#pragma unroll
+  for (int r = 0; r < RTW; ++r) {
+    if (rt0 + r >= nrt_used) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int p = (rt0 + r) * 8 + 2 * kgrp + h;
+      const double Prr = accd[0][r][2 * h], Pir = accd[0][r][2 * h + 1];
+      const double Pri = accd[1][r][2 * h], Pii = accd[1][r][2 * h + 1];
+      double* op = o + (size_t)p * (2 * C23p);
+      op[c] = Prr - Pii;
+      op[C23p + c] = Pri + Pir;
+      if (cp != c) {  // the centre column (0, 0) pairs with itself
+        op[cp] = Prr + Pii;
+        op[C23p + cp] = Pir - Pri;
       }
     }
-    // FP64 flush every second stage (64 atoms) and after the last
-    if ((stage & 1) || !more) {
-#pragma unroll
-      for (int t = 0; t < CT; ++t)
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-          for (int r = 0; r < NRT; ++r) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) accd[t][p][r][e] += (double)acc[t][p][r][e];
-            acc[t][p][r] = floatx4{0.f, 0.f, 0.f, 0.f};
-          }
-    }
-    if (more) MX_STAGE_A(buf ^ 1)
-    __syncthreads();
-    buf ^= 1;
-  }
-  // D layout: acc[..][rt][e] = D[rt*16 + 4*kgrp + e][col]
-  double* o = out + (size_t)blockIdx.y * (NRT * 16) * (2 * C23p);
-#pragma unroll
-  for (int t = 0; t < CT; ++t) {
-    if (!grpok[t]) continue;
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-      for (int rt = 0; rt < NRT; ++rt)
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          o[(size_t)(rt * 16 + 4 * kgrp + e) * (2 * C23p) + p * C23p + col[t]] = accd[t][p][rt][e];
   }
 }
 #undef MX_FETCH_A
@@ -410,7 +387,7 @@ __global__ __launch_bounds__(64 * NWV) void mx_rho_gemm(
 // (src/field.f90:2742-2745, 2803-2815), and max |g|.
 // Grid (column blocks, N1): one n1 plane per block row.
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
-    int N1, int N2, int N3, int C23, int C23p, int NW, int chg, int dip, int nsplit, int NRT16,
+    int N1, int N2, int N3, int C23, int C23p, int NW, int chg, int dip, int nsplit, int split_rows,
     const double* __restrict__ part, Cell cell, double alpha, int do_energy, double2* __restrict__ g,
     unsigned int* __restrict__ gmax, double* __restrict__ pout) {
   __shared__ double red[4 * 7];
@@ -424,7 +401,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
     const int ky = (N2 - 1) / 2, kz = (N3 - 1) / 2;
     const int n2 = c / N3 - ky, n3 = c % N3 - kz;
     double Mr[4] = {0, 0, 0, 0}, Mi[4] = {0, 0, 0, 0};
-    const size_t sstr = (size_t)NRT16 * (2 * C23p);
+    const size_t sstr = (size_t)split_rows * (2 * C23p);
     const double* p0 = part + (size_t)n1 * (2 * C23p) + c;
     for (int w = 0; w < NW; ++w) {
       const double* p = p0 + (size_t)w * N1 * (2 * C23p);
@@ -541,16 +518,22 @@ __device__ __forceinline__ float apply_wmax(int w, int ky, int kz) {
 }
 
 // ---------------------------------------------------------------------------
-// B fragments of the apply GEMM.  cols = (part, w, n1): col = part*(W*N1) +
-// w*N1 + n1, NCT tiles of 16; K: k' = 2c + t over C23p columns c.
-//   Re col: B[2c] = Gr, B[2c+1] = Gi;  Im col: B[2c] = Gi, B[2c+1] = -Gr
-// with G = APPLY_SCALE 2^-e(w,n1) w(n) g(n).  Layout [ks][hi|lo][ct][lane] of half8, so
-// that the fragments of one K step are contiguous.
-__global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, int C23p, int W,
-                                                        int NCT, const double2* __restrict__ g,
+// B fragments of the apply GEMM over the conjugate-paired half of the
+// columns.  With c' = C23 - 1 - c the partner of c (n2, n3 negated),
+// P_c' = conj(P_c) for the phases P = E2 E3, so the sum over all columns of
+// conj(P_c) G_c equals the sum over c < CH of
+//   Pr (G_c + G_c') + i Pi (G_c' - G_c) = Pr U + i Pi V
+// (the self-paired centre column (0, 0), where Pi = 0, takes U = G_c).
+// cols = (part, w, n1): col = part*(W*N1) + w*N1 + n1, NCT tiles of 16;
+// K: k' = 2c + t over CHp columns c:
+//   Re col: B[2c] = Ur, B[2c+1] = -Vi;  Im col: B[2c] = Ui, B[2c+1] = Vr
+// with G = APPLY_SCALE 2^-e(w,n1) w(n) g(n).  Layout [ks][hi|lo][ct][lane] of
+// half8, so that the fragments of one K step are contiguous.
+__global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, int CH, int CHp,
+                                                        int W, int NCT, const double2* __restrict__ g,
                                                         const unsigned int* __restrict__ gmax,
                                                         int ky, int kz, _Float16* __restrict__ bf) {
-  const int KS = (2 * C23p) / 32;
+  const int KS = (2 * CHp) / 32;
   const size_t total = (size_t)NCT * KS * 64;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
@@ -574,15 +557,24 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, 
     const int kp = ks * 32 + 8 * (lane >> 4) + e;
     const int c = kp >> 1, tt = kp & 1;
     float v = 0.f;
-    if (colg < ncols && c < C23) {
+    if (colg < ncols && c < CH) {
       const int n2 = c / N3 - ky, n3 = c % N3 - kz;
       const double2 gv = g[(size_t)n1 * C23 + c];
-      const float sc = cs * apply_weight(w, n2, n3);
-      const float gr = (float)gv.x * sc, gi = (float)gv.y * sc;
+      const double sc = (double)cs * apply_weight(w, n2, n3);
+      double ur = gv.x * sc, ui = gv.y * sc, vr = -ur, vi = -ui;
+      if (2 * c + 1 != C23) {
+        // partner column: n2, n3 negated (odd weights change sign)
+        const double2 gp = g[(size_t)n1 * C23 + (C23 - 1 - c)];
+        const double scp = (double)cs * apply_weight(w, -n2, -n3);
+        ur += gp.x * scp;
+        ui += gp.y * scp;
+        vr += gp.x * scp;
+        vi += gp.y * scp;
+      }
       if (part == 0)
-        v = tt ? gi : gr;
+        v = (float)(tt ? -vi : ur);
       else
-        v = tt ? -gr : gi;
+        v = (float)(tt ? vr : ui);
     }
     _Float16 hh, ll;
     split16(v, hh, ll);
@@ -595,7 +587,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, 
 }
 
 // ---------------------------------------------------------------------------
-// Apply GEMM: rows = atoms, K = 2 C23p, NCT column tiles.  A workgroup of
+// Apply GEMM: rows = atoms, K = 2 CHp (the conjugate-paired half of the
+// columns, operand of mx_gprep), NCT column tiles.  A workgroup of
 // APP_WAVES waves owns APP_WAVES*RT*16 atoms and one of S slices of K; wave
 // w owns row tiles RT*w .. RT*w+RT-1 and all column tiles.  A fragments
 // (A[i][2c] = Re P_i[c], A[i][2c+1] = Im P_i[c], P = E2 E3) are formed in
@@ -606,11 +599,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, 
 // XCD's L2 holds only its slice of the B fragments.
 // Output Xp[s][col][i] (FP32, NCT*16 columns, atoms contiguous).
 constexpr int APP_WAVES = 4;
-constexpr int NXCD = 8;
 
 template <int NCT, int RT>
 __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
-    int npad, int N3, int C23, int KS, int S, int ksper, int mblocks,
+    int npad, int N3, int CH, int KS, int S, int ksper, int mblocks,
     const float2* __restrict__ tj2, const float2* __restrict__ tj3,
     const _Float16* __restrict__ bf, float* __restrict__ Xp) {
   __shared__ u32x4 sB[2][2 * NCT * 64];
@@ -661,7 +653,7 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
   }
   __syncthreads();
   // phases E2, E3 of the K step's columns c = 16 ks + 4 kgrp + e for the RT
-  // atoms of this lane; columns past C23 read column 0 and are zeroed
+  // atoms of this lane; columns past CH read column 0 and are zeroed
   float2 pe2[RT][4], pe3[RT][4];
   unsigned okm = 0;
   auto load_e = [&](int ks) {
@@ -670,7 +662,7 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
     okm = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const bool ok = cb + e < C23;
+      const bool ok = cb + e < CH;
       okm |= ok ? (1u << e) : 0u;
       const int m2 = ok ? n2 : 0, m3 = ok ? n3 : 0;
 #pragma unroll
@@ -860,8 +852,9 @@ constexpr int APP_NCT_F = 10, APP_RT_F = 2;  // field calls (W = 3)
 constexpr int APP_NCT_E = 19, APP_RT_E = 1;  // final call with EFG (W = 6)
 constexpr int APP_ROWS_MAX = APP_WAVES * 2 * 16;
 constexpr int APP_SLICES = 2;
-constexpr int RHO_NRT_MAX = 7;  // 4 weights x 25 n1 rows
-constexpr int RHO_CT = 1, RHO_NWV = 4;
+// structure-factor GEMM shapes (row tiles of (pair, re|im) rows, row tiles
+// per wave): charges only, dipoles only, charges and dipoles, N1 <= 25
+constexpr int RHO_NRT2_MAX = 13;  // 2 x 4 weights x 25 n1 rows
 
 static bool sizes(MixedState* m) {
   CtxView v = view(m->ctx);
@@ -872,22 +865,24 @@ static bool sizes(MixedState* m) {
   m->N2 = 2 * m->ky + 1;
   m->N3 = 2 * m->kz + 1;
   m->C23 = m->N2 * m->N3;
-  m->C23p = (m->C23 + 63) / 64 * 64;  // whole column blocks of the rho GEMM
+  m->C23p = (m->C23 + 63) / 64 * 64;  // row stride of the structure-factor partials
+  m->CH = (m->C23 + 1) / 2;
+  m->CHp = (m->CH + 15) / 16 * 16;
   // whole row blocks of the apply GEMM (tables are zero past n)
   m->npad = (v.n + APP_ROWS_MAX - 1) / APP_ROWS_MAX * APP_ROWS_MAX;
   m->nsplit = (m->npad + RHO_ATOMS_PER_SPLIT - 1) / RHO_ATOMS_PER_SPLIT;
-  if (m->N1 > 25) return false;  // 4 N1 rows fit the 7 row tiles of mx_rho_gemm
+  if (m->N1 > 25) return false;  // 8 N1 rows fit the 13 row tiles of mx_rho_gemm
   const size_t np = m->npad;
   mx_alloc(&m->tj1, (size_t)m->N1 * np);
   mx_alloc(&m->tj2, (size_t)m->N2 * np);
   mx_alloc(&m->tj3, (size_t)m->N3 * np);
   mx_alloc(&m->g, (size_t)m->N1 * m->C23);
   mx_alloc(&m->gmax, 32);
-  m->rpart_cap = (size_t)m->nsplit * RHO_NRT_MAX * 16 * 2 * m->C23p;
+  m->rpart_cap = (size_t)m->nsplit * RHO_NRT2_MAX * 8 * 2 * m->C23p;
   mx_alloc(&m->rpart, m->rpart_cap);
   mx_alloc(&m->Wt, (size_t)4 * np);
-  mx_alloc(&m->afrag, (np / 16) * RHO_NRT_MAX * 2 * 64);
-  const int KS = 2 * m->C23p / 32;
+  mx_alloc(&m->afrag, (np / 32) * RHO_NRT2_MAX * 2 * 64);
+  const int KS = 2 * m->CHp / 32;
   m->bfrag_cap = (size_t)2 * APP_NCT_E * KS * 64 * 8;
   mx_alloc(&m->bfrag, m->bfrag_cap);
   m->X_cap = (size_t)APP_SLICES * np * APP_NCT_E * 16;
@@ -919,24 +914,28 @@ void on_cell(MixedState* m) {
   if (view(m->ctx).P->lcoulomb) sizes(m);
 }
 
-template <int NRT>
-static void launch_rho(MixedState* m, const CtxView& v) {
-  const int groups = m->C23p / 16;
-  dim3 grid((groups + RHO_CT * RHO_NWV - 1) / (RHO_CT * RHO_NWV), m->nsplit);
-  mx_rho_gemm<NRT, RHO_CT, RHO_NWV><<<grid, 64 * RHO_NWV, 0, v.st>>>(
-      m->npad, m->N3, m->C23, m->C23p, m->afrag, m->tj2, m->tj3, RHO_ATOMS_PER_SPLIT, m->rpart);
+template <int NRT2, int RTW>
+static void launch_rho(MixedState* m, const CtxView& v, int nrt_used) {
+  constexpr int CPB = 4 / ((NRT2 + RTW - 1) / RTW);
+  const int ctiles = (m->CH + 15) / 16;
+  const int ncb = (ctiles + CPB - 1) / CPB;
+  const int used = ncb * m->nsplit;
+  const int grid = (used + NXCD - 1) / NXCD * NXCD;
+  mx_rho_gemm<NRT2, RTW><<<grid, 256, 0, v.st>>>(m->npad, m->N3, m->C23, m->CH, m->C23p, nrt_used,
+                                                 ncb, used, RHO_ATOMS_PER_SPLIT, m->afrag, m->tj2,
+                                                 m->tj3, m->rpart);
 }
 
 template <int NCT, int RT>
 static void launch_apply(MixedState* m, const CtxView& v) {
-  const int KS = 2 * m->C23p / 32;
+  const int KS = 2 * m->CHp / 32;
   const int S = APP_SLICES;
   const int ksper = (KS + S - 1) / S;
   const int rows = APP_WAVES * RT * 16;
   const int mblocks = m->npad / rows;
   const int per = NXCD / S;
   const int grid = (mblocks + per - 1) / per * NXCD;
-  mx_apply_gemm<NCT, RT><<<grid, 64 * APP_WAVES, 0, v.st>>>(m->npad, m->N3, m->C23, KS, S, ksper,
+  mx_apply_gemm<NCT, RT><<<grid, 64 * APP_WAVES, 0, v.st>>>(m->npad, m->N3, m->CH, KS, S, ksper,
                                                            mblocks, m->tj2, m->tj3, m->bfrag,
                                                            m->X);
 }
@@ -954,36 +953,36 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
         n, m->npad, v.s, m->kx, m->ky, m->kz, m->tj1, m->tj2, m->tj3);
   }
   const int NW = (chg ? 1 : 0) + (dip ? 3 : 0);
-  const int NRT = (NW * m->N1 + 15) / 16;
-  const int NRTl = (NRT <= 2) ? 2 : (NRT <= 4) ? 4 : (NRT <= 5) ? 5 : 7;
+  const int NRT2 = (2 * NW * m->N1 + 15) / 16;  // row tiles with data
+  const int NRT2l = (NRT2 <= 4) ? 4 : (NRT2 <= 10) ? 10 : 13;
   {
     ProfScope _ps(m->ctx, K_MX_WEIGHTS);
     mx_weights<<<(m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK, 0, st>>>(
         n, m->npad, chg, dip, q, mu, v.cell, m->Wt, m->gmax, m->N1);
-    const size_t at = (size_t)(m->npad / 16) * NRTl * 64;
+    const size_t at = (size_t)(m->npad / 32) * NRT2l * 64;
     const unsigned ga = (unsigned)((at + MDFF_BLOCK - 1) / MDFF_BLOCK);
-    switch (NRTl) {
-      case 2: mx_rho_afrag<2><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag); break;
-      case 4: mx_rho_afrag<4><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag); break;
-      case 5: mx_rho_afrag<5><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag); break;
-      default: mx_rho_afrag<7><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag); break;
-    }
+    if (NRT2l == 4)
+      mx_rho_afrag<4><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag);
+    else if (NRT2l == 10)
+      mx_rho_afrag<10><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag);
+    else
+      mx_rho_afrag<13><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag);
   }
   {
     ProfScope _ps(m->ctx, K_MX_RHO);
-    switch (NRTl) {
-      case 2: launch_rho<2>(m, v); break;
-      case 4: launch_rho<4>(m, v); break;
-      case 5: launch_rho<5>(m, v); break;
-      default: launch_rho<7>(m, v); break;
-    }
+    if (NRT2l == 4)
+      launch_rho<4, 4>(m, v, NRT2);
+    else if (NRT2l == 10)
+      launch_rho<10, 5>(m, v, NRT2);
+    else
+      launch_rho<13, 7>(m, v, NRT2);
   }
   const dim3 gred((m->C23 + MDFF_BLOCK - 1) / MDFF_BLOCK, m->N1);
   const int gk = gred.x * gred.y;
   {
     ProfScope _ps(m->ctx, K_MX_RHO_RED);
     mx_rho_reduce<<<gred, MDFF_BLOCK, 0, st>>>(m->N1, m->N2, m->N3, m->C23, m->C23p, NW, chg, dip,
-                                             m->nsplit, NRTl * 16, m->rpart, v.cell,
+                                             m->nsplit, NRT2l * 8, m->rpart, v.cell,
                                              v.P->alphaES, mode == ES_FINAL, m->g, m->gmax,
                                              v.part);
   }
@@ -993,12 +992,12 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
   const int ncols = 2 * W * m->N1;
   const int NCT = (ncols + 15) / 16;
   const int NCTl = (NCT <= APP_NCT_F) ? APP_NCT_F : APP_NCT_E;
-  const int KS = 2 * m->C23p / 32;
+  const int KS = 2 * m->CHp / 32;
   {
     ProfScope _ps(m->ctx, K_MX_GPREP);
     const size_t total = (size_t)NCTl * KS * 64;
     mx_gprep<<<(unsigned)((total + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0, st>>>(
-        m->N1, m->N3, m->C23, m->C23p, W, NCTl, m->g, m->gmax, m->ky, m->kz, m->bfrag);
+        m->N1, m->N3, m->C23, m->CH, m->CHp, W, NCTl, m->g, m->gmax, m->ky, m->kz, m->bfrag);
   }
   {
     ProfScope _ps(m->ctx, K_MX_APPLY);
