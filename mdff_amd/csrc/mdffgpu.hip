@@ -103,6 +103,7 @@ struct mdffgpu_ctx {
   mixed::MixedState* mx = nullptr;
   // kernel instrumentation: HIP events around each launch of the own stream
   int prof = 0;
+  long long evals = 0;  // engforce evaluations run (positions may change between)
   int prof_only = -1;  // with prof: time only this kernel id (-1: all)
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_pending;
   std::vector<hipEvent_t> ev_pool;
@@ -770,6 +771,7 @@ void sym9(const double* t6, double scale, double* out) {
 // engforce_driver on the device buffers of ctx; d_pos and d_force are
 // loaded by the caller.
 void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
+  ++c->evals;
   const mdffgpu_params& P = c->P;
   const int n = c->n;
   const int gb = nblk(n);
@@ -1119,6 +1121,7 @@ CtxView view(mdffgpu_ctx* c) {
   v.Ak = c->d_Ak;
   v.kcoe = c->d_kcoe;
   v.kvec = c->d_kvec;
+  v.evals = c->evals;
   v.pos = c->d_pos;
   v.s = c->d_s;
   v.eph = c->d_eph;

@@ -21,6 +21,7 @@ struct CtxView {
   const double* Ak;
   const double* kcoe;
   const double* kvec;
+  long long evals;  // evaluation counter: phase tables are per evaluation
   const double* pos;
   const double* s;
   const double2* eph;

@@ -58,6 +58,8 @@ struct MixedState {
   float* X = nullptr;           // [slice][i][col]
   float* Wt = nullptr;          // [w][j] structure-factor weights
   u32x4* afrag = nullptr;       // structure-factor A fragments of one call
+  u32x4* btab = nullptr;        // structure-factor B fragments of one evaluation
+  long long tables_eval = -1;   // evaluation the phase tables belong to
   size_t X_cap = 0;
 };
 
@@ -178,6 +180,60 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_afrag(int npad, int N1, int
 }
 
 // ---------------------------------------------------------------------------
+// B fragments of the structure-factor GEMM for a whole evaluation (the
+// positions, hence the phases, are fixed during it): per K step g (32
+// atoms), column tile ct (16 columns c < CH), fragment f (Re hi, Re lo,
+// Im hi, Im lo), lane l: column c = ct*16 + l%16, atoms 32 g + 8 (l / 16) +
+// (0..7), values of B = E2[n2] E3[n3] split into fp16 hi + lo; zero past CH.
+__global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_btab(int npad, int N3, int CH, int CTt,
+                                                           const float2* __restrict__ tj2,
+                                                           const float2* __restrict__ tj3,
+                                                           u32x4* __restrict__ btab) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)(npad / 32) * CTt * 64;
+  if (t >= total) return;
+  const int l = (int)(t & 63);
+  const int ct = (int)((t >> 6) % CTt);
+  const int g = (int)((t >> 6) / CTt);
+  const int c = ct * 16 + (l & 15);
+  const int j = g * 32 + 8 * (l >> 4);
+  half8 rh, rl, ih, il;
+  if (c < CH) {
+    const float4* e2 = reinterpret_cast<const float4*>(tj2 + (size_t)(c / N3) * npad + j);
+    const float4* e3 = reinterpret_cast<const float4*>(tj3 + (size_t)(c % N3) * npad + j);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float4 a = e2[u], b = e3[u];  // atoms j + 2u, j + 2u + 1 as (re, im, re, im)
+      const float ar[2] = {a.x, a.z}, ai[2] = {a.y, a.w};
+      const float br[2] = {b.x, b.z}, bi[2] = {b.y, b.w};
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        _Float16 h, lo;
+        split16(ar[v] * br[v] - ai[v] * bi[v], h, lo);
+        rh[2 * u + v] = h;
+        rl[2 * u + v] = lo;
+        split16(ar[v] * bi[v] + ai[v] * br[v], h, lo);
+        ih[2 * u + v] = h;
+        il[2 * u + v] = lo;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      rh[e] = (_Float16)0.f;
+      rl[e] = (_Float16)0.f;
+      ih[e] = (_Float16)0.f;
+      il[e] = (_Float16)0.f;
+    }
+  }
+  u32x4* o = btab + (((size_t)g * CTt + ct) * 4) * 64 + l;
+  o[0] = __builtin_bit_cast(u32x4, rh);
+  o[64] = __builtin_bit_cast(u32x4, rl);
+  o[128] = __builtin_bit_cast(u32x4, ih);
+  o[192] = __builtin_bit_cast(u32x4, il);
+}
+
+// ---------------------------------------------------------------------------
 // Structure-factor GEMM with conjugate-paired columns.  The (n2, n3) mesh is
 // closed under negation: column c' = C23 - 1 - c has the phases
 // B_c' = E2[-n2] E3[-n3] = conj(B_c).  For the half c < CH, the four real
@@ -191,9 +247,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_afrag(int npad, int N1, int
 // two tiles (y = re, im) per 16 columns c, K = atoms (32 per MFMA step).
 // A fragments (mx_rho_afrag) are staged through LDS one K step ahead and
 // shared by the four waves; a wave owns one column tile and row tiles
-// rg*RTW .. of NRT2, and forms the B fragments of its 16 columns in
-// registers from the phase tables (the loads of the next K step in flight).
-// FP32 MFMA chains are flushed into FP64 every 128 atoms.  Blocks map to
+// rg*RTW .. of NRT2, and reads the B fragments of its 16 columns from the
+// per-evaluation table of mx_rho_btab, four K steps ahead.  FP32 MFMA chains
+// are flushed into FP64 every 128 atoms.  Blocks map to
 // (atom split, column block) XCD-aware: the blocks of one XCD take
 // consecutive indices of the split-major order, so each XCD's L2 holds the
 // A fragments and phases of about one split.  Output: part[split][p][col]
@@ -201,10 +257,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_afrag(int npad, int N1, int
 constexpr int NXCD = 8;
 
 template <int NRT2, int RTW>
-__global__ __launch_bounds__(256) void mx_rho_gemm(
-    int npad, int N3, int C23, int CH, int C23p, int nrt_used, int ncb, int nblk_used,
-    int atoms_per_split, const u32x4* __restrict__ afrag, const float2* __restrict__ tj2,
-    const float2* __restrict__ tj3, double* __restrict__ out) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW <= 5 ? 2 : 1))) void mx_rho_gemm(
+    int npad, int C23, int CH, int C23p, int nrt_used, int ncb, int nblk_used, int atoms_per_split,
+    const u32x4* __restrict__ afrag, const u32x4* __restrict__ btab, double* __restrict__ out) {
   constexpr int RG = (NRT2 + RTW - 1) / RTW;  // row groups
   constexpr int CPB = 4 / RG;                 // column tiles per block
   constexpr int NRTP = RG * RTW;              // row tiles incl. padding
@@ -223,8 +278,7 @@ __global__ __launch_bounds__(256) void mx_rho_gemm(
   const bool tile_ok = ctile * 16 < CH && rt0 < nrt_used;  // wave-uniform
   const int c = ctile * 16 + (lane & 15);
   const bool cok = c < CH;
-  const float2* e2row = tj2 + (size_t)(cok ? c / N3 : 0) * npad + 8 * kgrp;
-  const float2* e3row = tj3 + (size_t)(cok ? c % N3 : 0) * npad + 8 * kgrp;
+  const int CTt = (CH + 15) / 16;
   // padding row tiles (past NRT2) multiply zeros
   if (NRTP > NRT2) {
     constexpr int PADW = (NRTP - NRT2) * 2 * 64;
@@ -264,43 +318,28 @@ __global__ __launch_bounds__(256) void mx_rho_gemm(
       if (ABLK % 256 == 0 || idx_ < ABLK) dst_[idx_] = apre[u];                          \
     }                                                                                    \
   }
-  // phases of the lane's 8 atoms (j .. j+7 of the K step) for column c
-  float4 q2[4], q3[4];
-  auto load_p = [&](int jb) {
+  // B fragments (Re hi, Re lo, Im hi, Im lo) of a K step from the table,
+  // BPF K steps ahead in a ring of BPF register sets
+  constexpr int BPF = 4;
+  u32x4 bq[BPF][4];
+  // Every K step issues the same loads, unconditionally (addresses past the
+  // split are clamped to its last step and the data left unused), so that
+  // the compiler's count of loads in flight is exact around the loop and a
+  // wait for one step's data does not drain the prefetches behind it.
+  const int ct_ld = min(ctile, CTt - 1);
+  auto load_b = [&](int jb, u32x4 (&d)[4]) {
+    const u32x4* src = btab + (((size_t)(jb / 32) * CTt + ct_ld) * 4) * 64 + lane;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      q2[u] = *reinterpret_cast<const float4*>(e2row + jb + 2 * u);
-      q3[u] = *reinterpret_cast<const float4*>(e3row + jb + 2 * u);
-    }
+    for (int f = 0; f < 4; ++f) d[f] = src[64 * f];
   };
-  // B fragments: Re and Im of B = E2 E3 for the 8 atoms, hi and lo
-  half8 brh, brl, bih, bil;
-  auto make_b = [&]() {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const float e2r[2] = {q2[u].x, q2[u].z}, e2i[2] = {q2[u].y, q2[u].w};
-      const float e3r[2] = {q3[u].x, q3[u].z}, e3i[2] = {q3[u].y, q3[u].w};
-#pragma unroll
-      for (int v = 0; v < 2; ++v) {
-        const float br = cok ? e2r[v] * e3r[v] - e2i[v] * e3i[v] : 0.f;
-        const float bi = cok ? e2r[v] * e3i[v] + e2i[v] * e3r[v] : 0.f;
-        _Float16 a, b;
-        split16(br, a, b);
-        brh[2 * u + v] = a;
-        brl[2 * u + v] = b;
-        split16(bi, a, b);
-        bih[2 * u + v] = a;
-        bil[2 * u + v] = b;
-      }
-    }
-  };
+  const int jlast = j1 - 32;
 
   if (j0 < j1) {
+#pragma unroll
+    for (int k = 0; k < BPF; ++k) load_b(min(j0 + 32 * k, jlast), bq[k]);
     MX_FETCH_A(j0)
     MX_STAGE_A(0)
-    load_p(j0);
-    make_b();
-    load_p(j0 + 32);
+    MX_FETCH_A(min(j0 + 32, jlast))
   }
   __syncthreads();
   const floatx4 zero = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -310,40 +349,41 @@ __global__ __launch_bounds__(256) void mx_rho_gemm(
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const int js = jb + 32 * ks;
-      const bool more = js + 32 < j1;
-      if (more) MX_FETCH_A(js + 32)
       if (tile_ok) {
-        half8 ah[RTW], al[RTW];
-#pragma unroll
-        for (int r = 0; r < RTW; ++r) {
-          ah[r] = __builtin_bit_cast(half8, sA[ks & 1][rt0 + r][0][lane]);
-          al[r] = __builtin_bit_cast(half8, sA[ks & 1][rt0 + r][1][lane]);
-        }
-        // three passes (hi.hi, hi.lo, lo.hi), 2 RTW independent
+        const half8 brh = __builtin_bit_cast(half8, bq[ks % BPF][0]);
+        const half8 brl = __builtin_bit_cast(half8, bq[ks % BPF][1]);
+        const half8 bih = __builtin_bit_cast(half8, bq[ks % BPF][2]);
+        const half8 bil = __builtin_bit_cast(half8, bq[ks % BPF][3]);
+        // three passes (lo.hi, then hi.hi and hi.lo), 2 RTW independent
         // accumulators between dependent MFMAs
+        half8 al[RTW], ah[RTW];
 #pragma unroll
         for (int r = 0; r < RTW; ++r) {
-          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], brh, ks ? acc[0][r] : zero, 0, 0, 0);
-          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bih, ks ? acc[1][r] : zero, 0, 0, 0);
+          al[r] = __builtin_bit_cast(half8, sA[ks & 1][rt0 + r][1][lane]);
+          ah[r] = __builtin_bit_cast(half8, sA[ks & 1][rt0 + r][0][lane]);
+        }
+#pragma unroll
+        for (int r = 0; r < RTW; ++r) {
+          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], brh, ks ? acc[0][r] : zero, 0, 0, 0);
+          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], bih, ks ? acc[1][r] : zero, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < RTW; ++r) {
+          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], brh, acc[0][r], 0, 0, 0);
+          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bih, acc[1][r], 0, 0, 0);
         }
 #pragma unroll
         for (int r = 0; r < RTW; ++r) {
           acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], brl, acc[0][r], 0, 0, 0);
           acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bil, acc[1][r], 0, 0, 0);
         }
-#pragma unroll
-        for (int r = 0; r < RTW; ++r) {
-          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], brh, acc[0][r], 0, 0, 0);
-          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], bih, acc[1][r], 0, 0, 0);
-        }
       }
-      // the next K step's B fragments (phases already loaded), the loads of
-      // the one after, and its A fragments into the other LDS buffer
-      if (more) {
-        make_b();
-        if (js + 64 < j1) load_p(js + 64);
-        MX_STAGE_A((ks & 1) ^ 1)
-      }
+      // the B fragments BPF K steps ahead, the next step's A fragments into
+      // the other LDS buffer, and the A fragments of the step after
+      static_assert(4 % BPF == 0, "ring index by the unrolled step");
+      load_b(min(js + 32 * BPF, jlast), bq[ks % BPF]);
+      MX_STAGE_A((ks & 1) ^ 1)
+      MX_FETCH_A(min(js + 64, jlast))
       __syncthreads();
     }
     if (tile_ok) {
@@ -882,13 +922,15 @@ static bool sizes(MixedState* m) {
   mx_alloc(&m->rpart, m->rpart_cap);
   mx_alloc(&m->Wt, (size_t)4 * np);
   mx_alloc(&m->afrag, (np / 32) * RHO_NRT2_MAX * 2 * 64);
+  mx_alloc(&m->btab, (np / 32) * (size_t)((m->CH + 15) / 16) * 4 * 64);
+  m->tables_eval = -1;
   const int KS = 2 * m->CHp / 32;
   m->bfrag_cap = (size_t)2 * APP_NCT_E * KS * 64 * 8;
   mx_alloc(&m->bfrag, m->bfrag_cap);
   m->X_cap = (size_t)APP_SLICES * np * APP_NCT_E * 16;
   mx_alloc(&m->X, m->X_cap);
   return m->tj1 && m->tj2 && m->tj3 && m->g && m->gmax && m->rpart && m->bfrag && m->X && m->Wt &&
-         m->afrag;
+         m->afrag && m->btab;
 }
 
 MixedState* create(mdffgpu_ctx* c) {
@@ -904,7 +946,8 @@ MixedState* create(mdffgpu_ctx* c) {
 
 void destroy(MixedState* m) {
   if (!m) return;
-  void* p[] = {m->tj1, m->tj2, m->tj3, m->rpart, m->g, m->gmax, m->bfrag, m->X, m->Wt, m->afrag};
+  void* p[] = {m->tj1, m->tj2, m->tj3, m->rpart, m->g,    m->gmax,
+               m->bfrag, m->X, m->Wt, m->afrag, m->btab};
   for (void* x : p)
     if (x) (void)hipFree(x);
   delete m;
@@ -921,9 +964,8 @@ static void launch_rho(MixedState* m, const CtxView& v, int nrt_used) {
   const int ncb = (ctiles + CPB - 1) / CPB;
   const int used = ncb * m->nsplit;
   const int grid = (used + NXCD - 1) / NXCD * NXCD;
-  mx_rho_gemm<NRT2, RTW><<<grid, 256, 0, v.st>>>(m->npad, m->N3, m->C23, m->CH, m->C23p, nrt_used,
-                                                 ncb, used, RHO_ATOMS_PER_SPLIT, m->afrag, m->tj2,
-                                                 m->tj3, m->rpart);
+  mx_rho_gemm<NRT2, RTW><<<grid, 256, 0, v.st>>>(m->npad, m->C23, m->CH, m->C23p, nrt_used, ncb, used,
+                                                 RHO_ATOMS_PER_SPLIT, m->afrag, m->btab, m->rpart);
 }
 
 template <int NCT, int RT>
@@ -945,12 +987,17 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
   CtxView v = view(m->ctx);
   hipStream_t st = v.st;
   const int n = v.n;
-  // phase tables: positions are fixed within an evaluation, rebuild on the
-  // first reciprocal call of each evaluation (the static or final call)
-  if (mode != ES_IND) {
+  // phase tables and structure-factor B fragments: positions are fixed
+  // within an evaluation, built on its first reciprocal call
+  if (m->tables_eval != v.evals) {
     ProfScope _ps(m->ctx, K_MX_PHASE);
     mx_tables<<<(m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK, 0, st>>>(
         n, m->npad, v.s, m->kx, m->ky, m->kz, m->tj1, m->tj2, m->tj3);
+    const int CTt = (m->CH + 15) / 16;
+    const size_t tb = (size_t)(m->npad / 32) * CTt * 64;
+    mx_rho_btab<<<(unsigned)((tb + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0, st>>>(
+        m->npad, m->N3, m->CH, CTt, m->tj2, m->tj3, m->btab);
+    m->tables_eval = v.evals;
   }
   const int NW = (chg ? 1 : 0) + (dip ? 3 : 0);
   const int NRT2 = (2 * NW * m->N1 + 15) / 16;  // row tiles with data
