@@ -362,6 +362,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW <= 5 ? 
           al[r] = __builtin_bit_cast(half8, sA[ks & 1][rt0 + r][1][lane]);
           ah[r] = __builtin_bit_cast(half8, sA[ks & 1][rt0 + r][0][lane]);
         }
+        // all LDS reads issued before the first MFMA waits on one
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int r = 0; r < RTW; ++r) {
           acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], brh, ks ? acc[0][r] : zero, 0, 0, 0);
@@ -765,6 +767,8 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
         nh1 = B[(c0 + 3) * 64 + lane];
         nl1 = B[NCT * 64 + (c0 + 3) * 64 + lane];
       }
+      // the next pair's LDS reads in flight under this pair's MFMAs
+      __builtin_amdgcn_sched_barrier(0);
       const bool two = c0 + 1 < NCT;
       const half8 b10 = __builtin_bit_cast(half8, h0), b20 = __builtin_bit_cast(half8, l0);
       const half8 b11 = __builtin_bit_cast(half8, h1), b21 = __builtin_bit_cast(half8, l1);
@@ -841,6 +845,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int 
   const float* xre = Xp + (size_t)(w * N1) * npad + i;
   const float* xim = Xp + (size_t)(W * N1 + w * N1) * npad + i;
   double sr = 0, si = 0, pr = 0, pi = 0, qr = 0;
+#pragma unroll 5
   for (int m = 0; m < N1; ++m) {
     double xr = 0.0, xi = 0.0;
     for (int sl = 0; sl < S; ++sl) {
