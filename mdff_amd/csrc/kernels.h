@@ -74,6 +74,7 @@ struct SrArgs {
   double* part;
   const BatchCfg* bc;  // batched path: per-configuration constants
   const int* cfg;      // batched path: configuration of each atom
+  const int* perm;     // atoms in this order (spatial locality), or null
 };
 
 struct EsDirArgs {
@@ -96,6 +97,8 @@ struct EsDirArgs {
   double* part;
   const BatchCfg* bc;  // batched path: per-configuration constants
   const int* cfg;      // batched path: configuration of each atom
+  const int* rank;     // spatial rank of each atom (SCF pair table), or null
+  const int* perm;     // atoms in this order (spatial locality), or null
 };
 
 struct RhoArgs {
@@ -162,7 +165,7 @@ struct ExtrapArgs {
 
 __global__ void k_prepare(int n, const double* pos, Cell cell, double* s, double4* s4,
                           double2* eph);
-__global__ void k_nlist(int n, const double* s, Cell cell, double cut2, int maxnb, int* cnt,
+__global__ void k_nlist(int n, const double* s, const int* perm, Cell cell, double cut2, int maxnb, int* cnt,
                         int* nbr, int* overflow);
 template <typename R, bool BATCH = false>
 __global__ void k_shortrange(SrArgs a);
@@ -173,10 +176,11 @@ __global__ void k_pairs_t2(EsDirArgs a, double* tab);
 __global__ void k_dd_field(EsDirArgs a, const double* tab, const double4* mu4);
 template <typename R4>
 __global__ void k_pack_mu(int n, const double* mu, R4* mu4);
+__global__ void k_pack_mu_ranked(int n, const double* mu, const int* rank, float4* mu4);
 template <bool BATCH = false>
 __global__ void k_dd_table(EsDirArgs a, float4* tab, int* tj, int* tcnt);
 __global__ void k_dd_tab(int n, int maxnb, const float4* tab, const int* tj, const int* tcnt,
-                         const float4* mu4, double* ef);
+                         const float4* mu4, const int* perm, double* ef);
 template <bool CHG, bool DIP>
 __global__ void k_rho64(RhoArgs a);
 __global__ void k_rho_reduce64(RhoRedArgs a);

@@ -37,12 +37,15 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_prepare(
 // ---------------------------------------------------------------------------
 // Full neighbour list (replaces vnlist_pbc, src/tools.f90:274-370): every j
 // whose minimum image lies within cut (list cutoff incl. skin).  Brute force
-// over LDS tiles of fractional positions; j ascending; row-major rows
-// nbr[i*maxnb + k] so that the lanes sharing an atom read contiguously.
+// over LDS tiles of fractional positions, taken in the spatial order perm
+// (cell-sorted atom indices), so that each row lists its neighbours cell by
+// cell; row-major rows nbr[i*maxnb + k] so that the lanes sharing an atom
+// read contiguously.
 __global__ __launch_bounds__(MDFF_BLOCK) void k_nlist(
-    int n, const double* __restrict__ s, Cell cell, double cut2, int maxnb,
-    int* __restrict__ cnt, int* __restrict__ nbr, int* __restrict__ overflow) {
+    int n, const double* __restrict__ s, const int* __restrict__ perm, Cell cell, double cut2,
+    int maxnb, int* __restrict__ cnt, int* __restrict__ nbr, int* __restrict__ overflow) {
   __shared__ double ts[3][MDFF_BLOCK];
+  __shared__ int tj[MDFF_BLOCK];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   bool act = i < n;
   double si1 = 0, si2 = 0, si3 = 0;
@@ -53,9 +56,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_nlist(
   }
   int c = 0;
   for (int base = 0; base < n; base += MDFF_BLOCK) {
-    int j = base + threadIdx.x;
+    int r = base + threadIdx.x;
     __syncthreads();
-    if (j < n) {
+    if (r < n) {
+      const int j = perm ? perm[r] : r;
+      tj[threadIdx.x] = j;
       ts[0][threadIdx.x] = s[j];
       ts[1][threadIdx.x] = s[j + n];
       ts[2][threadIdx.x] = s[j + 2 * n];
@@ -64,7 +69,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_nlist(
     int m = min(MDFF_BLOCK, n - base);
     if (act) {
       for (int jj = 0; jj < m; ++jj) {
-        int jg = base + jj;
+        int jg = tj[jj];
         double rx, ry, rz;
         double d2 = min_image(cell, si1 - ts[0][jj], si2 - ts[1][jj],
                               si3 - ts[2][jj], rx, ry, rz);

@@ -73,6 +73,12 @@ struct mdffgpu_ctx {
   double *d_q = nullptr, *d_qzero = nullptr, *d_dip = nullptr, *d_mu = nullptr,
          *d_muind = nullptr, *d_mup = nullptr;
   double4* d_s4 = nullptr;   // fractional coordinates, one double4 per atom
+  // spatial order of the atoms (cells of about 4 A, row-major, atom index
+  // within a cell), refreshed with the neighbour list: perm[rank] = atom
+  int* d_perm = nullptr;
+  int* d_rank = nullptr;
+  std::vector<double> h_sfrac;
+  std::vector<int> h_perm, h_rank, h_cellof, h_cellcnt;
   double mx_floor = 0.0;     // SCF residual floor of the mixed path (0 = unknown)
   double4* d_mu4 = nullptr;  // packed dipoles of the SCF field (double4 or float4)
   double *d_Estat = nullptr, *d_E = nullptr, *d_efdir = nullptr, *d_efgdir = nullptr,
@@ -361,6 +367,8 @@ void alloc_atoms(mdffgpu_ctx* c) {
   dalloc(c, &c->d_muind, 3 * n);
   dalloc(c, &c->d_mu4, n);
   dalloc(c, &c->d_s4, n);
+  dalloc(c, &c->d_perm, n);
+  dalloc(c, &c->d_rank, n);
   dalloc(c, &c->d_mup, 3 * n);
   dalloc(c, &c->d_Estat, 3 * n);
   dalloc(c, &c->d_E, 3 * n);
@@ -412,6 +420,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   const int gbl = nblk(n * LPA);
   // real space
   EsDirArgs da;
+  std::memset(&da, 0, sizeof(da));
   da.n = n;
   da.maxnb = c->maxnb;
   da.s = c->d_s;
@@ -426,6 +435,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   da.mu = mu;
   da.cut2 = P.cutlongrange * P.cutlongrange;
   da.alpha = P.alphaES;
+  da.perm = c->d_perm;
   da.task = task;
   da.do_efield = do_efield;
   da.do_efg = do_efg;
@@ -450,9 +460,9 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
       const size_t cap = (size_t)c->maxnb * n;
       const float4* tab = reinterpret_cast<const float4*>(c->d_t2);
       const int* tj = reinterpret_cast<const int*>(tab + cap);
-      k_pack_mu<float4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, m4);
+      k_pack_mu_ranked<<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_rank, m4);
       k_dd_tab<<<nblk(n * 64), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4,
-                                                             c->d_efdir);
+                                                    c->d_perm, c->d_efdir);
     } else {
       k_pack_mu<double4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_mu4);
       k_dd_field<<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, (const double*)c->d_t2, c->d_mu4);
@@ -631,6 +641,8 @@ void build_t2(mdffgpu_ctx* c) {
   da.cell = c->cell;
   da.cut2 = c->P.cutlongrange * c->P.cutlongrange;
   da.alpha = c->P.alphaES;
+  da.rank = c->d_rank;
+  da.perm = c->d_perm;
   ProfScope _ps(c, K_PAIRS_T2);
   if (mx) {
     float4* tab = reinterpret_cast<float4*>(c->d_t2);
@@ -768,6 +780,52 @@ void sym9(const double* t6, double scale, double* out) {
   out[5] = out[7] = t6[5] * scale;
 }
 
+// Spatial order of the atoms for the neighbour list: cells of about 4 A
+// (row-major over the fractional grid), atoms of a cell by index.  A stable
+// counting sort on the host from the fractional coordinates; it runs with
+// each list rebuild (rare under the Verlet skin).
+void spatial_order(mdffgpu_ctx* c) {
+  const int n = c->n;
+  c->h_sfrac.resize(3 * (size_t)n);
+  HCK(c, hipMemcpyAsync(c->h_sfrac.data(), c->d_s, 3 * (size_t)n * sizeof(double),
+                        hipMemcpyDeviceToHost, c->st));
+  HCK(c, hipStreamSynchronize(c->st));
+  int nc[3];
+  for (int a = 0; a < 3; ++a) {
+    const double len = std::sqrt(c->cell.A[3 * a] * c->cell.A[3 * a] +
+                                 c->cell.A[3 * a + 1] * c->cell.A[3 * a + 1] +
+                                 c->cell.A[3 * a + 2] * c->cell.A[3 * a + 2]);
+    nc[a] = std::max(1, std::min(64, (int)(len / 4.0)));
+  }
+  const int ncell = nc[0] * nc[1] * nc[2];
+  c->h_cellof.resize(n);
+  c->h_cellcnt.assign((size_t)ncell + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    int ci[3];
+    for (int a = 0; a < 3; ++a) {
+      double f = c->h_sfrac[i + (size_t)a * n];
+      f -= std::floor(f);
+      ci[a] = std::min(nc[a] - 1, (int)(f * nc[a]));
+    }
+    const int cell = (ci[0] * nc[1] + ci[1]) * nc[2] + ci[2];
+    c->h_cellof[i] = cell;
+    ++c->h_cellcnt[cell + 1];
+  }
+  for (int k = 0; k < ncell; ++k) c->h_cellcnt[k + 1] += c->h_cellcnt[k];
+  c->h_perm.resize(n);
+  c->h_rank.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const int r = c->h_cellcnt[c->h_cellof[i]]++;
+    c->h_perm[r] = i;
+    c->h_rank[i] = r;
+  }
+  HCK(c, hipMemcpyAsync(c->d_perm, c->h_perm.data(), (size_t)n * sizeof(int),
+                        hipMemcpyHostToDevice, c->st));
+  HCK(c, hipMemcpyAsync(c->d_rank, c->h_rank.data(), (size_t)n * sizeof(int),
+                        hipMemcpyHostToDevice, c->st));
+  HCK(c, hipStreamSynchronize(c->st));  // the host vectors are reused
+}
+
 // engforce_driver on the device buffers of ctx; d_pos and d_force are
 // loaded by the caller.
 void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
@@ -804,12 +862,13 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
                           hipMemcpyDeviceToDevice, st));
     c->list_valid = true;
     ++c->list_builds;
+    spatial_order(c);
     for (;;) {
       HCK(c, hipMemsetAsync(c->d_ovf, 0, sizeof(int), st));
       {
         ProfScope _ps(c, K_NLIST);
         const double lc = c->list_cut + P.skindiff;
-      k_nlist<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->cell, lc * lc, c->maxnb,
+      k_nlist<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_s, nullptr, c->cell, lc * lc, c->maxnb,
                                             c->d_cnt, c->d_nbr, c->d_ovf);
       }
       check_launch(c, "k_nlist");
@@ -831,8 +890,10 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
     {
         ProfScope _ps(c, mxs ? K_MX_SR : K_SR);
         SrArgs sa;
+        std::memset(&sa, 0, sizeof(sa));
         sa.n = n;
         sa.maxnb = c->maxnb;
+        sa.perm = c->d_perm;
         sa.s = c->d_s;
         sa.s4 = c->d_s4;
         sa.typ = c->d_typ;
@@ -1180,7 +1241,7 @@ void mdffgpu_destroy(mdffgpu_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->d_t2) (void)hipFree(c->d_t2);
   void* vp[] = {c->d_eph, c->d_typ, c->d_cnt, c->d_nbr, c->d_ovf, c->d_rpart, c->d_g,
-                c->d_chunks, c->d_pt, c->d_tt, c->d_mu4, c->d_s4};
+                c->d_chunks, c->d_pt, c->d_tt, c->d_mu4, c->d_s4, c->d_perm, c->d_rank};
   for (void* p : vp)
     if (p) (void)hipFree(p);
   for (auto& p : c->prof_pending) {

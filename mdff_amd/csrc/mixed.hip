@@ -893,7 +893,7 @@ static void mx_alloc(T** p, size_t count) {
 }
 
 // apply launch shapes: (NCT, RT) and the number of K slices
-constexpr int APP_NCT_F = 10, APP_RT_F = 2;  // field calls (W = 3)
+constexpr int APP_NCT_F = 10, APP_RT_F = 1;  // field calls (W = 3)
 constexpr int APP_NCT_E = 19, APP_RT_E = 1;  // final call with EFG (W = 6)
 constexpr int APP_ROWS_MAX = APP_WAVES * 2 * 16;
 constexpr int APP_SLICES = 2;

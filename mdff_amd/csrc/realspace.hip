@@ -23,7 +23,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(SrArgs a) {
              MO = a.flags & SR_MO;
   const PairTables* T = a.T;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = tid / LPA, sub = tid % LPA;
+  const int i0 = tid / LPA, sub = tid % LPA;
+  const int i = (a.perm && i0 < n) ? a.perm[i0] : i0;  // atoms in spatial order
   R acc[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = R(0);
@@ -146,7 +147,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
   const bool FORCE = (MODE == ES_FINAL);
   const bool DAMP = (MODE != ES_IND);
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = tid / LPA, sub = tid % LPA;
+  const int i0 = tid / LPA, sub = tid % LPA;
+  const int i = (a.perm && i0 < n) ? a.perm[i0] : i0;  // atoms in spatial order
   const int cf = (BATCH && i < n) ? a.cfg[i] : 0;
   const Cell& cell = BATCH ? a.bc[cf].cell : a.cell;
   const R alpha = R(BATCH ? a.bc[cf].alpha : a.alpha), alpha2 = alpha * alpha,
@@ -366,8 +368,9 @@ template <bool BATCH>
 __global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, double* __restrict__ tab) {
   const int n = a.n;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = tid / LPA, sub = tid % LPA;
-  if (i >= n) return;
+  const int i0 = tid / LPA, sub = tid % LPA;
+  if (i0 >= n) return;
+  const int i = a.perm ? a.perm[i0] : i0;  // atoms in spatial order
   const int cf = BATCH ? a.cfg[i] : 0;
   const Cell& cell = BATCH ? a.bc[cf].cell : a.cell;
   const double cut2 = BATCH ? a.bc[cf].cut2 : a.cut2;
@@ -401,6 +404,17 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, double* __
   }
 }
 
+// Dipoles of the SCF iteration in FP32, stored at the atom's spatial rank:
+// a row of the compacted pair table lists its neighbours by rank, so the
+// lanes of one gather read neighbouring 16-byte slots.
+__global__ __launch_bounds__(MDFF_BLOCK) void k_pack_mu_ranked(int n, const double* __restrict__ mu,
+                                                               const int* __restrict__ rank,
+                                                               float4* __restrict__ mu4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  mu4[rank[i]] = make_float4((float)mu[i], (float)mu[i + n], (float)mu[i + 2 * n], 0.f);
+}
+
 // Dipoles of the SCF iteration packed as (x, y, z, 0) per atom, so that a
 // neighbour's dipole is one gather.
 template <typename R4>
@@ -426,7 +440,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a,
   using R4 = double4;
   const int n = a.n;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = tid / DD_LANES, sub = tid % DD_LANES;
+  const int i0 = tid / DD_LANES, sub = tid % DD_LANES;
+  const int i = (a.perm && i0 < n) ? a.perm[i0] : i0;  // atoms in spatial order
   R e0 = 0, e1 = 0, e2 = 0;
   const size_t plane = (size_t)a.maxnb * n;
   const R4* t4 = reinterpret_cast<const R4*>(tab);
@@ -475,7 +490,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a,
 // per evaluation k_dd_table keeps, per atom and in list order, only the list
 // entries inside the cutoff, as (v, A1) with v = r sqrt(A2), A2 = 3F2/d^5,
 // A1 = F1/d^3 (the undamped d-d terms of src/field.f90:2407-2421, formed in
-// FP32 from an FP64 fractional difference), plus the neighbour index.  Each
+// FP32 from an FP64 fractional difference), plus the neighbour's slot in the
+// dipole array (its spatial rank when a.rank is given, else its index).  Each
 // SCF iteration k_dd_tab then applies E_i = sum_j [v (v.mu_j) - A1 mu_j]:
 // 20 bytes of table and one dipole gather per pair.  One wave per atom.
 template <bool BATCH>
@@ -484,8 +500,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
                                                          int* __restrict__ tcnt) {
   const int n = a.n;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = tid >> 6, lane = tid & 63;
-  if (i >= n) return;  // whole waves
+  const int i0 = tid >> 6, lane = tid & 63;
+  if (i0 >= n) return;  // whole waves
+  const int i = a.perm ? a.perm[i0] : i0;  // atoms in spatial order
   const int cf = BATCH ? a.cfg[i] : 0;
   const Cell& cell = BATCH ? a.bc[cf].cell : a.cell;
   const float cut2 = (float)(BATCH ? a.bc[cf].cut2 : a.cut2);
@@ -523,7 +540,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
     if (in) {
       const size_t e = row + base + __popcll(m & below);
       tab[e] = v;
-      tj[e] = j;
+      tj[e] = a.rank ? a.rank[j] : j;
     }
     base += __popcll(m);
   }
@@ -535,10 +552,12 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_tab(int n, int maxnb,
                                                        const int* __restrict__ tj,
                                                        const int* __restrict__ tcnt,
                                                        const float4* __restrict__ mu4,
+                                                       const int* __restrict__ perm,
                                                        double* __restrict__ ef) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = tid >> 6, lane = tid & 63;
-  if (i >= n) return;  // whole waves
+  const int i0 = tid >> 6, lane = tid & 63;
+  if (i0 >= n) return;  // whole waves
+  const int i = perm ? perm[i0] : i0;  // atoms in spatial order
   const int c = tcnt[i];
   const size_t row = (size_t)i * maxnb;
   float e0 = 0.f, e1 = 0.f, e2 = 0.f;
