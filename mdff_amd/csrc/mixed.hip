@@ -245,19 +245,21 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_btab(int npad, int N3, int 
 // at half the MFMA work and half the B operand of complex products over all
 // columns.  GEMM rows (p, x) as row 2p + x (NRT2 tiles), columns (c, y) as
 // two tiles (y = re, im) per 16 columns c, K = atoms (32 per MFMA step).
-// A fragments (mx_rho_afrag) are staged through LDS one K step ahead and
-// shared by the four waves; a wave owns one column tile and row tiles
-// rg*RTW .. of NRT2, and reads the B fragments of its 16 columns from the
-// per-evaluation table of mx_rho_btab, four K steps ahead.  FP32 MFMA chains
-// are flushed into FP64 every 128 atoms.  Blocks map to
+// A fragments (mx_rho_afrag) are fetched three K steps ahead, staged through
+// LDS one step ahead and shared by the four waves; a wave owns one column
+// tile and row tiles rg*RTW .. of NRT2, and reads the B fragments of its 16
+// columns from the per-evaluation table of mx_rho_btab, two K steps ahead.
+// With charge weights (F64) the FP32 MFMA chains are flushed into FP64
+// every 128 atoms: neutral sums cancel at small |k|; dipole weights alone
+// keep one FP32 chain per split.  Blocks map to
 // (atom split, column block) XCD-aware: the blocks of one XCD take
 // consecutive indices of the split-major order, so each XCD's L2 holds the
 // A fragments and phases of about one split.  Output: part[split][p][col]
 // with S at Re plane col and Im plane C23p + col, for both c and c'.
 constexpr int NXCD = 8;
 
-template <int NRT2, int RTW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW <= 5 ? 2 : 1))) void mx_rho_gemm(
+template <int NRT2, int RTW, bool F64>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1 : F64 ? 2 : 3))) void mx_rho_gemm(
     int npad, int C23, int CH, int C23p, int nrt_used, int ncb, int nblk_used, int atoms_per_split,
     const u32x4* __restrict__ afrag, const u32x4* __restrict__ btab, double* __restrict__ out) {
   constexpr int RG = (NRT2 + RTW - 1) / RTW;  // row groups
@@ -298,29 +300,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW <= 5 ? 
   const int j0 = split * atoms_per_split;
   const int j1 = min(npad, j0 + atoms_per_split);  // j1 - j0: a multiple of 128
 
-  // A fragments of one K step: ABLK contiguous 16-byte words of afrag
+  // A fragments of one K step: ABLK contiguous 16-byte words of afrag,
+  // fetched two K steps before they are staged, in a ring of two register
+  // sets (slot = step parity)
   constexpr int ABLK = NRT2 * 2 * 64;
   constexpr int APER = (ABLK + 255) / 256;
-  u32x4 apre[APER];
-#define MX_FETCH_A(jb)                                                                   \
+  u32x4 apre[2][APER];
+#define MX_FETCH_A(jb, slot)                                                             \
   {                                                                                      \
     const u32x4* src_ = afrag + (size_t)((jb) / 32) * ABLK;                              \
     _Pragma("unroll") for (int u = 0; u < APER; ++u) {                                   \
       const int idx_ = threadIdx.x + u * 256;                                            \
-      if (ABLK % 256 == 0 || idx_ < ABLK) apre[u] = src_[idx_];                          \
+      if (ABLK % 256 == 0 || idx_ < ABLK) apre[slot][u] = src_[idx_];                    \
     }                                                                                    \
   }
-#define MX_STAGE_A(b)                                                                    \
+#define MX_STAGE_A(b, slot)                                                              \
   {                                                                                      \
     u32x4* dst_ = &sA[b][0][0][0];                                                       \
     _Pragma("unroll") for (int u = 0; u < APER; ++u) {                                   \
       const int idx_ = threadIdx.x + u * 256;                                            \
-      if (ABLK % 256 == 0 || idx_ < ABLK) dst_[idx_] = apre[u];                          \
+      if (ABLK % 256 == 0 || idx_ < ABLK) dst_[idx_] = apre[slot][u];                    \
     }                                                                                    \
   }
   // B fragments (Re hi, Re lo, Im hi, Im lo) of a K step from the table,
   // BPF K steps ahead in a ring of BPF register sets
-  constexpr int BPF = 4;
+  constexpr int BPF = 2;
   u32x4 bq[BPF][4];
   // Every K step issues the same loads, unconditionally (addresses past the
   // split are clamped to its last step and the data left unused), so that
@@ -337,9 +341,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW <= 5 ? 
   if (j0 < j1) {
 #pragma unroll
     for (int k = 0; k < BPF; ++k) load_b(min(j0 + 32 * k, jlast), bq[k]);
-    MX_FETCH_A(j0)
-    MX_STAGE_A(0)
-    MX_FETCH_A(min(j0 + 32, jlast))
+    MX_FETCH_A(j0, 0)
+    MX_STAGE_A(0, 0)
+    MX_FETCH_A(min(j0 + 32, jlast), 1)
+    MX_FETCH_A(min(j0 + 64, jlast), 0)
   }
   __syncthreads();
   const floatx4 zero = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -366,8 +371,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW <= 5 ? 
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int r = 0; r < RTW; ++r) {
-          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], brh, ks ? acc[0][r] : zero, 0, 0, 0);
-          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], bih, ks ? acc[1][r] : zero, 0, 0, 0);
+          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], brh, (ks || !F64) ? acc[0][r] : zero, 0, 0, 0);
+          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], bih, (ks || !F64) ? acc[1][r] : zero, 0, 0, 0);
         }
 #pragma unroll
         for (int r = 0; r < RTW; ++r) {
@@ -381,14 +386,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW <= 5 ? 
         }
       }
       // the B fragments BPF K steps ahead, the next step's A fragments into
-      // the other LDS buffer, and the A fragments of the step after
+      // the other LDS buffer, and the A fragments three steps ahead
       static_assert(4 % BPF == 0, "ring index by the unrolled step");
       load_b(min(js + 32 * BPF, jlast), bq[ks % BPF]);
-      MX_STAGE_A((ks & 1) ^ 1)
-      MX_FETCH_A(min(js + 64, jlast))
+      MX_STAGE_A((ks & 1) ^ 1, (ks & 1) ^ 1)
+      MX_FETCH_A(min(js + 96, jlast), (ks & 1) ^ 1)
       __syncthreads();
     }
-    if (tile_ok) {
+    if (F64 && tile_ok) {
 #pragma unroll
       for (int y = 0; y < 2; ++y)
 #pragma unroll
@@ -396,6 +401,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW <= 5 ? 
 #pragma unroll
           for (int e = 0; e < 4; ++e) accd[y][r][e] += (double)acc[y][r][e];
     }
+  }
+  if (!F64) {
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < RTW; ++r)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) accd[y][r][e] = (double)acc[y][r][e];
   }
   if (!tile_ok || !cok) return;
   // D layout: acc[y][r][e] = P_xy of row (rt0 + r)*16 + 4 kgrp + e: e = 0, 1
@@ -962,14 +975,14 @@ void on_cell(MixedState* m) {
   if (view(m->ctx).P->lcoulomb) sizes(m);
 }
 
-template <int NRT2, int RTW>
+template <int NRT2, int RTW, bool F64>
 static void launch_rho(MixedState* m, const CtxView& v, int nrt_used) {
   constexpr int CPB = 4 / ((NRT2 + RTW - 1) / RTW);
   const int ctiles = (m->CH + 15) / 16;
   const int ncb = (ctiles + CPB - 1) / CPB;
   const int used = ncb * m->nsplit;
   const int grid = (used + NXCD - 1) / NXCD * NXCD;
-  mx_rho_gemm<NRT2, RTW><<<grid, 256, 0, v.st>>>(m->npad, m->C23, m->CH, m->C23p, nrt_used, ncb, used,
+  mx_rho_gemm<NRT2, RTW, F64><<<grid, 256, 0, v.st>>>(m->npad, m->C23, m->CH, m->C23p, nrt_used, ncb, used,
                                                  RHO_ATOMS_PER_SPLIT, m->afrag, m->btab, m->rpart);
 }
 
@@ -1022,12 +1035,14 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
   }
   {
     ProfScope _ps(m->ctx, K_MX_RHO);
+    // FP32 chains over a whole split for dipole weights alone; with charges
+    // (neutral sums that cancel at small |k|) the chains are flushed to FP64
     if (NRT2l == 4)
-      launch_rho<4, 4>(m, v, NRT2);
+      chg ? launch_rho<4, 4, true>(m, v, NRT2) : launch_rho<4, 4, false>(m, v, NRT2);
     else if (NRT2l == 10)
-      launch_rho<10, 5>(m, v, NRT2);
+      chg ? launch_rho<10, 5, true>(m, v, NRT2) : launch_rho<10, 5, false>(m, v, NRT2);
     else
-      launch_rho<13, 7>(m, v, NRT2);
+      launch_rho<13, 7, true>(m, v, NRT2);
   }
   const dim3 gred((m->C23 + MDFF_BLOCK - 1) / MDFF_BLOCK, m->N1);
   const int gk = gred.x * gred.y;
