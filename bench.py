@@ -39,6 +39,10 @@ def parse():
     ap.add_argument("--workload", type=int, default=5, choices=[2, 3, 5])
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched force-matching gradient leg")
+    ap.add_argument("--batch-evals", type=int, default=5,
+                    help="timed loss+gradient evaluations of the batch leg")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this size first (calibrates HBM "
@@ -60,6 +64,20 @@ def frames_for(sys_, nframes, seed=20240601, amp=0.0125):
         out[f] = r.T
         r = r + amp * rng.standard_normal(r.shape)
     return out
+
+
+def profile_traffic(kernel):
+    """HBM bytes per launch of a kernel event scope, as measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-kernel calibrated) and
+    recorded in a committed summary under profiles/ as the line
+    'traffic_bytes_per_launch <scope> <bytes>'; (None, None) if absent."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+                return float(f[2]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def time_path(args, sys_, precision, frames_dev):
@@ -119,13 +137,85 @@ def time_path(args, sys_, precision, frames_dev):
         flop = None
     if flop:
         achieved = flop / avg_s / 1e12
+        traffic, source = profile_traffic(name)
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "kernel": name,
+                "frac": achieved / peak, "traffic": traffic, "kernel": name,
                 "avg_launch_us": avg_s * 1e6, "launches": cnt,
                 "flops_per_launch": flop, "pipe": pipe}
+        if source:
+            roof["traffic_source"] = source
     return {"ms_per_step": dt * 1e3, "atom_steps_per_s": n / dt, "n_scf": nscf,
             "kernels": {k: {"total_ms": v[0], "launches": v[1]} for k, v in stats.items()},
             "roofline": roof, "force": f_last}
+
+
+BATCH_SCF = 20  # fixed SCF iterations of the captured loss+gradient graph
+
+
+def batch_leg(args):
+    """Batched force matching: 64 polarizable silica-like configurations
+    with atom counts uniform in [96, 1536] (kmax 8, cutoffs min(10 A, L/2)),
+    target forces from the model at hidden parameters (2% seeded
+    perturbation) plus N(0, 0.05 eV/A) noise.  One loss + adjoint gradient
+    evaluation (BMHFTD parameters, charges, polarizabilities) is one replayed
+    hipGraph with BATCH_SCF SCF iterations; timed for FP64 and mixed, with
+    the mixed loss, gradient and forces compared against FP64."""
+    import numpy as np
+    import torch
+    from mdff_amd import configs as C
+    from mdff_amd import params as P
+    from mdff_amd.batch import Batch, autotune, get_theta, set_theta
+    t0 = time.perf_counter()
+    systems = C.batch(nconf=64, precision=P.PREC_FP64)
+    natm = sum(s.natm for s in systems)
+    log(args, f"[bench] batch of {len(systems)} configurations, {natm} atoms, generated in "
+              f"{time.perf_counter() - t0:.1f}s")
+    pos = torch.tensor(np.concatenate([s.r for s in systems], axis=0).T.copy(),
+                       dtype=torch.float64, device="cuda")
+    b64 = Batch.from_systems(systems, precision=P.PREC_FP64)
+    b32 = Batch.from_systems(systems, precision=P.PREC_MIXED)
+    b64.set_positions(pos)
+    b32.set_positions(pos)
+    rng = np.random.default_rng(41)
+    th0 = get_theta(b64.model)
+    hidden = th0 * (1.0 + 0.02 * rng.standard_normal(th0.shape))
+    b64.set_model(set_theta(b64.model, hidden))
+    f_hidden, _ = b64.forces(BATCH_SCF)
+    b64.set_model(set_theta(b64.model, th0))
+    noise = torch.tensor(0.05 * rng.standard_normal(tuple(f_hidden.shape)), device="cuda")
+    target = (f_hidden + noise).contiguous()
+    tune = autotune(b32, target, BATCH_SCF, reps=2)
+    out = {"nconf": len(systems), "natm": natm, "scf_iters": BATCH_SCF,
+           "evals": args.batch_evals, "apply_lanes": tune.get("apply_lanes") if tune else None}
+    ref = {}
+    for key, b in (("fp64", b64), ("mixed", b32)):
+        loss, grad, forces = b.loss_grad(target, BATCH_SCF, want_forces=True)
+        ref[key] = (loss, grad, forces)
+        b.graph(True)
+        for _ in range(2):
+            b.loss_grad(target, BATCH_SCF)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.batch_evals):
+            b.loss_grad(target, BATCH_SCF)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t1) / args.batch_evals
+        b.graph(False)
+        out[key] = {"grad_evals_per_s": 1.0 / dt, "ms_per_grad_eval": dt * 1e3,
+                    "atom_evals_per_s": natm / dt}
+        log(args, f"[bench] batch {key}: {dt * 1e3:.2f} ms per loss+gradient")
+    l64, g64, f64 = ref["fp64"]
+    l32, g32, f32 = ref["mixed"]
+    out["mixed_vs_fp64"] = {
+        "speedup": out["mixed"]["grad_evals_per_s"] / out["fp64"]["grad_evals_per_s"],
+        "loss_rel_err": abs(l32 - l64) / abs(l64),
+        "grad_max_rel_err": float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))),
+        "force_rel_rms_err": float(torch.sqrt(((f32 - f64) ** 2).mean())
+                                   / torch.sqrt((f64 ** 2).mean())),
+    }
+    b64.close()
+    b32.close()
+    return out
 
 
 def cpu_baseline(args, sys_, n_es_per_step):
@@ -254,6 +344,8 @@ def main():
     if err is not None:
         out["mixed_vs_fp64"] = {"speedup": res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"],
                                 "force_rel_rms_err": err}
+    if not args.no_batch:
+        out["batch"] = batch_leg(args)
     print(json.dumps(out), flush=True)
 
 

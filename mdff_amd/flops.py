@@ -44,15 +44,17 @@ def mixed_kspace_flops(kernel: str, natm: int, kmax, steps: int, n_scf_total: in
                        efg: bool) -> float:
     """Executed matrix-core flops of the mixed k-space GEMMs over a run of
     `steps` evaluations with `n_scf_total` SCF iterations in all: per call
-    3 fp16 products of a real GEMM.
-      mx_rho:   rows 2*NW*(kx+1), cols (2ky+1)(2kz+1), K 2*natm
-      mx_apply: rows natm, K 2*(2ky+1)(2kz+1), cols 2*W*(kx+1)
+    3 fp16 products of a real GEMM over the conjugate-paired half of the
+    (n2, n3) columns, CH = ((2ky+1)(2kz+1) + 1) / 2.
+      mx_rho:   rows 2*NW*(kx+1) (weight, n1, re|im), cols 2*CH, K natm
+      mx_apply: rows natm, K 2*CH, cols 2*W*(kx+1)
     NW = 1 (static call), 3 (SCF calls), 4 (final call); W = 3, or 6 in the
     final call when EFGs or dipole forces are needed."""
     kx, ky, kz = kmax
     n1, c23 = kx + 1, (2 * ky + 1) * (2 * kz + 1)
+    ch = (c23 + 1) // 2
     if kernel == "mx_rho":
         rows = [2 * 1 * n1] * steps + [2 * 3 * n1] * n_scf_total + [2 * 4 * n1] * steps
-        return sum(3 * 2.0 * r * c23 * 2 * natm for r in rows)
+        return sum(3 * 2.0 * r * 2 * ch * natm for r in rows)
     cols = [2 * 3 * n1] * (steps + n_scf_total) + [2 * 6 * n1] * steps
-    return sum(3 * 2.0 * natm * 2 * c23 * c for c in cols)
+    return sum(3 * 2.0 * natm * 2 * ch * c for c in cols)
