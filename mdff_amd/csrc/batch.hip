@@ -37,6 +37,7 @@
 
 #include "../../include/mdffgpu.h"
 #include "kernels.h"
+#include "mixed.h"
 #include "rtypes.h"
 #include "tables.h"
 
@@ -493,6 +494,11 @@ struct mdffgpu_batch {
   int nconf = 0, N = 0, KX = 0, KY = 0, KZ = 0, ntab = 0, NK = 0, nitems = 0, maxnb = 0;
   int ntype = 0, npair = 0, npol = 0, nparam = 0;
   bool mixed = false, pim = false;
+  // FP32 path: reciprocal sums on the matrix cores (configurations with a
+  // common kmax); positions' version for its phase tables
+  mdff::mixed::MixedState* mx = nullptr;
+  bool mx_tried = false;
+  long long epoch = 0;
   int pol_type[MDFF_NT];
   double pol_alpha[MDFF_NT];
   int upair_h[256];
@@ -672,6 +678,27 @@ void launch_apply(mdffgpu_batch* b, const double* q, const double* mu, double* E
   }
 }
 
+// Matrix-core reciprocal sums of the FP32 path, created on first use when
+// every configuration has the same kmax (else the per-k kernels serve).
+mdff::mixed::MixedState* mx_state(mdffgpu_batch* b) {
+  if (b->mx || b->mx_tried) return b->mx;
+  b->mx_tried = true;
+  const BatchCfg& x0 = b->bc_h[0];
+  for (const BatchCfg& x : b->bc_h)
+    for (int a = 0; a < 3; ++a)
+      if (x.kmax[a] != x0.kmax[a]) return nullptr;
+  std::vector<mdff::mixed::MxSeg> segs(b->bc_h.size());
+  for (size_t c = 0; c < segs.size(); ++c) {
+    std::memset(&segs[c], 0, sizeof(segs[c]));
+    segs[c].cell = b->bc_h[c].cell;
+    segs[c].alpha = b->bc_h[c].alpha;
+    segs[c].off = b->bc_h[c].off;
+    segs[c].n = b->bc_h[c].n;
+  }
+  b->mx = mdff::mixed::create_batch(b->N, x0.kmax, segs);
+  return b->mx;
+}
+
 // One multipole_ES call over the batch with sources (q, mu):
 //   STATIC  out = static field (charges only)
 //   IND     out = S + dipole field of mu (the SCF field operator)
@@ -709,12 +736,17 @@ void es_call(mdffgpu_batch* b, int mode, const double* q, const double* mu, cons
   launch_check(b, "batch real space");
   const int chg = (mode != ES_IND) ? 1 : 0;
   const int dip = (mode != ES_STATIC && b->pim) ? 1 : 0;
-  k_b_rho<R><<<b->nitems, BK, 0, b->st>>>(N, b->ntab, b->d_bc, b->d_items, b->d_kn, b->d_kv,
-                                          b->d_tab, q, mu, chg, dip, b->d_rho);
-  if (mode == ES_FINAL)
-    launch_apply<R, true>(b, q, mu, b->d_rec, b->d_frec);
-  else
-    launch_apply<R, false>(b, q, mu, b->d_rec, nullptr);
+  if (sizeof(R) == sizeof(float) && mx_state(b)) {
+    mdff::mixed::kspace_batch(b->mx, b->st, b->d_s, b->epoch, b->d_cfg, q, mu, chg, dip,
+                              mode == ES_FINAL, b->d_rec, b->d_frec);
+  } else {
+    k_b_rho<R><<<b->nitems, BK, 0, b->st>>>(N, b->ntab, b->d_bc, b->d_items, b->d_kn, b->d_kv,
+                                            b->d_tab, q, mu, chg, dip, b->d_rho);
+    if (mode == ES_FINAL)
+      launch_apply<R, true>(b, q, mu, b->d_rec, b->d_frec);
+    else
+      launch_apply<R, false>(b, q, mu, b->d_rec, nullptr);
+  }
   launch_check(b, "batch k space");
   const int gb = nblk(N);
   const double cu = b->P.coul_unit;
@@ -1103,6 +1135,8 @@ void set_positions_impl(mdffgpu_batch* b, const double* d_pos) {
       build_pairs<double>(b);
   }
   b->positions_set = true;
+  ++b->epoch;
+  if (b->mixed) mx_state(b);  // allocates: never first inside a graph capture
   if (b->gexec) {
     (void)hipGraphExecDestroy(b->gexec);
     b->gexec = nullptr;
@@ -1161,6 +1195,7 @@ int mdffgpu_batch_create(const mdffgpu_params* model, int32_t nconf,
 void mdffgpu_batch_destroy(mdffgpu_batch* b) {
   if (!b) return;
   if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->mx) mdff::mixed::destroy(b->mx);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   for (void* p : b->allocs) (void)hipFree(p);
   if (b->h_out) (void)hipHostFree(b->h_out);

@@ -128,7 +128,7 @@ const char* const kKernelNames[K_COUNT] = {
     "batch", "adjoint"};
 
 ProfScope::ProfScope(mdffgpu_ctx* ctx, int kid) : c(ctx), id(kid), a(nullptr), b(nullptr) {
-  if (!c->prof || (c->prof_only >= 0 && c->prof_only != kid)) return;
+  if (!c || !c->prof || (c->prof_only >= 0 && c->prof_only != kid)) return;
   if (c->ev_pool.size() < 2) {
     for (int k = 0; k < 2; ++k) {
       hipEvent_t e;

@@ -5,6 +5,8 @@
 #include "../../include/mdffgpu.h"
 #include "kernels.h"
 
+#include <vector>
+
 namespace mdff {
 namespace mixed {
 
@@ -47,6 +49,16 @@ struct CtxView {
 
 struct MixedState;
 
+// One configuration on the padded atom axis of the reciprocal-space GEMMs:
+// atoms off .. off+n-1 of the caller's arrays occupy padded slots
+// poff .. poff+np-1 (np a multiple of 128, slots past n are zero), and
+// structure-factor splits split0 .. split0+nsplit-1.
+struct MxSeg {
+  Cell cell;
+  double alpha;
+  int off, n, poff, np, split0, nsplit;
+};
+
 // defined in mdffgpu.hip
 CtxView view(mdffgpu_ctx* c);
 
@@ -58,6 +70,18 @@ void on_cell(MixedState* m);
 // energy / stress rows to the scalar slots.
 void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg, bool dip,
             bool efg, double* apart);
+
+// Batched path: configurations with a common kmax, one segment each (cell,
+// alpha, atom range of the batch arrays; the padding is laid out here).
+MixedState* create_batch(int N, const int kmax[3], const std::vector<MxSeg>& segs);
+// Reciprocal-space field (and, with force, force) of every atom of the
+// batch for sources (q, mu): Erec = -2 (4 pi/V) sum_k k t_k,
+// Frec = q Erec + 2 (4 pi/V) sum_k k (k.mu) s_k (src/field.f90:2746-2801).
+// epoch changes whenever the positions do (the phase tables follow it);
+// cfg is the configuration of each atom.
+void kspace_batch(MixedState* m, hipStream_t st, const double* s, long long epoch,
+                  const int* cfg, const double* q, const double* mu, bool chg, bool dip,
+                  bool force, double* Erec, double* Frec);
 
 }  // namespace mixed
 }  // namespace mdff

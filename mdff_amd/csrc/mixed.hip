@@ -61,17 +61,43 @@ struct MixedState {
   u32x4* btab = nullptr;        // structure-factor B fragments of one evaluation
   long long tables_eval = -1;   // evaluation the phase tables belong to
   size_t X_cap = 0;
+  // configurations on the padded atom axis: none (one configuration, the
+  // context's) or a segment table of the batched path
+  int N = 0;                    // atoms of the caller's arrays
+  int nseg = 1;
+  MxSeg* d_seg = nullptr;
+  int* d_kseg = nullptr;        // segment of each 32-slot group
+  int2* d_split = nullptr;      // [j0, j1) of each structure-factor split
+  double* sinv = nullptr;       // [seg][w][n1] apply scales
+  double* apart = nullptr;      // batched path: U, V sums [9][N]
 };
+
+// Atom of padded slot jp (-1 for padding) and its segment z.  Without a
+// segment table the slots are the atoms 0 .. n-1 of one configuration.
+__device__ __forceinline__ int slot_atom(int jp, int n, const MxSeg* __restrict__ seg,
+                                         const int* __restrict__ kseg, int& z) {
+  if (!seg) {
+    z = 0;
+    return jp < n ? jp : -1;
+  }
+  z = kseg[jp >> 5];
+  const int l = jp - seg[z].poff;
+  return l < seg[z].n ? seg[z].off + l : -1;
+}
 
 // ---------------------------------------------------------------------------
 // phase tables from the FP64 fractional coordinates
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_tables(int n, int npad, const double* __restrict__ s,
+                                                         const MxSeg* __restrict__ seg,
+                                                         const int* __restrict__ kseg,
                                                          int kx, int ky, int kz, float2* tj1,
                                                          float2* tj2, float2* tj3) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= npad) return;
-  const bool act = j < n;
-  const double s1 = act ? s[j] : 0.0, s2 = act ? s[j + n] : 0.0, s3 = act ? s[j + 2 * n] : 0.0;
+  int z;
+  const int i = slot_atom(j, n, seg, kseg, z);
+  const bool act = i >= 0;
+  const double s1 = act ? s[i] : 0.0, s2 = act ? s[i + n] : 0.0, s3 = act ? s[i + 2 * n] : 0.0;
   for (int m = 0; m <= kx; ++m) {
     double sn, cs;
     sincospi(2.0 * m * s1, &sn, &cs);
@@ -101,26 +127,32 @@ __device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
 }
 
 // ---------------------------------------------------------------------------
-// Weights of the structure-factor GEMM, FP32, [w][j] (zero past n):
-// w0 = q (when CHG), then 2^8 * 2 pi B^T mu (when DIP).  Also resets the
-// per-n1 maxima of |g| that mx_rho_reduce accumulates.
+// Weights of the structure-factor GEMM, FP32, [w][slot] (zero on padding):
+// w0 = q (when CHG), then 2^8 * 2 pi B^T mu (when DIP), B of the slot's
+// configuration.  Also resets the per-(configuration, n1) maxima of |g|
+// that mx_rho_reduce accumulates.
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_weights(int n, int npad, int chg, int dip,
                                                           const double* __restrict__ q,
                                                           const double* __restrict__ mu, Cell cell,
+                                                          const MxSeg* __restrict__ seg,
+                                                          const int* __restrict__ kseg,
                                                           float* __restrict__ Wt,
                                                           unsigned int* __restrict__ gmax,
                                                           int nreset) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x < nreset) gmax[threadIdx.x] = 0u;
+  if (blockIdx.x == 0)
+    for (int t = threadIdx.x; t < nreset; t += blockDim.x) gmax[t] = 0u;
   if (j >= npad) return;
+  int z;
+  const int i = slot_atom(j, n, seg, kseg, z);
   int w = 0;
-  if (chg) Wt[(size_t)(w++) * npad + j] = j < n ? (float)q[j] : 0.f;
+  if (chg) Wt[(size_t)(w++) * npad + j] = i >= 0 ? (float)q[i] : 0.f;
   if (dip) {
-    const double* B = cell.B;
+    const double* B = seg ? seg[z].cell.B : cell.B;
     for (int a = 0; a < 3; ++a) {
       float v = 0.f;
-      if (j < n) {
-        const double m = B[3 * a] * mu[j] + B[3 * a + 1] * mu[j + n] + B[3 * a + 2] * mu[j + 2 * n];
+      if (i >= 0) {
+        const double m = B[3 * a] * mu[i] + B[3 * a + 1] * mu[i + n] + B[3 * a + 2] * mu[i + 2 * n];
         v = (float)(m * 6.283185307179586) * DIP_SCALE;
       }
       Wt[(size_t)(w++) * npad + j] = v;
@@ -254,13 +286,14 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_btab(int npad, int N3, int 
 // keep one FP32 chain per split.  Blocks map to
 // (atom split, column block) XCD-aware: the blocks of one XCD take
 // consecutive indices of the split-major order, so each XCD's L2 holds the
-// A fragments and phases of about one split.  Output: part[split][p][col]
-// with S at Re plane col and Im plane C23p + col, for both c and c'.
+// A fragments and phases of about one split; a split is an atom range of
+// one configuration.  Output: part[split][p][col] with S at Re plane col and
+// Im plane C23p + col, for both c and c'.
 constexpr int NXCD = 8;
 
 template <int NRT2, int RTW, bool F64>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1 : F64 ? 2 : 3))) void mx_rho_gemm(
-    int npad, int C23, int CH, int C23p, int nrt_used, int ncb, int nblk_used, int atoms_per_split,
+    int C23, int CH, int C23p, int nrt_used, int ncb, int nblk_used, const int2* __restrict__ splits,
     const u32x4* __restrict__ afrag, const u32x4* __restrict__ btab, double* __restrict__ out) {
   constexpr int RG = (NRT2 + RTW - 1) / RTW;  // row groups
   constexpr int CPB = 4 / RG;                 // column tiles per block
@@ -282,7 +315,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1
   const bool cok = c < CH;
   const int CTt = (CH + 15) / 16;
   // padding row tiles (past NRT2) multiply zeros
-  if (NRTP > NRT2) {
+  if constexpr (NRTP > NRT2) {
     constexpr int PADW = (NRTP - NRT2) * 2 * 64;
     for (int idx = threadIdx.x; idx < 2 * PADW; idx += 256)
       (&sA[idx / PADW][NRT2][0][0])[idx % PADW] = u32x4{0u, 0u, 0u, 0u};
@@ -297,8 +330,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1
 #pragma unroll
       for (int e = 0; e < 4; ++e) accd[y][r][e] = 0.0;
     }
-  const int j0 = split * atoms_per_split;
-  const int j1 = min(npad, j0 + atoms_per_split);  // j1 - j0: a multiple of 128
+  const int j0 = splits[split].x;
+  const int j1 = splits[split].y;  // j1 - j0: a multiple of 128
 
   // A fragments of one K step: ABLK contiguous 16-byte words of afrag,
   // fetched two K steps before they are staged, in a ring of two register
@@ -440,13 +473,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1
 // Sum of the splits in FP64, rho(k) = M_q + i sum_c n_c M_c, g = A_k rho on
 // the half-space mesh (zero elsewhere), energy and stress rows
 // (src/field.f90:2742-2745, 2803-2815), and max |g|.
-// Grid (column blocks, N1): one n1 plane per block row.
+// Grid (column blocks, N1, configurations): one n1 plane of one
+// configuration per block row; g and the maxima are per configuration.
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
     int N1, int N2, int N3, int C23, int C23p, int NW, int chg, int dip, int nsplit, int split_rows,
-    const double* __restrict__ part, Cell cell, double alpha, int do_energy, double2* __restrict__ g,
-    unsigned int* __restrict__ gmax, double* __restrict__ pout) {
+    const double* __restrict__ part, Cell cell_, double alpha_, const MxSeg* __restrict__ seg,
+    int do_energy, double2* __restrict__ g, unsigned int* __restrict__ gmax,
+    double* __restrict__ pout) {
   __shared__ double red[4 * 7];
   __shared__ float wmax[MDFF_BLOCK / 64];
+  const int z = blockIdx.z;
+  const Cell& cell = seg ? seg[z].cell : cell_;
+  const double alpha = seg ? seg[z].alpha : alpha_;
+  const int sp0 = seg ? seg[z].split0 : 0;
+  if (seg) nsplit = seg[z].nsplit;
+  g += (size_t)z * N1 * C23;
+  gmax += (size_t)z * N1;
   const int n1 = blockIdx.y;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int idx = n1 * C23 + c;
@@ -457,7 +499,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
     const int n2 = c / N3 - ky, n3 = c % N3 - kz;
     double Mr[4] = {0, 0, 0, 0}, Mi[4] = {0, 0, 0, 0};
     const size_t sstr = (size_t)split_rows * (2 * C23p);
-    const double* p0 = part + (size_t)n1 * (2 * C23p) + c;
+    const double* p0 = part + (size_t)sp0 * sstr + (size_t)n1 * (2 * C23p) + c;
     for (int w = 0; w < NW; ++w) {
       const double* p = p0 + (size_t)w * N1 * (2 * C23p);
       double vr[4], vi[4];
@@ -583,15 +625,23 @@ __device__ __forceinline__ float apply_wmax(int w, int ky, int kz) {
 // K: k' = 2c + t over CHp columns c:
 //   Re col: B[2c] = Ur, B[2c+1] = -Vi;  Im col: B[2c] = Ui, B[2c+1] = Vr
 // with G = APPLY_SCALE 2^-e(w,n1) w(n) g(n).  Layout [ks][hi|lo][ct][lane] of
-// half8, so that the fragments of one K step are contiguous.
+// half8, so that the fragments of one K step are contiguous; one such block
+// per configuration (grid.y), and sinv[w][n1] = 2^e(w,n1) / APPLY_SCALE.
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, int CH, int CHp,
                                                         int W, int NCT, const double2* __restrict__ g,
                                                         const unsigned int* __restrict__ gmax,
-                                                        int ky, int kz, _Float16* __restrict__ bf) {
+                                                        int ky, int kz, _Float16* __restrict__ bf,
+                                                        double* __restrict__ sinv) {
   const int KS = (2 * CHp) / 32;
   const size_t total = (size_t)NCT * KS * 64;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
+  // configuration blockIdx.y: its g planes, maxima, fragments and scales
+  const int z = blockIdx.y;
+  g += (size_t)z * N1 * C23;
+  gmax += (size_t)z * N1;
+  bf += (size_t)z * total * 16;
+  sinv += (size_t)z * W * N1;
   const int lane = (int)(t % 64);
   const int ct = (int)((t / 64) % NCT);
   const int ks = (int)(t / (64 * (size_t)NCT));
@@ -605,6 +655,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, 
     w = wn / N1;
     n1 = wn % N1;
     cs = class_scale(gmax[n1], apply_wmax(w, ky, kz));
+    if (part == 0 && ks == 0 && lane < 16) sinv[w * N1 + n1] = 1.0 / (double)cs;
   }
   half8 h, l;
 #pragma unroll
@@ -659,7 +710,7 @@ template <int NCT, int RT>
 __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
     int npad, int N3, int CH, int KS, int S, int ksper, int mblocks,
     const float2* __restrict__ tj2, const float2* __restrict__ tj3,
-    const _Float16* __restrict__ bf, float* __restrict__ Xp) {
+    const _Float16* __restrict__ bf, const int* __restrict__ kseg, float* __restrict__ Xp) {
   __shared__ u32x4 sB[2][2 * NCT * 64];
   const int per = NXCD / S;
   const int xcd = blockIdx.x % NXCD;
@@ -682,7 +733,9 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
   constexpr int NT = 64 * APP_WAVES;
   constexpr int PER_FULL = NB / NT;
   constexpr int PER_T = (NB + NT - 1) / NT;
-  const u32x4* bsrc = reinterpret_cast<const u32x4*>(bf);
+  // B fragments of the row block's configuration (rows never straddle one)
+  const int zc = kseg ? kseg[(mb * APP_WAVES * RT * 16) >> 5] : 0;
+  const u32x4* bsrc = reinterpret_cast<const u32x4*>(bf) + (size_t)zc * KS * NB;
   u32x4 pre[PER_T];
 #define MX_FETCH_B(ks)                                                                   \
   {                                                                                      \
@@ -841,22 +894,24 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
 //   V13 = Re S'_n3, V23 = Re S_n2n3,
 // S_w = sum_n1 conj(E1[n1]) X_w[n1], S' with an extra n1, S'' with n1^2.
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int N1, int W,
-                                                            int ncolp, int S, int ky, int kz,
+                                                            int ncolp, int S,
+                                                            const MxSeg* __restrict__ seg,
+                                                            const int* __restrict__ kseg,
                                                             const float* __restrict__ Xp,
                                                             const float2* __restrict__ tj1,
-                                                            const unsigned int* __restrict__ gmax,
+                                                            const double* __restrict__ sinv,
                                                             double* __restrict__ out) {
-  // grid (atom blocks, W): one weight class per thread
+  // grid (slot blocks, W): one weight class per thread, slots of atoms only
   const int w = blockIdx.y;
-  __shared__ double sinv[25];
-  if (threadIdx.x < N1)
-    sinv[threadIdx.x] = 1.0 / (double)class_scale(gmax[threadIdx.x], apply_wmax(w, ky, kz));
-  __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int jp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jp >= npad) return;
+  int z;
+  const int i = slot_atom(jp, n, seg, kseg, z);
+  if (i < 0) return;
+  const double* sv = sinv + (size_t)(z * W + w) * N1;
   const size_t sstride = (size_t)npad * ncolp;
-  const float* xre = Xp + (size_t)(w * N1) * npad + i;
-  const float* xim = Xp + (size_t)(W * N1 + w * N1) * npad + i;
+  const float* xre = Xp + (size_t)(w * N1) * npad + jp;
+  const float* xim = Xp + (size_t)(W * N1 + w * N1) * npad + jp;
   double sr = 0, si = 0, pr = 0, pi = 0, qr = 0;
 #pragma unroll 5
   for (int m = 0; m < N1; ++m) {
@@ -865,9 +920,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int 
       xr += xre[sl * sstride + (size_t)m * npad];
       xi += xim[sl * sstride + (size_t)m * npad];
     }
-    xr *= sinv[m];
-    xi *= sinv[m];
-    const float2 e = tj1[(size_t)m * npad + i];
+    xr *= sv[m];
+    xi *= sv[m];
+    const float2 e = tj1[(size_t)m * npad + jp];
     const double er = e.x, ei = -(double)e.y;  // conj(E1)
     const double vr = er * xr - ei * xi, vi = er * xi + ei * xr;
     sr += vr;
@@ -896,6 +951,50 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int 
 }
 
 // ---------------------------------------------------------------------------
+// Batched path: reciprocal field and force of atom i from the U, V sums of
+// mx_apply_epi with its configuration's lattice (as k_combine64 forms them):
+//   Erec = -2 (4 pi/V) (2 pi) B U,
+//   Frec = q Erec + 2 (4 pi/V) (2 pi)^2 B V B^T mu.
+__global__ __launch_bounds__(MDFF_BLOCK) void mx_batch_fields(int N, const int* __restrict__ cfg,
+                                                               const MxSeg* __restrict__ seg,
+                                                               const double* __restrict__ ap,
+                                                               const double* __restrict__ q,
+                                                               const double* __restrict__ mu,
+                                                               int force, double* __restrict__ Erec,
+                                                               double* __restrict__ Frec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const Cell& cell = seg[cfg[i]].cell;
+  const double* B = cell.B;
+  const double tpi = 2.0 * MDFF_PI;
+  const double fpi_V = 4.0 * MDFF_PI / cell.omega;
+  const double fe = -2.0 * fpi_V * tpi;
+  const double U[3] = {ap[i], ap[i + N], ap[i + 2 * (size_t)N]};
+  double er[3];
+  for (int r = 0; r < 3; ++r) {
+    er[r] = fe * (B[r] * U[0] + B[r + 3] * U[1] + B[r + 6] * U[2]);
+    Erec[i + (size_t)r * N] = er[r];
+  }
+  if (!force) return;
+  double V[6];
+  for (int c = 0; c < 6; ++c) V[c] = ap[i + (size_t)(3 + c) * N];
+  const double Vm[3][3] = {{V[0], V[3], V[4]}, {V[3], V[1], V[5]}, {V[4], V[5], V[2]}};
+  const double fg = 2.0 * fpi_V * tpi * tpi;
+  const double m[3] = {mu[i], mu[i + N], mu[i + 2 * (size_t)N]};
+  const double qi = q[i];
+  for (int r = 0; r < 3; ++r) {
+    double fr = qi * er[r];
+    for (int c = 0; c < 3; ++c) {
+      double G = 0.0;
+      for (int x = 0; x < 3; ++x)
+        for (int y = 0; y < 3; ++y) G += B[r + 3 * x] * Vm[x][y] * B[c + 3 * y];
+      fr += fg * G * m[c];
+    }
+    Frec[i + (size_t)r * N] = fr;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 template <typename T>
@@ -906,6 +1005,7 @@ static void mx_alloc(T** p, size_t count) {
 }
 
 // apply launch shapes: (NCT, RT) and the number of K slices
+constexpr int APP_NCT_S = 4, APP_RT_S = 1;   // small meshes: 2 W N1 <= 64
 constexpr int APP_NCT_F = 10, APP_RT_F = 1;  // field calls (W = 3)
 constexpr int APP_NCT_E = 19, APP_RT_E = 1;  // final call with EFG (W = 6)
 constexpr int APP_ROWS_MAX = APP_WAVES * 2 * 16;
@@ -914,11 +1014,10 @@ constexpr int APP_SLICES = 2;
 // per wave): charges only, dipoles only, charges and dipoles, N1 <= 25
 constexpr int RHO_NRT2_MAX = 13;  // 2 x 4 weights x 25 n1 rows
 
-static bool sizes(MixedState* m) {
-  CtxView v = view(m->ctx);
-  m->kx = v.kmax[0];
-  m->ky = v.kmax[1];
-  m->kz = v.kmax[2];
+static void mesh(MixedState* m, const int kmax[3]) {
+  m->kx = kmax[0];
+  m->ky = kmax[1];
+  m->kz = kmax[2];
   m->N1 = m->kx + 1;
   m->N2 = 2 * m->ky + 1;
   m->N3 = 2 * m->kz + 1;
@@ -926,16 +1025,17 @@ static bool sizes(MixedState* m) {
   m->C23p = (m->C23 + 63) / 64 * 64;  // row stride of the structure-factor partials
   m->CH = (m->C23 + 1) / 2;
   m->CHp = (m->CH + 15) / 16 * 16;
-  // whole row blocks of the apply GEMM (tables are zero past n)
-  m->npad = (v.n + APP_ROWS_MAX - 1) / APP_ROWS_MAX * APP_ROWS_MAX;
-  m->nsplit = (m->npad + RHO_ATOMS_PER_SPLIT - 1) / RHO_ATOMS_PER_SPLIT;
-  if (m->N1 > 25) return false;  // 8 N1 rows fit the 13 row tiles of mx_rho_gemm
-  const size_t np = m->npad;
+}
+
+// buffers for npad slots, nseg configurations and nsplit splits
+static bool allocate(MixedState* m) {
+  const size_t np = m->npad, ns = m->nseg;
   mx_alloc(&m->tj1, (size_t)m->N1 * np);
   mx_alloc(&m->tj2, (size_t)m->N2 * np);
   mx_alloc(&m->tj3, (size_t)m->N3 * np);
-  mx_alloc(&m->g, (size_t)m->N1 * m->C23);
-  mx_alloc(&m->gmax, 32);
+  mx_alloc(&m->g, ns * m->N1 * m->C23);
+  mx_alloc(&m->gmax, ns * m->N1);
+  mx_alloc(&m->sinv, ns * 6 * m->N1);
   m->rpart_cap = (size_t)m->nsplit * RHO_NRT2_MAX * 8 * 2 * m->C23p;
   mx_alloc(&m->rpart, m->rpart_cap);
   mx_alloc(&m->Wt, (size_t)4 * np);
@@ -943,12 +1043,72 @@ static bool sizes(MixedState* m) {
   mx_alloc(&m->btab, (np / 32) * (size_t)((m->CH + 15) / 16) * 4 * 64);
   m->tables_eval = -1;
   const int KS = 2 * m->CHp / 32;
-  m->bfrag_cap = (size_t)2 * APP_NCT_E * KS * 64 * 8;
+  m->bfrag_cap = ns * 2 * APP_NCT_E * KS * 64 * 8;
   mx_alloc(&m->bfrag, m->bfrag_cap);
   m->X_cap = (size_t)APP_SLICES * np * APP_NCT_E * 16;
   mx_alloc(&m->X, m->X_cap);
-  return m->tj1 && m->tj2 && m->tj3 && m->g && m->gmax && m->rpart && m->bfrag && m->X && m->Wt &&
-         m->afrag && m->btab;
+  mx_alloc(&m->d_split, (size_t)m->nsplit);
+  return m->tj1 && m->tj2 && m->tj3 && m->g && m->gmax && m->sinv && m->rpart && m->bfrag &&
+         m->X && m->Wt && m->afrag && m->btab && m->d_split;
+}
+
+static bool sizes(MixedState* m) {
+  CtxView v = view(m->ctx);
+  mesh(m, v.kmax);
+  if (m->N1 > 25) return false;  // 8 N1 rows fit the 13 row tiles of mx_rho_gemm
+  m->N = v.n;
+  m->nseg = 1;
+  // whole row blocks of the apply GEMM (tables are zero past n)
+  m->npad = (v.n + APP_ROWS_MAX - 1) / APP_ROWS_MAX * APP_ROWS_MAX;
+  m->nsplit = (m->npad + RHO_ATOMS_PER_SPLIT - 1) / RHO_ATOMS_PER_SPLIT;
+  if (!allocate(m)) return false;
+  std::vector<int2> sp(m->nsplit);
+  for (int k = 0; k < m->nsplit; ++k)
+    sp[k] = make_int2(k * RHO_ATOMS_PER_SPLIT,
+                      std::min(m->npad, (k + 1) * RHO_ATOMS_PER_SPLIT));
+  return hipMemcpy(m->d_split, sp.data(), sp.size() * sizeof(int2), hipMemcpyHostToDevice) ==
+         hipSuccess;
+}
+
+MixedState* create_batch(int N, const int kmax[3], const std::vector<MxSeg>& segs_in) {
+  MixedState* m = new MixedState();
+  m->ctx = nullptr;
+  mesh(m, kmax);
+  m->N = N;
+  m->nseg = (int)segs_in.size();
+  std::vector<MxSeg> segs = segs_in;
+  std::vector<int2> sp;
+  int poff = 0;
+  for (MxSeg& sg : segs) {
+    sg.poff = poff;
+    sg.np = std::max(APP_ROWS_MAX, (sg.n + APP_ROWS_MAX - 1) / APP_ROWS_MAX * APP_ROWS_MAX);
+    sg.split0 = (int)sp.size();
+    for (int j = 0; j < sg.np; j += RHO_ATOMS_PER_SPLIT)
+      sp.push_back(make_int2(poff + j, poff + std::min(sg.np, j + RHO_ATOMS_PER_SPLIT)));
+    sg.nsplit = (int)sp.size() - sg.split0;
+    poff += sg.np;
+  }
+  m->npad = poff;
+  m->nsplit = (int)sp.size();
+  std::vector<int> kseg(m->npad / 32);
+  for (int z = 0; z < m->nseg; ++z)
+    for (int k = segs[z].poff / 32; k < (segs[z].poff + segs[z].np) / 32; ++k) kseg[k] = z;
+  bool ok = m->N1 <= 25 && allocate(m);
+  mx_alloc(&m->d_seg, segs.size());
+  mx_alloc(&m->d_kseg, kseg.size());
+  mx_alloc(&m->apart, (size_t)9 * N);
+  ok = ok && m->d_seg && m->d_kseg && m->apart;
+  ok = ok && hipMemcpy(m->d_seg, segs.data(), segs.size() * sizeof(MxSeg),
+                       hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(m->d_kseg, kseg.data(), kseg.size() * sizeof(int),
+                       hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(m->d_split, sp.data(), sp.size() * sizeof(int2),
+                       hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) {
+    destroy(m);
+    return nullptr;
+  }
+  return m;
 }
 
 MixedState* create(mdffgpu_ctx* c) {
@@ -964,8 +1124,8 @@ MixedState* create(mdffgpu_ctx* c) {
 
 void destroy(MixedState* m) {
   if (!m) return;
-  void* p[] = {m->tj1, m->tj2, m->tj3, m->rpart, m->g,    m->gmax,
-               m->bfrag, m->X, m->Wt, m->afrag, m->btab};
+  void* p[] = {m->tj1,   m->tj2,  m->tj3,  m->rpart, m->g,      m->gmax,   m->bfrag, m->X,
+               m->Wt,    m->afrag, m->btab, m->sinv,  m->d_seg, m->d_kseg, m->d_split, m->apart};
   for (void* x : p)
     if (x) (void)hipFree(x);
   delete m;
@@ -976,18 +1136,18 @@ void on_cell(MixedState* m) {
 }
 
 template <int NRT2, int RTW, bool F64>
-static void launch_rho(MixedState* m, const CtxView& v, int nrt_used) {
+static void launch_rho(MixedState* m, hipStream_t st, int nrt_used) {
   constexpr int CPB = 4 / ((NRT2 + RTW - 1) / RTW);
   const int ctiles = (m->CH + 15) / 16;
   const int ncb = (ctiles + CPB - 1) / CPB;
   const int used = ncb * m->nsplit;
   const int grid = (used + NXCD - 1) / NXCD * NXCD;
-  mx_rho_gemm<NRT2, RTW, F64><<<grid, 256, 0, v.st>>>(m->npad, m->C23, m->CH, m->C23p, nrt_used, ncb, used,
-                                                 RHO_ATOMS_PER_SPLIT, m->afrag, m->btab, m->rpart);
+  mx_rho_gemm<NRT2, RTW, F64><<<grid, 256, 0, st>>>(m->C23, m->CH, m->C23p, nrt_used, ncb, used,
+                                                    m->d_split, m->afrag, m->btab, m->rpart);
 }
 
 template <int NCT, int RT>
-static void launch_apply(MixedState* m, const CtxView& v) {
+static void launch_apply(MixedState* m, hipStream_t st) {
   const int KS = 2 * m->CHp / 32;
   const int S = APP_SLICES;
   const int ksper = (KS + S - 1) / S;
@@ -995,27 +1155,40 @@ static void launch_apply(MixedState* m, const CtxView& v) {
   const int mblocks = m->npad / rows;
   const int per = NXCD / S;
   const int grid = (mblocks + per - 1) / per * NXCD;
-  mx_apply_gemm<NCT, RT><<<grid, 64 * APP_WAVES, 0, v.st>>>(m->npad, m->N3, m->CH, KS, S, ksper,
-                                                           mblocks, m->tj2, m->tj3, m->bfrag,
-                                                           m->X);
+  mx_apply_gemm<NCT, RT><<<grid, 64 * APP_WAVES, 0, st>>>(m->npad, m->N3, m->CH, KS, S, ksper,
+                                                         mblocks, m->tj2, m->tj3, m->bfrag,
+                                                         m->d_kseg, m->X);
 }
 
-void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg, bool dip,
-            bool efg, double* apart) {
-  CtxView v = view(m->ctx);
-  hipStream_t st = v.st;
-  const int n = v.n;
+// One reciprocal-space call: the context's configuration (energy and stress
+// rows when `energy`) or the configurations of a batch (segment table).
+struct KsCall {
+  hipStream_t st;
+  const double* s;      // fractional coordinates, SoA over the caller's atoms
+  long long epoch;      // positions' version: the phase tables follow it
+  Cell cell;            // one configuration: its lattice and alpha
+  double alpha;
+  bool energy;
+  double* part;         // energy / stress partials and their slots
+  double* sc_rec;
+};
+
+static void run_kspace(MixedState* m, const KsCall& a, const double* q, const double* mu, bool chg,
+                       bool dip, bool efg, double* apart) {
+  hipStream_t st = a.st;
+  const int n = m->N;
+  const MxSeg* seg = m->d_seg;
   // phase tables and structure-factor B fragments: positions are fixed
   // within an evaluation, built on its first reciprocal call
-  if (m->tables_eval != v.evals) {
+  if (m->tables_eval != a.epoch) {
     ProfScope _ps(m->ctx, K_MX_PHASE);
     mx_tables<<<(m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK, 0, st>>>(
-        n, m->npad, v.s, m->kx, m->ky, m->kz, m->tj1, m->tj2, m->tj3);
+        n, m->npad, a.s, seg, m->d_kseg, m->kx, m->ky, m->kz, m->tj1, m->tj2, m->tj3);
     const int CTt = (m->CH + 15) / 16;
     const size_t tb = (size_t)(m->npad / 32) * CTt * 64;
     mx_rho_btab<<<(unsigned)((tb + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0, st>>>(
         m->npad, m->N3, m->CH, CTt, m->tj2, m->tj3, m->btab);
-    m->tables_eval = v.evals;
+    m->tables_eval = a.epoch;
   }
   const int NW = (chg ? 1 : 0) + (dip ? 3 : 0);
   const int NRT2 = (2 * NW * m->N1 + 15) / 16;  // row tiles with data
@@ -1023,7 +1196,7 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
   {
     ProfScope _ps(m->ctx, K_MX_WEIGHTS);
     mx_weights<<<(m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK, 0, st>>>(
-        n, m->npad, chg, dip, q, mu, v.cell, m->Wt, m->gmax, m->N1);
+        n, m->npad, chg, dip, q, mu, a.cell, seg, m->d_kseg, m->Wt, m->gmax, m->nseg * m->N1);
     const size_t at = (size_t)(m->npad / 32) * NRT2l * 64;
     const unsigned ga = (unsigned)((at + MDFF_BLOCK - 1) / MDFF_BLOCK);
     if (NRT2l == 4)
@@ -1038,46 +1211,79 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
     // FP32 chains over a whole split for dipole weights alone; with charges
     // (neutral sums that cancel at small |k|) the chains are flushed to FP64
     if (NRT2l == 4)
-      chg ? launch_rho<4, 4, true>(m, v, NRT2) : launch_rho<4, 4, false>(m, v, NRT2);
+      chg ? launch_rho<4, 4, true>(m, st, NRT2) : launch_rho<4, 4, false>(m, st, NRT2);
     else if (NRT2l == 10)
-      chg ? launch_rho<10, 5, true>(m, v, NRT2) : launch_rho<10, 5, false>(m, v, NRT2);
+      chg ? launch_rho<10, 5, true>(m, st, NRT2) : launch_rho<10, 5, false>(m, st, NRT2);
     else
-      launch_rho<13, 7, true>(m, v, NRT2);
+      launch_rho<13, 7, true>(m, st, NRT2);
   }
-  const dim3 gred((m->C23 + MDFF_BLOCK - 1) / MDFF_BLOCK, m->N1);
+  const dim3 gred((m->C23 + MDFF_BLOCK - 1) / MDFF_BLOCK, m->N1, m->nseg);
   const int gk = gred.x * gred.y;
   {
     ProfScope _ps(m->ctx, K_MX_RHO_RED);
     mx_rho_reduce<<<gred, MDFF_BLOCK, 0, st>>>(m->N1, m->N2, m->N3, m->C23, m->C23p, NW, chg, dip,
-                                             m->nsplit, NRT2l * 8, m->rpart, v.cell,
-                                             v.P->alphaES, mode == ES_FINAL, m->g, m->gmax,
-                                             v.part);
+                                             m->nsplit, NRT2l * 8, m->rpart, a.cell, a.alpha, seg,
+                                             a.energy, m->g, m->gmax, a.part);
   }
-  if (mode == ES_FINAL)
-    k_reduce_rows<<<7, MDFF_BLOCK, 0, st>>>(v.part, gk, v.sc + v.sc_rec);
+  if (a.energy) k_reduce_rows<<<7, MDFF_BLOCK, 0, st>>>(a.part, gk, a.sc_rec);
   const int W = efg ? 6 : 3;
   const int ncols = 2 * W * m->N1;
   const int NCT = (ncols + 15) / 16;
-  const int NCTl = (NCT <= APP_NCT_F) ? APP_NCT_F : APP_NCT_E;
+  const int NCTl = (NCT <= APP_NCT_S) ? APP_NCT_S : (NCT <= APP_NCT_F) ? APP_NCT_F : APP_NCT_E;
   const int KS = 2 * m->CHp / 32;
   {
     ProfScope _ps(m->ctx, K_MX_GPREP);
     const size_t total = (size_t)NCTl * KS * 64;
-    mx_gprep<<<(unsigned)((total + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0, st>>>(
-        m->N1, m->N3, m->C23, m->CH, m->CHp, W, NCTl, m->g, m->gmax, m->ky, m->kz, m->bfrag);
+    const dim3 gg((unsigned)((total + MDFF_BLOCK - 1) / MDFF_BLOCK), m->nseg);
+    mx_gprep<<<gg, MDFF_BLOCK, 0, st>>>(m->N1, m->N3, m->C23, m->CH, m->CHp, W, NCTl, m->g,
+                                        m->gmax, m->ky, m->kz, m->bfrag, m->sinv);
   }
   {
     ProfScope _ps(m->ctx, K_MX_APPLY);
-    if (NCTl == APP_NCT_F)
-      launch_apply<APP_NCT_F, APP_RT_F>(m, v);
+    if (NCTl == APP_NCT_S)
+      launch_apply<APP_NCT_S, APP_RT_S>(m, st);
+    else if (NCTl == APP_NCT_F)
+      launch_apply<APP_NCT_F, APP_RT_F>(m, st);
     else
-      launch_apply<APP_NCT_E, APP_RT_E>(m, v);
+      launch_apply<APP_NCT_E, APP_RT_E>(m, st);
   }
   {
     ProfScope _ps(m->ctx, K_MX_COMBINE);
-    mx_apply_epi<<<dim3((n + MDFF_BLOCK - 1) / MDFF_BLOCK, W), MDFF_BLOCK, 0, st>>>(
-        n, m->npad, m->N1, W, NCTl * 16, APP_SLICES, m->ky, m->kz, m->X, m->tj1, m->gmax, apart);
+    mx_apply_epi<<<dim3((m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, W), MDFF_BLOCK, 0, st>>>(
+        n, m->npad, m->N1, W, NCTl * 16, APP_SLICES, seg, m->d_kseg, m->X, m->tj1, m->sinv, apart);
   }
+}
+
+void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg, bool dip,
+            bool efg, double* apart) {
+  CtxView v = view(m->ctx);
+  KsCall a;
+  a.st = v.st;
+  a.s = v.s;
+  a.epoch = v.evals;
+  a.cell = v.cell;
+  a.alpha = v.P->alphaES;
+  a.energy = (mode == ES_FINAL);
+  a.part = v.part;
+  a.sc_rec = v.sc + v.sc_rec;
+  run_kspace(m, a, q, mu, chg, dip, efg, apart);
+}
+
+void kspace_batch(MixedState* m, hipStream_t st, const double* s, long long epoch,
+                  const int* cfg, const double* q, const double* mu, bool chg, bool dip,
+                  bool force, double* Erec, double* Frec) {
+  KsCall a;
+  a.st = st;
+  a.s = s;
+  a.epoch = epoch;
+  std::memset(&a.cell, 0, sizeof(a.cell));
+  a.alpha = 0.0;
+  a.energy = false;
+  a.part = nullptr;
+  a.sc_rec = nullptr;
+  run_kspace(m, a, q, mu, chg, dip, force, m->apart);
+  mx_batch_fields<<<(m->N + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK, 0, st>>>(
+      m->N, cfg, m->d_seg, m->apart, q, mu, force ? 1 : 0, Erec, Frec);
 }
 
 }  // namespace mixed
