@@ -148,6 +148,10 @@ struct CombineArgs {
   double* ef_out;
   double* efg_out;
   double* f;
+  // ES_IND: SCF residual partials (2 rows) when non-null
+  const int* typ;
+  const TypeTables* T;
+  double* rpart;
 };
 
 struct ExtrapArgs {

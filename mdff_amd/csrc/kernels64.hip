@@ -388,10 +388,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_moments(
 // field and E = E_stat + E_ind is formed; in ES_FINAL mode the forces are
 // added with coul_unit and ef_t / efg_t are written.
 template <int MODE>
-__global__ __launch_bounds__(MDFF_BLOCK) void k_combine64(CombineArgs a) {
+__device__ __forceinline__ void combine_atom(const CombineArgs& a, int i) {
   const int n = a.n;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   const bool EFG = (MODE == ES_FINAL) && a.do_efg;
   const bool EFIELD = (MODE != ES_FINAL) || a.do_efield;
   const int ncomp = (MODE == ES_FINAL && (a.do_efg || a.dip_force)) ? 9 : 3;
@@ -462,6 +460,42 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_combine64(CombineArgs a) {
       if (a.lsurf) v = v + (-(qi * (a.qm[r] + a.qm[3 + r])) * a.fpi_3V);
       a.f[i + r * n] = a.f[i + r * n] + v * a.coul_unit;
     }
+  }
+}
+
+// SCF stopping criterion of atom i (src/field.f90:3378-3400): squared
+// deviations of mu / alpha from the new field, and the polarizable count.
+__device__ __forceinline__ void resid_atom(const CombineArgs& a, int i, double (&acc)[2]) {
+  const int n = a.n;
+  const int t = a.typ[i];
+  if (!a.T->polar[t]) return;
+  acc[1] += 1.0;
+  for (int r = 0; r < 3; ++r) {
+    const double pa = a.T->pol[t][r + 3 * r];
+    if (pa == 0.0) continue;
+    const double d = a.mu[i + r * n] / pa - a.E[i + r * n];
+    acc[0] += d * d;
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_combine64(CombineArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (MODE != ES_IND || !a.rpart) {
+    if (i < a.n) combine_atom<MODE>(a, i);
+    return;
+  }
+  // ES_IND with the SCF residual fused: block partials rows 0, 1 of rpart
+  __shared__ double red[4 * 2];
+  double acc[2] = {0.0, 0.0};
+  if (i < a.n) {
+    combine_atom<MODE>(a, i);
+    resid_atom(a, i, acc);
+  }
+  block_sum<2>(acc, red);
+  if (threadIdx.x == 0) {
+    a.rpart[blockIdx.x] = acc[0];
+    a.rpart[gridDim.x + blockIdx.x] = acc[1];
   }
 }
 

@@ -101,6 +101,8 @@ struct mdffgpu_ctx {
   TypeTables tt_h;
   TTTable tt;
   double u_pol = 0.0;
+  double* d_upart = nullptr;  // block partials of u_pol (reduced once per SCF)
+  bool upol_pending = false;
   double* h_sc = nullptr;  // pinned
   int* h_ovf = nullptr;    // pinned
   hipStream_t own_stream = nullptr;
@@ -352,6 +354,11 @@ void reduce(mdffgpu_ctx* c, int rows, int nb, int slot) {
   check_launch(c, "k_reduce_rows");
 }
 
+void reduce_upol(mdffgpu_ctx* c) {
+  k_reduce_rows<<<1, MDFF_BLOCK, 0, c->st>>>(c->d_upart, nblk(c->n), c->d_sc + SC_UPOL);
+  check_launch(c, "k_reduce_rows");
+}
+
 void alloc_atoms(mdffgpu_ctx* c) {
   const size_t n = c->n;
   dalloc(c, &c->d_pos, 3 * n);
@@ -367,6 +374,7 @@ void alloc_atoms(mdffgpu_ctx* c) {
   dalloc(c, &c->d_muind, 3 * n);
   dalloc(c, &c->d_mu4, n);
   dalloc(c, &c->d_s4, n);
+  dalloc(c, &c->d_upart, nblk(c->n));
   dalloc(c, &c->d_perm, n);
   dalloc(c, &c->d_rank, n);
   dalloc(c, &c->d_mup, 3 * n);
@@ -571,6 +579,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     reduce(c, 8, gb, SC_MOM);
   }
   CombineArgs ca;
+  std::memset(&ca, 0, sizeof(ca));
   ca.n = n;
   ca.nsplit = (c->nchunk > 0) ? nsplit_apply : 1;
   ca.lsurf = P.lsurf;
@@ -602,6 +611,10 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     }
   else if (mode == ES_IND)
     {
+      // the SCF residual of the new field, fused (rows for SC_RMSD)
+      ca.typ = c->d_typ;
+      ca.T = c->d_tt;
+      ca.rpart = c->d_part;
       ProfScope _ps(c, K_COMBINE);
       k_combine64<ES_IND><<<gb, MDFF_BLOCK, 0, st>>>(ca);
     }
@@ -674,7 +687,7 @@ void extrap(mdffgpu_ctx* c, int op, int ord, double w) {
   ea.E = c->d_E;
   ea.typ = c->d_typ;
   ea.T = c->d_tt;
-  ea.part = c->d_part;
+  ea.part = c->d_upart;
   {
     ProfScope _ps(c, K_EXTRAP);
     k_extrap<<<nblk(c->n), MDFF_BLOCK, 0, c->st>>>(ea);
@@ -725,10 +738,9 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
     if (iscf != 1 || P.calc_mode == 0) {
       {
         ProfScope _ps(c, K_INDUCED);
-        k_induced<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, c->d_tt, c->d_E, c->d_muind, c->d_part);
+        k_induced<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, c->d_tt, c->d_E, c->d_muind, c->d_upart);
       }
       check_launch(c, "k_induced");
-      reduce(c, 1, gb, SC_UPOL);
       upol_new = true;
     } else if (P.algo_ext_dipole == MDFFGPU_EXT_POLY) {
       int op = (P.extrapolate_order == 0) ? 0 : (itime > P.extrapolate_order ? 1 : 2);
@@ -741,15 +753,9 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
     if (iscf == 1 && P.calc_mode == 1 && P.algo_ext_dipole == MDFFGPU_EXT_ASPC) {
       const int eo = aspc_order(P, itime);
       extrap(c, 5, eo, W_ASPC[eo]);
-      reduce(c, 1, gb, SC_UPOL);
       upol_new = true;
     }
-    {
-      ProfScope _ps(c, K_RESID);
-      k_scf_resid<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, c->d_tt, c->d_muind, c->d_E, c->d_part);
-    }
-    check_launch(c, "k_scf_resid");
-    reduce(c, 2, gb, SC_RMSD);
+    reduce(c, 2, gb, SC_RMSD);  // residual rows of the ES_IND combine
     sync_scalars(c);
     rmsd = std::sqrt(c->h_sc[SC_RMSD] / c->h_sc[SC_RMSD + 1]);
     if (iscf == 1) rmsd_first = rmsd;
@@ -765,7 +771,12 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
   }
   if (mixed_prec && (stalled || rmsd > tol)) c->mx_floor = rmsd_best;
   extrap(c, 4, 0, 0.0);
-  if (upol_new) c->u_pol = 0.5 * c->h_sc[SC_UPOL];
+  // u_pol of the last induced moments: the partials of the last k_induced /
+  // ASPC corrector, read with the evaluation's final scalars
+  if (upol_new) {
+    reduce_upol(c);
+    c->upol_pending = true;
+  }
   *n_scf = iscf;
   *rmsd_out = rmsd;
 }
@@ -928,6 +939,10 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
     ++n_es;
   }
   sync_scalars(c);
+  if (c->upol_pending) {
+    c->u_pol = 0.5 * c->h_sc[SC_UPOL];
+    c->upol_pending = false;
+  }
   const double* h = c->h_sc;
   const double omega = c->cell.omega;
   if (sr) {
@@ -1236,7 +1251,7 @@ void mdffgpu_destroy(mdffgpu_ctx* c) {
   double* dp[] = {c->d_rref, c->d_pos, c->d_s, c->d_q, c->d_qzero, c->d_dip, c->d_mu, c->d_muind,
                   c->d_mup, c->d_Estat, c->d_E, c->d_efdir, c->d_efgdir, c->d_fdir,
                   c->d_force, c->d_ef, c->d_efg, c->d_apart, c->d_Ak, c->d_kcoe, c->d_kvec,
-                  c->d_part, c->d_sc, c->d_hist};
+                  c->d_part, c->d_sc, c->d_hist, c->d_upart};
   for (double* p : dp)
     if (p) (void)hipFree(p);
   if (c->d_t2) (void)hipFree(c->d_t2);
