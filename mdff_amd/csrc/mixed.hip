@@ -893,6 +893,8 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
 //   V11 = Re S''_1, V22 = Re S_n2n2, V33 = Re S_n3n3, V12 = Re S'_n2,
 //   V13 = Re S'_n3, V23 = Re S_n2n3,
 // S_w = sum_n1 conj(E1[n1]) X_w[n1], S' with an extra n1, S'' with n1^2.
+constexpr int EPI_CHUNKS = 4;  // n1 ranges per slot, one wave each
+
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int N1, int W,
                                                             int ncolp, int S,
                                                             const MxSeg* __restrict__ seg,
@@ -901,35 +903,58 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int 
                                                             const float2* __restrict__ tj1,
                                                             const double* __restrict__ sinv,
                                                             double* __restrict__ out) {
-  // grid (slot blocks, W): one weight class per thread, slots of atoms only
+  // grid (64-slot blocks, W): wave c of the block folds the n1 range c of
+  // 64 consecutive slots for weight class w (more loads in flight than one
+  // thread per slot), then the ranges are summed in order
+  static_assert(MDFF_BLOCK == 64 * EPI_CHUNKS, "one wave per n1 range");
+  __shared__ double part[EPI_CHUNKS - 1][5][64];
   const int w = blockIdx.y;
-  const int jp = blockIdx.x * blockDim.x + threadIdx.x;
-  if (jp >= npad) return;
-  int z;
-  const int i = slot_atom(jp, n, seg, kseg, z);
-  if (i < 0) return;
-  const double* sv = sinv + (size_t)(z * W + w) * N1;
-  const size_t sstride = (size_t)npad * ncolp;
-  const float* xre = Xp + (size_t)(w * N1) * npad + jp;
-  const float* xim = Xp + (size_t)(W * N1 + w * N1) * npad + jp;
+  const int lane = threadIdx.x & 63, ch = threadIdx.x >> 6;
+  const int jp = blockIdx.x * 64 + lane;
+  int z = 0, i = -1;
+  if (jp < npad) i = slot_atom(jp, n, seg, kseg, z);
   double sr = 0, si = 0, pr = 0, pi = 0, qr = 0;
-#pragma unroll 5
-  for (int m = 0; m < N1; ++m) {
-    double xr = 0.0, xi = 0.0;
-    for (int sl = 0; sl < S; ++sl) {
-      xr += xre[sl * sstride + (size_t)m * npad];
-      xi += xim[sl * sstride + (size_t)m * npad];
+  if (i >= 0) {
+    const int per = (N1 + EPI_CHUNKS - 1) / EPI_CHUNKS;
+    const int m0 = ch * per, m1 = min(N1, m0 + per);
+    const double* sv = sinv + (size_t)(z * W + w) * N1;
+    const size_t sstride = (size_t)npad * ncolp;
+    const float* xre = Xp + (size_t)(w * N1) * npad + jp;
+    const float* xim = Xp + (size_t)(W * N1 + w * N1) * npad + jp;
+#pragma unroll 7
+    for (int m = m0; m < m1; ++m) {
+      double xr = 0.0, xi = 0.0;
+      for (int sl = 0; sl < S; ++sl) {
+        xr += xre[sl * sstride + (size_t)m * npad];
+        xi += xim[sl * sstride + (size_t)m * npad];
+      }
+      xr *= sv[m];
+      xi *= sv[m];
+      const float2 e = tj1[(size_t)m * npad + jp];
+      const double er = e.x, ei = -(double)e.y;  // conj(E1)
+      const double vr = er * xr - ei * xi, vi = er * xi + ei * xr;
+      sr += vr;
+      si += vi;
+      pr += m * vr;
+      pi += m * vi;
+      qr += (double)m * m * vr;
     }
-    xr *= sv[m];
-    xi *= sv[m];
-    const float2 e = tj1[(size_t)m * npad + jp];
-    const double er = e.x, ei = -(double)e.y;  // conj(E1)
-    const double vr = er * xr - ei * xi, vi = er * xi + ei * xr;
-    sr += vr;
-    si += vi;
-    pr += m * vr;
-    pi += m * vi;
-    qr += (double)m * m * vr;
+  }
+  if (ch > 0) {
+    part[ch - 1][0][lane] = sr;
+    part[ch - 1][1][lane] = si;
+    part[ch - 1][2][lane] = pr;
+    part[ch - 1][3][lane] = pi;
+    part[ch - 1][4][lane] = qr;
+  }
+  __syncthreads();
+  if (ch > 0 || i < 0) return;
+  for (int c = 0; c < EPI_CHUNKS - 1; ++c) {
+    sr += part[c][0][lane];
+    si += part[c][1][lane];
+    pr += part[c][2][lane];
+    pi += part[c][3][lane];
+    qr += part[c][4][lane];
   }
   switch (w) {
     case 0:
@@ -1249,7 +1274,7 @@ static void run_kspace(MixedState* m, const KsCall& a, const double* q, const do
   }
   {
     ProfScope _ps(m->ctx, K_MX_COMBINE);
-    mx_apply_epi<<<dim3((m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, W), MDFF_BLOCK, 0, st>>>(
+    mx_apply_epi<<<dim3((m->npad + 63) / 64, W), MDFF_BLOCK, 0, st>>>(
         n, m->npad, m->N1, W, NCTl * 16, APP_SLICES, seg, m->d_kseg, m->X, m->tj1, m->sinv, apart);
   }
 }

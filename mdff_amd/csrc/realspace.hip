@@ -155,6 +155,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
           alpha3 = alpha2 * alpha, alpha5 = alpha3 * alpha2;
   const R rpiroot = R(1.0 / sqrt(MDFF_PI));
   const R cut2 = R(BATCH ? a.bc[cf].cut2 : a.cut2);
+  const R c43a3 = R(4) * alpha3 / R(3), c815a5 = R(8) * alpha5 / R(15);  // FP32 constants
   R acc[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) acc[k] = R(0);
@@ -185,17 +186,29 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
       const int pij = ti + MDFF_NT * tj;
       const int pji = tj + MDFF_NT * ti;
       const bool ldamp = DAMP && a.T->dmp_on[o][pij];
-      const R d = sqrt(d2);
+      R d, dm1, dm3, dm5, dm7;
+      if constexpr (sizeof(R) == sizeof(float)) {
+        // FP32: one reciprocal square root, powers by products
+        dm1 = rsqrtf(d2);
+        d = d2 * dm1;
+        const R dm2 = dm1 * dm1;
+        dm3 = dm2 * dm1;
+        dm5 = dm3 * dm2;
+        dm7 = dm5 * dm2;
+      } else {
+        d = sqrt(d2);
+        dm1 = R(1) / d;
+        dm3 = dm1 / d2;
+        dm5 = dm3 / d2;
+        dm7 = dm5 / d2;
+      }
       const R d3 = d2 * d;
       const R d5 = d3 * d2;
-      const R dm1 = R(1) / d;
-      const R dm3 = dm1 / d2;
-      const R dm5 = dm3 / d2;
-      const R dm7 = dm5 / d2;
       const R expon = rexp<R>(-alpha2 * d2) * rpiroot;
       const R F0 = errfc_t<R>(alpha * d);
       const R F1 = F0 + R(2) * alpha * d * expon;
-      const R F2 = F1 + R(4) * alpha3 * d3 * expon / R(3);
+      const R F2 = F1 + (sizeof(R) == sizeof(float) ? c43a3 * d3 * expon
+                                                    : R(4) * alpha3 * d3 * expon / R(3));
       const R F1_dm3 = F1 * dm3;
       const R F2_dm5 = R(3) * F2 * dm5;
       R F1d_i = 0, F2d_i = 0, F1d_j = 0, F2d_j = 0;
@@ -227,7 +240,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
         e2 += rz * rmj * F2_dm5 - mj2 * F1_dm3;
       }
       if (EFG || FORCE) {
-        const R F3 = F2 + R(8) * alpha5 * d5 * expon / R(15);
+        const R F3 = F2 + (sizeof(R) == sizeof(float) ? c815a5 * d5 * expon
+                                                      : R(8) * alpha5 * d5 * expon / R(15));
         const R F3_dm7 = F3 * dm7 * R(15);
         if (EFG) {
           if (QQ) {
