@@ -184,9 +184,13 @@ def batch_leg(args):
     b64.set_model(set_theta(b64.model, th0))
     noise = torch.tensor(0.05 * rng.standard_normal(tuple(f_hidden.shape)), device="cuda")
     target = (f_hidden + noise).contiguous()
-    tune = autotune(b32, target, BATCH_SCF, reps=2)
+    # launch shapes by the autotuner: FP32 matrix-core K slices and split
+    # size, FP64 reciprocal-apply lanes per atom
+    tune32 = autotune(b32, target, BATCH_SCF, reps=2, use_cache=False)
+    tune64 = autotune(b64, target, BATCH_SCF, reps=2, use_cache=False)
     out = {"nconf": len(systems), "natm": natm, "scf_iters": BATCH_SCF,
-           "evals": args.batch_evals, "apply_lanes": tune.get("apply_lanes") if tune else None}
+           "evals": args.batch_evals, "tuning": {"mixed": tune32["best"],
+                                                 "fp64": tune64["best"]}}
     ref = {}
     for key, b in (("fp64", b64), ("mixed", b32)):
         loss, grad, forces = b.loss_grad(target, BATCH_SCF, want_forces=True)

@@ -301,6 +301,14 @@ int mdffgpu_batch_graph(mdffgpu_batch* b, int32_t enable);
    reciprocal-space field/force kernel (2, 4, 8 or 16). */
 int mdffgpu_batch_set_tuning(mdffgpu_batch* b, int32_t apply_lanes);
 
+/* Launch shape of the FP32 batch's matrix-core reciprocal sums, chosen by
+   the autotuner: K slices of the field apply (1, 2 or 4) and atoms per
+   structure-factor split (a multiple of 128).  MDFFGPU_ERR_UNSUPPORTED when
+   the batch has no such path (FP64, or configurations with different kmax).
+   No counterpart in the reference. */
+int mdffgpu_batch_set_mixed_tuning(mdffgpu_batch* b, int32_t apply_slices,
+                                   int32_t split_atoms);
+
 const char* mdffgpu_batch_last_error(const mdffgpu_batch* b);
 
 /* Number of visible HIP devices (0 without a GPU); never fails. */

@@ -51,6 +51,7 @@ PROTOTYPES = {
                                                _dp, _vp]),
     "mdffgpu_batch_graph": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "mdffgpu_batch_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mdffgpu_batch_set_mixed_tuning": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     "mdffgpu_batch_last_error": (ctypes.c_char_p, [_vp]),
 }
 

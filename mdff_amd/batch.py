@@ -26,6 +26,8 @@ import numpy as np
 from . import _lib
 from .params import MdffBatchConfig, MdffParams
 
+_ERR_UNSUPPORTED = 6  # MDFFGPU_ERR_UNSUPPORTED of include/mdffgpu.h
+
 _SR_NAMES = ("A", "B", "C6", "C8", "BD")
 
 
@@ -170,6 +172,17 @@ class Batch:
         self._check(_lib.lib().mdffgpu_batch_set_tuning(self._h, int(apply_lanes)))
         self.apply_lanes = int(apply_lanes)
 
+    def set_mixed_tuning(self, apply_slices: int, split_atoms: int) -> bool:
+        """Launch shape of the FP32 matrix-core reciprocal sums; False when
+        the batch has no such path (FP64, or configurations whose kmax
+        differ)."""
+        st = _lib.lib().mdffgpu_batch_set_mixed_tuning(self._h, int(apply_slices),
+                                                       int(split_atoms))
+        if st == _ERR_UNSUPPORTED:
+            return False
+        self._check(st)
+        return True
+
     # -- evaluation ---------------------------------------------------------
     def forces(self, scf_iters: int = 0):
         """Forces (3, natm) float64 CUDA tensor, and (n_scf, rmsd)."""
@@ -255,13 +268,30 @@ def _key(batch: Batch, scf_iters: int) -> str:
             f"kmax={kmax}|prec={batch.model.precision}|scf={scf_iters}")
 
 
-def autotune(batch: Batch, target, scf_iters: int, candidates=(2, 4, 8, 16), reps: int = 3,
+MIXED_CANDIDATES = tuple({"apply_slices": a, "split_atoms": b}
+                         for a in (1, 2, 4) for b in (512, 1024, 2048))
+FP64_CANDIDATES = tuple({"apply_lanes": k} for k in (2, 4, 8, 16))
+
+
+def _apply(batch: Batch, cand: dict) -> None:
+    if "apply_lanes" in cand:
+        batch.set_tuning(cand["apply_lanes"])
+    else:
+        batch.set_mixed_tuning(cand["apply_slices"], cand["split_atoms"])
+
+
+def autotune(batch: Batch, target, scf_iters: int, candidates=None, reps: int = 3,
              use_cache: bool = True, verbose: bool = False) -> dict:
     """Times one loss+gradient evaluation for every launch shape in
-    candidates and keeps the fastest (apply lanes per atom of the
-    reciprocal-space kernel).  Results are cached per device and batch
-    shape in a JSON file (MDFF_TUNE_CACHE)."""
+    candidates and keeps the fastest: for the FP32 batch the K slices of the
+    matrix-core field apply and the atoms per structure-factor split, for
+    the FP64 batch the lanes per atom of the reciprocal-space kernel.
+    Results are cached per device and batch shape in a JSON file
+    (MDFF_TUNE_CACHE)."""
     import torch
+    if candidates is None:
+        mixed = batch.set_mixed_tuning(2, 2048)
+        candidates = MIXED_CANDIDATES if mixed else FP64_CANDIDATES
     key = _key(batch, scf_iters)
     path = _cache_path()
     cache = {}
@@ -270,25 +300,24 @@ def autotune(batch: Batch, target, scf_iters: int, candidates=(2, 4, 8, 16), rep
             cache = json.load(open(path))
         except (OSError, ValueError):
             cache = {}
-    if use_cache and key in cache:
-        best = cache[key]["apply_lanes"]
-        batch.set_tuning(best)
+    if use_cache and key in cache and "best" in cache[key]:
+        _apply(batch, cache[key]["best"])
         return cache[key]
-    times = {}
-    for lk in candidates:
-        batch.set_tuning(lk)
+    times = []
+    for cand in candidates:
+        _apply(batch, cand)
         batch.loss_grad(target, scf_iters)  # warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
             batch.loss_grad(target, scf_iters)
         torch.cuda.synchronize()
-        times[lk] = (time.perf_counter() - t0) / reps
+        times.append((time.perf_counter() - t0) / reps)
         if verbose:
-            print(f"[autotune] apply_lanes={lk}: {times[lk] * 1e3:.3f} ms")
-    best = min(times, key=times.get)
-    batch.set_tuning(best)
-    rec = {"apply_lanes": best, "ms": {str(k): v * 1e3 for k, v in times.items()}}
+            print(f"[autotune] {cand}: {times[-1] * 1e3:.3f} ms")
+    best = candidates[int(np.argmin(times))]
+    _apply(batch, best)
+    rec = {"best": dict(best), "ms": [[dict(c), t * 1e3] for c, t in zip(candidates, times)]}
     if use_cache:
         cache[key] = rec
         try:

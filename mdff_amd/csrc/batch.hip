@@ -1327,6 +1327,19 @@ int mdffgpu_batch_set_tuning(mdffgpu_batch* b, int32_t apply_lanes) {
   return MDFFGPU_OK;
 }
 
+int mdffgpu_batch_set_mixed_tuning(mdffgpu_batch* b, int32_t apply_slices, int32_t split_atoms) {
+  if (!b) return MDFFGPU_ERR_ARG;
+  if (!b->mx) return MDFFGPU_ERR_UNSUPPORTED;
+  if (b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+    b->graph_iters = -1;
+  }
+  if (b->st) (void)hipStreamSynchronize(b->st);  // buffers may be replaced
+  return mdff::mixed::set_tuning(b->mx, apply_slices, split_atoms) ? MDFFGPU_OK
+                                                                   : MDFFGPU_ERR_ARG;
+}
+
 const char* mdffgpu_batch_last_error(const mdffgpu_batch* b) {
   if (!b) return g_batch_create_error.c_str();
   return b->err.c_str();

@@ -74,6 +74,9 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
 // Batched path: configurations with a common kmax, one segment each (cell,
 // alpha, atom range of the batch arrays; the padding is laid out here).
 MixedState* create_batch(int N, const int kmax[3], const std::vector<MxSeg>& segs);
+// Launch shape: K slices of the apply GEMM (1, 2 or 4) and atoms per
+// structure-factor split (a multiple of 128); false if rejected.
+bool set_tuning(MixedState* m, int app_slices, int split_atoms);
 // Reciprocal-space field (and, with force, force) of every atom of the
 // batch for sources (q, mu): Erec = -2 (4 pi/V) sum_k k t_k,
 // Frec = q Erec + 2 (4 pi/V) sum_k k (k.mu) s_k (src/field.f90:2746-2801).

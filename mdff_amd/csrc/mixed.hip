@@ -30,7 +30,7 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // 16 raw bytes
 
-constexpr int RHO_ATOMS_PER_SPLIT = 2048;
+constexpr int RHO_ATOMS_PER_SPLIT = 2048;  // default atoms per structure-factor split
 constexpr float DIP_SCALE = 256.0f;     // dipole weights are scaled by 2^8
 // The apply operand w g of each n1 plane is normalised to |.| <= APPLY_SCALE
 // (a power of two): most elements of a plane are far below its maximum, and
@@ -70,6 +70,10 @@ struct MixedState {
   int2* d_split = nullptr;      // [j0, j1) of each structure-factor split
   double* sinv = nullptr;       // [seg][w][n1] apply scales
   double* apart = nullptr;      // batched path: U, V sums [9][N]
+  // launch shape (autotuned for batches): K slices of the apply GEMM and
+  // atoms per structure-factor split; the segment table as laid out
+  int app_slices = 2, split_atoms = 2048, x_slices = 0;
+  std::vector<MxSeg> h_seg;
 };
 
 // Atom of padded slot jp (-1 for padding) and its segment z.  Without a
@@ -1034,7 +1038,7 @@ constexpr int APP_NCT_S = 4, APP_RT_S = 1;   // small meshes: 2 W N1 <= 64
 constexpr int APP_NCT_F = 10, APP_RT_F = 1;  // field calls (W = 3)
 constexpr int APP_NCT_E = 19, APP_RT_E = 1;  // final call with EFG (W = 6)
 constexpr int APP_ROWS_MAX = APP_WAVES * 2 * 16;
-constexpr int APP_SLICES = 2;
+constexpr int APP_SLICES_MAX = 4;  // K slices of the apply GEMM: 1, 2 or 4
 // structure-factor GEMM shapes (row tiles of (pair, re|im) rows, row tiles
 // per wave): charges only, dipoles only, charges and dipoles, N1 <= 25
 constexpr int RHO_NRT2_MAX = 13;  // 2 x 4 weights x 25 n1 rows
@@ -1070,11 +1074,30 @@ static bool allocate(MixedState* m) {
   const int KS = 2 * m->CHp / 32;
   m->bfrag_cap = ns * 2 * APP_NCT_E * KS * 64 * 8;
   mx_alloc(&m->bfrag, m->bfrag_cap);
-  m->X_cap = (size_t)APP_SLICES * np * APP_NCT_E * 16;
+  m->x_slices = m->app_slices;
+  m->X_cap = (size_t)m->x_slices * np * APP_NCT_E * 16;
   mx_alloc(&m->X, m->X_cap);
   mx_alloc(&m->d_split, (size_t)m->nsplit);
   return m->tj1 && m->tj2 && m->tj3 && m->g && m->gmax && m->sinv && m->rpart && m->bfrag &&
          m->X && m->Wt && m->afrag && m->btab && m->d_split;
+}
+
+// structure-factor splits of split_atoms slots over the segments (their
+// split0 / nsplit are set), or over [0, npad) without a segment table
+static std::vector<int2> make_splits(MixedState* m) {
+  std::vector<int2> sp;
+  const int sa = m->split_atoms;
+  if (m->h_seg.empty()) {
+    for (int j = 0; j < m->npad; j += sa) sp.push_back(make_int2(j, std::min(m->npad, j + sa)));
+    return sp;
+  }
+  for (MxSeg& sg : m->h_seg) {
+    sg.split0 = (int)sp.size();
+    for (int j = 0; j < sg.np; j += sa)
+      sp.push_back(make_int2(sg.poff + j, sg.poff + std::min(sg.np, j + sa)));
+    sg.nsplit = (int)sp.size() - sg.split0;
+  }
+  return sp;
 }
 
 static bool sizes(MixedState* m) {
@@ -1085,12 +1108,9 @@ static bool sizes(MixedState* m) {
   m->nseg = 1;
   // whole row blocks of the apply GEMM (tables are zero past n)
   m->npad = (v.n + APP_ROWS_MAX - 1) / APP_ROWS_MAX * APP_ROWS_MAX;
-  m->nsplit = (m->npad + RHO_ATOMS_PER_SPLIT - 1) / RHO_ATOMS_PER_SPLIT;
+  const std::vector<int2> sp = make_splits(m);
+  m->nsplit = (int)sp.size();
   if (!allocate(m)) return false;
-  std::vector<int2> sp(m->nsplit);
-  for (int k = 0; k < m->nsplit; ++k)
-    sp[k] = make_int2(k * RHO_ATOMS_PER_SPLIT,
-                      std::min(m->npad, (k + 1) * RHO_ATOMS_PER_SPLIT));
   return hipMemcpy(m->d_split, sp.data(), sp.size() * sizeof(int2), hipMemcpyHostToDevice) ==
          hipSuccess;
 }
@@ -1101,19 +1121,16 @@ MixedState* create_batch(int N, const int kmax[3], const std::vector<MxSeg>& seg
   mesh(m, kmax);
   m->N = N;
   m->nseg = (int)segs_in.size();
-  std::vector<MxSeg> segs = segs_in;
-  std::vector<int2> sp;
+  m->h_seg = segs_in;
+  std::vector<MxSeg>& segs = m->h_seg;
   int poff = 0;
   for (MxSeg& sg : segs) {
     sg.poff = poff;
     sg.np = std::max(APP_ROWS_MAX, (sg.n + APP_ROWS_MAX - 1) / APP_ROWS_MAX * APP_ROWS_MAX);
-    sg.split0 = (int)sp.size();
-    for (int j = 0; j < sg.np; j += RHO_ATOMS_PER_SPLIT)
-      sp.push_back(make_int2(poff + j, poff + std::min(sg.np, j + RHO_ATOMS_PER_SPLIT)));
-    sg.nsplit = (int)sp.size() - sg.split0;
     poff += sg.np;
   }
   m->npad = poff;
+  const std::vector<int2> sp = make_splits(m);
   m->nsplit = (int)sp.size();
   std::vector<int> kseg(m->npad / 32);
   for (int z = 0; z < m->nseg; ++z)
@@ -1134,6 +1151,34 @@ MixedState* create_batch(int N, const int kmax[3], const std::vector<MxSeg>& seg
     return nullptr;
   }
   return m;
+}
+
+bool set_tuning(MixedState* m, int app_slices, int split_atoms) {
+  if (app_slices != 1 && app_slices != 2 && app_slices != 4) return false;
+  if (split_atoms < 128 || split_atoms % 128 != 0) return false;
+  if (app_slices > m->x_slices) {
+    m->x_slices = app_slices;
+    m->X_cap = (size_t)m->x_slices * m->npad * APP_NCT_E * 16;
+    mx_alloc(&m->X, m->X_cap);
+    if (!m->X) return false;
+  }
+  m->app_slices = app_slices;
+  if (split_atoms == m->split_atoms) return true;
+  m->split_atoms = split_atoms;
+  const std::vector<int2> sp = make_splits(m);
+  if ((int)sp.size() > m->nsplit) {
+    mx_alloc(&m->d_split, sp.size());
+    m->rpart_cap = sp.size() * RHO_NRT2_MAX * 8 * 2 * m->C23p;
+    mx_alloc(&m->rpart, m->rpart_cap);
+    if (!m->d_split || !m->rpart) return false;
+  }
+  m->nsplit = (int)sp.size();
+  if (hipMemcpy(m->d_split, sp.data(), sp.size() * sizeof(int2), hipMemcpyHostToDevice) !=
+      hipSuccess)
+    return false;
+  return m->h_seg.empty() ||
+         hipMemcpy(m->d_seg, m->h_seg.data(), m->h_seg.size() * sizeof(MxSeg),
+                   hipMemcpyHostToDevice) == hipSuccess;
 }
 
 MixedState* create(mdffgpu_ctx* c) {
@@ -1174,7 +1219,7 @@ static void launch_rho(MixedState* m, hipStream_t st, int nrt_used) {
 template <int NCT, int RT>
 static void launch_apply(MixedState* m, hipStream_t st) {
   const int KS = 2 * m->CHp / 32;
-  const int S = APP_SLICES;
+  const int S = m->app_slices;
   const int ksper = (KS + S - 1) / S;
   const int rows = APP_WAVES * RT * 16;
   const int mblocks = m->npad / rows;
@@ -1275,7 +1320,8 @@ static void run_kspace(MixedState* m, const KsCall& a, const double* q, const do
   {
     ProfScope _ps(m->ctx, K_MX_COMBINE);
     mx_apply_epi<<<dim3((m->npad + 63) / 64, W), MDFF_BLOCK, 0, st>>>(
-        n, m->npad, m->N1, W, NCTl * 16, APP_SLICES, seg, m->d_kseg, m->X, m->tj1, m->sinv, apart);
+        n, m->npad, m->N1, W, NCTl * 16, m->app_slices, seg, m->d_kseg, m->X, m->tj1, m->sinv,
+        apart);
   }
 }
 

@@ -159,3 +159,30 @@ def test_mixed_batch_against_fp64():
     assert gerr <= 1e-3
     b64.close()
     b32.close()
+
+
+def test_mixed_launch_shapes_agree_and_autotune():
+    """Every launch shape of the FP32 matrix-core reciprocal sums (K slices
+    of the apply, atoms per structure-factor split) gives the same loss and
+    gradient to FP32 summation-order noise; the autotuner keeps one of them;
+    an FP64 batch has no such shape."""
+    from mdff_amd.batch import MIXED_CANDIDATES, Batch, autotune
+    systems = _systems(3, nmax=300, kmax=4, tol=1e-10)
+    b = Batch.from_systems(systems, precision=P.PREC_MIXED)
+    b.set_positions(_positions(systems))
+    target = _target(b, systems)
+    ref = None
+    for slices, split in ((2, 2048), (1, 512), (4, 128)):
+        assert b.set_mixed_tuning(slices, split)
+        loss, grad = b.loss_grad(target, 20)
+        if ref is None:
+            ref = (loss, grad)
+            continue
+        assert abs(loss - ref[0]) <= 1e-6 * abs(ref[0]), (slices, split)
+        assert np.allclose(grad, ref[1], rtol=1e-4, atol=1e-6 * np.max(np.abs(ref[1])))
+    rec = autotune(b, target, 20, reps=1, use_cache=False)
+    assert rec["best"] in [dict(c) for c in MIXED_CANDIDATES]
+    b.close()
+    b64 = Batch.from_systems(systems, precision=P.PREC_FP64)
+    assert not b64.set_mixed_tuning(2, 2048)
+    b64.close()
