@@ -83,13 +83,23 @@ def profile_traffic(kernel):
 def time_path(args, sys_, precision, frames_dev):
     import copy
     import torch
+    from mdff_amd import params as P
     from mdff_amd.field import ForceField
     from mdff_amd import flops as FL
     p = copy.deepcopy(sys_.params)
     p.precision = precision
-    ff = ForceField(p, sys_.itype, sys_.A)
     n = sys_.natm
     force = torch.zeros((3, n), dtype=torch.float64, device="cuda")
+    tuning = None
+    if precision == P.PREC_MIXED:
+        # launch shape of the matrix-core reciprocal sums, picked on a
+        # separate context (its dipole history is not the timed one's)
+        ft = ForceField(p, sys_.itype, sys_.A)
+        tuning = ft.autotune(frames_dev[0], force, itime=1, reps=2)
+        ft.close()
+    ff = ForceField(p, sys_.itype, sys_.A)
+    if tuning:
+        ff.set_mixed_tuning(tuning["apply_slices"], tuning["split_atoms"])
     W, K = args.warmup, args.steps
     for w in range(W):
         ff.engforce_device(frames_dev[w], force, itime=w + 1)
@@ -146,7 +156,9 @@ def time_path(args, sys_, precision, frames_dev):
             roof["traffic_source"] = source
     return {"ms_per_step": dt * 1e3, "atom_steps_per_s": n / dt, "n_scf": nscf,
             "kernels": {k: {"total_ms": v[0], "launches": v[1]} for k, v in stats.items()},
-            "roofline": roof, "force": f_last}
+            "roofline": roof, "force": f_last,
+            "tuning": ({"apply_slices": tuning["apply_slices"],
+                        "split_atoms": tuning["split_atoms"]} if tuning else None)}
 
 
 BATCH_SCF = 20  # fixed SCF iterations of the captured loss+gradient graph
@@ -345,6 +357,8 @@ def main():
             out[key] = {"atom_steps_per_s": r["atom_steps_per_s"], "ms_per_step": r["ms_per_step"],
                         "mean_n_scf": float(np.mean(r["n_scf"])), "roofline": r["roofline"],
                         "kernels": r["kernels"]}
+            if r["tuning"]:
+                out[key]["tuning"] = r["tuning"]
     if err is not None:
         out["mixed_vs_fp64"] = {"speedup": res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"],
                                 "force_rel_rms_err": err}

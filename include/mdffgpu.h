@@ -231,6 +231,12 @@ int mdffgpu_profile_enable(mdffgpu_ctx* ctx, int32_t on); /* also resets */
 /* While enabled, bracket only the launches of one kernel id (-1: all), so
  * that a timed run carries the events of the kernel it reports alone. */
 int mdffgpu_profile_only(mdffgpu_ctx* ctx, int32_t kernel);
+
+/* Launch shape of the mixed path's matrix-core reciprocal sums (picked by
+ * the Python autotuner): K slices of the field apply (1, 2 or 4) and atoms
+ * per structure-factor split (a multiple of 128).  MDFFGPU_ERR_UNSUPPORTED
+ * on an FP64 context.  No counterpart in the reference. */
+int mdffgpu_set_mixed_tuning(mdffgpu_ctx* ctx, int32_t apply_slices, int32_t split_atoms);
 int mdffgpu_profile_read(const mdffgpu_ctx* ctx, int32_t kernel,
                          double* total_ms, int64_t* launches);
 const char* mdffgpu_kernel_name(int32_t kernel);

@@ -34,6 +34,7 @@ PROTOTYPES = {
     "mdffgpu_device_count": (ctypes.c_int32, []),
     "mdffgpu_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "mdffgpu_profile_only": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mdffgpu_set_mixed_tuning": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     "mdffgpu_profile_read": (ctypes.c_int, [_vp, ctypes.c_int32, _dp,
                                             ctypes.POINTER(ctypes.c_int64)]),
     "mdffgpu_kernel_name": (ctypes.c_char_p, [ctypes.c_int32]),

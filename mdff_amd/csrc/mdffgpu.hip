@@ -1370,6 +1370,13 @@ const char* mdffgpu_last_error(const mdffgpu_ctx* c) {
   return c->err.c_str();
 }
 
+int mdffgpu_set_mixed_tuning(mdffgpu_ctx* c, int32_t apply_slices, int32_t split_atoms) {
+  if (!c) return MDFFGPU_ERR_ARG;
+  if (!c->mx) return MDFFGPU_ERR_UNSUPPORTED;
+  if (hipStreamSynchronize(c->st) != hipSuccess) return MDFFGPU_ERR_DEVICE;
+  return mixed::set_tuning(c->mx, apply_slices, split_atoms) ? MDFFGPU_OK : MDFFGPU_ERR_ARG;
+}
+
 int mdffgpu_profile_enable(mdffgpu_ctx* c, int32_t on) {
   if (!c) return MDFFGPU_ERR_ARG;
   c->prof = on ? 1 : 0;

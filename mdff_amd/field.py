@@ -8,6 +8,7 @@ between calls (k mesh, dipole history dipia_ind_t, stale u_pol).
 from __future__ import annotations
 
 import ctypes
+import time
 
 import numpy as np
 
@@ -122,6 +123,41 @@ class ForceField:
         if on and only is not None:
             names = [L.mdffgpu_kernel_name(k).decode() for k in range(L.mdffgpu_kernel_count())]
             _lib.check(L.mdffgpu_profile_only(self._h, names.index(only)), self._h)
+
+    def set_mixed_tuning(self, apply_slices: int, split_atoms: int) -> bool:
+        """Launch shape of the mixed path's matrix-core reciprocal sums;
+        False on an FP64 context."""
+        st = _lib.lib().mdffgpu_set_mixed_tuning(self._h, int(apply_slices), int(split_atoms))
+        if st == 6:  # MDFFGPU_ERR_UNSUPPORTED
+            return False
+        _lib.check(st, self._h)
+        return True
+
+    def autotune(self, pos, force, itime: int = 1, reps: int = 2,
+                 candidates=None) -> dict:
+        """Times engforce_device on one frame for every mixed-path launch
+        shape (apply K slices x structure-factor split size) and keeps the
+        fastest.  Each timed call is an evaluation of that frame (dipole
+        history included), so tune a context that is not mid-trajectory."""
+        import torch
+        if candidates is None:
+            candidates = [(a, b) for a in (1, 2, 4) for b in (1024, 2048, 4096)]
+        if not self.set_mixed_tuning(*candidates[0]):
+            return {}
+        times = []
+        for cand in candidates:
+            self.set_mixed_tuning(*cand)
+            self.engforce_device(pos, force, itime=itime)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                self.engforce_device(pos, force, itime=itime)
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - t0) / reps)
+        best = candidates[int(np.argmin(times))]
+        self.set_mixed_tuning(*best)
+        return {"apply_slices": best[0], "split_atoms": best[1],
+                "ms": [[list(c), t * 1e3] for c, t in zip(candidates, times)]}
 
     def profile_stats(self) -> dict:
         """{kernel name: (total ms, launches)} of the instrumented launches."""
