@@ -43,6 +43,9 @@ def parse():
                     help="skip the batched force-matching gradient leg")
     ap.add_argument("--batch-evals", type=int, default=5,
                     help="timed loss+gradient evaluations of the batch leg")
+    ap.add_argument("--tuning", default=None, metavar="SLICES,SPLIT",
+                    help="fix the mixed path's matrix-core launch shape (apply K slices, "
+                         "structure-factor split atoms) instead of autotuning it")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this size first (calibrates HBM "
@@ -91,15 +94,18 @@ def time_path(args, sys_, precision, frames_dev):
     n = sys_.natm
     force = torch.zeros((3, n), dtype=torch.float64, device="cuda")
     tuning = None
-    if precision == P.PREC_MIXED:
+    if precision == P.PREC_MIXED and args.tuning:
+        sl, sp = (int(x) for x in args.tuning.split(","))
+        tuning = {"apply_slices": sl, "split_atoms": sp}
+    elif precision == P.PREC_MIXED:
         # launch shape of the matrix-core reciprocal sums, picked on a
         # separate context (its dipole history is not the timed one's)
         ft = ForceField(p, sys_.itype, sys_.A)
         tuning = ft.autotune(frames_dev[0], force, itime=1, reps=2)
         ft.close()
     ff = ForceField(p, sys_.itype, sys_.A)
-    if tuning:
-        ff.set_mixed_tuning(tuning["apply_slices"], tuning["split_atoms"])
+    if tuning and not ff.set_mixed_tuning(tuning["apply_slices"], tuning["split_atoms"]):
+        raise SystemExit(f"bench: launch shape {tuning} rejected by the library")
     W, K = args.warmup, args.steps
     for w in range(W):
         ff.engforce_device(frames_dev[w], force, itime=w + 1)
