@@ -58,7 +58,6 @@ struct MixedState {
   float* X = nullptr;           // [slice][i][col]
   float* Wt = nullptr;          // [w][j] structure-factor weights
   u32x4* afrag = nullptr;       // structure-factor A fragments of one call
-  u32x4* btab = nullptr;        // structure-factor B fragments of one evaluation
   long long tables_eval = -1;   // evaluation the phase tables belong to
   size_t X_cap = 0;
   // configurations on the padded atom axis: none (one configuration, the
@@ -72,7 +71,7 @@ struct MixedState {
   double* apart = nullptr;      // batched path: U, V sums [9][N]
   // launch shape (autotuned for batches): K slices of the apply GEMM and
   // atoms per structure-factor split; the segment table as laid out
-  int app_slices = 2, split_atoms = 2048, x_slices = 0;
+  int app_slices = 2, split_atoms = RHO_ATOMS_PER_SPLIT, x_slices = 0;
   std::vector<MxSeg> h_seg;
 };
 
@@ -123,11 +122,25 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_tables(int n, int npad, const d
 // otherwise the compiler may fuse the producing multiply into both
 // conversions (v_fma_mix*_f16 from the exact product for lo, v_cvt_pk_f16_f32
 // of the FP32-rounded product for hi), and at an fp16 rounding tie the two
-// roundings of "hi" differ by one ulp, so hi + lo misses x by an ulp.
+// roundings of "hi" differ by one ulp, so hi + lo misses x by an ulp.  The
+// pin is not volatile: it has no side effects, so the scheduler may move
+// memory operations across it.
 __device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
-  __asm__ volatile("" : "+v"(x));
+  __asm__("" : "+v"(x));
   hi = (_Float16)x;
   lo = (_Float16)(x - (float)hi);
+}
+
+// split16 of two values at once: one packed conversion for both hi parts
+// (v_cvt_pk_f16_f32), their FP32 values back, two residuals and one packed
+// conversion for the lo parts; the same roundings as split16.
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split16x2(float x0, float x1, half2v& hi, half2v& lo) {
+  __asm__("" : "+v"(x0), "+v"(x1));
+  hi = __builtin_convertvector((float2v){x0, x1}, half2v);
+  const float2v hf = __builtin_convertvector(hi, float2v);
+  lo = __builtin_convertvector((float2v){x0 - hf.x, x1 - hf.y}, half2v);
 }
 
 // ---------------------------------------------------------------------------
@@ -216,60 +229,6 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_afrag(int npad, int N1, int
 }
 
 // ---------------------------------------------------------------------------
-// B fragments of the structure-factor GEMM for a whole evaluation (the
-// positions, hence the phases, are fixed during it): per K step g (32
-// atoms), column tile ct (16 columns c < CH), fragment f (Re hi, Re lo,
-// Im hi, Im lo), lane l: column c = ct*16 + l%16, atoms 32 g + 8 (l / 16) +
-// (0..7), values of B = E2[n2] E3[n3] split into fp16 hi + lo; zero past CH.
-__global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_btab(int npad, int N3, int CH, int CTt,
-                                                           const float2* __restrict__ tj2,
-                                                           const float2* __restrict__ tj3,
-                                                           u32x4* __restrict__ btab) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t total = (size_t)(npad / 32) * CTt * 64;
-  if (t >= total) return;
-  const int l = (int)(t & 63);
-  const int ct = (int)((t >> 6) % CTt);
-  const int g = (int)((t >> 6) / CTt);
-  const int c = ct * 16 + (l & 15);
-  const int j = g * 32 + 8 * (l >> 4);
-  half8 rh, rl, ih, il;
-  if (c < CH) {
-    const float4* e2 = reinterpret_cast<const float4*>(tj2 + (size_t)(c / N3) * npad + j);
-    const float4* e3 = reinterpret_cast<const float4*>(tj3 + (size_t)(c % N3) * npad + j);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const float4 a = e2[u], b = e3[u];  // atoms j + 2u, j + 2u + 1 as (re, im, re, im)
-      const float ar[2] = {a.x, a.z}, ai[2] = {a.y, a.w};
-      const float br[2] = {b.x, b.z}, bi[2] = {b.y, b.w};
-#pragma unroll
-      for (int v = 0; v < 2; ++v) {
-        _Float16 h, lo;
-        split16(ar[v] * br[v] - ai[v] * bi[v], h, lo);
-        rh[2 * u + v] = h;
-        rl[2 * u + v] = lo;
-        split16(ar[v] * bi[v] + ai[v] * br[v], h, lo);
-        ih[2 * u + v] = h;
-        il[2 * u + v] = lo;
-      }
-    }
-  } else {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      rh[e] = (_Float16)0.f;
-      rl[e] = (_Float16)0.f;
-      ih[e] = (_Float16)0.f;
-      il[e] = (_Float16)0.f;
-    }
-  }
-  u32x4* o = btab + (((size_t)g * CTt + ct) * 4) * 64 + l;
-  o[0] = __builtin_bit_cast(u32x4, rh);
-  o[64] = __builtin_bit_cast(u32x4, rl);
-  o[128] = __builtin_bit_cast(u32x4, ih);
-  o[192] = __builtin_bit_cast(u32x4, il);
-}
-
-// ---------------------------------------------------------------------------
 // Structure-factor GEMM with conjugate-paired columns.  The (n2, n3) mesh is
 // closed under negation: column c' = C23 - 1 - c has the phases
 // B_c' = E2[-n2] E3[-n3] = conj(B_c).  For the half c < CH, the four real
@@ -281,24 +240,70 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_btab(int npad, int N3, int 
 // at half the MFMA work and half the B operand of complex products over all
 // columns.  GEMM rows (p, x) as row 2p + x (NRT2 tiles), columns (c, y) as
 // two tiles (y = re, im) per 16 columns c, K = atoms (32 per MFMA step).
-// A fragments (mx_rho_afrag) are fetched three K steps ahead, staged through
-// LDS one step ahead and shared by the four waves; a wave owns one column
-// tile and row tiles rg*RTW .. of NRT2, and reads the B fragments of its 16
-// columns from the per-evaluation table of mx_rho_btab, two K steps ahead.
+// The B operand is never stored: B[j][c] = E2_j[n2] E3_j[n3] (c = n2 N3 +
+// n3) is combined and split into fp16 hi + lo in registers from the
+// separable phase tables, each lane its column and 8 atoms (mx_rho_bfrag).
+// Two kernels share this scheme:
+//  - mx_rho_gen (FP32 chains per split, and the 4-tile calls): a wave owns
+//    one column tile and all row tiles, one wave per SIMD, A fragments read
+//    from LDS one step ahead into a second register set;
+//  - mx_rho_gemm (FP64 flushes, up to 13 row tiles): a wave owns one column
+//    tile and RTW row tiles of NRT2.
 // With charge weights (F64) the FP32 MFMA chains are flushed into FP64
 // every 128 atoms: neutral sums cancel at small |k|; dipole weights alone
-// keep one FP32 chain per split.  Blocks map to
-// (atom split, column block) XCD-aware: the blocks of one XCD take
-// consecutive indices of the split-major order, so each XCD's L2 holds the
-// A fragments and phases of about one split; a split is an atom range of
-// one configuration.  Output: part[split][p][col] with S at Re plane col and
-// Im plane C23p + col, for both c and c'.
+// keep one FP32 chain per split.  Blocks map to (atom split, column block)
+// XCD-aware: the blocks of one XCD take consecutive indices of the
+// split-major order, so each XCD's L2 holds the A fragments and phases of
+// about one split; a split is an atom range of one configuration.  Every
+// K step issues the same loads, unconditionally (addresses past the split
+// are clamped to its last step and the data left unused), so that the
+// compiler's count of loads in flight is exact around the loop and a wait
+// for one step's data does not drain the prefetches behind it.  Columns
+// past CH load the last column's phases: a B column only reaches the same
+// D column, and those are not stored.  Output: part[split][p][col] with S
+// at Re plane col and Im plane C23p + col, for both c and c'.
 constexpr int NXCD = 8;
 
+// B fragments (Re hi, Re lo, Im hi, Im lo) of one K step and column from
+// the phases of 8 atoms, e = E2 rows (4 float4 = 8 atoms as re, im pairs)
+// then E3 rows.
+__device__ __forceinline__ void mx_rho_bfrag(half8 (&d)[4], const float4 (&e)[8]) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const float4 a = e[u], b = e[4 + u];  // atoms 2u, 2u + 1 as (re, im, re, im)
+    half2v h, lo;
+    split16x2(a.x * b.x - a.y * b.y, a.z * b.z - a.w * b.w, h, lo);
+    d[0][2 * u] = h.x;
+    d[0][2 * u + 1] = h.y;
+    d[1][2 * u] = lo.x;
+    d[1][2 * u + 1] = lo.y;
+    split16x2(a.x * b.y + a.y * b.x, a.z * b.w + a.w * b.z, h, lo);
+    d[2][2 * u] = h.x;
+    d[2][2 * u + 1] = h.y;
+    d[3][2 * u] = lo.x;
+    d[3][2 * u + 1] = lo.y;
+  }
+}
+
+// the E2, E3 phases of a lane's column for the 8 atoms jb + 8 kgrp + (0..7)
+__device__ __forceinline__ void mx_rho_phases(float4 (&d)[8], const float4* e2row,
+                                              const float4* e3row, int jb, int kgrp) {
+  const int q = (jb + 8 * kgrp) >> 1;  // float4 = two atoms
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    d[u] = e2row[q + u];
+    d[4 + u] = e3row[q + u];
+  }
+}
+
 template <int NRT2, int RTW, bool F64>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1 : F64 ? 2 : 3))) void mx_rho_gemm(
-    int C23, int CH, int C23p, int nrt_used, int ncb, int nblk_used, const int2* __restrict__ splits,
-    const u32x4* __restrict__ afrag, const u32x4* __restrict__ btab, double* __restrict__ out) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    (RTW > 5 || F64) ? 1 : 2))) void mx_rho_gemm(int C23, int CH, int C23p, int nrt_used, int ncb,
+                                                 int nblk_used, const int2* __restrict__ splits,
+                                                 const u32x4* __restrict__ afrag, int npad, int N3,
+                                                 const float2* __restrict__ tj2,
+                                                 const float2* __restrict__ tj3,
+                                                 double* __restrict__ out) {
   constexpr int RG = (NRT2 + RTW - 1) / RTW;  // row groups
   constexpr int CPB = 4 / RG;                 // column tiles per block
   constexpr int NRTP = RG * RTW;              // row tiles incl. padding
@@ -317,7 +322,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1
   const bool tile_ok = ctile * 16 < CH && rt0 < nrt_used;  // wave-uniform
   const int c = ctile * 16 + (lane & 15);
   const bool cok = c < CH;
-  const int CTt = (CH + 15) / 16;
   // padding row tiles (past NRT2) multiply zeros
   if constexpr (NRTP > NRT2) {
     constexpr int PADW = (NRTP - NRT2) * 2 * 64;
@@ -336,6 +340,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1
     }
   const int j0 = splits[split].x;
   const int j1 = splits[split].y;  // j1 - j0: a multiple of 128
+  const int jlast = j1 - 32;
 
   // A fragments of one K step: ABLK contiguous 16-byte words of afrag,
   // fetched two K steps before they are staged, in a ring of two register
@@ -343,46 +348,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1
   constexpr int ABLK = NRT2 * 2 * 64;
   constexpr int APER = (ABLK + 255) / 256;
   u32x4 apre[2][APER];
-#define MX_FETCH_A(jb, slot)                                                             \
-  {                                                                                      \
-    const u32x4* src_ = afrag + (size_t)((jb) / 32) * ABLK;                              \
-    _Pragma("unroll") for (int u = 0; u < APER; ++u) {                                   \
-      const int idx_ = threadIdx.x + u * 256;                                            \
-      if (ABLK % 256 == 0 || idx_ < ABLK) apre[slot][u] = src_[idx_];                    \
-    }                                                                                    \
-  }
-#define MX_STAGE_A(b, slot)                                                              \
-  {                                                                                      \
-    u32x4* dst_ = &sA[b][0][0][0];                                                       \
-    _Pragma("unroll") for (int u = 0; u < APER; ++u) {                                   \
-      const int idx_ = threadIdx.x + u * 256;                                            \
-      if (ABLK % 256 == 0 || idx_ < ABLK) dst_[idx_] = apre[slot][u];                    \
-    }                                                                                    \
-  }
-  // B fragments (Re hi, Re lo, Im hi, Im lo) of a K step from the table,
-  // BPF K steps ahead in a ring of BPF register sets
-  constexpr int BPF = 2;
-  u32x4 bq[BPF][4];
-  // Every K step issues the same loads, unconditionally (addresses past the
-  // split are clamped to its last step and the data left unused), so that
-  // the compiler's count of loads in flight is exact around the loop and a
-  // wait for one step's data does not drain the prefetches behind it.
-  const int ct_ld = min(ctile, CTt - 1);
-  auto load_b = [&](int jb, u32x4 (&d)[4]) {
-    const u32x4* src = btab + (((size_t)(jb / 32) * CTt + ct_ld) * 4) * 64 + lane;
+  auto fetch_a = [&](int jb, int slot) {
+    const u32x4* src = afrag + (size_t)(jb / 32) * ABLK;
 #pragma unroll
-    for (int f = 0; f < 4; ++f) d[f] = src[64 * f];
+    for (int u = 0; u < APER; ++u) {
+      const int idx = threadIdx.x + u * 256;
+      if (ABLK % 256 == 0 || idx < ABLK) apre[slot][u] = src[idx];
+    }
   };
-  const int jlast = j1 - 32;
-
-  if (j0 < j1) {
+  auto stage_a = [&](int buf, int slot) {
+    u32x4* dst = &sA[buf][0][0][0];
 #pragma unroll
-    for (int k = 0; k < BPF; ++k) load_b(min(j0 + 32 * k, jlast), bq[k]);
-    MX_FETCH_A(j0, 0)
-    MX_STAGE_A(0, 0)
-    MX_FETCH_A(min(j0 + 32, jlast), 1)
-    MX_FETCH_A(min(j0 + 64, jlast), 0)
-  }
+    for (int u = 0; u < APER; ++u) {
+      const int idx = threadIdx.x + u * 256;
+      if (ABLK % 256 == 0 || idx < ABLK) dst[idx] = apre[slot][u];
+    }
+  };
+  // B fragments one K step ahead (ring of two), made from phases loaded
+  // EPF steps ahead (EPF divides the 4 unrolled steps: ring index = step
+  // index mod EPF)
+  const int cl = min(c, CH - 1);
+  const int n2c = cl / N3;
+  const float4* e2row = reinterpret_cast<const float4*>(tj2 + (size_t)n2c * npad);
+  const float4* e3row = reinterpret_cast<const float4*>(tj3 + (size_t)(cl - n2c * N3) * npad);
+  constexpr int EPF = RTW > 5 ? 4 : 2;
+  float4 raw[EPF][8];
+  half8 bq[2][4];
+
+  mx_rho_phases(raw[0], e2row, e3row, j0, kgrp);
+  mx_rho_bfrag(bq[0], raw[0]);
+#pragma unroll
+  for (int k = 1; k <= EPF; ++k) mx_rho_phases(raw[k % EPF], e2row, e3row, min(j0 + 32 * k, jlast), kgrp);
+  fetch_a(j0, 0);
+  stage_a(0, 0);
+  fetch_a(min(j0 + 32, jlast), 1);
+  fetch_a(min(j0 + 64, jlast), 0);
   __syncthreads();
   const floatx4 zero = floatx4{0.f, 0.f, 0.f, 0.f};
   // 128 atoms (4 K steps) per FP32 chain, then an FP64 flush; the first K
@@ -391,11 +391,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const int js = jb + 32 * ks;
+      const int cs = ks & 1, ns = cs ^ 1;
       if (tile_ok) {
-        const half8 brh = __builtin_bit_cast(half8, bq[ks % BPF][0]);
-        const half8 brl = __builtin_bit_cast(half8, bq[ks % BPF][1]);
-        const half8 bih = __builtin_bit_cast(half8, bq[ks % BPF][2]);
-        const half8 bil = __builtin_bit_cast(half8, bq[ks % BPF][3]);
         // three passes (lo.hi, then hi.hi and hi.lo), 2 RTW independent
         // accumulators between dependent MFMAs
         half8 al[RTW], ah[RTW];
@@ -406,28 +403,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1
         }
         // all LDS reads issued before the first MFMA waits on one
         __builtin_amdgcn_sched_barrier(0);
+        // the next step's B fragments, between this step's MFMAs
+        mx_rho_bfrag(bq[ns], raw[(ks + 1) % EPF]);
 #pragma unroll
         for (int r = 0; r < RTW; ++r) {
-          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], brh, (ks || !F64) ? acc[0][r] : zero, 0, 0, 0);
-          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], bih, (ks || !F64) ? acc[1][r] : zero, 0, 0, 0);
+          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], bq[cs][0], (ks || !F64) ? acc[0][r] : zero, 0, 0, 0);
+          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[r], bq[cs][2], (ks || !F64) ? acc[1][r] : zero, 0, 0, 0);
         }
 #pragma unroll
         for (int r = 0; r < RTW; ++r) {
-          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], brh, acc[0][r], 0, 0, 0);
-          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bih, acc[1][r], 0, 0, 0);
+          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bq[cs][0], acc[0][r], 0, 0, 0);
+          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bq[cs][2], acc[1][r], 0, 0, 0);
         }
 #pragma unroll
         for (int r = 0; r < RTW; ++r) {
-          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], brl, acc[0][r], 0, 0, 0);
-          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bil, acc[1][r], 0, 0, 0);
+          acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bq[cs][1], acc[0][r], 0, 0, 0);
+          acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[r], bq[cs][3], acc[1][r], 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 6 * RTW; ++i) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+          __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // VALU
         }
       }
-      // the B fragments BPF K steps ahead, the next step's A fragments into
-      // the other LDS buffer, and the A fragments three steps ahead
-      static_assert(4 % BPF == 0, "ring index by the unrolled step");
-      load_b(min(js + 32 * BPF, jlast), bq[ks % BPF]);
-      MX_STAGE_A((ks & 1) ^ 1, (ks & 1) ^ 1)
-      MX_FETCH_A(min(js + 96, jlast), (ks & 1) ^ 1)
+      // phases EPF steps ahead, the next step's A fragments into the other
+      // LDS buffer, and the A fragments three steps ahead
+      static_assert(4 % EPF == 0, "ring index by the unrolled step");
+      mx_rho_phases(raw[(ks + 1) % EPF], e2row, e3row, min(js + 32 * (1 + EPF), jlast), kgrp);
+      stage_a(ns, ns);
+      fetch_a(min(js + 96, jlast), ns);
       __syncthreads();
     }
     if (F64 && tile_ok) {
@@ -439,14 +443,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1
           for (int e = 0; e < 4; ++e) accd[y][r][e] += (double)acc[y][r][e];
     }
   }
-  if (!F64) {
-#pragma unroll
-    for (int y = 0; y < 2; ++y)
-#pragma unroll
-      for (int r = 0; r < RTW; ++r)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) accd[y][r][e] = (double)acc[y][r][e];
-  }
+  // FP32 chains are widened at the store
+  auto P = [&](int y, int r, int e) -> double {
+    if constexpr (F64) return accd[y][r][e];
+    else return (double)acc[y][r][e];
+  };
   if (!tile_ok || !cok) return;
   // D layout: acc[y][r][e] = P_xy of row (rt0 + r)*16 + 4 kgrp + e: e = 0, 1
   // are (re, im) of pair p0, e = 2, 3 of pair p0 + 1
@@ -458,8 +459,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int p = (rt0 + r) * 8 + 2 * kgrp + h;
-      const double Prr = accd[0][r][2 * h], Pir = accd[0][r][2 * h + 1];
-      const double Pri = accd[1][r][2 * h], Pii = accd[1][r][2 * h + 1];
+      const double Prr = P(0, r, 2 * h), Pir = P(0, r, 2 * h + 1);
+      const double Pri = P(1, r, 2 * h), Pii = P(1, r, 2 * h + 1);
       double* op = o + (size_t)p * (2 * C23p);
       op[c] = Prr - Pii;
       op[C23p + c] = Pri + Pir;
@@ -470,8 +471,171 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RTW > 5 ? 1
     }
   }
 }
-#undef MX_FETCH_A
-#undef MX_STAGE_A
+
+// ---------------------------------------------------------------------------
+// Structure-factor GEMM, one wave per SIMD: every wave owns one column
+// tile and all NRT2 row tiles (four column tiles per block, one block per
+// CU).  A fragments: fetched from afrag three K steps ahead, staged into a
+// ring of three LDS buffers two steps ahead, and read into a second
+// register set one step ahead, under the MFMAs of the current step, with
+// the next step's B fragments.  One barrier per K step; the MFMAs run
+// unconditionally (waves past CH work on clamped phases and store
+// nothing), so that a step is one scheduling region.
+template <int NRT2, bool F64>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void mx_rho_gen(
+    int C23, int CH, int C23p, int nrt_used, int ncb, int nblk_used, const int2* __restrict__ splits,
+    const u32x4* __restrict__ afrag, int npad, int N3, const float2* __restrict__ tj2,
+    const float2* __restrict__ tj3, double* __restrict__ out) {
+  constexpr int NBUF = 3;
+  __shared__ u32x4 sA[NBUF][NRT2][2][64];  // [buf][rt][hi|lo][lane], fp16 x8
+  const int per = gridDim.x / NXCD;
+  const int gid = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+  if (gid >= nblk_used) return;  // whole block, before any barrier
+  const int split = gid / ncb, cblk = gid % ncb;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int kgrp = lane >> 4;
+  const int ctile = cblk * 4 + wave;
+  const int c = ctile * 16 + (lane & 15);
+  const int cl = min(c, CH - 1);
+  const int n2c = cl / N3;
+  const float4* e2row = reinterpret_cast<const float4*>(tj2 + (size_t)n2c * npad);
+  const float4* e3row = reinterpret_cast<const float4*>(tj3 + (size_t)(cl - n2c * N3) * npad);
+  const int j0 = splits[split].x;
+  const int j1 = splits[split].y;  // j1 - j0: a multiple of 128
+  const int jlast = j1 - 32;
+
+  constexpr int ABLK = NRT2 * 2 * 64;
+  constexpr int APER = (ABLK + 255) / 256;
+  u32x4 apre[APER];
+  auto fetch_a = [&](int jb) {
+    const u32x4* src = afrag + (size_t)(jb / 32) * ABLK;
+#pragma unroll
+    for (int u = 0; u < APER; ++u) {
+      const int idx = threadIdx.x + u * 256;
+      if (ABLK % 256 == 0 || idx < ABLK) apre[u] = src[idx];
+    }
+  };
+  auto stage_a = [&](int buf) {
+    u32x4* dst = &sA[buf][0][0][0];
+#pragma unroll
+    for (int u = 0; u < APER; ++u) {
+      const int idx = threadIdx.x + u * 256;
+      if (ABLK % 256 == 0 || idx < ABLK) dst[idx] = apre[u];
+    }
+  };
+  half8 ah[2][NRT2], al[2][NRT2];
+  auto read_a = [&](int buf, int set) {
+#pragma unroll
+    for (int r = 0; r < NRT2; ++r) {
+      al[set][r] = __builtin_bit_cast(half8, sA[buf][r][1][lane]);
+      ah[set][r] = __builtin_bit_cast(half8, sA[buf][r][0][lane]);
+    }
+  };
+  // phases EPF steps ahead (EPF divides the 4 unrolled steps), B fragments
+  // one step ahead
+  constexpr int EPF = NRT2 > 4 ? 2 : 4;
+  float4 raw[EPF][8];
+  half8 bq[2][4];  // Re hi, Re lo, Im hi, Im lo
+
+  floatx4 acc[2][NRT2];
+  double accd[2][NRT2][4];
+#pragma unroll
+  for (int y = 0; y < 2; ++y)
+#pragma unroll
+    for (int r = 0; r < NRT2; ++r) {
+      acc[y][r] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) accd[y][r][e] = 0.0;
+    }
+  // prologue: A(0) in registers, A(1) in LDS buffer 1, A(2) fetched, B(0)
+  // made, phases of steps 1..EPF in flight
+  fetch_a(j0);
+  stage_a(0);
+  fetch_a(min(j0 + 32, jlast));
+  stage_a(1);
+  fetch_a(min(j0 + 64, jlast));
+  mx_rho_phases(raw[0], e2row, e3row, j0, kgrp);
+  mx_rho_bfrag(bq[0], raw[0]);
+#pragma unroll
+  for (int k = 1; k <= EPF; ++k) mx_rho_phases(raw[k % EPF], e2row, e3row, min(j0 + 32 * k, jlast), kgrp);
+  __syncthreads();
+  read_a(0, 0);
+  const floatx4 zero = floatx4{0.f, 0.f, 0.f, 0.f};
+  int bnext = 1;  // LDS buffer of step t + 1
+  for (int jb = j0; jb < j1; jb += 128) {
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int js = jb + 32 * ks;
+      const int cs = ks & 1, ns = cs ^ 1;
+      // A(t+1) from LDS and B(t+1) from the phases, under the MFMAs of t
+      read_a(bnext, ns);
+      mx_rho_bfrag(bq[ns], raw[(ks + 1) % EPF]);
+#pragma unroll
+      for (int r = 0; r < NRT2; ++r) {
+        acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[cs][r], bq[cs][0], (ks || !F64) ? acc[0][r] : zero, 0, 0, 0);
+        acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[cs][r], bq[cs][2], (ks || !F64) ? acc[1][r] : zero, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < NRT2; ++r) {
+        acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[cs][r], bq[cs][0], acc[0][r], 0, 0, 0);
+        acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[cs][r], bq[cs][2], acc[1][r], 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < NRT2; ++r) {
+        acc[0][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[cs][r], bq[cs][1], acc[0][r], 0, 0, 0);
+        acc[1][r] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[cs][r], bq[cs][3], acc[1][r], 0, 0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 6 * NRT2; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
+        if (i < 2 * NRT2) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+      }
+      // phases EPF steps ahead; A(t+2) into the LDS buffer A(t-1) left
+      // (read before the last barrier), then A(t+3) fetched
+      static_assert(4 % EPF == 0, "ring index by the unrolled step");
+      mx_rho_phases(raw[(ks + 1) % EPF], e2row, e3row, min(js + 32 * (1 + EPF), jlast), kgrp);
+      const int bstage = bnext == NBUF - 1 ? 0 : bnext + 1;
+      stage_a(bstage);
+      fetch_a(min(js + 96, jlast));
+      bnext = bstage;
+      __syncthreads();
+    }
+    if (F64) {
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int r = 0; r < NRT2; ++r)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) accd[y][r][e] += (double)acc[y][r][e];
+    }
+  }
+  auto P = [&](int y, int r, int e) -> double {
+    if constexpr (F64) return accd[y][r][e];
+    else return (double)acc[y][r][e];
+  };
+  if (ctile * 16 >= CH || c >= CH) return;
+  double* o = out + (size_t)split * (NRT2 * 8) * (2 * C23p);
+  const int cp = C23 - 1 - c;
+#pragma unroll
+  for (int r = 0; r < NRT2; ++r) {
+    if (r >= nrt_used) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int p = r * 8 + 2 * kgrp + h;
+      const double Prr = P(0, r, 2 * h), Pir = P(0, r, 2 * h + 1);
+      const double Pri = P(1, r, 2 * h), Pii = P(1, r, 2 * h + 1);
+      double* op = o + (size_t)p * (2 * C23p);
+      op[c] = Prr - Pii;
+      op[C23p + c] = Pri + Pir;
+      if (cp != c) {  // the centre column (0, 0) pairs with itself
+        op[cp] = Prr + Pii;
+        op[C23p + cp] = Pir - Pri;
+      }
+    }
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Sum of the splits in FP64, rho(k) = M_q + i sum_c n_c M_c, g = A_k rho on
@@ -1038,7 +1202,6 @@ constexpr int APP_NCT_S = 4, APP_RT_S = 1;   // small meshes: 2 W N1 <= 64
 constexpr int APP_NCT_F = 10, APP_RT_F = 1;  // field calls (W = 3)
 constexpr int APP_NCT_E = 19, APP_RT_E = 1;  // final call with EFG (W = 6)
 constexpr int APP_ROWS_MAX = APP_WAVES * 2 * 16;
-constexpr int APP_SLICES_MAX = 4;  // K slices of the apply GEMM: 1, 2 or 4
 // structure-factor GEMM shapes (row tiles of (pair, re|im) rows, row tiles
 // per wave): charges only, dipoles only, charges and dipoles, N1 <= 25
 constexpr int RHO_NRT2_MAX = 13;  // 2 x 4 weights x 25 n1 rows
@@ -1069,7 +1232,6 @@ static bool allocate(MixedState* m) {
   mx_alloc(&m->rpart, m->rpart_cap);
   mx_alloc(&m->Wt, (size_t)4 * np);
   mx_alloc(&m->afrag, (np / 32) * RHO_NRT2_MAX * 2 * 64);
-  mx_alloc(&m->btab, (np / 32) * (size_t)((m->CH + 15) / 16) * 4 * 64);
   m->tables_eval = -1;
   const int KS = 2 * m->CHp / 32;
   m->bfrag_cap = ns * 2 * APP_NCT_E * KS * 64 * 8;
@@ -1079,7 +1241,7 @@ static bool allocate(MixedState* m) {
   mx_alloc(&m->X, m->X_cap);
   mx_alloc(&m->d_split, (size_t)m->nsplit);
   return m->tj1 && m->tj2 && m->tj3 && m->g && m->gmax && m->sinv && m->rpart && m->bfrag &&
-         m->X && m->Wt && m->afrag && m->btab && m->d_split;
+         m->X && m->Wt && m->afrag && m->d_split;
 }
 
 // structure-factor splits of split_atoms slots over the segments (their
@@ -1195,7 +1357,7 @@ MixedState* create(mdffgpu_ctx* c) {
 void destroy(MixedState* m) {
   if (!m) return;
   void* p[] = {m->tj1,   m->tj2,  m->tj3,  m->rpart, m->g,      m->gmax,   m->bfrag, m->X,
-               m->Wt,    m->afrag, m->btab, m->sinv,  m->d_seg, m->d_kseg, m->d_split, m->apart};
+               m->Wt,    m->afrag, m->sinv,  m->d_seg, m->d_kseg, m->d_split, m->apart};
   for (void* x : p)
     if (x) (void)hipFree(x);
   delete m;
@@ -1213,7 +1375,19 @@ static void launch_rho(MixedState* m, hipStream_t st, int nrt_used) {
   const int used = ncb * m->nsplit;
   const int grid = (used + NXCD - 1) / NXCD * NXCD;
   mx_rho_gemm<NRT2, RTW, F64><<<grid, 256, 0, st>>>(m->C23, m->CH, m->C23p, nrt_used, ncb, used,
-                                                    m->d_split, m->afrag, m->btab, m->rpart);
+                                                    m->d_split, m->afrag, m->npad, m->N3, m->tj2,
+                                                    m->tj3, m->rpart);
+}
+
+template <int NRT2, bool F64>
+static void launch_rho_gen(MixedState* m, hipStream_t st, int nrt_used) {
+  const int ctiles = (m->CH + 15) / 16;
+  const int ncb = (ctiles + 3) / 4;
+  const int used = ncb * m->nsplit;
+  const int grid = (used + NXCD - 1) / NXCD * NXCD;
+  mx_rho_gen<NRT2, F64><<<grid, 256, 0, st>>>(m->C23, m->CH, m->C23p, nrt_used, ncb, used,
+                                               m->d_split, m->afrag, m->npad, m->N3, m->tj2,
+                                               m->tj3, m->rpart);
 }
 
 template <int NCT, int RT>
@@ -1248,16 +1422,12 @@ static void run_kspace(MixedState* m, const KsCall& a, const double* q, const do
   hipStream_t st = a.st;
   const int n = m->N;
   const MxSeg* seg = m->d_seg;
-  // phase tables and structure-factor B fragments: positions are fixed
-  // within an evaluation, built on its first reciprocal call
+  // phase tables: positions are fixed within an evaluation, built on its
+  // first reciprocal call
   if (m->tables_eval != a.epoch) {
     ProfScope _ps(m->ctx, K_MX_PHASE);
     mx_tables<<<(m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK, 0, st>>>(
         n, m->npad, a.s, seg, m->d_kseg, m->kx, m->ky, m->kz, m->tj1, m->tj2, m->tj3);
-    const int CTt = (m->CH + 15) / 16;
-    const size_t tb = (size_t)(m->npad / 32) * CTt * 64;
-    mx_rho_btab<<<(unsigned)((tb + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0, st>>>(
-        m->npad, m->N3, m->CH, CTt, m->tj2, m->tj3, m->btab);
     m->tables_eval = a.epoch;
   }
   const int NW = (chg ? 1 : 0) + (dip ? 3 : 0);
@@ -1280,10 +1450,12 @@ static void run_kspace(MixedState* m, const KsCall& a, const double* q, const do
     ProfScope _ps(m->ctx, K_MX_RHO);
     // FP32 chains over a whole split for dipole weights alone; with charges
     // (neutral sums that cancel at small |k|) the chains are flushed to FP64
+    // one wave per SIMD over all row tiles where the registers allow (FP32
+    // chains, or 4 row tiles), row groups with FP64 flushes otherwise
     if (NRT2l == 4)
-      chg ? launch_rho<4, 4, true>(m, st, NRT2) : launch_rho<4, 4, false>(m, st, NRT2);
+      chg ? launch_rho_gen<4, true>(m, st, NRT2) : launch_rho_gen<4, false>(m, st, NRT2);
     else if (NRT2l == 10)
-      chg ? launch_rho<10, 5, true>(m, st, NRT2) : launch_rho<10, 5, false>(m, st, NRT2);
+      chg ? launch_rho<10, 5, true>(m, st, NRT2) : launch_rho_gen<10, false>(m, st, NRT2);
     else
       launch_rho<13, 7, true>(m, st, NRT2);
   }
