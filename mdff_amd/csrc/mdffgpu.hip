@@ -104,6 +104,7 @@ struct mdffgpu_ctx {
   double* d_upart = nullptr;  // block partials of u_pol (reduced once per SCF)
   bool upol_pending = false;
   double* h_sc = nullptr;  // pinned
+  hipEvent_t ev_sc = nullptr;  // the scalars' copy into h_sc has landed
   int* h_ovf = nullptr;    // pinned
   hipStream_t own_stream = nullptr;
   hipStream_t st = nullptr;
@@ -132,9 +133,13 @@ const char* const kKernelNames[K_COUNT] = {
 ProfScope::ProfScope(mdffgpu_ctx* ctx, int kid) : c(ctx), id(kid), a(nullptr), b(nullptr) {
   if (!c || !c->prof || (c->prof_only >= 0 && c->prof_only != kid)) return;
   if (c->ev_pool.size() < 2) {
+    // timing only, read after a stream synchronisation: no system-scope
+    // fence at record (it writes back and invalidates the caches, a gap of
+    // several microseconds before the next launch)
     for (int k = 0; k < 2; ++k) {
       hipEvent_t e;
-      if (hipEventCreate(&e) == hipSuccess) c->ev_pool.push_back(e);
+      if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) == hipSuccess)
+        c->ev_pool.push_back(e);
     }
   }
   if (c->ev_pool.size() < 2) return;
@@ -419,8 +424,43 @@ void update_cutoffs(mdffgpu_ctx* c) {
 
 // ---------------------------------------------------------------------------
 // One multipole_ES call on device buffers.
+// Real-space part of the SCF dipole field (ES_IND): E_dir of the dipoles mu
+// into d_efdir, from the pair table of the evaluation.
+void es_ind_dir(mdffgpu_ctx* c, const double* mu) {
+  const int n = c->n;
+  hipStream_t st = c->st;
+  const mdffgpu_params& P = c->P;
+  const int gb = nblk(n);
+  const bool mxf = (P.precision == MDFFGPU_PREC_MIXED) &&
+                   !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_SCF_FIELD);
+  ProfScope _ps(c, K_DD_FIELD);
+  if (mxf) {
+    float4* m4 = reinterpret_cast<float4*>(c->d_mu4);
+    const size_t cap = (size_t)c->maxnb * n;
+    const float4* tab = reinterpret_cast<const float4*>(c->d_t2);
+    const int* tj = reinterpret_cast<const int*>(tab + cap);
+    k_pack_mu_ranked<<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_rank, m4);
+    k_dd_tab<<<nblk(n * 64), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4, c->d_perm,
+                                                  c->d_efdir);
+  } else {
+    EsDirArgs da;
+    std::memset(&da, 0, sizeof(da));
+    da.n = n;
+    da.maxnb = c->maxnb;
+    da.cnt = c->d_cnt;
+    da.nbr = c->d_nbr;
+    da.perm = c->d_perm;
+    da.ef = c->d_efdir;
+    k_pack_mu<double4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_mu4);
+    k_dd_field<<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, (const double*)c->d_t2, c->d_mu4);
+  }
+  check_launch(c, "k_dd_field");
+}
+
+// One Ewald call; with dir_done (ES_IND only) the real-space part has
+// already been launched (es_ind_dir) for these dipoles.
 void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int task,
-             int do_efield, int do_efg) {
+             int do_efield, int do_efg, bool dir_done = false) {
   const int n = c->n;
   hipStream_t st = c->st;
   const mdffgpu_params& P = c->P;
@@ -453,7 +493,6 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   da.part = c->d_part;
   const bool mixed = (P.precision == MDFFGPU_PREC_MIXED);
   const bool mx = mixed && !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_DIRECT);
-  const bool mxf = mixed && !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_SCF_FIELD);
   const bool mxk = mixed && !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_KSPACE);
   if (mode == ES_STATIC) {
     ProfScope _ps(c, mx ? K_MX_DIR : K_DIR_STATIC);
@@ -462,19 +501,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     else
       k_es_dir<ES_STATIC, double><<<gbl, MDFF_BLOCK, 0, st>>>(da);
   } else if (mode == ES_IND) {
-    ProfScope _ps(c, K_DD_FIELD);
-    if (mxf) {
-      float4* m4 = reinterpret_cast<float4*>(c->d_mu4);
-      const size_t cap = (size_t)c->maxnb * n;
-      const float4* tab = reinterpret_cast<const float4*>(c->d_t2);
-      const int* tj = reinterpret_cast<const int*>(tab + cap);
-      k_pack_mu_ranked<<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_rank, m4);
-      k_dd_tab<<<nblk(n * 64), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4,
-                                                    c->d_perm, c->d_efdir);
-    } else {
-      k_pack_mu<double4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_mu4);
-      k_dd_field<<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, (const double*)c->d_t2, c->d_mu4);
-    }
+    if (!dir_done) es_ind_dir(c, mu);
   } else {
     ProfScope _ps(c, mx ? K_MX_DIR : K_DIR_FINAL);
     if (mx)
@@ -733,22 +760,39 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
   bool stalled = false;
   double rmsd_best = rmsd, rmsd_first = 0.0;
   int no_progress = 0;
+  // Mixed precision: while the host waits for an iteration's residual, the
+  // next iteration's first part is already queued behind it (new moments
+  // mu = alpha E and their real-space field), so that the GPU does not idle
+  // through the round trip.  A solve that stops there keeps those moments:
+  // one Jacobi step past the accepted iterate, with a change below the
+  // residual just accepted.  The FP64 path follows the reference's
+  // sequence exactly.
+  const bool spec = mixed_prec;
+  if (spec && !c->ev_sc)
+    HCK(c, hipEventCreateWithFlags(&c->ev_sc, hipEventDisableTiming));
+  auto induced = [&]() {
+    {
+      ProfScope _ps(c, K_INDUCED);
+      k_induced<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, c->d_tt, c->d_E, c->d_muind, c->d_upart);
+    }
+    check_launch(c, "k_induced");
+    upol_new = true;
+  };
+  bool queued = false;  // this iteration's moments and real-space field are queued
   while (((iscf < P.max_scf_pol_iter && rmsd > tol) || iscf < P.min_scf_pol_iter) && !stalled) {
     ++iscf;
-    if (iscf != 1 || P.calc_mode == 0) {
-      {
-        ProfScope _ps(c, K_INDUCED);
-        k_induced<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, c->d_tt, c->d_E, c->d_muind, c->d_upart);
-      }
-      check_launch(c, "k_induced");
-      upol_new = true;
+    if (queued) {
+      // k_induced and the real-space field were queued by the last iteration
+    } else if (iscf != 1 || P.calc_mode == 0) {
+      induced();
     } else if (P.algo_ext_dipole == MDFFGPU_EXT_POLY) {
       int op = (P.extrapolate_order == 0) ? 0 : (itime > P.extrapolate_order ? 1 : 2);
       extrap(c, op, 0, 0.0);
     } else {
       extrap(c, 3, aspc_order(P, itime), 0.0);
     }
-    es_call(c, ES_IND, c->d_qzero, c->d_muind, TASK_DD, 1, 0);
+    es_call(c, ES_IND, c->d_qzero, c->d_muind, TASK_DD, 1, 0, queued);
+    queued = false;
     ++*n_es;
     if (iscf == 1 && P.calc_mode == 1 && P.algo_ext_dipole == MDFFGPU_EXT_ASPC) {
       const int eo = aspc_order(P, itime);
@@ -756,7 +800,18 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
       upol_new = true;
     }
     reduce(c, 2, gb, SC_RMSD);  // residual rows of the ES_IND combine
-    sync_scalars(c);
+    if (spec) {
+      HCK(c, hipMemcpyAsync(c->h_sc, c->d_sc, SC_N * sizeof(double), hipMemcpyDeviceToHost, st));
+      HCK(c, hipEventRecord(c->ev_sc, st));
+      induced();
+      es_ind_dir(c, c->d_muind);
+      queued = true;
+      // the profiling events of the queued launches are folded in at the
+      // next stream synchronisation
+      HCK(c, hipEventSynchronize(c->ev_sc));
+    } else {
+      sync_scalars(c);
+    }
     rmsd = std::sqrt(c->h_sc[SC_RMSD] / c->h_sc[SC_RMSD + 1]);
     if (iscf == 1) rmsd_first = rmsd;
     if (rmsd < rmsd_best) {
@@ -1264,6 +1319,7 @@ void mdffgpu_destroy(mdffgpu_ctx* c) {
     (void)hipEventDestroy(p.second.second);
   }
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->ev_sc) (void)hipEventDestroy(c->ev_sc);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
   if (c->h_ovf) (void)hipHostFree(c->h_ovf);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
