@@ -778,7 +778,8 @@ int jacobi(mdffgpu_batch* b, const double* S, double* x, double* E, int iters, d
   const int nmax = iters > 0 ? iters : std::max(P.max_scf_pol_iter, 1);
   while (it < nmax) {
     ++it;
-    k_induced<<<gb, MDFF_BLOCK, 0, b->st>>>(N, b->d_typ, b->d_ty, E, x, b->d_part);
+    k_induced<<<gb, MDFF_BLOCK, 0, b->st>>>(N, b->d_typ, b->d_ty, E, x, b->d_part, nullptr,
+                                            nullptr);
     es_call<R>(b, ES_IND, b->d_zero, x, S, E);
     if (iters == 0) {
       k_scf_resid<<<gb, MDFF_BLOCK, 0, b->st>>>(N, b->d_typ, b->d_ty, x, E, b->d_part);

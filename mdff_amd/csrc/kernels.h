@@ -195,11 +195,12 @@ __global__ void k_moments(int n, const double* pos, const double* q, const doubl
 template <int MODE>
 __global__ void k_combine64(CombineArgs a);
 __global__ void k_induced(int n, const int* typ, const TypeTables* T, const double* E,
-                          double* mu, double* part);
+                          double* mu, double* part, const int* rank, float4* mu4);
 __global__ void k_scf_resid(int n, const int* typ, const TypeTables* T, const double* mu,
                             const double* E, double* part);
 __global__ void k_extrap(ExtrapArgs a);
 __global__ void k_reduce_rows(const double* part, int nb, double* out);
+__global__ void k_reduce_rows2(const double* part, int nb, double* out, double* out2);
 __global__ void k_disp(int n, const double* pos, const double* ref, double* part);
 __global__ void k_reduce_max(const double* part, int nb, double* out);
 

@@ -506,9 +506,12 @@ template __global__ void k_combine64<ES_FINAL>(CombineArgs);
 // ---------------------------------------------------------------------------
 // induced_moment (src/field.f90:1716-1803): mu = alpha E on polarizable
 // atoms; partial row 0 = sum mu alpha^-1 mu (u_pol before the 1/2).
+// With mu4 (the FP32 SCF field of the mixed path), the moments are also
+// stored as float4 at the atom's spatial rank (k_pack_mu_ranked fused).
 __global__ __launch_bounds__(MDFF_BLOCK) void k_induced(
     int n, const int* __restrict__ typ, const TypeTables* __restrict__ T,
-    const double* __restrict__ E, double* __restrict__ mu, double* __restrict__ part) {
+    const double* __restrict__ E, double* __restrict__ mu, double* __restrict__ part,
+    const int* __restrict__ rank, float4* __restrict__ mu4) {
   __shared__ double red[4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[1] = {0.0};
@@ -525,6 +528,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_induced(
     mu[i] = m[0];
     mu[i + n] = m[1];
     mu[i + 2 * n] = m[2];
+    if (mu4) mu4[rank[i]] = make_float4((float)m[0], (float)m[1], (float)m[2], 0.f);
   }
   block_sum<1>(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
@@ -658,6 +662,21 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_reduce_rows(
   for (int b = threadIdx.x; b < nb; b += blockDim.x) acc[0] += part[(size_t)r * nb + b];
   block_sum<1>(acc, red);
   if (threadIdx.x == 0) out[r] = acc[0];
+}
+
+// k_reduce_rows into two places: out and out2 (the device address of pinned
+// host memory, read by the host after a fenced event: no copy launch)
+__global__ __launch_bounds__(MDFF_BLOCK) void k_reduce_rows2(
+    const double* __restrict__ part, int nb, double* __restrict__ out, double* out2) {
+  __shared__ double red[4];
+  const int r = blockIdx.x;
+  double acc[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc[0] += part[(size_t)r * nb + b];
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) {
+    out[r] = acc[0];
+    out2[r] = acc[0];
+  }
 }
 
 // out = a + b elementwise (total dipoles mu = dipia + mu_ind)

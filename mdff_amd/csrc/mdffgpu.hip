@@ -104,7 +104,8 @@ struct mdffgpu_ctx {
   double* d_upart = nullptr;  // block partials of u_pol (reduced once per SCF)
   bool upol_pending = false;
   double* h_sc = nullptr;  // pinned
-  hipEvent_t ev_sc = nullptr;  // the scalars' copy into h_sc has landed
+  double* h_sc_dev = nullptr;  // device address of h_sc
+  hipEvent_t ev_sc = nullptr;  // the SCF residual has landed in h_sc
   int* h_ovf = nullptr;    // pinned
   hipStream_t own_stream = nullptr;
   hipStream_t st = nullptr;
@@ -424,22 +425,26 @@ void update_cutoffs(mdffgpu_ctx* c) {
 
 // ---------------------------------------------------------------------------
 // One multipole_ES call on device buffers.
+// FP32 SCF dipole field of the mixed path (packed rank-ordered dipoles)
+bool scf_field_fp32(const mdffgpu_ctx* c) {
+  return (c->P.precision == MDFFGPU_PREC_MIXED) &&
+         !(c->P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_SCF_FIELD);
+}
+
 // Real-space part of the SCF dipole field (ES_IND): E_dir of the dipoles mu
-// into d_efdir, from the pair table of the evaluation.
-void es_ind_dir(mdffgpu_ctx* c, const double* mu) {
+// into d_efdir, from the pair table of the evaluation.  With packed, the
+// FP32 rank-ordered copy of mu is already in d_mu4 (k_induced wrote it).
+void es_ind_dir(mdffgpu_ctx* c, const double* mu, bool packed = false) {
   const int n = c->n;
   hipStream_t st = c->st;
-  const mdffgpu_params& P = c->P;
   const int gb = nblk(n);
-  const bool mxf = (P.precision == MDFFGPU_PREC_MIXED) &&
-                   !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_SCF_FIELD);
   ProfScope _ps(c, K_DD_FIELD);
-  if (mxf) {
+  if (scf_field_fp32(c)) {
     float4* m4 = reinterpret_cast<float4*>(c->d_mu4);
     const size_t cap = (size_t)c->maxnb * n;
     const float4* tab = reinterpret_cast<const float4*>(c->d_t2);
     const int* tj = reinterpret_cast<const int*>(tab + cap);
-    k_pack_mu_ranked<<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_rank, m4);
+    if (!packed) k_pack_mu_ranked<<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_rank, m4);
     k_dd_tab<<<nblk(n * 64), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4, c->d_perm,
                                                   c->d_efdir);
   } else {
@@ -768,23 +773,29 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
   // residual just accepted.  The FP64 path follows the reference's
   // sequence exactly.
   const bool spec = mixed_prec;
-  if (spec && !c->ev_sc)
+  if (spec && !c->ev_sc) {
     HCK(c, hipEventCreateWithFlags(&c->ev_sc, hipEventDisableTiming));
-  auto induced = [&]() {
+    HCK(c, hipHostGetDevicePointer((void**)&c->h_sc_dev, c->h_sc, 0));
+  }
+  // with the FP32 SCF field, k_induced also packs the moments for k_dd_tab
+  auto induced = [&](bool pack) {
     {
       ProfScope _ps(c, K_INDUCED);
-      k_induced<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, c->d_tt, c->d_E, c->d_muind, c->d_upart);
+      k_induced<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, c->d_tt, c->d_E, c->d_muind, c->d_upart,
+                                           c->d_rank,
+                                           pack ? reinterpret_cast<float4*>(c->d_mu4) : nullptr);
     }
     check_launch(c, "k_induced");
     upol_new = true;
   };
+  const bool pack = spec && scf_field_fp32(c);
   bool queued = false;  // this iteration's moments and real-space field are queued
   while (((iscf < P.max_scf_pol_iter && rmsd > tol) || iscf < P.min_scf_pol_iter) && !stalled) {
     ++iscf;
     if (queued) {
       // k_induced and the real-space field were queued by the last iteration
     } else if (iscf != 1 || P.calc_mode == 0) {
-      induced();
+      induced(false);
     } else if (P.algo_ext_dipole == MDFFGPU_EXT_POLY) {
       int op = (P.extrapolate_order == 0) ? 0 : (itime > P.extrapolate_order ? 1 : 2);
       extrap(c, op, 0, 0.0);
@@ -799,17 +810,23 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
       extrap(c, 5, eo, W_ASPC[eo]);
       upol_new = true;
     }
-    reduce(c, 2, gb, SC_RMSD);  // residual rows of the ES_IND combine
     if (spec) {
-      HCK(c, hipMemcpyAsync(c->h_sc, c->d_sc, SC_N * sizeof(double), hipMemcpyDeviceToHost, st));
+      // residual rows of the ES_IND combine, straight into pinned memory
+      {
+        ProfScope _ps(c, K_REDUCE);
+        k_reduce_rows2<<<2, MDFF_BLOCK, 0, st>>>(c->d_part, gb, c->d_sc + SC_RMSD,
+                                                 c->h_sc_dev + SC_RMSD);
+      }
+      check_launch(c, "k_reduce_rows2");
       HCK(c, hipEventRecord(c->ev_sc, st));
-      induced();
-      es_ind_dir(c, c->d_muind);
+      induced(pack);
+      es_ind_dir(c, c->d_muind, pack);
       queued = true;
       // the profiling events of the queued launches are folded in at the
       // next stream synchronisation
       HCK(c, hipEventSynchronize(c->ev_sc));
     } else {
+      reduce(c, 2, gb, SC_RMSD);  // residual rows of the ES_IND combine
       sync_scalars(c);
     }
     rmsd = std::sqrt(c->h_sc[SC_RMSD] / c->h_sc[SC_RMSD + 1]);
