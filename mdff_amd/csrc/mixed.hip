@@ -67,6 +67,7 @@ struct MixedState {
   MxSeg* d_seg = nullptr;
   int* d_kseg = nullptr;        // segment of each 32-slot group
   int2* d_split = nullptr;      // [j0, j1) of each structure-factor split
+  int2* d_colofs = nullptr;     // apply: phase-table row offsets of each column
   double* sinv = nullptr;       // [seg][w][n1] apply scales
   double* apart = nullptr;      // batched path: U, V sums [9][N]
   // launch shape (autotuned for batches): K slices of the apply GEMM and
@@ -876,7 +877,7 @@ constexpr int APP_WAVES = 4;
 
 template <int NCT, int RT>
 __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
-    int npad, int N3, int CH, int KS, int S, int ksper, int mblocks,
+    int npad, int KS, int S, int ksper, int mblocks, const int2* __restrict__ colofs,
     const float2* __restrict__ tj2, const float2* __restrict__ tj3,
     const _Float16* __restrict__ bf, const int* __restrict__ kseg, float* __restrict__ Xp) {
   __shared__ u32x4 sB[2][2 * NCT * 64];
@@ -929,28 +930,21 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
   }
   __syncthreads();
   // phases E2, E3 of the K step's columns c = 16 ks + 4 kgrp + e for the RT
-  // atoms of this lane; columns past CH read column 0 and are zeroed
+  // atoms of this lane, through the column table (columns past CH read the
+  // last column: the operand rows they meet are zero)
   float2 pe2[RT][4], pe3[RT][4];
-  unsigned okm = 0;
   auto load_e = [&](int ks) {
-    const int cb = ks * 16 + 4 * kgrp;
-    int n2 = cb / N3, n3 = cb - (cb / N3) * N3;
-    okm = 0;
+    const int4* co = reinterpret_cast<const int4*>(colofs + ks * 16 + 4 * kgrp);
+    const int4 c01 = co[0], c23 = co[1];
+    const int o2[4] = {c01.x, c01.z, c23.x, c23.z};
+    const int o3[4] = {c01.y, c01.w, c23.y, c23.w};
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const bool ok = cb + e < CH;
-      okm |= ok ? (1u << e) : 0u;
-      const int m2 = ok ? n2 : 0, m3 = ok ? n3 : 0;
+    for (int e = 0; e < 4; ++e)
 #pragma unroll
       for (int r = 0; r < RT; ++r) {
-        pe2[r][e] = tj2[(size_t)m2 * npad + ibase + 16 * r];
-        pe3[r][e] = tj3[(size_t)m3 * npad + ibase + 16 * r];
+        pe2[r][e] = tj2[o2[e] + ibase + 16 * r];
+        pe3[r][e] = tj3[o3[e] + ibase + 16 * r];
       }
-      if (++n3 == N3) {
-        n3 = 0;
-        ++n2;
-      }
-    }
   };
   // A fragments: A[i][2c] = Re P_i[c], A[i][2c+1] = Im P_i[c]
   auto make_frags = [&](half8 (&h8)[RT], half8 (&l8)[RT]) {
@@ -959,16 +953,12 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
 #pragma unroll
       for (int r = 0; r < RT; ++r) {
         const float2 e2 = pe2[r][e], e3 = pe3[r][e];
-        const bool ok = (okm >> e) & 1u;
-        const float pr = ok ? e2.x * e3.x - e2.y * e3.y : 0.f;
-        const float pi = ok ? e2.x * e3.y + e2.y * e3.x : 0.f;
-        _Float16 h, l;
-        split16(pr, h, l);
-        h8[r][2 * e] = h;
-        l8[r][2 * e] = l;
-        split16(pi, h, l);
-        h8[r][2 * e + 1] = h;
-        l8[r][2 * e + 1] = l;
+        half2v h, l;
+        split16x2(e2.x * e3.x - e2.y * e3.y, e2.x * e3.y + e2.y * e3.x, h, l);
+        h8[r][2 * e] = h.x;
+        h8[r][2 * e + 1] = h.y;
+        l8[r][2 * e] = l.x;
+        l8[r][2 * e + 1] = l.y;
       }
   };
   half8 ah[RT], al[RT];
@@ -1240,6 +1230,21 @@ static bool allocate(MixedState* m) {
   m->X_cap = (size_t)m->x_slices * np * APP_NCT_E * 16;
   mx_alloc(&m->X, m->X_cap);
   mx_alloc(&m->d_split, (size_t)m->nsplit);
+  // per column c < CHp of the paired half: element offsets of its E2 and E3
+  // rows, (n2 npad, n3 npad), with c clamped to CH - 1 (padding columns
+  // multiply the zero rows of the apply operand)
+  {
+    std::vector<int2> co(m->CHp);
+    for (int c = 0; c < m->CHp; ++c) {
+      const int cc = std::min(c, m->CH - 1);
+      co[c] = make_int2((cc / m->N3) * (int)np, (cc % m->N3) * (int)np);
+    }
+    mx_alloc(&m->d_colofs, co.size());
+    if (!m->d_colofs ||
+        hipMemcpy(m->d_colofs, co.data(), co.size() * sizeof(int2), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      return false;
+  }
   return m->tj1 && m->tj2 && m->tj3 && m->g && m->gmax && m->sinv && m->rpart && m->bfrag &&
          m->X && m->Wt && m->afrag && m->d_split;
 }
@@ -1356,7 +1361,7 @@ MixedState* create(mdffgpu_ctx* c) {
 
 void destroy(MixedState* m) {
   if (!m) return;
-  void* p[] = {m->tj1,   m->tj2,  m->tj3,  m->rpart, m->g,      m->gmax,   m->bfrag, m->X,
+  void* p[] = {m->d_colofs, m->tj1, m->tj2, m->tj3, m->rpart, m->g, m->gmax, m->bfrag, m->X,
                m->Wt,    m->afrag, m->sinv,  m->d_seg, m->d_kseg, m->d_split, m->apart};
   for (void* x : p)
     if (x) (void)hipFree(x);
@@ -1399,8 +1404,8 @@ static void launch_apply(MixedState* m, hipStream_t st) {
   const int mblocks = m->npad / rows;
   const int per = NXCD / S;
   const int grid = (mblocks + per - 1) / per * NXCD;
-  mx_apply_gemm<NCT, RT><<<grid, 64 * APP_WAVES, 0, st>>>(m->npad, m->N3, m->CH, KS, S, ksper,
-                                                         mblocks, m->tj2, m->tj3, m->bfrag,
+  mx_apply_gemm<NCT, RT><<<grid, 64 * APP_WAVES, 0, st>>>(m->npad, KS, S, ksper, mblocks,
+                                                         m->d_colofs, m->tj2, m->tj3, m->bfrag,
                                                          m->d_kseg, m->X);
 }
 
