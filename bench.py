@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--tuning", default=None, metavar="SLICES,SPLIT",
                     help="fix the mixed path's matrix-core launch shape (apply K slices, "
                          "structure-factor split atoms) instead of autotuning it")
+    ap.add_argument("--mixed-scf", default="pcg", choices=["jacobi", "pcg"],
+                    help="SCF solver of the mixed path (the FP64 path always runs the "
+                         "reference's Jacobi iteration); with pcg the mixed path is also "
+                         "timed with the Jacobi iteration, reported beside the headline")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this size first (calibrates HBM "
@@ -83,7 +87,7 @@ def profile_traffic(kernel):
     return None, None
 
 
-def time_path(args, sys_, precision, frames_dev):
+def time_path(args, sys_, precision, frames_dev, scf=None, tuning=None):
     import copy
     import torch
     from mdff_amd import params as P
@@ -91,10 +95,13 @@ def time_path(args, sys_, precision, frames_dev):
     from mdff_amd import flops as FL
     p = copy.deepcopy(sys_.params)
     p.precision = precision
+    if precision == P.PREC_MIXED and (scf or args.mixed_scf) == "pcg":
+        p.scf_solver = P.SCF_PCG
     n = sys_.natm
     force = torch.zeros((3, n), dtype=torch.float64, device="cuda")
-    tuning = None
-    if precision == P.PREC_MIXED and args.tuning:
+    if tuning:
+        pass
+    elif precision == P.PREC_MIXED and args.tuning:
         sl, sp = (int(x) for x in args.tuning.split(","))
         tuning = {"apply_slices": sl, "split_atoms": sp}
     elif precision == P.PREC_MIXED:
@@ -319,12 +326,19 @@ def main():
         res[key] = time_path(args, sys_, prec, frames_dev)
         log(args, f"[bench] {key}: {res[key]['ms_per_step']:.2f} ms/step, "
                   f"{res[key]['atom_steps_per_s']:.4g} atom-steps/s, n_scf {res[key]['n_scf']}")
+    if "mixed" in res and "fp64" in res and args.mixed_scf == "pcg":
+        # the same mixed path with the reference's SCF iteration, for the
+        # speedup at equal algorithm
+        log(args, "[bench] timing mixed with the Jacobi SCF ...")
+        res["mixed_jacobi"] = time_path(args, sys_, P.PREC_MIXED, frames_dev, scf="jacobi",
+                                        tuning=res["mixed"]["tuning"])
     head = res.get("mixed", res.get("fp64"))
-    err = None
-    if "mixed" in res and "fp64" in res:
-        a = res["mixed"]["force"]
+
+    def ferr(key):
+        a = res[key]["force"]
         b = res["fp64"]["force"]
-        err = float(torch.sqrt(((a - b) ** 2).mean()) / torch.sqrt((b ** 2).mean()))
+        return float(torch.sqrt(((a - b) ** 2).mean()) / torch.sqrt((b ** 2).mean()))
+    err = ferr("mixed") if "mixed" in res and "fp64" in res else None
     cpu = None
     if not args.no_cpu:
         n_es = float(np.mean(res["fp64" if "fp64" in res else "mixed"]["n_scf"])) + 2.0
@@ -353,6 +367,7 @@ def main():
             "natm": sys_.natm,
             "kvectors": P.num_kvectors(list(p.kES)),
             "path": "mixed" if "mixed" in res else "fp64",
+            "scf_solver": {"mixed": args.mixed_scf, "fp64": "jacobi"},
         },
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
@@ -368,6 +383,14 @@ def main():
     if err is not None:
         out["mixed_vs_fp64"] = {"speedup": res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"],
                                 "force_rel_rms_err": err}
+    if "mixed_jacobi" in res:
+        r = res["mixed_jacobi"]
+        out["mixed_jacobi"] = {"atom_steps_per_s": r["atom_steps_per_s"],
+                               "ms_per_step": r["ms_per_step"],
+                               "mean_n_scf": float(np.mean(r["n_scf"]))}
+        if "fp64" in res:
+            out["mixed_jacobi"]["speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / r["ms_per_step"]
+            out["mixed_jacobi"]["force_rel_rms_err"] = ferr("mixed_jacobi")
     if not args.no_batch:
         out["batch"] = batch_leg(args)
     print(json.dumps(out), flush=True)

@@ -64,6 +64,18 @@ typedef enum mdffgpu_mixed_part {
   MDFFGPU_MIXED_FP64_KSPACE = 8      /* reciprocal-space sums */
 } mdffgpu_mixed_part;
 
+/* SCF solver of the induced dipoles.  JACOBI is the reference's
+   moment_from_pola_scf (src/field.f90:3200-3458), the only solver its
+   get_moments dispatches (src/field.f90:3716-3718; 'cng' is an accepted
+   algo_moment_from_pola value there that computes nothing).  PCG is this
+   library's preconditioned conjugate gradient on (alpha^-1 - T) mu = E_stat
+   for the same fixed point, with alpha as the preconditioner: fewer Ewald
+   calls per solve, results equal to JACOBI's to the SCF tolerance. */
+typedef enum mdffgpu_scf_solver {
+  MDFFGPU_SCF_JACOBI = 0,
+  MDFFGPU_SCF_PCG = 1
+} mdffgpu_scf_solver;
+
 typedef enum mdffgpu_ext_algo {
   MDFFGPU_EXT_POLY = 0, /* algo_ext_dipole = 'poly' */
   MDFFGPU_EXT_ASPC = 1  /* algo_ext_dipole = 'aspc' */
@@ -139,7 +151,8 @@ typedef struct mdffgpu_params {
   /* this library */
   int32_t precision;        /* mdffgpu_precision */
   int32_t mixed_fp64_parts; /* mdffgpu_mixed_part bits, 0 = all mixed */
-  int32_t reserved[6];
+  int32_t scf_solver;       /* mdffgpu_scf_solver */
+  int32_t reserved[5];
 } mdffgpu_params;
 
 /* Scalar outputs of one evaluation: the thermodynamic module variables

@@ -194,6 +194,32 @@ __global__ void k_moments(int n, const double* pos, const double* q, const doubl
                           double* part);
 template <int MODE>
 __global__ void k_combine64(CombineArgs a);
+// Preconditioned conjugate gradient for the induced dipoles,
+// (alpha^-1 - T) x = E_stat with the per-type polarizability alpha as the
+// preconditioner; atoms without polarizability keep zeros.  The operator's
+// T p is read as E - E_stat after an ES_IND call on p.
+struct CgArgs {
+  int n;
+  const int* typ;
+  const TypeTables* T;
+  double* x;        // induced dipoles (the SCF's mu_ind)
+  double* r;        // residual E_stat + T x - alpha^-1 x
+  double* z;        // alpha r
+  double* p;        // search direction
+  double* Ap;       // (alpha^-1 - T) p
+  const double* E;  // E_stat + T v of the last ES_IND call (v = x or p)
+  const double* Estat;
+  const double* sc;  // device scalars: [rz_old, pAp, rz_new] slots below
+  int s_rz, s_pap, s_rzn;
+  double* part;      // block partials, rows of nblk
+  const int* rank;   // with p4: FP32 rank-ordered copy of p for k_dd_tab
+  float4* p4;
+};
+__global__ void k_cg_init(CgArgs a);
+__global__ void k_cg_ap(CgArgs a);
+__global__ void k_cg_update(CgArgs a);
+__global__ void k_cg_dir(CgArgs a);
+__global__ void k_cg_upol(CgArgs a);
 __global__ void k_induced(int n, const int* typ, const TypeTables* T, const double* E,
                           double* mu, double* part, const int* rank, float4* mu4);
 __global__ void k_scf_resid(int n, const int* typ, const TypeTables* T, const double* mu,

@@ -679,6 +679,139 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_reduce_rows2(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Preconditioned conjugate gradient of the induced dipoles (an alternative
+// to the Jacobi iteration of moment_from_pola_scf, src/field.f90:3200-3458,
+// for the same fixed point mu = alpha (E_stat + T mu)).  Per-atom 3x3 blocks,
+// FP64; block partials in rows of a.part for k_reduce_rows.
+__device__ __forceinline__ void mat3(const double* m, const double* v, int i, int n,
+                                     double (&o)[3]) {
+  for (int r = 0; r < 3; ++r)
+    o[r] = m[r] * v[i] + m[r + 3] * v[i + n] + m[r + 6] * v[i + 2 * n];
+}
+
+// r = E - alpha^-1 x, z = alpha r, p = z; rows: r.z, |r|^2, polarizable count
+__global__ __launch_bounds__(MDFF_BLOCK) void k_cg_init(CgArgs a) {
+  __shared__ double red[4 * 3];
+  const int n = a.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    const int t = a.typ[i];
+    double r[3] = {0, 0, 0}, z[3] = {0, 0, 0};
+    if (a.T->polar[t]) {
+      double ax[3];
+      mat3(a.T->invpol[t], a.x, i, n, ax);
+      for (int c = 0; c < 3; ++c) r[c] = a.E[i + c * n] - ax[c];
+      for (int c = 0; c < 3; ++c)
+        z[c] = a.T->pol[t][c] * r[0] + a.T->pol[t][c + 3] * r[1] + a.T->pol[t][c + 6] * r[2];
+      for (int c = 0; c < 3; ++c) {
+        acc[0] += r[c] * z[c];
+        acc[1] += r[c] * r[c];
+      }
+      acc[2] = 1.0;
+    }
+    for (int c = 0; c < 3; ++c) {
+      a.r[i + c * n] = r[c];
+      a.z[i + c * n] = z[c];
+      a.p[i + c * n] = z[c];
+    }
+    if (a.p4) a.p4[a.rank[i]] = make_float4((float)z[0], (float)z[1], (float)z[2], 0.f);
+  }
+  block_sum<3>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) a.part[k * gridDim.x + blockIdx.x] = acc[k];
+}
+
+// Ap = alpha^-1 p - (E - E_stat); row: p.Ap
+__global__ __launch_bounds__(MDFF_BLOCK) void k_cg_ap(CgArgs a) {
+  __shared__ double red[4];
+  const int n = a.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < n) {
+    const int t = a.typ[i];
+    double ap[3] = {0, 0, 0};
+    if (a.T->polar[t]) {
+      double ip[3];
+      mat3(a.T->invpol[t], a.p, i, n, ip);
+      for (int c = 0; c < 3; ++c) {
+        ap[c] = ip[c] - (a.E[i + c * n] - a.Estat[i + c * n]);
+        acc[0] += a.p[i + c * n] * ap[c];
+      }
+    }
+    for (int c = 0; c < 3; ++c) a.Ap[i + c * n] = ap[c];
+  }
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) a.part[blockIdx.x] = acc[0];
+}
+
+// step = rz_old / pAp: x += step p, r -= step Ap, z = alpha r; rows: r.z (new),
+// |r|^2, polarizable count
+__global__ __launch_bounds__(MDFF_BLOCK) void k_cg_update(CgArgs a) {
+  __shared__ double red[4 * 3];
+  const int n = a.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double pap = a.sc[a.s_pap];
+  const double step = pap != 0.0 ? a.sc[a.s_rz] / pap : 0.0;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    const int t = a.typ[i];
+    if (a.T->polar[t]) {
+      double r[3];
+      for (int c = 0; c < 3; ++c) {
+        a.x[i + c * n] += step * a.p[i + c * n];
+        r[c] = a.r[i + c * n] - step * a.Ap[i + c * n];
+        a.r[i + c * n] = r[c];
+      }
+      for (int c = 0; c < 3; ++c) {
+        const double z =
+            a.T->pol[t][c] * r[0] + a.T->pol[t][c + 3] * r[1] + a.T->pol[t][c + 6] * r[2];
+        a.z[i + c * n] = z;
+        acc[0] += r[c] * z;
+        acc[1] += r[c] * r[c];
+      }
+      acc[2] = 1.0;
+    }
+  }
+  block_sum<3>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) a.part[k * gridDim.x + blockIdx.x] = acc[k];
+}
+
+// p = z + (rz_new / rz_old) p, and its FP32 rank-ordered copy
+__global__ __launch_bounds__(MDFF_BLOCK) void k_cg_dir(CgArgs a) {
+  const int n = a.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double rz = a.sc[a.s_rz];
+  const double beta = rz != 0.0 ? a.sc[a.s_rzn] / rz : 0.0;
+  double p[3];
+  for (int c = 0; c < 3; ++c) {
+    p[c] = a.z[i + c * n] + beta * a.p[i + c * n];
+    a.p[i + c * n] = p[c];
+  }
+  if (a.p4) a.p4[a.rank[i]] = make_float4((float)p[0], (float)p[1], (float)p[2], 0.f);
+}
+
+// u_pol partials of x (the quantity k_induced accumulates): x alpha^-1 x
+__global__ __launch_bounds__(MDFF_BLOCK) void k_cg_upol(CgArgs a) {
+  __shared__ double red[4];
+  const int n = a.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < n) {
+    const int t = a.typ[i];
+    if (a.T->polar[t]) {
+      double ix[3];
+      mat3(a.T->invpol[t], a.x, i, n, ix);
+      for (int c = 0; c < 3; ++c) acc[0] += a.x[i + c * n] * ix[c];
+    }
+  }
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) a.part[blockIdx.x] = acc[0];
+}
+
 // out = a + b elementwise (total dipoles mu = dipia + mu_ind)
 __global__ __launch_bounds__(MDFF_BLOCK) void k_add(int m, const double* __restrict__ a,
                                                     const double* __restrict__ b,

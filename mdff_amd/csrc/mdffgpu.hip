@@ -30,7 +30,8 @@ enum {
   SC_UPOL = 34,  // 1
   SC_RMSD = 35,  // 2: sum, npol
   SC_DISP = 37,  // max squared displacement since the list build
-  SC_N = 40
+  SC_CG = 40,    // 7: conjugate-gradient SCF, two (r.z, |r|^2, npol) triples, p.Ap
+  SC_N = 48
 };
 
 thread_local std::string g_create_error;
@@ -102,6 +103,7 @@ struct mdffgpu_ctx {
   TTTable tt;
   double u_pol = 0.0;
   double* d_upart = nullptr;  // block partials of u_pol (reduced once per SCF)
+  double* d_cg = nullptr;     // conjugate-gradient SCF: r, z, p, Ap (3N each)
   bool upol_pending = false;
   double* h_sc = nullptr;  // pinned
   double* h_sc_dev = nullptr;  // device address of h_sc
@@ -343,6 +345,8 @@ void validate(mdffgpu_ctx* c, const mdffgpu_params* p) {
     fail(c, MDFFGPU_ERR_ARG, "unknown precision");
   if (p->max_scf_pol_iter < 0 || p->min_scf_pol_iter < 0)
     fail(c, MDFFGPU_ERR_ARG, "SCF iteration bounds must be non-negative");
+  if (p->scf_solver != MDFFGPU_SCF_JACOBI && p->scf_solver != MDFFGPU_SCF_PCG)
+    fail(c, MDFFGPU_ERR_ARG, "unknown scf_solver");
 }
 
 void ensure_part(mdffgpu_ctx* c, size_t count) {
@@ -381,6 +385,7 @@ void alloc_atoms(mdffgpu_ctx* c) {
   dalloc(c, &c->d_mu4, n);
   dalloc(c, &c->d_s4, n);
   dalloc(c, &c->d_upart, nblk(c->n));
+  if (c->P.scf_solver == MDFFGPU_SCF_PCG) dalloc(c, &c->d_cg, 12 * n);
   dalloc(c, &c->d_perm, n);
   dalloc(c, &c->d_rank, n);
   dalloc(c, &c->d_mup, 3 * n);
@@ -790,7 +795,104 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
   };
   const bool pack = spec && scf_field_fp32(c);
   bool queued = false;  // this iteration's moments and real-space field are queued
-  while (((iscf < P.max_scf_pol_iter && rmsd > tol) || iscf < P.min_scf_pol_iter) && !stalled) {
+  if (P.scf_solver == MDFFGPU_SCF_PCG) {
+    // Preconditioned conjugate gradient on (alpha^-1 - T) mu = E_stat from
+    // the same initial guess as the Jacobi iteration; one ES_IND call per
+    // iteration (the first on the guess, then on the search directions).
+    // The residual r = E_stat + T mu - alpha^-1 mu is kept by the recursion;
+    // for a diagonal alpha its squares are the reference's criterion
+    // (mu_a / alpha_aa - E_a)^2.  While the host reads an iteration's
+    // residual, the next direction and its real-space field are queued.
+    const bool pk = scf_field_fp32(c);
+    CgArgs g;
+    std::memset(&g, 0, sizeof(g));
+    g.n = n;
+    g.typ = c->d_typ;
+    g.T = c->d_tt;
+    g.x = c->d_muind;
+    g.r = c->d_cg;
+    g.z = g.r + 3 * (size_t)n;
+    g.p = g.z + 3 * (size_t)n;
+    g.Ap = g.p + 3 * (size_t)n;
+    g.E = c->d_E;
+    g.Estat = c->d_Estat;
+    g.sc = c->d_sc;
+    g.part = c->d_part;
+    g.rank = c->d_rank;
+    g.p4 = pk ? reinterpret_cast<float4*>(c->d_mu4) : nullptr;
+    if (!c->ev_sc) {
+      HCK(c, hipEventCreateWithFlags(&c->ev_sc, hipEventDisableTiming));
+      HCK(c, hipHostGetDevicePointer((void**)&c->h_sc_dev, c->h_sc, 0));
+    }
+    ++iscf;
+    if (P.calc_mode == 0)
+      induced(false);
+    else if (P.algo_ext_dipole == MDFFGPU_EXT_POLY)
+      extrap(c, (P.extrapolate_order == 0) ? 0 : (itime > P.extrapolate_order ? 1 : 2), 0, 0.0);
+    else
+      extrap(c, 3, aspc_order(P, itime), 0.0);
+    es_call(c, ES_IND, c->d_qzero, c->d_muind, TASK_DD, 1, 0);
+    ++*n_es;
+    k_cg_init<<<gb, MDFF_BLOCK, 0, st>>>(g);
+    check_launch(c, "k_cg_init");
+    reduce(c, 3, gb, SC_CG);
+    sync_scalars(c);
+    int cur = 0;  // triple of the current r.z, |r|^2, npol
+    auto base = [](int k) { return SC_CG + 3 * k; };
+    rmsd = std::sqrt(c->h_sc[base(cur) + 1] / c->h_sc[base(cur) + 2]);
+    rmsd_first = rmsd;
+    rmsd_best = rmsd;
+    bool queued_p = false;  // the direction and its real-space field are queued
+    auto dir = [&](int from, int to) {
+      g.s_rz = base(from);
+      g.s_rzn = base(to);
+      k_cg_dir<<<gb, MDFF_BLOCK, 0, st>>>(g);
+      check_launch(c, "k_cg_dir");
+    };
+    (void)dir;
+    while (((iscf < P.max_scf_pol_iter && rmsd > tol) || iscf < P.min_scf_pol_iter) && !stalled) {
+      ++iscf;
+      es_call(c, ES_IND, c->d_qzero, g.p, TASK_DD, 1, 0, queued_p && pk);
+      queued_p = false;
+      ++*n_es;
+      k_cg_ap<<<gb, MDFF_BLOCK, 0, st>>>(g);
+      check_launch(c, "k_cg_ap");
+      reduce(c, 1, gb, SC_CG + 6);
+      g.s_rz = base(cur);
+      g.s_pap = SC_CG + 6;
+      k_cg_update<<<gb, MDFF_BLOCK, 0, st>>>(g);
+      check_launch(c, "k_cg_update");
+      const int nxt = cur ^ 1;
+      {
+        ProfScope _ps(c, K_REDUCE);
+        k_reduce_rows2<<<3, MDFF_BLOCK, 0, st>>>(c->d_part, gb, c->d_sc + base(nxt),
+                                                 c->h_sc_dev + base(nxt));
+      }
+      check_launch(c, "k_reduce_rows2");
+      HCK(c, hipEventRecord(c->ev_sc, st));
+      dir(cur, nxt);
+      if (pk) es_ind_dir(c, g.p, true);
+      queued_p = true;
+      HCK(c, hipEventSynchronize(c->ev_sc));
+      cur = nxt;
+      rmsd = std::sqrt(c->h_sc[base(cur) + 1] / c->h_sc[base(cur) + 2]);
+      if (rmsd < rmsd_best) {
+        rmsd_best = rmsd;
+        no_progress = 0;
+      } else {
+        ++no_progress;
+      }
+      if (mixed_prec && iscf >= std::max(P.min_scf_pol_iter, 3) && no_progress >= 2 &&
+          rmsd < 0.1 * rmsd_first && (c->mx_floor == 0.0 || rmsd < 5.0 * c->mx_floor))
+        stalled = true;
+    }
+    g.part = c->d_upart;
+    k_cg_upol<<<gb, MDFF_BLOCK, 0, st>>>(g);
+    check_launch(c, "k_cg_upol");
+    upol_new = true;
+  }
+  while (P.scf_solver != MDFFGPU_SCF_PCG &&
+         ((iscf < P.max_scf_pol_iter && rmsd > tol) || iscf < P.min_scf_pol_iter) && !stalled) {
     ++iscf;
     if (queued) {
       // k_induced and the real-space field were queued by the last iteration
@@ -1323,7 +1425,7 @@ void mdffgpu_destroy(mdffgpu_ctx* c) {
   double* dp[] = {c->d_rref, c->d_pos, c->d_s, c->d_q, c->d_qzero, c->d_dip, c->d_mu, c->d_muind,
                   c->d_mup, c->d_Estat, c->d_E, c->d_efdir, c->d_efgdir, c->d_fdir,
                   c->d_force, c->d_ef, c->d_efg, c->d_apart, c->d_Ak, c->d_kcoe, c->d_kvec,
-                  c->d_part, c->d_sc, c->d_hist, c->d_upart};
+                  c->d_part, c->d_sc, c->d_hist, c->d_upart, c->d_cg};
   for (double* p : dp)
     if (p) (void)hipFree(p);
   if (c->d_t2) (void)hipFree(c->d_t2);

@@ -16,6 +16,7 @@ module mdffgpu_mod
   integer, parameter, public :: MDFFGPU_NTYPEMAX = 16
   integer(c_int32_t), parameter, public :: MDFFGPU_PREC_FP64 = 0, MDFFGPU_PREC_MIXED = 1
   integer(c_int32_t), parameter, public :: MDFFGPU_EXT_POLY = 0, MDFFGPU_EXT_ASPC = 1
+  integer(c_int32_t), parameter, public :: MDFFGPU_SCF_JACOBI = 0, MDFFGPU_SCF_PCG = 1
 
   type, bind(C), public :: mdffgpu_params
     integer(c_int32_t) :: natm, ntype
@@ -40,7 +41,8 @@ module mdffgpu_mod
     integer(c_int32_t) :: algo_ext_dipole, doefield, doefg
     integer(c_int32_t) :: precision
     integer(c_int32_t) :: mixed_fp64_parts
-    integer(c_int32_t) :: reserved(6)
+    integer(c_int32_t) :: scf_solver
+    integer(c_int32_t) :: reserved(5)
   end type
 
   type, bind(C), public :: mdffgpu_energies

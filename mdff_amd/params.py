@@ -36,6 +36,10 @@ MIXED_FP64_SCF_FIELD = 4
 MIXED_FP64_KSPACE = 8
 EXT_POLY = 0
 EXT_ASPC = 1
+# SCF solver of the induced dipoles (mdffgpu_scf_solver): the reference's
+# Jacobi iteration, or the preconditioned conjugate gradient of this library
+SCF_JACOBI = 0
+SCF_PCG = 1
 CALC_OTHER = 0
 CALC_MD = 1
 CALC_OPT = 2
@@ -101,7 +105,8 @@ class MdffParams(ctypes.Structure):
         ("doefg", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("mixed_fp64_parts", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("scf_solver", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 

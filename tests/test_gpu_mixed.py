@@ -58,3 +58,35 @@ def test_mixed_ex9_reference_energy():
     p, itype, A, r, meta = G.ex9_test11(precision=P.PREC_MIXED)
     g = ForceField(p, itype, A).engforce(r, itime=0)
     assert abs(g["u_bmhft"] + g["u_coul"] - meta["reference_utot"]) <= 1e-6 * abs(meta["reference_utot"])
+
+
+@pytest.mark.parametrize("precision", [P.PREC_FP64, P.PREC_MIXED])
+def test_pcg_scf_matches_jacobi(precision):
+    """The conjugate-gradient SCF reaches the Jacobi iteration's fixed point:
+    forces, dipoles, u_pol and EFGs against the FP64 Jacobi solve (the
+    reference's algorithm) to the 1e-5 bar, over two perturbed steps (the
+    dipole history feeds the initial guess), in no more Ewald calls."""
+    from mdff_amd.field import ForceField
+    s = C.silica(200, 400, seed=31, rc=9.0, kmax=8, pim=True, efg=True)
+    pj = copy.deepcopy(s.params)
+    pj.precision = P.PREC_FP64
+    pc = copy.deepcopy(s.params)
+    pc.precision = precision
+    pc.scf_solver = P.SCF_PCG
+    fj = ForceField(pj, s.itype, s.A)
+    fc = ForceField(pc, s.itype, s.A)
+    rng = np.random.default_rng(5)
+    r = s.r.copy()
+    try:
+        for it in range(1, 3):
+            o = fj.engforce(r, itime=it)
+            g = fc.engforce(r, itime=it)
+            assert rel_rms(g["f"], o["f"]) <= 1e-5, rel_rms(g["f"], o["f"])
+            assert rel_rms(g["mu"], o["mu"]) <= 1e-5, rel_rms(g["mu"], o["mu"])
+            assert abs(g["u_pol"] - o["u_pol"]) <= 1e-5 * abs(o["u_pol"])
+            assert rel_rms(g["efg"], o["efg"]) <= 1e-5
+            assert g["n_scf"] <= o["n_scf"], (g["n_scf"], o["n_scf"])
+            r = r + 0.01 * rng.standard_normal(r.shape)
+    finally:
+        fj.close()
+        fc.close()
