@@ -90,3 +90,31 @@ def test_pcg_scf_matches_jacobi(precision):
     finally:
         fj.close()
         fc.close()
+
+
+@pytest.mark.parametrize("solver", [P.SCF_JACOBI, P.SCF_PCG])
+def test_mixed_vs_fp64_pim_3000(solver):
+    """The 3000-atom polarizable silica workload (1000 Si + 2000 O, rc 10 A,
+    kmax 10, SCF to 1e-8, EFG at every site): force, dipole and EFG error of
+    the mixed path against the FP64 path (Jacobi, the reference's SCF), with
+    either SCF solver in the mixed path."""
+    from mdff_amd.field import ForceField
+    s = C.config(3)
+    p64 = copy.deepcopy(s.params)
+    p64.precision = P.PREC_FP64
+    pmx = copy.deepcopy(s.params)
+    pmx.precision = P.PREC_MIXED
+    pmx.scf_solver = solver
+    f64 = ForceField(p64, s.itype, s.A)
+    fmx = ForceField(pmx, s.itype, s.A)
+    try:
+        o = f64.engforce(s.r, itime=1)
+        g = fmx.engforce(s.r, itime=1)
+    finally:
+        f64.close()
+        fmx.close()
+    ef, em, eg = rel_rms(g["f"], o["f"]), rel_rms(g["mu"], o["mu"]), rel_rms(g["efg"], o["efg"])
+    print(f"pim 3000 solver {solver}: force {ef:.2e} dipole {em:.2e} efg {eg:.2e} "
+          f"n_scf {g['n_scf']} (fp64 {o['n_scf']})")
+    assert ef <= 1e-5 and em <= 1e-5 and eg <= 1e-5, (ef, em, eg)
+    assert abs(g["u_pol"] - o["u_pol"]) <= 1e-5 * abs(o["u_pol"])
