@@ -22,6 +22,21 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(SrArgs a) {
   const bool LJ = a.flags & SR_LJ, BM = a.flags & SR_BM, BMD = a.flags & SR_BMD,
              MO = a.flags & SR_MO;
   const PairTables* T = a.T;
+  // FP32: the BMHFTD parameters of the type pairs in LDS, (A, B, C, D) and
+  // BD (the FP32 kernels convert them to float anyway)
+  constexpr bool F32 = sizeof(R) == sizeof(float);
+  __shared__ float4 sbm[F32 ? 256 : 1];
+  __shared__ float sbd[F32 ? 256 : 1];
+  if constexpr (F32) {
+    if (BM) {
+      for (int t = threadIdx.x; t < 256; t += blockDim.x) {
+        sbm[t] = make_float4((float)T->bm_A[t], (float)T->bm_B[t], (float)T->bm_C[t],
+                             (float)T->bm_D[t]);
+        sbd[t] = (float)T->bm_BD[t];
+      }
+      __syncthreads();
+    }
+  }
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i0 = tid / LPA, sub = tid % LPA;
   const int i = (a.perm && i0 < n) ? a.perm[i0] : i0;  // atoms in spatial order
@@ -37,13 +52,18 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(SrArgs a) {
     const Cell& cell = BATCH ? a.bc[cf].cell : a.cell;
     const R rc2 = R(BATCH ? a.bc[cf].rc2 : a.rc2);
     const int* nb = a.nbr + (size_t)i * a.maxnb;
+    // the next neighbour's index one iteration ahead, a neighbour's position
+    // and type loaded together
+    int jn = sub < c ? nb[sub] : 0;
     for (int k = sub; k < c; k += LPA) {
-      const int j = nb[k];
+      const int j = jn;
+      if (k + LPA < c) jn = nb[k + LPA];
+      const double4 sj = a.s4[j];
+      const int tj = a.typ[j];
       R rx, ry, rz;
-      const R rsq = min_image_t<R>(cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
-                                   rx, ry, rz);
+      const R rsq = min_image_t<R>(cell, si1 - sj.x, si2 - sj.y, si3 - sj.z, rx, ry, rz);
       if (!(rsq < rc2)) continue;
-      const int t = ti + MDFF_NT * a.typ[j];
+      const int t = ti + MDFF_NT * tj;
       R w = R(0);
       if (LJ) {
         const R sr2 = R(T->lj_sigsq[t]) / rsq;
@@ -57,26 +77,34 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(SrArgs a) {
         acc[1] += R(0.5) * wl * rsq;
         if (!BM) w += wl;
       }
-      if (BM && T->bm_A[t] != 0.0) {
+      float4 bmf = make_float4(0.f, 0.f, 0.f, 0.f);
+      if constexpr (F32)
+        if (BM) bmf = sbm[t];
+      const R bA = F32 ? R(bmf.x) : R(T->bm_A[t]);
+      if (BM && (F32 ? bmf.x != 0.f : T->bm_A[t] != 0.0)) {
+        const R bB = F32 ? R(bmf.y) : R(T->bm_B[t]);
+        const R bC = F32 ? R(bmf.z) : R(T->bm_C[t]);
+        const R bD = F32 ? R(bmf.w) : R(T->bm_D[t]);
         const R ir2 = R(1) / rsq;
         const R d = sqrt(rsq);
-        const R erh = R(T->bm_A[t]) * rexp<R>(-R(T->bm_B[t]) * d);
+        const R erh = bA * rexp<R>(-bB * d);
         R ir6 = ir2 * ir2 * ir2;
         R ir8 = ir6 * ir2;
-        ir6 = ir6 * R(T->bm_C[t]);
-        ir8 = ir8 * R(T->bm_D[t]);
+        ir6 = ir6 * bC;
+        ir8 = ir8 * bD;
         R ir6d = ir6, ir8d = ir8, fd6 = 0, fd8 = 0;
         if (BMD) {
           R f6, f8;
-          tt_damp_t<R>(a.tt, R(T->bm_BD[t]), R(1), d, 6, f6, fd6);
-          tt_damp_t<R>(a.tt, R(T->bm_BD[t]), R(1), d, 8, f8, fd8);
+          const R bBD = F32 ? R(sbd[t]) : R(T->bm_BD[t]);
+          tt_damp_t<R>(a.tt, bBD, R(1), d, 6, f6, fd6);
+          tt_damp_t<R>(a.tt, bBD, R(1), d, 8, f8, fd8);
           ir6d = ir6 * f6;
           ir8d = ir8 * f8;
         }
         const R ir7 = R(6) * ir6d / d;
         const R ir9 = R(8) * ir8d / d;
         acc[2] += R(0.5) * (erh - ir6d - ir8d);
-        const R wb = (R(T->bm_B[t]) * erh - ir7 - ir9 + ir6 * fd6 + ir8 * fd8) / d;
+        const R wb = (bB * erh - ir7 - ir9 + ir6 * fd6 + ir8 * fd8) / d;
         acc[3] += R(0.5) * wb * rsq;
         w += wb;
       }
@@ -140,6 +168,21 @@ template __global__ void k_shortrange<float, true>(SrArgs);
 template <int MODE, typename R, bool BATCH>
 __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
   __shared__ double red[4 * 7];
+  // FP32: the Tang-Toennies damping table of the pair types in LDS, one
+  // float4 (on, b, c, k) per (slot o, type pair) (the FP32 kernels convert
+  // b and c to float anyway)
+  constexpr bool F32 = sizeof(R) == sizeof(float);
+  __shared__ float4 sdmp[F32 ? 2 * 256 : 1];
+  if constexpr (F32) {
+    if (MODE != ES_IND) {
+      for (int t = threadIdx.x; t < 2 * 256; t += blockDim.x) {
+        const int o = t >> 8, pr = t & 255;
+        sdmp[t] = make_float4((float)a.T->dmp_on[o][pr], (float)a.T->dmp_b[o][pr],
+                              (float)a.T->dmp_c[o][pr], (float)a.T->dmp_k[o][pr]);
+      }
+      __syncthreads();
+    }
+  }
   const int n = a.n;
   const bool QQ = a.task & 1, QD = a.task & 2, DD = a.task & 4;
   const bool EFIELD = (MODE != ES_FINAL) || a.do_efield;
@@ -170,22 +213,31 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
     const bool dip_i = (mi0 != R(0) || mi1 != R(0) || mi2 != R(0));
     const int c = a.cnt[i];
     const int* nb = a.nbr + (size_t)i * a.maxnb;
+    // the next neighbour's index one iteration ahead, and all of a
+    // neighbour's data loaded together, before its cutoff test
+    int jn = sub < c ? nb[sub] : 0;
     for (int k = sub; k < c; k += LPA) {
-      const int j = nb[k];
+      const int j = jn;
+      if (k + LPA < c) jn = nb[k + LPA];
+      const double4 sj = a.s4[j];
+      const double qjd = a.q[j];
+      const double mj0d = a.mu[j], mj1d = a.mu[j + n], mj2d = a.mu[j + 2 * n];
+      const int tj = a.typ[j];
       R rx, ry, rz;
-      const R d2 = min_image_t<R>(cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
-                                  rx, ry, rz);
+      const R d2 = min_image_t<R>(cell, si1 - sj.x, si2 - sj.y, si3 - sj.z, rx, ry, rz);
       if (d2 > cut2) continue;
-      const R qj = R(a.q[j]);
-      const R mj0 = R(a.mu[j]), mj1 = R(a.mu[j + n]), mj2 = R(a.mu[j + 2 * n]);
+      const R qj = R(qjd);
+      const R mj0 = R(mj0d), mj1 = R(mj1d), mj2 = R(mj2d);
       const bool dip_j = (mj0 != R(0) || mj1 != R(0) || mj2 != R(0));
       const bool dd = DD && (dip_i || dip_j);
       const bool qd = QD && (dip_i || dip_j);
-      const int tj = a.typ[j];
       const int o = (i < j) ? 0 : 1;
       const int pij = ti + MDFF_NT * tj;
       const int pji = tj + MDFF_NT * ti;
-      const bool ldamp = DAMP && a.T->dmp_on[o][pij];
+      float4 dmij = make_float4(0.f, 0.f, 0.f, 0.f);
+      if constexpr (F32)
+        if (DAMP) dmij = sdmp[o * 256 + pij];
+      const bool ldamp = DAMP && (F32 ? dmij.x != 0.f : a.T->dmp_on[o][pij] != 0);
       R d, dm1, dm3, dm5, dm7;
       if constexpr (sizeof(R) == sizeof(float)) {
         // FP32: one reciprocal square root, powers by products
@@ -214,13 +266,21 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
       R F1d_i = 0, F2d_i = 0, F1d_j = 0, F2d_j = 0;
       if (ldamp) {
         R fdmp, fdd;
-        tt_damp_t<R>(a.tt, R(a.T->dmp_b[o][pij]), R(a.T->dmp_c[o][pij]), d,
-                     a.T->dmp_k[o][pij], fdmp, fdd);
+        if constexpr (F32)
+          tt_damp_t<R>(a.tt, R(dmij.y), R(dmij.z), d, (int)dmij.w, fdmp, fdd);
+        else
+          tt_damp_t<R>(a.tt, R(a.T->dmp_b[o][pij]), R(a.T->dmp_c[o][pij]), d,
+                       a.T->dmp_k[o][pij], fdmp, fdd);
         F1d_i = -fdmp + R(1);
         F2d_i = F1d_i + (d / R(3)) * fdd;
         if (FORCE && qd) {
-          tt_damp_t<R>(a.tt, R(a.T->dmp_b[1 - o][pji]), R(a.T->dmp_c[1 - o][pji]), d,
-                       a.T->dmp_k[1 - o][pji], fdmp, fdd);
+          if constexpr (F32) {
+            const float4 dmji = sdmp[(1 - o) * 256 + pji];
+            tt_damp_t<R>(a.tt, R(dmji.y), R(dmji.z), d, (int)dmji.w, fdmp, fdd);
+          } else {
+            tt_damp_t<R>(a.tt, R(a.T->dmp_b[1 - o][pji]), R(a.T->dmp_c[1 - o][pji]), d,
+                         a.T->dmp_k[1 - o][pji], fdmp, fdd);
+          }
           F1d_j = -fdmp + R(1);
           F2d_j = F1d_j + (d / R(3)) * fdd;
         }
