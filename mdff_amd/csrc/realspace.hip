@@ -52,14 +52,29 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(SrArgs a) {
     const Cell& cell = BATCH ? a.bc[cf].cell : a.cell;
     const R rc2 = R(BATCH ? a.bc[cf].rc2 : a.rc2);
     const int* nb = a.nbr + (size_t)i * a.maxnb;
-    // the next neighbour's index one iteration ahead, a neighbour's position
-    // and type loaded together
-    int jn = sub < c ? nb[sub] : 0;
+    // two-stage pipeline over the neighbours: a neighbour's position and
+    // type are loaded one iteration ahead, its index two ahead; the loads are
+    // unconditional (slots past the count read the centre atom), so that the
+    // count of loads in flight stays exact
+    auto nbj = [&](int k) {
+      const int v = nb[min(k, c - 1)];
+      return k < c ? v : i;
+    };
+    int jn = 0, jnn = 0, tjn = 0;
+    double4 sjn = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (c > 0) {
+      jn = nbj(sub);
+      sjn = a.s4[jn];
+      tjn = a.typ[jn];
+      jnn = nbj(sub + LPA);
+    }
     for (int k = sub; k < c; k += LPA) {
-      const int j = jn;
-      if (k + LPA < c) jn = nb[k + LPA];
-      const double4 sj = a.s4[j];
-      const int tj = a.typ[j];
+      const int tj = tjn;
+      const double4 sj = sjn;
+      jn = jnn;
+      sjn = a.s4[jn];
+      tjn = a.typ[jn];
+      jnn = nbj(k + 2 * LPA);
       R rx, ry, rz;
       const R rsq = min_image_t<R>(cell, si1 - sj.x, si2 - sj.y, si3 - sj.z, rx, ry, rz);
       if (!(rsq < rc2)) continue;
