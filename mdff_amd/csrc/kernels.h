@@ -152,6 +152,10 @@ struct CombineArgs {
   const int* typ;
   const TypeTables* T;
   double* rpart;
+  // ES_IND of the conjugate-gradient SCF, with rpart: mu is the search
+  // direction p; writes Ap = alpha^-1 p - E_ind(p) instead of E and the block
+  // partials of p.Ap into row 0 of rpart
+  double* cg_Ap;
 };
 
 struct ExtrapArgs {
@@ -209,14 +213,14 @@ struct CgArgs {
   double* Ap;       // (alpha^-1 - T) p
   const double* E;  // E_stat + T v of the last ES_IND call (v = x or p)
   const double* Estat;
-  const double* sc;  // device scalars: [rz_old, pAp, rz_new] slots below
-  int s_rz, s_pap, s_rzn;
+  const double* sc;  // device scalars: r.z before and after the update
+  int s_rz, s_rzn;
+  const double* pap_part;  // k_cg_update: block partials of p.Ap (nblk)
   double* part;      // block partials, rows of nblk
   const int* rank;   // with p4: FP32 rank-ordered copy of p for k_dd_tab
   float4* p4;
 };
 __global__ void k_cg_init(CgArgs a);
-__global__ void k_cg_ap(CgArgs a);
 __global__ void k_cg_update(CgArgs a);
 __global__ void k_cg_dir(CgArgs a);
 __global__ void k_cg_upol(CgArgs a);

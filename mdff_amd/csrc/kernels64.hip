@@ -388,7 +388,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_moments(
 // field and E = E_stat + E_ind is formed; in ES_FINAL mode the forces are
 // added with coul_unit and ef_t / efg_t are written.
 template <int MODE>
-__device__ __forceinline__ void combine_atom(const CombineArgs& a, int i) {
+__device__ __forceinline__ void combine_atom(const CombineArgs& a, int i, double* pap = nullptr) {
   const int n = a.n;
   const bool EFG = (MODE == ES_FINAL) && a.do_efg;
   const bool EFIELD = (MODE != ES_FINAL) || a.do_efield;
@@ -434,7 +434,21 @@ __device__ __forceinline__ void combine_atom(const CombineArgs& a, int i) {
   if (MODE == ES_STATIC) {
     for (int r = 0; r < 3; ++r) a.E_stat[i + r * n] = ef[r];
   } else if (MODE == ES_IND) {
-    for (int r = 0; r < 3; ++r) a.E[i + r * n] = a.E_stat[i + r * n] + ef[r];
+    if (a.cg_Ap) {
+      // conjugate-gradient SCF: Ap = alpha^-1 p - T p, T p = ef
+      const int t = a.typ[i];
+      double ap[3] = {0, 0, 0};
+      if (a.T->polar[t]) {
+        const double* ip = a.T->invpol[t];
+        for (int r = 0; r < 3; ++r) {
+          ap[r] = ip[r] * m0 + ip[r + 3] * m1 + ip[r + 6] * m2 - ef[r];
+          *pap += mu3[r] * ap[r];
+        }
+      }
+      for (int r = 0; r < 3; ++r) a.cg_Ap[i + r * n] = ap[r];
+    } else {
+      for (int r = 0; r < 3; ++r) a.E[i + r * n] = a.E_stat[i + r * n] + ef[r];
+    }
   } else {
     if (a.ef_out)
       for (int r = 0; r < 3; ++r) a.ef_out[i + r * n] = ef[r];
@@ -489,8 +503,12 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_combine64(CombineArgs a) {
   __shared__ double red[4 * 2];
   double acc[2] = {0.0, 0.0};
   if (i < a.n) {
-    combine_atom<MODE>(a, i);
-    resid_atom(a, i, acc);
+    if (a.cg_Ap) {
+      combine_atom<MODE>(a, i, &acc[0]);
+    } else {
+      combine_atom<MODE>(a, i);
+      resid_atom(a, i, acc);
+    }
   }
   block_sum<2>(acc, red);
   if (threadIdx.x == 0) {
@@ -723,36 +741,22 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_cg_init(CgArgs a) {
     for (int k = 0; k < 3; ++k) a.part[k * gridDim.x + blockIdx.x] = acc[k];
 }
 
-// Ap = alpha^-1 p - (E - E_stat); row: p.Ap
-__global__ __launch_bounds__(MDFF_BLOCK) void k_cg_ap(CgArgs a) {
-  __shared__ double red[4];
-  const int n = a.n;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[1] = {0.0};
-  if (i < n) {
-    const int t = a.typ[i];
-    double ap[3] = {0, 0, 0};
-    if (a.T->polar[t]) {
-      double ip[3];
-      mat3(a.T->invpol[t], a.p, i, n, ip);
-      for (int c = 0; c < 3; ++c) {
-        ap[c] = ip[c] - (a.E[i + c * n] - a.Estat[i + c * n]);
-        acc[0] += a.p[i + c * n] * ap[c];
-      }
-    }
-    for (int c = 0; c < 3; ++c) a.Ap[i + c * n] = ap[c];
-  }
-  block_sum<1>(acc, red);
-  if (threadIdx.x == 0) a.part[blockIdx.x] = acc[0];
-}
-
 // step = rz_old / pAp: x += step p, r -= step Ap, z = alpha r; rows: r.z (new),
-// |r|^2, polarizable count
+// |r|^2, polarizable count.  p.Ap is summed from the combine's block
+// partials by every block, in the fixed order of k_reduce_rows.
 __global__ __launch_bounds__(MDFF_BLOCK) void k_cg_update(CgArgs a) {
   __shared__ double red[4 * 3];
+  __shared__ double s_pap;
   const int n = a.n;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const double pap = a.sc[a.s_pap];
+  {
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) v[0] += a.pap_part[b];
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) s_pap = v[0];
+    __syncthreads();
+  }
+  const double pap = s_pap;
   const double step = pap != 0.0 ? a.sc[a.s_rz] / pap : 0.0;
   double acc[3] = {0.0, 0.0, 0.0};
   if (i < n) {

@@ -30,7 +30,7 @@ enum {
   SC_UPOL = 34,  // 1
   SC_RMSD = 35,  // 2: sum, npol
   SC_DISP = 37,  // max squared displacement since the list build
-  SC_CG = 40,    // 7: conjugate-gradient SCF, two (r.z, |r|^2, npol) triples, p.Ap
+  SC_CG = 40,    // 6: conjugate-gradient SCF, two (r.z, |r|^2, npol) triples
   SC_N = 48
 };
 
@@ -104,6 +104,7 @@ struct mdffgpu_ctx {
   double u_pol = 0.0;
   double* d_upart = nullptr;  // block partials of u_pol (reduced once per SCF)
   double* d_cg = nullptr;     // conjugate-gradient SCF: r, z, p, Ap (3N each)
+  double* cg_Ap = nullptr;    // set while ES_IND calls are the CG's T p products
   bool upol_pending = false;
   double* h_sc = nullptr;  // pinned
   double* h_sc_dev = nullptr;  // device address of h_sc
@@ -652,6 +653,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
       ca.typ = c->d_typ;
       ca.T = c->d_tt;
       ca.rpart = c->d_part;
+      ca.cg_Ap = c->cg_Ap;
       ProfScope _ps(c, K_COMBINE);
       k_combine64<ES_IND><<<gb, MDFF_BLOCK, 0, st>>>(ca);
     }
@@ -745,6 +747,7 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
   const int n = c->n;
   const int gb = nblk(n);
   hipStream_t st = c->st;
+  c->cg_Ap = nullptr;
   HCK(c, hipMemsetAsync(c->d_muind, 0, 3 * (size_t)n * sizeof(double), st));
   *n_scf = 0;
   *rmsd_out = 0.0;
@@ -852,20 +855,23 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
     (void)dir;
     while (((iscf < P.max_scf_pol_iter && rmsd > tol) || iscf < P.min_scf_pol_iter) && !stalled) {
       ++iscf;
+      // T p by an ES_IND call whose combine forms Ap = alpha^-1 p - T p and
+      // the partials of p.Ap (row 0); the update sums those itself and
+      // writes its own partials to rows 1..3
+      c->cg_Ap = g.Ap;
       es_call(c, ES_IND, c->d_qzero, g.p, TASK_DD, 1, 0, queued_p && pk);
+      c->cg_Ap = nullptr;
       queued_p = false;
       ++*n_es;
-      k_cg_ap<<<gb, MDFF_BLOCK, 0, st>>>(g);
-      check_launch(c, "k_cg_ap");
-      reduce(c, 1, gb, SC_CG + 6);
       g.s_rz = base(cur);
-      g.s_pap = SC_CG + 6;
+      g.pap_part = c->d_part;
+      g.part = c->d_part + gb;
       k_cg_update<<<gb, MDFF_BLOCK, 0, st>>>(g);
       check_launch(c, "k_cg_update");
       const int nxt = cur ^ 1;
       {
         ProfScope _ps(c, K_REDUCE);
-        k_reduce_rows2<<<3, MDFF_BLOCK, 0, st>>>(c->d_part, gb, c->d_sc + base(nxt),
+        k_reduce_rows2<<<3, MDFF_BLOCK, 0, st>>>(c->d_part + gb, gb, c->d_sc + base(nxt),
                                                  c->h_sc_dev + base(nxt));
       }
       check_launch(c, "k_reduce_rows2");
