@@ -50,6 +50,9 @@ def parse():
                     help="SCF solver of the mixed path (the FP64 path always runs the "
                          "reference's Jacobi iteration); with pcg the mixed path is also "
                          "timed with the Jacobi iteration, reported beside the headline")
+    ap.add_argument("--event-every", type=int, default=4,
+                    help="HIP events on the dominant kernel's launches of every N-th timed "
+                         "step (each event pair stalls the queue a few microseconds)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this size first (calibrates HBM "
@@ -124,15 +127,21 @@ def time_path(args, sys_, precision, frames_dev, scf=None, tuning=None):
     stats = ff.profile_stats()
     name = max(stats.items(), key=lambda kv: kv[1][0])[0]
     # timed region: the next K frames of the trajectory, events around the
-    # launches of the dominant kernel only
+    # launches of the dominant kernel on every event_every-th step (every
+    # step has the same launch mix)
     ff.profile(True, only=name)
+    every = max(1, args.event_every)
     nscf = []
+    sampled = []
     b = W + K
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(K):
+        ff.profile_select(name if k % every == 0 else None)
         en = ff.engforce_device(frames_dev[b + k], force, itime=b + k + 1)
         nscf.append(en.n_scf)
+        if k % every == 0:
+            sampled.append(k)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms, cnt = ff.profile_stats()[name]
@@ -151,10 +160,10 @@ def time_path(args, sys_, precision, frames_dev, scf=None, tuning=None):
         peak, pipe = FL.PEAK_FP64_TFLOPS, "fp64 vector ALU"
     elif name in ("mx_rho", "mx_apply"):
         # executed MFMA work of all launches of the timed region / launches
+        # executed MFMA work of the sampled launches / their number
         kmax = list(p.kES)
-        nsc = sum(nscf)
-        per_step = FL.mixed_kspace_flops(name, n, kmax, K, nsc, p.doefg)
-        flop = per_step / cnt
+        nsc = sum(nscf[k] for k in sampled)
+        flop = FL.mixed_kspace_flops(name, n, kmax, len(sampled), nsc, p.doefg) / cnt
         peak, pipe = FL.PEAK_FP16_MFMA_TFLOPS, "fp16 matrix core (3 hi/lo products)"
     else:
         flop = None
@@ -163,7 +172,7 @@ def time_path(args, sys_, precision, frames_dev, scf=None, tuning=None):
         traffic, source = profile_traffic(name)
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": traffic, "kernel": name,
-                "avg_launch_us": avg_s * 1e6, "launches": cnt,
+                "avg_launch_us": avg_s * 1e6, "launches": cnt, "sampled_steps": len(sampled),
                 "flops_per_launch": flop, "pipe": pipe}
         if source:
             roof["traffic_source"] = source

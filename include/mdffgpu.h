@@ -241,8 +241,9 @@ const char* mdffgpu_last_error(const mdffgpu_ctx* ctx);
  * folded in at the next stream synchronisation of an evaluation.
  * ------------------------------------------------------------------- */
 int mdffgpu_profile_enable(mdffgpu_ctx* ctx, int32_t on); /* also resets */
-/* While enabled, bracket only the launches of one kernel id (-1: all), so
- * that a timed run carries the events of the kernel it reports alone. */
+/* While enabled, bracket only the launches of one kernel id (-1: all; -2:
+ * none, the totals kept), so that a timed run carries the events of the
+ * kernel it reports alone, on the steps it samples. */
 int mdffgpu_profile_only(mdffgpu_ctx* ctx, int32_t kernel);
 
 /* Launch shape of the mixed path's matrix-core reciprocal sums (picked by

@@ -135,7 +135,7 @@ const char* const kKernelNames[K_COUNT] = {
     "batch", "adjoint"};
 
 ProfScope::ProfScope(mdffgpu_ctx* ctx, int kid) : c(ctx), id(kid), a(nullptr), b(nullptr) {
-  if (!c || !c->prof || (c->prof_only >= 0 && c->prof_only != kid)) return;
+  if (!c || !c->prof || (c->prof_only != -1 && c->prof_only != kid)) return;
   if (c->ev_pool.size() < 2) {
     // timing only, read after a stream synchronisation: no system-scope
     // fence at record (it writes back and invalidates the caches, a gap of
@@ -1570,7 +1570,7 @@ int mdffgpu_profile_enable(mdffgpu_ctx* c, int32_t on) {
 }
 
 int mdffgpu_profile_only(mdffgpu_ctx* c, int32_t kernel) {
-  if (!c || kernel < -1 || kernel >= K_COUNT) return MDFFGPU_ERR_ARG;
+  if (!c || kernel < -2 || kernel >= K_COUNT) return MDFFGPU_ERR_ARG;
   c->prof_only = kernel;
   return MDFFGPU_OK;
 }

@@ -124,6 +124,16 @@ class ForceField:
             names = [L.mdffgpu_kernel_name(k).decode() for k in range(L.mdffgpu_kernel_count())]
             _lib.check(L.mdffgpu_profile_only(self._h, names.index(only)), self._h)
 
+    def profile_select(self, only: str = None) -> None:
+        """While profiling is enabled, bracket only the launches of kernel
+        ``only`` from now on, or none (None), keeping the totals."""
+        L = _lib.lib()
+        k = -2
+        if only is not None:
+            names = [L.mdffgpu_kernel_name(i).decode() for i in range(L.mdffgpu_kernel_count())]
+            k = names.index(only)
+        _lib.check(L.mdffgpu_profile_only(self._h, k), self._h)
+
     def set_mixed_tuning(self, apply_slices: int, split_atoms: int) -> bool:
         """Launch shape of the mixed path's matrix-core reciprocal sums;
         False on an FP64 context."""
