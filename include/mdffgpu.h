@@ -70,7 +70,9 @@ typedef enum mdffgpu_mixed_part {
    algo_moment_from_pola value there that computes nothing).  PCG is this
    library's preconditioned conjugate gradient on (alpha^-1 - T) mu = E_stat
    for the same fixed point, with alpha as the preconditioner: fewer Ewald
-   calls per solve, results equal to JACOBI's to the SCF tolerance. */
+   calls per solve, results equal to JACOBI's to the SCF tolerance.  The
+   batched evaluation (mdffgpu_batch_*) runs its fixed-count Jacobi
+   iteration whatever this field says. */
 typedef enum mdffgpu_scf_solver {
   MDFFGPU_SCF_JACOBI = 0,
   MDFFGPU_SCF_PCG = 1
