@@ -99,6 +99,11 @@ struct EsDirArgs {
   const int* cfg;      // batched path: configuration of each atom
   const int* rank;     // spatial rank of each atom (SCF pair table), or null
   const int* perm;     // atoms in this order (spatial locality), or null
+  // FP32 static pass of a single configuration: the compacted SCF pair
+  // table (k_dd_table's layout) written on the way, or null
+  float4* ddtab;
+  int* ddtj;
+  int* ddcnt;
 };
 
 struct RhoArgs {

@@ -93,6 +93,7 @@ struct mdffgpu_ctx {
   size_t part_cap = 0;
   void* d_t2 = nullptr;  // SCF pair tensor table (double or float)
   size_t t2_cap = 0;
+  bool t2_in_static = false;  // the next static FP32 pass writes the FP32 table
   double* d_rref = nullptr;  // positions at the last list build
   bool list_valid = false;
   int list_builds = 0;
@@ -505,6 +506,15 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   const bool mixed = (P.precision == MDFFGPU_PREC_MIXED);
   const bool mx = mixed && !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_DIRECT);
   const bool mxk = mixed && !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_KSPACE);
+  if (mode == ES_STATIC && mx && c->t2_in_static) {
+    // the evaluation's FP32 SCF pair table, from the same pairs and terms
+    const size_t cap = (size_t)c->maxnb * n;
+    da.ddtab = reinterpret_cast<float4*>(c->d_t2);
+    da.ddtj = reinterpret_cast<int*>(da.ddtab + cap);
+    da.ddcnt = da.ddtj + cap;
+    da.rank = c->d_rank;
+    c->t2_in_static = false;
+  }
   if (mode == ES_STATIC) {
     ProfScope _ps(c, mx ? K_MX_DIR : K_DIR_STATIC);
     if (mx)
@@ -695,6 +705,10 @@ void build_t2(mdffgpu_ctx* c) {
   da.alpha = c->P.alphaES;
   da.rank = c->d_rank;
   da.perm = c->d_perm;
+  // the FP32 table with an FP32 static real-space pass: written by that
+  // pass (the SCF's first Ewald call, after this one in every evaluation)
+  c->t2_in_static = mx && !(c->P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_DIRECT);
+  if (c->t2_in_static) return;
   ProfScope _ps(c, K_PAIRS_T2);
   if (mx) {
     float4* tab = reinterpret_cast<float4*>(c->d_t2);

@@ -1053,8 +1053,9 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
 // S_w = sum_n1 conj(E1[n1]) X_w[n1], S' with an extra n1, S'' with n1^2.
 constexpr int EPI_CHUNKS = 4;  // n1 ranges per slot, one wave each
 
+template <int S>
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int N1, int W,
-                                                            int ncolp, int S,
+                                                            int ncolp,
                                                             const MxSeg* __restrict__ seg,
                                                             const int* __restrict__ kseg,
                                                             const float* __restrict__ Xp,
@@ -1082,6 +1083,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int 
 #pragma unroll 7
     for (int m = m0; m < m1; ++m) {
       double xr = 0.0, xi = 0.0;
+#pragma unroll
       for (int sl = 0; sl < S; ++sl) {
         xr += xre[sl * sstride + (size_t)m * npad];
         xi += xim[sl * sstride + (size_t)m * npad];
@@ -1496,9 +1498,18 @@ static void run_kspace(MixedState* m, const KsCall& a, const double* q, const do
   }
   {
     ProfScope _ps(m->ctx, K_MX_COMBINE);
-    mx_apply_epi<<<dim3((m->npad + 63) / 64, W), MDFF_BLOCK, 0, st>>>(
-        n, m->npad, m->N1, W, NCTl * 16, m->app_slices, seg, m->d_kseg, m->X, m->tj1, m->sinv,
-        apart);
+    // the K slices a compile-time count, so that all of a thread's loads
+    // issue together
+    const dim3 ge((m->npad + 63) / 64, W);
+    if (m->app_slices == 1)
+      mx_apply_epi<1><<<ge, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, W, NCTl * 16, seg, m->d_kseg,
+                                                 m->X, m->tj1, m->sinv, apart);
+    else if (m->app_slices == 2)
+      mx_apply_epi<2><<<ge, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, W, NCTl * 16, seg, m->d_kseg,
+                                                 m->X, m->tj1, m->sinv, apart);
+    else
+      mx_apply_epi<4><<<ge, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, W, NCTl * 16, seg, m->d_kseg,
+                                                 m->X, m->tj1, m->sinv, apart);
   }
 }
 

@@ -181,7 +181,8 @@ template __global__ void k_shortrange<float, true>(SrArgs);
 // pair (dmp_* slot 0 if i < j, slot 1 otherwise).
 // Partial rows: 0 u, 1..6 tau (xx yy zz xy xz yz), before the 1/(2 Omega).
 template <int MODE, typename R, bool BATCH>
-__global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
+__global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
+    (MODE == ES_STATIC && sizeof(R) == sizeof(float) && !BATCH) ? 7 : 1))) void k_es_dir(EsDirArgs a) {
   __shared__ double red[4 * 7];
   // FP32: the Tang-Toennies damping table of the pair types in LDS, one
   // float4 (on, b, c, k) per (slot o, type pair) (the FP32 kernels convert
@@ -204,6 +205,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
   const bool EFG = (MODE == ES_FINAL) && a.do_efg;
   const bool FORCE = (MODE == ES_FINAL);
   const bool DAMP = (MODE != ES_IND);
+  // the static FP32 pass of one configuration may write the SCF pair table
+  constexpr bool TAB = (MODE == ES_STATIC) && F32 && !BATCH;
+  static_assert(LPA <= 32 && 64 % LPA == 0, "lane groups inside one wave");
+  int tbase = 0;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i0 = tid / LPA, sub = tid % LPA;
   const int i = (a.perm && i0 < n) ? a.perm[i0] : i0;  // atoms in spatial order
@@ -240,6 +245,17 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
       const int tj = a.typ[j];
       R rx, ry, rz;
       const R d2 = min_image_t<R>(cell, si1 - sj.x, si2 - sj.y, si3 - sj.z, rx, ry, rz);
+      // SCF pair table: the entry's slot among the in-cutoff pairs of the
+      // atom's lane group, in the group's visiting order
+      int tpos = 0;
+      if constexpr (TAB) {
+        if (a.ddtab) {
+          const unsigned gm = (unsigned)(__ballot(!(d2 > cut2)) >> (threadIdx.x & (64 - LPA))) &
+                              ((1u << LPA) - 1u);
+          tpos = tbase + __popc(gm & ((1u << sub) - 1u));
+          tbase += __popc(gm);
+        }
+      }
       if (d2 > cut2) continue;
       const R qj = R(qjd);
       const R mj0 = R(mj0d), mj1 = R(mj1d), mj2 = R(mj2d);
@@ -278,6 +294,15 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
                                                     : R(4) * alpha3 * d3 * expon / R(3));
       const R F1_dm3 = F1 * dm3;
       const R F2_dm5 = R(3) * F2 * dm5;
+      if constexpr (TAB) {
+        if (a.ddtab) {
+          // (v, A1), v = r sqrt(3 F2 / d^5), A1 = F1 / d^3 (undamped)
+          const R sA2 = sqrtf(F2_dm5);
+          const size_t e = (size_t)i * a.maxnb + tpos;
+          a.ddtab[e] = make_float4(rx * sA2, ry * sA2, rz * sA2, F1_dm3);
+          a.ddtj[e] = a.rank ? a.rank[j] : j;
+        }
+      }
       R F1d_i = 0, F2d_i = 0, F1d_j = 0, F2d_j = 0;
       if (ldamp) {
         R fdmp, fdd;
@@ -403,6 +428,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_dir(EsDirArgs a) {
       f2 += __shfl_xor(f2, off, 64);
     }
   }
+  if constexpr (TAB)
+    if (a.ddtab && i < n && sub == 0) a.ddcnt[i] = tbase;  // lane 0 visits every slot
   if (i < n && sub == 0) {
     if (EFIELD) {
       a.ef[i] = e0;
