@@ -218,12 +218,20 @@ struct CgArgs {
   double* Ap;       // (alpha^-1 - T) p
   const double* E;  // E_stat + T v of the last ES_IND call (v = x or p)
   const double* Estat;
-  const double* sc;  // device scalars: r.z before and after the update
-  int s_rz, s_rzn;
+  const double* sc;  // device scalars: r.z of the current iterate at s_rz
+  int s_rz;
   const double* pap_part;  // k_cg_update: block partials of p.Ap (nblk)
   double* part;      // block partials, rows of nblk
   const int* rank;   // with p4: FP32 rank-ordered copy of p for k_dd_tab
   float4* p4;
+  double* sc_out;    // k_cg_dir: the update's three sums, device scalars
+  double* sc_host;   //   and the same in mapped host memory
+  // k_cg_dir, when wt is set: FP32 structure-factor weights of p,
+  // wt[c * wt_npad + i] = (float)(2 pi (B^T p_i)_c) * wscale
+  float* wt;
+  int wt_npad;
+  float wscale;
+  double wB[9];
 };
 __global__ void k_cg_init(CgArgs a);
 __global__ void k_cg_update(CgArgs a);

@@ -783,19 +783,46 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_cg_update(CgArgs a) {
     for (int k = 0; k < 3; ++k) a.part[k * gridDim.x + blockIdx.x] = acc[k];
 }
 
-// p = z + (rz_new / rz_old) p, and its FP32 rank-ordered copy
+// p = z + (rz_new / rz_old) p, and its FP32 rank-ordered copy.  rz_new and
+// the other sums of the update (rows r.z, |r|^2, polarizable count of
+// a.part, one partial per block of the same grid) are summed by every block
+// in the fixed order of k_reduce_rows; block 0 stores the three sums at
+// a.sc_out and a.sc_host.
 __global__ __launch_bounds__(MDFF_BLOCK) void k_cg_dir(CgArgs a) {
+  __shared__ double red[4 * 3];
+  __shared__ double s_rzn;
   const int n = a.n;
+  {
+    const int nb = gridDim.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+      for (int k = 0; k < 3; ++k) v[k] += a.part[(size_t)k * nb + b];
+    block_sum<3>(v, red);
+    if (threadIdx.x == 0) {
+      s_rzn = v[0];
+      if (blockIdx.x == 0)
+        for (int k = 0; k < 3; ++k) {
+          a.sc_out[k] = v[k];
+          a.sc_host[k] = v[k];
+        }
+    }
+    __syncthreads();
+  }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double rz = a.sc[a.s_rz];
-  const double beta = rz != 0.0 ? a.sc[a.s_rzn] / rz : 0.0;
+  const double beta = rz != 0.0 ? s_rzn / rz : 0.0;
   double p[3];
   for (int c = 0; c < 3; ++c) {
     p[c] = a.z[i + c * n] + beta * a.p[i + c * n];
     a.p[i + c * n] = p[c];
   }
   if (a.p4) a.p4[a.rank[i]] = make_float4((float)p[0], (float)p[1], (float)p[2], 0.f);
+  if (a.wt)
+    for (int c = 0; c < 3; ++c) {
+      const double m = a.wB[3 * c] * p[0] + a.wB[3 * c + 1] * p[1] + a.wB[3 * c + 2] * p[2];
+      a.wt[(size_t)c * a.wt_npad + i] = (float)(m * 6.283185307179586) * a.wscale;
+    }
 }
 
 // u_pol partials of x (the quantity k_induced accumulates): x alpha^-1 x

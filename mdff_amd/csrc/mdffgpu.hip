@@ -94,6 +94,7 @@ struct mdffgpu_ctx {
   void* d_t2 = nullptr;  // SCF pair tensor table (double or float)
   size_t t2_cap = 0;
   bool t2_in_static = false;  // the next static FP32 pass writes the FP32 table
+  bool wt_ready = false;      // the next ES_IND call's dipole weights are written
   double* d_rref = nullptr;  // positions at the last list build
   bool list_valid = false;
   int list_builds = 0;
@@ -537,8 +538,10 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   const bool dip = (task & (TASK_QD | TASK_DD)) != 0;
   const bool efgv = (mode == ES_FINAL) && (do_efg || dip);
   int nsplit_apply = c->app_split;
+  const bool wt_ready = c->wt_ready && mode == ES_IND;
+  c->wt_ready = false;
   if (mxk && c->nchunk > 0) {
-    mixed::kspace(c->mx, mode, q, mu, chg, dip, efgv, c->d_apart);
+    mixed::kspace(c->mx, mode, q, mu, chg, dip, efgv, c->d_apart, wt_ready);
     nsplit_apply = 1;
   } else if (c->nchunk > 0) {
     RhoArgs ra;
@@ -837,6 +840,14 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
     g.part = c->d_part;
     g.rank = c->d_rank;
     g.p4 = pk ? reinterpret_cast<float4*>(c->d_mu4) : nullptr;
+    // with the mixed reciprocal space, k_cg_dir also writes the FP32
+    // structure-factor weights of the direction
+    if (mixed_prec && !(P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_KSPACE) && c->mx &&
+        c->nchunk > 0) {
+      g.wt = mixed::dipole_weights(c->mx, &g.wt_npad);
+      g.wscale = mixed::DIP_WEIGHT_SCALE;
+      for (int k = 0; k < 9; ++k) g.wB[k] = c->cell.B[k];
+    }
     if (!c->ev_sc) {
       HCK(c, hipEventCreateWithFlags(&c->ev_sc, hipEventDisableTiming));
       HCK(c, hipHostGetDevicePointer((void**)&c->h_sc_dev, c->h_sc, 0));
@@ -860,13 +871,6 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
     rmsd_first = rmsd;
     rmsd_best = rmsd;
     bool queued_p = false;  // the direction and its real-space field are queued
-    auto dir = [&](int from, int to) {
-      g.s_rz = base(from);
-      g.s_rzn = base(to);
-      k_cg_dir<<<gb, MDFF_BLOCK, 0, st>>>(g);
-      check_launch(c, "k_cg_dir");
-    };
-    (void)dir;
     while (((iscf < P.max_scf_pol_iter && rmsd > tol) || iscf < P.min_scf_pol_iter) && !stalled) {
       ++iscf;
       // T p by an ES_IND call whose combine forms Ap = alpha^-1 p - T p and
@@ -882,15 +886,15 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
       g.part = c->d_part + gb;
       k_cg_update<<<gb, MDFF_BLOCK, 0, st>>>(g);
       check_launch(c, "k_cg_update");
+      // the new direction from the update's sums, which the same kernel
+      // stores for the next update and for the host's convergence test
       const int nxt = cur ^ 1;
-      {
-        ProfScope _ps(c, K_REDUCE);
-        k_reduce_rows2<<<3, MDFF_BLOCK, 0, st>>>(c->d_part + gb, gb, c->d_sc + base(nxt),
-                                                 c->h_sc_dev + base(nxt));
-      }
-      check_launch(c, "k_reduce_rows2");
+      g.sc_out = c->d_sc + base(nxt);
+      g.sc_host = c->h_sc_dev + base(nxt);
+      k_cg_dir<<<gb, MDFF_BLOCK, 0, st>>>(g);
+      check_launch(c, "k_cg_dir");
+      c->wt_ready = (g.wt != nullptr);
       HCK(c, hipEventRecord(c->ev_sc, st));
-      dir(cur, nxt);
       if (pk) es_ind_dir(c, g.p, true);
       queued_p = true;
       HCK(c, hipEventSynchronize(c->ev_sc));

@@ -67,9 +67,16 @@ void destroy(MixedState* m);
 void on_cell(MixedState* m);
 // Reciprocal-space part of one multipole_ES call: writes the per-atom U, V
 // sums (apply-partials layout of k_apply64 with one split) to apart and the
-// energy / stress rows to the scalar slots.
+// energy / stress rows to the scalar slots.  With weights_ready (dipole
+// weights alone) the caller has already written the FP32 weights of mu
+// (dipole_weights).
 void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg, bool dip,
-            bool efg, double* apart);
+            bool efg, double* apart, bool weights_ready = false);
+// Structure-factor weights of dipoles alone, one configuration (slot = atom):
+// Wt[c * npad + i] = (float)(2 pi (B^T mu_i)_c) * DIP_WEIGHT_SCALE, c = 0..2;
+// the slots past the atoms stay zero.
+constexpr float DIP_WEIGHT_SCALE = 256.0f;
+float* dipole_weights(MixedState* m, int* npad);
 
 // Batched path: configurations with a common kmax, one segment each (cell,
 // alpha, atom range of the batch arrays; the padding is laid out here).

@@ -31,7 +31,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // 16 raw bytes
 
 constexpr int RHO_ATOMS_PER_SPLIT = 2048;  // default atoms per structure-factor split
-constexpr float DIP_SCALE = 256.0f;     // dipole weights are scaled by 2^8
+constexpr float DIP_SCALE = DIP_WEIGHT_SCALE;  // dipole weights are scaled by 2^8
 // The apply operand w g of each n1 plane is normalised to |.| <= APPLY_SCALE
 // (a power of two): most elements of a plane are far below its maximum, and
 // this keeps their fp16 lo parts above the subnormal range (fp16 normals
@@ -147,19 +147,14 @@ __device__ __forceinline__ void split16x2(float x0, float x1, half2v& hi, half2v
 // ---------------------------------------------------------------------------
 // Weights of the structure-factor GEMM, FP32, [w][slot] (zero on padding):
 // w0 = q (when CHG), then 2^8 * 2 pi B^T mu (when DIP), B of the slot's
-// configuration.  Also resets the per-(configuration, n1) maxima of |g|
-// that mx_rho_reduce accumulates.
+// configuration.
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_weights(int n, int npad, int chg, int dip,
                                                           const double* __restrict__ q,
                                                           const double* __restrict__ mu, Cell cell,
                                                           const MxSeg* __restrict__ seg,
                                                           const int* __restrict__ kseg,
-                                                          float* __restrict__ Wt,
-                                                          unsigned int* __restrict__ gmax,
-                                                          int nreset) {
+                                                          float* __restrict__ Wt) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0)
-    for (int t = threadIdx.x; t < nreset; t += blockDim.x) gmax[t] = 0u;
   if (j >= npad) return;
   int z;
   const int i = slot_atom(j, n, seg, kseg, z);
@@ -184,12 +179,17 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_weights(int n, int npad, int ch
 // tile rt, hi|lo, lane l: row r = rt*16 + l%16 = 2 p + x for the pair
 // p = w*N1 + n1 and x = re|im, atoms 32 g + 8 (l / 16) + (0..7), values
 // x-part of W_w[j] E1_j[n1] split into fp16 hi + lo.  Rows past 2 NW N1 are
-// zero.
+// zero.  Also resets the per-(configuration, n1) maxima of |g| that
+// mx_rho_reduce accumulates.
 template <int NRT2>
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_afrag(int npad, int N1, int NW,
                                                             const float* __restrict__ Wt,
                                                             const float2* __restrict__ tj1,
-                                                            u32x4* __restrict__ afrag) {
+                                                            u32x4* __restrict__ afrag,
+                                                            unsigned int* __restrict__ gmax,
+                                                            int nreset) {
+  if (blockIdx.x == 0)
+    for (int t = threadIdx.x; t < nreset; t += blockDim.x) gmax[t] = 0u;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t total = (size_t)(npad / 32) * NRT2 * 64;
   if (t >= total) return;
@@ -1420,6 +1420,7 @@ struct KsCall {
   Cell cell;            // one configuration: its lattice and alpha
   double alpha;
   bool energy;
+  bool weights_ready = false;  // dipole weights already in Wt (one configuration)
   double* part;         // energy / stress partials and their slots
   double* sc_rec;
 };
@@ -1442,16 +1443,21 @@ static void run_kspace(MixedState* m, const KsCall& a, const double* q, const do
   const int NRT2l = (NRT2 <= 4) ? 4 : (NRT2 <= 10) ? 10 : 13;
   {
     ProfScope _ps(m->ctx, K_MX_WEIGHTS);
-    mx_weights<<<(m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK, 0, st>>>(
-        n, m->npad, chg, dip, q, mu, a.cell, seg, m->d_kseg, m->Wt, m->gmax, m->nseg * m->N1);
+    if (!(a.weights_ready && !chg && dip && !seg))
+      mx_weights<<<(m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK, 0, st>>>(
+          n, m->npad, chg, dip, q, mu, a.cell, seg, m->d_kseg, m->Wt);
     const size_t at = (size_t)(m->npad / 32) * NRT2l * 64;
     const unsigned ga = (unsigned)((at + MDFF_BLOCK - 1) / MDFF_BLOCK);
+    const int nreset = m->nseg * m->N1;
     if (NRT2l == 4)
-      mx_rho_afrag<4><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag);
+      mx_rho_afrag<4><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag,
+                                                 m->gmax, nreset);
     else if (NRT2l == 10)
-      mx_rho_afrag<10><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag);
+      mx_rho_afrag<10><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag,
+                                                  m->gmax, nreset);
     else
-      mx_rho_afrag<13><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag);
+      mx_rho_afrag<13><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag,
+                                                  m->gmax, nreset);
   }
   {
     ProfScope _ps(m->ctx, K_MX_RHO);
@@ -1514,9 +1520,10 @@ static void run_kspace(MixedState* m, const KsCall& a, const double* q, const do
 }
 
 void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg, bool dip,
-            bool efg, double* apart) {
+            bool efg, double* apart, bool weights_ready) {
   CtxView v = view(m->ctx);
   KsCall a;
+  a.weights_ready = weights_ready;
   a.st = v.st;
   a.s = v.s;
   a.epoch = v.evals;
@@ -1526,6 +1533,11 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
   a.part = v.part;
   a.sc_rec = v.sc + v.sc_rec;
   run_kspace(m, a, q, mu, chg, dip, efg, apart);
+}
+
+float* dipole_weights(MixedState* m, int* npad) {
+  *npad = m->npad;
+  return m->d_seg ? nullptr : m->Wt;  // a batch has its own slot layout
 }
 
 void kspace_batch(MixedState* m, hipStream_t st, const double* s, long long epoch,
