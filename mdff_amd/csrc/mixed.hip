@@ -19,6 +19,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "mixed.h"
@@ -49,7 +50,7 @@ struct MixedState {
   float2* tj1 = nullptr;  // [n1][j]   (N1 x npad)
   float2* tj2 = nullptr;  // [n2+ky][j]
   float2* tj3 = nullptr;  // [n3+kz][j]
-  double* rpart = nullptr;      // [split][row][col]  FP64 partials
+  double* rpart = nullptr;      // [split][row][col]  FP64 (or FP32) partials
   size_t rpart_cap = 0;
   double2* g = nullptr;         // [n1][c]
   unsigned int* gmax = nullptr; // max |g| per n1 plane, as float bits
@@ -264,6 +265,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_afrag(int npad, int N1, int
 // D column, and those are not stored.  Output: part[split][p][col] with S
 // at Re plane col and Im plane C23p + col, for both c and c'.
 constexpr int NXCD = 8;
+
+// element type of the per-split structure-factor sums: FP64 where the FP32
+// chains are flushed into FP64, FP32 where a split is one FP32 chain
+template <bool F64>
+using RhoPart = typename std::conditional<F64, double, float>::type;
 
 // B fragments (Re hi, Re lo, Im hi, Im lo) of one K step and column from
 // the phases of 8 atoms, e = E2 rows (4 float4 = 8 atoms as re, im pairs)
@@ -486,7 +492,7 @@ template <int NRT2, bool F64>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void mx_rho_gen(
     int C23, int CH, int C23p, int nrt_used, int ncb, int nblk_used, const int2* __restrict__ splits,
     const u32x4* __restrict__ afrag, int npad, int N3, const float2* __restrict__ tj2,
-    const float2* __restrict__ tj3, double* __restrict__ out) {
+    const float2* __restrict__ tj3, RhoPart<F64>* __restrict__ out) {
   constexpr int NBUF = 3;
   __shared__ u32x4 sA[NBUF][NRT2][2][64];  // [buf][rt][hi|lo][lane], fp16 x8
   const int per = gridDim.x / NXCD;
@@ -617,7 +623,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     else return (double)acc[y][r][e];
   };
   if (ctile * 16 >= CH || c >= CH) return;
-  double* o = out + (size_t)split * (NRT2 * 8) * (2 * C23p);
+  // FP32 chains: the split's sums are FP32 values, stored as such
+  using PT = RhoPart<F64>;
+  PT* o = out + (size_t)split * (NRT2 * 8) * (2 * C23p);
   const int cp = C23 - 1 - c;
 #pragma unroll
   for (int r = 0; r < NRT2; ++r) {
@@ -627,12 +635,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       const int p = r * 8 + 2 * kgrp + h;
       const double Prr = P(0, r, 2 * h), Pir = P(0, r, 2 * h + 1);
       const double Pri = P(1, r, 2 * h), Pii = P(1, r, 2 * h + 1);
-      double* op = o + (size_t)p * (2 * C23p);
-      op[c] = Prr - Pii;
-      op[C23p + c] = Pri + Pir;
+      PT* op = o + (size_t)p * (2 * C23p);
+      op[c] = (PT)(Prr - Pii);
+      op[C23p + c] = (PT)(Pri + Pir);
       if (cp != c) {  // the centre column (0, 0) pairs with itself
-        op[cp] = Prr + Pii;
-        op[C23p + cp] = Pir - Pri;
+        op[cp] = (PT)(Prr + Pii);
+        op[C23p + cp] = (PT)(Pir - Pri);
       }
     }
   }
@@ -644,9 +652,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 // (src/field.f90:2742-2745, 2803-2815), and max |g|.
 // Grid (column blocks, N1, configurations): one n1 plane of one
 // configuration per block row; g and the maxima are per configuration.
+template <typename PT>
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
     int N1, int N2, int N3, int C23, int C23p, int NW, int chg, int dip, int nsplit, int split_rows,
-    const double* __restrict__ part, Cell cell_, double alpha_, const MxSeg* __restrict__ seg,
+    const PT* __restrict__ part, Cell cell_, double alpha_, const MxSeg* __restrict__ seg,
     int do_energy, double2* __restrict__ g, unsigned int* __restrict__ gmax,
     double* __restrict__ pout) {
   __shared__ double red[4 * 7];
@@ -668,9 +677,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
     const int n2 = c / N3 - ky, n3 = c % N3 - kz;
     double Mr[4] = {0, 0, 0, 0}, Mi[4] = {0, 0, 0, 0};
     const size_t sstr = (size_t)split_rows * (2 * C23p);
-    const double* p0 = part + (size_t)sp0 * sstr + (size_t)n1 * (2 * C23p) + c;
+    const PT* p0 = part + (size_t)sp0 * sstr + (size_t)n1 * (2 * C23p) + c;
     for (int w = 0; w < NW; ++w) {
-      const double* p = p0 + (size_t)w * N1 * (2 * C23p);
+      const PT* p = p0 + (size_t)w * N1 * (2 * C23p);
       double vr[4], vi[4];
       int sp = 0;
       for (; sp + 4 <= nsplit; sp += 4) {
@@ -1394,7 +1403,7 @@ static void launch_rho_gen(MixedState* m, hipStream_t st, int nrt_used) {
   const int grid = (used + NXCD - 1) / NXCD * NXCD;
   mx_rho_gen<NRT2, F64><<<grid, 256, 0, st>>>(m->C23, m->CH, m->C23p, nrt_used, ncb, used,
                                                m->d_split, m->afrag, m->npad, m->N3, m->tj2,
-                                               m->tj3, m->rpart);
+                                               m->tj3, reinterpret_cast<RhoPart<F64>*>(m->rpart));
 }
 
 template <int NCT, int RT>
@@ -1476,9 +1485,16 @@ static void run_kspace(MixedState* m, const KsCall& a, const double* q, const do
   const int gk = gred.x * gred.y;
   {
     ProfScope _ps(m->ctx, K_MX_RHO_RED);
-    mx_rho_reduce<<<gred, MDFF_BLOCK, 0, st>>>(m->N1, m->N2, m->N3, m->C23, m->C23p, NW, chg, dip,
-                                             m->nsplit, NRT2l * 8, m->rpart, a.cell, a.alpha, seg,
-                                             a.energy, m->g, m->gmax, a.part);
+    // FP32 split sums from the FP32-chain kernels (dipole weights alone)
+    if (!chg && NRT2l != 13)
+      mx_rho_reduce<float><<<gred, MDFF_BLOCK, 0, st>>>(
+          m->N1, m->N2, m->N3, m->C23, m->C23p, NW, chg, dip, m->nsplit, NRT2l * 8,
+          reinterpret_cast<const float*>(m->rpart), a.cell, a.alpha, seg, a.energy, m->g, m->gmax,
+          a.part);
+    else
+      mx_rho_reduce<double><<<gred, MDFF_BLOCK, 0, st>>>(
+          m->N1, m->N2, m->N3, m->C23, m->C23p, NW, chg, dip, m->nsplit, NRT2l * 8, m->rpart,
+          a.cell, a.alpha, seg, a.energy, m->g, m->gmax, a.part);
   }
   if (a.energy) k_reduce_rows<<<7, MDFF_BLOCK, 0, st>>>(a.part, gk, a.sc_rec);
   const int W = efg ? 6 : 3;
