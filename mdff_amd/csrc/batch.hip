@@ -725,8 +725,8 @@ void es_call(mdffgpu_batch* b, int mode, const double* q, const double* mu, cons
       const size_t cap = (size_t)b->maxnb * N;
       const float4* tab = reinterpret_cast<const float4*>(b->d_t2);
       const int* tj = reinterpret_cast<const int*>(tab + cap);
-      k_dd_tab<<<nblk((size_t)N * 64), MDFF_BLOCK, 0, b->st>>>(N, b->maxnb, tab, tj, tj + cap, m4,
-                                                              nullptr, b->d_dir);
+      k_dd_tab<<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(N, b->maxnb, tab, tj, tj + cap, m4,
+                                                           nullptr, b->d_dir);
     }
   } else {
     da.task = TASK_QQ | (b->pim ? (TASK_QD | TASK_DD) : 0);

@@ -2,6 +2,8 @@
 // kernel files and the host orchestration (mdffgpu.hip).
 #pragma once
 
+#include <algorithm>
+
 #include "common.h"
 
 struct mdffgpu_ctx;
@@ -194,6 +196,20 @@ template <bool BATCH = false>
 __global__ void k_dd_table(EsDirArgs a, float4* tab, int* tj, int* tcnt);
 __global__ void k_dd_tab(int n, int maxnb, const float4* tab, const int* tj, const int* tcnt,
                          const float4* mu4, const int* perm, double* ef);
+// Grid of k_dd_tab (persistent waves): four per SIMD of the device, at most
+// one per atom.
+inline int dd_tab_blocks(int n) {
+  static int cus = 0;
+  if (cus <= 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+  }
+  const long long waves = std::min<long long>(n, (long long)cus * 16);
+  return (int)((waves * 64 + MDFF_BLOCK - 1) / MDFF_BLOCK);
+}
 template <bool CHG, bool DIP>
 __global__ void k_rho64(RhoArgs a);
 __global__ void k_rho_reduce64(RhoRedArgs a);

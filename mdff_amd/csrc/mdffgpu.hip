@@ -453,8 +453,8 @@ void es_ind_dir(mdffgpu_ctx* c, const double* mu, bool packed = false) {
     const float4* tab = reinterpret_cast<const float4*>(c->d_t2);
     const int* tj = reinterpret_cast<const int*>(tab + cap);
     if (!packed) k_pack_mu_ranked<<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_rank, m4);
-    k_dd_tab<<<nblk(n * 64), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4, c->d_perm,
-                                                  c->d_efdir);
+    k_dd_tab<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4,
+                                                      c->d_perm, c->d_efdir);
   } else {
     EsDirArgs da;
     std::memset(&da, 0, sizeof(da));

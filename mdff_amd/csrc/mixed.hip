@@ -652,9 +652,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 // (src/field.f90:2742-2745, 2803-2815), and max |g|.
 // Grid (column blocks, N1, configurations): one n1 plane of one
 // configuration per block row; g and the maxima are per configuration.
-template <typename PT>
+template <typename PT, int NW>
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
-    int N1, int N2, int N3, int C23, int C23p, int NW, int chg, int dip, int nsplit, int split_rows,
+    int N1, int N2, int N3, int C23, int C23p, int chg, int dip, int nsplit, int split_rows,
     const PT* __restrict__ part, Cell cell_, double alpha_, const MxSeg* __restrict__ seg,
     int do_energy, double2* __restrict__ g, unsigned int* __restrict__ gmax,
     double* __restrict__ pout) {
@@ -678,27 +678,35 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_reduce(
     double Mr[4] = {0, 0, 0, 0}, Mi[4] = {0, 0, 0, 0};
     const size_t sstr = (size_t)split_rows * (2 * C23p);
     const PT* p0 = part + (size_t)sp0 * sstr + (size_t)n1 * (2 * C23p) + c;
-    for (int w = 0; w < NW; ++w) {
-      const PT* p = p0 + (size_t)w * N1 * (2 * C23p);
-      double vr[4], vi[4];
-      int sp = 0;
-      for (; sp + 4 <= nsplit; sp += 4) {
+    const size_t wstr = (size_t)N1 * (2 * C23p);
+    // per weight class, the splits in order; the loads of all classes of
+    // four splits issue together
+    int sp = 0;
+    for (; sp + 4 <= nsplit; sp += 4) {
+      PT vr[NW][4], vi[NW][4];
+#pragma unroll
+      for (int w = 0; w < NW; ++w)
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          vr[u] = p[(sp + u) * sstr];
-          vi[u] = p[(sp + u) * sstr + C23p];
+          const PT* p = p0 + (size_t)w * wstr + (size_t)(sp + u) * sstr;
+          vr[w][u] = p[0];
+          vi[w][u] = p[C23p];
         }
 #pragma unroll
+      for (int w = 0; w < NW; ++w)
+#pragma unroll
         for (int u = 0; u < 4; ++u) {
-          Mr[w] += vr[u];
-          Mi[w] += vi[u];
+          Mr[w] += vr[w][u];
+          Mi[w] += vi[w][u];
         }
-      }
-      for (; sp < nsplit; ++sp) {
-        Mr[w] += p[sp * sstr];
-        Mi[w] += p[sp * sstr + C23p];
-      }
     }
+    for (; sp < nsplit; ++sp)
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        const PT* p = p0 + (size_t)w * wstr + (size_t)sp * sstr;
+        Mr[w] += p[0];
+        Mi[w] += p[C23p];
+      }
     double rR = 0.0, rI = 0.0;
     int w = 0;
     if (chg) {
@@ -1486,15 +1494,22 @@ static void run_kspace(MixedState* m, const KsCall& a, const double* q, const do
   {
     ProfScope _ps(m->ctx, K_MX_RHO_RED);
     // FP32 split sums from the FP32-chain kernels (dipole weights alone)
+    const double* rp = m->rpart;
+    const float* rpf = reinterpret_cast<const float*>(m->rpart);
+#define MX_RED(PT, NWC, P)                                                                     \
+  mx_rho_reduce<PT, NWC><<<gred, MDFF_BLOCK, 0, st>>>(m->N1, m->N2, m->N3, m->C23, m->C23p,    \
+                                                      chg, dip, m->nsplit, NRT2l * 8, P, a.cell, \
+                                                      a.alpha, seg, a.energy, m->g, m->gmax,    \
+                                                      a.part)
     if (!chg && NRT2l != 13)
-      mx_rho_reduce<float><<<gred, MDFF_BLOCK, 0, st>>>(
-          m->N1, m->N2, m->N3, m->C23, m->C23p, NW, chg, dip, m->nsplit, NRT2l * 8,
-          reinterpret_cast<const float*>(m->rpart), a.cell, a.alpha, seg, a.energy, m->g, m->gmax,
-          a.part);
+      MX_RED(float, 3, rpf);
+    else if (NW == 1)
+      MX_RED(double, 1, rp);
+    else if (NW == 3)
+      MX_RED(double, 3, rp);
     else
-      mx_rho_reduce<double><<<gred, MDFF_BLOCK, 0, st>>>(
-          m->N1, m->N2, m->N3, m->C23, m->C23p, NW, chg, dip, m->nsplit, NRT2l * 8, m->rpart,
-          a.cell, a.alpha, seg, a.energy, m->g, m->gmax, a.part);
+      MX_RED(double, 4, rp);
+#undef MX_RED
   }
   if (a.energy) k_reduce_rows<<<7, MDFF_BLOCK, 0, st>>>(a.part, gk, a.sc_rec);
   const int W = efg ? 6 : 3;

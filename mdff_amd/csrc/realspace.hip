@@ -298,7 +298,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
         if (a.ddtab) {
           // (v, A1), v = r sqrt(3 F2 / d^5), A1 = F1 / d^3 (undamped)
           const R sA2 = sqrtf(F2_dm5);
-          const size_t e = (size_t)i * a.maxnb + tpos;
+          const size_t e = (size_t)i0 * a.maxnb + tpos;
           a.ddtab[e] = make_float4(rx * sA2, ry * sA2, rz * sA2, F1_dm3);
           a.ddtj[e] = a.rank ? a.rank[j] : j;
         }
@@ -429,7 +429,13 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
     }
   }
   if constexpr (TAB)
-    if (a.ddtab && i < n && sub == 0) a.ddcnt[i] = tbase;  // lane 0 visits every slot
+    if (a.ddtab && i < n && sub == 0) {  // lane 0 visits every slot
+      a.ddcnt[i0] = tbase;
+      if (tbase == 0) {  // a row without pairs holds one zero entry (k_dd_tab reads it)
+        a.ddtab[(size_t)i0 * a.maxnb] = make_float4(0.f, 0.f, 0.f, 0.f);
+        a.ddtj[(size_t)i0 * a.maxnb] = 0;
+      }
+    }
   if (i < n && sub == 0) {
     if (EFIELD) {
       a.ef[i] = e0;
@@ -603,13 +609,15 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a,
 
 // ---------------------------------------------------------------------------
 // SCF dipole field of the mixed path, from a compacted FP32 pair table.  Once
-// per evaluation k_dd_table keeps, per atom and in list order, only the list
-// entries inside the cutoff, as (v, A1) with v = r sqrt(A2), A2 = 3F2/d^5,
-// A1 = F1/d^3 (the undamped d-d terms of src/field.f90:2407-2421, formed in
-// FP32 from an FP64 fractional difference), plus the neighbour's slot in the
-// dipole array (its spatial rank when a.rank is given, else its index).  Each
-// SCF iteration k_dd_tab then applies E_i = sum_j [v (v.mu_j) - A1 mu_j]:
-// 20 bytes of table and one dipole gather per pair.  One wave per atom.
+// per evaluation k_dd_table (or the static FP32 real-space pass) keeps, per
+// atom and in list order, only the list entries inside the cutoff, as
+// (v, A1) with v = r sqrt(A2), A2 = 3F2/d^5, A1 = F1/d^3 (the undamped d-d
+// terms of src/field.f90:2407-2421, formed in FP32 from an FP64 fractional
+// difference), plus the neighbour's slot in the dipole array (its spatial
+// rank when a.rank is given, else its index).  Rows and counts are in the
+// processing order i0 (atom perm[i0] when perm is given).  Each SCF
+// iteration k_dd_tab then applies E_i = sum_j [v (v.mu_j) - A1 mu_j]: 20
+// bytes of table and one dipole gather per pair.
 template <bool BATCH>
 __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __restrict__ tab,
                                                          int* __restrict__ tj,
@@ -627,7 +635,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
   const float c43a3 = (float)(4.0 / 3.0 * alphad * alphad * alphad);
   const double4 si = a.s4[i];
   const int c = a.cnt[i];
-  const size_t row = (size_t)i * a.maxnb;
+  const size_t row = (size_t)i0 * a.maxnb;  // rows in processing order
   const unsigned long long below = (1ull << lane) - 1ull;
   int base = 0;
   for (int k0 = 0; k0 < c; k0 += 64) {
@@ -660,42 +668,68 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
     }
     base += __popcll(m);
   }
-  if (lane == 0) tcnt[i] = base;
+  if (lane == 0) {
+    tcnt[i0] = base;
+    if (base == 0) {  // a row without pairs holds one zero entry (k_dd_tab reads it)
+      tab[row] = make_float4(0.f, 0.f, 0.f, 0.f);
+      tj[row] = 0;
+    }
+  }
 }
 
-__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_tab(int n, int maxnb,
-                                                       const float4* __restrict__ tab,
-                                                       const int* __restrict__ tj,
-                                                       const int* __restrict__ tcnt,
-                                                       const float4* __restrict__ mu4,
-                                                       const int* __restrict__ perm,
-                                                       double* __restrict__ ef) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i0 = tid >> 6, lane = tid & 63;
-  if (i0 >= n) return;  // whole waves
-  const int i = perm ? perm[i0] : i0;  // atoms in spatial order
-  const int c = tcnt[i];
-  const size_t row = (size_t)i * maxnb;
-  float e0 = 0.f, e1 = 0.f, e2 = 0.f;
-  // 8 pairs per lane in flight: one pass covers 512 neighbours
+// One wave per atom, persistent: the waves of the grid take the rows
+// i0 = wave, wave + waves, ...; while a wave gathers and sums one row, the
+// loads of its next row are already in flight, so that the table stream
+// does not pause.  The first 512 entries of a row are pipelined (8 per
+// lane); longer rows finish in a plain loop.  The pipelined loads are
+// issued unconditionally: indexes are clamped to the row's last entry (a
+// row without pairs holds one zero entry) and to the last row, and the
+// entries past the count are masked by a zero factor, so that no branch
+// splits the loads in flight.
+__global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_dd_tab(
+    int n, int maxnb, const float4* __restrict__ tab, const int* __restrict__ tj,
+    const int* __restrict__ tcnt, const float4* __restrict__ mu4, const int* __restrict__ perm,
+    double* __restrict__ ef) {
   constexpr int U = 8;
-  for (int k0 = lane; k0 < c; k0 += U * 64) {
-    float4 g[U], m[U];
-    int jj[U];
+  const int lane = threadIdx.x & 63;
+  const int wid =
+      __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const int nw = (int)((gridDim.x * blockDim.x) >> 6);
+  // row r (clamped to the last row) with count c: entries, zero past c
+  auto load = [&](int r, int c, float4 (&g)[U], int (&jj)[U]) {
+    const size_t row = (size_t)min(r, n - 1) * maxnb;
+    const float4* trow = tab + row;
+    const int* jrow = tj + row;
+    const int last = max(c - 1, 0);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int k = k0 + u * 64;
-      g[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-      jj[u] = i;
-      if (k < c) {
-        g[u] = tab[row + k];
-        jj[u] = tj[row + k];
-      }
+      const int k = lane + u * 64;
+      const int kc = min(k, last);
+      const float4 v = trow[kc];
+      jj[u] = jrow[kc];
+      const float mk = k < c ? 1.f : 0.f;
+      g[u] = make_float4(v.x * mk, v.y * mk, v.z * mk, v.w * mk);
     }
+  };
+  float4 g[U];
+  int jj[U];
+  int r = wid;
+  int c = tcnt[min(r, n - 1)];
+  int cn = tcnt[min(r + nw, n - 1)];
+  load(r, c, g, jj);
+  for (; r < n; r += nw) {
+    const int iout = perm ? perm[r] : r;
+    float4 m[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) m[u] = mu4[jj[u]];
-    // all gathers in flight before the first use (the scheduler would
-    // otherwise trade them for registers and serialise them)
+    // the next row and the count of the one after it (rows past the last
+    // are loaded clamped and never used)
+    float4 gn[U];
+    int jn[U];
+    load(r + nw, cn, gn, jn);
+    const int cnn = tcnt[min(r + 2 * nw, n - 1)];
+    // the gathers and the next row's loads all in flight before the first
+    // use of a gather
     static_assert(U == 8, "operand list below");
     asm volatile(""
                  : "+v"(m[0].x), "+v"(m[0].y), "+v"(m[0].z), "+v"(m[1].x), "+v"(m[1].y),
@@ -703,6 +737,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_tab(int n, int maxnb,
                    "+v"(m[3].y), "+v"(m[3].z), "+v"(m[4].x), "+v"(m[4].y), "+v"(m[4].z),
                    "+v"(m[5].x), "+v"(m[5].y), "+v"(m[5].z), "+v"(m[6].x), "+v"(m[6].y),
                    "+v"(m[6].z), "+v"(m[7].x), "+v"(m[7].y), "+v"(m[7].z));
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const float vm = g[u].x * m[u].x + g[u].y * m[u].y + g[u].z * m[u].z;
@@ -710,16 +745,33 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_tab(int n, int maxnb,
       e1 += g[u].y * vm - g[u].w * m[u].y;
       e2 += g[u].z * vm - g[u].w * m[u].z;
     }
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    e0 += __shfl_xor(e0, off, 64);
-    e1 += __shfl_xor(e1, off, 64);
-    e2 += __shfl_xor(e2, off, 64);
-  }
-  if (lane == 0) {
-    ef[i] = e0;
-    ef[i + n] = e1;
-    ef[i + 2 * n] = e2;
+    // entries past the pipelined 512
+    const size_t row = (size_t)r * maxnb;
+    for (int k = U * 64 + lane; k < c; k += 64) {
+      const float4 v = tab[row + k];
+      const float4 mj = mu4[tj[row + k]];
+      const float vm = v.x * mj.x + v.y * mj.y + v.z * mj.z;
+      e0 += v.x * vm - v.w * mj.x;
+      e1 += v.y * vm - v.w * mj.y;
+      e2 += v.z * vm - v.w * mj.z;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      e0 += __shfl_xor(e0, off, 64);
+      e1 += __shfl_xor(e1, off, 64);
+      e2 += __shfl_xor(e2, off, 64);
+    }
+    if (lane == 0) {
+      ef[iout] = e0;
+      ef[iout + n] = e1;
+      ef[iout + 2 * n] = e2;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      g[u] = gn[u];
+      jj[u] = jn[u];
+    }
+    c = cn;
+    cn = cnn;
   }
 }
 
