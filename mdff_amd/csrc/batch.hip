@@ -725,8 +725,13 @@ void es_call(mdffgpu_batch* b, int mode, const double* q, const double* mu, cons
       const size_t cap = (size_t)b->maxnb * N;
       const float4* tab = reinterpret_cast<const float4*>(b->d_t2);
       const int* tj = reinterpret_cast<const int*>(tab + cap);
-      k_dd_tab<<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(N, b->maxnb, tab, tj, tj + cap, m4,
-                                                           nullptr, b->d_dir);
+      if (dd_idx16(N))
+        k_dd_tab<<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(
+            N, b->maxnb, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4, nullptr,
+            b->d_dir);
+      else
+        k_dd_tab<<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(N, b->maxnb, tab, tj, tj + cap, m4,
+                                                             nullptr, b->d_dir);
     }
   } else {
     da.task = TASK_QQ | (b->pim ? (TASK_QD | TASK_DD) : 0);
@@ -1099,6 +1104,7 @@ void build_pairs(mdffgpu_batch* b) {
   } else {
     float4* tab = reinterpret_cast<float4*>(b->d_t2);
     int* tj = reinterpret_cast<int*>(tab + cap);
+    da.dd16 = dd_idx16(b->N);
     k_dd_table<true><<<nblk((size_t)b->N * 64), MDFF_BLOCK, 0, b->st>>>(da, tab, tj, tj + cap);
   }
   launch_check(b, "batch pair table");

@@ -106,6 +106,7 @@ struct EsDirArgs {
   float4* ddtab;
   int* ddtj;
   int* ddcnt;
+  int dd16;  // pair table: neighbour slots as 16-bit indexes (n <= 65536)
 };
 
 struct RhoArgs {
@@ -194,8 +195,11 @@ __global__ void k_pack_mu(int n, const double* mu, R4* mu4);
 __global__ void k_pack_mu_ranked(int n, const double* mu, const int* rank, float4* mu4);
 template <bool BATCH = false>
 __global__ void k_dd_table(EsDirArgs a, float4* tab, int* tj, int* tcnt);
-__global__ void k_dd_tab(int n, int maxnb, const float4* tab, const int* tj, const int* tcnt,
+template <typename IT>
+__global__ void k_dd_tab(int n, int maxnb, const float4* tab, const IT* tj, const int* tcnt,
                          const float4* mu4, const int* perm, double* ef);
+// neighbour slots of the FP32 pair table fit 16 bits
+inline bool dd_idx16(long long n) { return n <= 65536; }
 // Grid of k_dd_tab (persistent waves): four per SIMD of the device, at most
 // one per atom.
 inline int dd_tab_blocks(int n) {

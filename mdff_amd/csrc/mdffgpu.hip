@@ -453,8 +453,13 @@ void es_ind_dir(mdffgpu_ctx* c, const double* mu, bool packed = false) {
     const float4* tab = reinterpret_cast<const float4*>(c->d_t2);
     const int* tj = reinterpret_cast<const int*>(tab + cap);
     if (!packed) k_pack_mu_ranked<<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_rank, m4);
-    k_dd_tab<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4,
-                                                      c->d_perm, c->d_efdir);
+    if (dd_idx16(n))
+      k_dd_tab<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(
+          n, c->maxnb, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4, c->d_perm,
+          c->d_efdir);
+    else
+      k_dd_tab<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4,
+                                                        c->d_perm, c->d_efdir);
   } else {
     EsDirArgs da;
     std::memset(&da, 0, sizeof(da));
@@ -514,6 +519,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     da.ddtj = reinterpret_cast<int*>(da.ddtab + cap);
     da.ddcnt = da.ddtj + cap;
     da.rank = c->d_rank;
+    da.dd16 = dd_idx16(n);
     c->t2_in_static = false;
   }
   if (mode == ES_STATIC) {
@@ -708,6 +714,7 @@ void build_t2(mdffgpu_ctx* c) {
   da.alpha = c->P.alphaES;
   da.rank = c->d_rank;
   da.perm = c->d_perm;
+  da.dd16 = dd_idx16(n);
   // the FP32 table with an FP32 static real-space pass: written by that
   // pass (the SCF's first Ewald call, after this one in every evaluation)
   c->t2_in_static = mx && !(c->P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_DIRECT);
@@ -871,6 +878,7 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
     rmsd_first = rmsd;
     rmsd_best = rmsd;
     bool queued_p = false;  // the direction and its real-space field are queued
+    double rmsd_prev = 0.0;  // the residual before the current one
     while (((iscf < P.max_scf_pol_iter && rmsd > tol) || iscf < P.min_scf_pol_iter) && !stalled) {
       ++iscf;
       // T p by an ES_IND call whose combine forms Ap = alpha^-1 p - T p and
@@ -895,10 +903,18 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
       check_launch(c, "k_cg_dir");
       c->wt_ready = (g.wt != nullptr);
       HCK(c, hipEventRecord(c->ev_sc, st));
-      if (pk) es_ind_dir(c, g.p, true);
-      queued_p = true;
+      // the new direction's real-space field is queued behind the residual
+      // unless the last two residuals, extrapolated geometrically, put this
+      // one below the tolerance (then the solve most likely stops here)
+      const bool last_likely =
+          iscf >= P.min_scf_pol_iter &&
+          (iscf >= P.max_scf_pol_iter ||
+           (iscf >= 2 && rmsd_prev > 0.0 && rmsd * (rmsd / rmsd_prev) < tol));
+      queued_p = pk && !last_likely;
+      if (queued_p) es_ind_dir(c, g.p, true);
       HCK(c, hipEventSynchronize(c->ev_sc));
       cur = nxt;
+      rmsd_prev = rmsd;
       rmsd = std::sqrt(c->h_sc[base(cur) + 1] / c->h_sc[base(cur) + 2]);
       if (rmsd < rmsd_best) {
         rmsd_best = rmsd;

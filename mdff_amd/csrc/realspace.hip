@@ -300,7 +300,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
           const R sA2 = sqrtf(F2_dm5);
           const size_t e = (size_t)i0 * a.maxnb + tpos;
           a.ddtab[e] = make_float4(rx * sA2, ry * sA2, rz * sA2, F1_dm3);
-          a.ddtj[e] = a.rank ? a.rank[j] : j;
+          const int js = a.rank ? a.rank[j] : j;
+          if (a.dd16)
+            reinterpret_cast<unsigned short*>(a.ddtj)[e] = (unsigned short)js;
+          else
+            a.ddtj[e] = js;
         }
       }
       R F1d_i = 0, F2d_i = 0, F1d_j = 0, F2d_j = 0;
@@ -433,7 +437,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
       a.ddcnt[i0] = tbase;
       if (tbase == 0) {  // a row without pairs holds one zero entry (k_dd_tab reads it)
         a.ddtab[(size_t)i0 * a.maxnb] = make_float4(0.f, 0.f, 0.f, 0.f);
-        a.ddtj[(size_t)i0 * a.maxnb] = 0;
+        if (a.dd16)
+          reinterpret_cast<unsigned short*>(a.ddtj)[(size_t)i0 * a.maxnb] = 0;
+        else
+          a.ddtj[(size_t)i0 * a.maxnb] = 0;
       }
     }
   if (i < n && sub == 0) {
@@ -664,7 +671,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
     if (in) {
       const size_t e = row + base + __popcll(m & below);
       tab[e] = v;
-      tj[e] = a.rank ? a.rank[j] : j;
+      const int js = a.rank ? a.rank[j] : j;
+      if (a.dd16)
+        reinterpret_cast<unsigned short*>(tj)[e] = (unsigned short)js;
+      else
+        tj[e] = js;
     }
     base += __popcll(m);
   }
@@ -672,7 +683,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
     tcnt[i0] = base;
     if (base == 0) {  // a row without pairs holds one zero entry (k_dd_tab reads it)
       tab[row] = make_float4(0.f, 0.f, 0.f, 0.f);
-      tj[row] = 0;
+      if (a.dd16)
+        reinterpret_cast<unsigned short*>(tj)[row] = 0;
+      else
+        tj[row] = 0;
     }
   }
 }
@@ -686,8 +700,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
 // row without pairs holds one zero entry) and to the last row, and the
 // entries past the count are masked by a zero factor, so that no branch
 // splits the loads in flight.
+template <typename IT>
 __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_dd_tab(
-    int n, int maxnb, const float4* __restrict__ tab, const int* __restrict__ tj,
+    int n, int maxnb, const float4* __restrict__ tab, const IT* __restrict__ tj,
     const int* __restrict__ tcnt, const float4* __restrict__ mu4, const int* __restrict__ perm,
     double* __restrict__ ef) {
   constexpr int U = 8;
@@ -699,7 +714,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   auto load = [&](int r, int c, float4 (&g)[U], int (&jj)[U]) {
     const size_t row = (size_t)min(r, n - 1) * maxnb;
     const float4* trow = tab + row;
-    const int* jrow = tj + row;
+    const IT* jrow = tj + row;
     const int last = max(c - 1, 0);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -775,6 +790,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   }
 }
 
+template __global__ void k_dd_tab<int>(int, int, const float4*, const int*, const int*,
+                                       const float4*, const int*, double*);
+template __global__ void k_dd_tab<unsigned short>(int, int, const float4*, const unsigned short*,
+                                                  const int*, const float4*, const int*, double*);
 template __global__ void k_pairs_t2<false>(EsDirArgs, double*);
 template __global__ void k_pairs_t2<true>(EsDirArgs, double*);
 template __global__ void k_dd_table<false>(EsDirArgs, float4*, int*, int*);
