@@ -95,6 +95,9 @@ struct mdffgpu_ctx {
   size_t t2_cap = 0;
   bool t2_in_static = false;  // the next static FP32 pass writes the FP32 table
   bool wt_ready = false;      // the next ES_IND call's dipole weights are written
+  // device-resident outputs of the current call, copied before its final
+  // synchronisation (null: none)
+  double *out_force = nullptr, *out_mu = nullptr, *out_ef = nullptr, *out_efg = nullptr;
   double* d_rref = nullptr;  // positions at the last list build
   bool list_valid = false;
   int list_builds = 0;
@@ -1152,6 +1155,16 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
     es_call(c, ES_FINAL, c->d_q, c->d_mu, task, P.doefield, P.doefg);
     ++n_es;
   }
+  // device outputs queued ahead of the one synchronisation of the call
+  const size_t nb = (size_t)n * sizeof(double);
+  if (c->out_force)
+    HCK(c, hipMemcpyAsync(c->out_force, c->d_force, 3 * nb, hipMemcpyDeviceToDevice, st));
+  if (P.lcoulomb) {
+    if (c->out_mu) HCK(c, hipMemcpyAsync(c->out_mu, c->d_mu, 3 * nb, hipMemcpyDeviceToDevice, st));
+    if (c->out_ef) HCK(c, hipMemcpyAsync(c->out_ef, c->d_ef, 3 * nb, hipMemcpyDeviceToDevice, st));
+    if (c->out_efg)
+      HCK(c, hipMemcpyAsync(c->out_efg, c->d_efg, 9 * nb, hipMemcpyDeviceToDevice, st));
+  }
   sync_scalars(c);
   if (c->upol_pending) {
     c->u_pol = 0.5 * c->h_sc[SC_UPOL];
@@ -1543,16 +1556,17 @@ int mdffgpu_engforce_device(mdffgpu_ctx* c, const double* d_pos, int32_t itime,
     const size_t n = c->n, b = n * sizeof(double);
     HCK(c, hipMemcpyAsync(c->d_pos, d_pos, 3 * b, hipMemcpyDeviceToDevice, c->st));
     HCK(c, hipMemcpyAsync(c->d_force, d_force, 3 * b, hipMemcpyDeviceToDevice, c->st));
+    // the outputs are copied by run() ahead of its final synchronisation
+    c->out_force = d_force;
+    c->out_mu = d_mu;
+    c->out_ef = d_ef;
+    c->out_efg = d_efg;
     run(c, itime, en);
-    HCK(c, hipMemcpyAsync(d_force, c->d_force, 3 * b, hipMemcpyDeviceToDevice, c->st));
-    if (c->P.lcoulomb) {
-      if (d_mu) HCK(c, hipMemcpyAsync(d_mu, c->d_mu, 3 * b, hipMemcpyDeviceToDevice, c->st));
-      if (d_ef) HCK(c, hipMemcpyAsync(d_ef, c->d_ef, 3 * b, hipMemcpyDeviceToDevice, c->st));
-      if (d_efg) HCK(c, hipMemcpyAsync(d_efg, c->d_efg, 9 * b, hipMemcpyDeviceToDevice, c->st));
-    }
+    c->out_force = c->out_mu = c->out_ef = c->out_efg = nullptr;
     c->st = c->own_stream;
     return MDFFGPU_OK;
   } catch (Fail& f) {
+    c->out_force = c->out_mu = c->out_ef = c->out_efg = nullptr;
     c->st = c->own_stream;
     return f.code;
   }

@@ -19,6 +19,18 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 
 
+_KERNEL_NAMES: list | None = None
+
+
+def _kernel_index(name: str) -> int:
+    """Index of a kernel name in the library's instrumentation table."""
+    global _KERNEL_NAMES
+    if _KERNEL_NAMES is None:
+        L = _lib.lib()
+        _KERNEL_NAMES = [L.mdffgpu_kernel_name(i).decode() for i in range(L.mdffgpu_kernel_count())]
+    return _KERNEL_NAMES.index(name)
+
+
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
@@ -121,18 +133,13 @@ class ForceField:
         L = _lib.lib()
         _lib.check(L.mdffgpu_profile_enable(self._h, int(on)), self._h)
         if on and only is not None:
-            names = [L.mdffgpu_kernel_name(k).decode() for k in range(L.mdffgpu_kernel_count())]
-            _lib.check(L.mdffgpu_profile_only(self._h, names.index(only)), self._h)
+            _lib.check(L.mdffgpu_profile_only(self._h, _kernel_index(only)), self._h)
 
     def profile_select(self, only: str = None) -> None:
         """While profiling is enabled, bracket only the launches of kernel
         ``only`` from now on, or none (None), keeping the totals."""
-        L = _lib.lib()
-        k = -2
-        if only is not None:
-            names = [L.mdffgpu_kernel_name(i).decode() for i in range(L.mdffgpu_kernel_count())]
-            k = names.index(only)
-        _lib.check(L.mdffgpu_profile_only(self._h, k), self._h)
+        k = -2 if only is None else _kernel_index(only)
+        _lib.check(_lib.lib().mdffgpu_profile_only(self._h, k), self._h)
 
     def set_mixed_tuning(self, apply_slices: int, split_atoms: int) -> bool:
         """Launch shape of the mixed path's matrix-core reciprocal sums;
