@@ -642,7 +642,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
   const float c43a3 = (float)(4.0 / 3.0 * alphad * alphad * alphad);
   const double4 si = a.s4[i];
   const int c = a.cnt[i];
-  const size_t row = (size_t)i0 * a.maxnb;  // rows in processing order
+  const size_t lrow = (size_t)i * a.maxnb;  // the atom's list row
+  const size_t row = (size_t)i0 * a.maxnb;  // its table row: processing order
   const unsigned long long below = (1ull << lane) - 1ull;
   int base = 0;
   for (int k0 = 0; k0 < c; k0 += 64) {
@@ -651,7 +652,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
     int j = 0;
     bool in = false;
     if (k < c) {
-      j = a.nbr[row + k];
+      j = a.nbr[lrow + k];
       const double4 sj = a.s4[j];
       float rx, ry, rz;
       const float d2 = min_image_t<float>(cell, si.x - sj.x, si.y - sj.y, si.z - sj.z, rx, ry, rz);

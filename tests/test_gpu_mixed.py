@@ -118,3 +118,59 @@ def test_mixed_vs_fp64_pim_3000(solver):
           f"n_scf {g['n_scf']} (fp64 {o['n_scf']})")
     assert ef <= 1e-5 and em <= 1e-5 and eg <= 1e-5, (ef, em, eg)
     assert abs(g["u_pol"] - o["u_pol"]) <= 1e-5 * abs(o["u_pol"])
+
+
+@pytest.mark.parametrize("part", [P.MIXED_FP64_SHORTRANGE, P.MIXED_FP64_DIRECT,
+                                  P.MIXED_FP64_SCF_FIELD, P.MIXED_FP64_KSPACE])
+def test_mixed_with_fp64_part(part):
+    """Each part of the mixed path switched back to FP64 on its own
+    (mixed_fp64_parts): the combinations route the SCF pair table through
+    the separate table pass (FP64 static real space), the FP64 table, or the
+    FP64 reciprocal space beside the FP32 SCF field.  Same bar as the full
+    mixed path."""
+    s = C.silica(200, 400, seed=31, rc=9.0, kmax=8, pim=True, efg=True)
+    s.params.mixed_fp64_parts = part
+    for g, o in _pair(s, steps=2, perturb=0.01):
+        assert rel_rms(g["f"], o["f"]) <= 1e-5, (part, rel_rms(g["f"], o["f"]))
+        assert rel_rms(g["mu"], o["mu"]) <= 1e-5, (part, rel_rms(g["mu"], o["mu"]))
+        assert rel_rms(g["efg"], o["efg"]) <= 1e-5
+        assert abs(g["u_pol"] - o["u_pol"]) <= 1e-5 * abs(o["u_pol"])
+
+
+def _min_image_dist(r, i, L):
+    d = r - r[i]
+    d -= L * np.round(d / L)
+    dist = np.sqrt((d ** 2).sum(axis=1))
+    dist[i] = np.inf
+    return float(dist.min())
+
+
+def test_mixed_isolated_polarizable_atom():
+    """A polarizable O atom with no neighbour inside the cutoff: its row of
+    the compacted SCF pair table is empty (the table keeps one zero entry for
+    it), and its dipole comes from the reciprocal-space field alone.  The
+    600-atom melt sits in one octant of a box of twice its edge; the atom is
+    moved to the opposite corner."""
+    s = C.silica(200, 400, seed=31, rc=9.0, kmax=8, pim=True)
+    L = float(s.A[0, 0])
+    s.A = 2.0 * s.A
+    s.r = s.r.copy()
+    s.r[-1] = 1.5 * L
+    assert _min_image_dist(s.r, s.natm - 1, 2.0 * L) > s.params.cutlongrange + s.params.skindiff
+    (g, o), = _pair(s)
+    assert rel_rms(g["f"], o["f"]) <= 1e-5, rel_rms(g["f"], o["f"])
+    assert rel_rms(g["mu"], o["mu"]) <= 1e-5, rel_rms(g["mu"], o["mu"])
+    assert np.abs(o["mu"][-1]).max() > 0.0
+    assert np.abs(g["mu"][-1] - o["mu"][-1]).max() <= 1e-5 * np.abs(o["mu"]).max()
+
+
+def test_mixed_pair_table_int32_slots():
+    """More than 65,536 atoms: the compacted SCF pair table holds int32
+    neighbour slots instead of 16-bit ones (21,846 Si + 43,692 O, rc 6 A,
+    kmax 4)."""
+    s = C.silica(21846, 43692, seed=5, rc=6.0, kmax=4, pim=True)
+    assert s.natm > 65536
+    (g, o), = _pair(s)
+    assert rel_rms(g["f"], o["f"]) <= 1e-5, rel_rms(g["f"], o["f"])
+    assert rel_rms(g["mu"], o["mu"]) <= 1e-5, rel_rms(g["mu"], o["mu"])
+    assert abs(g["u_pol"] - o["u_pol"]) <= 1e-5 * abs(o["u_pol"])
