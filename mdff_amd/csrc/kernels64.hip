@@ -749,6 +749,18 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_cg_update(CgArgs a) {
   __shared__ double s_pap;
   const int n = a.n;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // the atom's vectors first: their loads run under the reduction of p.Ap
+  const int t = i < n ? a.typ[i] : 0;
+  const bool pol = i < n && a.T->polar[t];
+  double xv[3] = {0, 0, 0}, pv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, av[3] = {0, 0, 0};
+  if (pol)
+    for (int c = 0; c < 3; ++c) {
+      xv[c] = a.x[i + c * n];
+      pv[c] = a.p[i + c * n];
+      rv[c] = a.r[i + c * n];
+      av[c] = a.Ap[i + c * n];
+    }
+  const double rz = a.sc[a.s_rz];
   {
     double v[1] = {0.0};
     for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) v[0] += a.pap_part[b];
@@ -757,15 +769,14 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_cg_update(CgArgs a) {
     __syncthreads();
   }
   const double pap = s_pap;
-  const double step = pap != 0.0 ? a.sc[a.s_rz] / pap : 0.0;
+  const double step = pap != 0.0 ? rz / pap : 0.0;
   double acc[3] = {0.0, 0.0, 0.0};
   if (i < n) {
-    const int t = a.typ[i];
-    if (a.T->polar[t]) {
+    if (pol) {
       double r[3];
       for (int c = 0; c < 3; ++c) {
-        a.x[i + c * n] += step * a.p[i + c * n];
-        r[c] = a.r[i + c * n] - step * a.Ap[i + c * n];
+        a.x[i + c * n] = xv[c] + step * pv[c];
+        r[c] = rv[c] - step * av[c];
         a.r[i + c * n] = r[c];
       }
       for (int c = 0; c < 3; ++c) {
@@ -792,6 +803,15 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_cg_dir(CgArgs a) {
   __shared__ double red[4 * 3];
   __shared__ double s_rzn;
   const int n = a.n;
+  // the atom's vectors first: their loads run under the reduction
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double zv[3] = {0, 0, 0}, pv[3] = {0, 0, 0};
+  if (i < n)
+    for (int c = 0; c < 3; ++c) {
+      zv[c] = a.z[i + c * n];
+      pv[c] = a.p[i + c * n];
+    }
+  const double rz = a.sc[a.s_rz];
   {
     const int nb = gridDim.x;
     double v[3] = {0.0, 0.0, 0.0};
@@ -808,13 +828,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_cg_dir(CgArgs a) {
     }
     __syncthreads();
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double rz = a.sc[a.s_rz];
   const double beta = rz != 0.0 ? s_rzn / rz : 0.0;
   double p[3];
   for (int c = 0; c < 3; ++c) {
-    p[c] = a.z[i + c * n] + beta * a.p[i + c * n];
+    p[c] = zv[c] + beta * pv[c];
     a.p[i + c * n] = p[c];
   }
   if (a.p4) a.p4[a.rank[i]] = make_float4((float)p[0], (float)p[1], (float)p[2], 0.f);
