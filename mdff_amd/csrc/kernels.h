@@ -240,7 +240,8 @@ struct CgArgs {
   const double* Estat;
   const double* sc;  // device scalars: r.z of the current iterate at s_rz
   int s_rz;
-  const double* pap_part;  // k_cg_update: block partials of p.Ap (nblk)
+  const double* pap_part;  // k_cg_update: block partials of p.Ap (npap of them)
+  int npap;
   double* part;      // block partials, rows of nblk
   const int* rank;   // with p4: FP32 rank-ordered copy of p for k_dd_tab
   float4* p4;

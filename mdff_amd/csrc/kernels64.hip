@@ -763,7 +763,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_cg_update(CgArgs a) {
   const double rz = a.sc[a.s_rz];
   {
     double v[1] = {0.0};
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) v[0] += a.pap_part[b];
+    for (int b = threadIdx.x; b < a.npap; b += blockDim.x) v[0] += a.pap_part[b];
     block_sum<1>(v, red);
     if (threadIdx.x == 0) s_pap = v[0];
     __syncthreads();

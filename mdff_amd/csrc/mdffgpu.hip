@@ -110,6 +110,7 @@ struct mdffgpu_ctx {
   double* d_upart = nullptr;  // block partials of u_pol (reduced once per SCF)
   double* d_cg = nullptr;     // conjugate-gradient SCF: r, z, p, Ap (3N each)
   double* cg_Ap = nullptr;    // set while ES_IND calls are the CG's T p products
+  int cg_npap = 0;            // block partials of p.Ap the last such call wrote
   bool upol_pending = false;
   double* h_sc = nullptr;  // pinned
   double* h_sc_dev = nullptr;  // device address of h_sc
@@ -542,6 +543,37 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   }
   check_launch(c, "k_es_dir");
   if (mode == ES_FINAL) reduce(c, 7, gbl, SC_DIR);
+  CombineArgs ca;
+  std::memset(&ca, 0, sizeof(ca));
+  ca.n = n;
+  ca.lsurf = P.lsurf;
+  ca.do_efield = do_efield;
+  ca.do_efg = do_efg;
+  ca.apart = c->d_apart;
+  ca.cell = c->cell;
+  ca.fpi_V = 2.0 * (2.0 * MDFF_PI / c->cell.omega);
+  ca.fpi_3V = 2.0 * (2.0 * MDFF_PI / c->cell.omega / 3.0);
+  const double selfa = P.alphaES / std::sqrt(MDFF_PI);
+  ca.selfa2 = 2.0 * selfa * P.alphaES * P.alphaES / 3.0;
+  ca.coul_unit = P.coul_unit;
+  ca.mu = mu;
+  ca.q = q;
+  ca.ef_dir = c->d_efdir;
+  ca.efg_dir = c->d_efgdir;
+  ca.f_dir = c->d_fdir;
+  ca.qm = c->d_sc + SC_MOM;
+  ca.E_stat = c->d_Estat;
+  ca.E = c->d_E;
+  ca.ef_out = c->d_ef;
+  ca.efg_out = c->d_efg;
+  ca.f = c->d_force;
+  if (mode == ES_IND) {
+    // the SCF residual of the new field, fused (rows for SC_RMSD)
+    ca.typ = c->d_typ;
+    ca.T = c->d_tt;
+    ca.rpart = c->d_part;
+    ca.cg_Ap = c->cg_Ap;
+  }
   // reciprocal space
   const bool chg = (mode != ES_IND);
   const bool dip = (task & (TASK_QD | TASK_DD)) != 0;
@@ -549,8 +581,14 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   int nsplit_apply = c->app_split;
   const bool wt_ready = c->wt_ready && mode == ES_IND;
   c->wt_ready = false;
+  // the conjugate-gradient T p products: the mixed reciprocal space also
+  // does the combine (no surface term)
+  int fused_parts = 0;
+  c->cg_npap = gb;
   if (mxk && c->nchunk > 0) {
-    mixed::kspace(c->mx, mode, q, mu, chg, dip, efgv, c->d_apart, wt_ready);
+    const bool fuse = (mode == ES_IND) && c->cg_Ap && !P.lsurf;
+    fused_parts = mixed::kspace(c->mx, mode, q, mu, chg, dip, efgv, c->d_apart, wt_ready,
+                                fuse ? &ca : nullptr);
     nsplit_apply = 1;
   } else if (c->nchunk > 0) {
     RhoArgs ra;
@@ -638,32 +676,13 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     check_launch(c, "k_moments");
     reduce(c, 8, gb, SC_MOM);
   }
-  CombineArgs ca;
-  std::memset(&ca, 0, sizeof(ca));
-  ca.n = n;
+  if (fused_parts > 0) {
+    c->cg_npap = fused_parts;
+    check_launch(c, "mx_apply_epi_cg");
+    return;
+  }
   ca.nsplit = (c->nchunk > 0) ? nsplit_apply : 1;
-  ca.lsurf = P.lsurf;
-  ca.do_efield = do_efield;
-  ca.do_efg = do_efg;
   ca.dip_force = dip;
-  ca.apart = c->d_apart;
-  ca.cell = c->cell;
-  ca.fpi_V = 2.0 * (2.0 * MDFF_PI / c->cell.omega);
-  ca.fpi_3V = 2.0 * (2.0 * MDFF_PI / c->cell.omega / 3.0);
-  const double selfa = P.alphaES / std::sqrt(MDFF_PI);
-  ca.selfa2 = 2.0 * selfa * P.alphaES * P.alphaES / 3.0;
-  ca.coul_unit = P.coul_unit;
-  ca.mu = mu;
-  ca.q = q;
-  ca.ef_dir = c->d_efdir;
-  ca.efg_dir = c->d_efgdir;
-  ca.f_dir = c->d_fdir;
-  ca.qm = c->d_sc + SC_MOM;
-  ca.E_stat = c->d_Estat;
-  ca.E = c->d_E;
-  ca.ef_out = c->d_ef;
-  ca.efg_out = c->d_efg;
-  ca.f = c->d_force;
   if (mode == ES_STATIC)
     {
       ProfScope _ps(c, K_COMBINE);
@@ -671,11 +690,6 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     }
   else if (mode == ES_IND)
     {
-      // the SCF residual of the new field, fused (rows for SC_RMSD)
-      ca.typ = c->d_typ;
-      ca.T = c->d_tt;
-      ca.rpart = c->d_part;
-      ca.cg_Ap = c->cg_Ap;
       ProfScope _ps(c, K_COMBINE);
       k_combine64<ES_IND><<<gb, MDFF_BLOCK, 0, st>>>(ca);
     }
@@ -894,7 +908,8 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
       ++*n_es;
       g.s_rz = base(cur);
       g.pap_part = c->d_part;
-      g.part = c->d_part + gb;
+      g.npap = c->cg_npap;
+      g.part = c->d_part + std::max(gb, c->cg_npap);
       k_cg_update<<<gb, MDFF_BLOCK, 0, st>>>(g);
       check_launch(c, "k_cg_update");
       // the new direction from the update's sums, which the same kernel

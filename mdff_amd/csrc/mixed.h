@@ -70,8 +70,11 @@ void on_cell(MixedState* m);
 // energy / stress rows to the scalar slots.  With weights_ready (dipole
 // weights alone) the caller has already written the FP32 weights of mu
 // (dipole_weights).
-void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg, bool dip,
-            bool efg, double* apart, bool weights_ready = false);
+// With cg (an ES_IND call of the conjugate-gradient SCF, one configuration,
+// no surface term) the combine of k_combine64<ES_IND> is done here too;
+// returns the number of p.Ap block partials written to cg->rpart, 0 if not.
+int kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg, bool dip,
+           bool efg, double* apart, bool weights_ready = false, const CombineArgs* cg = nullptr);
 // Structure-factor weights of dipoles alone, one configuration (slot = atom):
 // Wt[c * npad + i] = (float)(2 pi (B^T mu_i)_c) * DIP_WEIGHT_SCALE, c = 0..2;
 // the slots past the atoms stay zero.

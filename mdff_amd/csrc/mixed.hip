@@ -1153,6 +1153,84 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int 
 }
 
 // ---------------------------------------------------------------------------
+// ES_IND call of the conjugate-gradient SCF, one configuration, field only:
+// mx_apply_epi's fold of the three field classes (U1 = Im S'_0, U2 = Im S_1,
+// U3 = Im S_2) for the 64 atoms of a block, then, for those atoms, the
+// combine of k_combine64<ES_IND> in conjugate-gradient mode (no surface
+// term): E = E_dir + E_rec(U) + 2 selfa2 p, Ap = alpha^-1 p - E, and the
+// block partial of p.Ap at ca.rpart[block].  The operations and their order
+// are those of the two kernels it replaces.
+template <int S>
+__global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi_cg(int n, int npad, int N1, int ncolp,
+                                                               const float* __restrict__ Xp,
+                                                               const float2* __restrict__ tj1,
+                                                               const double* __restrict__ sinv,
+                                                               CombineArgs ca) {
+  static_assert(MDFF_BLOCK == 64 * EPI_CHUNKS, "one wave per n1 range");
+  constexpr int W = 3;
+  __shared__ double part[EPI_CHUNKS - 1][W][64];
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, ch = threadIdx.x >> 6;
+  const int jp = blockIdx.x * 64 + lane;
+  const int i = jp < n ? jp : -1;  // one configuration: slot = atom
+  double u[W] = {0.0, 0.0, 0.0};
+  if (i >= 0) {
+    const int per = (N1 + EPI_CHUNKS - 1) / EPI_CHUNKS;
+    const int m0 = ch * per, m1 = min(N1, m0 + per);
+    const size_t sstride = (size_t)npad * ncolp;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const double* sv = sinv + (size_t)w * N1;
+      const float* xre = Xp + (size_t)(w * N1) * npad + jp;
+      const float* xim = Xp + (size_t)(W * N1 + w * N1) * npad + jp;
+#pragma unroll 7
+      for (int m = m0; m < m1; ++m) {
+        double xr = 0.0, xi = 0.0;
+#pragma unroll
+        for (int sl = 0; sl < S; ++sl) {
+          xr += xre[sl * sstride + (size_t)m * npad];
+          xi += xim[sl * sstride + (size_t)m * npad];
+        }
+        xr *= sv[m];
+        xi *= sv[m];
+        const float2 e = tj1[(size_t)m * npad + jp];
+        const double er = e.x, ei = -(double)e.y;  // conj(E1)
+        const double vi = er * xi + ei * xr;
+        u[w] += (w == 0) ? m * vi : vi;
+      }
+    }
+  }
+  if (ch > 0)
+    for (int w = 0; w < W; ++w) part[ch - 1][w][lane] = u[w];
+  __syncthreads();
+  double pap[1] = {0.0};
+  if (ch == 0 && i >= 0) {
+    for (int c = 0; c < EPI_CHUNKS - 1; ++c)
+      for (int w = 0; w < W; ++w) u[w] += part[c][w][lane];
+    const double* B = ca.cell.B;
+    const double fe = -2.0 * ca.fpi_V * (2.0 * MDFF_PI);
+    const double m3[3] = {ca.mu[i], ca.mu[i + n], ca.mu[i + 2 * n]};
+    double ef[3];
+    for (int r = 0; r < 3; ++r) {
+      const double er = fe * (B[r] * u[0] + B[r + 3] * u[1] + B[r + 6] * u[2]);
+      ef[r] = (ca.ef_dir[i + r * n] + er) + 2.0 * ca.selfa2 * m3[r];
+    }
+    const int t = ca.typ[i];
+    double ap[3] = {0, 0, 0};
+    if (ca.T->polar[t]) {
+      const double* ip = ca.T->invpol[t];
+      for (int r = 0; r < 3; ++r) {
+        ap[r] = ip[r] * m3[0] + ip[r + 3] * m3[1] + ip[r + 6] * m3[2] - ef[r];
+        pap[0] += m3[r] * ap[r];
+      }
+    }
+    for (int r = 0; r < 3; ++r) ca.cg_Ap[i + r * n] = ap[r];
+  }
+  block_sum<1>(pap, red);
+  if (threadIdx.x == 0) ca.rpart[blockIdx.x] = pap[0];
+}
+
+// ---------------------------------------------------------------------------
 // Batched path: reciprocal field and force of atom i from the U, V sums of
 // mx_apply_epi with its configuration's lattice (as k_combine64 forms them):
 //   Erec = -2 (4 pi/V) (2 pi) B U,
@@ -1442,8 +1520,8 @@ struct KsCall {
   double* sc_rec;
 };
 
-static void run_kspace(MixedState* m, const KsCall& a, const double* q, const double* mu, bool chg,
-                       bool dip, bool efg, double* apart) {
+static bool run_kspace(MixedState* m, const KsCall& a, const double* q, const double* mu, bool chg,
+                       bool dip, bool efg, double* apart, const CombineArgs* cg = nullptr) {
   hipStream_t st = a.st;
   const int n = m->N;
   const MxSeg* seg = m->d_seg;
@@ -1533,6 +1611,21 @@ static void run_kspace(MixedState* m, const KsCall& a, const double* q, const do
     else
       launch_apply<APP_NCT_E, APP_RT_E>(m, st);
   }
+  if (cg && W == 3 && !seg) {
+    // ES_IND of the conjugate-gradient SCF: fold and combine in one kernel
+    ProfScope _ps(m->ctx, K_MX_COMBINE);
+    const unsigned gb = (unsigned)((m->npad + 63) / 64);
+    if (m->app_slices == 1)
+      mx_apply_epi_cg<1><<<gb, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, NCTl * 16, m->X, m->tj1,
+                                                    m->sinv, *cg);
+    else if (m->app_slices == 2)
+      mx_apply_epi_cg<2><<<gb, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, NCTl * 16, m->X, m->tj1,
+                                                    m->sinv, *cg);
+    else
+      mx_apply_epi_cg<4><<<gb, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, NCTl * 16, m->X, m->tj1,
+                                                    m->sinv, *cg);
+    return true;
+  }
   {
     ProfScope _ps(m->ctx, K_MX_COMBINE);
     // the K slices a compile-time count, so that all of a thread's loads
@@ -1548,10 +1641,11 @@ static void run_kspace(MixedState* m, const KsCall& a, const double* q, const do
       mx_apply_epi<4><<<ge, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, W, NCTl * 16, seg, m->d_kseg,
                                                  m->X, m->tj1, m->sinv, apart);
   }
+  return false;
 }
 
-void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg, bool dip,
-            bool efg, double* apart, bool weights_ready) {
+int kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg, bool dip,
+           bool efg, double* apart, bool weights_ready, const CombineArgs* cg) {
   CtxView v = view(m->ctx);
   KsCall a;
   a.weights_ready = weights_ready;
@@ -1563,7 +1657,7 @@ void kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg
   a.energy = (mode == ES_FINAL);
   a.part = v.part;
   a.sc_rec = v.sc + v.sc_rec;
-  run_kspace(m, a, q, mu, chg, dip, efg, apart);
+  return run_kspace(m, a, q, mu, chg, dip, efg, apart, cg) ? (m->npad + 63) / 64 : 0;
 }
 
 float* dipole_weights(MixedState* m, int* npad) {
