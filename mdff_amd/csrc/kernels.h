@@ -181,8 +181,11 @@ struct ExtrapArgs {
 
 __global__ void k_prepare(int n, const double* pos, Cell cell, double* s, double4* s4,
                           double2* eph);
-__global__ void k_nlist(int n, const double* s, const int* perm, Cell cell, double cut2, int maxnb, int* cnt,
-                        int* nbr, int* overflow);
+template <bool FILL>
+__global__ void k_nlist(int n, const double* s, Cell cell, double cut2, int maxnb, int jlen,
+                        int* scnt, int* nbr);
+__global__ void k_nlist_offsets(int n, int nsp, int maxnb, int* scnt, int* cnt, int* overflow);
+constexpr int NLIST_RANGES = 16;  // candidate ranges of the list build (grid.y)
 template <typename R, bool BATCH = false>
 __global__ void k_shortrange(SrArgs a);
 template <int MODE, typename R, bool BATCH = false>

@@ -36,55 +36,79 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_prepare(
 
 // ---------------------------------------------------------------------------
 // Full neighbour list (replaces vnlist_pbc, src/tools.f90:274-370): every j
-// whose minimum image lies within cut (list cutoff incl. skin).  Brute force
-// over LDS tiles of fractional positions, taken in the spatial order perm
-// (cell-sorted atom indices), so that each row lists its neighbours cell by
-// cell; row-major rows nbr[i*maxnb + k] so that the lanes sharing an atom
-// read contiguously.
-__global__ __launch_bounds__(MDFF_BLOCK) void k_nlist(
-    int n, const double* __restrict__ s, const int* __restrict__ perm, Cell cell, double cut2,
-    int maxnb, int* __restrict__ cnt, int* __restrict__ nbr, int* __restrict__ overflow) {
+// whose minimum image lies within cut (list cutoff incl. skin), in index
+// order; row-major rows nbr[i*maxnb + k] so that the lanes sharing an atom
+// read contiguously.  Brute force over LDS tiles of fractional positions,
+// one thread per atom i, with the candidates j split into gridDim.y ranges
+// of jlen atoms so that the grid fills the device: a COUNT pass stores each
+// range's count per atom (scnt[range][i]), k_nlist_offsets turns them into
+// offsets and the row count, and a FILL pass writes each range's neighbours
+// at its offset, so that every row stays in index order.
+template <bool FILL>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_nlist(int n, const double* __restrict__ s,
+                                                      Cell cell, double cut2, int maxnb, int jlen,
+                                                      int* __restrict__ scnt,
+                                                      int* __restrict__ nbr) {
   __shared__ double ts[3][MDFF_BLOCK];
-  __shared__ int tj[MDFF_BLOCK];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool act = i < n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sp = blockIdx.y;
+  const int j0 = sp * jlen, j1 = min(n, j0 + jlen);
+  const bool act = i < n;
   double si1 = 0, si2 = 0, si3 = 0;
   if (act) {
     si1 = s[i];
     si2 = s[i + n];
     si3 = s[i + 2 * n];
   }
+  const int off = (FILL && act) ? scnt[(size_t)sp * n + i] : 0;
   int c = 0;
-  for (int base = 0; base < n; base += MDFF_BLOCK) {
-    int r = base + threadIdx.x;
+  for (int base = j0; base < j1; base += MDFF_BLOCK) {
+    const int r = base + threadIdx.x;
     __syncthreads();
-    if (r < n) {
-      const int j = perm ? perm[r] : r;
-      tj[threadIdx.x] = j;
-      ts[0][threadIdx.x] = s[j];
-      ts[1][threadIdx.x] = s[j + n];
-      ts[2][threadIdx.x] = s[j + 2 * n];
+    if (r < j1) {
+      ts[0][threadIdx.x] = s[r];
+      ts[1][threadIdx.x] = s[r + n];
+      ts[2][threadIdx.x] = s[r + 2 * n];
     }
     __syncthreads();
-    int m = min(MDFF_BLOCK, n - base);
+    const int m = min(MDFF_BLOCK, j1 - base);
     if (act) {
       for (int jj = 0; jj < m; ++jj) {
-        int jg = tj[jj];
         double rx, ry, rz;
-        double d2 = min_image(cell, si1 - ts[0][jj], si2 - ts[1][jj],
-                              si3 - ts[2][jj], rx, ry, rz);
+        const double d2 =
+            min_image(cell, si1 - ts[0][jj], si2 - ts[1][jj], si3 - ts[2][jj], rx, ry, rz);
+        const int jg = base + jj;
         if (jg != i && d2 <= cut2) {
-          if (c < maxnb) nbr[(size_t)i * maxnb + c] = jg;
+          if (FILL && off + c < maxnb) nbr[(size_t)i * maxnb + off + c] = jg;
           ++c;
         }
       }
     }
   }
-  if (act) {
-    cnt[i] = c;
-    if (c > maxnb) atomicMax(overflow, c);
-  }
+  if (!FILL && act) scnt[(size_t)sp * n + i] = c;
 }
+
+// per atom: the range counts of k_nlist<false> become exclusive offsets (in
+// place), their sum the row count; the largest row count above maxnb goes to
+// overflow
+__global__ __launch_bounds__(MDFF_BLOCK) void k_nlist_offsets(int n, int nsp, int maxnb,
+                                                              int* __restrict__ scnt,
+                                                              int* __restrict__ cnt,
+                                                              int* __restrict__ overflow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int acc = 0;
+  for (int sp = 0; sp < nsp; ++sp) {
+    const int v = scnt[(size_t)sp * n + i];
+    scnt[(size_t)sp * n + i] = acc;
+    acc += v;
+  }
+  cnt[i] = acc;
+  if (acc > maxnb) atomicMax(overflow, acc);
+}
+
+template __global__ void k_nlist<false>(int, const double*, Cell, double, int, int, int*, int*);
+template __global__ void k_nlist<true>(int, const double*, Cell, double, int, int, int*, int*);
 
 // ---------------------------------------------------------------------------
 // Structure factor (multipole_ES_rec, src/field.f90:2697-2742) for all

@@ -71,6 +71,7 @@ struct mdffgpu_ctx {
   double *d_pos = nullptr, *d_s = nullptr;
   double2* d_eph = nullptr;
   int *d_typ = nullptr, *d_cnt = nullptr, *d_nbr = nullptr, *d_ovf = nullptr;
+  int* d_nlr = nullptr;  // per-range neighbour counts, then offsets, of a list build
   double *d_q = nullptr, *d_qzero = nullptr, *d_dip = nullptr, *d_mu = nullptr,
          *d_muind = nullptr, *d_mup = nullptr;
   double4* d_s4 = nullptr;   // fractional coordinates, one double4 per atom
@@ -384,6 +385,7 @@ void alloc_atoms(mdffgpu_ctx* c) {
   dalloc(c, &c->d_typ, n);
   dalloc(c, &c->d_cnt, n);
   dalloc(c, &c->d_ovf, 1);
+  dalloc(c, &c->d_nlr, (size_t)NLIST_RANGES * n);
   dalloc(c, &c->d_q, n);
   dalloc(c, &c->d_qzero, n);
   dalloc(c, &c->d_dip, 3 * n);
@@ -1111,8 +1113,15 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
       {
         ProfScope _ps(c, K_NLIST);
         const double lc = c->list_cut + P.skindiff;
-      k_nlist<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_s, nullptr, c->cell, lc * lc, c->maxnb,
-                                            c->d_cnt, c->d_nbr, c->d_ovf);
+        // candidate ranges of whole tiles
+        const int jlen = (n + NLIST_RANGES * MDFF_BLOCK - 1) / (NLIST_RANGES * MDFF_BLOCK) * MDFF_BLOCK;
+        const dim3 gl(gb, (n + jlen - 1) / jlen);
+        k_nlist<false><<<gl, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->cell, lc * lc, c->maxnb, jlen,
+                                                  c->d_nlr, c->d_nbr);
+        k_nlist_offsets<<<gb, MDFF_BLOCK, 0, st>>>(n, (int)gl.y, c->maxnb, c->d_nlr, c->d_cnt,
+                                                   c->d_ovf);
+        k_nlist<true><<<gl, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->cell, lc * lc, c->maxnb, jlen,
+                                                 c->d_nlr, c->d_nbr);
       }
       check_launch(c, "k_nlist");
       HCK(c, hipMemcpyAsync(c->h_ovf, c->d_ovf, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1498,7 +1507,8 @@ void mdffgpu_destroy(mdffgpu_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->d_t2) (void)hipFree(c->d_t2);
   void* vp[] = {c->d_eph, c->d_typ, c->d_cnt, c->d_nbr, c->d_ovf, c->d_rpart, c->d_g,
-                c->d_chunks, c->d_pt, c->d_tt, c->d_mu4, c->d_s4, c->d_perm, c->d_rank};
+                c->d_chunks, c->d_pt, c->d_tt, c->d_mu4, c->d_s4, c->d_perm, c->d_rank,
+                c->d_nlr};
   for (void* p : vp)
     if (p) (void)hipFree(p);
   for (auto& p : c->prof_pending) {
