@@ -61,6 +61,12 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_nlist(int n, const double* __res
     si3 = s[i + 2 * n];
   }
   const int off = (FILL && act) ? scnt[(size_t)sp * n + i] : 0;
+  // orthorhombic cell: r_a = A_aa (ds_a - round(ds_a)); rounding half to
+  // even instead of away from zero gives the same |r| (a difference of
+  // exactly half the cell maps to -0.5 or 0.5)
+  const bool ortho = cell.A[1] == 0.0 && cell.A[2] == 0.0 && cell.A[3] == 0.0 &&
+                     cell.A[5] == 0.0 && cell.A[6] == 0.0 && cell.A[7] == 0.0;
+  const double a0 = cell.A[0], a4 = cell.A[4], a8 = cell.A[8];
   int c = 0;
   for (int base = j0; base < j1; base += MDFF_BLOCK) {
     const int r = base + threadIdx.x;
@@ -74,9 +80,16 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_nlist(int n, const double* __res
     const int m = min(MDFF_BLOCK, j1 - base);
     if (act) {
       for (int jj = 0; jj < m; ++jj) {
-        double rx, ry, rz;
-        const double d2 =
-            min_image(cell, si1 - ts[0][jj], si2 - ts[1][jj], si3 - ts[2][jj], rx, ry, rz);
+        double rx, ry, rz, d2;
+        if (ortho) {
+          const double dx = si1 - ts[0][jj], dy = si2 - ts[1][jj], dz = si3 - ts[2][jj];
+          rx = (dx - rint(dx)) * a0;
+          ry = (dy - rint(dy)) * a4;
+          rz = (dz - rint(dz)) * a8;
+          d2 = rx * rx + ry * ry + rz * rz;
+        } else {
+          d2 = min_image(cell, si1 - ts[0][jj], si2 - ts[1][jj], si3 - ts[2][jj], rx, ry, rz);
+        }
         const int jg = base + jj;
         if (jg != i && d2 <= cut2) {
           if (FILL && off + c < maxnb) nbr[(size_t)i * maxnb + off + c] = jg;
