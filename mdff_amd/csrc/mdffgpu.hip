@@ -889,10 +889,25 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
     ++*n_es;
     k_cg_init<<<gb, MDFF_BLOCK, 0, st>>>(g);
     check_launch(c, "k_cg_init");
-    reduce(c, 3, gb, SC_CG);
-    sync_scalars(c);
     int cur = 0;  // triple of the current r.z, |r|^2, npol
     auto base = [](int k) { return SC_CG + 3 * k; };
+    // the initial sums into the device scalars and mapped host memory; the
+    // first T p product is queued before the host reads them (unused if the
+    // initial iterate already meets the tolerance)
+    {
+      ProfScope _ps(c, K_REDUCE);
+      k_reduce_rows2<<<3, MDFF_BLOCK, 0, st>>>(c->d_part, gb, c->d_sc + base(cur),
+                                               c->h_sc_dev + base(cur));
+    }
+    check_launch(c, "k_reduce_rows2");
+    HCK(c, hipEventRecord(c->ev_sc, st));
+    bool first_queued = iscf < P.max_scf_pol_iter;
+    if (first_queued) {
+      c->cg_Ap = g.Ap;
+      es_call(c, ES_IND, c->d_qzero, g.p, TASK_DD, 1, 0);
+      c->cg_Ap = nullptr;
+    }
+    HCK(c, hipEventSynchronize(c->ev_sc));
     rmsd = std::sqrt(c->h_sc[base(cur) + 1] / c->h_sc[base(cur) + 2]);
     rmsd_first = rmsd;
     rmsd_best = rmsd;
@@ -903,9 +918,13 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
       // T p by an ES_IND call whose combine forms Ap = alpha^-1 p - T p and
       // the partials of p.Ap (row 0); the update sums those itself and
       // writes its own partials to rows 1..3
-      c->cg_Ap = g.Ap;
-      es_call(c, ES_IND, c->d_qzero, g.p, TASK_DD, 1, 0, queued_p && pk);
-      c->cg_Ap = nullptr;
+      if (first_queued) {
+        first_queued = false;  // the product queued after the initial sums
+      } else {
+        c->cg_Ap = g.Ap;
+        es_call(c, ES_IND, c->d_qzero, g.p, TASK_DD, 1, 0, queued_p && pk);
+        c->cg_Ap = nullptr;
+      }
       queued_p = false;
       ++*n_es;
       g.s_rz = base(cur);
