@@ -218,7 +218,7 @@ int mdffgpu_engforce(mdffgpu_ctx* ctx, const double* rx, const double* ry,
    (3*natm doubles), d_force = [fx|fy|fz] in/out, d_mu = [mx|my|mz],
    d_ef (3*natm), d_efg (9*natm, Fortran (natm,3,3) order); any output may
    be NULL.  stream is a hipStream_t (NULL = default stream).  Returns after
-   the scalar outputs are on the host. */
+   the scalar outputs are on the host and the device outputs are written. */
 int mdffgpu_engforce_device(mdffgpu_ctx* ctx, const double* d_pos,
                             int32_t itime, double* d_force, double* d_mu,
                             double* d_ef, double* d_efg,
