@@ -28,6 +28,7 @@ namespace mdff {
 namespace mixed {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // 16 raw bytes
 
@@ -820,7 +821,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, 
                                                         double* __restrict__ sinv) {
   const int KS = (2 * CHp) / 32;
   const size_t total = (size_t)NCT * KS * 64;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // two threads per fragment lane, 4 of its 8 elements each
+  const size_t t2 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int hh = (int)(t2 & 1);
+  const size_t t = t2 >> 1;
   if (t >= total) return;
   // configuration blockIdx.y: its g planes, maxima, fragments and scales
   const int z = blockIdx.y;
@@ -841,12 +845,12 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, 
     w = wn / N1;
     n1 = wn % N1;
     cs = class_scale(gmax[n1], apply_wmax(w, ky, kz));
-    if (part == 0 && ks == 0 && lane < 16) sinv[w * N1 + n1] = 1.0 / (double)cs;
+    if (part == 0 && ks == 0 && lane < 16 && hh == 0) sinv[w * N1 + n1] = 1.0 / (double)cs;
   }
-  half8 h, l;
+  half4 h, l;
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const int kp = ks * 32 + 8 * (lane >> 4) + e;
+  for (int e = 0; e < 4; ++e) {
+    const int kp = ks * 32 + 8 * (lane >> 4) + 4 * hh + e;
     const int c = kp >> 1, tt = kp & 1;
     float v = 0.f;
     if (colg < ncols && c < CH) {
@@ -873,9 +877,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, 
     h[e] = hh;
     l[e] = ll;
   }
-  half8* o = (half8*)bf + (size_t)ks * 2 * NCT * 64 + ct * 64 + lane;
+  half4* o = (half4*)bf + 2 * ((size_t)ks * 2 * NCT * 64 + ct * 64 + lane) + hh;
   o[0] = h;
-  o[NCT * 64] = l;
+  o[2 * NCT * 64] = l;
 }
 
 // ---------------------------------------------------------------------------
@@ -1597,7 +1601,7 @@ static bool run_kspace(MixedState* m, const KsCall& a, const double* q, const do
   const int KS = 2 * m->CHp / 32;
   {
     ProfScope _ps(m->ctx, K_MX_GPREP);
-    const size_t total = (size_t)NCTl * KS * 64;
+    const size_t total = (size_t)NCTl * KS * 64 * 2;  // two threads per fragment lane
     const dim3 gg((unsigned)((total + MDFF_BLOCK - 1) / MDFF_BLOCK), m->nseg);
     mx_gprep<<<gg, MDFF_BLOCK, 0, st>>>(m->N1, m->N3, m->C23, m->CH, m->CHp, W, NCTl, m->g,
                                         m->gmax, m->ky, m->kz, m->bfrag, m->sinv);
