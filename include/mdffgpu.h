@@ -154,7 +154,11 @@ typedef struct mdffgpu_params {
   int32_t precision;        /* mdffgpu_precision */
   int32_t mixed_fp64_parts; /* mdffgpu_mixed_part bits, 0 = all mixed */
   int32_t scf_solver;       /* mdffgpu_scf_solver */
-  int32_t reserved[5];
+  /* task_coul(4:6) of the reference (charge-quadrupole, dipole-quadrupole,
+     quadrupole-quadrupole, src/field.f90:166): any non-zero entry is
+     refused with MDFFGPU_ERR_UNSUPPORTED */
+  int32_t task_coul_quad[3];
+  int32_t reserved[2];
 } mdffgpu_params;
 
 /* Scalar outputs of one evaluation: the thermodynamic module variables
@@ -173,6 +177,9 @@ typedef struct mdffgpu_energies {
   int32_t n_scf;       /* SCF iterations of the last evaluation */
   int32_t n_es_calls;  /* Ewald evaluations of the last evaluation */
   double rmsd;         /* final SCF residual (src/field.f90:3378-3400) */
+  double scf_tol;      /* residual tolerance the SCF applied: conv_tol_ind,
+                          or in mixed precision the context's estimate of
+                          the FP32 noise floor times 1.5 when that is larger */
 } mdffgpu_energies;
 
 typedef struct mdffgpu_ctx mdffgpu_ctx;

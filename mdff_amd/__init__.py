@@ -4,6 +4,7 @@ The compute path is the HIP library ``libmdffgpu.so`` (mdff_amd/csrc), reached
 through its C ABI (include/mdffgpu.h).  This package holds the host-side
 mirror of the reference's interface: parameter set-up (``params``), the
 engforce front-end (``field``), synthetic configurations (``configs``) and
-the PyTorch tooling for force matching (``autograd``).
+the batched force-matching loss with its ``torch.autograd.Function``
+(``batch.force_match_loss``).
 """
 from .params import MdffEnergies, MdffParams  # noqa: F401

@@ -19,8 +19,7 @@ OUT = os.path.join(HERE, "libmdffgpu.so")
 ARCH = os.environ.get("MDFF_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels64.hip", "realspace.hip", "mixed.hip", "batch.hip", "adjoint.hip",
-           "mdffgpu.hip"]
+SOURCES = ["kernels64.hip", "realspace.hip", "mixed.hip", "batch.hip", "mdffgpu.hip"]
 
 DEVICE_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -61,7 +60,7 @@ def build_fortran(verbose: bool = False) -> str:
 
 
 def build(verbose: bool = False) -> str:
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = list(SOURCES)
     with tempfile.TemporaryDirectory() as td:
         objs = []
         for s in srcs:

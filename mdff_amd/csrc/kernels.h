@@ -177,6 +177,7 @@ struct ExtrapArgs {
   const int* typ;
   const TypeTables* T;
   double* part;
+  double* rpart;  // op 5: SCF residual rows 0, 1 of the corrected moments, or null
 };
 
 __global__ void k_prepare(int n, const double* pos, Cell cell, double* s, double4* s4,

@@ -623,13 +623,16 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_scf_resid(
 //   op 2: mu = h0, h1 = h0                (poly order 1, itime <= 1)
 //   op 3: ASPC predictor, mu = sum_k b_k h_k, shift h_k <- h_{k-1}
 //   op 4: h0 = mu                         (store after the SCF)
-//   op 5: ASPC corrector, mu = w alpha E + (1 - w) mu_p (+ u_pol row)
+//   op 5: ASPC corrector, mu = w alpha E + (1 - w) mu_p (+ u_pol row, and
+//         with rpart the SCF residual rows of the corrected moments)
 // hist layout: Fortran dipia_ind_t(nh, n, 3): [t + nh*(i + n*a)].
 __global__ __launch_bounds__(MDFF_BLOCK) void k_extrap(ExtrapArgs a) {
   __shared__ double red[4];
+  __shared__ double red2[4 * 2];
   const int n = a.n;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[1] = {0.0};
+  double rs[2] = {0.0, 0.0};
   if (i < n) {
     const int nh = a.nh;
     for (int r = 0; r < 3; ++r) {
@@ -663,11 +666,29 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_extrap(ExtrapArgs a) {
       }
       for (int r = 0; r < 3; ++r)
         a.mu[i + r * n] = a.w * mm[r] + (1.0 - a.w) * a.mup[i + r * n];
+      // the stopping criterion of this iteration is taken on the corrected
+      // moments (src/field.f90:3368-3400: the corrector runs before it)
+      if (a.rpart && a.T->polar[t]) {
+        rs[1] = 1.0;
+        for (int r = 0; r < 3; ++r) {
+          const double pa = a.T->pol[t][r + 3 * r];
+          if (pa == 0.0) continue;
+          const double d = a.mu[i + r * n] / pa - a.E[i + r * n];
+          rs[0] += d * d;
+        }
+      }
     }
   }
   if (a.op == 5) {
     block_sum<1>(acc, red);
     if (threadIdx.x == 0) a.part[blockIdx.x] = acc[0];
+    if (a.rpart) {
+      block_sum<2>(rs, red2);
+      if (threadIdx.x == 0) {
+        a.rpart[blockIdx.x] = rs[0];
+        a.rpart[gridDim.x + blockIdx.x] = rs[1];
+      }
+    }
   }
 }
 

@@ -5,10 +5,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -31,6 +33,7 @@ enum {
   SC_RMSD = 35,  // 2: sum, npol
   SC_DISP = 37,  // max squared displacement since the list build
   SC_CG = 40,    // 6: conjugate-gradient SCF, two (r.z, |r|^2, npol) triples
+  SC_ESTAT = 46, // 2: sum of |E_stat|^2 over polarizable atoms, their count
   SC_N = 48
 };
 
@@ -82,6 +85,10 @@ struct mdffgpu_ctx {
   std::vector<double> h_sfrac;
   std::vector<int> h_perm, h_rank, h_cellof, h_cellcnt;
   double mx_floor = 0.0;     // SCF residual floor of the mixed path (0 = unknown)
+  // outcome of the last mixed-precision solve, folded into mx_floor once the
+  // evaluation's scalars are on the host (update_floor)
+  bool scf_stalled = false, scf_done = false;
+  double scf_best = 0.0, scf_last = 0.0;
   double4* d_mu4 = nullptr;  // packed dipoles of the SCF field (double4 or float4)
   double *d_Estat = nullptr, *d_E = nullptr, *d_efdir = nullptr, *d_efgdir = nullptr,
          *d_fdir = nullptr, *d_force = nullptr, *d_ef = nullptr, *d_efg = nullptr;
@@ -355,6 +362,15 @@ void validate(mdffgpu_ctx* c, const mdffgpu_params* p) {
     fail(c, MDFFGPU_ERR_ARG, "SCF iteration bounds must be non-negative");
   if (p->scf_solver != MDFFGPU_SCF_JACOBI && p->scf_solver != MDFFGPU_SCF_PCG)
     fail(c, MDFFGPU_ERR_ARG, "unknown scf_solver");
+  if (p->lcoulomb && (p->task_coul_quad[0] || p->task_coul_quad[1] || p->task_coul_quad[2]))
+    fail(c, MDFFGPU_ERR_UNSUPPORTED, "quadrupole Coulomb tasks (task_coul(4:6)) are not supported");
+  // Tang-Toennies orders (src/field.f90:355-358), before any device work
+  std::unique_ptr<PairTables> pt(new PairTables());
+  std::unique_ptr<TypeTables> ty(new TypeTables());
+  TTTable t3;
+  const int bad = make_tables(*p, *pt, *ty, t3);
+  if (bad >= 0)
+    fail(c, MDFFGPU_ERR_TT_ORDER, "Tang-Toennies order %d > %d", bad, MDFFGPU_TT_MAX_ORDER);
 }
 
 void ensure_part(mdffgpu_ctx* c, size_t count) {
@@ -755,7 +771,7 @@ void sync_scalars(mdffgpu_ctx* c) {
   prof_collect(c);
 }
 
-void extrap(mdffgpu_ctx* c, int op, int ord, double w) {
+void extrap(mdffgpu_ctx* c, int op, int ord, double w, double* rpart = nullptr) {
   ExtrapArgs ea;
   ea.n = c->n;
   ea.nh = c->P.extrapolate_order + 1;
@@ -770,6 +786,7 @@ void extrap(mdffgpu_ctx* c, int op, int ord, double w) {
   ea.typ = c->d_typ;
   ea.T = c->d_tt;
   ea.part = c->d_upart;
+  ea.rpart = rpart;
   {
     ProfScope _ps(c, K_EXTRAP);
     k_extrap<<<nblk(c->n), MDFF_BLOCK, 0, c->st>>>(ea);
@@ -785,7 +802,8 @@ int aspc_order(const mdffgpu_params& P, int itime) {
 
 // moment_from_pola_scf (src/field.f90:3200-3458) on device; returns the
 // iteration count and final rmsd.
-void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
+void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es,
+         double* tol_out) {
   const mdffgpu_params& P = c->P;
   const int n = c->n;
   const int gb = nblk(n);
@@ -794,6 +812,7 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
   HCK(c, hipMemsetAsync(c->d_muind, 0, 3 * (size_t)n * sizeof(double), st));
   *n_scf = 0;
   *rmsd_out = 0.0;
+  *tol_out = P.conv_tol_ind;
   if (!c->linduced) return;
   int task_static = TASK_QQ | (c->ldip_static ? (TASK_QD | TASK_DD) : 0);
   es_call(c, ES_STATIC, c->d_q, c->d_dip, task_static, 1, 0);
@@ -813,7 +832,18 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
   // Jacobi residual need not fall monotonically; a diverging iteration
   // stays above the first residual).
   const double tol = mixed_prec ? std::max(P.conv_tol_ind, 1.5 * c->mx_floor) : P.conv_tol_ind;
+  *tol_out = tol;
   bool stalled = false;
+  if (mixed_prec) {
+    // scale of the static field (mu_ind is still zero here): caps the floor
+    {
+      ProfScope _ps(c, K_RESID);
+      k_scf_resid<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, c->d_tt, c->d_muind, c->d_Estat,
+                                             c->d_part);
+    }
+    check_launch(c, "k_scf_resid");
+    reduce(c, 2, gb, SC_ESTAT);
+  }
   double rmsd_best = rmsd, rmsd_first = 0.0;
   int no_progress = 0;
   // Mixed precision: while the host waits for an iteration's residual, the
@@ -987,8 +1017,10 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
     queued = false;
     ++*n_es;
     if (iscf == 1 && P.calc_mode == 1 && P.algo_ext_dipole == MDFFGPU_EXT_ASPC) {
+      // the corrector, then the residual rows again from the corrected
+      // moments (the reference tests convergence after the corrector)
       const int eo = aspc_order(P, itime);
-      extrap(c, 5, eo, W_ASPC[eo]);
+      extrap(c, 5, eo, W_ASPC[eo], c->d_part);
       upol_new = true;
     }
     if (spec) {
@@ -1022,7 +1054,12 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es) {
         rmsd < 0.1 * rmsd_first && (c->mx_floor == 0.0 || rmsd < 5.0 * c->mx_floor))
       stalled = true;
   }
-  if (mixed_prec && (stalled || rmsd > tol)) c->mx_floor = rmsd_best;
+  if (mixed_prec) {
+    c->scf_done = true;
+    c->scf_stalled = stalled;
+    c->scf_best = rmsd_best;
+    c->scf_last = rmsd;
+  }
   extrap(c, 4, 0, 0.0);
   // u_pol of the last induced moments: the partials of the last k_induced /
   // ASPC corrector, read with the evaluation's final scalars
@@ -1088,6 +1125,26 @@ void spatial_order(mdffgpu_ctx* c) {
   HCK(c, hipMemcpyAsync(c->d_rank, c->h_rank.data(), (size_t)n * sizeof(int),
                         hipMemcpyHostToDevice, c->st));
   HCK(c, hipStreamSynchronize(c->st));  // the host vectors are reused
+}
+
+// Noise-floor estimate of the mixed SCF after a solve (scalars on the
+// host).  Only a stalled solve raises it, and never above FLOOR_CAP FP32
+// epsilons of the RMS static field; a solve that exits at the iteration
+// limit leaves it as it is; one that reaches conv_tol_ind drops it, and one
+// that converges below it lowers it.
+constexpr double FLOOR_CAP = 64.0;
+void update_floor(mdffgpu_ctx* c) {
+  if (!c->scf_done) return;
+  c->scf_done = false;
+  const double* h = c->h_sc;
+  const double estat = (h[SC_ESTAT + 1] > 0.0) ? std::sqrt(h[SC_ESTAT] / h[SC_ESTAT + 1]) : 0.0;
+  const double cap = FLOOR_CAP * (double)FLT_EPSILON * estat;
+  if (c->scf_stalled)
+    c->mx_floor = std::min(c->scf_best, cap);
+  else if (c->scf_last <= c->P.conv_tol_ind)
+    c->mx_floor = 0.0;
+  else if (c->mx_floor > 0.0 && c->scf_last <= 1.5 * c->mx_floor)
+    c->mx_floor = std::min(c->mx_floor, c->scf_best);
 }
 
 // engforce_driver on the device buffers of ctx; d_pos and d_force are
@@ -1187,7 +1244,7 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
   }
   int n_es = 0;
   if (P.lcoulomb) {
-    scf(c, itime, &en->n_scf, &en->rmsd, &n_es);
+    scf(c, itime, &en->n_scf, &en->rmsd, &n_es, &en->scf_tol);
     {
       ProfScope _ps(c, K_ADD);
       k_add<<<nblk(3 * n), MDFF_BLOCK, 0, st>>>(3 * n, c->d_dip, c->d_muind, c->d_mu);
@@ -1209,6 +1266,7 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
       HCK(c, hipMemcpyAsync(c->out_efg, c->d_efg, 9 * nb, hipMemcpyDeviceToDevice, st));
   }
   sync_scalars(c);
+  update_floor(c);
   if (c->upol_pending) {
     c->u_pol = 0.5 * c->h_sc[SC_UPOL];
     c->upol_pending = false;
@@ -1548,6 +1606,7 @@ int mdffgpu_set_cell(mdffgpu_ctx* c, const double* A) {
     set_lattice(c, A);
     update_cutoffs(c);
     c->list_valid = false;
+    c->mx_floor = 0.0;
     if (c->list_cut > 0.0) set_list_capacity(c, std::max(c->maxnb, initial_capacity(c)));
     if (c->P.lcoulomb) build_kmesh(c);
     if (c->mx) mixed::on_cell(c->mx);

@@ -42,7 +42,7 @@ module mdffgpu_mod
     integer(c_int32_t) :: precision
     integer(c_int32_t) :: mixed_fp64_parts
     integer(c_int32_t) :: scf_solver
-    integer(c_int32_t) :: reserved(5)
+    integer(c_int32_t) :: task_coul_quad(3), reserved(2)
   end type
 
   type, bind(C), public :: mdffgpu_energies
@@ -50,7 +50,7 @@ module mdffgpu_mod
     real(c_double) :: u_coul, u_pol, pvirial_coul
     real(c_double) :: tau_nonb(3, 3), tau_coul(3, 3)
     integer(c_int32_t) :: n_scf, n_es_calls
-    real(c_double) :: rmsd
+    real(c_double) :: rmsd, scf_tol
   end type
 
   interface

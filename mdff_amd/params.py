@@ -106,7 +106,8 @@ class MdffParams(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("mixed_fp64_parts", ctypes.c_int32),
         ("scf_solver", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("task_coul_quad", ctypes.c_int32 * 3),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -128,6 +129,7 @@ class MdffEnergies(ctypes.Structure):
         ("n_scf", ctypes.c_int32),
         ("n_es_calls", ctypes.c_int32),
         ("rmsd", ctypes.c_double),
+        ("scf_tol", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:

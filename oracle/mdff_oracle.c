@@ -934,10 +934,89 @@ static const double B_ASPC[6][6] = {
     {22.0 / 7.0, -55.0 / 14.0, 55.0 / 21.0, -22.0 / 21.0, 5.0 / 21.0, -1.0 / 42.0}};
 static const double W_ASPC[6] = {0.0, 2.0 / 3.0, 0.6, 4.0 / 7.0, 5.0 / 9.0, 6.0 / 11.0};
 
-static int aspc_order(const mdffgpu_params* p, int itime) {
+/* order switch of extrapolate_dipole_aspc (src/field.f90:3918-3927): the
+   order ramps up while the history fills */
+int orc_aspc_order(const mdffgpu_params* p, int itime) {
   if (itime >= p->extrapolate_order + 1) return p->extrapolate_order;
   if (itime == 1 || itime == 0) return 0;
   return itime - 1;
+}
+
+/* B_ASPC(1..ord+1) and w_ASPC of src/field.f90:3929-3967 */
+void orc_aspc_coeffs(int ord, double* b, double* w) {
+  for (int k = 0; k < 6; ++k) b[k] = (ord >= 0 && ord < 6) ? B_ASPC[ord][k] : 0.0;
+  *w = (ord >= 0 && ord < 6) ? W_ASPC[ord] : 0.0;
+}
+
+#define H(t, i, a) h[(t) + nh * ((i) + (size_t)n * (a))]
+/* The first SCF iterate in MD: extrapolate_dipole_poly (src/field.f90:
+   3807-3893, orders 0 and 1) or the ASPC predictor (key 1,
+   src/field.f90:3969-3979), with their history shifts.  hist is
+   dipia_ind_t(extrapolate_order+1, natm, 3) in Fortran order; mu (natm,3)
+   is written. */
+int orc_extrapolate(const mdffgpu_params* p, int itime, double* h, double* mu) {
+  int n = p->natm;
+  int ord = p->extrapolate_order;
+  int nh = ord + 1;
+  if (p->algo_ext_dipole == MDFFGPU_EXT_POLY) {
+    if (ord > 1) return MDFFGPU_ERR_UNSUPPORTED;
+    if (ord == 0) {
+      for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) mu[i + n * a] = H(0, i, a);
+    } else if (itime > ord) {
+      for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+          mu[i + n * a] = 2.0 * H(0, i, a) - H(1, i, a);
+          H(1, i, a) = H(0, i, a);
+        }
+    } else {
+      for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+          mu[i + n * a] = H(0, i, a);
+          H(1, i, a) = H(0, i, a);
+        }
+    }
+    return 0;
+  }
+  int eo = orc_aspc_order(p, itime);
+  for (int i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      double v = 0.0;
+      for (int k = 0; k <= eo; ++k) v = v + B_ASPC[eo][k] * H(k, i, a);
+      mu[i + n * a] = v;
+    }
+  for (int k = eo; k >= 1; --k)
+    for (int i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) H(k, i, a) = H(k - 1, i, a);
+  return 0;
+}
+#undef H
+
+/* ASPC corrector (key 2, src/field.f90:3982-3987): mu = w alpha E +
+   (1 - w) mu_p, mu_p the predictor's moments (in mu on entry); u_pol is
+   that of alpha E, as induced_moment leaves it.  it holds 0-based types. */
+static void aspc_correct(const mdffgpu_params* p, const int* it, int itime,
+                         const double* E, double* mu, double* u_pol) {
+  size_t n3 = 3 * (size_t)p->natm;
+  double* mup = (double*)malloc(n3 * sizeof(double));
+  if (!mup) return;
+  int eo = orc_aspc_order(p, itime);
+  memcpy(mup, mu, n3 * sizeof(double));
+  induced_moment(p, it, E, mu, u_pol);
+  for (size_t k = 0; k < n3; ++k)
+    mu[k] = W_ASPC[eo] * mu[k] + (1.0 - W_ASPC[eo]) * mup[k];
+  free(mup);
+}
+
+/* the same with 1-based itype (exposed for the tests) */
+void orc_aspc_correct(const mdffgpu_params* p, const int* itype, int itime,
+                      const double* E, double* mu, double* u_pol) {
+  int n = p->natm;
+  int* it = (int*)malloc((size_t)n * sizeof(int));
+  if (!it) return;
+  for (int i = 0; i < n; ++i) it[i] = itype[i] - 1;
+  aspc_correct(p, it, itime, E, mu, u_pol);
+  free(it);
 }
 
 /* moment_from_pola_scf, src/field.f90:3200-3458 */
@@ -989,36 +1068,10 @@ static int scf(const es_ctx* e, const double* rx, const double* ry,
     ++iscf;
     if (iscf != 1 || p->calc_mode == 0) {
       induced_moment(p, it, E, mu_ind, &st->u_pol);
-    } else if (p->algo_ext_dipole == MDFFGPU_EXT_POLY) {
-      /* extrapolate_dipole_poly, :3807-3893 (orders 0 and 1) */
-      if (ord == 0) {
-        for (int i = 0; i < n; ++i)
-          for (int a = 0; a < 3; ++a) mu_ind[i + n * a] = H(0, i, a);
-      } else if (itime > ord) {
-        for (int i = 0; i < n; ++i)
-          for (int a = 0; a < 3; ++a) {
-            mu_ind[i + n * a] = 2.0 * H(0, i, a) - H(1, i, a);
-            H(1, i, a) = H(0, i, a);
-          }
-      } else {
-        for (int i = 0; i < n; ++i)
-          for (int a = 0; a < 3; ++a) {
-            mu_ind[i + n * a] = H(0, i, a);
-            H(1, i, a) = H(0, i, a);
-          }
-      }
     } else {
-      /* extrapolate_dipole_aspc predictor, :3897-3992 */
-      int eo = aspc_order(p, itime);
-      for (int i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) {
-          double v = 0.0;
-          for (int k = 0; k <= eo; ++k) v = v + B_ASPC[eo][k] * H(k, i, a);
-          mu_ind[i + n * a] = v;
-        }
-      for (int k = eo; k >= 1; --k)
-        for (int i = 0; i < n; ++i)
-          for (int a = 0; a < 3; ++a) H(k, i, a) = H(k - 1, i, a);
+      /* first iterate in 'md'/'opt': extrapolated, :3321-3325 */
+      rc = orc_extrapolate(p, itime, h, mu_ind);
+      if (rc) break;
     }
     /* induced field: charges and static dipoles zeroed, :3290-3301 */
     rc = es_call(e, rx, ry, rz, qzero, mu_ind, task_ind, 0, 1, 0, 0, 0,
@@ -1027,13 +1080,8 @@ static int scf(const es_ctx* e, const double* rx, const double* ry,
     ++*n_es;
     for (size_t k = 0; k < n3; ++k) E[k] = Estat[k] + Eind[k];
     for (int i = 0; i < n; ++i) fx[i] = fy[i] = fz[i] = 0.0;
-    if (iscf == 1 && p->calc_mode == 1 && p->algo_ext_dipole == MDFFGPU_EXT_ASPC) {
-      int eo = aspc_order(p, itime);
-      memcpy(mup, mu_ind, n3 * sizeof(double));
-      induced_moment(p, it, E, mu_ind, &st->u_pol);
-      for (size_t k = 0; k < n3; ++k)
-        mu_ind[k] = W_ASPC[eo] * mu_ind[k] + (1.0 - W_ASPC[eo]) * mup[k];
-    }
+    if (iscf == 1 && p->calc_mode == 1 && p->algo_ext_dipole == MDFFGPU_EXT_ASPC)
+      aspc_correct(p, it, itime, E, mu_ind, &st->u_pol);
     /* stopping criterion, :3378-3400 */
     rmsd = 0.0;
     int npol = 0;

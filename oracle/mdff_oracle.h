@@ -57,6 +57,19 @@ typedef struct orc_state {
   double u_pol;
 } orc_state;
 
+/* Dipole extrapolation of the first SCF iterate in MD
+   (src/field.f90:3807-3992), exposed for the tests.  hist is
+   dipia_ind_t(extrapolate_order+1, natm, 3) in Fortran order (updated as
+   the reference shifts it); mu (natm,3) receives the predictor. */
+int orc_aspc_order(const mdffgpu_params* p, int itime);
+void orc_aspc_coeffs(int ord, double* b /* [6] */, double* w);
+int orc_extrapolate(const mdffgpu_params* p, int itime, double* hist,
+                    double* mu);
+/* ASPC corrector: mu (in: predictor) <- w alpha E + (1 - w) mu; itype
+   1-based; u_pol of alpha E. */
+void orc_aspc_correct(const mdffgpu_params* p, const int* itype, int itime,
+                      const double* E, double* mu, double* u_pol);
+
 /* engforce_driver, src/field.f90:1111-1166.  Arguments as
    mdffgpu_engforce; A is the cell.  Returns 0 or an mdffgpu_status. */
 int orc_engforce(const mdffgpu_params* p, const int* itype, const double* A,

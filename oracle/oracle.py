@@ -53,6 +53,15 @@ def lib():
                                        _dp, _dp, _ip] + [ctypes.c_int] * 5 + [
                                            ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp,
                                            _dp, _dp, _dp]
+        L.orc_aspc_order.restype = ctypes.c_int
+        L.orc_aspc_order.argtypes = [ctypes.POINTER(MdffParams), ctypes.c_int]
+        L.orc_aspc_coeffs.restype = None
+        L.orc_aspc_coeffs.argtypes = [ctypes.c_int, _dp, _dp]
+        L.orc_extrapolate.restype = ctypes.c_int
+        L.orc_extrapolate.argtypes = [ctypes.POINTER(MdffParams), ctypes.c_int, _dp, _dp]
+        L.orc_aspc_correct.restype = None
+        L.orc_aspc_correct.argtypes = [ctypes.POINTER(MdffParams), _ip, ctypes.c_int, _dp, _dp,
+                                       _dp]
         _lib = L
     return _lib
 
@@ -72,6 +81,48 @@ def tt_damp(b: float, c: float, r: float, order: int):
     fd = ctypes.c_double()
     lib().orc_tt_damp(b, c, r, order, ctypes.byref(f), ctypes.byref(fd))
     return f.value, fd.value
+
+
+def aspc_order(params: MdffParams, itime: int) -> int:
+    """Order switch of extrapolate_dipole_aspc, src/field.f90:3918-3927."""
+    return lib().orc_aspc_order(ctypes.byref(params), int(itime))
+
+
+def aspc_coeffs(order: int):
+    """(B_ASPC(1..6), w_ASPC) of src/field.f90:3929-3967."""
+    b = np.zeros(6)
+    w = ctypes.c_double()
+    lib().orc_aspc_coeffs(int(order), _p(b), ctypes.byref(w))
+    return b, w.value
+
+
+def extrapolate(params: MdffParams, itime: int, hist):
+    """First SCF iterate in MD (extrapolate_dipole_poly / the ASPC
+    predictor).  hist: dipia_ind_t as an (order+1, natm, 3) array, updated
+    in place; returns mu (natm, 3)."""
+    n = params.natm
+    h = np.asfortranarray(hist, dtype=np.float64)
+    if h.shape != (params.extrapolate_order + 1, n, 3):
+        raise ValueError("hist must be (extrapolate_order+1, natm, 3)")
+    mu = np.zeros(3 * n)
+    rc = lib().orc_extrapolate(ctypes.byref(params), int(itime), _p(h), _p(mu))
+    if rc != 0:
+        raise RuntimeError(f"oracle extrapolation failed with status {rc}")
+    hist[...] = h
+    return mu.reshape(3, n).T.copy()
+
+
+def aspc_correct(params: MdffParams, itype, itime: int, E, mu_pred):
+    """ASPC corrector: returns (mu, u_pol) from the field E (natm,3) and the
+    predictor mu_pred (natm,3)."""
+    n = params.natm
+    Ef = np.ascontiguousarray(np.asarray(E, dtype=np.float64).T.reshape(-1))
+    mu = np.ascontiguousarray(np.asarray(mu_pred, dtype=np.float64).T.reshape(-1))
+    it = np.ascontiguousarray(itype, dtype=np.int32)
+    u = ctypes.c_double()
+    lib().orc_aspc_correct(ctypes.byref(params), it.ctypes.data_as(_ip), int(itime), _p(Ef),
+                           _p(mu), ctypes.byref(u))
+    return mu.reshape(3, n).T.copy(), u.value
 
 
 class Oracle:

@@ -71,3 +71,69 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.LibraryMissing):
         _lib.lib()
+
+
+def _create_status(p, itype=None):
+    """mdffgpu_create's status and message for parameters that fail its
+    checks; these run before any device work, so no GPU is needed."""
+    import numpy as np
+    L = _lib.lib()
+    it = np.ones(max(p.natm, 1), np.int32) if itype is None else np.asarray(itype, np.int32)
+    A = np.ascontiguousarray((np.eye(3) * 20.0).reshape(-1, order="F"))
+    h = ctypes.c_void_p()
+    st = L.mdffgpu_create(ctypes.byref(p), it.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                          A.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(h))
+    assert not h.value
+    return st, L.mdffgpu_last_error(None).decode()
+
+
+def _pim_params(n=8):
+    from mdff_amd import params as P
+    p = P.field_defaults(n, 2)
+    p.lcoulomb = 1
+    p.cutlongrange = 6.0
+    p.qch[0], p.qch[1] = 1.0, -1.0
+    p.kES[:] = [4, 4, 4]
+    p.ldip_polar[1] = 1
+    P.field_check_tag(p, poldip_iso=[0.0, 1.0])
+    return p
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libmdffgpu.so not built")
+def test_create_failure_statuses():
+    """The failure statuses of the C ABI for host-detectable errors, as the
+    reference stops on them: Tang-Toennies order above 8 (src/field.f90:
+    355-358) -> MDFFGPU_ERR_TT_ORDER; quadrupole tasks task_coul(4:6) and
+    polynomial extrapolation of order >= 2 -> MDFFGPU_ERR_UNSUPPORTED; bad
+    sizes -> MDFFGPU_ERR_ARG."""
+    from mdff_amd import params as P
+    p = _pim_params()
+    k = P.didx(2, 2, 1)
+    p.ldip_damping[k] = 1
+    p.dip_pol_damp_b[k], p.dip_pol_damp_c[k], p.dip_pol_damp_k[k] = 2.0, 1.0, 9
+    st, msg = _create_status(p)
+    assert st == 5 and "Tang-Toennies" in msg, (st, msg)
+    for q in range(3):
+        p = _pim_params()
+        p.task_coul_quad[q] = 1
+        st, msg = _create_status(p)
+        assert st == 6 and "quadrupole" in msg, (st, msg)
+    p = _pim_params()
+    p.algo_ext_dipole, p.extrapolate_order = P.EXT_POLY, 2
+    st, msg = _create_status(p)
+    assert st == 6 and "extrapolation" in msg, (st, msg)
+    p = _pim_params()
+    p.algo_ext_dipole, p.extrapolate_order = P.EXT_ASPC, 6
+    assert _create_status(p)[0] == 1
+    p = _pim_params()
+    p.natm = 0
+    assert _create_status(p)[0] == 1
+    p = _pim_params()
+    p.ntype = 17
+    assert _create_status(p)[0] == 1
+    p = _pim_params()
+    p.kES[1] = 0
+    assert _create_status(p)[0] == 1
+    p = _pim_params()
+    p.precision = 7
+    assert _create_status(p)[0] == 1

@@ -27,7 +27,7 @@ def test_fortran_types_match_c_layout(tmp_path):
   type(mdffgpu_energies) :: e
   print *, c_sizeof(p), c_sizeof(e)
   print *, (loc(p%alphaES) - loc(p)), (loc(p%conv_tol_ind) - loc(p)), (loc(p%precision) - loc(p))
-  print *, (loc(e%tau_coul) - loc(e)), (loc(e%rmsd) - loc(e))
+  print *, (loc(e%tau_coul) - loc(e)), (loc(e%rmsd) - loc(e)), (loc(e%scf_tol) - loc(e))
 end program
 """)
     exe = tmp_path / "sz"
@@ -45,6 +45,7 @@ end program
     assert vals[4] == MdffParams.precision.offset
     assert vals[5] == MdffEnergies.tau_coul.offset
     assert vals[6] == MdffEnergies.rmsd.offset
+    assert vals[7] == MdffEnergies.scf_tol.offset
 
 
 def write_driver_input(path, p, itype, A, frames):

@@ -115,7 +115,10 @@ def test_mixed_vs_fp64_pim_3000(solver):
         fmx.close()
     ef, em, eg = rel_rms(g["f"], o["f"]), rel_rms(g["mu"], o["mu"]), rel_rms(g["efg"], o["efg"])
     print(f"pim 3000 solver {solver}: force {ef:.2e} dipole {em:.2e} efg {eg:.2e} "
-          f"n_scf {g['n_scf']} (fp64 {o['n_scf']})")
+          f"n_scf {g['n_scf']} (fp64 {o['n_scf']}) rmsd {g['rmsd']:.2e} "
+          f"(fp64 {o['rmsd']:.2e}) scf_tol {g['scf_tol']:.2e}")
+    assert o["rmsd"] <= s.params.conv_tol_ind
+    assert g["rmsd"] <= g["scf_tol"] or g["n_scf"] < s.params.max_scf_pol_iter
     assert ef <= 1e-5 and em <= 1e-5 and eg <= 1e-5, (ef, em, eg)
     assert abs(g["u_pol"] - o["u_pol"]) <= 1e-5 * abs(o["u_pol"])
 
@@ -174,3 +177,25 @@ def test_mixed_pair_table_int32_slots():
     assert rel_rms(g["f"], o["f"]) <= 1e-5, rel_rms(g["f"], o["f"])
     assert rel_rms(g["mu"], o["mu"]) <= 1e-5, rel_rms(g["mu"], o["mu"])
     assert abs(g["u_pol"] - o["u_pol"]) <= 1e-5 * abs(o["u_pol"])
+
+
+def test_mixed_floor_not_raised_by_iteration_limit():
+    """The mixed SCF's noise-floor estimate (the tolerance it may apply above
+    conv_tol_ind) is raised only by a stalled solve: evaluations that stop at
+    max_scf_pol_iter above the tolerance leave the next evaluation's
+    tolerance at conv_tol_ind.  scf_tol reports the tolerance applied."""
+    from mdff_amd.field import ForceField
+    s = C.silica(200, 400, seed=31, rc=9.0, kmax=8, pim=True)
+    p = copy.deepcopy(s.params)
+    p.precision = P.PREC_MIXED
+    p.conv_tol_ind = 1e-7
+    p.min_scf_pol_iter = 1
+    p.max_scf_pol_iter = 2
+    ff = ForceField(p, s.itype, s.A)
+    try:
+        for it in range(1, 4):
+            g = ff.engforce(s.r, itime=it)
+            assert g["n_scf"] == 2 and g["rmsd"] > p.conv_tol_ind, (g["n_scf"], g["rmsd"])
+            assert g["scf_tol"] == p.conv_tol_ind, (it, g["scf_tol"])
+    finally:
+        ff.close()

@@ -58,13 +58,30 @@ def test_silica_pim_efg_small(oracle_lib):
         compare_engforce(g, o)
 
 
-@pytest.mark.parametrize("algo,order", [(P.EXT_POLY, 1), (P.EXT_ASPC, 2), (P.EXT_ASPC, 4)])
-def test_pim_extrapolation_history(oracle_lib, algo, order):
+@pytest.mark.parametrize("iters", [(3, 100), (1, 100), (1, 1)])
+@pytest.mark.parametrize("algo,order", [(P.EXT_POLY, 1), (P.EXT_ASPC, 1), (P.EXT_ASPC, 2),
+                                        (P.EXT_ASPC, 4)])
+def test_pim_extrapolation_history(oracle_lib, algo, order, iters):
+    """Extrapolated first iterates over 6 MD steps.  With min_scf_pol_iter = 1
+    the first iteration may be the last, so the ASPC corrector's effect on
+    the residual and on the stopping test is compared too (the reference
+    tests convergence after the corrector, src/field.f90:3368-3400)."""
     s = C.silica(40, 80, seed=12, rc=5.5, kmax=4, pim=True, efg=False)
     s.params.algo_ext_dipole = algo
     s.params.extrapolate_order = order
-    for g, o in _run_both(s, oracle_lib, steps=6, perturb=0.005):
+    s.params.min_scf_pol_iter, s.params.max_scf_pol_iter = iters
+    perturb = 0.005
+    if iters[0] == 1:
+        s.params.conv_tol_ind = 3e-3
+        perturb = 0.001
+    nscf = []
+    for g, o in _run_both(s, oracle_lib, steps=6, perturb=perturb):
         compare_engforce(g, o)
+        assert abs(g["rmsd"] - o["rmsd"]) <= 1e-6 * o["rmsd"], (g["rmsd"], o["rmsd"])
+        nscf.append(o["n_scf"])
+    if iters == (1, 100) and algo == P.EXT_ASPC:
+        # the extrapolated iterate converges in one step on some MD step
+        assert min(nscf) == 1, nscf
 
 
 def test_pim_static_calc_mode(oracle_lib):
