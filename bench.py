@@ -90,7 +90,10 @@ def profile_traffic(kernel):
     return None, None
 
 
-def time_path(args, sys_, precision, frames_dev, scf=None, tuning=None):
+def time_path(args, sys_, precision, frames_dev, scf=None, tuning=None, table=True):
+    """Times K steps of one path (precision, SCF solver); with table, K more
+    untimed steps first fill the per-kernel table and the dominant kernel's
+    roofline is measured on the timed steps."""
     import copy
     import torch
     from mdff_amd import params as P
@@ -98,7 +101,9 @@ def time_path(args, sys_, precision, frames_dev, scf=None, tuning=None):
     from mdff_amd import flops as FL
     p = copy.deepcopy(sys_.params)
     p.precision = precision
-    if precision == P.PREC_MIXED and (scf or args.mixed_scf) == "pcg":
+    if scf is None:
+        scf = args.mixed_scf if precision == P.PREC_MIXED else "jacobi"
+    if scf == "pcg":
         p.scf_solver = P.SCF_PCG
     n = sys_.natm
     force = torch.zeros((3, n), dtype=torch.float64, device="cuda")
@@ -119,64 +124,79 @@ def time_path(args, sys_, precision, frames_dev, scf=None, tuning=None):
     W, K = args.warmup, args.steps
     for w in range(W):
         ff.engforce_device(frames_dev[w], force, itime=w + 1)
-    # per-kernel table: every launch bracketed by HIP events, K untimed steps
-    ff.profile(True)
-    for k in range(K):
-        ff.engforce_device(frames_dev[W + k], force, itime=W + k + 1)
-    torch.cuda.synchronize()
-    stats = ff.profile_stats()
-    name = max(stats.items(), key=lambda kv: kv[1][0])[0]
-    # timed region: the next K frames of the trajectory, events around the
-    # launches of the dominant kernel on every event_every-th step (every
-    # step has the same launch mix)
-    ff.profile(True, only=name)
+    stats = {}
+    name = None
+    if table:
+        # per-kernel table: every launch bracketed by HIP events, K untimed
+        # steps
+        ff.profile(True)
+        for k in range(K):
+            ff.engforce_device(frames_dev[W + k], force, itime=W + k + 1)
+        torch.cuda.synchronize()
+        stats = ff.profile_stats()
+        name = max(stats.items(), key=lambda kv: kv[1][0])[0]
+        # timed region: the next K frames of the trajectory, events around
+        # the launches of the dominant kernel on every event_every-th step
+        # (every step has the same launch mix)
+        ff.profile(True, only=name)
     every = max(1, args.event_every)
-    nscf = []
+    nscf, nind, rmsd, scf_tol = [], [], [], []
     sampled = []
     b = W + K
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(K):
-        ff.profile_select(name if k % every == 0 else None)
+        if table:
+            ff.profile_select(name if k % every == 0 else None)
         en = ff.engforce_device(frames_dev[b + k], force, itime=b + k + 1)
         nscf.append(en.n_scf)
+        nind.append(en.n_es_calls - 2)  # induced-dipole Ewald calls of the step
+        rmsd.append(en.rmsd)
+        scf_tol.append(en.scf_tol)
         if k % every == 0:
             sampled.append(k)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ms, cnt = ff.profile_stats()[name]
-    ff.profile(False)
+    if table:
+        ms, cnt = ff.profile_stats()[name]
+        ff.profile(False)
     # last frame forces for the error statement
     f_last = force.clone()
     dt = (t1 - t0) / K
     nk = ff.num_kvectors()
     ff.close()
-    # roofline of the dominant kernel from its live launches
-    avg_s = ms / cnt * 1e-3
+    # roofline of the dominant kernel from its live launches: achieved =
+    # algorithmic flops per launch (per-(atom, k) weights of mdff_amd/flops.py
+    # times N K) / the average HIP-event launch time
     roof = None
-    per_unit = FL.KERNEL_FLOPS_PER_UNIT.get(name)
-    if per_unit:
-        flop = per_unit * n * nk
-        peak, pipe = FL.PEAK_FP64_TFLOPS, "fp64 vector ALU"
-    elif name in ("mx_rho", "mx_apply"):
-        # executed MFMA work of all launches of the timed region / launches
-        # executed MFMA work of the sampled launches / their number
-        kmax = list(p.kES)
-        nsc = sum(nscf[k] for k in sampled)
-        flop = FL.mixed_kspace_flops(name, n, kmax, len(sampled), nsc, p.doefg) / cnt
-        peak, pipe = FL.PEAK_FP16_MFMA_TFLOPS, "fp16 matrix core (3 hi/lo products)"
-    else:
-        flop = None
+    flop = None
+    if table:
+        avg_s = ms / cnt * 1e-3
+        per_unit = FL.KERNEL_FLOPS_PER_UNIT.get(name)
+        nsc = sum(nind[k] for k in sampled)
+        if per_unit:
+            flop = per_unit * n * nk
+            peak, bound, pipe = FL.PEAK_FP64_TFLOPS, "valu", "fp64 vector ALU"
+        elif name in ("mx_rho", "mx_apply"):
+            flop = FL.kspace_algorithmic_flops(name, n, nk, len(sampled), nsc, p.doefg) / cnt
+            executed = FL.mixed_kspace_flops(name, n, list(p.kES), len(sampled), nsc,
+                                             p.doefg) / cnt
+            peak, bound = FL.PEAK_FP16_MFMA_TFLOPS, "mfma"
+            pipe = "fp16 matrix core (operands split in fp16 hi + lo, 3 products)"
     if flop:
         achieved = flop / avg_s / 1e12
         traffic, source = profile_traffic(name)
-        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        roof = {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": traffic, "kernel": name,
                 "avg_launch_us": avg_s * 1e6, "launches": cnt, "sampled_steps": len(sampled),
                 "flops_per_launch": flop, "pipe": pipe}
+        if bound == "mfma":
+            roof["executed_mfma_flops_per_launch"] = executed
+            roof["executed_mfma_frac"] = executed / avg_s / 1e12 / peak
         if source:
             roof["traffic_source"] = source
     return {"ms_per_step": dt * 1e3, "atom_steps_per_s": n / dt, "n_scf": nscf,
+            "rmsd_max": max(rmsd), "scf_tol_max": max(scf_tol), "scf": scf,
             "kernels": {k: {"total_ms": v[0], "launches": v[1]} for k, v in stats.items()},
             "roofline": roof, "force": f_last,
             "tuning": ({"apply_slices": tuning["apply_slices"],
@@ -256,12 +276,16 @@ def batch_leg(args):
     return out
 
 
-def cpu_baseline(args, sys_, n_es_per_step):
-    """Oracle (plain C, 1 thread) on a bounded sample of the same workload:
-    the full real-space BMHFTD pair sum and one Ewald call with all atoms,
-    the real-space sum over all pairs and the reciprocal sum restricted to
-    kmax = 3 (the k-space cost is linear in the number of k vectors, so it
-    is scaled to kmax 24); per step = BMHFTD + n_es Ewald calls."""
+def cpu_baseline(args, sys_, n_ind_per_step):
+    """Oracle (plain C, 1 thread) on a bounded sample of the same workload.
+    Timed in full: the BMHFTD pair sum.  Each of the step's three kinds of
+    Ewald call (static: charges, field; induced: dipoles, field; final:
+    charges + dipoles, field, EFG, forces, stress) is timed with all atoms
+    and the full real-space sum at kmax 1, and on the first quarter of the
+    atoms at kmax 3 and kmax 6: the k-space cost is linear in atoms x k
+    vectors, so the difference of the last two gives the cost per (atom, k),
+    scaled to all atoms and the workload's k vectors.  Per step = BMHFTD +
+    static + n_ind induced + final."""
     import copy
     import numpy as np
     from oracle import oracle as O
@@ -270,33 +294,48 @@ def cpu_baseline(args, sys_, n_es_per_step):
     n = sys_.natm
     p = copy.deepcopy(sys_.params)
     kfull = P.num_kvectors(list(p.kES))
-    p.kES[:] = [3, 3, 3]
-    ks = P.num_kvectors([3, 3, 3])
     q = np.array([p.qch[t - 1] for t in sys_.itype])
-    mu = np.zeros((n, 3))
+    mu = np.where((sys_.itype == 2)[:, None],
+                  0.05 * np.random.default_rng(7).standard_normal((n, 3)), 0.0)
+    measured = 0.0
+    # short range: BMHFTD only
+    psr = copy.deepcopy(p)
+    psr.lcoulomb = 0
     t0 = time.perf_counter()
-    O.multipole_es(p, sys_.itype, sys_.A, sys_.r, q, mu, [1, 0, 0], do_efg=False,
-                   do_forces=False, do_stress=False)
-    t_es_small = time.perf_counter() - t0
-    # real-space-only part (no k vectors): kES = 0 not allowed, time the
-    # k-space share separately with kmax 1
-    p1 = copy.deepcopy(p)
-    p1.kES[:] = [1, 1, 1]
-    k1 = P.num_kvectors([1, 1, 1])
-    t0 = time.perf_counter()
-    O.multipole_es(p1, sys_.itype, sys_.A, sys_.r, q, mu, [1, 0, 0], do_efg=False,
-                   do_forces=False, do_stress=False)
-    t_es_k1 = time.perf_counter() - t0
-    per_k = max(t_es_small - t_es_k1, 0.0) / (ks - k1)
-    t_real = max(t_es_k1 - per_k * k1, 0.0)
-    t_es = t_real + per_k * kfull
-    step = n_es_per_step * t_es
+    O.Oracle(psr, sys_.itype, sys_.A).engforce(sys_.r)
+    t_sr = time.perf_counter() - t0
+    measured += t_sr
+    kinds = {"static": (q, np.zeros((n, 3)), [1, 0, 0], dict(do_efg=False, do_forces=False,
+                                                             do_stress=False)),
+             "induced": (np.zeros(n), mu, [0, 0, 1], dict(do_efg=False, do_forces=False,
+                                                          do_stress=False, damp_ind=False)),
+             "final": (q, mu, [1, 1, 1], dict(do_efg=bool(p.doefg)))}
+    per_call = {}
+    m = n // 4
+    for kind, (qq, mm, task, kw) in kinds.items():
+        t = []
+        for km, na in ((3, m), (6, m), (1, n)):
+            pk = copy.deepcopy(p)
+            pk.kES[:] = [km, km, km]
+            pk.natm = na
+            t0 = time.perf_counter()
+            O.multipole_es(pk, sys_.itype[:na], sys_.A, sys_.r[:na], qq[:na], mm[:na], task, **kw)
+            t.append((P.num_kvectors([km] * 3), time.perf_counter() - t0))
+            measured += t[-1][1]
+        (k1, t1), (k2, t2), (k0, t0) = t
+        per_atom_k = max(t2 - t1, 0.0) / ((k2 - k1) * m)
+        real = max(t0 - per_atom_k * k0 * n, 0.0)
+        per_call[kind] = real + per_atom_k * kfull * n
+    step = t_sr + per_call["static"] + n_ind_per_step * per_call["induced"] + per_call["final"]
     return {"value": n / step, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle (C, 1 thread) on the same {n}-atom configuration: one Ewald "
-                       f"call with the full real-space sum and {ks} k vectors, k-space scaled "
-                       f"linearly to {kfull} k vectors, x {n_es_per_step:.1f} Ewald calls per "
-                       f"step (short-range pairs not included)"),
-            "seconds_measured": t_es_small + t_es_k1}
+            "sample": (f"oracle (C, 1 thread) on the same {n}-atom configuration: the BMHFTD "
+                       f"pair sum, and each kind of Ewald call (static, induced, final) with "
+                       f"all atoms at kmax 1 (real space) and with {m} atoms at kmax 3 and 6 "
+                       f"(k-space cost per atom and k vector), scaled to {n} atoms and "
+                       f"{kfull} k vectors; per step BMHFTD + static + {n_ind_per_step:.2f} "
+                       f"induced + final call"),
+            "seconds_per_step_estimate": step,
+            "seconds_measured": measured}
 
 
 def main():
@@ -336,11 +375,14 @@ def main():
         log(args, f"[bench] {key}: {res[key]['ms_per_step']:.2f} ms/step, "
                   f"{res[key]['atom_steps_per_s']:.4g} atom-steps/s, n_scf {res[key]['n_scf']}")
     if "mixed" in res and "fp64" in res and args.mixed_scf == "pcg":
-        # the same mixed path with the reference's SCF iteration, for the
-        # speedup at equal algorithm
+        # the speedup at equal algorithm: the mixed path with the
+        # reference's Jacobi iteration, and the FP64 path with the
+        # conjugate-gradient SCF
         log(args, "[bench] timing mixed with the Jacobi SCF ...")
         res["mixed_jacobi"] = time_path(args, sys_, P.PREC_MIXED, frames_dev, scf="jacobi",
-                                        tuning=res["mixed"]["tuning"])
+                                        tuning=res["mixed"]["tuning"], table=False)
+        log(args, "[bench] timing fp64 with the conjugate-gradient SCF ...")
+        res["fp64_pcg"] = time_path(args, sys_, P.PREC_FP64, frames_dev, scf="pcg", table=False)
     head = res.get("mixed", res.get("fp64"))
 
     def ferr(key):
@@ -350,8 +392,10 @@ def main():
     err = ferr("mixed") if "mixed" in res and "fp64" in res else None
     cpu = None
     if not args.no_cpu:
-        n_es = float(np.mean(res["fp64" if "fp64" in res else "mixed"]["n_scf"])) + 2.0
-        cpu = cpu_baseline(args, sys_, n_es)
+        # the reference's Jacobi iteration: one induced-dipole call per SCF
+        # iteration
+        n_ind = float(np.mean(res["fp64" if "fp64" in res else "mixed"]["n_scf"]))
+        cpu = cpu_baseline(args, sys_, n_ind)
         log(args, f"[bench] cpu baseline {cpu['value']:.4g} atom-steps/s "
                   f"(measured {cpu['seconds_measured']:.1f}s)")
     p = sys_.params
@@ -366,12 +410,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if "mixed" in res else "f64",
+        # the mixed path: FP32 pair and real-space arithmetic, reciprocal
+        # sums on the fp16 matrix cores with every operand split in fp16
+        # hi + lo (22-bit significands, 3 products, FP32 accumulation),
+        # FP64 reductions and SCF residual
+        "dtype": "f16x2-split mfma + f32 + f64" if "mixed" in res else "f64",
         "data": "synthetic",
         "config": {
             "workload": (f"{sys_.name}: polarizable silica-like melt, {sys_.natm} atoms, "
                          f"L={sys_.info['L']:.4f} A, rc {p.cutlongrange} A, kmax {p.kES[0]}, "
-                         f"alpha {p.alphaES:.5f}, SCF tol {p.conv_tol_ind:g}, EFG at all sites"
+                         f"alpha {p.alphaES:.5f}, SCF tol {p.conv_tol_ind:g}, EFG at all sites; "
+                         f"random placement with minimum separations Si-Si "
+                         f"{C.PIM_DMIN[(1, 1)]} A, Si-O {C.PIM_DMIN[(1, 2)]} A, O-O "
+                         f"{C.PIM_DMIN[(2, 2)]} A (a common 1.2 A floor lets O-O pairs "
+                         f"polarize catastrophically)"
                          if p.ldip_polar[1] else sys_.name),
             "natm": sys_.natm,
             "kvectors": P.num_kvectors(list(p.kES)),
@@ -385,21 +437,27 @@ def main():
         if key in res:
             r = res[key]
             out[key] = {"atom_steps_per_s": r["atom_steps_per_s"], "ms_per_step": r["ms_per_step"],
-                        "mean_n_scf": float(np.mean(r["n_scf"])), "roofline": r["roofline"],
-                        "kernels": r["kernels"]}
+                        "mean_n_scf": float(np.mean(r["n_scf"])), "scf_solver": r["scf"],
+                        "scf_rmsd_max": r["rmsd_max"], "scf_tol_max": r["scf_tol_max"],
+                        "roofline": r["roofline"], "kernels": r["kernels"]}
             if r["tuning"]:
                 out[key]["tuning"] = r["tuning"]
     if err is not None:
         out["mixed_vs_fp64"] = {"speedup": res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"],
                                 "force_rel_rms_err": err}
-    if "mixed_jacobi" in res:
-        r = res["mixed_jacobi"]
-        out["mixed_jacobi"] = {"atom_steps_per_s": r["atom_steps_per_s"],
-                               "ms_per_step": r["ms_per_step"],
-                               "mean_n_scf": float(np.mean(r["n_scf"]))}
-        if "fp64" in res:
-            out["mixed_jacobi"]["speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / r["ms_per_step"]
-            out["mixed_jacobi"]["force_rel_rms_err"] = ferr("mixed_jacobi")
+    for key in ("mixed_jacobi", "fp64_pcg"):
+        if key not in res:
+            continue
+        r = res[key]
+        out[key] = {"atom_steps_per_s": r["atom_steps_per_s"], "ms_per_step": r["ms_per_step"],
+                    "mean_n_scf": float(np.mean(r["n_scf"])), "scf_rmsd_max": r["rmsd_max"],
+                    "scf_tol_max": r["scf_tol_max"], "force_rel_rms_err": ferr(key)}
+    if "mixed_jacobi" in res and "fp64_pcg" in res:
+        # mixed against FP64 with the same SCF algorithm
+        out["mixed_vs_fp64"]["speedup_jacobi"] = (res["fp64"]["ms_per_step"]
+                                                  / res["mixed_jacobi"]["ms_per_step"])
+        out["mixed_vs_fp64"]["speedup_pcg"] = (res["fp64_pcg"]["ms_per_step"]
+                                               / res["mixed"]["ms_per_step"])
     if not args.no_batch:
         out["batch"] = batch_leg(args)
     print(json.dumps(out), flush=True)

@@ -40,6 +40,23 @@ PEAK_FP16_MFMA_TFLOPS = 2500.0  # dense FP16/BF16 matrix
 PEAK_HBM_GBS = 8000.0
 
 
+def kspace_algorithmic_flops(kernel: str, natm: int, nk: int, steps: int, n_ind_total: int,
+                             efg: bool) -> float:
+    """Algorithmic flops of the reciprocal-space sums over `steps`
+    evaluations with `n_ind_total` induced-dipole Ewald calls in all, from
+    the per-(atom, k) weights above: per evaluation one static call
+    (charges), the induced-dipole calls and one final call (charges and
+    dipoles; field, and EFG / dipole forces when requested).
+      rho:   RHO_CHG, RHO_DIP per SCF call, RHO_BOTH
+      apply: APPLY_FIELD, APPLY_FIELD per SCF call, APPLY_EFG (or FIELD)"""
+    nK = float(natm) * nk
+    if kernel in ("mx_rho", "k_rho64"):
+        w = steps * (RHO_CHG + RHO_BOTH) + n_ind_total * RHO_DIP
+    else:
+        w = steps * (APPLY_FIELD + (APPLY_EFG if efg else APPLY_FIELD)) + n_ind_total * APPLY_FIELD
+    return w * nK
+
+
 def mixed_kspace_flops(kernel: str, natm: int, kmax, steps: int, n_scf_total: int,
                        efg: bool) -> float:
     """Executed matrix-core flops of the mixed k-space GEMMs over a run of
