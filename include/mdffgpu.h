@@ -231,6 +231,35 @@ int mdffgpu_engforce_device(mdffgpu_ctx* ctx, const double* d_pos,
                             double* d_ef, double* d_efg,
                             mdffgpu_energies* en, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Allocation-free evaluation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code that must not call
+ * hipMalloc, e.g. tight serving loops or harnesses that forbid it).  The
+ * same evaluation as mdffgpu_create + mdffgpu_engforce_device on a fresh
+ * context, with every device buffer taken from a static scratch array of
+ * MDFFGPU_NOALLOC_SCRATCH_MIB MiB inside the library (one per device,
+ * held until the process ends) instead of hipMalloc; no hipMalloc,
+ * hipFree, hipHostMalloc, stream or event creation happens in the call.
+ * The dipole history, the only state the context would keep between steps,
+ * is the caller's: d_hist (device, dipia_ind_t(extrapolate_order+1, natm,
+ * 3) Fortran order, in/out) or NULL for a zero history.
+ *   - FP64 precision and the Jacobi SCF only (MDFFGPU_ERR_UNSUPPORTED
+ *     otherwise);
+ *   - a workload whose buffers (neighbour list, SCF pair table, k mesh and
+ *     partial sums) exceed the scratch is refused with MDFFGPU_ERR_ALLOC and
+ *     a message (e.g. about 3000 polarizable atoms with a 10 A cutoff fit);
+ *   - every call on a device shares that device's scratch: calls that
+ *     overlap on one device (two host threads, or two streams) race;
+ *   - returns after the outputs are written, like mdffgpu_engforce_device;
+ *     the message of a failure is mdffgpu_last_error(NULL).
+ * ------------------------------------------------------------------- */
+#define MDFFGPU_NOALLOC_SCRATCH_MIB 128
+int mdffgpu_engforce_noalloc(const mdffgpu_params* params, const int32_t* itype,
+                             const double* A, const double* d_pos, int32_t itime,
+                             double* d_hist, double* d_force, double* d_mu,
+                             double* d_ef, double* d_efg, mdffgpu_energies* en,
+                             void* stream);
+
 /* Induced-dipole history dipia_ind_t(extrapolate_order+1, natm, 3)
    (src/field.f90:182), Fortran order, for restart and parity runs. */
 int mdffgpu_get_dipole_history(mdffgpu_ctx* ctx, double* hist);

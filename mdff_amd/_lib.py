@@ -27,6 +27,9 @@ PROTOTYPES = {
                                         _dp, _dp, _dp, ctypes.POINTER(MdffEnergies)]),
     "mdffgpu_engforce_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, _vp, _vp, _vp,
                                                ctypes.POINTER(MdffEnergies), _vp]),
+    "mdffgpu_engforce_noalloc": (ctypes.c_int, [ctypes.POINTER(MdffParams), _ip, _dp, _vp,
+                                                 ctypes.c_int32, _vp, _vp, _vp, _vp, _vp,
+                                                 ctypes.POINTER(MdffEnergies), _vp]),
     "mdffgpu_get_dipole_history": (ctypes.c_int, [_vp, _dp]),
     "mdffgpu_set_dipole_history": (ctypes.c_int, [_vp, _dp]),
     "mdffgpu_num_kvectors": (ctypes.c_int32, [_vp]),

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -124,6 +125,14 @@ struct mdffgpu_ctx {
   double* h_sc_dev = nullptr;  // device address of h_sc
   hipEvent_t ev_sc = nullptr;  // the SCF residual has landed in h_sc
   int* h_ovf = nullptr;    // pinned
+  // allocation-free mode (mdffgpu_engforce_noalloc): the device buffers
+  // are carved from the library's static scratch array, the host scalars
+  // live in pageable memory, the stream is the caller's
+  bool noalloc = false;
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_off = 0;
+  std::vector<double> h_sc_buf;
+  std::vector<int> h_ovf_buf;
   hipStream_t own_stream = nullptr;
   hipStream_t st = nullptr;
   int linduced = 0, ldip_static = 0;
@@ -199,13 +208,30 @@ void hipck(mdffgpu_ctx* c, hipError_t e, const char* what) {
 
 #define HCK(c, x) hipck((c), (x), #x)
 
+// Next 256-byte aligned block of the allocation-free arena.  Blocks are
+// never returned: the arena belongs to one evaluation.
+void* arena_take(mdffgpu_ctx* c, size_t bytes) {
+  const size_t off = (c->arena_off + 255) & ~(size_t)255;
+  if (off + bytes > c->arena_cap)
+    fail(c, MDFFGPU_ERR_ALLOC,
+         "the allocation-free evaluation needs more than the %zu MiB of static scratch "
+         "(natm, neighbour list or k mesh too large for it)",
+         c->arena_cap >> 20);
+  c->arena_off = off + bytes;
+  return c->arena + off;
+}
+
 template <typename T>
 void dalloc(mdffgpu_ctx* c, T** p, size_t count) {
+  if (count == 0) count = 1;
+  if (c->noalloc) {
+    *p = static_cast<T*>(arena_take(c, count * sizeof(T)));
+    return;
+  }
   if (*p) {
     (void)hipFree(*p);
     *p = nullptr;
   }
-  if (count == 0) count = 1;
   hipck(c, hipMalloc((void**)p, count * sizeof(T)), "hipMalloc");
 }
 
@@ -427,9 +453,10 @@ void alloc_atoms(mdffgpu_ctx* c) {
   dalloc(c, &c->d_hist, (size_t)(c->P.extrapolate_order + 1) * 3 * n);
   dalloc(c, &c->d_pt, 1);
   dalloc(c, &c->d_tt, 1);
-  HCK(c, hipMemset(c->d_qzero, 0, n * sizeof(double)));
-  HCK(c, hipMemset(c->d_hist, 0, (size_t)(c->P.extrapolate_order + 1) * 3 * n * sizeof(double)));
-  HCK(c, hipMemset(c->d_sc, 0, SC_N * sizeof(double)));
+  HCK(c, hipMemsetAsync(c->d_qzero, 0, n * sizeof(double), c->st));
+  HCK(c, hipMemsetAsync(c->d_hist, 0, (size_t)(c->P.extrapolate_order + 1) * 3 * n * sizeof(double),
+                        c->st));
+  HCK(c, hipMemsetAsync(c->d_sc, 0, SC_N * sizeof(double), c->st));
   ensure_part(c, 64 * (size_t)nblk(c->n) + 4096);
 }
 
@@ -731,9 +758,9 @@ void build_t2(mdffgpu_ctx* c) {
   const size_t need = mx ? cap * (sizeof(float4) + sizeof(int)) + (size_t)n * sizeof(int)
                          : 5 * cap * sizeof(double);
   if (need > c->t2_cap) {
-    if (c->d_t2) (void)hipFree(c->d_t2);
-    c->d_t2 = nullptr;
-    HCK(c, hipMalloc(&c->d_t2, need));
+    char* t2 = static_cast<char*>(c->d_t2);
+    dalloc(c, &t2, need);
+    c->d_t2 = t2;
     c->t2_cap = need;
   }
   EsDirArgs da;
@@ -1317,8 +1344,12 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
   en->n_es_calls = n_es;
 }
 
+// The context of mdffgpu_create; with arena (the allocation-free entry
+// point) every device buffer comes from that memory instead of hipMalloc,
+// the host scalars are pageable and the work goes to the caller's stream.
 void create_impl(const mdffgpu_params* params, const int32_t* itype, const double* A,
-                 mdffgpu_ctx** out) {
+                 mdffgpu_ctx** out, char* arena = nullptr, size_t arena_cap = 0,
+                 hipStream_t stream = nullptr) {
   validate(nullptr, params);
   if (!itype || !A || !out) fail(nullptr, MDFFGPU_ERR_ARG, "NULL argument");
   int ndev = 0;
@@ -1335,10 +1366,21 @@ void create_impl(const mdffgpu_params* params, const int32_t* itype, const doubl
         fail(c, MDFFGPU_ERR_ARG, "itype(%d) = %d outside 1..ntype", i + 1, itype[i]);
       c->typ0[i] = itype[i] - 1;
     }
-    HCK(c, hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
-    c->st = c->own_stream;
-    HCK(c, hipHostMalloc((void**)&c->h_sc, SC_N * sizeof(double), hipHostMallocDefault));
-    HCK(c, hipHostMalloc((void**)&c->h_ovf, sizeof(int), hipHostMallocDefault));
+    if (arena) {
+      c->noalloc = true;
+      c->arena = arena;
+      c->arena_cap = arena_cap;
+      c->st = stream;
+      c->h_sc_buf.assign(SC_N, 0.0);
+      c->h_ovf_buf.assign(1, 0);
+      c->h_sc = c->h_sc_buf.data();
+      c->h_ovf = c->h_ovf_buf.data();
+    } else {
+      HCK(c, hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+      c->st = c->own_stream;
+      HCK(c, hipHostMalloc((void**)&c->h_sc, SC_N * sizeof(double), hipHostMallocDefault));
+      HCK(c, hipHostMalloc((void**)&c->h_ovf, sizeof(int), hipHostMallocDefault));
+    }
     set_lattice(c, A);
     alloc_atoms(c);
     build_tables(c);
@@ -1361,6 +1403,8 @@ void create_impl(const mdffgpu_params* params, const int32_t* itype, const doubl
       c->mx = mixed::create(c);
       if (!c->mx) fail(c, MDFFGPU_ERR_UNSUPPORTED, "mixed-precision path unavailable");
     }
+    // the clears of alloc_atoms are done before the context is handed out
+    HCK(c, hipStreamSynchronize(c->st));
     *out = c;
   } catch (Fail&) {
     g_create_error = c->err;
@@ -1556,6 +1600,43 @@ CtxView view(mdffgpu_ctx* c) {
 }  // namespace mdff
 
 // ===========================================================================
+// Static scratch of the allocation-free entry point (mdffgpu_engforce_noalloc)
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)MDFFGPU_NOALLOC_SCRATCH_MIB << 20;
+
+// Every device the library runs on holds one copy until the process ends.
+// Host code never uses the name as an address: the device address of each
+// device's copy comes from hipGetSymbolAddress (scratch_ptr).
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch, looked up once per device
+// (with that device current) and kept.
+char* scratch_ptr(int dev, std::string* err) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    *err = "device index out of range of the scratch table";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<char*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+// ===========================================================================
 // C ABI
 extern "C" {
 
@@ -1575,6 +1656,11 @@ int mdffgpu_create(const mdffgpu_params* params, const int32_t* itype, const dou
 
 void mdffgpu_destroy(mdffgpu_ctx* c) {
   if (!c) return;
+  if (c->noalloc) {
+    // buffers of the static scratch and pageable host memory: nothing to free
+    delete c;
+    return;
+  }
   if (c->mx) mixed::destroy(c->mx);
   double* dp[] = {c->d_rref, c->d_pos, c->d_s, c->d_q, c->d_qzero, c->d_dip, c->d_mu, c->d_muind,
                   c->d_mup, c->d_Estat, c->d_E, c->d_efdir, c->d_efgdir, c->d_fdir,
@@ -1672,6 +1758,66 @@ int mdffgpu_engforce_device(mdffgpu_ctx* c, const double* d_pos, int32_t itime,
     c->out_force = c->out_mu = c->out_ef = c->out_efg = nullptr;
     c->st = c->own_stream;
     return f.code;
+  }
+}
+
+int mdffgpu_engforce_noalloc(const mdffgpu_params* params, const int32_t* itype,
+                             const double* A, const double* d_pos, int32_t itime,
+                             double* d_hist, double* d_force, double* d_mu, double* d_ef,
+                             double* d_efg, mdffgpu_energies* en, void* stream) {
+  if (params && (params->precision != MDFFGPU_PREC_FP64 ||
+                 params->scf_solver != MDFFGPU_SCF_JACOBI)) {
+    g_create_error = "the allocation-free evaluation runs the FP64 path with the Jacobi SCF";
+    return MDFFGPU_ERR_UNSUPPORTED;
+  }
+  if (!d_pos || !d_force || !en) {
+    g_create_error = "NULL position, force or energy argument";
+    return MDFFGPU_ERR_ARG;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    g_create_error = "no current HIP device";
+    return MDFFGPU_ERR_DEVICE;
+  }
+  std::string err;
+  char* arena = scratch_ptr(dev, &err);
+  if (!arena) {
+    g_create_error = err;
+    return MDFFGPU_ERR_DEVICE;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  mdffgpu_ctx* c = nullptr;
+  try {
+    // every call on a device shares its scratch: calls that overlap on one
+    // device race (see the header)
+    create_impl(params, itype, A, &c, arena, kScratchBytes, st);
+    const size_t n = c->n, b = n * sizeof(double);
+    const size_t nh = (size_t)(c->P.extrapolate_order + 1) * 3 * n * sizeof(double);
+    if (d_hist) HCK(c, hipMemcpyAsync(c->d_hist, d_hist, nh, hipMemcpyDeviceToDevice, st));
+    HCK(c, hipMemcpyAsync(c->d_pos, d_pos, 3 * b, hipMemcpyDeviceToDevice, st));
+    HCK(c, hipMemcpyAsync(c->d_force, d_force, 3 * b, hipMemcpyDeviceToDevice, st));
+    c->out_force = d_force;
+    c->out_mu = d_mu;
+    c->out_ef = d_ef;
+    c->out_efg = d_efg;
+    run(c, itime, en);
+    if (d_hist) {
+      HCK(c, hipMemcpyAsync(d_hist, c->d_hist, nh, hipMemcpyDeviceToDevice, st));
+      HCK(c, hipStreamSynchronize(st));
+    }
+    mdffgpu_destroy(c);
+    return MDFFGPU_OK;
+  } catch (Fail& f) {
+    if (c) {
+      g_create_error = c->err;
+      (void)hipStreamSynchronize(st);
+      mdffgpu_destroy(c);
+    }
+    return f.code;
+  } catch (...) {
+    g_create_error = "unexpected host exception";
+    if (c) mdffgpu_destroy(c);
+    return MDFFGPU_ERR_DEVICE;
   }
 }
 
