@@ -260,6 +260,61 @@ int mdffgpu_engforce_noalloc(const mdffgpu_params* params, const int32_t* itype,
                              double* d_ef, double* d_efg, mdffgpu_energies* en,
                              void* stream);
 
+/* ---------------------------------------------------------------------
+ * Standalone electric field gradients, the per-configuration body of
+ * efgcalc (calc = 'efg', src/efg.f90:325-616; ex4 of the reference's
+ * examples runs it): positions wrapped into the cell (:424-426), the
+ * moments of get_moments (src/field.f90:3686-3765: permanent dipoles plus
+ * the induced ones of the SCF, computed by this context as in
+ * mdffgpu_engforce with calc_mode 0), then the EFG at every site:
+ *   longrange 0 'ewald': multipole_efg_ES (src/efg.f90:1278-1625), every
+ *     pair j != i under the minimum image with no cutoff, the reciprocal
+ *     sum of the context's k mesh, the self term and the optional charged
+ *     cell correction;
+ *   longrange 1 'direct': multipole_efg_DS (src/efg.f90:1080-1268), bare
+ *     tensors over the image cells -ncelldirect..ncelldirect along each
+ *     lattice vector, pairs closer than cutdirect.
+ * The context's own Coulomb cutoff, alpha and k mesh serve the SCF and the
+ * Ewald reciprocal sum; nothing here changes them.  The direct sum uses none
+ * of them (the context still needs valid values, e.g. kES 1 1 1 and a
+ * cutlongrange within half the cell).
+ * ------------------------------------------------------------------- */
+typedef struct mdffgpu_efg_opts {
+  int32_t longrange;       /* 0 'ewald', 1 'direct' */
+  int32_t ncelldirect;     /* image cells per direction (default 2,
+                              src/field.f90:225) */
+  double cutdirect;        /* cutlongrange of the direct sum; ewald_param
+                              does not clamp it there (src/field.f90:532) */
+  int32_t lefg_it_contrib; /* only atoms of species it_efg contribute
+                              (src/efg.f90:1163, 1406, 1575, 2922) */
+  int32_t it_efg;          /* 1-based species */
+  int32_t lmp_correction;  /* + 4 pi Q / (3 V) on the diagonal, Q the
+                              total (contributing) charge (:1592-1605) */
+  int32_t lefg_vasp_sign;  /* efg_t = -efg_t (src/efg.f90:486-488) */
+  int32_t lefg_old;        /* charge-only routines efg_ES / efg_DS:
+                              refused with MDFFGPU_ERR_UNSUPPORTED */
+  int32_t reserved[3];
+} mdffgpu_efg_opts;
+
+/* efg_t out: (natm,3,3) Fortran order, symmetric, times coul_unit and the
+   sign option; mu_t out (may be NULL): the moments the EFG was taken with;
+   en out (may be NULL): n_scf, rmsd and scf_tol of the SCF, the rest 0.
+   rx, ry, rz are host positions. */
+int mdffgpu_efg(mdffgpu_ctx* ctx, const mdffgpu_efg_opts* opts,
+                const double* rx, const double* ry, const double* rz,
+                double* mu_t, double* efg_t, mdffgpu_energies* en);
+
+/* Per-site NMR parameters of efg_stat (src/efg.f90:2233-2269) for n sites:
+   efg6 in (n,6) Fortran order with the columns vxx vyy vzz vxy vxz vyz
+   (the EFGALL record); nmr out (n,4): principal components ordered by
+   nmr_convention (src/efg.f90:2546-2648) as Vxx, Vyy, Vzz and the
+   asymmetry eta; U out (n,5) the Czjzek components, S out (n) their norm;
+   bad out (may be NULL): sites whose ordering fails the checks of
+   src/efg.f90:2618-2629, where the reference stops.  Runs on the current
+   device; host arrays. */
+int mdffgpu_efg_nmr(int32_t n, const double* efg6, double* nmr, double* U,
+                    double* S, int32_t* bad);
+
 /* Induced-dipole history dipia_ind_t(extrapolate_order+1, natm, 3)
    (src/field.f90:182), Fortran order, for restart and parity runs. */
 int mdffgpu_get_dipole_history(mdffgpu_ctx* ctx, double* hist);

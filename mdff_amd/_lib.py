@@ -10,7 +10,7 @@ import ctypes
 import importlib.util
 import os
 
-from .params import MdffBatchConfig, MdffEnergies, MdffParams
+from .params import MdffBatchConfig, MdffEfgOpts, MdffEnergies, MdffParams
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmdffgpu.so")
 
@@ -30,6 +30,9 @@ PROTOTYPES = {
     "mdffgpu_engforce_noalloc": (ctypes.c_int, [ctypes.POINTER(MdffParams), _ip, _dp, _vp,
                                                  ctypes.c_int32, _vp, _vp, _vp, _vp, _vp,
                                                  ctypes.POINTER(MdffEnergies), _vp]),
+    "mdffgpu_efg": (ctypes.c_int, [_vp, ctypes.POINTER(MdffEfgOpts), _dp, _dp, _dp, _dp, _dp,
+                                   ctypes.POINTER(MdffEnergies)]),
+    "mdffgpu_efg_nmr": (ctypes.c_int, [ctypes.c_int32, _dp, _dp, _dp, _dp, _ip]),
     "mdffgpu_get_dipole_history": (ctypes.c_int, [_vp, _dp]),
     "mdffgpu_set_dipole_history": (ctypes.c_int, [_vp, _dp]),
     "mdffgpu_num_kvectors": (ctypes.c_int32, [_vp]),

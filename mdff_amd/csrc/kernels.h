@@ -24,7 +24,8 @@ enum KernelId {
   K_PREPARE, K_NLIST, K_SR, K_DIR_STATIC, K_DIR_IND, K_DIR_FINAL, K_RHO_CHG, K_RHO_DIP,
   K_RHO_BOTH, K_RHO_RED, K_APPLY_F, K_APPLY_EFG, K_MOMENTS, K_COMBINE, K_INDUCED,
   K_RESID, K_EXTRAP, K_REDUCE, K_ADD, K_PAIRS_T2, K_DD_FIELD, K_MX_SR, K_MX_DIR, K_MX_PHASE, K_MX_RHO,
-  K_MX_RHO_RED, K_MX_APPLY, K_MX_COMBINE, K_MX_WEIGHTS, K_MX_GPREP, K_BATCH, K_ADJ, K_COUNT
+  K_MX_RHO_RED, K_MX_APPLY, K_MX_COMBINE, K_MX_WEIGHTS, K_MX_GPREP, K_BATCH, K_ADJ,
+  K_EFG_DIR, K_EFG_DS, K_EFG_COMBINE, K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
 
@@ -275,5 +276,47 @@ __global__ void k_reduce_max(const double* part, int nb, double* out);
 // lanes that share one centre atom in the real-space kernels
 constexpr int LPA = 16;
 __global__ void k_add(int m, const double* a, const double* b, double* out);
+
+// ---------------------------------------------------------------------------
+// standalone electric field gradients (efg.hip)
+struct EfgDirArgs {
+  int n, jlen;
+  const double* s;   // fractional coordinates SoA (3n)
+  const double* q;   // contributing charges
+  const double* mu;  // contributing dipoles SoA (3n)
+  Cell cell;
+  double alpha;
+  double* part;      // (grid.y, 6, n)
+};
+struct EfgDsArgs {
+  int n, ncell;
+  const double* s;
+  const double* q;
+  const double* mu;
+  const double* cells;  // image cells (n1, n2, n3) as doubles, 3 per cell
+  Cell cell;
+  double cut2;
+  double* part;         // (6, n)
+};
+struct EfgCombineArgs {
+  int n, ndir, nrec;
+  const double* dir;  // (ndir, 6, n) real-space partials
+  const double* rec;  // (nrec, 9, n) reciprocal partials of k_apply64<true>
+  Cell cell;
+  double fpi_V;       // 4 pi / V
+  double selfa;       // -4 alpha^3 / (3 sqrt(pi))
+  const double* q;    // contributing charges (self term)
+  double corr;        // charged-cell correction on the diagonal
+  double scale;       // coul_unit, negated with lefg_vasp_sign
+  double* out;        // (n, 3, 3) Fortran order
+};
+__global__ void k_efg_mask(int n, const int* typ, int it_efg, const double* q, const double* mu,
+                           double* qe, double* mue);
+template <bool DIP>
+__global__ void k_efg_es_dir(EfgDirArgs a);
+template <bool DIP>
+__global__ void k_efg_ds(EfgDsArgs a);
+__global__ void k_efg_combine(EfgCombineArgs a);
+__global__ void k_efg_nmr(int n, const double* efg6, double* nmr, double* U, double* S, int* bad);
 
 }  // namespace mdff

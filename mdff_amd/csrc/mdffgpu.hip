@@ -107,6 +107,11 @@ struct mdffgpu_ctx {
   // device-resident outputs of the current call, copied before its final
   // synchronisation (null: none)
   double *out_force = nullptr, *out_mu = nullptr, *out_ef = nullptr, *out_efg = nullptr;
+  // standalone EFG (mdffgpu_efg): contributing moments, real-space
+  // partials, image cells of the direct sum, the (natm,3,3) result
+  double *d_efgq = nullptr, *d_efgmu = nullptr, *d_efgpart = nullptr, *d_efgcells = nullptr,
+         *d_efgout = nullptr;
+  size_t efgpart_cap = 0, efgcells_cap = 0;
   double* d_rref = nullptr;  // positions at the last list build
   bool list_valid = false;
   int list_builds = 0;
@@ -155,7 +160,7 @@ const char* const kKernelNames[K_COUNT] = {
     "k_scf_resid", "k_extrap", "k_reduce_rows", "k_add", "k_pairs_t2", "k_dd_field",
     "mx_shortrange", "mx_es_dir",
     "mx_phase", "mx_rho", "mx_rho_reduce", "mx_apply", "mx_combine", "mx_weights", "mx_gprep",
-    "batch", "adjoint"};
+    "batch", "adjoint", "k_efg_es_dir", "k_efg_ds", "k_efg_combine"};
 
 ProfScope::ProfScope(mdffgpu_ctx* ctx, int kid) : c(ctx), id(kid), a(nullptr), b(nullptr) {
   if (!c || !c->prof || (c->prof_only != -1 && c->prof_only != kid)) return;
@@ -1176,18 +1181,13 @@ void update_floor(mdffgpu_ctx* c) {
 
 // engforce_driver on the device buffers of ctx; d_pos and d_force are
 // loaded by the caller.
-void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
-  ++c->evals;
+// Verlet list of the current positions: rebuilt when invalid or when the
+// skin test fails, grown until it holds every row.
+void update_list(mdffgpu_ctx* c) {
   const mdffgpu_params& P = c->P;
   const int n = c->n;
   const int gb = nblk(n);
   hipStream_t st = c->st;
-  std::memset(en, 0, sizeof(*en));
-  {
-    ProfScope _ps(c, K_PREPARE);
-    k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->d_s, c->d_s4, c->d_eph);
-  }
-  check_launch(c, "k_prepare");
   bool rebuild = !c->list_valid;
   if (c->list_cut > 0.0 && !rebuild) {
     // Verlet skin (vnlistcheck, src/tools.f90:446-535): the list holds every
@@ -1234,6 +1234,21 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
       set_list_capacity(c, (int)(*c->h_ovf * 1.2) + 16);
     }
   }
+}
+
+void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
+  ++c->evals;
+  const mdffgpu_params& P = c->P;
+  const int n = c->n;
+  const int gb = nblk(n);
+  hipStream_t st = c->st;
+  std::memset(en, 0, sizeof(*en));
+  {
+    ProfScope _ps(c, K_PREPARE);
+    k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->d_s, c->d_s4, c->d_eph);
+  }
+  check_launch(c, "k_prepare");
+  update_list(c);
   if (P.lcoulomb && c->linduced) build_t2(c);
   const bool sr = P.lnmlj || P.lbmhft || P.lbmhftd || P.lmorse;
   const bool mxs = (P.precision == MDFFGPU_PREC_MIXED) &&
@@ -1342,6 +1357,256 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
         1.0 / 3.0 * (en->tau_coul[0] + en->tau_coul[4] + en->tau_coul[8]) * P.press_unit;
   }
   en->n_es_calls = n_es;
+}
+
+// Device buffer of at least count doubles, grown (contents dropped) when
+// too small.
+void grow(mdffgpu_ctx* c, double** p, size_t* cap, size_t count) {
+  if (*p && *cap >= count) return;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  dalloc(c, p, count);
+  *cap = count;
+}
+
+// Image cells of multipole_efg_DS that can hold a pair closer than cut:
+// cells n with |A n| < cut + |a1| + |a2| + |a3| (wrapped fractional
+// differences lie in [-1, 1]^3), in the order of direct_sum_init /
+// reorder_rpt (src/rspace.f90:64-114: x outer, z inner, stably sorted by
+// |n|^2).
+std::vector<double> ds_cells(const Cell& cl, int nc, double cut) {
+  double reach = cut;
+  for (int k = 0; k < 3; ++k)
+    reach += std::sqrt(cl.A[3 * k] * cl.A[3 * k] + cl.A[3 * k + 1] * cl.A[3 * k + 1] +
+                       cl.A[3 * k + 2] * cl.A[3 * k + 2]);
+  struct Cn {
+    int x, y, z, rr;
+  };
+  std::vector<Cn> keep;
+  for (int x = -nc; x <= nc; ++x)
+    for (int y = -nc; y <= nc; ++y)
+      for (int z = -nc; z <= nc; ++z) {
+        double v[3];
+        for (int a = 0; a < 3; ++a) v[a] = x * cl.A[a] + y * cl.A[3 + a] + z * cl.A[6 + a];
+        if (std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) >= reach) continue;
+        keep.push_back({x, y, z, x * x + y * y + z * z});
+      }
+  std::stable_sort(keep.begin(), keep.end(), [](const Cn& a, const Cn& b) { return a.rr < b.rr; });
+  std::vector<double> out;
+  out.reserve(3 * keep.size());
+  for (const Cn& k : keep) {
+    out.push_back(k.x);
+    out.push_back(k.y);
+    out.push_back(k.z);
+  }
+  return out;
+}
+
+// One configuration of efgcalc (src/efg.f90:404-488) on the positions in
+// d_pos: moments, then the EFG of every site into d_efgout.
+void efg_frame(mdffgpu_ctx* c, const mdffgpu_efg_opts& o, mdffgpu_energies* en) {
+  const int n = c->n;
+  const int gb = nblk(n);
+  hipStream_t st = c->st;
+  const bool ewald = (o.longrange == 0);
+  ++c->evals;
+  {
+    ProfScope _ps(c, K_PREPARE);
+    k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->d_s, c->d_s4, c->d_eph);
+  }
+  check_launch(c, "k_prepare");
+  // get_moments (src/field.f90:3686-3765) with calc = 'efg': the SCF starts
+  // from the static field, no extrapolation
+  std::memset(en, 0, sizeof(*en));
+  if (c->linduced) {
+    const int calc_mode = c->P.calc_mode;
+    c->P.calc_mode = 0;
+    try {
+      update_list(c);
+      build_t2(c);
+      int n_es = 0;
+      scf(c, 0, &en->n_scf, &en->rmsd, &n_es, &en->scf_tol);
+      en->n_es_calls = n_es;
+    } catch (...) {
+      c->P.calc_mode = calc_mode;
+      throw;
+    }
+    c->P.calc_mode = calc_mode;
+    {
+      ProfScope _ps(c, K_ADD);
+      k_add<<<nblk(3 * n), MDFF_BLOCK, 0, st>>>(3 * n, c->d_dip, c->d_muind, c->d_mu);
+    }
+    check_launch(c, "k_add");
+  } else {
+    HCK(c, hipMemcpyAsync(c->d_mu, c->d_dip, 3 * (size_t)n * sizeof(double),
+                          hipMemcpyDeviceToDevice, st));
+  }
+  const bool dip = c->linduced || c->ldip_static;
+  // moments of the contributing atoms
+  const double* q = c->d_q;
+  const double* mu = c->d_mu;
+  if (o.lefg_it_contrib) {
+    if (!c->d_efgq) {
+      dalloc(c, &c->d_efgq, n);
+      dalloc(c, &c->d_efgmu, 3 * (size_t)n);
+    }
+    k_efg_mask<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, o.it_efg - 1, c->d_q, c->d_mu, c->d_efgq,
+                                          c->d_efgmu);
+    check_launch(c, "k_efg_mask");
+    q = c->d_efgq;
+    mu = c->d_efgmu;
+  }
+  if (!c->d_efgout) dalloc(c, &c->d_efgout, 9 * (size_t)n);
+  EfgCombineArgs ca;
+  std::memset(&ca, 0, sizeof(ca));
+  ca.n = n;
+  ca.cell = c->cell;
+  ca.fpi_V = 4.0 * MDFF_PI / c->cell.omega;
+  ca.q = q;
+  ca.out = c->d_efgout;
+  ca.scale = c->P.coul_unit * (o.lefg_vasp_sign ? -1.0 : 1.0);
+  if (ewald) {
+    // real space: j slices so that a launch has about 2048 blocks
+    const int ib = gb;
+    int js = std::max(1, std::min((2048 + ib - 1) / ib, (n + MDFF_BLOCK - 1) / MDFF_BLOCK));
+    int jlen = (n + js - 1) / js;
+    jlen = (jlen + MDFF_BLOCK - 1) / MDFF_BLOCK * MDFF_BLOCK;
+    js = (n + jlen - 1) / jlen;
+    grow(c, &c->d_efgpart, &c->efgpart_cap, (size_t)js * 6 * n);
+    EfgDirArgs da;
+    da.n = n;
+    da.jlen = jlen;
+    da.s = c->d_s;
+    da.q = q;
+    da.mu = mu;
+    da.cell = c->cell;
+    da.alpha = c->P.alphaES;
+    da.part = c->d_efgpart;
+    {
+      ProfScope _ps(c, K_EFG_DIR);
+      const dim3 g(ib, js);
+      if (dip)
+        k_efg_es_dir<true><<<g, MDFF_BLOCK, 0, st>>>(da);
+      else
+        k_efg_es_dir<false><<<g, MDFF_BLOCK, 0, st>>>(da);
+    }
+    check_launch(c, "k_efg_es_dir");
+    ca.dir = c->d_efgpart;
+    ca.ndir = js;
+    // reciprocal space: charge_density_k (src/efg.f90:2888-2930) and the
+    // EFG of every k (:1507-1556), on the FP64 structure-factor kernels
+    if (c->nchunk > 0) {
+      RhoArgs ra;
+      ra.n = n;
+      ra.nchunk = c->nchunk;
+      ra.KC = c->KC;
+      for (int a = 0; a < 3; ++a) ra.kmax[a] = c->kmax[a];
+      ra.chunks = c->d_chunks;
+      ra.eph = c->d_eph;
+      ra.q = q;
+      ra.mu = mu;
+      ra.cell = c->cell;
+      ra.atoms_per_split = c->rho_atoms;
+      ra.kpad = c->kpad;
+      ra.part = c->d_rpart;
+      dim3 g((c->nchunk + MDFF_BLOCK - 1) / MDFF_BLOCK, c->rho_split);
+      size_t lds = (size_t)RHO_TILE * (c->kmax[0] + c->kmax[1] + c->kmax[2] + 3) *
+                       sizeof(double2) + 4 * RHO_TILE * sizeof(double);
+      if (dip) {
+        ProfScope _ps(c, K_RHO_BOTH);
+        k_rho64<true, true><<<g, MDFF_BLOCK, lds, st>>>(ra);
+      } else {
+        ProfScope _ps(c, K_RHO_CHG);
+        k_rho64<true, false><<<g, MDFF_BLOCK, lds, st>>>(ra);
+      }
+      check_launch(c, "k_rho64");
+      RhoRedArgs rr;
+      rr.kpad = c->kpad;
+      rr.nsplit = c->rho_split;
+      rr.do_energy = 0;
+      rr.part = c->d_rpart;
+      rr.Ak = c->d_Ak;
+      rr.kcoe = c->d_kcoe;
+      rr.kvec = c->d_kvec;
+      rr.g = c->d_g;
+      rr.pout = c->d_part;
+      {
+        ProfScope _ps(c, K_RHO_RED);
+        k_rho_reduce64<<<nblk(c->kpad), MDFF_BLOCK, 0, st>>>(rr);
+      }
+      check_launch(c, "k_rho_reduce64");
+      ApplyArgs aa;
+      aa.n = n;
+      aa.nchunk = c->nchunk;
+      aa.KC = c->KC;
+      aa.chunks_per_split = c->app_chunks;
+      aa.chunks = c->d_chunks;
+      aa.g = c->d_g;
+      aa.eph = c->d_eph;
+      aa.s = c->d_s;
+      aa.s4 = c->d_s4;
+      aa.part = c->d_apart;
+      {
+        ProfScope _ps(c, K_APPLY_EFG);
+        k_apply64<true><<<dim3(gb, c->app_split), MDFF_BLOCK, 0, st>>>(aa);
+      }
+      check_launch(c, "k_apply64");
+      ca.rec = c->d_apart;
+      ca.nrec = c->app_split;
+    }
+    const double alpha = c->P.alphaES;
+    ca.selfa = -4.0 * alpha * alpha * alpha / 3.0 / std::sqrt(MDFF_PI);
+    if (o.lmp_correction) {
+      // total charge of the contributing atoms (src/efg.f90:1596-1608)
+      double Q = 0.0;
+      for (int i = 0; i < n; ++i) {
+        const int t = c->typ0[i];
+        if (o.lefg_it_contrib && t != o.it_efg - 1) continue;
+        Q = Q + c->P.qch[t];
+      }
+      ca.corr = Q * ca.fpi_V / 3.0;
+    }
+  } else {
+    std::vector<double> cells = ds_cells(c->cell, o.ncelldirect, o.cutdirect);
+    const int ncell = (int)(cells.size() / 3);
+    if (ncell == 0) fail(c, MDFFGPU_ERR_ARG, "no image cell within the direct-sum cutoff");
+    grow(c, &c->d_efgcells, &c->efgcells_cap, cells.size());
+    grow(c, &c->d_efgpart, &c->efgpart_cap, 6 * (size_t)n);
+    HCK(c, hipMemcpyAsync(c->d_efgcells, cells.data(), cells.size() * sizeof(double),
+                          hipMemcpyHostToDevice, st));
+    EfgDsArgs da;
+    da.n = n;
+    da.ncell = ncell;
+    da.s = c->d_s;
+    da.q = q;
+    da.mu = mu;
+    da.cells = c->d_efgcells;
+    da.cell = c->cell;
+    da.cut2 = o.cutdirect * o.cutdirect;
+    da.part = c->d_efgpart;
+    {
+      ProfScope _ps(c, K_EFG_DS);
+      if (dip)
+        k_efg_ds<true><<<n, MDFF_BLOCK, 0, st>>>(da);
+      else
+        k_efg_ds<false><<<n, MDFF_BLOCK, 0, st>>>(da);
+    }
+    check_launch(c, "k_efg_ds");
+    ca.dir = c->d_efgpart;
+    ca.ndir = 1;
+  }
+  {
+    ProfScope _ps(c, K_EFG_COMBINE);
+    k_efg_combine<<<gb, MDFF_BLOCK, 0, st>>>(ca);
+  }
+  check_launch(c, "k_efg_combine");
+  sync_scalars(c);
+  if (c->linduced) update_floor(c);
+  if (c->upol_pending) {
+    c->u_pol = 0.5 * c->h_sc[SC_UPOL];
+    c->upol_pending = false;
+  }
 }
 
 // The context of mdffgpu_create; with arena (the allocation-free entry
@@ -1665,7 +1930,8 @@ void mdffgpu_destroy(mdffgpu_ctx* c) {
   double* dp[] = {c->d_rref, c->d_pos, c->d_s, c->d_q, c->d_qzero, c->d_dip, c->d_mu, c->d_muind,
                   c->d_mup, c->d_Estat, c->d_E, c->d_efdir, c->d_efgdir, c->d_fdir,
                   c->d_force, c->d_ef, c->d_efg, c->d_apart, c->d_Ak, c->d_kcoe, c->d_kvec,
-                  c->d_part, c->d_sc, c->d_hist, c->d_upart, c->d_cg};
+                  c->d_part, c->d_sc, c->d_hist, c->d_upart, c->d_cg, c->d_efgq,
+                  c->d_efgmu, c->d_efgpart, c->d_efgcells, c->d_efgout};
   for (double* p : dp)
     if (p) (void)hipFree(p);
   if (c->d_t2) (void)hipFree(c->d_t2);
@@ -1819,6 +2085,99 @@ int mdffgpu_engforce_noalloc(const mdffgpu_params* params, const int32_t* itype,
     if (c) mdffgpu_destroy(c);
     return MDFFGPU_ERR_DEVICE;
   }
+}
+
+int mdffgpu_efg(mdffgpu_ctx* c, const mdffgpu_efg_opts* o, const double* rx, const double* ry,
+                const double* rz, double* mu_t, double* efg_t, mdffgpu_energies* en) {
+  if (!c) return MDFFGPU_ERR_ARG;
+  if (!o || !rx || !ry || !rz || !efg_t) {
+    c->err = "NULL options, position or EFG argument";
+    return MDFFGPU_ERR_ARG;
+  }
+  if (c->noalloc) {
+    c->err = "not available on an allocation-free context";
+    return MDFFGPU_ERR_UNSUPPORTED;
+  }
+  if (o->lefg_old) {
+    c->err = "lefg_old (charge-only efg_ES / efg_DS) is not supported";
+    return MDFFGPU_ERR_UNSUPPORTED;
+  }
+  if (o->longrange != 0 && o->longrange != 1) {
+    c->err = "longrange must be 0 (ewald) or 1 (direct)";
+    return MDFFGPU_ERR_ARG;
+  }
+  if (!c->P.lcoulomb) {
+    c->err = "the EFG evaluation needs lcoulomb";
+    return MDFFGPU_ERR_ARG;
+  }
+  if (o->longrange == 1 && c->linduced) {
+    c->err = "induced dipoles need the Ewald sum (longrange = 'ewald')";
+    return MDFFGPU_ERR_UNSUPPORTED;
+  }
+  if (o->longrange == 1 && (o->ncelldirect < 0 || o->ncelldirect > 256 || !(o->cutdirect > 0.0))) {
+    c->err = "direct sum: ncelldirect must be 0..256 and the cutoff positive";
+    return MDFFGPU_ERR_ARG;
+  }
+  if (o->longrange == 0 && !(c->P.alphaES > 0.0)) {
+    c->err = "Ewald sum: alphaES must be positive";
+    return MDFFGPU_ERR_ARG;
+  }
+  if (o->lefg_it_contrib && (o->it_efg < 1 || o->it_efg > c->P.ntype)) {
+    c->err = "it_efg must be a species index 1..ntype";
+    return MDFFGPU_ERR_ARG;
+  }
+  try {
+    c->st = c->own_stream;
+    const size_t n = c->n, b = n * sizeof(double);
+    HCK(c, hipMemcpyAsync(c->d_pos, rx, b, hipMemcpyHostToDevice, c->st));
+    HCK(c, hipMemcpyAsync(c->d_pos + n, ry, b, hipMemcpyHostToDevice, c->st));
+    HCK(c, hipMemcpyAsync(c->d_pos + 2 * n, rz, b, hipMemcpyHostToDevice, c->st));
+    mdffgpu_energies tmp;
+    efg_frame(c, *o, en ? en : &tmp);
+    HCK(c, hipMemcpyAsync(efg_t, c->d_efgout, 9 * b, hipMemcpyDeviceToHost, c->st));
+    if (mu_t) HCK(c, hipMemcpyAsync(mu_t, c->d_mu, 3 * b, hipMemcpyDeviceToHost, c->st));
+    HCK(c, hipStreamSynchronize(c->st));
+    return MDFFGPU_OK;
+  } catch (Fail& f) {
+    (void)hipStreamSynchronize(c->st);
+    return f.code;
+  }
+}
+
+int mdffgpu_efg_nmr(int32_t n, const double* efg6, double* nmr, double* U, double* S,
+                    int32_t* bad) {
+  if (n < 0 || (n > 0 && (!efg6 || !nmr || !U || !S))) {
+    g_create_error = "mdffgpu_efg_nmr: NULL array or negative count";
+    return MDFFGPU_ERR_ARG;
+  }
+  if (bad) *bad = 0;
+  if (n == 0) return MDFFGPU_OK;
+  const size_t b = (size_t)n * sizeof(double);
+  double* d = nullptr;
+  int* dbad = nullptr;
+  int rc = MDFFGPU_OK;
+  if (hipMalloc((void**)&d, 16 * b) != hipSuccess || hipMalloc((void**)&dbad, sizeof(int)) != hipSuccess) {
+    g_create_error = "mdffgpu_efg_nmr: device allocation failed";
+    rc = MDFFGPU_ERR_ALLOC;
+  } else if (hipMemcpy(d, efg6, 6 * b, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemset(dbad, 0, sizeof(int)) != hipSuccess) {
+    rc = MDFFGPU_ERR_DEVICE;
+  } else {
+    k_efg_nmr<<<(n + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK>>>(n, d, d + 6 * n, d + 10 * n,
+                                                              d + 15 * n, dbad);
+    int hbad = 0;
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(nmr, d + 6 * n, 4 * b, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(U, d + 10 * n, 5 * b, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(S, d + 15 * n, b, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&hbad, dbad, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = MDFFGPU_ERR_DEVICE;
+    if (bad) *bad = hbad;
+  }
+  if (rc == MDFFGPU_ERR_DEVICE) g_create_error = "mdffgpu_efg_nmr: HIP runtime failure";
+  if (d) (void)hipFree(d);
+  if (dbad) (void)hipFree(dbad);
+  return rc;
 }
 
 int mdffgpu_get_dipole_history(mdffgpu_ctx* c, double* hist) {

@@ -109,6 +109,24 @@ class ForceField:
         out["theta"] = theta.reshape(3, 3, n).transpose(2, 1, 0).copy()
         return out
 
+    def efg(self, r, opts) -> dict:
+        """Standalone EFG of one configuration (efgcalc's per-frame body,
+        src/efg.f90:404-488) on host positions r (natm,3); opts is an
+        MdffEfgOpts (params.efg_opts).  Returns efg (natm,3,3), mu (natm,3)
+        and the SCF statistics."""
+        n = self.n
+        r = np.asarray(r, dtype=np.float64)
+        rx, ry, rz = (np.ascontiguousarray(r[:, k]) for k in range(3))
+        mu = np.zeros(3 * n)
+        efg = np.zeros(9 * n)
+        en = MdffEnergies()
+        st = _lib.lib().mdffgpu_efg(self._h, ctypes.byref(opts), _ptr(rx), _ptr(ry), _ptr(rz),
+                                    _ptr(mu), _ptr(efg), ctypes.byref(en))
+        _lib.check(st, self._h)
+        return {"efg": efg.reshape(3, 3, n).transpose(2, 1, 0).copy(),
+                "mu": mu.reshape(3, n).T.copy(), "n_scf": en.n_scf, "rmsd": en.rmsd,
+                "scf_tol": en.scf_tol}
+
     def engforce_device(self, pos, force, mu=None, ef=None, efg=None, itime: int = 0,
                         stream=None) -> MdffEnergies:
         """Evaluation on device-resident torch tensors (float64, contiguous):

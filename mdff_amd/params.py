@@ -284,3 +284,44 @@ class MdffBatchConfig(ctypes.Structure):
         ("cutlongrange", ctypes.c_double),
         ("cutshortrange", ctypes.c_double),
     ]
+
+
+class MdffEfgOpts(ctypes.Structure):
+    """Mirror of mdffgpu_efg_opts (include/mdffgpu.h): the &efgtag options of
+    the standalone EFG run (src/efg.f90:102-164) that reach the per-frame
+    evaluation, and longrange / ncelldirect / cutlongrange of the
+    control and field tags."""
+
+    _fields_ = [
+        ("longrange", ctypes.c_int32),
+        ("ncelldirect", ctypes.c_int32),
+        ("cutdirect", ctypes.c_double),
+        ("lefg_it_contrib", ctypes.c_int32),
+        ("it_efg", ctypes.c_int32),
+        ("lmp_correction", ctypes.c_int32),
+        ("lefg_vasp_sign", ctypes.c_int32),
+        ("lefg_old", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
+    ]
+
+
+EFG_EWALD = 0
+EFG_DIRECT = 1
+
+
+def efg_opts(longrange: str = "ewald", ncelldirect: int = 2, cutlongrange: float = 0.0,
+             lefg_it_contrib: bool = False, it_efg: int = 0, lmp_correction: bool = False,
+             lefg_vasp_sign: bool = False) -> MdffEfgOpts:
+    """Options with the reference defaults (efg_default_tag,
+    src/efg.f90:171-195; ncelldirect 2, src/field.f90:225)."""
+    if longrange not in ("ewald", "direct"):
+        raise ValueError("longrange must be 'ewald' or 'direct'")
+    o = MdffEfgOpts()
+    o.longrange = EFG_EWALD if longrange == "ewald" else EFG_DIRECT
+    o.ncelldirect = int(ncelldirect)
+    o.cutdirect = float(cutlongrange)
+    o.lefg_it_contrib = int(bool(lefg_it_contrib))
+    o.it_efg = int(it_efg)
+    o.lmp_correction = int(bool(lmp_correction))
+    o.lefg_vasp_sign = int(bool(lefg_vasp_sign))
+    return o

@@ -1198,3 +1198,318 @@ done:
   free(it);
   return rc;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Standalone EFG (calc = 'efg')                                           */
+
+/* reorder_rpt / merge_sort order of direct_sum_init (src/rspace.f90:64-114):
+   image cells n in -nc..nc (x outer, z inner) sorted by |n|^2, stable. */
+static int ds_cells(int nc, double* cells) {
+  int m = (2 * nc + 1) * (2 * nc + 1) * (2 * nc + 1);
+  double* rr = (double*)malloc((size_t)m * sizeof(double));
+  double* raw = (double*)malloc(3 * (size_t)m * sizeof(double));
+  int* idx = (int*)malloc(2 * (size_t)m * sizeof(int));
+  if (!rr || !raw || !idx) {
+    free(rr);
+    free(raw);
+    free(idx);
+    return -1;
+  }
+  int k = 0;
+  for (int x = -nc; x <= nc; ++x)
+    for (int y = -nc; y <= nc; ++y)
+      for (int z = -nc; z <= nc; ++z) {
+        raw[3 * k] = x;
+        raw[3 * k + 1] = y;
+        raw[3 * k + 2] = z;
+        rr[k] = (double)(x * x + y * y + z * z);
+        idx[k] = k;
+        ++k;
+      }
+  msort(rr, idx, idx + m, m);
+  for (int c = 0; c < m; ++c)
+    for (int a = 0; a < 3; ++a) cells[3 * c + a] = raw[3 * idx[c] + a];
+  free(rr);
+  free(raw);
+  free(idx);
+  return m;
+}
+
+/* multipole_efg_DS (src/efg.f90:1080-1268): s wrapped fractional
+   coordinates, efg (natm,3,3) accumulated (upper six components). */
+static int efg_ds(const mdffgpu_params* p, const mdffgpu_efg_opts* o, const int* it,
+                  const orc_cell* c, const double* s, const double* q, const double* mu,
+                  double* efg) {
+  int n = p->natm;
+  int nc = o->ncelldirect;
+  int m = (2 * nc + 1) * (2 * nc + 1) * (2 * nc + 1);
+  double* cells = (double*)malloc(3 * (size_t)m * sizeof(double));
+  if (!cells || ds_cells(nc, cells) != m) {
+    free(cells);
+    return MDFFGPU_ERR_ALLOC;
+  }
+  const double* A = c->A;
+  double cutsq = o->cutdirect * o->cutdirect;
+#define EFG(i, a, b) efg[(i) + n * ((a) + 3 * (b))]
+  for (int ia = 0; ia < n; ++ia) {
+    const double* si = s + 3 * ia;
+    for (int nc_ = 0; nc_ < m; ++nc_) {
+      const double* bx = cells + 3 * nc_;
+      int central = (bx[0] == 0.0 && bx[1] == 0.0 && bx[2] == 0.0);
+      for (int ja = 0; ja < n; ++ja) {
+        if (o->lefg_it_contrib && it[ja] != o->it_efg - 1) continue;
+        if (central && ja == ia) continue;
+        double sx = si[0] - (s[3 * ja] + bx[0]);
+        double sy = si[1] - (s[3 * ja + 1] + bx[1]);
+        double sz = si[2] - (s[3 * ja + 2] + bx[2]);
+        double x = sx * A[0] + sy * A[3] + sz * A[6];
+        double y = sx * A[1] + sy * A[4] + sz * A[7];
+        double z = sx * A[2] + sy * A[5] + sz * A[8];
+        double d2 = x * x + y * y + z * z;
+        if (!(d2 < cutsq && d2 != 0.0)) continue;
+        double qj = q[ja];
+        double mx = mu[ja], my = mu[ja + n], mz = mu[ja + 2 * n];
+        double d = sqrt(d2);
+        double dm5 = 1.0 / (d2 * d2 * d);
+        double dm7 = dm5 / d2 * 3.0;
+        double Txx = (3.0 * x * x - d2) * dm5, Tyy = (3.0 * y * y - d2) * dm5,
+               Tzz = (3.0 * z * z - d2) * dm5;
+        double Txy = (3.0 * x * y) * dm5, Txz = (3.0 * x * z) * dm5, Tyz = (3.0 * y * z) * dm5;
+        double Txxx = (5.0 * x * x * x - 3.0 * d2 * x) * dm7;
+        double Tyyy = (5.0 * y * y * y - 3.0 * d2 * y) * dm7;
+        double Tzzz = (5.0 * z * z * z - 3.0 * d2 * z) * dm7;
+        double Txxy = (5.0 * x * x * y - d2 * y) * dm7;
+        double Txxz = (5.0 * x * x * z - d2 * z) * dm7;
+        double Tyyx = (5.0 * y * y * x - d2 * x) * dm7;
+        double Tyyz = (5.0 * y * y * z - d2 * z) * dm7;
+        double Tzzx = (5.0 * z * z * x - d2 * x) * dm7;
+        double Tzzy = (5.0 * z * z * y - d2 * y) * dm7;
+        double Txyz = (5.0 * x * y * z) * dm7;
+        EFG(ia, 0, 0) = EFG(ia, 0, 0) - qj * Txx;
+        EFG(ia, 1, 1) = EFG(ia, 1, 1) - qj * Tyy;
+        EFG(ia, 2, 2) = EFG(ia, 2, 2) - qj * Tzz;
+        EFG(ia, 0, 1) = EFG(ia, 0, 1) - qj * Txy;
+        EFG(ia, 0, 2) = EFG(ia, 0, 2) - qj * Txz;
+        EFG(ia, 1, 2) = EFG(ia, 1, 2) - qj * Tyz;
+        EFG(ia, 0, 0) = EFG(ia, 0, 0) - (Txxx * mx + Txxy * my + Txxz * mz);
+        EFG(ia, 1, 1) = EFG(ia, 1, 1) - (Tyyx * mx + Tyyy * my + Tyyz * mz);
+        EFG(ia, 2, 2) = EFG(ia, 2, 2) - (Tzzx * mx + Tzzy * my + Tzzz * mz);
+        EFG(ia, 0, 1) = EFG(ia, 0, 1) - (Txxy * mx + Tyyx * my + Txyz * mz);
+        EFG(ia, 0, 2) = EFG(ia, 0, 2) - (Txxz * mx + Txyz * my + Tzzx * mz);
+        EFG(ia, 1, 2) = EFG(ia, 1, 2) - (Txyz * mx + Tyyz * my + Tzzy * mz);
+      }
+    }
+  }
+#undef EFG
+  free(cells);
+  return 0;
+}
+
+/* multipole_efg_ES (src/efg.f90:1278-1625) without the final symmetrisation,
+   correction and units: efg (natm,3,3) upper six = dir + rec + self. */
+static int efg_es(const es_ctx* e, const mdffgpu_efg_opts* o, const double* rx,
+                  const double* ry, const double* rz, const double* q, const double* mu,
+                  double* efg) {
+  const mdffgpu_params* p = e->p;
+  const int* it = e->it;
+  int n = p->natm;
+  const int itc = o->lefg_it_contrib ? o->it_efg - 1 : -1;
+  double alpha = p->alphaES, alpha2 = alpha * alpha, alpha3 = alpha2 * alpha,
+         alpha5 = alpha3 * alpha2;
+  double piroot = sqrt(PI);
+  double selfa = -4.0 * alpha3 / 3.0 / piroot;
+  double fpi_V = 4.0 * PI / e->cell.omega;
+  double* rec = (double*)calloc(6 * (size_t)n + 2 * (size_t)e->nk, sizeof(double));
+  if (!rec) return MDFFGPU_ERR_ALLOC;
+  double* rhoR = rec + 6 * (size_t)n;
+  double* rhoI = rhoR + e->nk;
+  /* charge_density_k (src/efg.f90:2888-2930) */
+  for (int ik = 0; ik < e->nk; ++ik) {
+    double kx = e->kx[ik], ky = e->ky[ik], kz = e->kz[ik];
+    double sr = 0.0, si = 0.0;
+    for (int ia = 0; ia < n; ++ia) {
+      double kr = kx * rx[ia] + ky * ry[ia] + kz * rz[ia];
+      double kmu = mu[ia] * kx + mu[ia + n] * ky + mu[ia + 2 * n] * kz;
+      double c = cos(kr), s = sin(kr);
+      if (itc >= 0 && it[ia] != itc) continue;
+      /* (q + i k.mu)(c + i s) */
+      sr = sr + (q[ia] * c - kmu * s);
+      si = si + (q[ia] * s + kmu * c);
+    }
+    rhoR[ik] = sr;
+    rhoI[ik] = si;
+  }
+  /* direct space, every pair under the minimum image (:1401-1499) */
+#define EFG(i, a, b) efg[(i) + n * ((a) + 3 * (b))]
+  for (int ia = 0; ia < n; ++ia) {
+    for (int ja = 0; ja < n; ++ja) {
+      if (itc >= 0 && it[ja] != itc) continue;
+      if (ja == ia) continue;
+      double r[3];
+      double d2 = min_image(&e->cell, e->s + 3 * ia, e->s + 3 * ja, r);
+      double x = r[0], y = r[1], z = r[2];
+      double qj = q[ja], mx = mu[ja], my = mu[ja + n], mz = mu[ja + 2 * n];
+      double d = sqrt(d2), d3 = d2 * d, d5 = d3 * d2;
+      double dm5 = 1.0 / d2 / d3;
+      double dm7 = dm5 / d2 * 3.0;
+      double expon = exp(-alpha2 * d2) / piroot;
+      double F1 = orc_errfc(alpha * d) + 2.0 * alpha * d * expon;
+      double F2 = F1 + 4.0 * alpha3 * d3 * expon / 3.0;
+      double F3 = F2 + 8.0 * alpha5 * d5 * expon / 15.0;
+      double Txx = (3.0 * x * x * F2 - d2 * F1) * dm5;
+      double Tyy = (3.0 * y * y * F2 - d2 * F1) * dm5;
+      double Tzz = (3.0 * z * z * F2 - d2 * F1) * dm5;
+      double Txy = (3.0 * x * y * F2) * dm5;
+      double Txz = (3.0 * x * z * F2) * dm5;
+      double Tyz = (3.0 * y * z * F2) * dm5;
+      double Txxx = (-5.0 * x * x * x * F3 + 3.0 * d2 * (x)*F2) * dm7;
+      double Tyyy = (-5.0 * y * y * y * F3 + 3.0 * d2 * (y)*F2) * dm7;
+      double Tzzz = (-5.0 * z * z * z * F3 + 3.0 * d2 * (z)*F2) * dm7;
+      double Txxy = (-5.0 * x * x * y * F3 + d2 * (y)*F2) * dm7;
+      double Txxz = (-5.0 * x * x * z * F3 + d2 * (z)*F2) * dm7;
+      double Tyyx = (-5.0 * y * y * x * F3 + d2 * (x)*F2) * dm7;
+      double Tyyz = (-5.0 * y * y * z * F3 + d2 * (z)*F2) * dm7;
+      double Tzzx = (-5.0 * z * z * x * F3 + d2 * (x)*F2) * dm7;
+      double Tzzy = (-5.0 * z * z * y * F3 + d2 * (y)*F2) * dm7;
+      double Txyz = (-5.0 * x * y * z * F3) * dm7;
+      EFG(ia, 0, 0) = EFG(ia, 0, 0) - qj * Txx;
+      EFG(ia, 1, 1) = EFG(ia, 1, 1) - qj * Tyy;
+      EFG(ia, 2, 2) = EFG(ia, 2, 2) - qj * Tzz;
+      EFG(ia, 0, 1) = EFG(ia, 0, 1) - qj * Txy;
+      EFG(ia, 0, 2) = EFG(ia, 0, 2) - qj * Txz;
+      EFG(ia, 1, 2) = EFG(ia, 1, 2) - qj * Tyz;
+      EFG(ia, 0, 0) = EFG(ia, 0, 0) + (Txxx * mx + Txxy * my + Txxz * mz);
+      EFG(ia, 1, 1) = EFG(ia, 1, 1) + (Tyyx * mx + Tyyy * my + Tyyz * mz);
+      EFG(ia, 2, 2) = EFG(ia, 2, 2) + (Tzzx * mx + Tzzy * my + Tzzz * mz);
+      EFG(ia, 0, 1) = EFG(ia, 0, 1) + (Txxy * mx + Tyyx * my + Txyz * mz);
+      EFG(ia, 0, 2) = EFG(ia, 0, 2) + (Txxz * mx + Txyz * my + Tzzx * mz);
+      EFG(ia, 1, 2) = EFG(ia, 1, 2) + (Txyz * mx + Tyyz * my + Tzzy * mz);
+    }
+  }
+  /* reciprocal space (:1507-1556): recarg = Re(conj(rho) e^{ikr}) A_k */
+  for (int ik = 0; ik < e->nk; ++ik) {
+    if (e->kk[ik] == 0.0) continue;
+    double kx = e->kx[ik], ky = e->ky[ik], kz = e->kz[ik];
+    for (int ia = 0; ia < n; ++ia) {
+      double kr = rx[ia] * kx + ry[ia] * ky + rz[ia] * kz;
+      double c = cos(kr), s = sin(kr);
+      double recarg = (rhoR[ik] * c + rhoI[ik] * s) * e->Ak[ik];
+      rec[ia] += kx * kx * recarg;
+      rec[ia + n] += ky * ky * recarg;
+      rec[ia + 2 * n] += kz * kz * recarg;
+      rec[ia + 3 * n] += kx * ky * recarg;
+      rec[ia + 4 * n] += kx * kz * recarg;
+      rec[ia + 5 * n] += ky * kz * recarg;
+    }
+  }
+  for (int ia = 0; ia < n; ++ia) {
+    double f = fpi_V * 2.0;
+    double self = (itc >= 0 && it[ia] != itc) ? 0.0 : selfa * q[ia];
+    EFG(ia, 0, 0) = EFG(ia, 0, 0) + rec[ia] * f + self;
+    EFG(ia, 1, 1) = EFG(ia, 1, 1) + rec[ia + n] * f + self;
+    EFG(ia, 2, 2) = EFG(ia, 2, 2) + rec[ia + 2 * n] * f + self;
+    EFG(ia, 0, 1) = EFG(ia, 0, 1) + rec[ia + 3 * n] * f;
+    EFG(ia, 0, 2) = EFG(ia, 0, 2) + rec[ia + 4 * n] * f;
+    EFG(ia, 1, 2) = EFG(ia, 1, 2) + rec[ia + 5 * n] * f;
+  }
+#undef EFG
+  free(rec);
+  return 0;
+}
+
+int orc_efgcalc(const mdffgpu_params* p, const mdffgpu_efg_opts* o, const int* itype,
+                const double* A, const double* rx0, const double* ry0, const double* rz0,
+                double* mu_t, double* efg_t, int* n_scf, double* rmsd) {
+  int n = p->natm;
+  if (n <= 0 || p->ntype < 1 || p->ntype > NT) return MDFFGPU_ERR_ARG;
+  if (o->lefg_old) return MDFFGPU_ERR_UNSUPPORTED;
+  if (o->longrange != 0 && o->longrange != 1) return MDFFGPU_ERR_ARG;
+  if (o->longrange == 1 && o->ncelldirect < 0) return MDFFGPU_ERR_ARG;
+  int linduced = 0;
+  for (int t = 0; t < p->ntype; ++t)
+    if (p->ldip_polar[t]) linduced = 1;
+  if (linduced && o->longrange == 1) return MDFFGPU_ERR_UNSUPPORTED;
+  size_t n3 = 3 * (size_t)n, n9 = 9 * (size_t)n;
+  int* it = (int*)malloc((size_t)n * sizeof(int));
+  double* buf = (double*)calloc(3 * (size_t)n + 4 * n3 + (size_t)n + n9 + n3, sizeof(double));
+  if (!it || !buf) {
+    free(it);
+    free(buf);
+    return MDFFGPU_ERR_ALLOC;
+  }
+  for (int i = 0; i < n; ++i) it[i] = itype[i] - 1;
+  double *rw = buf, *q = rw + n3, *dip = q + n, *muind = dip + n3, *mu = muind + n3,
+         *f = mu + n3, *efg = f + n3, *hist = efg + n9;
+  /* kardir, periodicbc, dirkar (src/efg.f90:424-426) */
+  orc_cell cell;
+  orc_lattice(A, &cell);
+  for (int i = 0; i < n; ++i) {
+    double r3[3] = {rx0[i], ry0[i], rz0[i]}, s[3];
+    for (int a = 0; a < 3; ++a)
+      s[a] = r3[0] * cell.B[3 * a] + r3[1] * cell.B[3 * a + 1] + r3[2] * cell.B[3 * a + 2];
+    for (int a = 0; a < 3; ++a) s[a] = s[a] - round(s[a]);
+    for (int a = 0; a < 3; ++a)
+      rw[i + n * a] = s[0] * cell.A[a] + s[1] * cell.A[3 + a] + s[2] * cell.A[6 + a];
+  }
+  const double *rx = rw, *ry = rw + n, *rz = rw + 2 * n;
+  for (int i = 0; i < n; ++i) {
+    q[i] = p->qch[it[i]];
+    for (int a = 0; a < 3; ++a) dip[i + n * a] = p->dip[it[i] + NT * a];
+  }
+  es_ctx e;
+  int rc = es_init(&e, p, it, A, rx, ry, rz);
+  *n_scf = 0;
+  *rmsd = 0.0;
+  if (!rc && linduced) {
+    /* get_moments with calc = 'efg': SCF from the static field */
+    mdffgpu_params pe = *p;
+    pe.calc_mode = 0;
+    e.p = &pe;
+    orc_state st;
+    st.hist = hist;
+    st.u_pol = 0.0;
+    int n_es = 0;
+    rc = scf(&e, rx, ry, rz, q, dip, 0, &st, muind, f, f + n, f + 2 * n, n_scf, rmsd, &n_es);
+    e.p = p;
+  }
+  if (!rc) {
+    for (size_t k = 0; k < n3; ++k) mu[k] = dip[k] + muind[k];
+    if (o->longrange == 0) {
+      rc = efg_es(&e, o, rx, ry, rz, q, mu, efg);
+    } else {
+      double* s = (double*)malloc(n3 * sizeof(double));
+      if (!s) {
+        rc = MDFFGPU_ERR_ALLOC;
+      } else {
+        for (int i = 0; i < n; ++i)
+          for (int a = 0; a < 3; ++a) s[3 * i + a] = e.s[3 * i + a];
+        rc = efg_ds(p, o, it, &cell, s, q, mu, efg);
+        free(s);
+      }
+    }
+  }
+  if (!rc) {
+    double corr = 0.0;
+    if (o->lmp_correction && o->longrange == 0) {
+      for (int ia = 0; ia < n; ++ia) {
+        if (o->lefg_it_contrib && it[ia] != o->it_efg - 1) continue;
+        corr = corr + q[ia];
+      }
+      corr = corr * (4.0 * PI / cell.omega) / 3.0;
+    }
+    double sign = o->lefg_vasp_sign ? -1.0 : 1.0;
+    for (int ia = 0; ia < n; ++ia)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          int lo = a < b ? a : b, hi = a < b ? b : a;
+          double v = efg[ia + n * (lo + 3 * hi)];
+          if (a == b) v = v + corr;
+          efg_t[ia + n * (a + 3 * b)] = sign * (v * p->coul_unit);
+        }
+    if (mu_t) memcpy(mu_t, mu, n3 * sizeof(double));
+  }
+  es_free(&e);
+  free(it);
+  free(buf);
+  return rc;
+}

@@ -70,6 +70,18 @@ int orc_extrapolate(const mdffgpu_params* p, int itime, double* hist,
 void orc_aspc_correct(const mdffgpu_params* p, const int* itype, int itime,
                       const double* E, double* mu, double* u_pol);
 
+/* The per-configuration body of efgcalc (calc = 'efg', src/efg.f90:
+   325-616): positions wrapped (:424-426), get_moments with an SCF from the
+   static field when polarizable (src/field.f90:3686-3765), then
+   multipole_efg_ES (src/efg.f90:1278-1625) or multipole_efg_DS
+   (:1080-1268) as opts->longrange says, symmetrised, the charged-cell
+   correction (ewald), times coul_unit and the sign option.  efg_t out
+   (natm,3,3), mu_t out (natm,3) or NULL. */
+int orc_efgcalc(const mdffgpu_params* p, const mdffgpu_efg_opts* opts,
+                const int* itype, const double* A, const double* rx,
+                const double* ry, const double* rz, double* mu_t,
+                double* efg_t, int* n_scf, double* rmsd);
+
 /* engforce_driver, src/field.f90:1111-1166.  Arguments as
    mdffgpu_engforce; A is the cell.  Returns 0 or an mdffgpu_status. */
 int orc_engforce(const mdffgpu_params* p, const int* itype, const double* A,

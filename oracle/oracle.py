@@ -12,7 +12,7 @@ import subprocess
 
 import numpy as np
 
-from mdff_amd.params import MdffEnergies, MdffParams
+from mdff_amd.params import MdffEfgOpts, MdffEnergies, MdffParams
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
@@ -62,6 +62,9 @@ def lib():
         L.orc_aspc_correct.restype = None
         L.orc_aspc_correct.argtypes = [ctypes.POINTER(MdffParams), _ip, ctypes.c_int, _dp, _dp,
                                        _dp]
+        L.orc_efgcalc.restype = ctypes.c_int
+        L.orc_efgcalc.argtypes = [ctypes.POINTER(MdffParams), ctypes.POINTER(MdffEfgOpts), _ip,
+                                  _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp]
         _lib = L
     return _lib
 
@@ -196,3 +199,56 @@ def multipole_es(params: MdffParams, itype, A, r, q, mu, task, damp_ind=True,
             "ef": ef.reshape(3, n).T.copy(), "efg": efg.reshape(3, 3, n).transpose(2, 1, 0).copy(),
             "f": f.T.copy(), "u_dir": parts[0], "u_rec": parts[1], "u_self": parts[2],
             "u_surf": parts[3]}
+
+
+def efgcalc(params: MdffParams, opts: MdffEfgOpts, itype, A, r):
+    """One configuration of the standalone EFG run (efgcalc,
+    src/efg.f90:325-616).  Returns dict efg (natm,3,3), mu (natm,3),
+    n_scf, rmsd."""
+    n = params.natm
+    r = np.asarray(r, dtype=np.float64)
+    rx, ry, rz = (np.ascontiguousarray(r[:, k]) for k in range(3))
+    it = np.ascontiguousarray(itype, dtype=np.int32)
+    Af = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(3, 3).reshape(-1, order="F"))
+    efg = np.zeros(9 * n)
+    mu = np.zeros(3 * n)
+    ns = ctypes.c_int32()
+    rm = ctypes.c_double()
+    rc = lib().orc_efgcalc(ctypes.byref(params), ctypes.byref(opts), it.ctypes.data_as(_ip),
+                           _p(Af), _p(rx), _p(ry), _p(rz), _p(mu), _p(efg), ctypes.byref(ns),
+                           ctypes.byref(rm))
+    if rc != 0:
+        raise RuntimeError(f"oracle efgcalc failed with status {rc}")
+    return {"efg": efg.reshape(3, 3, n).transpose(2, 1, 0).copy(),
+            "mu": mu.reshape(3, n).T.copy(), "n_scf": ns.value, "rmsd": rm.value}
+
+
+def nmr_convention(w):
+    """nmr_convention (src/efg.f90:2546-2648) on eigenvalues w in the
+    ascending order DSYEV returns them: (Vxx, Vyy, Vzz, eta) and whether the
+    ordering checks that stop the reference (:2618-2629) pass."""
+    w = [float(x) for x in w]
+    diso = (w[0] + w[1] + w[2]) / 3.0
+    if abs(diso) != 0.0 and abs(diso) > 1e-3:
+        diso = 0.0
+    a = [abs(x - diso) for x in w]
+    if a[0] >= a[1] and a[0] >= a[2]:
+        d11 = w[0]
+        d22, d33 = (w[1], w[2]) if a[1] >= a[2] else (w[2], w[1])
+    elif a[1] >= a[0] and a[1] >= a[2]:
+        d11 = w[1]
+        d22, d33 = (w[0], w[2]) if a[0] >= a[2] else (w[2], w[0])
+    else:
+        d11 = w[2]
+        d22, d33 = (w[0], w[1]) if a[0] >= a[1] else (w[1], w[0])
+    xx, yy, zz = d33, d22, d11
+    eta = (xx - yy) / zz if zz != 0.0 else 0.0
+    if xx == yy:
+        eta = 0.0
+    if zz == 0.0:
+        eta = 0.0
+    if abs(xx) < 1.0e-5 and abs(yy) < 1.0e-5 and abs(zz) < 1.0e-5:
+        eta = 0.0
+    ok = not (abs(zz - diso) < abs(xx - diso) or abs(zz - diso) < abs(yy - diso)
+              or abs(yy - diso) < abs(xx - diso))
+    return (xx, yy, zz, eta), ok

@@ -9,7 +9,7 @@ fixtures are data and travel with the repository).
                      Database energies of examples/ex2/REFERENCE
   ex4_efg.json       examples/ex4/TRAJFF.twocharges, TRAJFF.cluster, the
                      Nymand-Linse values (run_ex4.sh:78-79) and the GULP
-                     principal EFGs (run_ex4.sh:159-170)
+                     principal EFGs and asymmetries (run_ex4.sh:161-169)
   ex9_pim.npz/json   examples/ex9/config/POSFF and pot_mdff/*.POT (test 11:
                      BMHFTD + damped PIM + Ewald), with the reference's
                      Utot and SCF iteration count from a run of mdff.x on
@@ -67,13 +67,14 @@ def ex4():
             [8.6231, 10.8748, -19.4979], [22.1626, 24.1320, -46.2946],
             [7.7041, 9.9389, -17.6430], [16.7300, 17.0914, -33.8214],
             [-0.8228, -3.4544, 4.2772]]
+    gulp_eta = [0.0391, 0.0834, 0.0685, 0.0969, 0.1155, 0.0425, 0.1267, 0.0107, 0.6153]
     json.dump({"twocharges": _traj(f"{REF}/ex4/TRAJFF.twocharges"),
                "twocharges_expected": {"V1xx": 2.0003749, "V1yy": -1.0001874, "alpha": 0.8,
                                        "kmax": 11, "cutlongrange": 1000.0},
                "cluster": _traj(f"{REF}/ex4/TRAJFF.cluster"),
                "cluster_charges": [0.1, -0.8],
                "cluster_ewald": {"cutlongrange": 5.0, "epsw": 1e-8},
-               "gulp_principal": gulp}, open(f"{OUT}/ex4_efg.json", "w"), indent=1)
+               "gulp_principal": gulp, "gulp_eta": gulp_eta}, open(f"{OUT}/ex4_efg.json", "w"), indent=1)
 
 
 def _pot(path):

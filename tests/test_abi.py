@@ -10,7 +10,7 @@ import tempfile
 import pytest
 
 from mdff_amd import _lib
-from mdff_amd.params import MdffBatchConfig, MdffEnergies, MdffParams
+from mdff_amd.params import MdffBatchConfig, MdffEfgOpts, MdffEnergies, MdffParams
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "mdffgpu.h")
@@ -57,7 +57,8 @@ def _c_offsets(struct, fields):
 
 @pytest.mark.parametrize("py,cname", [(MdffParams, "mdffgpu_params"),
                                        (MdffEnergies, "mdffgpu_energies"),
-                                       (MdffBatchConfig, "mdffgpu_batch_config")])
+                                       (MdffBatchConfig, "mdffgpu_batch_config"),
+                                       (MdffEfgOpts, "mdffgpu_efg_opts")])
 def test_struct_layout_matches_header(py, cname):
     names = [f[0] for f in py._fields_]
     size, offs = _c_offsets(cname, names)
