@@ -232,6 +232,70 @@ int mdffgpu_engforce_device(mdffgpu_ctx* ctx, const double* d_pos,
                             mdffgpu_energies* en, void* stream);
 
 /* ---------------------------------------------------------------------
+ * Device-resident molecular dynamics: the time loop of md_run
+ * (src/runmd.f90:233-330) for the integrators below, with positions,
+ * velocities and forces kept in HBM between steps; only scalars (energies,
+ * kinetic energy, thermostat state) come back to the host each step.
+ * Each step is the reference's integrator subroutine with its one
+ * engforce_driver call, and the Verlet-list check of the evaluation.
+ * ------------------------------------------------------------------- */
+typedef enum mdffgpu_integrator {
+  MDFFGPU_NVE_VV = 0,   /* 'nve-vv': prop_velocity_verlet
+                           (src/integration.f90:131-194) */
+  MDFFGPU_NVT_NHC2 = 1, /* 'nvt-nhc2': nhc2 + chain_nhc2 (:205-324) */
+  MDFFGPU_NVT_NHCN = 2  /* 'nvt-nhcn': nhcn + chain_nhcn (:476-713), the
+                           trotter_order -1 branch nhcn calls */
+} mdffgpu_integrator;
+
+#define MDFFGPU_NHC_MAX 16
+
+/* &mdtag after md_check_tag's unit conversions (src/md.f90:316-321). */
+typedef struct mdffgpu_md_params {
+  int32_t integrator;     /* mdffgpu_integrator */
+  int32_t nhc_n;          /* chain length, 2..16 (src/md.f90:302-305; nhc2
+                             uses 2 whatever this says) */
+  int32_t nhc_yosh_order; /* 1, 3, 5, 7 or 9 (src/integration.f90:596-632) */
+  int32_t nhc_mults;      /* >= 1 */
+  double dt;              /* dt * time_unit */
+  double temp;            /* temp * boltz_unit, i.e. kB T */
+  double timesca_thermo;  /* timesca_thermo * time_unit */
+  double boltz_unit;      /* for temp_r (src/kinetic.f90:531-556) */
+  double mass[MDFFGPU_NTYPEMAX]; /* per species (massia, src/read_pos.f90) */
+  int32_t reserved[4];
+} mdffgpu_md_params;
+
+/* Scalars of one MD step, after it (the thermodynamic module variables the
+   integrators write, src/integration.f90:185-189, 532-541). */
+typedef struct mdffgpu_md_step {
+  mdffgpu_energies en; /* the step's engforce_driver outputs */
+  double e_kin;        /* kinetic energy, 1/2 sum m v^2 */
+  double temp_r;       /* 2 e_kin / (3 natm boltz_unit) */
+  double e_nvt;        /* thermostat energy (nhc2 :237-243, nhcn :524-541);
+                          0 for nve-vv */
+} mdffgpu_md_step;
+
+/* nsteps steps from step itime0 (the step index passed to each
+ * engforce_driver call, as md_run's itime).
+ *   d_pos    in/out  device SoA [x|y|z] positions
+ *   d_vel    in/out  device SoA velocities
+ *   d_force  in/out  forces at d_pos on entry (md_run evaluates them at
+ *                    t = 0, src/runmd.f90:162-168), at the final d_pos on
+ *                    return; passed through each evaluation with the
+ *                    reference's overwrite/accumulate semantics
+ *   xi, vxi  in/out  host thermostat coordinates (nhc_n entries; 2 for
+ *                    nhc2), NULL for nve-vv
+ *   steps    out     host, nsteps records, may be NULL
+ * Returns after the last step's outputs are written. */
+int mdffgpu_md_run(mdffgpu_ctx* ctx, const mdffgpu_md_params* mp, int32_t itime0,
+                   int32_t nsteps, double* d_pos, double* d_vel, double* d_force,
+                   double* xi, double* vxi, mdffgpu_md_step* steps, void* stream);
+
+/* Neighbour-list statistics of the context: list builds so far and
+   whether the last one used the cell-grid search (1) or the all-pairs scan
+   (0, cells too small for the stencil). */
+int mdffgpu_list_info(const mdffgpu_ctx* ctx, int32_t* builds, int32_t* cell_search);
+
+/* ---------------------------------------------------------------------
  * Allocation-free evaluation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code that must not call
  * hipMalloc, e.g. tight serving loops or harnesses that forbid it).  The

@@ -10,7 +10,7 @@ import ctypes
 import importlib.util
 import os
 
-from .params import MdffBatchConfig, MdffEfgOpts, MdffEnergies, MdffParams
+from .params import MdffBatchConfig, MdffEfgOpts, MdffEnergies, MdffMdParams, MdffMdStep, MdffParams
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmdffgpu.so")
 
@@ -30,6 +30,10 @@ PROTOTYPES = {
     "mdffgpu_engforce_noalloc": (ctypes.c_int, [ctypes.POINTER(MdffParams), _ip, _dp, _vp,
                                                  ctypes.c_int32, _vp, _vp, _vp, _vp, _vp,
                                                  ctypes.POINTER(MdffEnergies), _vp]),
+    "mdffgpu_md_run": (ctypes.c_int, [_vp, ctypes.POINTER(MdffMdParams), ctypes.c_int32,
+                                      ctypes.c_int32, _vp, _vp, _vp, _dp, _dp,
+                                      ctypes.POINTER(MdffMdStep), _vp]),
+    "mdffgpu_list_info": (ctypes.c_int, [_vp, _ip, _ip]),
     "mdffgpu_efg": (ctypes.c_int, [_vp, ctypes.POINTER(MdffEfgOpts), _dp, _dp, _dp, _dp, _dp,
                                    ctypes.POINTER(MdffEnergies)]),
     "mdffgpu_efg_nmr": (ctypes.c_int, [ctypes.c_int32, _dp, _dp, _dp, _dp, _ip]),

@@ -198,6 +198,27 @@ def silica(n_si: int, n_o: int, seed: int, rc: float = 10.0, kmax: int = 10,
                   {"L": L, "seed": seed, "pot": pot, "alpha": p.alphaES, "kmax": kmax})
 
 
+def species_masses(sys_: System) -> tuple:
+    """Masses per species: 1 for the reduced-unit LJ fluid, Si and O (amu)
+    for the silica-like melts."""
+    return (1.0,) if sys_.params.ntype == 1 else (MASS_SI, MASS_O)
+
+
+def maxwell_velocities(sys_: System, temp: float, seed: int, lreduced=None) -> np.ndarray:
+    """Seeded Maxwell-Boltzmann velocities (natm, 3) at temp (K, or reduced
+    with lreduced) with zero net momentum, rescaled to exactly temp as
+    calc_temp measures it (src/kinetic.f90:531-556)."""
+    if lreduced is None:
+        lreduced = sys_.params.ntype == 1
+    kb = 1.0 if lreduced else P.BOLTZ_UNIT
+    m = np.asarray(species_masses(sys_))[sys_.itype - 1]
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((sys_.natm, 3)) * np.sqrt(kb * temp / m)[:, None]
+    v -= (m[:, None] * v).sum(axis=0) / m.sum()
+    t_now = (m[:, None] * v * v).sum() / (3.0 * sys_.natm * kb)
+    return v * math.sqrt(temp / t_now)
+
+
 def config(k: int, precision: int = P.PREC_FP64) -> System:
     """The five workloads (1-based)."""
     if k == 1:

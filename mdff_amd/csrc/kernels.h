@@ -25,7 +25,7 @@ enum KernelId {
   K_RHO_BOTH, K_RHO_RED, K_APPLY_F, K_APPLY_EFG, K_MOMENTS, K_COMBINE, K_INDUCED,
   K_RESID, K_EXTRAP, K_REDUCE, K_ADD, K_PAIRS_T2, K_DD_FIELD, K_MX_SR, K_MX_DIR, K_MX_PHASE, K_MX_RHO,
   K_MX_RHO_RED, K_MX_APPLY, K_MX_COMBINE, K_MX_WEIGHTS, K_MX_GPREP, K_BATCH, K_ADJ,
-  K_EFG_DIR, K_EFG_DS, K_EFG_COMBINE, K_COUNT
+  K_EFG_DIR, K_EFG_DS, K_EFG_COMBINE, K_MD, K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
 
@@ -188,6 +188,61 @@ __global__ void k_nlist(int n, const double* s, Cell cell, double cut2, int maxn
                         int* scnt, int* nbr);
 __global__ void k_nlist_offsets(int n, int nsp, int maxnb, int* scnt, int* cnt, int* overflow);
 constexpr int NLIST_RANGES = 16;  // candidate ranges of the list build (grid.y)
+// O(N) list build over a cell grid (nlist.hip)
+struct NlCellArgs {
+  int n, maxnb;
+  int nc[3];  // cells along each lattice vector
+  int m[3];   // stencil half-widths, 2 m + 1 <= nc, (2 m0 + 1)(2 m1 + 1) <= 64
+  const double4* s4;
+  const int* cellof;
+  const int* cstart;  // first sorted slot of each cell (ncell + 1)
+  const int* perm;    // atoms sorted by cell
+  Cell cell;
+  double cut2;
+  int* cnt;
+  int* nbr;
+  int* overflow;
+};
+__global__ void k_cell_zero(int m, int* cnt);
+__global__ void k_cell_bin(int n, const double4* s4, int nc0, int nc1, int nc2, int* cellof,
+                           int* cnt);
+__global__ void k_cell_scan(int m, int* cnt, int* cur);
+__global__ void k_cell_fill(int n, const int* cellof, int* cur, int* perm);
+__global__ void k_cell_order(int m, const int* cnt, int* perm, int* rank);
+__global__ void k_nlist_cells(NlCellArgs a);
+
+// device-resident MD (md.hip)
+constexpr int MDFF_NHC_MAX = 16;
+struct MdArgs {
+  int n;
+  const double* mass;  // per atom
+  double* pos;
+  double* vel;
+  const double* f;
+  double* fs;  // forces of the previous step (velocity Verlet)
+  double dt, dt2, dtsq2;
+  double* part;  // block partials of sum m |v|^2
+};
+// Thermostat state of the Nose-Hoover chains on the device.
+struct NhcState {
+  double vxi[MDFF_NHC_MAX], xi[MDFF_NHC_MAX], Q[MDFF_NHC_MAX];
+  double yw[9];  // Yoshida weights
+  double s;      // scale factor of the last chain call
+  double e_nvt;
+};
+struct NhcMass {
+  double m[MDFF_NT];
+};
+__global__ void k_vv_pos(MdArgs a);
+__global__ void k_vv_vel(MdArgs a);
+__global__ void k_kick_drift(MdArgs a);
+__global__ void k_kick(MdArgs a);
+__global__ void k_vscale(MdArgs a, const NhcState* s);
+__global__ void k_massia(int n, const int* typ, NhcMass m, double* mass);
+__global__ void k_nhcn_chain(NhcState* st, const double* sum_mv2, int nhc_n, int yosh, int mults,
+                             double dt, double temp, double L, int final);
+__global__ void k_nhc2_chain(NhcState* st, const double* sum_mv2, double dt, double temp,
+                             double L, int final);
 template <typename R, bool BATCH = false>
 __global__ void k_shortrange(SrArgs a);
 template <int MODE, typename R, bool BATCH = false>

@@ -35,7 +35,8 @@ enum {
   SC_DISP = 37,  // max squared displacement since the list build
   SC_CG = 40,    // 6: conjugate-gradient SCF, two (r.z, |r|^2, npol) triples
   SC_ESTAT = 46, // 2: sum of |E_stat|^2 over polarizable atoms, their count
-  SC_N = 48
+  SC_KIN = 48,   // sum m |v|^2 of the MD loop
+  SC_N = 50
 };
 
 thread_local std::string g_create_error;
@@ -83,8 +84,17 @@ struct mdffgpu_ctx {
   // within a cell), refreshed with the neighbour list: perm[rank] = atom
   int* d_perm = nullptr;
   int* d_rank = nullptr;
-  std::vector<double> h_sfrac;
-  std::vector<int> h_perm, h_rank, h_cellof, h_cellcnt;
+  // cell grid of the list build (nlist.hip): cells per axis, stencil
+  // half-widths, whether the stencil search applies; cell of each atom,
+  // first sorted slot of each cell, fill cursors
+  int gnc[3] = {1, 1, 1}, gm[3] = {0, 0, 0};
+  bool gstencil = false;
+  int ncell_cap = 0;
+  int *d_cellof = nullptr, *d_cstart = nullptr, *d_ccur = nullptr;
+  // device-resident MD (mdffgpu_md_run): velocities, previous forces,
+  // per-atom masses, thermostat state
+  double *d_vel = nullptr, *d_fs = nullptr, *d_massia = nullptr;
+  NhcState* d_nhc = nullptr;
   double mx_floor = 0.0;     // SCF residual floor of the mixed path (0 = unknown)
   // outcome of the last mixed-precision solve, folded into mx_floor once the
   // evaluation's scalars are on the host (update_floor)
@@ -160,7 +170,7 @@ const char* const kKernelNames[K_COUNT] = {
     "k_scf_resid", "k_extrap", "k_reduce_rows", "k_add", "k_pairs_t2", "k_dd_field",
     "mx_shortrange", "mx_es_dir",
     "mx_phase", "mx_rho", "mx_rho_reduce", "mx_apply", "mx_combine", "mx_weights", "mx_gprep",
-    "batch", "adjoint", "k_efg_es_dir", "k_efg_ds", "k_efg_combine"};
+    "batch", "adjoint", "k_efg_es_dir", "k_efg_ds", "k_efg_combine", "md"};
 
 ProfScope::ProfScope(mdffgpu_ctx* ctx, int kid) : c(ctx), id(kid), a(nullptr), b(nullptr) {
   if (!c || !c->prof || (c->prof_only != -1 && c->prof_only != kid)) return;
@@ -444,6 +454,7 @@ void alloc_atoms(mdffgpu_ctx* c) {
   if (c->P.scf_solver == MDFFGPU_SCF_PCG) dalloc(c, &c->d_cg, 12 * n);
   dalloc(c, &c->d_perm, n);
   dalloc(c, &c->d_rank, n);
+  dalloc(c, &c->d_cellof, n);
   dalloc(c, &c->d_mup, 3 * n);
   dalloc(c, &c->d_Estat, 3 * n);
   dalloc(c, &c->d_E, 3 * n);
@@ -1113,50 +1124,43 @@ void sym9(const double* t6, double scale, double* out) {
   out[5] = out[7] = t6[5] * scale;
 }
 
-// Spatial order of the atoms for the neighbour list: cells of about 4 A
-// (row-major over the fractional grid), atoms of a cell by index.  A stable
-// counting sort on the host from the fractional coordinates; it runs with
-// each list rebuild (rare under the Verlet skin).
-void spatial_order(mdffgpu_ctx* c) {
-  const int n = c->n;
-  c->h_sfrac.resize(3 * (size_t)n);
-  HCK(c, hipMemcpyAsync(c->h_sfrac.data(), c->d_s, 3 * (size_t)n * sizeof(double),
-                        hipMemcpyDeviceToHost, c->st));
-  HCK(c, hipStreamSynchronize(c->st));
-  int nc[3];
+// Cell grid of the list build: cells of at least 4 A and at least a third of
+// the list cutoff across (at most 64 per axis), so that the stencil
+// half-width m = ceil(lc nc / W) is at most 3 and its rows fit one wave.
+// The stencil search needs 2 m + 1 <= nc on every axis (no cell visited
+// twice); smaller cells fall back to the brute-force build.
+void grid_setup(mdffgpu_ctx* c) {
+  const double lc = c->list_cut + c->P.skindiff;
+  const double w = std::max(4.0, lc / 3.0);
+  bool ok = lc > 0.0;
   for (int a = 0; a < 3; ++a) {
-    const double len = std::sqrt(c->cell.A[3 * a] * c->cell.A[3 * a] +
-                                 c->cell.A[3 * a + 1] * c->cell.A[3 * a + 1] +
-                                 c->cell.A[3 * a + 2] * c->cell.A[3 * a + 2]);
-    nc[a] = std::max(1, std::min(64, (int)(len / 4.0)));
+    c->gnc[a] = std::max(1, std::min(64, (int)(c->W[a] / w)));
+    c->gm[a] = (int)std::ceil(lc * c->gnc[a] / c->W[a]);
+    ok = ok && (2 * c->gm[a] + 1 <= c->gnc[a]);
   }
-  const int ncell = nc[0] * nc[1] * nc[2];
-  c->h_cellof.resize(n);
-  c->h_cellcnt.assign((size_t)ncell + 1, 0);
-  for (int i = 0; i < n; ++i) {
-    int ci[3];
-    for (int a = 0; a < 3; ++a) {
-      double f = c->h_sfrac[i + (size_t)a * n];
-      f -= std::floor(f);
-      ci[a] = std::min(nc[a] - 1, (int)(f * nc[a]));
-    }
-    const int cell = (ci[0] * nc[1] + ci[1]) * nc[2] + ci[2];
-    c->h_cellof[i] = cell;
-    ++c->h_cellcnt[cell + 1];
+  c->gstencil = ok && (2 * c->gm[0] + 1) * (2 * c->gm[1] + 1) <= 64;
+  const int ncell = c->gnc[0] * c->gnc[1] * c->gnc[2];
+  if (ncell > c->ncell_cap) {
+    dalloc(c, &c->d_cstart, (size_t)ncell + 1);
+    dalloc(c, &c->d_ccur, (size_t)ncell);
+    c->ncell_cap = ncell;
   }
-  for (int k = 0; k < ncell; ++k) c->h_cellcnt[k + 1] += c->h_cellcnt[k];
-  c->h_perm.resize(n);
-  c->h_rank.resize(n);
-  for (int i = 0; i < n; ++i) {
-    const int r = c->h_cellcnt[c->h_cellof[i]]++;
-    c->h_perm[r] = i;
-    c->h_rank[i] = r;
-  }
-  HCK(c, hipMemcpyAsync(c->d_perm, c->h_perm.data(), (size_t)n * sizeof(int),
-                        hipMemcpyHostToDevice, c->st));
-  HCK(c, hipMemcpyAsync(c->d_rank, c->h_rank.data(), (size_t)n * sizeof(int),
-                        hipMemcpyHostToDevice, c->st));
-  HCK(c, hipStreamSynchronize(c->st));  // the host vectors are reused
+}
+
+// Spatial order of the atoms on the device: atoms sorted by cell of the
+// list grid (row-major), by index within a cell; perm[rank] = atom.  The
+// pair kernels visit the atoms in this order.
+void cell_sort(mdffgpu_ctx* c) {
+  const int n = c->n;
+  const int ncell = c->gnc[0] * c->gnc[1] * c->gnc[2];
+  hipStream_t st = c->st;
+  k_cell_zero<<<nblk(ncell + 1), MDFF_BLOCK, 0, st>>>(ncell + 1, c->d_cstart);
+  k_cell_bin<<<nblk(n), MDFF_BLOCK, 0, st>>>(n, c->d_s4, c->gnc[0], c->gnc[1], c->gnc[2],
+                                             c->d_cellof, c->d_cstart);
+  k_cell_scan<<<1, 1024, 0, st>>>(ncell, c->d_cstart, c->d_ccur);
+  k_cell_fill<<<nblk(n), MDFF_BLOCK, 0, st>>>(n, c->d_cellof, c->d_ccur, c->d_perm);
+  k_cell_order<<<nblk(ncell), MDFF_BLOCK, 0, st>>>(ncell, c->d_cstart, c->d_perm, c->d_rank);
+  check_launch(c, "cell sort");
 }
 
 // Noise-floor estimate of the mixed SCF after a solve (scalars on the
@@ -1210,21 +1214,47 @@ void update_list(mdffgpu_ctx* c) {
                           hipMemcpyDeviceToDevice, st));
     c->list_valid = true;
     ++c->list_builds;
-    spatial_order(c);
+    {
+      ProfScope _ps(c, K_NLIST);
+      grid_setup(c);
+      cell_sort(c);
+    }
     for (;;) {
       HCK(c, hipMemsetAsync(c->d_ovf, 0, sizeof(int), st));
       {
         ProfScope _ps(c, K_NLIST);
         const double lc = c->list_cut + P.skindiff;
-        // candidate ranges of whole tiles
-        const int jlen = (n + NLIST_RANGES * MDFF_BLOCK - 1) / (NLIST_RANGES * MDFF_BLOCK) * MDFF_BLOCK;
-        const dim3 gl(gb, (n + jlen - 1) / jlen);
-        k_nlist<false><<<gl, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->cell, lc * lc, c->maxnb, jlen,
-                                                  c->d_nlr, c->d_nbr);
-        k_nlist_offsets<<<gb, MDFF_BLOCK, 0, st>>>(n, (int)gl.y, c->maxnb, c->d_nlr, c->d_cnt,
-                                                   c->d_ovf);
-        k_nlist<true><<<gl, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->cell, lc * lc, c->maxnb, jlen,
-                                                 c->d_nlr, c->d_nbr);
+        if (c->gstencil) {
+          NlCellArgs na;
+          na.n = n;
+          na.maxnb = c->maxnb;
+          for (int a = 0; a < 3; ++a) {
+            na.nc[a] = c->gnc[a];
+            na.m[a] = c->gm[a];
+          }
+          na.s4 = c->d_s4;
+          na.cellof = c->d_cellof;
+          na.cstart = c->d_cstart;
+          na.perm = c->d_perm;
+          na.cell = c->cell;
+          na.cut2 = lc * lc;
+          na.cnt = c->d_cnt;
+          na.nbr = c->d_nbr;
+          na.overflow = c->d_ovf;
+          k_nlist_cells<<<(int)(((long long)n * 64 + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0,
+                          st>>>(na);
+        } else {
+          // candidate ranges of whole tiles
+          const int jlen =
+              (n + NLIST_RANGES * MDFF_BLOCK - 1) / (NLIST_RANGES * MDFF_BLOCK) * MDFF_BLOCK;
+          const dim3 gl(gb, (n + jlen - 1) / jlen);
+          k_nlist<false><<<gl, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->cell, lc * lc, c->maxnb, jlen,
+                                                    c->d_nlr, c->d_nbr);
+          k_nlist_offsets<<<gb, MDFF_BLOCK, 0, st>>>(n, (int)gl.y, c->maxnb, c->d_nlr, c->d_cnt,
+                                                     c->d_ovf);
+          k_nlist<true><<<gl, MDFF_BLOCK, 0, st>>>(n, c->d_s, c->cell, lc * lc, c->maxnb, jlen,
+                                                   c->d_nlr, c->d_nbr);
+        }
       }
       check_launch(c, "k_nlist");
       HCK(c, hipMemcpyAsync(c->h_ovf, c->d_ovf, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1937,7 +1967,8 @@ void mdffgpu_destroy(mdffgpu_ctx* c) {
   if (c->d_t2) (void)hipFree(c->d_t2);
   void* vp[] = {c->d_eph, c->d_typ, c->d_cnt, c->d_nbr, c->d_ovf, c->d_rpart, c->d_g,
                 c->d_chunks, c->d_pt, c->d_tt, c->d_mu4, c->d_s4, c->d_perm, c->d_rank,
-                c->d_nlr};
+                c->d_nlr, c->d_cellof, c->d_cstart, c->d_ccur, c->d_vel, c->d_fs,
+                c->d_massia, c->d_nhc};
   for (void* p : vp)
     if (p) (void)hipFree(p);
   for (auto& p : c->prof_pending) {
@@ -2025,6 +2056,247 @@ int mdffgpu_engforce_device(mdffgpu_ctx* c, const double* d_pos, int32_t itime,
     c->st = c->own_stream;
     return f.code;
   }
+}
+
+// Yoshida weights of chain_nhcn (src/integration.f90:596-632): the
+// published Yoshida/Suzuki factorisation weights of orders 1 to 9.
+static bool yoshida_weights(int order, double* w) {
+  switch (order) {
+    case 1:
+      w[0] = 1.0;
+      return true;
+    case 3:
+      w[0] = 1.0 / (2.0 - std::pow(2.0, 1.0 / 3.0));
+      w[1] = 1.0 - 2.0 * w[0];
+      w[2] = w[0];
+      return true;
+    case 5:
+      w[0] = 1.0 / (4.0 - std::pow(4.0, 1.0 / 3.0));
+      w[1] = w[2] = w[3] = w[0];
+      w[4] = 1.0 - 4.0 * w[0];
+      return true;
+    case 7:
+      w[0] = .78451361047756;
+      w[1] = .235573213359357;
+      w[2] = -1.17767998417887;
+      w[3] = 1.0 - 2.0 * (w[0] + w[1] + w[2]);
+      w[4] = w[2];
+      w[5] = w[1];
+      w[6] = w[0];
+      return true;
+    case 9:
+      w[0] = 0.192;
+      w[1] = 0.554910818409783619692725006662999;
+      w[2] = 0.124659619941888644216504240951585;
+      w[3] = -0.843182063596933505315033808282941;
+      w[4] = 1.0 - 2.0 * (w[0] + w[1] + w[2] + w[3]);
+      w[5] = w[3];
+      w[6] = w[2];
+      w[7] = w[1];
+      w[8] = w[0];
+      return true;
+    default:
+      return false;
+  }
+}
+
+// Kinetic sum m |v|^2 into d_sc[SC_KIN]: s = nullptr only sums, otherwise
+// scales the velocities by the chain's factor first.
+static void md_kin(mdffgpu_ctx* c, MdArgs& a, const NhcState* s) {
+  const int gb = nblk(c->n);
+  {
+    ProfScope _ps(c, K_MD);
+    k_vscale<<<gb, MDFF_BLOCK, 0, c->st>>>(a, s);
+    k_reduce_rows<<<1, MDFF_BLOCK, 0, c->st>>>(c->d_part, gb, c->d_sc + SC_KIN);
+  }
+  check_launch(c, "k_vscale");
+}
+
+static void md_chain(mdffgpu_ctx* c, const mdffgpu_md_params& mp, double L, int final) {
+  ProfScope _ps(c, K_MD);
+  if (mp.integrator == MDFFGPU_NVT_NHC2)
+    k_nhc2_chain<<<1, 64, 0, c->st>>>(c->d_nhc, c->d_sc + SC_KIN, mp.dt, mp.temp, L, final);
+  else
+    k_nhcn_chain<<<1, 64, 0, c->st>>>(c->d_nhc, c->d_sc + SC_KIN, mp.nhc_n, mp.nhc_yosh_order,
+                                      mp.nhc_mults, mp.dt, mp.temp, L, final);
+}
+
+// The md_run time loop (src/runmd.f90:233-330) on device buffers; see the
+// header.
+static void md_loop(mdffgpu_ctx* c, const mdffgpu_md_params& mp, int itime0, int nsteps, double* xi,
+             double* vxi, mdffgpu_md_step* steps) {
+  const int n = c->n;
+  const int gb = nblk(n);
+  hipStream_t st = c->st;
+  const bool nvt = mp.integrator != MDFFGPU_NVE_VV;
+  const int nhc = (mp.integrator == MDFFGPU_NVT_NHC2) ? 2 : mp.nhc_n;
+  // degrees of freedom: nhc2 3(N-1) (src/integration.f90:220), nhcn 3N (:505)
+  const double L = (mp.integrator == MDFFGPU_NVT_NHC2) ? 3.0 * ((double)n - 1.0) : 3.0 * (double)n;
+  NhcMass nm;
+  for (int t = 0; t < MDFF_NT; ++t) nm.m[t] = mp.mass[t];
+  k_massia<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, nm, c->d_massia);
+  check_launch(c, "k_massia");
+  NhcState hs;
+  std::memset(&hs, 0, sizeof(hs));
+  if (nvt) {
+    // thermostat masses: nhcn Q = timesca^2 temp, Q(1) *= L (:508-510);
+    // nhc2 Q(1) = timesca^2 temp L, Q(2) = timesca^2 temp (:223-224)
+    const double tq = std::pow(mp.timesca_thermo, 2.0) * mp.temp;
+    for (int h = 0; h < nhc; ++h) {
+      hs.Q[h] = tq;
+      hs.xi[h] = xi[h];
+      hs.vxi[h] = vxi[h];
+    }
+    hs.Q[0] = hs.Q[0] * L;
+    yoshida_weights(mp.nhc_yosh_order, hs.yw);
+    HCK(c, hipMemcpyAsync(c->d_nhc, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
+  }
+  MdArgs a;
+  a.n = n;
+  a.mass = c->d_massia;
+  a.pos = c->d_pos;
+  a.vel = c->d_vel;
+  a.f = c->d_force;
+  a.fs = c->d_fs;
+  a.dt = mp.dt;
+  a.dt2 = mp.dt * 0.5;
+  a.dtsq2 = mp.dt * mp.dt * 0.5;
+  a.part = c->d_part;
+  for (int k = 0; k < nsteps; ++k) {
+    const int itime = itime0 + k;
+    mdffgpu_energies en;
+    if (!nvt) {
+      // prop_velocity_verlet (src/integration.f90:131-194)
+      {
+        ProfScope _ps(c, K_MD);
+        k_vv_pos<<<gb, MDFF_BLOCK, 0, st>>>(a);
+      }
+      check_launch(c, "k_vv_pos");
+      run(c, itime, &en);
+      {
+        ProfScope _ps(c, K_MD);
+        k_vv_vel<<<gb, MDFF_BLOCK, 0, st>>>(a);
+        k_reduce_rows<<<1, MDFF_BLOCK, 0, st>>>(c->d_part, gb, c->d_sc + SC_KIN);
+      }
+      check_launch(c, "k_vv_vel");
+    } else {
+      // nhc2 / nhcn: chain(kin of calc_temp), prop_pos_vel_verlet, chain
+      // (kin of the second kick), calc_temp of the scaled velocities
+      md_kin(c, a, nullptr);
+      md_chain(c, mp, L, 0);
+      {
+        ProfScope _ps(c, K_MD);
+        k_vscale<<<gb, MDFF_BLOCK, 0, st>>>(a, c->d_nhc);
+        k_kick_drift<<<gb, MDFF_BLOCK, 0, st>>>(a);
+      }
+      check_launch(c, "k_kick_drift");
+      run(c, itime, &en);
+      {
+        ProfScope _ps(c, K_MD);
+        k_kick<<<gb, MDFF_BLOCK, 0, st>>>(a);
+        k_reduce_rows<<<1, MDFF_BLOCK, 0, st>>>(c->d_part, gb, c->d_sc + SC_KIN);
+      }
+      check_launch(c, "k_kick");
+      md_chain(c, mp, L, 1);
+      md_kin(c, a, c->d_nhc);
+      HCK(c, hipMemcpyAsync(c->h_sc + SC_KIN + 1, &c->d_nhc->e_nvt, sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    }
+    HCK(c, hipMemcpyAsync(c->h_sc + SC_KIN, c->d_sc + SC_KIN, sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+    HCK(c, hipStreamSynchronize(st));
+    prof_collect(c);
+    if (steps) {
+      mdffgpu_md_step& o = steps[k];
+      o.en = en;
+      o.e_kin = 0.5 * c->h_sc[SC_KIN];
+      // calc_temp: T = 2 ekin / boltz_unit / L, L = 3 natm (src/kinetic.f90:541-553)
+      o.temp_r = 2.0 * o.e_kin / mp.boltz_unit;
+      o.temp_r = o.temp_r / (3.0 * (double)n);
+      o.e_nvt = nvt ? c->h_sc[SC_KIN + 1] : 0.0;
+    }
+  }
+  if (nvt) {
+    HCK(c, hipMemcpyAsync(&hs, c->d_nhc, sizeof(hs), hipMemcpyDeviceToHost, st));
+    HCK(c, hipStreamSynchronize(st));
+    for (int h = 0; h < nhc; ++h) {
+      xi[h] = hs.xi[h];
+      vxi[h] = hs.vxi[h];
+    }
+  }
+}
+
+int mdffgpu_md_run(mdffgpu_ctx* c, const mdffgpu_md_params* mp, int32_t itime0, int32_t nsteps,
+                   double* d_pos, double* d_vel, double* d_force, double* xi, double* vxi,
+                   mdffgpu_md_step* steps, void* stream) {
+  if (!c) return MDFFGPU_ERR_ARG;
+  if (!mp || !d_pos || !d_vel || !d_force || nsteps < 0) {
+    c->err = "NULL MD parameter, position, velocity or force argument, or nsteps < 0";
+    return MDFFGPU_ERR_ARG;
+  }
+  if (c->noalloc) {
+    c->err = "the MD loop needs a context of mdffgpu_create";
+    return MDFFGPU_ERR_ARG;
+  }
+  const bool nvt = mp->integrator != MDFFGPU_NVE_VV;
+  if (mp->integrator < MDFFGPU_NVE_VV || mp->integrator > MDFFGPU_NVT_NHCN) {
+    c->err = "integrator must be nve-vv, nvt-nhc2 or nvt-nhcn";
+    return MDFFGPU_ERR_UNSUPPORTED;
+  }
+  double yw[9];
+  if (nvt && (!xi || !vxi)) {
+    c->err = "the Nose-Hoover integrators need the thermostat coordinates xi, vxi";
+    return MDFFGPU_ERR_ARG;
+  }
+  if (mp->integrator == MDFFGPU_NVT_NHCN &&
+      (mp->nhc_n < 2 || mp->nhc_n > MDFFGPU_NHC_MAX || mp->nhc_mults < 1 ||
+       !yoshida_weights(mp->nhc_yosh_order, yw))) {
+    // nhc_n = 1 is refused by md_check_tag (src/md.f90:302-305); the
+    // Yoshida order by chain_nhcn (src/integration.f90:596-599)
+    c->err = "nvt-nhcn needs 2 <= nhc_n <= 16, nhc_mults >= 1 and a Yoshida order of 1, 3, 5, 7 "
+             "or 9";
+    return MDFFGPU_ERR_ARG;
+  }
+  if (!(mp->dt > 0.0) || !(mp->boltz_unit > 0.0) || (nvt && !(mp->temp > 0.0)) ||
+      (nvt && !(mp->timesca_thermo > 0.0))) {
+    c->err = "dt and boltz_unit must be positive; temp and timesca_thermo too with a thermostat";
+    return MDFFGPU_ERR_ARG;
+  }
+  for (int t = 0; t < c->P.ntype; ++t)
+    if (!(mp->mass[t] > 0.0)) {
+      c->err = "every species needs a positive mass";
+      return MDFFGPU_ERR_ARG;
+    }
+  try {
+    c->st = stream ? (hipStream_t)stream : c->own_stream;
+    const size_t b = 3 * (size_t)c->n * sizeof(double);
+    if (!c->d_vel) {
+      dalloc(c, &c->d_vel, 3 * (size_t)c->n);
+      dalloc(c, &c->d_fs, 3 * (size_t)c->n);
+      dalloc(c, &c->d_massia, (size_t)c->n);
+      dalloc(c, &c->d_nhc, 1);
+    }
+    HCK(c, hipMemcpyAsync(c->d_pos, d_pos, b, hipMemcpyDeviceToDevice, c->st));
+    HCK(c, hipMemcpyAsync(c->d_vel, d_vel, b, hipMemcpyDeviceToDevice, c->st));
+    HCK(c, hipMemcpyAsync(c->d_force, d_force, b, hipMemcpyDeviceToDevice, c->st));
+    md_loop(c, *mp, itime0, nsteps, xi, vxi, steps);
+    HCK(c, hipMemcpyAsync(d_pos, c->d_pos, b, hipMemcpyDeviceToDevice, c->st));
+    HCK(c, hipMemcpyAsync(d_vel, c->d_vel, b, hipMemcpyDeviceToDevice, c->st));
+    HCK(c, hipMemcpyAsync(d_force, c->d_force, b, hipMemcpyDeviceToDevice, c->st));
+    HCK(c, hipStreamSynchronize(c->st));
+    c->st = c->own_stream;
+    return MDFFGPU_OK;
+  } catch (Fail& f) {
+    c->st = c->own_stream;
+    return f.code;
+  }
+}
+
+int mdffgpu_list_info(const mdffgpu_ctx* c, int32_t* builds, int32_t* cell_search) {
+  if (!c) return MDFFGPU_ERR_ARG;
+  if (builds) *builds = c->list_builds;
+  if (cell_search) *cell_search = c->gstencil ? 1 : 0;
+  return MDFFGPU_OK;
 }
 
 int mdffgpu_engforce_noalloc(const mdffgpu_params* params, const int32_t* itype,

@@ -13,7 +13,7 @@ import time
 import numpy as np
 
 from . import _lib
-from .params import MdffEnergies, MdffParams
+from .params import MdffEnergies, MdffMdParams, MdffMdStep, MdffParams
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -144,6 +144,38 @@ class ForceField:
             None if efg is None else ctypes.c_void_p(efg.data_ptr()), ctypes.byref(en), sp)
         _lib.check(st, self._h)
         return en
+
+    def md_run(self, mdp: MdffMdParams, pos, vel, force, nsteps: int, itime0: int = 1,
+               xi=None, vxi=None, stream=None) -> list:
+        """nsteps MD steps on device-resident torch tensors (float64,
+        contiguous (3, natm) SoA): pos, vel and force are in/out, force
+        holding the forces at pos on entry.  xi / vxi: numpy arrays of the
+        thermostat coordinates (nhc_n entries), updated in place.  Returns
+        one dict per step (energies, e_kin, temp_r, e_nvt)."""
+        for t in (pos, vel, force):
+            if t.dtype.itemsize != 8 or not t.is_contiguous() or t.numel() != 3 * self.n:
+                raise ValueError("pos, vel and force must be contiguous float64 (3, natm) tensors")
+        steps = (MdffMdStep * max(1, int(nsteps)))()
+        nvt = mdp.integrator != 0
+        if nvt:
+            for a in (xi, vxi):
+                if a is None or a.dtype != np.float64 or a.size < mdp.nhc_n:
+                    raise ValueError("a thermostat needs float64 xi and vxi of nhc_n entries")
+        sp = None if stream is None else ctypes.c_void_p(stream)
+        st = _lib.lib().mdffgpu_md_run(
+            self._h, ctypes.byref(mdp), int(itime0), int(nsteps), ctypes.c_void_p(pos.data_ptr()),
+            ctypes.c_void_p(vel.data_ptr()), ctypes.c_void_p(force.data_ptr()),
+            _ptr(xi) if nvt else None, _ptr(vxi) if nvt else None, steps, sp)
+        _lib.check(st, self._h)
+        return [steps[k].as_dict() for k in range(int(nsteps))]
+
+    def list_info(self) -> dict:
+        """Neighbour-list builds so far and whether the last one searched
+        the cell grid."""
+        b = ctypes.c_int32()
+        cs = ctypes.c_int32()
+        _lib.check(_lib.lib().mdffgpu_list_info(self._h, ctypes.byref(b), ctypes.byref(cs)), self._h)
+        return {"builds": b.value, "cell_search": bool(cs.value)}
 
     def profile(self, on: bool = True, only: str = None) -> None:
         """Enable (and reset) the per-kernel HIP-event instrumentation; with

@@ -325,3 +325,70 @@ def efg_opts(longrange: str = "ewald", ncelldirect: int = 2, cutlongrange: float
     o.lmp_correction = int(bool(lmp_correction))
     o.lefg_vasp_sign = int(bool(lefg_vasp_sign))
     return o
+
+
+# integrators of the device-resident MD loop (mdffgpu_integrator)
+NVE_VV = 0
+NVT_NHC2 = 1
+NVT_NHCN = 2
+INTEGRATORS = {"nve-vv": NVE_VV, "nvt-nhc2": NVT_NHC2, "nvt-nhcn": NVT_NHCN}
+NHC_MAX = 16
+
+
+class MdffMdParams(ctypes.Structure):
+    """Mirror of mdffgpu_md_params (include/mdffgpu.h): &mdtag after the
+    unit conversions of md_check_tag (src/md.f90:316-321)."""
+
+    _fields_ = [
+        ("integrator", ctypes.c_int32),
+        ("nhc_n", ctypes.c_int32),
+        ("nhc_yosh_order", ctypes.c_int32),
+        ("nhc_mults", ctypes.c_int32),
+        ("dt", ctypes.c_double),
+        ("temp", ctypes.c_double),
+        ("timesca_thermo", ctypes.c_double),
+        ("boltz_unit", ctypes.c_double),
+        ("mass", ctypes.c_double * NTYPEMAX),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+class MdffMdStep(ctypes.Structure):
+    """Mirror of mdffgpu_md_step (include/mdffgpu.h)."""
+
+    _fields_ = [
+        ("en", MdffEnergies),
+        ("e_kin", ctypes.c_double),
+        ("temp_r", ctypes.c_double),
+        ("e_nvt", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        out = self.en.as_dict()
+        out.update(e_kin=self.e_kin, temp_r=self.temp_r, e_nvt=self.e_nvt)
+        return out
+
+
+def md_params(integrator: str = "nve-vv", dt: float = 0.0, temp: float = 1.0,
+              timesca_thermo: float = 1.0, nhc_n: int = 4, nhc_yosh_order: int = 3,
+              nhc_mults: int = 2, mass=(1.0,), lreduced: bool = False) -> MdffMdParams:
+    """&mdtag values in the reference's input units (dt and timesca_thermo
+    in ps, temp in K; reduced units with lreduced), converted as
+    md_check_tag does (src/md.f90:316-321, units src/constants.f90:85-87,
+    139-142).  Defaults as md_default_tag (src/md.f90:191-203)."""
+    if integrator not in INTEGRATORS:
+        raise ValueError(f"integrator must be one of {sorted(INTEGRATORS)}")
+    boltz = 1.0 if lreduced else BOLTZ_UNIT
+    tu = 1.0 if lreduced else TIME_UNIT
+    m = MdffMdParams()
+    m.integrator = INTEGRATORS[integrator]
+    m.nhc_n = 2 if integrator == "nvt-nhc2" else int(nhc_n)
+    m.nhc_yosh_order = int(nhc_yosh_order)
+    m.nhc_mults = int(nhc_mults)
+    m.dt = float(dt) * tu
+    m.temp = float(temp) * boltz
+    m.timesca_thermo = float(timesca_thermo) * tu
+    m.boltz_unit = boltz
+    for k, v in enumerate(mass):
+        m.mass[k] = float(v)
+    return m

@@ -1513,3 +1513,228 @@ int orc_efgcalc(const mdffgpu_params* p, const mdffgpu_efg_opts* o, const int* i
   free(buf);
   return rc;
 }
+
+/* ------------------------------------------------------------------------
+ * Molecular dynamics time loop: the integrators md_run dispatches
+ * (src/runmd.f90:244-251) for 'nve-vv', 'nvt-nhc2' and 'nvt-nhcn'.
+ * ---------------------------------------------------------------------- */
+
+/* calc_temp (src/kinetic.f90:531-556): ekin = 1/2 sum m v^2. */
+static double orc_kin(int n, const double* m, const double* vx, const double* vy,
+                      const double* vz) {
+  double e = 0.0;
+  for (int i = 0; i < n; ++i) e = e + (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]) * m[i];
+  return e * 0.5;
+}
+
+/* Yoshida weights of chain_nhcn (src/integration.f90:596-632). */
+int orc_yoshida(int order, double* w) {
+  switch (order) {
+    case 1: w[0] = 1.0; return 0;
+    case 3:
+      w[0] = 1.0 / (2.0 - pow(2.0, 1.0 / 3.0));
+      w[1] = 1.0 - 2.0 * w[0];
+      w[2] = w[0];
+      return 0;
+    case 5:
+      w[0] = 1.0 / (4.0 - pow(4.0, 1.0 / 3.0));
+      w[1] = w[0]; w[2] = w[0]; w[3] = w[0];
+      w[4] = 1.0 - 4.0 * w[0];
+      return 0;
+    case 7:
+      w[0] = .78451361047756; w[1] = .235573213359357; w[2] = -1.17767998417887;
+      w[3] = 1.0 - 2.0 * (w[0] + w[1] + w[2]);
+      w[4] = w[2]; w[5] = w[1]; w[6] = w[0];
+      return 0;
+    case 9:
+      w[0] = 0.192; w[1] = 0.554910818409783619692725006662999;
+      w[2] = 0.124659619941888644216504240951585; w[3] = -0.843182063596933505315033808282941;
+      w[4] = 1.0 - 2.0 * (w[0] + w[1] + w[2] + w[3]);
+      w[5] = w[3]; w[6] = w[2]; w[7] = w[1]; w[8] = w[0];
+      return 0;
+    default: return MDFFGPU_ERR_ARG;
+  }
+}
+
+/* chain_nhcn (src/integration.f90:564-713) with trotter_order = -1 (the
+   branch nhcn calls): returns the scale factor; the caller scales the
+   velocities and recomputes kin.  G is zeroed per call as the reference's
+   allocate + G = 0 (:634-635). */
+double orc_chain_nhcn(double kin, double* vxi, double* xi, const double* Q, double L,
+                      double temp, double dt, int nhc_n, int yosh, int mults) {
+  double w[9], G[MDFFGPU_NHC_MAX];
+  orc_yoshida(yosh, w);
+  for (int k = 0; k < nhc_n; ++k) G[k] = 0.0;
+  double s = 1.0;
+  G[0] = 2.0 * kin - L * temp;
+  const int N = nhc_n - 1;
+  for (int k = 0; k < mults; ++k) {
+    for (int j = 0; j < yosh; ++j) {
+      double dts = (w[j] * dt) / (double)mults;
+      double dts2 = dts * 0.5, dts4 = dts2 * 0.5, dts8 = dts4 * 0.5;
+      G[N] = vxi[N - 1] * vxi[N - 1] / Q[N - 1] - temp;
+      vxi[N] = vxi[N] + G[N] * dts4;
+      for (int h = N - 1; h >= 0; --h) {
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+        vxi[h] = vxi[h] + G[h] * dts4;
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+      }
+      s = s * exp(-vxi[0] * dts2 / Q[0]);
+      for (int h = 0; h <= N; ++h) xi[h] = xi[h] + vxi[h] * dts2 / Q[h];
+      G[0] = 2.0 * kin * s * s - L * temp;
+      for (int h = 0; h < N; ++h) {
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+        vxi[h] = vxi[h] + G[h] * dts4;
+        vxi[h] = vxi[h] * exp(-vxi[h + 1] * dts8 / Q[h + 1]);
+        G[h + 1] = vxi[h] * vxi[h] / Q[h] - temp;
+      }
+      vxi[N] = vxi[N] + G[N] * dts4;
+    }
+  }
+  return s;
+}
+
+/* chain_nhc2 (src/integration.f90:261-324): returns s. */
+double orc_chain_nhc2(double kin, double* vxi, double* xi, const double* Q, double L,
+                      double temp, double dt) {
+  double s = 1.0;
+  double dt2 = dt * 0.5, dt4 = dt2 * 0.5, dt8 = dt4 * 0.5;
+  double G1 = (2.0 * kin - L * temp) / Q[0];
+  double G2 = (Q[0] * vxi[0] * vxi[0] - temp) / Q[1];
+  vxi[1] = vxi[1] + G2 * dt4;
+  vxi[0] = vxi[0] * exp(-vxi[1] * dt8);
+  vxi[0] = vxi[0] + G1 * dt4;
+  vxi[0] = vxi[0] * exp(-vxi[1] * dt8);
+  s = s * exp(-vxi[0] * dt2);
+  kin = s * s * kin;
+  xi[0] = xi[0] + vxi[0] * dt2;
+  xi[1] = xi[1] + vxi[1] * dt2;
+  G1 = (2.0 * kin - L * temp) / Q[0];
+  vxi[0] = vxi[0] * exp(-vxi[1] * dt8);
+  vxi[0] = vxi[0] + G1 * dt4;
+  vxi[0] = vxi[0] * exp(-vxi[1] * dt8);
+  G2 = (Q[0] * vxi[0] * vxi[0] - temp) / Q[1];
+  vxi[1] = vxi[1] + G2 * dt4;
+  return s;
+}
+
+int orc_md_run(const mdffgpu_params* p, const int* itype, const double* A,
+               const mdffgpu_md_params* mp, int itime0, int nsteps, double* rx, double* ry,
+               double* rz, double* vx, double* vy, double* vz, double* fx, double* fy,
+               double* fz, double* xi, double* vxi, orc_state* st, mdffgpu_md_step* steps) {
+  const int n = p->natm;
+  const int integ = mp->integrator;
+  if (integ < MDFFGPU_NVE_VV || integ > MDFFGPU_NVT_NHCN) return MDFFGPU_ERR_UNSUPPORTED;
+  double w[9];
+  if (integ == MDFFGPU_NVT_NHCN &&
+      (mp->nhc_n < 2 || mp->nhc_n > MDFFGPU_NHC_MAX || mp->nhc_mults < 1 ||
+       orc_yoshida(mp->nhc_yosh_order, w)))
+    return MDFFGPU_ERR_ARG;
+  double* m = (double*)malloc((size_t)n * sizeof(double));
+  double* fs = (double*)malloc(3 * (size_t)n * sizeof(double));
+  if (!m || !fs) {
+    free(m);
+    free(fs);
+    return MDFFGPU_ERR_ALLOC;
+  }
+  for (int i = 0; i < n; ++i) m[i] = mp->mass[itype[i] - 1];
+  const double dt = mp->dt, temp = mp->temp;
+  /* thermostat masses and degrees of freedom (:218-224, :504-510) */
+  const int nhc = (integ == MDFFGPU_NVT_NHC2) ? 2 : mp->nhc_n;
+  const double L = (integ == MDFFGPU_NVT_NHC2) ? 3.0 * ((double)n - 1.0) : 3.0 * (double)n;
+  double Q[MDFFGPU_NHC_MAX];
+  for (int h = 0; h < nhc; ++h) Q[h] = pow(mp->timesca_thermo, 2.0) * temp;
+  Q[0] = Q[0] * L;
+  int rc = 0;
+  for (int k = 0; k < nsteps && rc == 0; ++k) {
+    mdffgpu_energies en;
+    double kin = 0.0, e_nvt = 0.0;
+    if (integ == MDFFGPU_NVE_VV) {
+      /* prop_velocity_verlet (src/integration.f90:131-194) */
+      double dtsq2 = dt * dt * 0.5, dt2 = dt * 0.5;
+      for (int i = 0; i < n; ++i) {
+        fs[i] = fx[i];
+        fs[i + n] = fy[i];
+        fs[i + 2 * n] = fz[i];
+      }
+      for (int i = 0; i < n; ++i) {
+        rx[i] = rx[i] + vx[i] * dt + (fx[i] * dtsq2) / m[i];
+        ry[i] = ry[i] + vy[i] * dt + (fy[i] * dtsq2) / m[i];
+        rz[i] = rz[i] + vz[i] * dt + (fz[i] * dtsq2) / m[i];
+      }
+      rc = orc_engforce(p, itype, A, rx, ry, rz, itime0 + k, st, fx, fy, fz, NULL, NULL, NULL,
+                        NULL, &en);
+      for (int i = 0; i < n; ++i) {
+        vx[i] = vx[i] + (fs[i] + fx[i]) * dt2 / m[i];
+        vy[i] = vy[i] + (fs[i + n] + fy[i]) * dt2 / m[i];
+        vz[i] = vz[i] + (fs[i + 2 * n] + fz[i]) * dt2 / m[i];
+      }
+      kin = orc_kin(n, m, vx, vy, vz);
+    } else {
+      /* nhc2 (:205-249) / nhcn (:476-553) */
+      double s;
+      kin = orc_kin(n, m, vx, vy, vz);
+      if (integ == MDFFGPU_NVT_NHC2)
+        s = orc_chain_nhc2(kin, vxi, xi, Q, L, temp, dt);
+      else
+        s = orc_chain_nhcn(kin, vxi, xi, Q, L, temp, dt, nhc, mp->nhc_yosh_order, mp->nhc_mults);
+      for (int i = 0; i < n; ++i) {
+        vx[i] = s * vx[i];
+        vy[i] = s * vy[i];
+        vz[i] = s * vz[i];
+      }
+      /* prop_pos_vel_verlet (:333-385) */
+      double dt2 = dt * 0.5;
+      for (int i = 0; i < n; ++i) {
+        vx[i] = vx[i] + fx[i] * dt2 / m[i];
+        vy[i] = vy[i] + fy[i] * dt2 / m[i];
+        vz[i] = vz[i] + fz[i] * dt2 / m[i];
+        rx[i] = rx[i] + vx[i] * dt;
+        ry[i] = ry[i] + vy[i] * dt;
+        rz[i] = rz[i] + vz[i] * dt;
+      }
+      rc = orc_engforce(p, itype, A, rx, ry, rz, itime0 + k, st, fx, fy, fz, NULL, NULL, NULL,
+                        NULL, &en);
+      for (int i = 0; i < n; ++i) {
+        vx[i] = vx[i] + fx[i] * dt2 / m[i];
+        vy[i] = vy[i] + fy[i] * dt2 / m[i];
+        vz[i] = vz[i] + fz[i] * dt2 / m[i];
+      }
+      kin = orc_kin(n, m, vx, vy, vz);
+      if (integ == MDFFGPU_NVT_NHC2)
+        s = orc_chain_nhc2(kin, vxi, xi, Q, L, temp, dt);
+      else
+        s = orc_chain_nhcn(kin, vxi, xi, Q, L, temp, dt, nhc, mp->nhc_yosh_order, mp->nhc_mults);
+      for (int i = 0; i < n; ++i) {
+        vx[i] = s * vx[i];
+        vy[i] = s * vy[i];
+        vz[i] = s * vz[i];
+      }
+      kin = orc_kin(n, m, vx, vy, vz);
+      if (integ == MDFFGPU_NVT_NHC2) {
+        e_nvt = 0.0;
+        e_nvt = e_nvt + L * temp * xi[0];
+        e_nvt = e_nvt + vxi[0] * vxi[0] * 0.5 / Q[0];
+        e_nvt = e_nvt + vxi[1] * vxi[1] * 0.5 / Q[1];
+        e_nvt = e_nvt + temp * xi[1];
+      } else {
+        double nvt1 = L * temp * xi[0], nvt2 = vxi[0] * vxi[0] * 0.5 / Q[0], nvt3 = 0.0,
+               nvt4 = 0.0;
+        for (int h = 1; h < nhc; ++h) {
+          nvt3 = nvt3 + vxi[h] * vxi[h] * 0.5 / Q[h];
+          nvt4 = nvt4 + temp * xi[h];
+        }
+        e_nvt = nvt1 + nvt2 + nvt3 + nvt4;
+      }
+    }
+    if (steps) {
+      steps[k].en = en;
+      steps[k].e_kin = kin;
+      steps[k].temp_r = 2.0 * kin / mp->boltz_unit / (3.0 * (double)n);
+      steps[k].e_nvt = e_nvt;
+    }
+  }
+  free(m);
+  free(fs);
+  return rc;
+}

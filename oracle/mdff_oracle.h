@@ -103,6 +103,25 @@ int orc_multipole_es(const mdffgpu_params* p, const int* itype,
                      double* u_coul, double* tau_coul, double* pvirial,
                      double* u_parts /* [4]: dir, rec, self, surf (raw) */);
 
+/* The md_run time loop (src/runmd.f90:233-330) for 'nve-vv'
+   (prop_velocity_verlet, src/integration.f90:131-194), 'nvt-nhc2' (nhc2 +
+   chain_nhc2, :205-324) and 'nvt-nhcn' (nhcn + chain_nhcn, :476-713):
+   arguments as mdffgpu_md_run on host arrays; f? are the forces at r? on
+   entry.  Returns 0 or an mdffgpu_status. */
+int orc_md_run(const mdffgpu_params* p, const int* itype, const double* A,
+               const mdffgpu_md_params* mp, int itime0, int nsteps, double* rx,
+               double* ry, double* rz, double* vx, double* vy, double* vz,
+               double* fx, double* fy, double* fz, double* xi, double* vxi,
+               orc_state* st, mdffgpu_md_step* steps);
+/* The thermostat pieces, exposed for the tests: Yoshida weights and one
+   chain call each (kin in, returns the velocity scale factor). */
+int orc_yoshida(int order, double* w);
+double orc_chain_nhcn(double kin, double* vxi, double* xi, const double* Q,
+                      double L, double temp, double dt, int nhc_n, int yosh,
+                      int mults);
+double orc_chain_nhc2(double kin, double* vxi, double* xi, const double* Q,
+                      double L, double temp, double dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,7 @@ import subprocess
 
 import numpy as np
 
-from mdff_amd.params import MdffEfgOpts, MdffEnergies, MdffParams
+from mdff_amd.params import MdffEfgOpts, MdffEnergies, MdffMdParams, MdffMdStep, MdffParams
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
@@ -65,6 +65,19 @@ def lib():
         L.orc_efgcalc.restype = ctypes.c_int
         L.orc_efgcalc.argtypes = [ctypes.POINTER(MdffParams), ctypes.POINTER(MdffEfgOpts), _ip,
                                   _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp]
+        L.orc_md_run.restype = ctypes.c_int
+        L.orc_md_run.argtypes = [ctypes.POINTER(MdffParams), _ip, _dp, ctypes.POINTER(MdffMdParams),
+                                 ctypes.c_int, ctypes.c_int] + [_dp] * 11 + [
+                                     ctypes.POINTER(_State), ctypes.POINTER(MdffMdStep)]
+        L.orc_yoshida.restype = ctypes.c_int
+        L.orc_yoshida.argtypes = [ctypes.c_int, _dp]
+        L.orc_chain_nhcn.restype = ctypes.c_double
+        L.orc_chain_nhcn.argtypes = [ctypes.c_double, _dp, _dp, _dp, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int]
+        L.orc_chain_nhc2.restype = ctypes.c_double
+        L.orc_chain_nhc2.argtypes = [ctypes.c_double, _dp, _dp, _dp, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -167,6 +180,49 @@ class Oracle:
         out["efg"] = efg.reshape(3, 3, n).transpose(2, 1, 0).copy()
         out["theta"] = theta.reshape(3, 3, n).transpose(2, 1, 0).copy()
         return out
+
+
+    def md_run(self, mdp: MdffMdParams, r, v, f, nsteps: int, itime0: int = 1, xi=None,
+               vxi=None):
+        """nsteps steps of the reference integrator (orc_md_run); r, v, f are
+        (natm, 3) with f the forces at r.  Returns (r, v, f, xi, vxi, steps)
+        with steps a list of dicts as ForceField.md_run's."""
+        n = self.p.natm
+        cols = [np.ascontiguousarray(np.asarray(a, dtype=np.float64)[:, k])
+                for a in (r, v, f) for k in range(3)]
+        xi = np.zeros(16) if xi is None else np.array(xi, dtype=np.float64)
+        vxi = np.zeros(16) if vxi is None else np.array(vxi, dtype=np.float64)
+        steps = (MdffMdStep * max(1, int(nsteps)))()
+        Aflat = np.ascontiguousarray(self.A.reshape(-1, order="F"))
+        rc = lib().orc_md_run(ctypes.byref(self.p), self.itype.ctypes.data_as(_ip), _p(Aflat),
+                              ctypes.byref(mdp), int(itime0), int(nsteps),
+                              *[_p(c) for c in cols], _p(xi), _p(vxi),
+                              ctypes.byref(self.state), steps)
+        if rc != 0:
+            raise RuntimeError(f"oracle md_run failed with status {rc}")
+        out = [np.stack(cols[3 * k:3 * k + 3], axis=1) for k in range(3)]
+        return (out[0], out[1], out[2], xi, vxi, [steps[k].as_dict() for k in range(int(nsteps))])
+
+
+def yoshida(order: int) -> np.ndarray:
+    w = np.zeros(9)
+    if lib().orc_yoshida(int(order), _p(w)) != 0:
+        raise ValueError("Yoshida order must be 1, 3, 5, 7 or 9")
+    return w[:order].copy()
+
+
+def chain_nhcn(kin, vxi, xi, Q, L, temp, dt, nhc_n, yosh=3, mults=2):
+    """One chain_nhcn call; vxi and xi (float64 arrays) are updated in place;
+    returns the velocity scale factor."""
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    return lib().orc_chain_nhcn(float(kin), _p(vxi), _p(xi), _p(Q), float(L), float(temp),
+                                float(dt), int(nhc_n), int(yosh), int(mults))
+
+
+def chain_nhc2(kin, vxi, xi, Q, L, temp, dt):
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    return lib().orc_chain_nhc2(float(kin), _p(vxi), _p(xi), _p(Q), float(L), float(temp),
+                                float(dt))
 
 
 def multipole_es(params: MdffParams, itype, A, r, q, mu, task, damp_ind=True,
