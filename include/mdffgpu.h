@@ -34,7 +34,7 @@ extern "C" {
 
 #define MDFFGPU_NTYPEMAX 16 /* ntypemax, src/config.f90 */
 #define MDFFGPU_TT_MAX_ORDER 8 /* maximum_of_TT_expansion, src/tt_damp.f90:8 */
-#define MDFFGPU_ABI_VERSION 1
+#define MDFFGPU_ABI_VERSION 2
 
 typedef enum mdffgpu_status {
   MDFFGPU_OK = 0,
@@ -45,8 +45,10 @@ typedef enum mdffgpu_status {
                                    (src/tools.f90:339-342) */
   MDFFGPU_ERR_TT_ORDER = 5,     /* Tang-Toennies order > 8
                                    (src/field.f90:355-358) */
-  MDFFGPU_ERR_UNSUPPORTED = 6,  /* quadrupole tasks, Wannier centres, ... */
-  MDFFGPU_ERR_NO_LIBRARY = 7    /* reported by host wrappers only */
+  MDFFGPU_ERR_UNSUPPORTED = 6,  /* an option outside the supported set */
+  MDFFGPU_ERR_NO_LIBRARY = 7,   /* reported by host wrappers only */
+  MDFFGPU_ERR_WFC = 8           /* an ion without its lwfc Wannier centres
+                                   within reach (src/field.f90:3594-3599) */
 } mdffgpu_status;
 
 typedef enum mdffgpu_precision {
@@ -139,8 +141,8 @@ typedef struct mdffgpu_params {
   /* Ewald (src/field.f90:163-167) */
   double alphaES;
   int32_t kES[3];
-  /* task_coul(1:3): charge-charge, charge-dipole, dipole-dipole.  The
-     quadrupole tasks 4..6 are not supported. */
+  /* task_coul(1:3): charge-charge, charge-dipole, dipole-dipole; the
+     quadrupole tasks 4..6 are task_coul_quad below. */
   int32_t task_coul[3];
 
   /* polarizable-ion SCF (src/field.f90:139-143, 150-151) */
@@ -154,11 +156,34 @@ typedef struct mdffgpu_params {
   int32_t precision;        /* mdffgpu_precision */
   int32_t mixed_fp64_parts; /* mdffgpu_mixed_part bits, 0 = all mixed */
   int32_t scf_solver;       /* mdffgpu_scf_solver */
-  /* task_coul(4:6) of the reference (charge-quadrupole, dipole-quadrupole,
-     quadrupole-quadrupole, src/field.f90:166): any non-zero entry is
-     refused with MDFFGPU_ERR_UNSUPPORTED */
+  /* task_coul(4:6) of the reference: charge-quadrupole,
+     dipole-quadrupole, quadrupole-quadrupole (src/field.f90:166, set by
+     field_check_tag :391-395).  Quadrupoles run on the FP64 path only: a
+     mixed-precision context or a batch refuses them with
+     MDFFGPU_ERR_UNSUPPORTED. */
   int32_t task_coul_quad[3];
   int32_t reserved[2];
+
+  /* ABI version 2 */
+  /* permanent quadrupoles quad(it,a,b) -> [it + 16*(a + 3*b)]
+     (src/field.f90:133), symmetric in (a,b) */
+  double quad[144];
+  /* quadrupole polarizabilities polquad(it,g,a,b)
+     -> [it + 16*(g + 3*(a + 3*b))] (src/field.f90:138, polquad_iso already
+     expanded as :422-426), symmetric in (a,b); theta_ind(a,b) =
+     sum_g polquad(g,a,b) E_g on species with lquad_polar (induced_moment,
+     :1716-1803).  The Jacobi SCF only. */
+  double polquad[432];
+  int32_t lquad_polar[16];
+  /* Wannier centres (moment_from_WFc, src/field.f90:3492-3672): lwfc(it) > 0
+     centres per ion of species it, < 0 on the species of the centres;
+     their dipoles -2 sum (r_c - r_ion) join the total moments when
+     ldip_wfc (:3725-3740).  rcut_wfc is compared with the squared distance,
+     as the reference does (:3572). */
+  int32_t lwfc[16];
+  int32_t ldip_wfc;
+  int32_t reserved2;
+  double rcut_wfc;
 } mdffgpu_params;
 
 /* Scalar outputs of one evaluation: the thermodynamic module variables

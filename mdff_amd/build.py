@@ -19,7 +19,7 @@ OUT = os.path.join(HERE, "libmdffgpu.so")
 ARCH = os.environ.get("MDFF_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels64.hip", "nlist.hip", "md.hip", "realspace.hip", "mixed.hip", "batch.hip", "efg.hip", "mdffgpu.hip"]
+SOURCES = ["kernels64.hip", "nlist.hip", "md.hip", "quad.hip", "realspace.hip", "mixed.hip", "batch.hip", "efg.hip", "mdffgpu.hip"]
 
 DEVICE_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -784,7 +784,7 @@ int jacobi(mdffgpu_batch* b, const double* S, double* x, double* E, int iters, d
   while (it < nmax) {
     ++it;
     k_induced<<<gb, MDFF_BLOCK, 0, b->st>>>(N, b->d_typ, b->d_ty, E, x, b->d_part, nullptr,
-                                            nullptr);
+                                            nullptr, nullptr);
     es_call<R>(b, ES_IND, b->d_zero, x, S, E);
     if (iters == 0) {
       k_scf_resid<<<gb, MDFF_BLOCK, 0, b->st>>>(N, b->d_typ, b->d_ty, x, E, b->d_part);
@@ -895,6 +895,13 @@ void set_model(mdffgpu_batch* b, const mdffgpu_params& P, bool first) {
   for (int k = 0; k < 48; ++k)
     if (P.dip[k] != 0.0) bfail(b, MDFFGPU_ERR_UNSUPPORTED, "permanent dipoles in a batch");
   if (P.lsurf) bfail(b, MDFFGPU_ERR_UNSUPPORTED, "surface term in a batch");
+  for (int k = 0; k < 144; ++k)
+    if (P.quad[k] != 0.0) bfail(b, MDFFGPU_ERR_UNSUPPORTED, "quadrupoles in a batch");
+  for (int t = 0; t < 16; ++t)
+    if (P.lquad_polar[t] || P.lwfc[t] != 0)
+      bfail(b, MDFFGPU_ERR_UNSUPPORTED, "quadrupole polarizabilities or Wannier centres in a batch");
+  if (P.task_coul_quad[0] || P.task_coul_quad[1] || P.task_coul_quad[2])
+    bfail(b, MDFFGPU_ERR_UNSUPPORTED, "quadrupole tasks in a batch");
   if (P.precision != MDFFGPU_PREC_FP64 && P.precision != MDFFGPU_PREC_MIXED)
     bfail(b, MDFFGPU_ERR_ARG, "unknown precision");
   if (!first) {

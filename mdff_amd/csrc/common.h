@@ -55,7 +55,11 @@ struct TypeTables {
   double dip[MDFF_NT][3];
   double pol[MDFF_NT][9];     // poldip(it,a,b) at [a + 3*b]
   double invpol[MDFF_NT][9];  // 3x3 inverse (src/read_pos.f90:201-210)
-  int32_t polar[MDFF_NT];
+  int32_t polar[MDFF_NT];     // ldip_polar
+  // quadrupole polarizability polquad(it,g,a,b) at [g + 3*(a + 3*b)], and
+  // lquad_polar (src/field.f90:138, 154)
+  double polq[MDFF_NT][27];
+  int32_t qpolar[MDFF_NT];
 };
 
 // E_TT(k) = 1/k! (src/tt_damp.f90:13-29)

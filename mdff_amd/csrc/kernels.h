@@ -17,7 +17,8 @@ constexpr int DD_LANES = 64;    // lanes per atom of the SCF dipole field
 
 enum { ES_STATIC = 0, ES_IND = 1, ES_FINAL = 2 };
 enum { SR_LJ = 1, SR_BM = 2, SR_BMD = 4, SR_MO = 8, SR_TRUNC1 = 16, SR_TRUNC2 = 32 };
-enum { TASK_QQ = 1, TASK_QD = 2, TASK_DD = 4 };
+enum { TASK_QQ = 1, TASK_QD = 2, TASK_DD = 4, TASK_QQUAD = 8, TASK_DQUAD = 16, TASK_QUADQ = 32 };
+constexpr int TASK_QUAD_ANY = TASK_QQUAD | TASK_DQUAD | TASK_QUADQ;
 
 // Kernel identifiers of the launch instrumentation (mdffgpu_profile_*).
 enum KernelId {
@@ -25,7 +26,7 @@ enum KernelId {
   K_RHO_BOTH, K_RHO_RED, K_APPLY_F, K_APPLY_EFG, K_MOMENTS, K_COMBINE, K_INDUCED,
   K_RESID, K_EXTRAP, K_REDUCE, K_ADD, K_PAIRS_T2, K_DD_FIELD, K_MX_SR, K_MX_DIR, K_MX_PHASE, K_MX_RHO,
   K_MX_RHO_RED, K_MX_APPLY, K_MX_COMBINE, K_MX_WEIGHTS, K_MX_GPREP, K_BATCH, K_ADJ,
-  K_EFG_DIR, K_EFG_DS, K_EFG_COMBINE, K_MD, K_COUNT
+  K_EFG_DIR, K_EFG_DS, K_EFG_COMBINE, K_MD, K_QUAD, K_WFC, K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
 
@@ -116,6 +117,7 @@ struct RhoArgs {
   const double2* eph;
   const double* q;
   const double* mu;
+  const double* th;  // quadrupoles (n,3,3) Fortran order, for the QUAD variant
   Cell cell;
   int atoms_per_split, kpad;
   double2* part;
@@ -165,6 +167,9 @@ struct CombineArgs {
   // direction p; writes Ap = alpha^-1 p - E_ind(p) instead of E and the block
   // partials of p.Ap into row 0 of rpart
   double* cg_Ap;
+  // ES_IND with quadrupole polarizabilities: the induced quadrupoles of the
+  // residual (n,3,3), or null
+  const double* th;
 };
 
 struct ExtrapArgs {
@@ -179,6 +184,7 @@ struct ExtrapArgs {
   const TypeTables* T;
   double* part;
   double* rpart;  // op 5: SCF residual rows 0, 1 of the corrected moments, or null
+  double* th;     // op 5 with quadrupole polarizabilities: theta_ind = polquad E, or null
 };
 
 __global__ void k_prepare(int n, const double* pos, Cell cell, double* s, double4* s4,
@@ -274,7 +280,7 @@ inline int dd_tab_blocks(int n) {
   const long long waves = std::min<long long>(n, (long long)cus * 16);
   return (int)((waves * 64 + MDFF_BLOCK - 1) / MDFF_BLOCK);
 }
-template <bool CHG, bool DIP>
+template <bool CHG, bool DIP, bool QUAD = false>
 __global__ void k_rho64(RhoArgs a);
 __global__ void k_rho_reduce64(RhoRedArgs a);
 template <bool EFG>
@@ -319,9 +325,48 @@ __global__ void k_cg_update(CgArgs a);
 __global__ void k_cg_dir(CgArgs a);
 __global__ void k_cg_upol(CgArgs a);
 __global__ void k_induced(int n, const int* typ, const TypeTables* T, const double* E,
-                          double* mu, double* part, const int* rank, float4* mu4);
+                          double* mu, double* part, const int* rank, float4* mu4, double* th);
 __global__ void k_scf_resid(int n, const int* typ, const TypeTables* T, const double* mu,
                             const double* E, double* part);
+
+// Quadrupole terms of the real-space Ewald sum (realspace.hip): charge-,
+// dipole- and quadrupole-quadrupole energies, the quadrupole-quadrupole
+// field and EFG (src/field.f90:2542-2601), FP64.
+struct QuadArgs {
+  int n, maxnb;
+  const double* s;
+  const double4* s4;
+  const int* cnt;
+  const int* nbr;
+  const int* perm;
+  Cell cell;
+  double cut2, alpha;
+  const double* q;
+  const double* mu;
+  const double* th;  // (n,3,3) Fortran order
+  int task;          // TASK_QQUAD / TASK_DQUAD / TASK_QUADQ bits
+  int do_energy, do_efield, do_efg;
+  double* ef;    // added to
+  double* efg;   // added to
+  double* part;  // block partials of the energy (one row)
+};
+__global__ void k_es_quad(QuadArgs a);
+
+// Wannier-centre dipoles (moment_from_WFc, src/field.f90:3492-3672) from the
+// neighbour list, or from all atoms when nbr is null.
+struct WfcArgs {
+  int n, maxnb;
+  const double* s;
+  const int* typ;
+  const int* cnt;
+  const int* nbr;
+  int lwfc[MDFF_NT];  // per species
+  Cell cell;
+  double rcut;      // compared with the squared distance
+  double* mu;       // (n,3), written for every atom
+  int* bad;         // set to 1 + an ion without its lwfc centres
+};
+__global__ void k_wfc(WfcArgs a);
 __global__ void k_extrap(ExtrapArgs a);
 __global__ void k_reduce_rows(const double* part, int nb, double* out);
 __global__ void k_reduce_rows2(const double* part, int nb, double* out, double* out2);

@@ -130,8 +130,10 @@ template __global__ void k_nlist<true>(int, const double*, Cell, double, int, in
 // builds E_a[t][m] = e^{2 pi i m s_a}, m = 0..kmax_a, by recurrence from the
 // per-atom bases; e^{-i x} = conj(e^{i x}) gives the negative m.  Along the
 // chunk the phase advances by e3.
-//   rho(k) = sum_j (q_j + i k.mu_j) e^{i k.r_j},  k.mu_j = 2 pi n.(B^T mu_j)
-template <bool CHG, bool DIP>
+//   rho(k) = sum_j (q_j + i k.mu_j - k.Q_j.k / 3) e^{i k.r_j},
+//   k.mu_j = 2 pi n.(B^T mu_j),  k.Q_j.k = 4 pi^2 n.(B^T Q_j B).n
+// (the quadrupole term with QUAD, src/field.f90:2723-2740)
+template <bool CHG, bool DIP, bool QUAD>
 __global__ __launch_bounds__(MDFF_BLOCK) void k_rho64(RhoArgs a) {
   extern __shared__ double2 sh[];
   const int TA = RHO_TILE;
@@ -141,6 +143,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_rho64(RhoArgs a) {
   double2* E3 = E2 + TA * L2;       // [TA][L3]
   double* aq = (double*)(E3 + TA * L3);  // [TA]
   double* am = aq + TA;                  // [3][TA]  2 pi B^T mu
+  // QUAD: [6][TA] 4 pi^2 / 3 (B^T Q B) as xx yy zz xy xz yz (2 x off-diagonal)
+  double* aqd = am + 3 * TA;
   const int ck = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = ck < a.nchunk;
   KChunk C = act ? a.chunks[ck] : a.chunks[0];
@@ -180,6 +184,34 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_rho64(RhoArgs a) {
         am[TA + t] = tpi * (B[3] * mx + B[4] * my + B[5] * mz);
         am[2 * TA + t] = tpi * (B[6] * mx + B[7] * my + B[8] * mz);
       }
+      if (QUAD) {
+        // k.Q.k with the reference's upper-triangle components, Q_xy used
+        // for both xy and yx (src/field.f90:2729-2732)
+        double Q[3][3];
+        const double* th = a.th;
+        Q[0][0] = th[j];
+        Q[1][1] = th[j + 4 * n];
+        Q[2][2] = th[j + 8 * n];
+        Q[0][1] = Q[1][0] = th[j + 3 * n];
+        Q[0][2] = Q[2][0] = th[j + 6 * n];
+        Q[1][2] = Q[2][1] = th[j + 7 * n];
+        const double* B = a.cell.B;  // column c = reciprocal vector c
+        double P[3][3];
+        for (int u = 0; u < 3; ++u)
+          for (int v = 0; v < 3; ++v) {
+            double acc = 0.0;
+            for (int x = 0; x < 3; ++x)
+              for (int y = 0; y < 3; ++y) acc += B[x + 3 * u] * Q[x][y] * B[y + 3 * v];
+            P[u][v] = acc;
+          }
+        const double f = tpi * tpi / 3.0;
+        aqd[t] = f * P[0][0];
+        aqd[TA + t] = f * P[1][1];
+        aqd[2 * TA + t] = f * P[2][2];
+        aqd[3 * TA + t] = 2.0 * f * P[0][1];
+        aqd[4 * TA + t] = 2.0 * f * P[0][2];
+        aqd[5 * TA + t] = 2.0 * f * P[1][2];
+      }
     }
     __syncthreads();
     if (act) {
@@ -199,6 +231,14 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_rho64(RhoArgs a) {
           tt = C.n1 * am[t] + C.n2 * am[TA + t] + C.n3s * am[2 * TA + t];
           dt = am[2 * TA + t];
         }
+        // QUAD: kq(n3) = c0 + n3 (c1 + n3 c2) along the chunk
+        double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+        if (QUAD) {
+          const double n1 = C.n1, n2 = C.n2;
+          c0 = n1 * n1 * aqd[t] + n2 * n2 * aqd[TA + t] + n1 * n2 * aqd[3 * TA + t];
+          c1 = n1 * aqd[4 * TA + t] + n2 * aqd[5 * TA + t];
+          c2 = aqd[2 * TA + t];
+        }
 #pragma unroll
         for (int c = 0; c < KC_MAX; ++c) {
           if (c < KC) {
@@ -210,6 +250,12 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_rho64(RhoArgs a) {
               accR[c] = fma(-pi, tt, accR[c]);
               accI[c] = fma(pr, tt, accI[c]);
               tt += dt;
+            }
+            if (QUAD) {
+              const double n3 = (double)(C.n3s + c);
+              const double kq = c0 + n3 * (c1 + n3 * c2);
+              accR[c] = fma(-pr, kq, accR[c]);
+              accI[c] = fma(-pi, kq, accI[c]);
             }
             const double nr = pr * st.x - pi * st.y;
             pi = pr * st.y + pi * st.x;
@@ -225,9 +271,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_rho64(RhoArgs a) {
   }
 }
 
-template __global__ void k_rho64<true, false>(RhoArgs);
-template __global__ void k_rho64<false, true>(RhoArgs);
-template __global__ void k_rho64<true, true>(RhoArgs);
+template __global__ void k_rho64<true, false, false>(RhoArgs);
+template __global__ void k_rho64<false, true, false>(RhoArgs);
+template __global__ void k_rho64<true, true, false>(RhoArgs);
+template __global__ void k_rho64<false, true, true>(RhoArgs);
+template __global__ void k_rho64<true, true, true>(RhoArgs);
 
 // Sum of the atom-split partials, g = A_k rho, energy and stress per k
 // (src/field.f90:2742-2745, 2803-2815).  Partial rows: 0 u, 1..6 tau.
@@ -516,17 +564,35 @@ __device__ __forceinline__ void combine_atom(const CombineArgs& a, int i, double
 
 // SCF stopping criterion of atom i (src/field.f90:3378-3400): squared
 // deviations of mu / alpha from the new field, and the polarizable count.
-__device__ __forceinline__ void resid_atom(const CombineArgs& a, int i, double (&acc)[2]) {
-  const int n = a.n;
-  const int t = a.typ[i];
-  if (!a.T->polar[t]) return;
-  acc[1] += 1.0;
-  for (int r = 0; r < 3; ++r) {
-    const double pa = a.T->pol[t][r + 3 * r];
-    if (pa == 0.0) continue;
-    const double d = a.mu[i + r * n] / pa - a.E[i + r * n];
-    acc[0] += d * d;
+// the rows of the SCF stopping criterion of atom i (src/field.f90:3378-3400):
+// dipole terms on ldip_polar species, quadrupole terms (theta_aa /
+// polquad_aaa - E_a) on lquad_polar species, each species kind counted in
+// npol
+__device__ __forceinline__ void resid_terms(const TypeTables* T, int t, int n, int i,
+                                            const double* mu, const double* th,
+                                            const double* E, double (&acc)[2]) {
+  if (T->polar[t]) {
+    acc[1] += 1.0;
+    for (int r = 0; r < 3; ++r) {
+      const double pa = T->pol[t][r + 3 * r];
+      if (pa == 0.0) continue;
+      const double d = mu[i + r * n] / pa - E[i + r * n];
+      acc[0] += d * d;
+    }
   }
+  if (th && T->qpolar[t]) {
+    acc[1] += 1.0;
+    for (int r = 0; r < 3; ++r) {
+      const double pq = T->polq[t][r + 3 * (r + 3 * r)];
+      if (pq == 0.0) continue;
+      const double d = th[i + (size_t)(r + 3 * r) * n] / pq - E[i + r * n];
+      acc[0] += d * d;
+    }
+  }
+}
+
+__device__ __forceinline__ void resid_atom(const CombineArgs& a, int i, double (&acc)[2]) {
+  resid_terms(a.T, a.typ[i], a.n, i, a.mu, a.th, a.E, acc);
 }
 
 template <int MODE>
@@ -563,23 +629,51 @@ template __global__ void k_combine64<ES_FINAL>(CombineArgs);
 // atoms; partial row 0 = sum mu alpha^-1 mu (u_pol before the 1/2).
 // With mu4 (the FP32 SCF field of the mixed path), the moments are also
 // stored as float4 at the atom's spatial rank (k_pack_mu_ranked fused).
+// induced_moment (src/field.f90:1716-1803) of one atom: mu = alpha E on
+// ldip_polar or lquad_polar species, theta = polquad E (when th is given),
+// and its u_pol term: mu alpha^-1 mu / 2 on ldip_polar species,
+// sum theta_ab^2 / (6 polquad_111) on lquad_polar ones.
+__device__ __forceinline__ double induced_atom(const TypeTables* T, int t, int n, int i,
+                                               const double* E, double (&m)[3], double* th) {
+  double u = 0.0;
+  m[0] = m[1] = m[2] = 0.0;
+  if (!T->polar[t] && !T->qpolar[t]) {
+    if (th)
+      for (int k = 0; k < 9; ++k) th[i + (size_t)k * n] = 0.0;
+    return 0.0;
+  }
+  const double e[3] = {E[i], E[i + n], E[i + 2 * n]};
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) m[r] = m[r] + T->pol[t][r + 3 * c] * e[c];
+  if (T->polar[t]) {
+    double v = 0.0;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) v = v + m[r] * T->invpol[t][r + 3 * c] * m[c];
+    u = 0.5 * v;
+  }
+  if (th) {
+    double uq = 0.0;
+    for (int k = 0; k < 9; ++k) {
+      double v = 0.0;
+      for (int g = 0; g < 3; ++g) v = v + T->polq[t][g + 3 * k] * e[g];
+      th[i + (size_t)k * n] = v;
+      if (T->qpolar[t]) uq = uq + v * v / T->polq[t][0];
+    }
+    u = u + uq / 6.0;
+  }
+  return u;
+}
+
 __global__ __launch_bounds__(MDFF_BLOCK) void k_induced(
     int n, const int* __restrict__ typ, const TypeTables* __restrict__ T,
     const double* __restrict__ E, double* __restrict__ mu, double* __restrict__ part,
-    const int* __restrict__ rank, float4* __restrict__ mu4) {
+    const int* __restrict__ rank, float4* __restrict__ mu4, double* __restrict__ th) {
   __shared__ double red[4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[1] = {0.0};
   if (i < n) {
-    const int t = typ[i];
-    double m[3] = {0, 0, 0};
-    if (T->polar[t]) {
-      const double e[3] = {E[i], E[i + n], E[i + 2 * n]};
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) m[r] = m[r] + T->pol[t][r + 3 * c] * e[c];
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) acc[0] = acc[0] + m[r] * T->invpol[t][r + 3 * c] * m[c];
-    }
+    double m[3];
+    acc[0] = 2.0 * induced_atom(T, typ[i], n, i, E, m, th);  // halved with the sum
     mu[i] = m[0];
     mu[i + n] = m[1];
     mu[i + 2 * n] = m[2];
@@ -655,28 +749,16 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_extrap(ExtrapArgs a) {
       }
     }
     if (a.op == 5) {
+      // induced_moment of the new field (theta_ind as it leaves it), then
+      // the dipoles mixed with the predictor's (src/field.f90:3982-3987)
       const int t = a.typ[i];
-      double mm[3] = {0, 0, 0};
-      if (a.T->polar[t]) {
-        const double e[3] = {a.E[i], a.E[i + n], a.E[i + 2 * n]};
-        for (int r = 0; r < 3; ++r)
-          for (int c = 0; c < 3; ++c) mm[r] = mm[r] + a.T->pol[t][r + 3 * c] * e[c];
-        for (int r = 0; r < 3; ++r)
-          for (int c = 0; c < 3; ++c) acc[0] = acc[0] + mm[r] * a.T->invpol[t][r + 3 * c] * mm[c];
-      }
+      double mm[3];
+      acc[0] = 2.0 * induced_atom(a.T, t, n, i, a.E, mm, a.th);  // halved with the sum
       for (int r = 0; r < 3; ++r)
         a.mu[i + r * n] = a.w * mm[r] + (1.0 - a.w) * a.mup[i + r * n];
       // the stopping criterion of this iteration is taken on the corrected
       // moments (src/field.f90:3368-3400: the corrector runs before it)
-      if (a.rpart && a.T->polar[t]) {
-        rs[1] = 1.0;
-        for (int r = 0; r < 3; ++r) {
-          const double pa = a.T->pol[t][r + 3 * r];
-          if (pa == 0.0) continue;
-          const double d = a.mu[i + r * n] / pa - a.E[i + r * n];
-          rs[0] += d * d;
-        }
-      }
+      if (a.rpart) resid_terms(a.T, t, n, i, a.mu, a.th, a.E, rs);
     }
   }
   if (a.op == 5) {

@@ -35,8 +35,9 @@ enum {
   SC_DISP = 37,  // max squared displacement since the list build
   SC_CG = 40,    // 6: conjugate-gradient SCF, two (r.z, |r|^2, npol) triples
   SC_ESTAT = 46, // 2: sum of |E_stat|^2 over polarizable atoms, their count
-  SC_KIN = 48,   // sum m |v|^2 of the MD loop
-  SC_N = 50
+  SC_KIN = 48,   // sum m |v|^2 of the MD loop, then the thermostat energy
+  SC_QUAD = 50,  // real-space quadrupole energy of the final call
+  SC_N = 52
 };
 
 thread_local std::string g_create_error;
@@ -95,6 +96,16 @@ struct mdffgpu_ctx {
   // per-atom masses, thermostat state
   double *d_vel = nullptr, *d_fs = nullptr, *d_massia = nullptr;
   NhcState* d_nhc = nullptr;
+  // quadrupoles (FP64 path): permanent ones (quadia), induced ones of the
+  // SCF, their sum; any static quadrupole, any quadrupole polarizability,
+  // any quadrupole term at all
+  double *d_thstat = nullptr, *d_thind = nullptr, *d_theta = nullptr;
+  bool lqstat = false, lqpol = false, lquad = false;
+  bool quad_final = false;  // the last final call had quadrupole terms (SC_QUAD)
+  // Wannier-centre dipoles and the index + 1 of an ion missing centres
+  double* d_muwfc = nullptr;
+  int* d_wfcbad = nullptr;
+  bool lwannier = false;
   double mx_floor = 0.0;     // SCF residual floor of the mixed path (0 = unknown)
   // outcome of the last mixed-precision solve, folded into mx_floor once the
   // evaluation's scalars are on the host (update_floor)
@@ -170,7 +181,7 @@ const char* const kKernelNames[K_COUNT] = {
     "k_scf_resid", "k_extrap", "k_reduce_rows", "k_add", "k_pairs_t2", "k_dd_field",
     "mx_shortrange", "mx_es_dir",
     "mx_phase", "mx_rho", "mx_rho_reduce", "mx_apply", "mx_combine", "mx_weights", "mx_gprep",
-    "batch", "adjoint", "k_efg_es_dir", "k_efg_ds", "k_efg_combine", "md"};
+    "batch", "adjoint", "k_efg_es_dir", "k_efg_ds", "k_efg_combine", "md", "k_es_quad", "k_wfc"};
 
 ProfScope::ProfScope(mdffgpu_ctx* ctx, int kid) : c(ctx), id(kid), a(nullptr), b(nullptr) {
   if (!c || !c->prof || (c->prof_only != -1 && c->prof_only != kid)) return;
@@ -374,7 +385,7 @@ void build_tables(mdffgpu_ctx* c) {
   for (int k = 0; k < 48; ++k)
     if (c->P.dip[k] != 0.0) c->ldip_static = 1;
   for (int t = 0; t < c->P.ntype; ++t)
-    if (c->tt_h.polar[t]) c->linduced = 1;
+    if (c->tt_h.polar[t] || c->tt_h.qpolar[t]) c->linduced = 1;
   HCK(c, hipMemcpy(c->d_pt, &c->pt_h, sizeof(c->pt_h), hipMemcpyHostToDevice));
   HCK(c, hipMemcpy(c->d_tt, &c->tt_h, sizeof(c->tt_h), hipMemcpyHostToDevice));
 }
@@ -403,8 +414,38 @@ void validate(mdffgpu_ctx* c, const mdffgpu_params* p) {
     fail(c, MDFFGPU_ERR_ARG, "SCF iteration bounds must be non-negative");
   if (p->scf_solver != MDFFGPU_SCF_JACOBI && p->scf_solver != MDFFGPU_SCF_PCG)
     fail(c, MDFFGPU_ERR_ARG, "unknown scf_solver");
-  if (p->lcoulomb && (p->task_coul_quad[0] || p->task_coul_quad[1] || p->task_coul_quad[2]))
-    fail(c, MDFFGPU_ERR_UNSUPPORTED, "quadrupole Coulomb tasks (task_coul(4:6)) are not supported");
+  if (p->lcoulomb) {
+    bool anyq = p->task_coul_quad[0] || p->task_coul_quad[1] || p->task_coul_quad[2];
+    bool qpol = false;
+    for (int t = 0; t < p->ntype; ++t) {
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          const double v = p->quad[t + 16 * (a + 3 * b)];
+          if (v != 0.0) anyq = true;
+          if (v != p->quad[t + 16 * (b + 3 * a)])
+            fail(c, MDFFGPU_ERR_ARG, "quad(%d,:,:) is not symmetric", t + 1);
+          for (int g = 0; g < 3; ++g)
+            if (p->polquad[t + 16 * (g + 3 * (a + 3 * b))] !=
+                p->polquad[t + 16 * (g + 3 * (b + 3 * a))])
+              fail(c, MDFFGPU_ERR_ARG, "polquad(%d,g,:,:) is not symmetric", t + 1);
+        }
+      if (p->lquad_polar[t]) {
+        qpol = anyq = true;
+        // u_pol divides by polquad(it,1,1,1) (src/field.f90:1793)
+        if (p->polquad[t] == 0.0)
+          fail(c, MDFFGPU_ERR_ARG, "lquad_polar(%d) with polquad(%d,1,1,1) = 0", t + 1, t + 1);
+      }
+    }
+    if (anyq && p->precision != MDFFGPU_PREC_FP64)
+      fail(c, MDFFGPU_ERR_UNSUPPORTED, "quadrupoles run on the FP64 path only");
+    if (qpol && p->scf_solver != MDFFGPU_SCF_JACOBI)
+      fail(c, MDFFGPU_ERR_UNSUPPORTED,
+           "quadrupole polarizabilities need the Jacobi SCF (scf_solver = 0)");
+    bool wfc = false;
+    for (int t = 0; t < p->ntype; ++t)
+      if (p->lwfc[t] > 0) wfc = true;
+    if (wfc && !(p->rcut_wfc > 0.0)) fail(c, MDFFGPU_ERR_ARG, "rcut_wfc must be positive");
+  }
   // Tang-Toennies orders (src/field.f90:355-358), before any device work
   std::unique_ptr<PairTables> pt(new PairTables());
   std::unique_ptr<TypeTables> ty(new TypeTables());
@@ -542,8 +583,10 @@ void es_ind_dir(mdffgpu_ctx* c, const double* mu, bool packed = false) {
 
 // One Ewald call; with dir_done (ES_IND only) the real-space part has
 // already been launched (es_ind_dir) for these dipoles.
+// th: quadrupoles (n,3,3) of the call, or null; their terms run with the
+// TASK_Q*QUAD bits of task (FP64 path only).
 void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int task,
-             int do_efield, int do_efg, bool dir_done = false) {
+             int do_efield, int do_efg, bool dir_done = false, const double* th = nullptr) {
   const int n = c->n;
   hipStream_t st = c->st;
   const mdffgpu_params& P = c->P;
@@ -604,6 +647,38 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   }
   check_launch(c, "k_es_dir");
   if (mode == ES_FINAL) reduce(c, 7, gbl, SC_DIR);
+  const bool quad = th && (task & TASK_QUAD_ANY);
+  if (mode == ES_FINAL) c->quad_final = quad;
+  if (quad) {
+    // quadrupole terms of the real-space sum, added to its field and EFG
+    QuadArgs qa;
+    qa.n = n;
+    qa.maxnb = c->maxnb;
+    qa.s = c->d_s;
+    qa.s4 = c->d_s4;
+    qa.cnt = c->d_cnt;
+    qa.nbr = c->d_nbr;
+    qa.perm = c->d_perm;
+    qa.cell = c->cell;
+    qa.cut2 = P.cutlongrange * P.cutlongrange;
+    qa.alpha = P.alphaES;
+    qa.q = q;
+    qa.mu = mu;
+    qa.th = th;
+    qa.task = task;
+    qa.do_energy = (mode == ES_FINAL);
+    qa.do_efield = (mode != ES_FINAL) || do_efield;
+    qa.do_efg = (mode == ES_FINAL) && do_efg;
+    qa.ef = c->d_efdir;
+    qa.efg = c->d_efgdir;
+    qa.part = c->d_part;
+    {
+      ProfScope _ps(c, K_QUAD);
+      k_es_quad<<<gbl, MDFF_BLOCK, 0, st>>>(qa);
+    }
+    check_launch(c, "k_es_quad");
+    if (qa.do_energy) reduce(c, 1, gbl, SC_QUAD);
+  }
   CombineArgs ca;
   std::memset(&ca, 0, sizeof(ca));
   ca.n = n;
@@ -634,6 +709,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     ca.T = c->d_tt;
     ca.rpart = c->d_part;
     ca.cg_Ap = c->cg_Ap;
+    ca.th = c->lqpol ? c->d_thind : nullptr;
   }
   // reciprocal space
   const bool chg = (mode != ES_IND);
@@ -661,14 +737,23 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     ra.eph = c->d_eph;
     ra.q = q;
     ra.mu = mu;
+    ra.th = th;
     ra.cell = c->cell;
     ra.atoms_per_split = c->rho_atoms;
     ra.kpad = c->kpad;
     ra.part = c->d_rpart;
     dim3 g((c->nchunk + MDFF_BLOCK - 1) / MDFF_BLOCK, c->rho_split);
     size_t lds = (size_t)RHO_TILE * (c->kmax[0] + c->kmax[1] + c->kmax[2] + 3) * sizeof(double2) +
-                 4 * RHO_TILE * sizeof(double);
-    if (chg && dip)
+                 10 * RHO_TILE * sizeof(double);
+    // the quadrupoles enter rho only together with the dipoles
+    // (src/field.f90:2723-2740 sit behind the ldip test)
+    if (quad && dip) {
+      ProfScope _ps(c, chg ? K_RHO_BOTH : K_RHO_DIP);
+      if (chg)
+        k_rho64<true, true, true><<<g, MDFF_BLOCK, lds, st>>>(ra);
+      else
+        k_rho64<false, true, true><<<g, MDFF_BLOCK, lds, st>>>(ra);
+    } else if (chg && dip)
       {
         ProfScope _ps(c, K_RHO_BOTH);
         k_rho64<true, true><<<g, MDFF_BLOCK, lds, st>>>(ra);
@@ -830,6 +915,7 @@ void extrap(mdffgpu_ctx* c, int op, int ord, double w, double* rpart = nullptr) 
   ea.T = c->d_tt;
   ea.part = c->d_upart;
   ea.rpart = rpart;
+  ea.th = c->lqpol ? c->d_thind : nullptr;
   {
     ProfScope _ps(c, K_EXTRAP);
     k_extrap<<<nblk(c->n), MDFF_BLOCK, 0, c->st>>>(ea);
@@ -853,12 +939,17 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es,
   hipStream_t st = c->st;
   c->cg_Ap = nullptr;
   HCK(c, hipMemsetAsync(c->d_muind, 0, 3 * (size_t)n * sizeof(double), st));
+  if (c->lqpol) HCK(c, hipMemsetAsync(c->d_thind, 0, 9 * (size_t)n * sizeof(double), st));
   *n_scf = 0;
   *rmsd_out = 0.0;
   *tol_out = P.conv_tol_ind;
   if (!c->linduced) return;
-  int task_static = TASK_QQ | (c->ldip_static ? (TASK_QD | TASK_DD) : 0);
-  es_call(c, ES_STATIC, c->d_q, c->d_dip, task_static, 1, 0);
+  // static moments: every task when there are static quadrupoles
+  // (src/field.f90:3255-3268)
+  int task_static = TASK_QQ | ((c->ldip_static || c->lqstat) ? (TASK_QD | TASK_DD) : 0) |
+                    (c->lqstat ? TASK_QUAD_ANY : 0);
+  es_call(c, ES_STATIC, c->d_q, c->d_dip, task_static, 1, 0, false,
+          c->lqstat ? c->d_thstat : nullptr);
   ++*n_es;
   HCK(c, hipMemcpyAsync(c->d_E, c->d_Estat, 3 * (size_t)n * sizeof(double),
                         hipMemcpyDeviceToDevice, st));
@@ -907,7 +998,8 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es,
       ProfScope _ps(c, K_INDUCED);
       k_induced<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_typ, c->d_tt, c->d_E, c->d_muind, c->d_upart,
                                            c->d_rank,
-                                           pack ? reinterpret_cast<float4*>(c->d_mu4) : nullptr);
+                                           pack ? reinterpret_cast<float4*>(c->d_mu4) : nullptr,
+                                           c->lqpol ? c->d_thind : nullptr);
     }
     check_launch(c, "k_induced");
     upol_new = true;
@@ -1056,7 +1148,10 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es,
     } else {
       extrap(c, 3, aspc_order(P, itime), 0.0);
     }
-    es_call(c, ES_IND, c->d_qzero, c->d_muind, TASK_DD, 1, 0, queued);
+    // induced moments: dipole-dipole and quadrupole-quadrupole tasks
+    // (src/field.f90:3304-3306)
+    es_call(c, ES_IND, c->d_qzero, c->d_muind, TASK_DD | (c->lqpol ? TASK_QUADQ : 0), 1, 0,
+            queued, c->lqpol ? c->d_thind : nullptr);
     queued = false;
     ++*n_es;
     if (iscf == 1 && P.calc_mode == 1 && P.algo_ext_dipole == MDFFGPU_EXT_ASPC) {
@@ -1322,9 +1417,42 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
       k_add<<<nblk(3 * n), MDFF_BLOCK, 0, st>>>(3 * n, c->d_dip, c->d_muind, c->d_mu);
     }
     check_launch(c, "k_add");
+    if (c->lquad) {
+      // theta = quadia + theta_ind (get_moments, src/field.f90:3744)
+      ProfScope _ps(c, K_ADD);
+      k_add<<<nblk(9 * n), MDFF_BLOCK, 0, st>>>(9 * n, c->d_thstat, c->d_thind, c->d_theta);
+    }
+    if (c->lwannier) {
+      // moment_from_WFc after the SCF (src/field.f90:3723-3737)
+      HCK(c, hipMemsetAsync(c->d_wfcbad, 0, sizeof(int), st));
+      WfcArgs wa;
+      wa.n = n;
+      wa.maxnb = c->maxnb;
+      wa.s = c->d_s;
+      wa.typ = c->d_typ;
+      // the list holds every pair within list_cut; centres farther out
+      // need the scan of all atoms
+      const bool use_list = c->list_cut > 0.0 && P.rcut_wfc <= c->list_cut * c->list_cut;
+      wa.cnt = use_list ? c->d_cnt : nullptr;
+      wa.nbr = use_list ? c->d_nbr : nullptr;
+      for (int t = 0; t < MDFF_NT; ++t) wa.lwfc[t] = P.lwfc[t];
+      wa.cell = c->cell;
+      wa.rcut = P.rcut_wfc;
+      wa.mu = c->d_muwfc;
+      wa.bad = c->d_wfcbad;
+      {
+        ProfScope _ps(c, K_WFC);
+        k_wfc<<<gb, MDFF_BLOCK, 0, st>>>(wa);
+      }
+      check_launch(c, "k_wfc");
+      HCK(c, hipMemcpyAsync(c->h_ovf, c->d_wfcbad, sizeof(int), hipMemcpyDeviceToHost, st));
+      if (P.ldip_wfc) k_add<<<nblk(3 * n), MDFF_BLOCK, 0, st>>>(3 * n, c->d_mu, c->d_muwfc, c->d_mu);
+    }
     int task = (P.task_coul[0] ? TASK_QQ : 0) | (P.task_coul[1] ? TASK_QD : 0) |
-               (P.task_coul[2] ? TASK_DD : 0);
-    es_call(c, ES_FINAL, c->d_q, c->d_mu, task, P.doefield, P.doefg);
+               (P.task_coul[2] ? TASK_DD : 0) | (P.task_coul_quad[0] ? TASK_QQUAD : 0) |
+               (P.task_coul_quad[1] ? TASK_DQUAD : 0) | (P.task_coul_quad[2] ? TASK_QUADQ : 0);
+    es_call(c, ES_FINAL, c->d_q, c->d_mu, task, P.doefield, P.doefg, false,
+            c->lquad ? c->d_theta : nullptr);
     ++n_es;
   }
   // device outputs queued ahead of the one synchronisation of the call
@@ -1343,6 +1471,11 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
     c->u_pol = 0.5 * c->h_sc[SC_UPOL];
     c->upol_pending = false;
   }
+  if (P.lcoulomb && c->lwannier && *c->h_ovf != 0)
+    fail(c, MDFFGPU_ERR_WFC,
+         "moment_from_WFc: wannier center is missing for atom %d (fewer or more than lwfc "
+         "centres with r^2 < rcut_wfc)",
+         *c->h_ovf);
   const double* h = c->h_sc;
   const double omega = c->cell.omega;
   if (sr) {
@@ -1365,7 +1498,7 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
     const double alpha = P.alphaES, alpha2 = alpha * alpha;
     const double selfa = alpha / std::sqrt(MDFF_PI);
     const double selfa2 = 2.0 * selfa * alpha2 / 3.0;
-    const double u_dir = h[SC_DIR];
+    const double u_dir = h[SC_DIR] + (c->quad_final ? h[SC_QUAD] : 0.0);
     const double u_rec = h[SC_REC] * 2.0 * tpi_V;
     const double* m = h + SC_MOM;
     const double u_self = -selfa * m[6] - selfa2 * m[7];
@@ -1642,6 +1775,38 @@ void efg_frame(mdffgpu_ctx* c, const mdffgpu_efg_opts& o, mdffgpu_energies* en) 
 // The context of mdffgpu_create; with arena (the allocation-free entry
 // point) every device buffer comes from that memory instead of hipMalloc,
 // the host scalars are pageable and the work goes to the caller's stream.
+// Quadrupole and Wannier-centre state (FP64 path): the permanent
+// quadrupoles of typeinfo_init (quadia, src/read_pos.f90:197), buffers of
+// the induced and total quadrupoles, the Wannier-centre dipoles.
+void setup_quad_wfc(mdffgpu_ctx* c) {
+  const mdffgpu_params& P = c->P;
+  const int n = c->n;
+  c->lqstat = c->lqpol = c->lwannier = false;
+  for (int k = 0; k < 144; ++k)
+    if (P.quad[k] != 0.0) c->lqstat = true;
+  for (int t = 0; t < P.ntype; ++t) {
+    if (P.lquad_polar[t]) c->lqpol = true;
+    if (P.lwfc[t] > 0) c->lwannier = true;
+  }
+  c->lquad = c->lqstat || c->lqpol || P.task_coul_quad[0] || P.task_coul_quad[1] ||
+             P.task_coul_quad[2];
+  if (c->lquad) {
+    dalloc(c, &c->d_thstat, 9 * (size_t)n);
+    dalloc(c, &c->d_thind, 9 * (size_t)n);
+    dalloc(c, &c->d_theta, 9 * (size_t)n);
+    std::vector<double> th(9 * (size_t)n);
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 9; ++k) th[i + (size_t)n * k] = P.quad[c->typ0[i] + 16 * k];
+    HCK(c, hipMemcpy(c->d_thstat, th.data(), th.size() * sizeof(double), hipMemcpyHostToDevice));
+    HCK(c, hipMemsetAsync(c->d_thind, 0, 9 * (size_t)n * sizeof(double), c->st));
+    HCK(c, hipMemsetAsync(c->d_theta, 0, 9 * (size_t)n * sizeof(double), c->st));
+  }
+  if (c->lwannier) {
+    dalloc(c, &c->d_muwfc, 3 * (size_t)n);
+    dalloc(c, &c->d_wfcbad, 1);
+  }
+}
+
 void create_impl(const mdffgpu_params* params, const int32_t* itype, const double* A,
                  mdffgpu_ctx** out, char* arena = nullptr, size_t arena_cap = 0,
                  hipStream_t stream = nullptr) {
@@ -1689,6 +1854,7 @@ void create_impl(const mdffgpu_params* params, const int32_t* itype, const doubl
     }
     HCK(c, hipMemcpy(c->d_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
     HCK(c, hipMemcpy(c->d_dip, dip.data(), dip.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (c->P.lcoulomb) setup_quad_wfc(c);
     update_cutoffs(c);
     if (c->list_cut > 0.0) set_list_capacity(c, initial_capacity(c));
     if (c->P.lcoulomb) build_kmesh(c);
@@ -1838,6 +2004,8 @@ int make_tables(const mdffgpu_params& P, PairTables& T, TypeTables& Y, TTTable& 
       for (int b = 0; b < 3; ++b) Y.pol[t][a + 3 * b] = P.poldip[t + 16 * (a + 3 * b)];
     Y.polar[t] = P.ldip_polar[t] ? 1 : 0;
     if (Y.polar[t]) inv3(Y.pol[t], Y.invpol[t]);
+    for (int k = 0; k < 27; ++k) Y.polq[t][k] = P.polquad[t + 16 * k];
+    Y.qpolar[t] = P.lquad_polar[t] ? 1 : 0;
   }
   tt.e[0] = 1.0;
   for (int k = 1; k <= 8; ++k) tt.e[k] = tt.e[k - 1] / (double)k;
@@ -1968,7 +2136,8 @@ void mdffgpu_destroy(mdffgpu_ctx* c) {
   void* vp[] = {c->d_eph, c->d_typ, c->d_cnt, c->d_nbr, c->d_ovf, c->d_rpart, c->d_g,
                 c->d_chunks, c->d_pt, c->d_tt, c->d_mu4, c->d_s4, c->d_perm, c->d_rank,
                 c->d_nlr, c->d_cellof, c->d_cstart, c->d_ccur, c->d_vel, c->d_fs,
-                c->d_massia, c->d_nhc};
+                c->d_massia, c->d_nhc, c->d_thstat, c->d_thind, c->d_theta, c->d_muwfc,
+                c->d_wfcbad};
   for (void* p : vp)
     if (p) (void)hipFree(p);
   for (auto& p : c->prof_pending) {
@@ -2020,8 +2189,13 @@ int mdffgpu_engforce(mdffgpu_ctx* c, const double* rx, const double* ry, const d
       if (ef_t) HCK(c, hipMemcpyAsync(ef_t, c->d_ef, 3 * b, hipMemcpyDeviceToHost, c->st));
       if (efg_t) HCK(c, hipMemcpyAsync(efg_t, c->d_efg, 9 * b, hipMemcpyDeviceToHost, c->st));
     }
+    if (theta_t) {
+      if (c->P.lcoulomb && c->lquad)
+        HCK(c, hipMemcpyAsync(theta_t, c->d_theta, 9 * b, hipMemcpyDeviceToHost, c->st));
+      else
+        std::memset(theta_t, 0, 9 * b);
+    }
     HCK(c, hipStreamSynchronize(c->st));
-    if (theta_t) std::memset(theta_t, 0, 9 * b);
     return MDFFGPU_OK;
   } catch (Fail& f) {
     (void)hipStreamSynchronize(c->st);
@@ -2372,6 +2546,11 @@ int mdffgpu_efg(mdffgpu_ctx* c, const mdffgpu_efg_opts* o, const double* rx, con
   }
   if (o->lefg_old) {
     c->err = "lefg_old (charge-only efg_ES / efg_DS) is not supported";
+    return MDFFGPU_ERR_UNSUPPORTED;
+  }
+  if (c->lquad || c->lwannier) {
+    // multipole_efg_ES / _DS take charges and dipoles (src/efg.f90:1278-1625)
+    c->err = "the standalone EFG takes charges and dipoles, not quadrupoles or Wannier centres";
     return MDFFGPU_ERR_UNSUPPORTED;
   }
   if (o->longrange != 0 && o->longrange != 1) {

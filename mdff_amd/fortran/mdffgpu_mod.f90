@@ -43,6 +43,12 @@ module mdffgpu_mod
     integer(c_int32_t) :: mixed_fp64_parts
     integer(c_int32_t) :: scf_solver
     integer(c_int32_t) :: task_coul_quad(3), reserved(2)
+    real(c_double) :: quad(16, 3, 3)
+    real(c_double) :: polquad(16, 3, 3, 3)
+    integer(c_int32_t) :: lquad_polar(16)
+    integer(c_int32_t) :: lwfc(16)
+    integer(c_int32_t) :: ldip_wfc, reserved2
+    real(c_double) :: rcut_wfc
   end type
 
   type, bind(C), public :: mdffgpu_energies

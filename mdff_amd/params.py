@@ -108,6 +108,13 @@ class MdffParams(ctypes.Structure):
         ("scf_solver", ctypes.c_int32),
         ("task_coul_quad", ctypes.c_int32 * 3),
         ("reserved", ctypes.c_int32 * 2),
+        ("quad", ctypes.c_double * 144),
+        ("polquad", ctypes.c_double * 432),
+        ("lquad_polar", ctypes.c_int32 * 16),
+        ("lwfc", ctypes.c_int32 * 16),
+        ("ldip_wfc", ctypes.c_int32),
+        ("reserved2", ctypes.c_int32),
+        ("rcut_wfc", ctypes.c_double),
     ]
 
 
@@ -167,6 +174,8 @@ def field_defaults(natm: int, ntype: int) -> MdffParams:
     p.alphaES = 1.0
     p.kES[:] = [10, 10, 10]
     p.conv_tol_ind = 1e-6
+    p.ldip_wfc = 1  # src/field.f90:266-267
+    p.rcut_wfc = 0.5
     p.min_scf_pol_iter = 3
     p.max_scf_pol_iter = 100
     p.extrapolate_order = 0
@@ -185,7 +194,18 @@ _PAIR_TABLES = ("plj", "qlj", "epslj", "sigmalj", "epsmor", "sigmamor", "rhomor"
                 "Abmhftd", "Bbmhftd", "Cbmhftd", "Dbmhftd", "BDbmhftd")
 
 
-def field_check_tag(p: MdffParams, poldip_iso=None, symmetric_pot: bool = True) -> None:
+def qidx(it: int, a: int, b: int) -> int:
+    """Flat index of quad(it, a, b), all 1-based."""
+    return (it - 1) + NTYPEMAX * ((a - 1) + 3 * (b - 1))
+
+
+def pqidx(it: int, g: int, a: int, b: int) -> int:
+    """Flat index of polquad(it, g, a, b), all 1-based."""
+    return (it - 1) + NTYPEMAX * ((g - 1) + 3 * ((a - 1) + 3 * (b - 1)))
+
+
+def field_check_tag(p: MdffParams, poldip_iso=None, symmetric_pot: bool = True,
+                    polquad_iso=None) -> None:
     """Parameter checks and derived flags of field_check_tag
     (src/field.f90:282-433).
 
@@ -196,7 +216,10 @@ def field_check_tag(p: MdffParams, poldip_iso=None, symmetric_pot: bool = True) 
     * Tang-Toennies orders above 8 stop the reference (:355-358): ValueError;
     * task_coul from the non-zero charges, static dipoles and dipole
       polarizabilities (:367-399);
-    * isotropic polarizabilities expand to a diagonal tensor (:418-431).
+    * isotropic polarizabilities expand to a diagonal tensor (:418-431),
+      the quadrupole ones to polquad(g,g,g) = polquad_iso (:422-426);
+    * task_coul(4:6) from the quadrupoles and quadrupole polarizabilities
+      (:376, 383-395).
     """
     nt = p.ntype
     if symmetric_pot:
@@ -221,13 +244,25 @@ def field_check_tag(p: MdffParams, poldip_iso=None, symmetric_pot: bool = True) 
                 for x in range(3):
                     for y in range(3):
                         p.poldip[it + NTYPEMAX * (x + 3 * y)] = a if x == y else 0.0
+    if polquad_iso is not None:
+        for it, a in enumerate(polquad_iso):
+            if a != 0.0:
+                for k in range(27):
+                    p.polquad[it + NTYPEMAX * k] = 0.0
+                for g in range(1, 4):
+                    p.polquad[pqidx(it + 1, g, g, g)] = a
     if p.lcoulomb:
         lqch = any(p.qch[k] != 0.0 for k in range(16))
         ldip = any(p.dip[k] != 0.0 for k in range(48))
         ldip = ldip or any(p.ldip_polar[it] for it in range(nt))
+        lqua = any(p.quad[k] != 0.0 for k in range(144))
+        lqua = lqua or any(p.lquad_polar[it] for it in range(nt))
         p.task_coul[0] = int(lqch)
         p.task_coul[1] = int(ldip and lqch)
         p.task_coul[2] = int(ldip)
+        p.task_coul_quad[0] = int(lqua and lqch)
+        p.task_coul_quad[1] = int(lqua and ldip)
+        p.task_coul_quad[2] = int(lqua)
 
 
 def lattice(A) -> dict:

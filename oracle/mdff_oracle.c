@@ -418,11 +418,14 @@ static void es_free(es_ctx* e) {
   free(e->s);
 }
 
-/* multipole_ES_dir, src/field.f90:2057-2643 (charges and dipoles; the
-   quadrupole tasks are not supported and their terms are zero here).
-   mu is (natm,3) Fortran order; ef (natm,3), efg (natm,3,3), f (natm) are
-   accumulated; tau (9) accumulated before the final scaling. */
+/* multipole_ES_dir, src/field.f90:2057-2643.  mu is (natm,3) and th
+   (natm,3,3) Fortran order (th may be NULL: no quadrupoles); task[6] as
+   task_coul(1:6).  ef (natm,3), efg (natm,3,3), f (natm) are accumulated;
+   tau (9) accumulated before the final scaling.  The quadrupole terms
+   (:2542-2601) carry energy, and for quadrupole-quadrupole field and EFG,
+   but no force or stress, as in the reference. */
 static void es_dir(const es_ctx* e, const double* q, const double* mu,
+                   const double* th,
                    const int* task, int damp_ind, int do_efield, int do_efg,
                    int do_forces, int do_stress, double* u_dir, double* ef,
                    double* efg, double* fx, double* fy, double* fz,
@@ -433,9 +436,11 @@ static void es_dir(const es_ctx* e, const double* q, const double* mu,
   double alpha = p->alphaES;
   double cutsq = p->cutlongrange * p->cutlongrange;
   double alpha2 = alpha * alpha, alpha3 = alpha2 * alpha, alpha5 = alpha3 * alpha2;
+  double alpha7 = alpha5 * alpha2;
   double piroot = sqrt(PI);
   double u = 0.0;
   double taud[9] = {0};
+  const int lq = th && (task[3] || task[4] || task[5]);
 #define EF(i, a) ef[(i) + n * (a)]
 #define EFG(i, a, b) efg[(i) + n * ((a) + 3 * (b))]
   for (int ia = 0; ia < n; ++ia) {
@@ -639,6 +644,72 @@ static void es_dir(const es_ctx* e, const double* q, const double* mu,
                     taud[a + 3 * b] + (rij[a] * fij[b] + rij[b] * fij[a]);
         }
       }
+      if (lq) {
+        double quadi[3][3], quadj[3][3];
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            quadi[a][b] = th[ia + n * (a + 3 * b)];
+            quadj[a][b] = th[ja + n * (a + 3 * b)];
+          }
+        /* charge-quadrupole, :2542-2551 */
+        if (task[3])
+          for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < 3; ++j) {
+              u = u + qi * T2[k][j] * quadj[k][j] / 3.0;
+              u = u + qj * T2[k][j] * quadi[k][j] / 3.0;
+            }
+        /* dipole-quadrupole, :2552-2565 */
+        if (task[4])
+          for (int l = 0; l < 3; ++l)
+            for (int k = 0; k < 3; ++k)
+              for (int j = 0; j < 3; ++j) {
+                u = u + mui[l] * T3[l][k][j] * quadj[k][j] / 3.0;
+                u = u - muj[l] * T3[l][k][j] * quadi[k][j] / 3.0;
+              }
+        /* quadrupole-quadrupole, :2566-2601, T4 of :2298-2318 */
+        if (task[5]) {
+          double d7 = d5 * d2, dm9 = dm7 / d2;
+          double F4 = F3 + 16.0 * alpha7 * d7 * expon / 105.0;
+          double F4_dm9 = dm9 * F4 * 105.0;
+          double T4[3][3][3][3];
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+              for (int g = 0; g < 3; ++g)
+                for (int h = 0; h < 3; ++h) {
+                  double v = rij[a] * rij[b] * rij[g] * rij[h] * F4_dm9;
+                  if (g == h) v = v - rij[a] * rij[b] * F3_dm7;
+                  if (b == h) v = v - rij[a] * rij[g] * F3_dm7;
+                  if (b == g) v = v - rij[a] * rij[h] * F3_dm7;
+                  if (a == h) v = v - rij[b] * rij[g] * F3_dm7;
+                  if (a == g) v = v - rij[b] * rij[h] * F3_dm7;
+                  if (a == b) v = v - rij[g] * rij[h] * F3_dm7;
+                  if (a == b && g == h) v = v + F2_dm5;
+                  if (a == g && b == h) v = v + F2_dm5;
+                  if (a == h && b == g) v = v + F2_dm5;
+                  T4[a][b][g][h] = v;
+                }
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+              for (int g = 0; g < 3; ++g)
+                for (int h = 0; h < 3; ++h)
+                  u = u + quadi[a][b] * T4[a][b][g][h] * quadj[g][h] / 9.0;
+          if (do_efield)
+            for (int k = 0; k < 3; ++k)
+              for (int j = 0; j < 3; ++j)
+                for (int a = 0; a < 3; ++a) {
+                  EF(ia, a) = EF(ia, a) - T3[a][k][j] * quadj[k][j];
+                  EF(ja, a) = EF(ja, a) + T3[a][k][j] * quadi[k][j];
+                }
+          if (do_efg)
+            for (int k = 0; k < 3; ++k)
+              for (int j = 0; j < 3; ++j)
+                for (int b = 0; b < 3; ++b)
+                  for (int a = 0; a < 3; ++a) {
+                    EFG(ia, a, b) = EFG(ia, a, b) + T4[a][b][k][j] * quadj[k][j];
+                    EFG(ja, a, b) = EFG(ja, a, b) - T4[a][b][k][j] * quadi[k][j];
+                  }
+        }
+      }
     }
   }
   if (do_stress)
@@ -651,7 +722,7 @@ static void es_dir(const es_ctx* e, const double* q, const double* mu,
 /* multipole_ES_rec, src/field.f90:2646-2873 (Cartesian positions) */
 static void es_rec(const es_ctx* e, const double* rx, const double* ry,
                    const double* rz, const double* q, const double* mu,
-                   const int* task, int do_efield, int do_efg, int do_forces,
+                   const double* th, const int* task, int do_efield, int do_efg, int do_forces,
                    int do_stress, double* u_rec, double* ef, double* efg,
                    double* fx, double* fy, double* fz, double* tau) {
   const mdffgpu_params* p = e->p;
@@ -659,6 +730,9 @@ static void es_rec(const es_ctx* e, const double* rx, const double* ry,
   double tpi_V = 2.0 * PI / e->cell.omega;
   double fpi_V = tpi_V * 2.0;
   int ldip = task[1] || task[2];
+  /* the quadrupoles enter rho only with ldip (:2723-2740 sit behind the
+     ldip cycle) */
+  int lquad = th && (task[3] || task[4] || task[5]);
   double* ckr = (double*)malloc(2 * (size_t)n * sizeof(double));
   double* skr = ckr + n;
   double u = 0.0;
@@ -681,6 +755,15 @@ static void es_rec(const es_ctx* e, const double* rx, const double* ry,
       double kmu = (mu[ia] * kx + mu[ia + n] * ky + mu[ia + 2 * n] * kz);
       rR = rR - kmu * skr[ia];
       rI = rI + kmu * ckr[ia];
+      if (!lquad) continue;
+      double txx = th[ia], tyy = th[ia + 4 * n], tzz = th[ia + 8 * n];
+      double txy = th[ia + 3 * n], txz = th[ia + 6 * n], tyz = th[ia + 7 * n];
+      double kq = txx * kx * kx + txy * kx * ky + txz * kx * kz;
+      kq = kq + txy * kx * ky + tyy * ky * ky + tyz * ky * kz;
+      kq = kq + txz * kz * kx + tyz * ky * kz + tzz * kz * kz;
+      kq = kq / 3.0;
+      rR = rR - kq * ckr[ia];
+      rI = rI - kq * skr[ia];
     }
     double str = (rR * rR + rI * rI) * Ak;
     u = u + str;
@@ -753,7 +836,7 @@ static void es_rec(const es_ctx* e, const double* rx, const double* ry,
 /* multipole_ES, src/field.f90:1817-2054.  ef/efg overwritten, f added. */
 static int es_call(const es_ctx* e, const double* rx, const double* ry,
                    const double* rz, const double* q, const double* mu,
-                   const int* task, int damp_ind, int do_efield, int do_efg,
+                   const double* th, const int* task, int damp_ind, int do_efield, int do_efg,
                    int do_forces, int do_stress, double u_pol, double* ef,
                    double* efg, double* fx, double* fy, double* fz,
                    double* u_coul, double* tau_coul, double* pvirial,
@@ -786,9 +869,9 @@ static int es_call(const es_ctx* e, const double* rx, const double* ry,
   double selfa = p->alphaES / sqrt(PI);
   double selfa2 = 2.0 * selfa * alpha2 / 3.0;
   double u_dir = 0.0, u_rec = 0.0, tau_dir[9] = {0}, tau_rec[9] = {0};
-  es_dir(e, q, mu, task, damp_ind, do_efield, do_efg, do_forces, do_stress,
+  es_dir(e, q, mu, th, task, damp_ind, do_efield, do_efg, do_forces, do_stress,
          &u_dir, ef_dir, efg_dir, fdx, fdy, fdz, tau_dir);
-  es_rec(e, rx, ry, rz, q, mu, task, do_efield, do_efg, do_forces, do_stress,
+  es_rec(e, rx, ry, rz, q, mu, th, task, do_efield, do_efg, do_forces, do_stress,
          &u_rec, ef_rec, efg_rec, frx, fry, frz, tau_rec);
   double u_surf = (qtot[0] * qtot[0] + qtot[1] * qtot[1] + qtot[2] * qtot[2]) +
                   2.0 * (qtot[0] * mutot[0] + qtot[1] * mutot[1] + qtot[2] * mutot[2]) +
@@ -841,6 +924,7 @@ static int es_call(const es_ctx* e, const double* rx, const double* ry,
 int orc_multipole_es(const mdffgpu_params* p, const int* itype,
                      const double* A, const double* rx, const double* ry,
                      const double* rz, const double* q, const double* mu,
+                     const double* theta,
                      const int* task, int damp_ind, int do_efield, int do_efg,
                      int do_forces, int do_stress, double u_pol, double* ef,
                      double* efg, double* fx, double* fy, double* fz,
@@ -853,7 +937,7 @@ int orc_multipole_es(const mdffgpu_params* p, const int* itype,
   es_ctx e;
   int rc = es_init(&e, p, it0, A, rx, ry, rz);
   if (!rc)
-    rc = es_call(&e, rx, ry, rz, q, mu, task, damp_ind, do_efield, do_efg,
+    rc = es_call(&e, rx, ry, rz, q, mu, theta, task, damp_ind, do_efield, do_efg,
                  do_forces, do_stress, u_pol, ef, efg, fx, fy, fz, u_coul,
                  tau_coul, pvirial, u_parts);
   es_free(&e);
@@ -896,20 +980,28 @@ static void inv3(const double* m, double* out) {
     for (int j = 0; j < 3; ++j) out[i + 3 * j] = a[i][j + 3];
 }
 
-/* induced_moment, src/field.f90:1716-1803 (dipoles; no quadrupole
-   polarizability).  E and mu are (natm,3). */
+/* induced_moment, src/field.f90:1716-1803.  E and mu are (natm,3); th
+   (natm,3,3) receives theta_ind = sum_g polquad(g,a,b) E_g (may be NULL
+   when no species has lquad_polar). */
 static void induced_moment(const mdffgpu_params* p, const int* it,
-                           const double* E, double* mu, double* u_pol) {
+                           const double* E, double* mu, double* th, double* u_pol) {
   int n = p->natm;
   double u = 0.0;
   for (int ia = 0; ia < n; ++ia) {
     int t = it[ia];
     for (int a = 0; a < 3; ++a) mu[ia + n * a] = 0.0;
-    if (!p->ldip_polar[t]) continue;
+    if (th)
+      for (int k = 0; k < 9; ++k) th[ia + n * k] = 0.0;
+    if (!p->ldip_polar[t] && !p->lquad_polar[t]) continue;
     for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b)
+      for (int b = 0; b < 3; ++b) {
         mu[ia + n * a] =
             mu[ia + n * a] + p->poldip[t + NT * (a + 3 * b)] * E[ia + n * b];
+        if (th)
+          for (int g = 0; g < 3; ++g)
+            th[ia + n * (a + 3 * b)] = th[ia + n * (a + 3 * b)] +
+                                       p->polquad[t + NT * (g + 3 * (a + 3 * b))] * E[ia + n * g];
+      }
   }
   for (int ia = 0; ia < n; ++ia) {
     int t = it[ia];
@@ -922,7 +1014,17 @@ static void induced_moment(const mdffgpu_params* p, const int* it,
       for (int b = 0; b < 3; ++b)
         u = u + mu[ia + n * a] * inv[a + 3 * b] * mu[ia + n * b];
   }
-  *u_pol = u * 0.5;
+  u = u * 0.5;
+  double uq = 0.0;
+  if (th)
+    for (int ia = 0; ia < n; ++ia) {
+      int t = it[ia];
+      if (!p->lquad_polar[t]) continue;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          uq = uq + th[ia + n * (a + 3 * b)] * th[ia + n * (a + 3 * b)] / p->polquad[t];
+    }
+  *u_pol = u + uq / 6.0;
 }
 
 static const double B_ASPC[6][6] = {
@@ -996,13 +1098,13 @@ int orc_extrapolate(const mdffgpu_params* p, int itime, double* h, double* mu) {
    (1 - w) mu_p, mu_p the predictor's moments (in mu on entry); u_pol is
    that of alpha E, as induced_moment leaves it.  it holds 0-based types. */
 static void aspc_correct(const mdffgpu_params* p, const int* it, int itime,
-                         const double* E, double* mu, double* u_pol) {
+                         const double* E, double* mu, double* th, double* u_pol) {
   size_t n3 = 3 * (size_t)p->natm;
   double* mup = (double*)malloc(n3 * sizeof(double));
   if (!mup) return;
   int eo = orc_aspc_order(p, itime);
   memcpy(mup, mu, n3 * sizeof(double));
-  induced_moment(p, it, E, mu, u_pol);
+  induced_moment(p, it, E, mu, th, u_pol);
   for (size_t k = 0; k < n3; ++k)
     mu[k] = W_ASPC[eo] * mu[k] + (1.0 - W_ASPC[eo]) * mup[k];
   free(mup);
@@ -1015,32 +1117,39 @@ void orc_aspc_correct(const mdffgpu_params* p, const int* itype, int itime,
   int* it = (int*)malloc((size_t)n * sizeof(int));
   if (!it) return;
   for (int i = 0; i < n; ++i) it[i] = itype[i] - 1;
-  aspc_correct(p, it, itime, E, mu, u_pol);
+  aspc_correct(p, it, itime, E, mu, NULL, u_pol);
   free(it);
 }
 
 /* moment_from_pola_scf, src/field.f90:3200-3458 */
 static int scf(const es_ctx* e, const double* rx, const double* ry,
                const double* rz, const double* qia, const double* dipia,
-               int itime, orc_state* st, double* mu_ind, double* fx,
-               double* fy, double* fz, int* n_scf, double* rmsd_out,
-               int* n_es) {
+               const double* quadia, int itime, orc_state* st, double* mu_ind,
+               double* th_ind, double* fx, double* fy, double* fz, int* n_scf,
+               double* rmsd_out, int* n_es) {
   const mdffgpu_params* p = e->p;
   int n = p->natm;
   const int* it = e->it;
-  size_t n3 = 3 * (size_t)n;
-  int linduced = 0;
-  for (int t = 0; t < p->ntype; ++t)
+  size_t n3 = 3 * (size_t)n, n9 = 9 * (size_t)n;
+  int linduced = 0, lqpol = 0;
+  for (int t = 0; t < p->ntype; ++t) {
     if (p->ldip_polar[t]) linduced = 1;
+    if (p->lquad_polar[t]) linduced = lqpol = 1;
+  }
   for (size_t k = 0; k < n3; ++k) mu_ind[k] = 0.0;
+  for (size_t k = 0; k < n9; ++k) th_ind[k] = 0.0;
   *n_scf = 0;
   *rmsd_out = 0.0;
   if (!linduced) return 0;
-  int ldip = 0;
+  int ldip = 0, lqua = 0;
   for (int k = 0; k < 48; ++k)
     if (p->dip[k] != 0.0) ldip = 1;
-  int task_static[3] = {1, ldip, ldip};
-  int task_ind[3] = {0, 0, 1};
+  for (int k = 0; k < 144; ++k)
+    if (p->quad[k] != 0.0) lqua = 1;
+  /* :3255-3268: with static quadrupoles every task of the static call */
+  int task_static[6] = {1, ldip || lqua, ldip || lqua, lqua, lqua, lqua};
+  int task_ind[6] = {0, 0, 1, 0, 0, 1};
+  double* thi = lqpol ? th_ind : NULL;
   double* buf = (double*)calloc(4 * n3 + 9 * (size_t)n + (size_t)n + n3, sizeof(double));
   if (!buf) return MDFFGPU_ERR_ALLOC;
   double *Estat = buf, *Eind = Estat + n3, *E = Eind + n3, *mup = E + n3,
@@ -1048,7 +1157,7 @@ static int scf(const es_ctx* e, const double* rx, const double* ry,
   (void)muzero;
   double u_coul, tau[9], pv;
   for (int i = 0; i < n; ++i) fx[i] = fy[i] = fz[i] = 0.0;
-  int rc = es_call(e, rx, ry, rz, qia, dipia, task_static, 1, 1, 0, 0, 0,
+  int rc = es_call(e, rx, ry, rz, qia, dipia, lqua ? quadia : NULL, task_static, 1, 1, 0, 0, 0,
                    st->u_pol, Estat, efg, fx, fy, fz, &u_coul, tau, &pv, NULL);
   if (rc) {
     free(buf);
@@ -1067,33 +1176,45 @@ static int scf(const es_ctx* e, const double* rx, const double* ry,
          iscf < p->min_scf_pol_iter) {
     ++iscf;
     if (iscf != 1 || p->calc_mode == 0) {
-      induced_moment(p, it, E, mu_ind, &st->u_pol);
+      induced_moment(p, it, E, mu_ind, thi, &st->u_pol);
     } else {
-      /* first iterate in 'md'/'opt': extrapolated, :3321-3325 */
+      /* first iterate in 'md'/'opt': extrapolated dipoles, :3321-3325;
+         theta_ind stays zero (the predictor zeroes it, :3979; the poly
+         extrapolation leaves the zeroed array of get_moments, :3711) */
       rc = orc_extrapolate(p, itime, h, mu_ind);
       if (rc) break;
     }
-    /* induced field: charges and static dipoles zeroed, :3290-3301 */
-    rc = es_call(e, rx, ry, rz, qzero, mu_ind, task_ind, 0, 1, 0, 0, 0,
+    /* induced field: charges and static moments zeroed, :3290-3303 */
+    rc = es_call(e, rx, ry, rz, qzero, mu_ind, thi, task_ind, 0, 1, 0, 0, 0,
                  st->u_pol, Eind, efg, fx, fy, fz, &u_coul, tau, &pv, NULL);
     if (rc) break;
     ++*n_es;
     for (size_t k = 0; k < n3; ++k) E[k] = Estat[k] + Eind[k];
     for (int i = 0; i < n; ++i) fx[i] = fy[i] = fz[i] = 0.0;
     if (iscf == 1 && p->calc_mode == 1 && p->algo_ext_dipole == MDFFGPU_EXT_ASPC)
-      aspc_correct(p, it, itime, E, mu_ind, &st->u_pol);
+      aspc_correct(p, it, itime, E, mu_ind, thi, &st->u_pol);
     /* stopping criterion, :3378-3400 */
     rmsd = 0.0;
     int npol = 0;
     for (int ia = 0; ia < n; ++ia) {
       int t = it[ia];
-      if (!p->ldip_polar[t]) continue;
-      ++npol;
-      for (int a = 0; a < 3; ++a) {
-        double pa = p->poldip[t + NT * (a + 3 * a)];
-        if (pa == 0.0) continue;
-        double r = mu_ind[ia + n * a] / pa - E[ia + n * a];
-        rmsd = rmsd + r * r;
+      if (p->ldip_polar[t]) {
+        ++npol;
+        for (int a = 0; a < 3; ++a) {
+          double pa = p->poldip[t + NT * (a + 3 * a)];
+          if (pa == 0.0) continue;
+          double r = mu_ind[ia + n * a] / pa - E[ia + n * a];
+          rmsd = rmsd + r * r;
+        }
+      }
+      if (p->lquad_polar[t]) {
+        ++npol;
+        for (int a = 0; a < 3; ++a) {
+          double pq = p->polquad[t + NT * (a + 3 * (a + 3 * a))];
+          if (pq == 0.0) continue;
+          double r = th_ind[ia + n * (a + 3 * a)] / pq - E[ia + n * a];
+          rmsd = rmsd + r * r;
+        }
       }
     }
     rmsd = sqrt(rmsd / (double)npol);
@@ -1106,6 +1227,41 @@ static int scf(const es_ctx* e, const double* rx, const double* ry,
   *rmsd_out = rmsd;
   free(buf);
   return rc;
+}
+
+/* moment_from_WFc, src/field.f90:3492-3672: the dipole of each ion of a
+   species with lwfc > 0 from the Wannier centres (atoms of species with
+   lwfc < 0) whose squared minimum-image distance is below rcut_wfc (the
+   reference compares the square with rcut_wfc itself, :3572), in index
+   order: mu = -2 sum (r_c - r_ion) (:3606-3629).  An ion with a centre
+   count other than lwfc is the reference's STOP (:3594-3599):
+   MDFFGPU_ERR_WFC.  Ions with lwfc = 0 get no contribution (the
+   reference's list pointers of such ions are left unset, :3585-3588). */
+static int wfc_dipoles(const es_ctx* e, double* mu) {
+  const mdffgpu_params* p = e->p;
+  int n = p->natm;
+  const int* it = e->it;
+  for (size_t k = 0; k < 3 * (size_t)n; ++k) mu[k] = 0.0;
+  for (int ia = 0; ia < n; ++ia) {
+    int t = it[ia];
+    if (p->lwfc[t] <= 0) continue;
+    int cnt = 0;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int ja = 0; ja < n; ++ja) {
+      if (p->lwfc[it[ja]] >= 0) continue;
+      double rij[3];
+      double d2 = min_image(&e->cell, e->s + 3 * ia, e->s + 3 * ja, rij);
+      if (!(d2 < p->rcut_wfc)) continue;
+      ++cnt;
+      double rji[3];
+      min_image(&e->cell, e->s + 3 * ja, e->s + 3 * ia, rji);
+      for (int a = 0; a < 3; ++a) acc[a] = acc[a] + rji[a];
+    }
+    if (cnt != p->lwfc[t]) return MDFFGPU_ERR_WFC;
+    for (int a = 0; a < 3; ++a) mu[ia + (size_t)n * a] = acc[a];
+  }
+  for (size_t k = 0; k < 3 * (size_t)n; ++k) mu[k] = -2.0 * mu[k];
+  return 0;
 }
 
 int orc_engforce(const mdffgpu_params* p, const int* itype, const double* A,
@@ -1153,33 +1309,44 @@ int orc_engforce(const mdffgpu_params* p, const int* itype, const double* A,
     for (int k = 0; k < 9; ++k) en->tau_nonb[k] += o.tau[k];
   }
   if (p->lcoulomb) {
-    double* buf = (double*)calloc(5 * n3 + 2 * n9 + (size_t)n, sizeof(double));
+    double* buf = (double*)calloc(6 * n3 + 5 * n9 + (size_t)n, sizeof(double));
     if (!buf) {
       rc = MDFFGPU_ERR_ALLOC;
       goto done;
     }
     double *qia = buf, *dipia = qia + n, *muind = dipia + n3, *mu = muind + n3,
-           *ef = mu + n3, *fsave = ef + n3, *efg = fsave + n3, *efgs = efg + n9;
+           *ef = mu + n3, *fsave = ef + n3, *efg = fsave + n3, *efgs = efg + n9,
+           *quadia = efgs + n9, *thind = quadia + n9, *theta = thind + n9,
+           *muwfc = theta + n9;
     (void)efgs;
     for (int i = 0; i < n; ++i) {
       qia[i] = p->qch[it[i]];
       for (int a = 0; a < 3; ++a) dipia[i + n * a] = p->dip[it[i] + NT * a];
+      for (int k = 0; k < 9; ++k) quadia[i + n * k] = p->quad[it[i] + NT * k];
     }
     /* get_moments, :3686-3765: forces saved around the SCF */
     memcpy(fsave, fx, n * sizeof(double));
     memcpy(fsave + n, fy, n * sizeof(double));
     memcpy(fsave + 2 * n, fz, n * sizeof(double));
     int n_es = 0;
-    rc = scf(&e, rx, ry, rz, qia, dipia, itime, st, muind, fx, fy, fz,
+    rc = scf(&e, rx, ry, rz, qia, dipia, quadia, itime, st, muind, thind, fx, fy, fz,
              &en->n_scf, &en->rmsd, &n_es);
     memcpy(fx, fsave, n * sizeof(double));
     memcpy(fy, fsave + n, n * sizeof(double));
     memcpy(fz, fsave + 2 * n, n * sizeof(double));
+    int lwannier = 0;
+    for (int t = 0; t < p->ntype; ++t)
+      if (p->lwfc[t] > 0) lwannier = 1;
+    if (!rc && lwannier) rc = wfc_dipoles(&e, muwfc);
     if (!rc) {
       for (size_t k = 0; k < n3; ++k) mu[k] = dipia[k] + muind[k];
-      int task[3] = {p->task_coul[0], p->task_coul[1], p->task_coul[2]};
+      if (lwannier && p->ldip_wfc)
+        for (size_t k = 0; k < n3; ++k) mu[k] = mu[k] + muwfc[k];
+      for (size_t k = 0; k < n9; ++k) theta[k] = quadia[k] + thind[k];
+      int task[6] = {p->task_coul[0], p->task_coul[1], p->task_coul[2],
+                     p->task_coul_quad[0], p->task_coul_quad[1], p->task_coul_quad[2]};
       double pv;
-      rc = es_call(&e, rx, ry, rz, qia, mu, task, 1, p->doefield, p->doefg, 1,
+      rc = es_call(&e, rx, ry, rz, qia, mu, theta, task, 1, p->doefield, p->doefg, 1,
                    1, st->u_pol, ef, efg, fx, fy, fz, &en->u_coul,
                    en->tau_coul, &pv, NULL);
       ++n_es;
@@ -1187,7 +1354,7 @@ int orc_engforce(const mdffgpu_params* p, const int* itype, const double* A,
       en->u_pol = st->u_pol;
       en->n_es_calls = n_es;
       if (mu_t) memcpy(mu_t, mu, n3 * sizeof(double));
-      if (theta_t) memset(theta_t, 0, n9 * sizeof(double));
+      if (theta_t) memcpy(theta_t, theta, n9 * sizeof(double));
       if (ef_t) memcpy(ef_t, ef, n3 * sizeof(double));
       if (efg_t) memcpy(efg_t, efg, n9 * sizeof(double));
     }
@@ -1469,7 +1636,16 @@ int orc_efgcalc(const mdffgpu_params* p, const mdffgpu_efg_opts* o, const int* i
     st.hist = hist;
     st.u_pol = 0.0;
     int n_es = 0;
-    rc = scf(&e, rx, ry, rz, q, dip, 0, &st, muind, f, f + n, f + 2 * n, n_scf, rmsd, &n_es);
+    /* the standalone EFG takes charges and dipoles only (quadrupoles are
+       refused by its caller) */
+    double* zq = (double*)calloc(18 * (size_t)n, sizeof(double));
+    if (!zq) {
+      rc = MDFFGPU_ERR_ALLOC;
+    } else {
+      rc = scf(&e, rx, ry, rz, q, dip, zq, 0, &st, muind, zq + 9 * (size_t)n, f, f + n,
+               f + 2 * n, n_scf, rmsd, &n_es);
+      free(zq);
+    }
     e.p = p;
   }
   if (!rc) {

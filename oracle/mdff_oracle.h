@@ -91,12 +91,14 @@ int orc_engforce(const mdffgpu_params* p, const int* itype, const double* A,
                  mdffgpu_energies* en);
 
 /* One multipole_ES call (src/field.f90:1817-2054) with explicit moments:
-   q (natm), mu (natm,3).  task[3] as task_coul(1:3).  ef (natm,3) and efg
+   q (natm), mu (natm,3), theta (natm,3,3) or NULL.  task[6] as
+   task_coul(1:6).  ef (natm,3) and efg
    (natm,3,3) are outputs; f? are added to; u_coul, tau_coul (9) and
    pvirial are outputs.  Exposed for the unit tests of the Ewald sum. */
 int orc_multipole_es(const mdffgpu_params* p, const int* itype,
                      const double* A, const double* rx, const double* ry,
                      const double* rz, const double* q, const double* mu,
+                     const double* theta,
                      const int* task, int damp_ind, int do_efield, int do_efg,
                      int do_forces, int do_stress, double u_pol, double* ef,
                      double* efg, double* fx, double* fy, double* fz,

@@ -50,7 +50,7 @@ def lib():
                                    _dp, _dp, _dp, _dp, ctypes.POINTER(MdffEnergies)]
         L.orc_multipole_es.restype = ctypes.c_int
         L.orc_multipole_es.argtypes = [ctypes.POINTER(MdffParams), _ip, _dp, _dp, _dp, _dp,
-                                       _dp, _dp, _ip] + [ctypes.c_int] * 5 + [
+                                       _dp, _dp, _dp, _ip] + [ctypes.c_int] * 5 + [
                                            ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp,
                                            _dp, _dp, _dp]
         L.orc_aspc_order.restype = ctypes.c_int
@@ -226,16 +226,21 @@ def chain_nhc2(kin, vxi, xi, Q, L, temp, dt):
 
 
 def multipole_es(params: MdffParams, itype, A, r, q, mu, task, damp_ind=True,
-                 do_efield=True, do_efg=True, do_forces=True, do_stress=True, u_pol=0.0):
+                 do_efield=True, do_efg=True, do_forces=True, do_stress=True, u_pol=0.0,
+                 theta=None):
     """One multipole_ES call (src/field.f90:1817-2054) with explicit charges
-    q (natm) and dipoles mu (natm,3)."""
+    q (natm), dipoles mu (natm,3) and quadrupoles theta (natm,3,3) or None;
+    task holds task_coul(1:3) or (1:6)."""
     n = params.natm
     r = np.asarray(r, dtype=np.float64)
     rx, ry, rz = (np.ascontiguousarray(r[:, k]) for k in range(3))
     q = np.ascontiguousarray(q, dtype=np.float64)
     muf = np.ascontiguousarray(np.asarray(mu, dtype=np.float64).T.reshape(-1))
     it = np.ascontiguousarray(itype, dtype=np.int32)
-    t = (ctypes.c_int32 * 3)(*[int(x) for x in task])
+    task = list(task) + [0] * (6 - len(task))
+    t = (ctypes.c_int32 * 6)(*[int(x) for x in task])
+    thf = None if theta is None else np.ascontiguousarray(
+        np.asarray(theta, dtype=np.float64).transpose(2, 1, 0).reshape(-1))
     ef = np.zeros(3 * n)
     efg = np.zeros(9 * n)
     f = np.zeros((3, n))
@@ -245,7 +250,8 @@ def multipole_es(params: MdffParams, itype, A, r, q, mu, task, damp_ind=True,
     parts = np.zeros(4)
     Aflat = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(3, 3).reshape(-1, order="F"))
     rc = lib().orc_multipole_es(ctypes.byref(params), it.ctypes.data_as(_ip), _p(Aflat), _p(rx),
-                                _p(ry), _p(rz), _p(q), _p(muf), t, int(damp_ind),
+                                _p(ry), _p(rz), _p(q), _p(muf),
+                                None if thf is None else _p(thf), t, int(damp_ind),
                                 int(do_efield), int(do_efg), int(do_forces), int(do_stress),
                                 float(u_pol), _p(ef), _p(efg), _p(f[0]), _p(f[1]), _p(f[2]),
                                 ctypes.byref(u), _p(tau), ctypes.byref(pv), _p(parts))

@@ -104,9 +104,11 @@ def _pim_params(n=8):
 def test_create_failure_statuses():
     """The failure statuses of the C ABI for host-detectable errors, as the
     reference stops on them: Tang-Toennies order above 8 (src/field.f90:
-    355-358) -> MDFFGPU_ERR_TT_ORDER; quadrupole tasks task_coul(4:6) and
-    polynomial extrapolation of order >= 2 -> MDFFGPU_ERR_UNSUPPORTED; bad
-    sizes -> MDFFGPU_ERR_ARG."""
+    355-358) -> MDFFGPU_ERR_TT_ORDER; quadrupole tasks task_coul(4:6) in
+    mixed precision, quadrupole polarizabilities with the conjugate-gradient
+    SCF and polynomial extrapolation of order >= 2 ->
+    MDFFGPU_ERR_UNSUPPORTED; bad sizes and non-symmetric quadrupole tables
+    -> MDFFGPU_ERR_ARG."""
     from mdff_amd import params as P
     p = _pim_params()
     k = P.didx(2, 2, 1)
@@ -115,10 +117,25 @@ def test_create_failure_statuses():
     st, msg = _create_status(p)
     assert st == 5 and "Tang-Toennies" in msg, (st, msg)
     for q in range(3):
+        # quadrupole tasks run on the FP64 path only
         p = _pim_params()
         p.task_coul_quad[q] = 1
+        p.precision = P.PREC_MIXED
         st, msg = _create_status(p)
         assert st == 6 and "quadrupole" in msg, (st, msg)
+    p = _pim_params()
+    p.lquad_polar[1] = 1
+    P.field_check_tag(p, polquad_iso=[0.0, 0.5])
+    p.scf_solver = P.SCF_PCG
+    st, msg = _create_status(p)
+    assert st == 6 and "Jacobi" in msg, (st, msg)
+    p.scf_solver = P.SCF_JACOBI
+    p.polquad[1] = 0.0  # polquad(2,1,1,1), the divisor of u_pol
+    assert _create_status(p)[0] == 1
+    p = _pim_params()
+    p.quad[P.qidx(1, 1, 2)] = 0.3  # quad(1,1,2) without quad(1,2,1)
+    st, msg = _create_status(p)
+    assert st == 1 and "symmetric" in msg, (st, msg)
     p = _pim_params()
     p.algo_ext_dipole, p.extrapolate_order = P.EXT_POLY, 2
     st, msg = _create_status(p)
