@@ -448,7 +448,7 @@ int32_t mdffgpu_kernel_count(void);
  * configurations; device arrays are SoA [x|y|z] over all atoms.
  * Supported model: one or two species, BMHFTD/LJ/Morse pair forces,
  * point charges, isotropic induced dipoles with Tang-Toennies damping,
- * no permanent dipoles, no surface term.
+ * no permanent dipoles, quadrupoles or Wannier centres, no surface term.
  * ------------------------------------------------------------------- */
 typedef struct mdffgpu_batch mdffgpu_batch;
 
@@ -484,8 +484,14 @@ int mdffgpu_batch_forces(mdffgpu_batch* b, double* d_force, int32_t scf_iters,
 /* Force-matching loss L = sum |F - F_target|^2 / (3 N) over all atoms and
    its gradient with respect to the parameter vector
      [A, B, C6, C8, BD] per unordered species pair (11, 12, 22),
-     qch per species, polarizability per polarizable species
-   (mdffgpu_batch_num_params entries).  d_force (may be NULL) receives the
+     [epsmor, sigmamor, rhomor] per unordered pair when lmorse,
+     qch per species, polarizability per polarizable species,
+     the Tang-Toennies dipole damping b per species t whose own field terms
+     are damped (dip_pol_damp_b(t,t,:), one value per species; with
+     polarizable species only)
+   (mdffgpu_batch_num_params entries).  Pair parameters come from pair
+   terms recomputed in the backward pass; charges, polarizabilities and the
+   dipole part of b from tangents of the SCF fixed point.  d_force (may be NULL) receives the
    forces.  The SCF iteration count is as in mdffgpu_batch_forces. */
 int32_t mdffgpu_batch_num_params(const mdffgpu_batch* b);
 int64_t mdffgpu_batch_num_atoms(const mdffgpu_batch* b);

@@ -10,8 +10,11 @@ Ewald parameter, k mesh and cutoffs.
 
 Parameter vector theta (mdffgpu_batch_num_params entries):
     [A, B, C6, C8, BD] for each unordered species pair (1,1), (1,2), (2,2)
+    [eps, sigma, rho] (Morse) for each unordered pair, when lmorse
     qch for each species
     alpha for each polarizable species (isotropic poldip)
+    b (Tang-Toennies dipole damping, dip_pol_damp_b(t,t,:)) for each species
+      whose own field terms are damped, when some species is polarizable
 """
 from __future__ import annotations
 
@@ -29,18 +32,53 @@ from .params import MdffBatchConfig, MdffParams
 _ERR_UNSUPPORTED = 6  # MDFFGPU_ERR_UNSUPPORTED of include/mdffgpu.h
 
 _SR_NAMES = ("A", "B", "C6", "C8", "BD")
+_MO_NAMES = ("eps_mo", "sigma_mo", "rho_mo")
 
 
 def _pairs(ntype: int):
     return [(ti, tj) for ti in range(ntype) for tj in range(ti, ntype)]
 
 
+def _damp_entries(params: MdffParams) -> dict:
+    """Species whose damping b is a parameter, with the damping-table
+    entries their centres use: (t, t, u) when the centre has the smaller
+    index and (t, u, t) otherwise (src/field.f90:2207-2211), on damped pairs
+    with c != 0; only when some species is polarizable."""
+    nt = params.ntype
+    if not any(params.ldip_polar[t] for t in range(nt)):
+        return {}
+    L = params.ldip_damping
+
+    def idx(x, y, z):
+        return x + 16 * (y + 16 * z)
+
+    out = {}
+    for t in range(nt):
+        ent = []
+        for u in range(nt):
+            for e, act in ((idx(t, t, u), L[idx(t, t, u)] or L[idx(u, t, u)]),
+                           (idx(t, u, t), L[idx(u, u, t)] or L[idx(t, u, t)])):
+                if act and params.dip_pol_damp_c[e] != 0.0 and e not in ent:
+                    ent.append(e)
+        if ent:
+            out[t] = ent
+    return out
+
+
+def _damp_types(params: MdffParams) -> list:
+    return sorted(_damp_entries(params))
+
+
 def theta_names(params: MdffParams) -> list:
     names = []
     for ti, tj in _pairs(params.ntype):
         names += [f"{n}[{ti + 1}{tj + 1}]" for n in _SR_NAMES]
+    if params.lmorse:
+        for ti, tj in _pairs(params.ntype):
+            names += [f"{n}[{ti + 1}{tj + 1}]" for n in _MO_NAMES]
     names += [f"q[{t + 1}]" for t in range(params.ntype)]
     names += [f"alpha[{t + 1}]" for t in range(params.ntype) if params.ldip_polar[t]]
+    names += [f"b_TT[{t + 1}]" for t in _damp_types(params)]
     return names
 
 
@@ -51,8 +89,14 @@ def get_theta(params: MdffParams) -> np.ndarray:
         k = ti + 16 * tj
         out += [params.Abmhftd[k], params.Bbmhftd[k], params.Cbmhftd[k], params.Dbmhftd[k],
                 params.BDbmhftd[k]]
+    if params.lmorse:
+        for ti, tj in _pairs(params.ntype):
+            k = ti + 16 * tj
+            out += [params.epsmor[k], params.sigmamor[k], params.rhomor[k]]
     out += [params.qch[t] for t in range(params.ntype)]
     out += [params.poldip[t] for t in range(params.ntype) if params.ldip_polar[t]]
+    ent = _damp_entries(params)
+    out += [params.dip_pol_damp_b[ent[t][0]] for t in sorted(ent)]
     return np.array(out, dtype=np.float64)
 
 
@@ -66,6 +110,11 @@ def set_theta(params: MdffParams, theta) -> MdffParams:
         for k in (ti + 16 * tj, tj + 16 * ti):
             p.Abmhftd[k], p.Bbmhftd[k], p.Cbmhftd[k], p.Dbmhftd[k], p.BDbmhftd[k] = th[x:x + 5]
         x += 5
+    if p.lmorse:
+        for ti, tj in _pairs(p.ntype):
+            for k in (ti + 16 * tj, tj + 16 * ti):
+                p.epsmor[k], p.sigmamor[k], p.rhomor[k] = th[x:x + 3]
+            x += 3
     for t in range(p.ntype):
         p.qch[t] = th[x]
         x += 1
@@ -74,6 +123,11 @@ def set_theta(params: MdffParams, theta) -> MdffParams:
             for a in range(3):
                 p.poldip[t + 16 * (a + 3 * a)] = th[x]
             x += 1
+    ent = _damp_entries(p)
+    for t in sorted(ent):
+        for e in ent[t]:
+            p.dip_pol_damp_b[e] = th[x]
+        x += 1
     return p
 
 
@@ -223,8 +277,9 @@ def _loss_function():
 
     class ForceMatchLoss(torch.autograd.Function):
         """L(theta) = sum |F(theta) - F_target|^2 / (3 N) over a Batch; the
-        gradient comes from the library's adjoint (BMHFTD pair terms
-        recomputed, exact tangents for charges and polarizabilities)."""
+        gradient comes from the library's adjoint (BMHFTD and Morse pair
+        terms recomputed, exact tangents for charges, polarizabilities and
+        the dipole part of the damping b)."""
 
         @staticmethod
         def forward(ctx, theta, batch, target, scf_iters):

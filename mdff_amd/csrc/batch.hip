@@ -51,6 +51,8 @@ constexpr int BTA = 32;        // atoms per LDS tile of the structure factor
 constexpr int NTAB_MAX = 75;   // phase-table entries per atom (kmax <= 12)
 constexpr int NPAIR_MAX = 3;   // unordered type pairs (two species)
 constexpr int NSR = 5;         // A, B, C6, C8, BD per pair
+constexpr int NMO = 3;         // Morse eps, sigma, rho per pair
+constexpr int NDAMP_MAX = 2;   // species with a Tang-Toennies damping parameter b
 
 // ---------------------------------------------------------------------------
 // compensated accumulation (Kahan-Babuska / Neumaier) in type R
@@ -469,6 +471,154 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_b_sr_adj(SrArgs a, const int* __
     for (int k = 0; k < NSR * NPAIR_MAX; ++k) part[(size_t)k * gridDim.x + blockIdx.x] = acc[k];
 }
 
+// Adjoint of the Morse pair forces of k_shortrange: X = e^{-rho (d - sigma)},
+// w = 2 eps rho X (X - 1) / d, so that for s = r . g_i
+//   dw/deps   = 2 rho X (X - 1) / d
+//   dw/dsigma = 2 eps rho^2 X (2 X - 1) / d
+//   dw/drho   = 2 eps X [(X - 1) - rho (d - sigma) (2 X - 1)] / d
+// Rows: NMO * pair index.
+template <typename R>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_b_mo_adj(SrArgs a, const int* __restrict__ upair,
+                                                          const double* __restrict__ g,
+                                                          double* __restrict__ part) {
+  __shared__ double red[4 * NMO * NPAIR_MAX];
+  const int n = a.n;
+  const PairTables* T = a.T;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = tid / LPA, sub = tid % LPA;
+  double acc[NMO * NPAIR_MAX];
+#pragma unroll
+  for (int k = 0; k < NMO * NPAIR_MAX; ++k) acc[k] = 0.0;
+  if (i < n) {
+    const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
+    const int ti = a.typ[i];
+    const int c = a.cnt[i];
+    const BatchCfg& bcf = a.bc[a.cfg[i]];
+    const R rc2 = R(bcf.rc2);
+    const R g0 = R(g[i]), g1 = R(g[i + n]), g2 = R(g[i + 2 * n]);
+    const int* nb = a.nbr + (size_t)i * a.maxnb;
+    for (int k = sub; k < c; k += LPA) {
+      const int j = nb[k];
+      R rx, ry, rz;
+      const R rsq = min_image_t<R>(bcf.cell, si1 - a.s4[j].x, si2 - a.s4[j].y, si3 - a.s4[j].z,
+                                   rx, ry, rz);
+      if (!(rsq < rc2)) continue;
+      const int t = ti + MDFF_NT * a.typ[j];
+      const R eps = R(T->mo_eps[t]), rho = R(T->mo_rho[t]), sig = R(T->mo_sig[t]);
+      const R d = sqrt(rsq);
+      const R X = rexp<R>(-d * rho) * R(T->mo_rs[t]);
+      const R s = (rx * g0 + ry * g1 + rz * g2) / d;
+      const int p = upair[t];
+      acc[NMO * p + 0] += (double)(s * R(2) * rho * X * (X - R(1)));
+      acc[NMO * p + 1] += (double)(s * R(2) * eps * rho * rho * X * (R(2) * X - R(1)));
+      acc[NMO * p + 2] +=
+          (double)(s * R(2) * eps * X * ((X - R(1)) - rho * (d - sig) * (R(2) * X - R(1))));
+    }
+  }
+  block_sum<NMO * NPAIR_MAX>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NMO * NPAIR_MAX; ++k) part[(size_t)k * gridDim.x + blockIdx.x] = acc[k];
+}
+
+// Species of the damping parameters: slot[t] >= 0 for a species whose
+// Tang-Toennies b (dip_pol_damp_b(t, t, :)) is a fitted parameter.
+struct DampSlots {
+  int slot[MDFF_NT];
+};
+
+// Explicit derivatives with respect to the damping b of the Tang-Toennies
+// dipole damping (src/field.f90:2205-2231, 2468-2538), FP64, pair terms
+// recomputed.  The damping of centre i's own terms uses the parameters of
+// its species (dmp slot o of k_es_dir), so b_t enters
+//   the static field   E_i += q_j (F1 - F1d_i) r / d^3
+//   the final force    F_i += [q_i (T2 - T2d_j) mu_j - q_j (T2 - T2d_i) mu_i] coul_unit
+// with F1d = 1 - f, F2d = F1d + (d / 3) fd and, for the Tang-Toennies f, fd
+// of order n, df/db = (d / b) fd, dfd/db = fd ((n + 1) / b - d).  Writes
+// dEs[m] = dE_static / db_m (3N each, every atom) and the block partials of
+// sum_i g_i . dF_i / db_m (rows m).
+__global__ __launch_bounds__(MDFF_BLOCK) void k_b_damp_adj(EsDirArgs a, DampSlots ds,
+                                                            const double* __restrict__ g,
+                                                            double coul_unit, int ndamp,
+                                                            double* __restrict__ dEs,
+                                                            double* __restrict__ part) {
+  __shared__ double red[4 * NDAMP_MAX];
+  const int n = a.n;
+  const PairTables* T = a.T;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = tid / LPA, sub = tid % LPA;
+  double de[NDAMP_MAX][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  double df[NDAMP_MAX][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  if (i < n) {
+    const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
+    const int ti = a.typ[i];
+    const BatchCfg& bcf = a.bc[a.cfg[i]];
+    const double qi = a.q[i];
+    const double mi[3] = {a.mu[i], a.mu[i + n], a.mu[i + 2 * n]};
+    const int mi_slot = ds.slot[ti];
+    const int c = a.cnt[i];
+    const int* nb = a.nbr + (size_t)i * a.maxnb;
+    for (int k = sub; k < c; k += LPA) {
+      const int j = nb[k];
+      const double4 sj = a.s4[j];
+      double rx, ry, rz;
+      const double d2 = min_image(bcf.cell, si1 - sj.x, si2 - sj.y, si3 - sj.z, rx, ry, rz);
+      if (d2 > bcf.cut2) continue;
+      const int tj = a.typ[j];
+      const int o = (i < j) ? 0 : 1;
+      const int pij = ti + MDFF_NT * tj, pji = tj + MDFF_NT * ti;
+      if (!T->dmp_on[o][pij]) continue;
+      const int mj_slot = ds.slot[tj];
+      if (mi_slot < 0 && mj_slot < 0) continue;
+      const double d = sqrt(d2);
+      const double dm3 = 1.0 / (d2 * d), dm5 = dm3 / d2;
+      const double qj = a.q[j];
+      const double mj[3] = {a.mu[j], a.mu[j + n], a.mu[j + 2 * n]};
+      const double r3[3] = {rx, ry, rz};
+      if (mi_slot >= 0 && T->dmp_b[o][pij] != 0.0) {
+        const double b = T->dmp_b[o][pij];
+        const int kk = T->dmp_k[o][pij];
+        double f, fd;
+        tt_damp_t<double>(a.tt, b, T->dmp_c[o][pij], d, kk, f, fd);
+        const double dF1 = -(d / b) * fd;
+        const double dF2 = dF1 + (d / 3.0) * fd * ((kk + 1) / b - d);
+        const double rmi = rx * mi[0] + ry * mi[1] + rz * mi[2];
+        for (int x = 0; x < 3; ++x) {
+          de[mi_slot][x] += -qj * dF1 * dm3 * r3[x];
+          df[mi_slot][x] += qj * (r3[x] * rmi * 3.0 * dF2 * dm5 - mi[x] * dF1 * dm3) * coul_unit;
+        }
+      }
+      if (mj_slot >= 0 && T->dmp_b[1 - o][pji] != 0.0) {
+        const double b = T->dmp_b[1 - o][pji];
+        const int kk = T->dmp_k[1 - o][pji];
+        double f, fd;
+        tt_damp_t<double>(a.tt, b, T->dmp_c[1 - o][pji], d, kk, f, fd);
+        const double dF1 = -(d / b) * fd;
+        const double dF2 = dF1 + (d / 3.0) * fd * ((kk + 1) / b - d);
+        const double rmj = rx * mj[0] + ry * mj[1] + rz * mj[2];
+        for (int x = 0; x < 3; ++x)
+          df[mj_slot][x] -= qi * (r3[x] * rmj * 3.0 * dF2 * dm5 - mj[x] * dF1 * dm3) * coul_unit;
+      }
+    }
+  }
+  for (int off = LPA / 2; off > 0; off >>= 1)
+    for (int m = 0; m < NDAMP_MAX; ++m)
+      for (int x = 0; x < 3; ++x) {
+        de[m][x] += __shfl_xor(de[m][x], off, 64);
+        df[m][x] += __shfl_xor(df[m][x], off, 64);
+      }
+  double acc[NDAMP_MAX] = {0.0, 0.0};
+  if (i < n && sub == 0) {
+    for (int m = 0; m < ndamp; ++m)
+      for (int x = 0; x < 3; ++x) {
+        dEs[(size_t)m * 3 * n + i + (size_t)x * n] = de[m][x];
+        acc[m] += g[i + (size_t)x * n] * df[m][x];
+      }
+  }
+  block_sum<NDAMP_MAX>(acc, red);
+  if (threadIdx.x == 0)
+    for (int m = 0; m < NDAMP_MAX; ++m) part[(size_t)m * gridDim.x + blockIdx.x] = acc[m];
+}
+
 }  // namespace batch
 }  // namespace mdff
 
@@ -493,6 +643,12 @@ struct mdffgpu_batch {
   mdffgpu_params P;
   int nconf = 0, N = 0, KX = 0, KY = 0, KZ = 0, ntab = 0, NK = 0, nitems = 0, maxnb = 0;
   int ntype = 0, npair = 0, npol = 0, nparam = 0;
+  // Morse parameters in theta; species with a fitted damping b
+  bool lmo = false;
+  int ndamp = 0;
+  int damp_type[mdff::batch::NDAMP_MAX];
+  mdff::batch::DampSlots dslots;
+  double* d_dEs = nullptr;
   bool mixed = false, pim = false;
   // FP32 path: reciprocal sums on the matrix cores (configurations with a
   // common kmax); positions' version for its phase tables
@@ -853,9 +1009,16 @@ void backward(mdffgpu_batch* b, int n_scf) {
     fill(b, b->d_out + 1, NSR * b->npair, 0.0);
   }
   int slot = 1 + NSR * b->npair;
+  if (b->lmo) {
+    SrArgs sa = sr_args(b);
+    k_b_mo_adj<R><<<gbl, MDFF_BLOCK, 0, b->st>>>(sa, b->d_upair, b->d_g, b->d_part);
+    launch_check(b, "batch Morse adjoint");
+    reduce(b, NMO * b->npair, gbl, slot);
+    slot += NMO * b->npair;
+  }
   double rms;
   if (!P.lcoulomb) {
-    fill(b, b->d_out + slot, b->ntype + b->npol, 0.0);
+    fill(b, b->d_out + slot, b->ntype + b->npol + 2 * b->ndamp, 0.0);
     return;
   }
   for (int a = 0; a < b->ntype; ++a, ++slot) {
@@ -886,6 +1049,24 @@ void backward(mdffgpu_batch* b, int n_scf) {
     es_call<R>(b, ES_FINAL, b->d_q, b->d_mum, nullptr, b->d_Fm);
     k_b_contract<<<gb, MDFF_BLOCK, 0, b->st>>>(N, b->d_g, b->d_Fp, b->d_Fm, b->d_part);
     reduce(b, 1, gb, slot);
+  }
+  if (b->ndamp > 0) {
+    // damping b: explicit derivatives of the static field and the final
+    // force (after the parameter slots, 1 + nparam ..), then the tangent
+    // dmu = alpha (dE_static/db + T dmu) through the quadratic form
+    k_b_damp_adj<<<gbl, MDFF_BLOCK, 0, b->st>>>(es_args(b, b->d_q, b->d_mu), b->dslots, b->d_g,
+                                                P.coul_unit, b->ndamp, b->d_dEs, b->d_part);
+    launch_check(b, "batch damping adjoint");
+    reduce(b, b->ndamp, gbl, 1 + b->nparam);
+    for (int m = 0; m < b->ndamp; ++m, ++slot) {
+      jacobi<R>(b, b->d_dEs + (size_t)m * n3, b->d_dmu, b->d_Et, n_scf, &rms);
+      k_b_axpy<<<nblk(n3), MDFF_BLOCK, 0, b->st>>>(n3, b->d_mu, 1.0, b->d_dmu, b->d_mup);
+      k_b_axpy<<<nblk(n3), MDFF_BLOCK, 0, b->st>>>(n3, b->d_mu, -1.0, b->d_dmu, b->d_mum);
+      es_call<R>(b, ES_FINAL, b->d_q, b->d_mup, nullptr, b->d_Fp);
+      es_call<R>(b, ES_FINAL, b->d_q, b->d_mum, nullptr, b->d_Fm);
+      k_b_contract<<<gb, MDFF_BLOCK, 0, b->st>>>(N, b->d_g, b->d_Fp, b->d_Fm, b->d_part);
+      reduce(b, 1, gb, slot);
+    }
   }
 }
 
@@ -933,7 +1114,43 @@ void set_model(mdffgpu_batch* b, const mdffgpu_params& P, bool first) {
       b->pim = true;
     }
   if (b->pim && !P.lcoulomb) bfail(b, MDFFGPU_ERR_ARG, "polarizable ions need lcoulomb");
-  b->nparam = NSR * b->npair + b->ntype + b->npol;
+  // the damping b of each species whose own field terms are damped: the
+  // entries its centres use, (t, t, u) when i < j and (t, u, t) otherwise
+  // (src/field.f90:2207-2211), on damped pairs with c != 0 -- all with one
+  // value of b per species
+  int ndamp = 0;
+  int dtype[NDAMP_MAX] = {0, 0};
+  DampSlots ds;
+  for (int t = 0; t < MDFF_NT; ++t) ds.slot[t] = -1;
+  auto L3 = [&](int x, int y, int z) { return P.ldip_damping[x + 16 * (y + 16 * z)] != 0; };
+  auto I3 = [](int x, int y, int z) { return x + 16 * (y + 16 * z); };
+  if (b->pim)
+    for (int t = 0; t < P.ntype; ++t) {
+      bool on = false;
+      double b0 = 0.0;
+      for (int u = 0; u < P.ntype; ++u) {
+        const int e[2] = {I3(t, t, u), I3(t, u, t)};
+        const bool act[2] = {L3(t, t, u) || L3(u, t, u), L3(u, u, t) || L3(t, u, t)};
+        for (int o = 0; o < 2; ++o) {
+          if (!act[o] || P.dip_pol_damp_c[e[o]] == 0.0) continue;
+          if (!on) b0 = P.dip_pol_damp_b[e[o]];
+          if (P.dip_pol_damp_b[e[o]] != b0)
+            bfail(b, MDFFGPU_ERR_UNSUPPORTED,
+                  "the batch fits one damping b per species: dip_pol_damp_b(%d,:,:) differ", t + 1);
+          on = true;
+        }
+      }
+      if (!on) continue;
+      ds.slot[t] = ndamp;
+      dtype[ndamp++] = t;
+    }
+  if (!first && (ndamp != b->ndamp || (P.lmorse != 0) != b->lmo))
+    bfail(b, MDFFGPU_ERR_ARG, "set_model may change parameter values only");
+  b->ndamp = ndamp;
+  b->dslots = ds;
+  for (int m = 0; m < NDAMP_MAX; ++m) b->damp_type[m] = dtype[m];
+  b->lmo = P.lmorse != 0;
+  b->nparam = (NSR + (b->lmo ? NMO : 0)) * b->npair + b->ntype + b->npol + b->ndamp;
   int p = 0;
   for (int x = 0; x < 256; ++x) b->upair_h[x] = 0;
   for (int ti = 0; ti < P.ntype; ++ti)
@@ -1074,6 +1291,7 @@ void create_impl(const mdffgpu_params* model, int nconf, const mdffgpu_batch_con
   balloc(b, &b->d_qp, N);
   balloc(b, &b->d_qm, N);
   balloc(b, &b->d_out, 64);
+  if (b->ndamp > 0) balloc(b, &b->d_dEs, (size_t)b->ndamp * n3);
   BCK(b, hipMemcpy(b->d_q, q.data(), N * sizeof(double), hipMemcpyHostToDevice));
   BCK(b, hipMemset(b->d_zero, 0, n3 * sizeof(double)));
   BCK(b, hipMemset(b->d_mu, 0, n3 * sizeof(double)));
@@ -1167,7 +1385,7 @@ void loss_grad_seq(mdffgpu_batch* b, int iters, int* n_scf, double* rmsd) {
     *n_scf = forward<double>(b, iters, rmsd);
     backward<double>(b, *n_scf);
   }
-  BCK(b, hipMemcpyAsync(b->h_out, b->d_out, (1 + b->nparam) * sizeof(double),
+  BCK(b, hipMemcpyAsync(b->h_out, b->d_out, (1 + b->nparam + b->ndamp) * sizeof(double),
                         hipMemcpyDeviceToHost, b->st));
 }
 
@@ -1306,9 +1524,13 @@ int mdffgpu_batch_loss_grad(mdffgpu_batch* b, const double* d_target, double* d_
                             b->st));
     BCK(b, hipStreamSynchronize(b->st));
     *loss = b->h_out[0] / (3.0 * b->N);
-    const int nsr = NSR * b->npair;
+    // pair parameters directly; charges, polarizabilities and the dipole
+    // tangent of the damping b as 2 B(x, x') (halved), plus b's explicit part
+    const int nsr = (NSR + (b->lmo ? NMO : 0)) * b->npair;
     for (int k = 0; k < nsr; ++k) grad[k] = b->h_out[1 + k];
     for (int k = nsr; k < b->nparam; ++k) grad[k] = 0.5 * b->h_out[1 + k];
+    for (int m = 0; m < b->ndamp; ++m)
+      grad[b->nparam - b->ndamp + m] += b->h_out[1 + b->nparam + m];
     if (n_scf) *n_scf = ns;
     if (rmsd) *rmsd = r;
     return MDFFGPU_OK;

@@ -39,7 +39,7 @@ struct PairTables {
   // BMHFTD (src/field.f90:2952-2976)
   double bm_A[256], bm_B[256], bm_C[256], bm_D[256], bm_BD[256];
   // Morse
-  double mo_eps[256], mo_rho[256], mo_rs[256];
+  double mo_eps[256], mo_rho[256], mo_rs[256], mo_sig[256];
   // Tang-Toennies dipole damping seen from the centre atom ti of a pair
   // with tj.  Slot o = 0 when ti's atom has the smaller index (it is 'ia'
   // of src/field.f90:2207-2211), o = 1 otherwise.

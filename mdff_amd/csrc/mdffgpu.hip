@@ -1983,6 +1983,7 @@ int make_tables(const mdffgpu_params& P, PairTables& T, TypeTables& Y, TTTable& 
       T.mo_eps[t] = P.epsmor[t];
       T.mo_rho[t] = P.rhomor[t];
       T.mo_rs[t] = std::exp(P.rhomor[t] * P.sigmamor[t]);
+      T.mo_sig[t] = P.sigmamor[t];
       // damping as seen from ti (src/field.f90:2207-2211)
       T.dmp_on[0][t] = L(ti, ti, tj) || L(tj, ti, tj);
       T.dmp_on[1][t] = L(tj, tj, ti) || L(ti, tj, ti);

@@ -111,6 +111,48 @@ def test_adjoint_gradient_matches_finite_differences():
     b.close()
 
 
+def test_adjoint_morse_and_damping_b_match_finite_differences():
+    """Morse pair parameters (recomputed pair terms) and the Tang-Toennies
+    damping b (explicit damped static field and force, plus the dipole
+    tangent) against central differences of the FP64 loss."""
+    from mdff_amd.batch import Batch, get_theta, theta_names
+    systems = _systems(2, nmax=150, kmax=3)
+    p = systems[0].params
+    p.lmorse = 1
+    for ti in range(2):
+        for tj in range(2):
+            k = ti + 16 * tj
+            p.epsmor[k] = 0.04 + 0.01 * (ti + tj)
+            p.sigmamor[k] = 1.9 + 0.2 * (ti + tj)
+            p.rhomor[k] = 1.8 + 0.1 * ti * tj
+    b = Batch.from_systems(systems, precision=P.PREC_FP64)
+    b.set_positions(_positions(systems))
+    names = theta_names(b.model)
+    assert "eps_mo[12]" in names and "b_TT[2]" in names, names
+    target = _target(b, systems)
+    th0 = get_theta(b.model)
+    loss, grad = b.loss_grad(target, 0)
+    h = 1e-5
+    worst = 0.0
+    for k, name in enumerate(names):
+        if not (name.endswith("_mo[11]") or "_mo[" in name or name.startswith("b_TT")):
+            continue
+        step = h * max(abs(th0[k]), 1.0)
+        lv = []
+        for sgn in (1.0, -1.0):
+            th = th0.copy()
+            th[k] += sgn * step
+            b.set_theta(th)
+            lv.append(b.loss_grad(target, 0)[0])
+        b.set_theta(th0)
+        fd = (lv[0] - lv[1]) / (2 * step)
+        err = abs(fd - grad[k]) / max(abs(fd), 1e-12)
+        print(f"{name:12s} adjoint {grad[k]: .10e} fd {fd: .10e} rel {err:.2e}")
+        worst = max(worst, err)
+    b.close()
+    assert worst <= 1e-6, worst
+
+
 def test_autograd_function_and_graph_replay():
     import torch
     from mdff_amd.batch import Batch, force_match_loss, get_theta
