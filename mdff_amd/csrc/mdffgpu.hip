@@ -1336,8 +1336,7 @@ void update_list(mdffgpu_ctx* c) {
           na.cnt = c->d_cnt;
           na.nbr = c->d_nbr;
           na.overflow = c->d_ovf;
-          k_nlist_cells<<<(int)(((long long)n * 64 + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0,
-                          st>>>(na);
+          k_nlist_cells<<<n, 64, 0, st>>>(na);
         } else {
           // candidate ranges of whole tiles
           const int jlen =

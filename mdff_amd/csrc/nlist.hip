@@ -16,8 +16,8 @@
 // stencil row along the third axis are one contiguous range of the sorted
 // atoms (two when the row wraps).  The search gives one wave to each centre
 // atom: the lanes take the candidates of all stencil rows as one sequence,
-// and a ballot compacts the hits, so each row holds its neighbours in
-// stencil order, atoms of a cell by index -- a deterministic order.
+// a ballot compacts the hits, and the row is sorted by atom index in LDS,
+// the order of the all-pairs build.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -104,17 +104,24 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_cell_order(int m, const int* __r
   for (int r = b; r < e; ++r) rank[perm[r]] = r;
 }
 
-// Neighbour rows from the stencil: one wave per centre atom, the centres in
-// cell order.  Rows nbr[i * maxnb + k]; cnt[i] the row length; the longest
-// row above maxnb goes to overflow (the caller grows the list and runs
-// again).
-// The stencil rows live in the lanes (row r in lane r, at most 64 rows);
-// the lanes find the row of a candidate by binary lifting over the row
-// bases with wave shuffles, so the wave needs no LDS and no barrier.
-__global__ __launch_bounds__(MDFF_BLOCK) void k_nlist_cells(NlCellArgs a) {
-  const int lane = threadIdx.x & 63;
-  const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (wv >= a.n) return;  // the whole wave
+// Neighbour rows from the stencil: one wave (and workgroup) per centre
+// atom, the centres in cell order.  Rows nbr[i * maxnb + k]; cnt[i] the row
+// length; the longest row above maxnb goes to overflow (the caller grows
+// the list and runs again).  The stencil rows live in the lanes (row r in
+// lane r, at most 64 rows); the lanes find the row of a candidate by binary
+// lifting over the row bases with wave shuffles.  The hits are gathered in
+// LDS and sorted by atom index (bitonic, NL_SORT entries at most) before
+// they are written, so that a row lists its neighbours in index order as
+// the all-pairs build does: the pair kernels then meet the species in long
+// runs (the atoms of a species are contiguous in index), which keeps their
+// species-dependent work converged across a lane group.  Longer rows are
+// written unsorted.
+constexpr int NL_SORT = 1024;
+__global__ __launch_bounds__(64) void k_nlist_cells(NlCellArgs a) {
+  __shared__ int srow[NL_SORT];
+  const int lane = threadIdx.x;
+  const int wv = blockIdx.x;
+  if (wv >= a.n) return;  // the whole (one-wave) workgroup
   const int i = a.perm[wv];
   const double4 si = a.s4[i];
   const int c = a.cellof[i];
@@ -181,10 +188,38 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_nlist_cells(NlCellArgs a) {
     const unsigned long long mask = __ballot(hit);
     if (hit) {
       const int k = count + __popcll(mask & ((1ull << lane) - 1ull));
-      if (k < a.maxnb) row[k] = j;
+      if (k < NL_SORT)
+        srow[k] = j;
+      else if (k < a.maxnb)
+        row[k] = j;
     }
     count += __popcll(mask);
   }
+  __syncthreads();
+  if (count <= NL_SORT) {
+    int P = 1;
+    while (P < count) P <<= 1;
+    for (int k = count + lane; k < P; k += 64) srow[k] = 0x7fffffff;
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1)
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int t = lane; t < (P >> 1); t += 64) {
+          // compare-exchange pair (x, x + stride) of the t-th comparator
+          const int x = 2 * t - (t & (stride - 1));
+          const int y = x + stride;
+          const bool up = (x & size) == 0;
+          const int vx = srow[x], vy = srow[y];
+          if ((vx > vy) == up) {
+            srow[x] = vy;
+            srow[y] = vx;
+          }
+        }
+        __syncthreads();
+      }
+  }
+  // the first min(count, NL_SORT) entries, sorted when the row fits the LDS
+  const int nw = min(min(count, NL_SORT), a.maxnb);
+  for (int k = lane; k < nw; k += 64) row[k] = srow[k];
   if (lane == 0) {
     a.cnt[i] = count;
     if (count > a.maxnb) atomicMax(a.overflow, count);
