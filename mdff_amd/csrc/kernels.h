@@ -187,8 +187,8 @@ struct ExtrapArgs {
   double* th;     // op 5 with quadrupole polarizabilities: theta_ind = polquad E, or null
 };
 
-__global__ void k_prepare(int n, const double* pos, Cell cell, double* s, double4* s4,
-                          double2* eph);
+__global__ void k_prepare(int n, const double* pos, Cell cell, const int* ext, double* s,
+                          double4* s4, double2* eph);
 template <bool FILL>
 __global__ void k_nlist(int n, const double* s, Cell cell, double cut2, int maxnb, int jlen,
                         int* scnt, int* nbr);
@@ -216,6 +216,10 @@ __global__ void k_cell_scan(int m, int* cnt, int* cur);
 __global__ void k_cell_fill(int n, const int* cellof, int* cur, int* perm);
 __global__ void k_cell_order(int m, const int* cnt, int* perm, int* rank);
 __global__ void k_nlist_cells(NlCellArgs a);
+__global__ void k_perm_gather(int n, int inner, int planes, const int* idx, const double* src,
+                              double* dst);
+__global__ void k_perm_scatter(int n, int inner, int planes, const int* idx, const double* src,
+                               double* dst);
 
 // device-resident MD (md.hip)
 constexpr int MDFF_NHC_MAX = 16;
@@ -365,6 +369,7 @@ struct WfcArgs {
   double rcut;      // compared with the squared distance
   double* mu;       // (n,3), written for every atom
   int* bad;         // set to 1 + an ion without its lwfc centres
+  const int* ext;   // the caller's index of each atom (null: the identity)
 };
 __global__ void k_wfc(WfcArgs a);
 __global__ void k_extrap(ExtrapArgs a);

@@ -85,6 +85,15 @@ struct mdffgpu_ctx {
   // within a cell), refreshed with the neighbour list: perm[rank] = atom
   int* d_perm = nullptr;
   int* d_rank = nullptr;
+  // internal numbering of the atoms, fixed at the first evaluation: species
+  // by species, by cell of about 4 A (row-major) and caller index within a
+  // species, so that a neighbour row meets the species in long runs and
+  // neighbours sit close in memory.  int2ext[k] is the caller's index of
+  // internal atom k; every entry point gathers its per-atom inputs and
+  // scatters its outputs across it (renum false: the identity).
+  bool renum_decided = false, renum = false;
+  int* d_int2ext = nullptr;
+  double* d_xbuf = nullptr;  // staging of the boundary permutation
   // cell grid of the list build (nlist.hip): cells per axis, stencil
   // half-widths, whether the stencil search applies; cell of each atom,
   // first sorted slot of each cell, fill cursors
@@ -1258,6 +1267,142 @@ void cell_sort(mdffgpu_ctx* c) {
   check_launch(c, "cell sort");
 }
 
+// Per-atom species data of typeinfo_init (src/read_pos.f90:175-199) in the
+// context's numbering (typ0): species, charge, static dipole and, with
+// quadrupoles, the permanent quadrupole (quadia, :197).
+void load_species(mdffgpu_ctx* c) {
+  const mdffgpu_params& P = c->P;
+  const int n = c->n;
+  HCK(c, hipMemcpy(c->d_typ, c->typ0.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<double> q(n), dip(3 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const int t = c->typ0[i];
+    q[i] = P.qch[t];
+    for (int a = 0; a < 3; ++a) dip[i + (size_t)n * a] = P.dip[t + 16 * a];
+  }
+  HCK(c, hipMemcpy(c->d_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+  HCK(c, hipMemcpy(c->d_dip, dip.data(), dip.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (c->d_thstat) {
+    std::vector<double> th(9 * (size_t)n);
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 9; ++k) th[i + (size_t)n * k] = P.quad[c->typ0[i] + 16 * k];
+    HCK(c, hipMemcpy(c->d_thstat, th.data(), th.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+}
+
+// Internal numbering, decided once, at the first evaluation, from the
+// caller's positions (host arrays): key (species, cell of about 4 A on the
+// lattice, caller index).  The allocation-free context keeps the caller's
+// order.  The dipole history already set is carried over.
+void ensure_renum(mdffgpu_ctx* c, const double* rx, const double* ry, const double* rz) {
+  if (c->renum_decided) return;
+  c->renum_decided = true;
+  const int n = c->n;
+  if (c->noalloc || n < 2) return;
+  int nc[3];
+  for (int a = 0; a < 3; ++a) nc[a] = std::max(1, std::min(64, (int)(c->W[a] / 4.0)));
+  const long long ncell = (long long)nc[0] * nc[1] * nc[2];
+  const double* B = c->cell.B;
+  std::vector<long long> key(n);
+  for (int i = 0; i < n; ++i) {
+    const double x = rx[i], y = ry[i], z = rz[i];
+    int cc[3];
+    for (int a = 0; a < 3; ++a) {
+      double s = x * B[3 * a] + y * B[3 * a + 1] + z * B[3 * a + 2];
+      s -= std::floor(s);
+      int k = std::isfinite(s) ? (int)(s * nc[a]) : 0;
+      cc[a] = std::max(0, std::min(nc[a] - 1, k));
+    }
+    key[i] = (long long)c->typ0[i] * ncell + ((long long)cc[0] * nc[1] + cc[1]) * nc[2] + cc[2];
+  }
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+  bool ident = true;
+  for (int i = 0; i < n && ident; ++i) ident = ord[i] == i;
+  if (ident) return;
+  const int K = c->P.extrapolate_order + 1;
+  dalloc(c, &c->d_int2ext, (size_t)n);
+  dalloc(c, &c->d_xbuf, (size_t)std::max(9, 3 * K) * n);
+  HCK(c, hipMemcpy(c->d_int2ext, ord.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<int> t(n);
+  for (int i = 0; i < n; ++i) t[i] = c->typ0[ord[i]];
+  c->typ0.swap(t);
+  load_species(c);
+  c->renum = true;
+  const size_t nh = (size_t)3 * K * n;
+  k_perm_gather<<<(int)((nh + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0, c->st>>>(
+      n, K, 3, c->d_int2ext, c->d_hist, c->d_xbuf);
+  check_launch(c, "k_perm_gather");
+  HCK(c, hipMemcpyAsync(c->d_hist, c->d_xbuf, nh * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  HCK(c, hipStreamSynchronize(c->st));
+}
+
+// The same from positions on the device (SoA [x|y|z], caller's order).
+void ensure_renum_dev(mdffgpu_ctx* c, const double* d_pos) {
+  if (c->renum_decided) return;
+  if (c->noalloc || c->n < 2) {
+    c->renum_decided = true;
+    return;
+  }
+  const size_t n = c->n;
+  std::vector<double> h(3 * n);
+  HCK(c, hipMemcpyAsync(h.data(), d_pos, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HCK(c, hipStreamSynchronize(c->st));
+  ensure_renum(c, h.data(), h.data() + n, h.data() + 2 * n);
+}
+
+// Boundary copies across the numbering, (inner, n, planes) doubles with the
+// inner index fastest: gather_dev fills dst (internal numbering) from the
+// caller's device array src, scatter_dev the caller's dst from src.
+void gather_dev(mdffgpu_ctx* c, double* dst, const double* src, int planes, int inner = 1) {
+  const size_t cnt = (size_t)planes * inner * c->n;
+  if (!c->renum) {
+    HCK(c, hipMemcpyAsync(dst, src, cnt * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    return;
+  }
+  k_perm_gather<<<(int)((cnt + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0, c->st>>>(
+      c->n, inner, planes, c->d_int2ext, src, dst);
+  check_launch(c, "k_perm_gather");
+}
+
+void scatter_dev(mdffgpu_ctx* c, double* dst, const double* src, int planes, int inner = 1) {
+  const size_t cnt = (size_t)planes * inner * c->n;
+  if (!c->renum) {
+    HCK(c, hipMemcpyAsync(dst, src, cnt * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    return;
+  }
+  k_perm_scatter<<<(int)((cnt + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0, c->st>>>(
+      c->n, inner, planes, c->d_int2ext, src, dst);
+  check_launch(c, "k_perm_scatter");
+}
+
+// The same for host arrays of n doubles, one pointer per plane.
+void gather_host(mdffgpu_ctx* c, double* dst, const double* const* src, int planes) {
+  const size_t n = c->n, b = n * sizeof(double);
+  double* stage = c->renum ? c->d_xbuf : dst;
+  for (int p = 0; p < planes; ++p)
+    HCK(c, hipMemcpyAsync(stage + p * n, src[p], b, hipMemcpyHostToDevice, c->st));
+  if (c->renum) gather_dev(c, dst, stage, planes);
+}
+
+void scatter_host(mdffgpu_ctx* c, double* const* dst, const double* src, int planes) {
+  const size_t n = c->n, b = n * sizeof(double);
+  if (c->renum) {
+    scatter_dev(c, c->d_xbuf, src, planes);
+    src = c->d_xbuf;
+  }
+  for (int p = 0; p < planes; ++p)
+    HCK(c, hipMemcpyAsync(dst[p], src + p * n, b, hipMemcpyDeviceToHost, c->st));
+}
+
+// A contiguous host array of planes x n doubles.
+void scatter_host(mdffgpu_ctx* c, double* dst, const double* src, int planes) {
+  double* d[9];
+  for (int p = 0; p < planes; ++p) d[p] = dst + (size_t)p * c->n;
+  scatter_host(c, d, src, planes);
+}
+
 // Noise-floor estimate of the mixed SCF after a solve (scalars on the
 // host).  Only a stalled solve raises it, and never above FLOOR_CAP FP32
 // epsilons of the RMS static field; a solve that exits at the iteration
@@ -1369,7 +1514,8 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
   std::memset(en, 0, sizeof(*en));
   {
     ProfScope _ps(c, K_PREPARE);
-    k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->d_s, c->d_s4, c->d_eph);
+    k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->renum ? c->d_int2ext : nullptr,
+                                         c->d_s, c->d_s4, c->d_eph);
   }
   check_launch(c, "k_prepare");
   update_list(c);
@@ -1439,6 +1585,7 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
       wa.rcut = P.rcut_wfc;
       wa.mu = c->d_muwfc;
       wa.bad = c->d_wfcbad;
+      wa.ext = c->renum ? c->d_int2ext : nullptr;
       {
         ProfScope _ps(c, K_WFC);
         k_wfc<<<gb, MDFF_BLOCK, 0, st>>>(wa);
@@ -1455,14 +1602,11 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
     ++n_es;
   }
   // device outputs queued ahead of the one synchronisation of the call
-  const size_t nb = (size_t)n * sizeof(double);
-  if (c->out_force)
-    HCK(c, hipMemcpyAsync(c->out_force, c->d_force, 3 * nb, hipMemcpyDeviceToDevice, st));
+  if (c->out_force) scatter_dev(c, c->out_force, c->d_force, 3);
   if (P.lcoulomb) {
-    if (c->out_mu) HCK(c, hipMemcpyAsync(c->out_mu, c->d_mu, 3 * nb, hipMemcpyDeviceToDevice, st));
-    if (c->out_ef) HCK(c, hipMemcpyAsync(c->out_ef, c->d_ef, 3 * nb, hipMemcpyDeviceToDevice, st));
-    if (c->out_efg)
-      HCK(c, hipMemcpyAsync(c->out_efg, c->d_efg, 9 * nb, hipMemcpyDeviceToDevice, st));
+    if (c->out_mu) scatter_dev(c, c->out_mu, c->d_mu, 3);
+    if (c->out_ef) scatter_dev(c, c->out_ef, c->d_ef, 3);
+    if (c->out_efg) scatter_dev(c, c->out_efg, c->d_efg, 9);
   }
   sync_scalars(c);
   update_floor(c);
@@ -1575,7 +1719,8 @@ void efg_frame(mdffgpu_ctx* c, const mdffgpu_efg_opts& o, mdffgpu_energies* en) 
   ++c->evals;
   {
     ProfScope _ps(c, K_PREPARE);
-    k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->d_s, c->d_s4, c->d_eph);
+    k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->renum ? c->d_int2ext : nullptr,
+                                         c->d_s, c->d_s4, c->d_eph);
   }
   check_launch(c, "k_prepare");
   // get_moments (src/field.f90:3686-3765) with calc = 'efg': the SCF starts
@@ -1774,9 +1919,9 @@ void efg_frame(mdffgpu_ctx* c, const mdffgpu_efg_opts& o, mdffgpu_energies* en) 
 // The context of mdffgpu_create; with arena (the allocation-free entry
 // point) every device buffer comes from that memory instead of hipMalloc,
 // the host scalars are pageable and the work goes to the caller's stream.
-// Quadrupole and Wannier-centre state (FP64 path): the permanent
-// quadrupoles of typeinfo_init (quadia, src/read_pos.f90:197), buffers of
-// the induced and total quadrupoles, the Wannier-centre dipoles.
+// Quadrupole and Wannier-centre state (FP64 path): buffers of the
+// permanent (filled by load_species), induced and total quadrupoles, the
+// Wannier-centre dipoles.
 void setup_quad_wfc(mdffgpu_ctx* c) {
   const mdffgpu_params& P = c->P;
   const int n = c->n;
@@ -1793,10 +1938,6 @@ void setup_quad_wfc(mdffgpu_ctx* c) {
     dalloc(c, &c->d_thstat, 9 * (size_t)n);
     dalloc(c, &c->d_thind, 9 * (size_t)n);
     dalloc(c, &c->d_theta, 9 * (size_t)n);
-    std::vector<double> th(9 * (size_t)n);
-    for (int i = 0; i < n; ++i)
-      for (int k = 0; k < 9; ++k) th[i + (size_t)n * k] = P.quad[c->typ0[i] + 16 * k];
-    HCK(c, hipMemcpy(c->d_thstat, th.data(), th.size() * sizeof(double), hipMemcpyHostToDevice));
     HCK(c, hipMemsetAsync(c->d_thind, 0, 9 * (size_t)n * sizeof(double), c->st));
     HCK(c, hipMemsetAsync(c->d_theta, 0, 9 * (size_t)n * sizeof(double), c->st));
   }
@@ -1843,17 +1984,8 @@ void create_impl(const mdffgpu_params* params, const int32_t* itype, const doubl
     set_lattice(c, A);
     alloc_atoms(c);
     build_tables(c);
-    HCK(c, hipMemcpy(c->d_typ, c->typ0.data(), c->n * sizeof(int), hipMemcpyHostToDevice));
-    // per-atom moments of typeinfo_init (src/read_pos.f90:175-199)
-    std::vector<double> q(c->n), dip(3 * (size_t)c->n);
-    for (int i = 0; i < c->n; ++i) {
-      const int t = c->typ0[i];
-      q[i] = c->P.qch[t];
-      for (int a = 0; a < 3; ++a) dip[i + (size_t)c->n * a] = c->P.dip[t + 16 * a];
-    }
-    HCK(c, hipMemcpy(c->d_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
-    HCK(c, hipMemcpy(c->d_dip, dip.data(), dip.size() * sizeof(double), hipMemcpyHostToDevice));
     if (c->P.lcoulomb) setup_quad_wfc(c);
+    load_species(c);
     update_cutoffs(c);
     if (c->list_cut > 0.0) set_list_capacity(c, initial_capacity(c));
     if (c->P.lcoulomb) build_kmesh(c);
@@ -1875,13 +2007,10 @@ void create_impl(const mdffgpu_params* params, const int32_t* itype, const doubl
 
 void copy_in_host(mdffgpu_ctx* c, const double* rx, const double* ry, const double* rz,
                   const double* fx, const double* fy, const double* fz) {
-  const size_t n = c->n, b = n * sizeof(double);
-  HCK(c, hipMemcpyAsync(c->d_pos, rx, b, hipMemcpyHostToDevice, c->st));
-  HCK(c, hipMemcpyAsync(c->d_pos + n, ry, b, hipMemcpyHostToDevice, c->st));
-  HCK(c, hipMemcpyAsync(c->d_pos + 2 * n, rz, b, hipMemcpyHostToDevice, c->st));
-  HCK(c, hipMemcpyAsync(c->d_force, fx, b, hipMemcpyHostToDevice, c->st));
-  HCK(c, hipMemcpyAsync(c->d_force + n, fy, b, hipMemcpyHostToDevice, c->st));
-  HCK(c, hipMemcpyAsync(c->d_force + 2 * n, fz, b, hipMemcpyHostToDevice, c->st));
+  const double* r[3] = {rx, ry, rz};
+  const double* f[3] = {fx, fy, fz};
+  gather_host(c, c->d_pos, r, 3);
+  gather_host(c, c->d_force, f, 3);
 }
 
 }  // namespace
@@ -2137,7 +2266,7 @@ void mdffgpu_destroy(mdffgpu_ctx* c) {
                 c->d_chunks, c->d_pt, c->d_tt, c->d_mu4, c->d_s4, c->d_perm, c->d_rank,
                 c->d_nlr, c->d_cellof, c->d_cstart, c->d_ccur, c->d_vel, c->d_fs,
                 c->d_massia, c->d_nhc, c->d_thstat, c->d_thind, c->d_theta, c->d_muwfc,
-                c->d_wfcbad};
+                c->d_wfcbad, c->d_int2ext, c->d_xbuf};
   for (void* p : vp)
     if (p) (void)hipFree(p);
   for (auto& p : c->prof_pending) {
@@ -2178,20 +2307,20 @@ int mdffgpu_engforce(mdffgpu_ctx* c, const double* rx, const double* ry, const d
   }
   try {
     c->st = c->own_stream;
+    ensure_renum(c, rx, ry, rz);
     copy_in_host(c, rx, ry, rz, fx, fy, fz);
     run(c, itime, en);
-    const size_t n = c->n, b = n * sizeof(double);
-    HCK(c, hipMemcpyAsync(fx, c->d_force, b, hipMemcpyDeviceToHost, c->st));
-    HCK(c, hipMemcpyAsync(fy, c->d_force + n, b, hipMemcpyDeviceToHost, c->st));
-    HCK(c, hipMemcpyAsync(fz, c->d_force + 2 * n, b, hipMemcpyDeviceToHost, c->st));
+    const size_t b = (size_t)c->n * sizeof(double);
+    double* f[3] = {fx, fy, fz};
+    scatter_host(c, f, c->d_force, 3);
     if (c->P.lcoulomb) {
-      if (mu_t) HCK(c, hipMemcpyAsync(mu_t, c->d_mu, 3 * b, hipMemcpyDeviceToHost, c->st));
-      if (ef_t) HCK(c, hipMemcpyAsync(ef_t, c->d_ef, 3 * b, hipMemcpyDeviceToHost, c->st));
-      if (efg_t) HCK(c, hipMemcpyAsync(efg_t, c->d_efg, 9 * b, hipMemcpyDeviceToHost, c->st));
+      if (mu_t) scatter_host(c, mu_t, c->d_mu, 3);
+      if (ef_t) scatter_host(c, ef_t, c->d_ef, 3);
+      if (efg_t) scatter_host(c, efg_t, c->d_efg, 9);
     }
     if (theta_t) {
       if (c->P.lcoulomb && c->lquad)
-        HCK(c, hipMemcpyAsync(theta_t, c->d_theta, 9 * b, hipMemcpyDeviceToHost, c->st));
+        scatter_host(c, theta_t, c->d_theta, 9);
       else
         std::memset(theta_t, 0, 9 * b);
     }
@@ -2213,9 +2342,9 @@ int mdffgpu_engforce_device(mdffgpu_ctx* c, const double* d_pos, int32_t itime,
   }
   try {
     c->st = stream ? (hipStream_t)stream : c->own_stream;
-    const size_t n = c->n, b = n * sizeof(double);
-    HCK(c, hipMemcpyAsync(c->d_pos, d_pos, 3 * b, hipMemcpyDeviceToDevice, c->st));
-    HCK(c, hipMemcpyAsync(c->d_force, d_force, 3 * b, hipMemcpyDeviceToDevice, c->st));
+    ensure_renum_dev(c, d_pos);
+    gather_dev(c, c->d_pos, d_pos, 3);
+    gather_dev(c, c->d_force, d_force, 3);
     // the outputs are copied by run() ahead of its final synchronisation
     c->out_force = d_force;
     c->out_mu = d_mu;
@@ -2443,20 +2572,20 @@ int mdffgpu_md_run(mdffgpu_ctx* c, const mdffgpu_md_params* mp, int32_t itime0, 
     }
   try {
     c->st = stream ? (hipStream_t)stream : c->own_stream;
-    const size_t b = 3 * (size_t)c->n * sizeof(double);
     if (!c->d_vel) {
       dalloc(c, &c->d_vel, 3 * (size_t)c->n);
       dalloc(c, &c->d_fs, 3 * (size_t)c->n);
       dalloc(c, &c->d_massia, (size_t)c->n);
       dalloc(c, &c->d_nhc, 1);
     }
-    HCK(c, hipMemcpyAsync(c->d_pos, d_pos, b, hipMemcpyDeviceToDevice, c->st));
-    HCK(c, hipMemcpyAsync(c->d_vel, d_vel, b, hipMemcpyDeviceToDevice, c->st));
-    HCK(c, hipMemcpyAsync(c->d_force, d_force, b, hipMemcpyDeviceToDevice, c->st));
+    ensure_renum_dev(c, d_pos);
+    gather_dev(c, c->d_pos, d_pos, 3);
+    gather_dev(c, c->d_vel, d_vel, 3);
+    gather_dev(c, c->d_force, d_force, 3);
     md_loop(c, *mp, itime0, nsteps, xi, vxi, steps);
-    HCK(c, hipMemcpyAsync(d_pos, c->d_pos, b, hipMemcpyDeviceToDevice, c->st));
-    HCK(c, hipMemcpyAsync(d_vel, c->d_vel, b, hipMemcpyDeviceToDevice, c->st));
-    HCK(c, hipMemcpyAsync(d_force, c->d_force, b, hipMemcpyDeviceToDevice, c->st));
+    scatter_dev(c, d_pos, c->d_pos, 3);
+    scatter_dev(c, d_vel, c->d_vel, 3);
+    scatter_dev(c, d_force, c->d_force, 3);
     HCK(c, hipStreamSynchronize(c->st));
     c->st = c->own_stream;
     return MDFFGPU_OK;
@@ -2579,14 +2708,13 @@ int mdffgpu_efg(mdffgpu_ctx* c, const mdffgpu_efg_opts* o, const double* rx, con
   }
   try {
     c->st = c->own_stream;
-    const size_t n = c->n, b = n * sizeof(double);
-    HCK(c, hipMemcpyAsync(c->d_pos, rx, b, hipMemcpyHostToDevice, c->st));
-    HCK(c, hipMemcpyAsync(c->d_pos + n, ry, b, hipMemcpyHostToDevice, c->st));
-    HCK(c, hipMemcpyAsync(c->d_pos + 2 * n, rz, b, hipMemcpyHostToDevice, c->st));
+    ensure_renum(c, rx, ry, rz);
+    const double* r[3] = {rx, ry, rz};
+    gather_host(c, c->d_pos, r, 3);
     mdffgpu_energies tmp;
     efg_frame(c, *o, en ? en : &tmp);
-    HCK(c, hipMemcpyAsync(efg_t, c->d_efgout, 9 * b, hipMemcpyDeviceToHost, c->st));
-    if (mu_t) HCK(c, hipMemcpyAsync(mu_t, c->d_mu, 3 * b, hipMemcpyDeviceToHost, c->st));
+    scatter_host(c, efg_t, c->d_efgout, 9);
+    if (mu_t) scatter_host(c, mu_t, c->d_mu, 3);
     HCK(c, hipStreamSynchronize(c->st));
     return MDFFGPU_OK;
   } catch (Fail& f) {
@@ -2633,22 +2761,35 @@ int mdffgpu_efg_nmr(int32_t n, const double* efg6, double* nmr, double* U, doubl
 
 int mdffgpu_get_dipole_history(mdffgpu_ctx* c, double* hist) {
   if (!c || !hist) return MDFFGPU_ERR_ARG;
-  size_t cnt = (size_t)(c->P.extrapolate_order + 1) * 3 * c->n;
-  if (hipMemcpy(hist, c->d_hist, cnt * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
-    c->err = "hipMemcpy of the dipole history failed";
-    return MDFFGPU_ERR_DEVICE;
+  const int K = c->P.extrapolate_order + 1;
+  const size_t cnt = (size_t)K * 3 * c->n;
+  try {
+    const double* src = c->d_hist;
+    if (c->renum) {
+      scatter_dev(c, c->d_xbuf, c->d_hist, 3, K);
+      src = c->d_xbuf;
+    }
+    HCK(c, hipMemcpyAsync(hist, src, cnt * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HCK(c, hipStreamSynchronize(c->st));
+    return MDFFGPU_OK;
+  } catch (Fail& f) {
+    return f.code;
   }
-  return MDFFGPU_OK;
 }
 
 int mdffgpu_set_dipole_history(mdffgpu_ctx* c, const double* hist) {
   if (!c || !hist) return MDFFGPU_ERR_ARG;
-  size_t cnt = (size_t)(c->P.extrapolate_order + 1) * 3 * c->n;
-  if (hipMemcpy(c->d_hist, hist, cnt * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-    c->err = "hipMemcpy of the dipole history failed";
-    return MDFFGPU_ERR_DEVICE;
+  const int K = c->P.extrapolate_order + 1;
+  const size_t cnt = (size_t)K * 3 * c->n;
+  try {
+    double* dst = c->renum ? c->d_xbuf : c->d_hist;
+    HCK(c, hipMemcpyAsync(dst, hist, cnt * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (c->renum) gather_dev(c, c->d_hist, c->d_xbuf, 3, K);
+    HCK(c, hipStreamSynchronize(c->st));
+    return MDFFGPU_OK;
+  } catch (Fail& f) {
+    return f.code;
   }
-  return MDFFGPU_OK;
 }
 
 int32_t mdffgpu_num_kvectors(const mdffgpu_ctx* c) { return c ? c->nk : 0; }

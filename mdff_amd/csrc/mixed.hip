@@ -32,7 +32,7 @@ typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // 16 raw bytes
 
-constexpr int RHO_ATOMS_PER_SPLIT = 2048;  // default atoms per structure-factor split
+constexpr int RHO_ATOMS_PER_SPLIT = 1024;  // default atoms per structure-factor split
 constexpr float DIP_SCALE = DIP_WEIGHT_SCALE;  // dipole weights are scaled by 2^8
 // The apply operand w g of each n1 plane is normalised to |.| <= APPLY_SCALE
 // (a power of two): most elements of a plane are far below its maximum, and

@@ -226,4 +226,32 @@ __global__ __launch_bounds__(64) void k_nlist_cells(NlCellArgs a) {
   }
 }
 
+// Per-atom arrays across the context's internal numbering, laid out
+// (inner, n, planes) with the inner index fastest: gather dst(k, i, p) =
+// src(k, idx[i], p), scatter dst(k, idx[i], p) = src(k, i, p), with idx[i]
+// the caller's index of internal atom i.
+__global__ __launch_bounds__(MDFF_BLOCK) void k_perm_gather(int n, int inner, int planes,
+                                                            const int* __restrict__ idx,
+                                                            const double* __restrict__ src,
+                                                            double* __restrict__ dst) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t tot = (size_t)n * inner * planes;
+  if (t >= tot) return;
+  const size_t k = t % inner, r = t / inner;
+  const size_t i = r % n, p = r / n;
+  dst[t] = src[k + (size_t)inner * (idx[i] + (size_t)n * p)];
+}
+
+__global__ __launch_bounds__(MDFF_BLOCK) void k_perm_scatter(int n, int inner, int planes,
+                                                             const int* __restrict__ idx,
+                                                             const double* __restrict__ src,
+                                                             double* __restrict__ dst) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t tot = (size_t)n * inner * planes;
+  if (t >= tot) return;
+  const size_t k = t % inner, r = t / inner;
+  const size_t i = r % n, p = r / n;
+  dst[k + (size_t)inner * (idx[i] + (size_t)n * p)] = src[t];
+}
+
 }  // namespace mdff

@@ -12,7 +12,8 @@
 // and no force or stress.  Every energy term is symmetric under the swap
 // of the pair, so each centre atom takes half of it; the fields follow from
 // T3(-r) = -T3(r); the quadrupole-quadrupole EFG takes the sign of the
-// pair's order, + at the smaller index ('ia' of the all-pairs loop i < j).
+// pair's order, + at the smaller index ('ia' of the all-pairs loop i < j) in
+// the caller's numbering (s4.w).
 // With symmetric Q the contractions reduce to u = Q r, s = r.Q.r and
 // tr Q:
 //   T2:Q      = s F2' - tr F1'
@@ -68,6 +69,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_quad(QuadArgs a) {
   double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < n) {
     const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
+    const double wi = a.s4[i].w;  // the caller's index of atom i
     const double qi = a.q[i];
     const double mi[3] = {a.mu[i], a.mu[i + n], a.mu[i + 2 * n]};
     double Qi[6];
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_es_quad(QuadArgs a) {
         if (EF)
           for (int x = 0; x < 3; ++x) e[x] -= v3[x];
         if (EFG) {
-          const double sg = (i < j) ? 1.0 : -1.0;
+          const double sg = (wi < sj.w) ? 1.0 : -1.0;
           // (T4:Q_j)_ab for ab = xx yy zz xy xz yz
           const int ia_[6] = {0, 1, 2, 0, 0, 1}, ib_[6] = {0, 1, 2, 1, 2, 2};
           for (int m = 0; m < 6; ++m) {
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_wfc(WfcArgs a) {
       acc[1] += ry;
       acc[2] += rz;
     }
-    if (found != a.lwfc[t]) atomicMax(a.bad, i + 1);
+    if (found != a.lwfc[t]) atomicMax(a.bad, (a.ext ? a.ext[i] : i) + 1);
   }
   for (int x = 0; x < 3; ++x) a.mu[i + (size_t)x * n] = -2.0 * acc[x];
 }

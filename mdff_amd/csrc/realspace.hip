@@ -178,7 +178,8 @@ template __global__ void k_shortrange<float, true>(SrArgs);
 //   energy     qq  q_i q_j T0 / 2                     dd  -mu_i T2 mu_j / 2
 //              qd  q_j (T1 - T1d(p_i)) . mu_i
 // where p_i are the damping parameters the reference uses for atom i of the
-// pair (dmp_* slot 0 if i < j, slot 1 otherwise).
+// pair (dmp_* slot 0 if i < j in the caller's numbering, s4.w, slot 1
+// otherwise).
 // Partial rows: 0 u, 1..6 tau (xx yy zz xy xz yz), before the 1/(2 Omega).
 template <int MODE, typename R, bool BATCH>
 __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
@@ -227,6 +228,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
   R f0 = 0, f1 = 0, f2 = 0;
   if (i < n) {
     const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
+    // the caller's index of atom i (s4.w), for the pair-order convention
+    const double wi = BATCH ? 0.0 : a.s4[i].w;
     const int ti = a.typ[i];
     const R qi = R(a.q[i]);
     const R mi0 = R(a.mu[i]), mi1 = R(a.mu[i + n]), mi2 = R(a.mu[i + 2 * n]);
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
       const bool dip_j = (mj0 != R(0) || mj1 != R(0) || mj2 != R(0));
       const bool dd = DD && (dip_i || dip_j);
       const bool qd = QD && (dip_i || dip_j);
-      const int o = (i < j) ? 0 : 1;
+      const int o = (BATCH ? (i < j) : (wi < sj.w)) ? 0 : 1;
       const int pij = ti + MDFF_NT * tj;
       const int pji = tj + MDFF_NT * ti;
       float4 dmij = make_float4(0.f, 0.f, 0.f, 0.f);

@@ -208,7 +208,7 @@ class ForceField:
         history included), so tune a context that is not mid-trajectory."""
         import torch
         if candidates is None:
-            candidates = [(a, b) for a in (1, 2, 4) for b in (1024, 2048, 4096)]
+            candidates = [(a, b) for a in (1, 2, 4) for b in (512, 1024, 2048)]
         if not self.set_mixed_tuning(*candidates[0]):
             return {}
         times = []
