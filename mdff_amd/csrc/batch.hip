@@ -1330,7 +1330,7 @@ void build_pairs(mdffgpu_batch* b) {
     float4* tab = reinterpret_cast<float4*>(b->d_t2);
     int* tj = reinterpret_cast<int*>(tab + cap);
     da.dd16 = dd_idx16(b->N);
-    k_dd_table<true><<<nblk((size_t)b->N * 64), MDFF_BLOCK, 0, b->st>>>(da, tab, tj, tj + cap);
+    k_dd_table<true, float4><<<nblk((size_t)b->N * 64), MDFF_BLOCK, 0, b->st>>>(da, tab, tj, tj + cap);
   }
   launch_check(b, "batch pair table");
 }

@@ -105,7 +105,7 @@ struct EsDirArgs {
   const int* perm;     // atoms in this order (spatial locality), or null
   // FP32 static pass of a single configuration: the compacted SCF pair
   // table (k_dd_table's layout) written on the way, or null
-  float4* ddtab;
+  float2* ddtab;
   int* ddtj;
   int* ddcnt;
   int dd16;  // pair table: neighbour slots as 16-bit indexes (n <= 65536)
@@ -263,11 +263,16 @@ __global__ void k_dd_field(EsDirArgs a, const double* tab, const double4* mu4);
 template <typename R4>
 __global__ void k_pack_mu(int n, const double* mu, R4* mu4);
 __global__ void k_pack_mu_ranked(int n, const double* mu, const int* rank, float4* mu4);
-template <bool BATCH = false>
-__global__ void k_dd_table(EsDirArgs a, float4* tab, int* tj, int* tcnt);
+template <bool BATCH, typename E>
+__global__ void k_dd_table(EsDirArgs a, E* tab, int* tj, int* tcnt);
 template <typename IT>
 __global__ void k_dd_tab(int n, int maxnb, const float4* tab, const IT* tj, const int* tcnt,
                          const float4* mu4, const int* perm, double* ef);
+__global__ void k_pack_u4(int n, const double* s, const int* rank, uint4* u4);
+template <typename IT>
+__global__ void k_dd_tab2(int n, int maxnb, const float2* tab, const IT* tj, const int* tcnt,
+                          const float4* mu4, const uint4* u4, Cell cell, const int* perm,
+                          double* ef);
 // neighbour slots of the FP32 pair table fit 16 bits
 inline bool dd_idx16(long long n) { return n <= 65536; }
 // Grid of k_dd_tab (persistent waves): four per SIMD of the device, at most

@@ -557,6 +557,15 @@ bool scf_field_fp32(const mdffgpu_ctx* c) {
 // Real-space part of the SCF dipole field (ES_IND): E_dir of the dipoles mu
 // into d_efdir, from the pair table of the evaluation.  With packed, the
 // FP32 rank-ordered copy of mu is already in d_mu4 (k_induced wrote it).
+// FP32 SCF pair table of the mixed path in d_t2: the evaluation's
+// fixed-point positions (uint4 per rank), the (A2, A1) entries and the
+// neighbour slots (maxnb per row), the row counts.
+uint4* dd_u4(mdffgpu_ctx* c) { return reinterpret_cast<uint4*>(c->d_t2); }
+float2* dd_tab(mdffgpu_ctx* c) { return reinterpret_cast<float2*>(dd_u4(c) + c->n); }
+int* dd_tj(mdffgpu_ctx* c) {
+  return reinterpret_cast<int*>(dd_tab(c) + (size_t)c->maxnb * c->n);
+}
+
 void es_ind_dir(mdffgpu_ctx* c, const double* mu, bool packed = false) {
   const int n = c->n;
   hipStream_t st = c->st;
@@ -565,16 +574,17 @@ void es_ind_dir(mdffgpu_ctx* c, const double* mu, bool packed = false) {
   if (scf_field_fp32(c)) {
     float4* m4 = reinterpret_cast<float4*>(c->d_mu4);
     const size_t cap = (size_t)c->maxnb * n;
-    const float4* tab = reinterpret_cast<const float4*>(c->d_t2);
-    const int* tj = reinterpret_cast<const int*>(tab + cap);
+    const float2* tab = dd_tab(c);
+    const int* tj = dd_tj(c);
     if (!packed) k_pack_mu_ranked<<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_rank, m4);
     if (dd_idx16(n))
-      k_dd_tab<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(
-          n, c->maxnb, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4, c->d_perm,
-          c->d_efdir);
+      k_dd_tab2<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(
+          n, c->maxnb, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4, dd_u4(c),
+          c->cell, c->d_perm, c->d_efdir);
     else
-      k_dd_tab<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4,
-                                                        c->d_perm, c->d_efdir);
+      k_dd_tab2<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4,
+                                                         dd_u4(c), c->cell, c->d_perm,
+                                                         c->d_efdir);
   } else {
     EsDirArgs da;
     std::memset(&da, 0, sizeof(da));
@@ -632,8 +642,8 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   if (mode == ES_STATIC && mx && c->t2_in_static) {
     // the evaluation's FP32 SCF pair table, from the same pairs and terms
     const size_t cap = (size_t)c->maxnb * n;
-    da.ddtab = reinterpret_cast<float4*>(c->d_t2);
-    da.ddtj = reinterpret_cast<int*>(da.ddtab + cap);
+    da.ddtab = dd_tab(c);
+    da.ddtj = dd_tj(c);
     da.ddcnt = da.ddtj + cap;
     da.rank = c->d_rank;
     da.dd16 = dd_idx16(n);
@@ -862,10 +872,11 @@ void build_t2(mdffgpu_ctx* c) {
   const int n = c->n;
   const bool mx = (c->P.precision == MDFFGPU_PREC_MIXED) &&
                   !(c->P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_SCF_FIELD);
-  // FP64: five doubles per list entry; mixed: (v, A1) float4 + index per
-  // entry, then the per-atom counts
+  // FP64: five doubles per list entry; mixed: the fixed-point positions,
+  // (A2, A1) float2 + index per entry, the per-atom counts
   const size_t cap = (size_t)c->maxnb * n;
-  const size_t need = mx ? cap * (sizeof(float4) + sizeof(int)) + (size_t)n * sizeof(int)
+  const size_t need = mx ? (size_t)n * sizeof(uint4) + cap * (sizeof(float2) + sizeof(int)) +
+                               (size_t)n * sizeof(int)
                          : 5 * cap * sizeof(double);
   if (need > c->t2_cap) {
     char* t2 = static_cast<char*>(c->d_t2);
@@ -889,13 +900,18 @@ void build_t2(mdffgpu_ctx* c) {
   da.dd16 = dd_idx16(n);
   // the FP32 table with an FP32 static real-space pass: written by that
   // pass (the SCF's first Ewald call, after this one in every evaluation)
+  if (mx) {
+    // positions are fixed during the SCF: the fixed-point copy of this
+    // evaluation, in the rank order of the dipole slots
+    k_pack_u4<<<nblk(n), MDFF_BLOCK, 0, c->st>>>(n, c->d_s, c->d_rank, dd_u4(c));
+    check_launch(c, "k_pack_u4");
+  }
   c->t2_in_static = mx && !(c->P.mixed_fp64_parts & MDFFGPU_MIXED_FP64_DIRECT);
   if (c->t2_in_static) return;
   ProfScope _ps(c, K_PAIRS_T2);
   if (mx) {
-    float4* tab = reinterpret_cast<float4*>(c->d_t2);
-    int* tj = reinterpret_cast<int*>(tab + cap);
-    k_dd_table<<<nblk(n * 64), MDFF_BLOCK, 0, c->st>>>(da, tab, tj, tj + cap);
+    int* tj = dd_tj(c);
+    k_dd_table<false, float2><<<nblk(n * 64), MDFF_BLOCK, 0, c->st>>>(da, dd_tab(c), tj, tj + cap);
   } else {
     k_pairs_t2<<<nblk(n * LPA), MDFF_BLOCK, 0, c->st>>>(da, (double*)c->d_t2);
   }

@@ -2,6 +2,8 @@
 // real-space multipole sum), templated on the arithmetic type R: double for
 // the plain FP64 path, float for the mixed path.  Block partial sums of
 // energies, virials and stress are always reduced in double.
+#include <type_traits>
+
 #include "kernels.h"
 #include "rtypes.h"
 
@@ -299,10 +301,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
       const R F2_dm5 = R(3) * F2 * dm5;
       if constexpr (TAB) {
         if (a.ddtab) {
-          // (v, A1), v = r sqrt(3 F2 / d^5), A1 = F1 / d^3 (undamped)
-          const R sA2 = sqrtf(F2_dm5);
+          // (A2, A1), A2 = 3 F2 / d^5, A1 = F1 / d^3 (undamped)
           const size_t e = (size_t)i0 * a.maxnb + tpos;
-          a.ddtab[e] = make_float4(rx * sA2, ry * sA2, rz * sA2, F1_dm3);
+          a.ddtab[e] = make_float2(F2_dm5, F1_dm3);
           const int js = a.rank ? a.rank[j] : j;
           if (a.dd16)
             reinterpret_cast<unsigned short*>(a.ddtj)[e] = (unsigned short)js;
@@ -439,7 +440,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
     if (a.ddtab && i < n && sub == 0) {  // lane 0 visits every slot
       a.ddcnt[i0] = tbase;
       if (tbase == 0) {  // a row without pairs holds one zero entry (k_dd_tab reads it)
-        a.ddtab[(size_t)i0 * a.maxnb] = make_float4(0.f, 0.f, 0.f, 0.f);
+        a.ddtab[(size_t)i0 * a.maxnb] = make_float2(0.f, 0.f);
         if (a.dd16)
           reinterpret_cast<unsigned short*>(a.ddtj)[(size_t)i0 * a.maxnb] = 0;
         else
@@ -620,18 +621,22 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a,
 // ---------------------------------------------------------------------------
 // SCF dipole field of the mixed path, from a compacted FP32 pair table.  Once
 // per evaluation k_dd_table (or the static FP32 real-space pass) keeps, per
-// atom and in list order, only the list entries inside the cutoff, as
-// (v, A1) with v = r sqrt(A2), A2 = 3F2/d^5, A1 = F1/d^3 (the undamped d-d
-// terms of src/field.f90:2407-2421, formed in FP32 from an FP64 fractional
-// difference), plus the neighbour's slot in the dipole array (its spatial
+// atom and in list order, only the list entries inside the cutoff, with
+// A2 = 3F2/d^5, A1 = F1/d^3 (the undamped d-d terms of
+// src/field.f90:2407-2421, formed in FP32 from an FP64 fractional
+// difference) and the neighbour's slot in the dipole array (its spatial
 // rank when a.rank is given, else its index).  Rows and counts are in the
 // processing order i0 (atom perm[i0] when perm is given).  Each SCF
-// iteration k_dd_tab then applies E_i = sum_j [v (v.mu_j) - A1 mu_j]: 20
-// bytes of table and one dipole gather per pair.
-template <bool BATCH>
-__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __restrict__ tab,
+// iteration then applies E_i = sum_j [A2 r (r.mu_j) - A1 mu_j].  The entry
+// of a single configuration is (A2, A1) and k_dd_tab2 forms r from 32-bit
+// fixed-point positions (10 bytes of table and one 32-byte gather per
+// pair); the batch keeps (v, A1) with v = r sqrt(A2) for k_dd_tab (18
+// bytes and a 16-byte gather).
+template <bool BATCH, typename E>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, E* __restrict__ tab,
                                                          int* __restrict__ tj,
                                                          int* __restrict__ tcnt) {
+  constexpr bool RV = std::is_same<E, float4>::value;  // (v, A1) entries
   const int n = a.n;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i0 = tid >> 6, lane = tid & 63;
@@ -651,7 +656,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
   int base = 0;
   for (int k0 = 0; k0 < c; k0 += 64) {
     const int k = k0 + lane;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    E v;
+    if constexpr (RV)
+      v = make_float4(0.f, 0.f, 0.f, 0.f);
+    else
+      v = make_float2(0.f, 0.f);
     int j = 0;
     bool in = false;
     if (k < c) {
@@ -667,8 +676,12 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
         const float expon = __expf(-alpha2 * d2) * 0.56418958354775628f;  // 1/sqrt(pi)
         const float F1 = errfc_t<float>(alpha * d) + 2.f * alpha * d * expon;
         const float F2 = F1 + c43a3 * d2 * d * expon;
-        const float sA2 = sqrtf(3.f * F2) * dm1 * dm1 * sqrtf(dm1);
-        v = make_float4(rx * sA2, ry * sA2, rz * sA2, F1 * dm3);
+        if constexpr (RV) {
+          const float sA2 = sqrtf(3.f * F2) * dm1 * dm1 * sqrtf(dm1);
+          v = make_float4(rx * sA2, ry * sA2, rz * sA2, F1 * dm3);
+        } else {
+          v = make_float2(3.f * F2 * dm3 * dm1 * dm1, F1 * dm3);
+        }
       }
     }
     const unsigned long long m = __ballot(in);
@@ -686,7 +699,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float4* __
   if (lane == 0) {
     tcnt[i0] = base;
     if (base == 0) {  // a row without pairs holds one zero entry (k_dd_tab reads it)
-      tab[row] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if constexpr (RV)
+        tab[row] = make_float4(0.f, 0.f, 0.f, 0.f);
+      else
+        tab[row] = make_float2(0.f, 0.f);
       if (a.dd16)
         reinterpret_cast<unsigned short*>(tj)[row] = 0;
       else
@@ -798,10 +814,131 @@ template __global__ void k_dd_tab<int>(int, int, const float4*, const int*, cons
                                        const float4*, const int*, double*);
 template __global__ void k_dd_tab<unsigned short>(int, int, const float4*, const unsigned short*,
                                                   const int*, const float4*, const int*, double*);
+
+// Fractional coordinates as 32-bit fixed point in [0, 1), rank ordered:
+// the difference of two, read as a signed integer, is the minimum-image
+// fractional difference s_i - s_j - nint(s_i - s_j) to 2^-32.
+__global__ __launch_bounds__(MDFF_BLOCK) void k_pack_u4(int n, const double* __restrict__ s,
+                                                        const int* __restrict__ rank,
+                                                        uint4* __restrict__ u4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned int u[3];
+  for (int a = 0; a < 3; ++a) {
+    const double x = s[i + (size_t)a * n];
+    const double f = x - floor(x);  // [0, 1]
+    u[a] = (unsigned int)(unsigned long long)(f * 4294967296.0);  // 1 wraps to 0
+  }
+  u4[rank ? rank[i] : i] = make_uint4(u[0], u[1], u[2], 0u);
+}
+
+// The SCF dipole field from the (A2, A1) table of k_dd_table<false, float2>
+// (or the static FP32 pass): r = A (u_i - u_j) 2^-32 from the fixed-point
+// positions, E_i = sum_j [A2 r (r.mu_j) - A1 mu_j].  The wave schedule and
+// the unconditional pipelined loads are those of k_dd_tab; a neighbour's
+// position and dipole are two 16-byte gathers from the same slot.
+template <typename IT>
+__global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_dd_tab2(
+    int n, int maxnb, const float2* __restrict__ tab, const IT* __restrict__ tj,
+    const int* __restrict__ tcnt, const float4* __restrict__ mu4, const uint4* __restrict__ u4,
+    Cell cell, const int* __restrict__ perm, double* __restrict__ ef) {
+  constexpr int U = 8;
+  constexpr float Q = 2.3283064365386963e-10f;  // 2^-32
+  const float A0 = (float)cell.A[0] * Q, A1c = (float)cell.A[1] * Q, A2c = (float)cell.A[2] * Q,
+              A3 = (float)cell.A[3] * Q, A4 = (float)cell.A[4] * Q, A5 = (float)cell.A[5] * Q,
+              A6 = (float)cell.A[6] * Q, A7 = (float)cell.A[7] * Q, A8 = (float)cell.A[8] * Q;
+  const int lane = threadIdx.x & 63;
+  const int wid =
+      __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const int nw = (int)((gridDim.x * blockDim.x) >> 6);
+  auto load = [&](int r, int c, float2 (&g)[U], int (&jj)[U]) {
+    const size_t row = (size_t)min(r, n - 1) * maxnb;
+    const float2* trow = tab + row;
+    const IT* jrow = tj + row;
+    const int last = max(c - 1, 0);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = lane + u * 64;
+      const int kc = min(k, last);
+      const float2 v = trow[kc];
+      jj[u] = jrow[kc];
+      const float mk = k < c ? 1.f : 0.f;
+      g[u] = make_float2(v.x * mk, v.y * mk);
+    }
+  };
+  // the field of one pair: r from the fixed-point difference
+  auto pair = [&](const uint4 ui, const uint4 uj, const float2 t, const float4 m, float& e0,
+                  float& e1, float& e2) {
+    const float dx = (float)(int)(ui.x - uj.x), dy = (float)(int)(ui.y - uj.y),
+                dz = (float)(int)(ui.z - uj.z);
+    const float rx = dx * A0 + dy * A3 + dz * A6;
+    const float ry = dx * A1c + dy * A4 + dz * A7;
+    const float rz = dx * A2c + dy * A5 + dz * A8;
+    const float rm = (rx * m.x + ry * m.y + rz * m.z) * t.x;
+    e0 += rx * rm - t.y * m.x;
+    e1 += ry * rm - t.y * m.y;
+    e2 += rz * rm - t.y * m.z;
+  };
+  float2 g[U];
+  int jj[U];
+  int r = wid;
+  int c = tcnt[min(r, n - 1)];
+  int cn = tcnt[min(r + nw, n - 1)];
+  load(r, c, g, jj);
+  for (; r < n; r += nw) {
+    const int iout = perm ? perm[r] : r;
+    const uint4 ui = u4[r];
+    float4 m[U];
+    uint4 q[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      m[u] = mu4[jj[u]];
+      q[u] = u4[jj[u]];
+    }
+    // the next row and the count of the one after it (rows past the last
+    // are loaded clamped and never used)
+    float2 gn[U];
+    int jn[U];
+    load(r + nw, cn, gn, jn);
+    const int cnn = tcnt[min(r + 2 * nw, n - 1)];
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
+#pragma unroll
+    for (int u = 0; u < U; ++u) pair(ui, q[u], g[u], m[u], e0, e1, e2);
+    // entries past the pipelined 512
+    const size_t row = (size_t)r * maxnb;
+    for (int k = U * 64 + lane; k < c; k += 64) {
+      const int j = tj[row + k];
+      pair(ui, u4[j], tab[row + k], mu4[j], e0, e1, e2);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      e0 += __shfl_xor(e0, off, 64);
+      e1 += __shfl_xor(e1, off, 64);
+      e2 += __shfl_xor(e2, off, 64);
+    }
+    if (lane == 0) {
+      ef[iout] = e0;
+      ef[iout + n] = e1;
+      ef[iout + 2 * n] = e2;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      g[u] = gn[u];
+      jj[u] = jn[u];
+    }
+    c = cn;
+    cn = cnn;
+  }
+}
+
+template __global__ void k_dd_tab2<int>(int, int, const float2*, const int*, const int*,
+                                        const float4*, const uint4*, Cell, const int*, double*);
+template __global__ void k_dd_tab2<unsigned short>(int, int, const float2*, const unsigned short*,
+                                                   const int*, const float4*, const uint4*, Cell,
+                                                   const int*, double*);
 template __global__ void k_pairs_t2<false>(EsDirArgs, double*);
 template __global__ void k_pairs_t2<true>(EsDirArgs, double*);
-template __global__ void k_dd_table<false>(EsDirArgs, float4*, int*, int*);
-template __global__ void k_dd_table<true>(EsDirArgs, float4*, int*, int*);
+template __global__ void k_dd_table<false, float2>(EsDirArgs, float2*, int*, int*);
+template __global__ void k_dd_table<true, float4>(EsDirArgs, float4*, int*, int*);
 template __global__ void k_pack_mu<double4>(int, const double*, double4*);
 template __global__ void k_pack_mu<float4>(int, const double*, float4*);
 
