@@ -65,7 +65,7 @@ struct KChunk {
 struct SrArgs {
   int n, maxnb;
   const double* s;
-  const double4* s4;  // the same, one (s1, s2, s3, 0) per atom
+  const double4* s4;  // the same, one (s1, s2, s3, w) per atom (w: see s4_type)
   const int* typ;
   const int* cnt;
   const int* nbr;
@@ -84,7 +84,7 @@ struct SrArgs {
 struct EsDirArgs {
   int n, maxnb;
   const double* s;
-  const double4* s4;  // the same, one (s1, s2, s3, 0) per atom
+  const double4* s4;  // the same, one (s1, s2, s3, w) per atom (w: see s4_type)
   const int* typ;
   const int* cnt;
   const int* nbr;
@@ -105,6 +105,10 @@ struct EsDirArgs {
   const int* perm;     // atoms in this order (spatial locality), or null
   // FP32 static pass of a single configuration: the compacted SCF pair
   // table (k_dd_table's layout) written on the way, or null
+  // neighbours' charge and dipole packed per atom as (q, mu_x, mu_y, mu_z),
+  // float4 for the FP32 kernels and double4 for the FP64 ones, or null
+  // (q and mu read separately)
+  const void* qm;
   float2* ddtab;
   int* ddtj;
   int* ddcnt;
@@ -139,7 +143,7 @@ struct ApplyArgs {
   const double2* g;
   const double2* eph;
   const double* s;
-  const double4* s4;  // the same, one (s1, s2, s3, 0) per atom
+  const double4* s4;  // the same, one (s1, s2, s3, w) per atom (w: see s4_type)
   double* part;
 };
 
@@ -187,8 +191,12 @@ struct ExtrapArgs {
   double* th;     // op 5 with quadrupole polarizabilities: theta_ind = polquad E, or null
 };
 
-__global__ void k_prepare(int n, const double* pos, Cell cell, const int* ext, double* s,
-                          double4* s4, double2* eph);
+__global__ void k_prepare(int n, const double* pos, Cell cell, const int* ext, const int* typ,
+                          double* s, double4* s4, double2* eph);
+// s4.w of a single configuration: 16 x (the caller's index of the atom) +
+// its species, so that one gather gives position, species and the index
+// the pair-order conventions compare (the batch writes 0)
+__device__ __forceinline__ int s4_type(const double4& s) { return ((int)s.w) & (MDFF_NT - 1); }
 template <bool FILL>
 __global__ void k_nlist(int n, const double* s, Cell cell, double cut2, int maxnb, int jlen,
                         int* scnt, int* nbr);
@@ -263,6 +271,8 @@ __global__ void k_dd_field(EsDirArgs a, const double* tab, const double4* mu4);
 template <typename R4>
 __global__ void k_pack_mu(int n, const double* mu, R4* mu4);
 __global__ void k_pack_mu_ranked(int n, const double* mu, const int* rank, float4* mu4);
+template <typename R4>
+__global__ void k_pack_qmu(int n, const double* q, const double* mu, R4* qm);
 template <bool BATCH, typename E>
 __global__ void k_dd_table(EsDirArgs a, E* tab, int* tj, int* tcnt);
 template <typename IT>

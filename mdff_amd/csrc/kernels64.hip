@@ -13,7 +13,8 @@ namespace mdff {
 // and the per-atom phase bases e^{2 pi i s_a} used by the k-space kernels.
 __global__ __launch_bounds__(MDFF_BLOCK) void k_prepare(
     int n, const double* __restrict__ pos, Cell cell, const int* __restrict__ ext,
-    double* __restrict__ s, double4* __restrict__ s4, double2* __restrict__ eph) {
+    const int* __restrict__ typ, double* __restrict__ s, double4* __restrict__ s4,
+    double2* __restrict__ eph) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double x = pos[i], y = pos[i + n], z = pos[i + 2 * n];
@@ -24,8 +25,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_prepare(
   s[i] = s1;
   s[i + n] = s2;
   s[i + 2 * n] = s3;
-  // w: the caller's index of the atom (the pair-order conventions compare it)
-  s4[i] = make_double4(s1, s2, s3, (double)(ext ? ext[i] : i));
+  // w: 16 x the caller's index of the atom (the pair-order conventions
+  // compare it) + its species (s4_type)
+  const long long w = 16ll * (ext ? ext[i] : i) + typ[i];
+  s4[i] = make_double4(s1, s2, s3, (double)w);
   double sn, cs;
   sincospi(2.0 * s1, &sn, &cs);
   eph[i] = make_double2(cs, sn);

@@ -649,6 +649,16 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     da.dd16 = dd_idx16(n);
     c->t2_in_static = false;
   }
+  if (mode != ES_IND) {
+    // the neighbours' (q, mu) in one gather, in the SCF dipole buffer
+    // (free outside the SCF iterations)
+    if (mx)
+      k_pack_qmu<<<gb, MDFF_BLOCK, 0, st>>>(n, q, mu, reinterpret_cast<float4*>(c->d_mu4));
+    else
+      k_pack_qmu<<<gb, MDFF_BLOCK, 0, st>>>(n, q, mu, c->d_mu4);
+    check_launch(c, "k_pack_qmu");
+    da.qm = c->d_mu4;
+  }
   if (mode == ES_STATIC) {
     ProfScope _ps(c, mx ? K_MX_DIR : K_DIR_STATIC);
     if (mx)
@@ -1531,7 +1541,7 @@ void run(mdffgpu_ctx* c, int itime, mdffgpu_energies* en) {
   {
     ProfScope _ps(c, K_PREPARE);
     k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->renum ? c->d_int2ext : nullptr,
-                                         c->d_s, c->d_s4, c->d_eph);
+                                         c->d_typ, c->d_s, c->d_s4, c->d_eph);
   }
   check_launch(c, "k_prepare");
   update_list(c);
@@ -1736,7 +1746,7 @@ void efg_frame(mdffgpu_ctx* c, const mdffgpu_efg_opts& o, mdffgpu_energies* en) 
   {
     ProfScope _ps(c, K_PREPARE);
     k_prepare<<<gb, MDFF_BLOCK, 0, st>>>(n, c->d_pos, c->cell, c->renum ? c->d_int2ext : nullptr,
-                                         c->d_s, c->d_s4, c->d_eph);
+                                         c->d_typ, c->d_s, c->d_s4, c->d_eph);
   }
   check_launch(c, "k_prepare");
   // get_moments (src/field.f90:3686-3765) with calc = 'efg': the SCF starts

@@ -67,15 +67,15 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_shortrange(SrArgs a) {
     if (c > 0) {
       jn = nbj(sub);
       sjn = a.s4[jn];
-      tjn = a.typ[jn];
+      tjn = BATCH ? a.typ[jn] : 0;
       jnn = nbj(sub + LPA);
     }
     for (int k = sub; k < c; k += LPA) {
-      const int tj = tjn;
+      const int tj = BATCH ? tjn : s4_type(sjn);
       const double4 sj = sjn;
       jn = jnn;
       sjn = a.s4[jn];
-      tjn = a.typ[jn];
+      if (BATCH) tjn = a.typ[jn];
       jnn = nbj(k + 2 * LPA);
       R rx, ry, rz;
       const R rsq = min_image_t<R>(cell, si1 - sj.x, si2 - sj.y, si3 - sj.z, rx, ry, rz);
@@ -191,6 +191,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
   // float4 (on, b, c, k) per (slot o, type pair) (the FP32 kernels convert
   // b and c to float anyway)
   constexpr bool F32 = sizeof(R) == sizeof(float);
+  using R4 = typename std::conditional<F32, float4, double4>::type;
   __shared__ float4 sdmp[F32 ? 2 * 256 : 1];
   if constexpr (F32) {
     if (MODE != ES_IND) {
@@ -245,9 +246,21 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
       const int j = jn;
       if (k + LPA < c) jn = nb[k + LPA];
       const double4 sj = a.s4[j];
-      const double qjd = a.q[j];
-      const double mj0d = a.mu[j], mj1d = a.mu[j + n], mj2d = a.mu[j + 2 * n];
-      const int tj = a.typ[j];
+      // charge and dipole: one gather of the packed (q, mu) when given
+      R qj, mj0, mj1, mj2;
+      if (a.qm) {
+        const R4 v = reinterpret_cast<const R4*>(a.qm)[j];
+        qj = v.x;
+        mj0 = v.y;
+        mj1 = v.z;
+        mj2 = v.w;
+      } else {
+        qj = R(a.q[j]);
+        mj0 = R(a.mu[j]);
+        mj1 = R(a.mu[j + n]);
+        mj2 = R(a.mu[j + 2 * n]);
+      }
+      const int tj = BATCH ? a.typ[j] : s4_type(sj);
       R rx, ry, rz;
       const R d2 = min_image_t<R>(cell, si1 - sj.x, si2 - sj.y, si3 - sj.z, rx, ry, rz);
       // SCF pair table: the entry's slot among the in-cutoff pairs of the
@@ -262,8 +275,6 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
         }
       }
       if (d2 > cut2) continue;
-      const R qj = R(qjd);
-      const R mj0 = R(mj0d), mj1 = R(mj1d), mj2 = R(mj2d);
       const bool dip_j = (mj0 != R(0) || mj1 != R(0) || mj2 != R(0));
       const bool dd = DD && (dip_i || dip_j);
       const bool qd = QD && (dip_i || dip_j);
@@ -547,6 +558,24 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pack_mu_ranked(int n, const doub
   if (i >= n) return;
   mu4[rank[i]] = make_float4((float)mu[i], (float)mu[i + n], (float)mu[i + 2 * n], 0.f);
 }
+
+// Charges and dipoles packed as (q, mu_x, mu_y, mu_z) per atom for the
+// static and final real-space passes: a neighbour's moments are one gather.
+template <typename R4>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_pack_qmu(int n, const double* __restrict__ q,
+                                                         const double* __restrict__ mu,
+                                                         R4* __restrict__ qm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R4 v;
+  v.x = q[i];
+  v.y = mu[i];
+  v.z = mu[i + n];
+  v.w = mu[i + 2 * n];
+  qm[i] = v;
+}
+template __global__ void k_pack_qmu<float4>(int, const double*, const double*, float4*);
+template __global__ void k_pack_qmu<double4>(int, const double*, const double*, double4*);
 
 // Dipoles of the SCF iteration packed as (x, y, z, 0) per atom, so that a
 // neighbour's dipole is one gather.
