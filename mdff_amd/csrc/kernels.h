@@ -394,7 +394,7 @@ __global__ void k_disp(int n, const double* pos, const double* ref, double* part
 __global__ void k_reduce_max(const double* part, int nb, double* out);
 
 // lanes that share one centre atom in the real-space kernels
-constexpr int LPA = 16;
+constexpr int LPA = 32;
 __global__ void k_add(int m, const double* a, const double* b, double* out);
 
 // ---------------------------------------------------------------------------

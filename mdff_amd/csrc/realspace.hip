@@ -269,7 +269,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
       if constexpr (TAB) {
         if (a.ddtab) {
           const unsigned gm = (unsigned)(__ballot(!(d2 > cut2)) >> (threadIdx.x & (64 - LPA))) &
-                              ((1u << LPA) - 1u);
+                              (unsigned)((1ull << LPA) - 1ull);
           tpos = tbase + __popc(gm & ((1u << sub) - 1u));
           tbase += __popc(gm);
         }
