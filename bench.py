@@ -327,15 +327,80 @@ def cpu_baseline(args, sys_, n_ind_per_step):
         real = max(t0 - per_atom_k * k0 * n, 0.0)
         per_call[kind] = real + per_atom_k * kfull * n
     step = t_sr + per_call["static"] + n_ind_per_step * per_call["induced"] + per_call["final"]
-    return {"value": n / step, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle (C, 1 thread) on the same {n}-atom configuration: the BMHFTD "
+    # the host's cores: the reference decomposes atoms and k points over its
+    # MPI ranks, modelled as `cores` equal shares of the step; their
+    # concurrent throughput is measured with that many worker processes
+    # running the induced-dipole k-space sample at once against one alone
+    cores = max(1, min(CPU_WORKERS, os.cpu_count() or 1))
+    pk = copy.deepcopy(p)
+    pk.kES[:] = [6, 6, 6]
+    pk.natm = m
+    eff, t_one, t_all = _parallel_efficiency(pk, sys_.itype[:m], sys_.A, sys_.r[:m], mu[:m], cores)
+    measured += t_one + t_all
+    one_core = n / step
+    return {"value": one_core * cores * eff, "unit": "atom-steps/s", "cores": cores,
+            "kind": "port",
+            "sample": (f"oracle (C) on the same {n}-atom configuration: on one core the BMHFTD "
                        f"pair sum, and each kind of Ewald call (static, induced, final) with "
                        f"all atoms at kmax 1 (real space) and with {m} atoms at kmax 3 and 6 "
                        f"(k-space cost per atom and k vector), scaled to {n} atoms and "
                        f"{kfull} k vectors; per step BMHFTD + static + {n_ind_per_step:.2f} "
-                       f"induced + final call"),
-            "seconds_per_step_estimate": step,
+                       f"induced + final call; times {cores} cores at the efficiency of "
+                       f"{cores} concurrent worker processes each running the induced call "
+                       f"on {m} atoms at kmax 6 ({t_all:.2f} s) against one alone "
+                       f"({t_one:.2f} s)"),
+            "one_core_value": one_core,
+            "parallel_efficiency": eff,
+            "seconds_per_step_estimate": step / (cores * eff),
             "seconds_measured": measured}
+
+
+CPU_WORKERS = 16  # the host's CPU share per GPU
+
+_CPU_WORKER = r"""
+import sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from oracle import oracle as O
+from mdff_amd import params as P
+d = np.load(sys.argv[2])
+p = P.MdffParams.from_buffer_copy(d["params"].tobytes())
+n = len(d["itype"])
+t0 = time.perf_counter()
+O.multipole_es(p, d["itype"], d["A"], d["r"], np.zeros(n), d["mu"], [0, 0, 1], damp_ind=False,
+               do_efg=False, do_forces=False, do_stress=False)
+print(time.perf_counter() - t0, flush=True)
+"""
+
+
+def _parallel_efficiency(p, itype, A, r, mu, cores):
+    """Time of one oracle induced-dipole call alone and of `cores` copies in
+    concurrent worker processes (CPU only, each timing its own call);
+    efficiency = alone / slowest concurrent copy."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "sample.npz")
+        np.savez(path, params=np.frombuffer(bytes(p), dtype=np.uint8), itype=itype, A=A, r=r,
+                 mu=mu)
+        cmd = [sys.executable, "-c", _CPU_WORKER, ROOT, path]
+
+        def run(k):
+            # the longest of the workers' own timings of the call (process
+            # start-up excluded)
+            procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+                     for _ in range(k)]
+            worst = 0.0
+            for pr in procs:
+                out, err = pr.communicate()
+                if pr.returncode != 0:
+                    raise RuntimeError(f"cpu baseline worker failed: {err.decode()[-400:]}")
+                worst = max(worst, float(out.decode().split()[-1]))
+            return worst
+        t_one = run(1)
+        t_all = run(cores)
+    return min(1.0, t_one / t_all), t_one, t_all
 
 
 def main():
