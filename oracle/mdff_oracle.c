@@ -5,14 +5,15 @@
  * precision, no vectorisation tricks: each function follows the reference
  * routine cited above it.  Citations are relative to the MDFF source tree.
  *
- * Parity-unpinned parts (no golden vector or closed form in the reference's
- * own tests reaches them; they are pinned here only by the invariants in
- * tests/test_oracle_*.py):
+ * Parts the reference cannot run as written, pinned by textbook closed
+ * forms instead (tests/test_oracle_pins.py, dimers):
  *   - Morse: the reference routine is never dispatched and reads
  *     uninitialised erh2/u2 (src/field.f90:3124,3132); this file evaluates
- *     the reference's energy expression and its exact derivative.
+ *     the reference's energy expression, eps ((exp(-rho (r - sigma)) - 1)^2
+ *     - 1), which is the Morse potential, and its exact derivative.
  *   - BMHFT without damping (lbmhft): the reference reads uninitialised
- *     fdiff6/fdiff8 (src/field.f90:2900,2975); this file uses 0.
+ *     fdiff6/fdiff8 (src/field.f90:2900,2975); this file uses 0, which gives
+ *     A exp(-B r) - C / r^6 - D / r^8, the limit of the damped form.
  */
 #include "mdff_oracle.h"
 

@@ -289,3 +289,66 @@ def test_surface_field_is_unscaled_total_moment(O):
     assert np.abs((b["ef"] - a["ef"]) - M[None, :]).max() < 1e-12 * np.abs(M).max()
     assert np.array_equal(b["efg"], a["efg"])
     assert np.array_equal(b["tau"], a["tau"])
+
+
+# --------------------------------------------------------------------------
+# pair potentials whose reference routines cannot run as written: the
+# undamped BMHFT branch reads uninitialised damping factors
+# (src/field.f90:2900, 2975) and the Morse routine is never dispatched and
+# reads an uninitialised exponential in its force (:3124, 3132).  Both are
+# pinned by their textbook closed forms on a dimer.
+
+def _dimer(O, d, term, **kw):
+    p = P.field_defaults(2, 1)
+    p.cutshortrange = 9.0
+    k = P.pidx(1, 1)
+    if term == "morse":
+        p.lmorse = 1
+        p.epsmor[k], p.sigmamor[k], p.rhomor[k] = kw["eps"], kw["sig"], kw["rho"]
+    else:
+        setattr(p, term, 1)
+        p.Abmhftd[k], p.Bbmhftd[k], p.Cbmhftd[k], p.Dbmhftd[k] = kw["A"], kw["B"], kw["C"], kw["D"]
+        p.BDbmhftd[k] = kw.get("BD", 0.0)
+    P.field_check_tag(p)
+    r = np.array([[1.0, 1.0, 1.0], [1.0 + d, 1.0, 1.0]])
+    return O.Oracle(p, np.array([1, 1], np.int32), np.eye(3) * 30.0).engforce(r)
+
+
+def test_morse_dimer_closed_form(O):
+    """V(r) = eps (exp(-2 rho (r - sigma)) - 2 exp(-rho (r - sigma))): depth
+    eps at r = sigma with zero force there, a zero at sigma - ln 2 / rho, and
+    the force -dV/dr on the second atom at every distance."""
+    eps, sig, rho = 0.37, 2.1, 1.45
+    kw = dict(eps=eps, sig=sig, rho=rho)
+    out = _dimer(O, sig, "morse", **kw)
+    assert abs(out["u_morse"] + eps) < 1e-14
+    assert np.abs(out["f"]).max() < 1e-13
+    assert abs(_dimer(O, sig - math.log(2.0) / rho, "morse", **kw)["u_morse"]) < 1e-14
+    for d in (1.6, 2.5, 3.7, 6.0):
+        x = math.exp(-rho * (d - sig))
+        v = eps * (x * x - 2.0 * x)
+        dv = eps * (-2.0 * rho * x * x + 2.0 * rho * x)
+        out = _dimer(O, d, "morse", **kw)
+        assert abs(out["u_morse"] - v) <= 1e-13 * max(1.0, abs(v))
+        # atom 2 sits at +d along x: its force is -dV/dr
+        assert abs(out["f"][1, 0] + dv) <= 1e-12 * max(1.0, abs(dv))
+        assert abs(out["f"][0, 0] - dv) <= 1e-12 * max(1.0, abs(dv))
+
+
+def test_bmhft_dimer_closed_form_and_damped_limit(O):
+    """Undamped BMHFT: V(r) = A exp(-B r) - C / r^6 - D / r^8 and its
+    derivative; the Tang-Toennies damped form tends to it when the damping
+    constant grows (f_n(b r) -> 1)."""
+    kw = dict(A=1500.0, B=3.3, C=21.0, D=4.5)
+    for d in (1.4, 2.0, 3.1, 5.5):
+        v = kw["A"] * math.exp(-kw["B"] * d) - kw["C"] / d ** 6 - kw["D"] / d ** 8
+        dv = (-kw["B"] * kw["A"] * math.exp(-kw["B"] * d) + 6.0 * kw["C"] / d ** 7
+              + 8.0 * kw["D"] / d ** 9)
+        out = _dimer(O, d, "lbmhft", **kw)
+        assert abs(out["u_bmhft"] - v) <= 1e-13 * max(1.0, abs(v)), (d, out["u_bmhft"], v)
+        assert abs(out["f"][1, 0] + dv) <= 1e-12 * max(1.0, abs(dv)), (d, out["f"][1, 0], dv)
+        damped = _dimer(O, d, "lbmhftd", BD=400.0, **kw)
+        assert abs(damped["u_bmhft"] - v) <= 1e-13 * max(1.0, abs(v))
+        assert abs(damped["f"][1, 0] + dv) <= 1e-12 * max(1.0, abs(dv))
+        # a finite damping constant lowers the dispersion attraction
+        assert _dimer(O, d, "lbmhftd", BD=1.0, **kw)["u_bmhft"] > v
