@@ -107,3 +107,40 @@ def test_wannier_error_names_the_callers_atom():
     assert e.value.status == 8
     assert f"atom {inv[1] + 1} " in str(e.value), str(e.value)
     ff.close()
+
+
+def test_shuffled_md_pim_aspc(oracle_lib):
+    """md_run on interleaved species: positions, velocities and forces go in
+    and come out in the caller's order across the ASPC-driven trajectory."""
+    from test_gpu_md import _check, _md_both
+    s = _shuffled(C.silica(100, 200, seed=11, rc=6.0, kmax=5, pim=True, efg=False,
+                           scf_tol=1e-9), 7)
+    p = s.params
+    p.calc_mode = P.CALC_MD
+    p.algo_ext_dipole = P.EXT_ASPC
+    p.extrapolate_order = 2
+    p.min_scf_pol_iter = 1
+    v0 = C.maxwell_velocities(s, 3000.0, seed=3)
+    mdp = P.md_params("nvt-nhc2", dt=0.001, temp=3000.0, timesca_thermo=0.05,
+                      mass=C.species_masses(s))
+    gs, os_, g, o, _, _ = _md_both(oracle_lib, s, mdp, v0, 6, chunks=2)
+    _check(gs, os_, g, o, ("u_bmhft", "u_coul", "u_pol", "e_kin", "e_nvt"))
+
+
+def test_shuffled_standalone_efg(oracle_lib):
+    """The standalone EFG run (mdffgpu_efg) on interleaved species: the EFG
+    and dipoles of every site in the caller's order."""
+    import copy
+    from mdff_amd import efg as E
+    from mdff_amd.field import ForceField
+    from test_oracle_efg import _pim_cell
+    s = _shuffled(_pim_cell(21, n_si=30, n_o=60, kmax=7), 8)
+    p = copy.deepcopy(s.params)
+    o = P.efg_opts("ewald")
+    ff = ForceField(E.context_params(p, o, s.A), s.itype, s.A)
+    g = ff.efg(s.r, o)
+    ff.close()
+    ref = oracle_lib.efgcalc(p, o, s.itype, s.A, s.r)
+    assert g["n_scf"] == ref["n_scf"]
+    assert_close(g["efg"], ref["efg"], what="efg")
+    assert_close(g["mu"], ref["mu"], what="mu")
