@@ -97,7 +97,8 @@ __device__ __forceinline__ int slot_atom(int jp, int n, const MxSeg* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// phase tables from the FP64 fractional coordinates
+// phase tables from the FP64 fractional coordinates: one thread per (slot,
+// table row), rows 0..kx of E1, then the 2 ky + 1 of E2 and 2 kz + 1 of E3
 __global__ __launch_bounds__(MDFF_BLOCK) void mx_tables(int n, int npad, const double* __restrict__ s,
                                                          const MxSeg* __restrict__ seg,
                                                          const int* __restrict__ kseg,
@@ -105,25 +106,30 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_tables(int n, int npad, const d
                                                          float2* tj2, float2* tj3) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= npad) return;
+  int row = blockIdx.y, axis = 0, m = row;
+  float2* out = tj1;
+  if (row > kx) {
+    row -= kx + 1;
+    if (row < 2 * ky + 1) {
+      axis = 1;
+      m = row - ky;
+      out = tj2;
+    } else {
+      row -= 2 * ky + 1;
+      axis = 2;
+      m = row - kz;
+      out = tj3;
+    }
+  }
   int z;
   const int i = slot_atom(j, n, seg, kseg, z);
-  const bool act = i >= 0;
-  const double s1 = act ? s[i] : 0.0, s2 = act ? s[i + n] : 0.0, s3 = act ? s[i + 2 * n] : 0.0;
-  for (int m = 0; m <= kx; ++m) {
+  float2 v = make_float2(0.f, 0.f);
+  if (i >= 0) {
     double sn, cs;
-    sincospi(2.0 * m * s1, &sn, &cs);
-    tj1[(size_t)m * npad + j] = act ? make_float2((float)cs, (float)sn) : make_float2(0.f, 0.f);
+    sincospi(2.0 * m * s[i + (size_t)axis * n], &sn, &cs);
+    v = make_float2((float)cs, (float)sn);
   }
-  for (int m = -ky; m <= ky; ++m) {
-    double sn, cs;
-    sincospi(2.0 * m * s2, &sn, &cs);
-    tj2[(size_t)(m + ky) * npad + j] = act ? make_float2((float)cs, (float)sn) : make_float2(0.f, 0.f);
-  }
-  for (int m = -kz; m <= kz; ++m) {
-    double sn, cs;
-    sincospi(2.0 * m * s3, &sn, &cs);
-    tj3[(size_t)(m + kz) * npad + j] = act ? make_float2((float)cs, (float)sn) : make_float2(0.f, 0.f);
-  }
+  out[(size_t)row * npad + j] = v;
 }
 
 // hi + lo = x to 22 bits.  x is first pinned as a rounded FP32 value:
@@ -1808,7 +1814,9 @@ static bool run_kspace(MixedState* m, const KsCall& a, const double* q, const do
   // first reciprocal call
   if (m->tables_eval != a.epoch) {
     ProfScope _ps(m->ctx, K_MX_PHASE);
-    mx_tables<<<(m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK, MDFF_BLOCK, 0, st>>>(
+    const dim3 gt((m->npad + MDFF_BLOCK - 1) / MDFF_BLOCK,
+                  m->kx + 1 + (2 * m->ky + 1) + (2 * m->kz + 1));
+    mx_tables<<<gt, MDFF_BLOCK, 0, st>>>(
         n, m->npad, a.s, seg, m->d_kseg, m->kx, m->ky, m->kz, m->tj1, m->tj2, m->tj3);
     m->tables_eval = a.epoch;
   }
