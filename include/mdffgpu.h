@@ -215,6 +215,14 @@ typedef struct mdffgpu_ctx mdffgpu_ctx;
  * mu_t, theta_t, dipia_ind_t) and the per-atom tables of typeinfo_init
  * (src/read_pos.f90:162-218).  itype is 1-based like config%itype; A is
  * simu_cell%A (column j = lattice vector j).
+ *
+ * Atom order: every per-atom input and output of a context is in the
+ * caller's order (the order of itype).  Inside, a context renumbers its
+ * atoms once, at its first evaluation, species by species and by cell,
+ * and gathers and scatters across that numbering at each entry point; the
+ * reference's pair-order conventions (damping slots, the sign of the
+ * quadrupole-quadrupole EFG, the atom a Wannier error names) follow the
+ * caller's indices.
  * ------------------------------------------------------------------- */
 int mdffgpu_create(const mdffgpu_params* params, const int32_t* itype,
                    const double* A, mdffgpu_ctx** out);
@@ -405,7 +413,8 @@ int mdffgpu_efg_nmr(int32_t n, const double* efg6, double* nmr, double* U,
                     double* S, int32_t* bad);
 
 /* Induced-dipole history dipia_ind_t(extrapolate_order+1, natm, 3)
-   (src/field.f90:182), Fortran order, for restart and parity runs. */
+   (src/field.f90:182), Fortran order and the caller's atom order, for
+   restart and parity runs. */
 int mdffgpu_get_dipole_history(mdffgpu_ctx* ctx, double* hist);
 int mdffgpu_set_dipole_history(mdffgpu_ctx* ctx, const double* hist);
 
