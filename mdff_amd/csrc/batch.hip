@@ -879,15 +879,17 @@ void es_call(mdffgpu_batch* b, int mode, const double* q, const double* mu, cons
                                                                      m4);
     } else {
       const size_t cap = (size_t)b->maxnb * N;
-      const float4* tab = reinterpret_cast<const float4*>(b->d_t2);
+      const uint4* u4 = reinterpret_cast<const uint4*>(b->d_t2);
+      const float2* tab = reinterpret_cast<const float2*>(u4 + N);
       const int* tj = reinterpret_cast<const int*>(tab + cap);
       if (dd_idx16(N))
-        k_dd_tab<<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(
-            N, b->maxnb, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4, nullptr,
-            b->d_dir);
+        k_dd_tab2<<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(
+            N, b->maxnb, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4, u4,
+            Cell{}, b->d_bc, b->d_cfg, nullptr, b->d_dir);
       else
-        k_dd_tab<<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(N, b->maxnb, tab, tj, tj + cap, m4,
-                                                             nullptr, b->d_dir);
+        k_dd_tab2<<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(N, b->maxnb, tab, tj, tj + cap, m4,
+                                                              u4, Cell{}, b->d_bc, b->d_cfg,
+                                                              nullptr, b->d_dir);
     }
   } else {
     da.task = TASK_QQ | (b->pim ? (TASK_QD | TASK_DD) : 0);
@@ -1308,12 +1310,14 @@ void create_impl(const mdffgpu_params* model, int nconf, const mdffgpu_batch_con
 template <typename R>
 void build_pairs(mdffgpu_batch* b) {
   EsDirArgs da = es_args(b, b->d_q, b->d_zero);
-  // FP64: five doubles per list entry; FP32: compacted (v, A1) float4 +
-  // index per entry, then the per-atom counts
+  // FP64: five doubles per list entry; FP32: the fixed-point positions
+  // (uint4 per atom), compacted (A2, A1) float2 + index per entry, then the
+  // per-atom counts
   const size_t cap = (size_t)b->maxnb * b->N;
   const size_t need = sizeof(R) == sizeof(double)
                           ? 5 * cap * sizeof(double)
-                          : cap * (sizeof(float4) + sizeof(int)) + (size_t)b->N * sizeof(int);
+                          : (size_t)b->N * sizeof(uint4) + cap * (sizeof(float2) + sizeof(int)) +
+                                (size_t)b->N * sizeof(int);
   if (need > b->t2_cap) {
     if (b->d_t2) {
       (void)hipFree(b->d_t2);
@@ -1327,10 +1331,15 @@ void build_pairs(mdffgpu_batch* b) {
   if constexpr (sizeof(R) == sizeof(double)) {
     k_pairs_t2<true><<<nblk((size_t)b->N * LPA), MDFF_BLOCK, 0, b->st>>>(da, (double*)b->d_t2);
   } else {
-    float4* tab = reinterpret_cast<float4*>(b->d_t2);
+    uint4* u4 = reinterpret_cast<uint4*>(b->d_t2);
+    float2* tab = reinterpret_cast<float2*>(u4 + b->N);
     int* tj = reinterpret_cast<int*>(tab + cap);
     da.dd16 = dd_idx16(b->N);
-    k_dd_table<true, float4><<<nblk((size_t)b->N * 64), MDFF_BLOCK, 0, b->st>>>(da, tab, tj, tj + cap);
+    // fractional coordinates in each configuration's own cell: the fixed-
+    // point difference is that cell's minimum image
+    k_pack_u4<<<nblk(b->N), MDFF_BLOCK, 0, b->st>>>(b->N, b->d_s, nullptr, u4);
+    k_dd_table<true><<<nblk((size_t)b->N * 64), MDFF_BLOCK, 0, b->st>>>(da, tab, tj,
+                                                                                tj + cap);
   }
   launch_check(b, "batch pair table");
 }

@@ -273,19 +273,16 @@ __global__ void k_pack_mu(int n, const double* mu, R4* mu4);
 __global__ void k_pack_mu_ranked(int n, const double* mu, const int* rank, float4* mu4);
 template <typename R4>
 __global__ void k_pack_qmu(int n, const double* q, const double* mu, R4* qm);
-template <bool BATCH, typename E>
-__global__ void k_dd_table(EsDirArgs a, E* tab, int* tj, int* tcnt);
-template <typename IT>
-__global__ void k_dd_tab(int n, int maxnb, const float4* tab, const IT* tj, const int* tcnt,
-                         const float4* mu4, const int* perm, double* ef);
+template <bool BATCH>
+__global__ void k_dd_table(EsDirArgs a, float2* tab, int* tj, int* tcnt);
 __global__ void k_pack_u4(int n, const double* s, const int* rank, uint4* u4);
 template <typename IT>
 __global__ void k_dd_tab2(int n, int maxnb, const float2* tab, const IT* tj, const int* tcnt,
-                          const float4* mu4, const uint4* u4, Cell cell, const int* perm,
-                          double* ef);
+                          const float4* mu4, const uint4* u4, Cell cell, const BatchCfg* bc,
+                          const int* cfg, const int* perm, double* ef);
 // neighbour slots of the FP32 pair table fit 16 bits
 inline bool dd_idx16(long long n) { return n <= 65536; }
-// Grid of k_dd_tab (persistent waves): four per SIMD of the device, at most
+// Grid of k_dd_tab2 (persistent waves): four per SIMD of the device, at most
 // one per atom.
 inline int dd_tab_blocks(int n) {
   static int cus = 0;

@@ -580,11 +580,11 @@ void es_ind_dir(mdffgpu_ctx* c, const double* mu, bool packed = false) {
     if (dd_idx16(n))
       k_dd_tab2<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(
           n, c->maxnb, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4, dd_u4(c),
-          c->cell, c->d_perm, c->d_efdir);
+          c->cell, nullptr, nullptr, c->d_perm, c->d_efdir);
     else
       k_dd_tab2<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4,
-                                                         dd_u4(c), c->cell, c->d_perm,
-                                                         c->d_efdir);
+                                                         dd_u4(c), c->cell, nullptr, nullptr,
+                                                         c->d_perm, c->d_efdir);
   } else {
     EsDirArgs da;
     std::memset(&da, 0, sizeof(da));
@@ -921,7 +921,7 @@ void build_t2(mdffgpu_ctx* c) {
   ProfScope _ps(c, K_PAIRS_T2);
   if (mx) {
     int* tj = dd_tj(c);
-    k_dd_table<false, float2><<<nblk(n * 64), MDFF_BLOCK, 0, c->st>>>(da, dd_tab(c), tj, tj + cap);
+    k_dd_table<false><<<nblk(n * 64), MDFF_BLOCK, 0, c->st>>>(da, dd_tab(c), tj, tj + cap);
   } else {
     k_pairs_t2<<<nblk(n * LPA), MDFF_BLOCK, 0, c->st>>>(da, (double*)c->d_t2);
   }
@@ -1027,7 +1027,7 @@ void scf(mdffgpu_ctx* c, int itime, int* n_scf, double* rmsd_out, int* n_es,
     HCK(c, hipEventCreateWithFlags(&c->ev_sc, hipEventDisableTiming));
     HCK(c, hipHostGetDevicePointer((void**)&c->h_sc_dev, c->h_sc, 0));
   }
-  // with the FP32 SCF field, k_induced also packs the moments for k_dd_tab
+  // with the FP32 SCF field, k_induced also packs the moments for k_dd_tab2
   auto induced = [&](bool pack) {
     {
       ProfScope _ps(c, K_INDUCED);

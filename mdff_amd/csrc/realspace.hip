@@ -450,7 +450,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
   if constexpr (TAB)
     if (a.ddtab && i < n && sub == 0) {  // lane 0 visits every slot
       a.ddcnt[i0] = tbase;
-      if (tbase == 0) {  // a row without pairs holds one zero entry (k_dd_tab reads it)
+      if (tbase == 0) {  // a row without pairs holds one zero entry (k_dd_tab2 reads it)
         a.ddtab[(size_t)i0 * a.maxnb] = make_float2(0.f, 0.f);
         if (a.dd16)
           reinterpret_cast<unsigned short*>(a.ddtj)[(size_t)i0 * a.maxnb] = 0;
@@ -656,16 +656,13 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a,
 // difference) and the neighbour's slot in the dipole array (its spatial
 // rank when a.rank is given, else its index).  Rows and counts are in the
 // processing order i0 (atom perm[i0] when perm is given).  Each SCF
-// iteration then applies E_i = sum_j [A2 r (r.mu_j) - A1 mu_j].  The entry
-// of a single configuration is (A2, A1) and k_dd_tab2 forms r from 32-bit
-// fixed-point positions (10 bytes of table and one 32-byte gather per
-// pair); the batch keeps (v, A1) with v = r sqrt(A2) for k_dd_tab (18
-// bytes and a 16-byte gather).
-template <bool BATCH, typename E>
-__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, E* __restrict__ tab,
+// iteration then applies E_i = sum_j [A2 r (r.mu_j) - A1 mu_j] (k_dd_tab2),
+// forming r from 32-bit fixed-point positions: 10 bytes of table and one
+// 32-byte gather per pair.
+template <bool BATCH>
+__global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float2* __restrict__ tab,
                                                          int* __restrict__ tj,
                                                          int* __restrict__ tcnt) {
-  constexpr bool RV = std::is_same<E, float4>::value;  // (v, A1) entries
   const int n = a.n;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i0 = tid >> 6, lane = tid & 63;
@@ -685,11 +682,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, E* __restr
   int base = 0;
   for (int k0 = 0; k0 < c; k0 += 64) {
     const int k = k0 + lane;
-    E v;
-    if constexpr (RV)
-      v = make_float4(0.f, 0.f, 0.f, 0.f);
-    else
-      v = make_float2(0.f, 0.f);
+    float2 v = make_float2(0.f, 0.f);
     int j = 0;
     bool in = false;
     if (k < c) {
@@ -705,12 +698,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, E* __restr
         const float expon = __expf(-alpha2 * d2) * 0.56418958354775628f;  // 1/sqrt(pi)
         const float F1 = errfc_t<float>(alpha * d) + 2.f * alpha * d * expon;
         const float F2 = F1 + c43a3 * d2 * d * expon;
-        if constexpr (RV) {
-          const float sA2 = sqrtf(3.f * F2) * dm1 * dm1 * sqrtf(dm1);
-          v = make_float4(rx * sA2, ry * sA2, rz * sA2, F1 * dm3);
-        } else {
-          v = make_float2(3.f * F2 * dm3 * dm1 * dm1, F1 * dm3);
-        }
+        v = make_float2(3.f * F2 * dm3 * dm1 * dm1, F1 * dm3);
       }
     }
     const unsigned long long m = __ballot(in);
@@ -727,11 +715,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, E* __restr
   }
   if (lane == 0) {
     tcnt[i0] = base;
-    if (base == 0) {  // a row without pairs holds one zero entry (k_dd_tab reads it)
-      if constexpr (RV)
-        tab[row] = make_float4(0.f, 0.f, 0.f, 0.f);
-      else
-        tab[row] = make_float2(0.f, 0.f);
+    if (base == 0) {  // a row without pairs holds one zero entry (k_dd_tab2 reads it)
+      tab[row] = make_float2(0.f, 0.f);
       if (a.dd16)
         reinterpret_cast<unsigned short*>(tj)[row] = 0;
       else
@@ -739,110 +724,6 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, E* __restr
     }
   }
 }
-
-// One wave per atom, persistent: the waves of the grid take the rows
-// i0 = wave, wave + waves, ...; while a wave gathers and sums one row, the
-// loads of its next row are already in flight, so that the table stream
-// does not pause.  The first 512 entries of a row are pipelined (8 per
-// lane); longer rows finish in a plain loop.  The pipelined loads are
-// issued unconditionally: indexes are clamped to the row's last entry (a
-// row without pairs holds one zero entry) and to the last row, and the
-// entries past the count are masked by a zero factor, so that no branch
-// splits the loads in flight.
-template <typename IT>
-__global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_dd_tab(
-    int n, int maxnb, const float4* __restrict__ tab, const IT* __restrict__ tj,
-    const int* __restrict__ tcnt, const float4* __restrict__ mu4, const int* __restrict__ perm,
-    double* __restrict__ ef) {
-  constexpr int U = 8;
-  const int lane = threadIdx.x & 63;
-  const int wid =
-      __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-  const int nw = (int)((gridDim.x * blockDim.x) >> 6);
-  // row r (clamped to the last row) with count c: entries, zero past c
-  auto load = [&](int r, int c, float4 (&g)[U], int (&jj)[U]) {
-    const size_t row = (size_t)min(r, n - 1) * maxnb;
-    const float4* trow = tab + row;
-    const IT* jrow = tj + row;
-    const int last = max(c - 1, 0);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = lane + u * 64;
-      const int kc = min(k, last);
-      const float4 v = trow[kc];
-      jj[u] = jrow[kc];
-      const float mk = k < c ? 1.f : 0.f;
-      g[u] = make_float4(v.x * mk, v.y * mk, v.z * mk, v.w * mk);
-    }
-  };
-  float4 g[U];
-  int jj[U];
-  int r = wid;
-  int c = tcnt[min(r, n - 1)];
-  int cn = tcnt[min(r + nw, n - 1)];
-  load(r, c, g, jj);
-  for (; r < n; r += nw) {
-    const int iout = perm ? perm[r] : r;
-    float4 m[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) m[u] = mu4[jj[u]];
-    // the next row and the count of the one after it (rows past the last
-    // are loaded clamped and never used)
-    float4 gn[U];
-    int jn[U];
-    load(r + nw, cn, gn, jn);
-    const int cnn = tcnt[min(r + 2 * nw, n - 1)];
-    // the gathers and the next row's loads all in flight before the first
-    // use of a gather
-    static_assert(U == 8, "operand list below");
-    asm volatile(""
-                 : "+v"(m[0].x), "+v"(m[0].y), "+v"(m[0].z), "+v"(m[1].x), "+v"(m[1].y),
-                   "+v"(m[1].z), "+v"(m[2].x), "+v"(m[2].y), "+v"(m[2].z), "+v"(m[3].x),
-                   "+v"(m[3].y), "+v"(m[3].z), "+v"(m[4].x), "+v"(m[4].y), "+v"(m[4].z),
-                   "+v"(m[5].x), "+v"(m[5].y), "+v"(m[5].z), "+v"(m[6].x), "+v"(m[6].y),
-                   "+v"(m[6].z), "+v"(m[7].x), "+v"(m[7].y), "+v"(m[7].z));
-    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const float vm = g[u].x * m[u].x + g[u].y * m[u].y + g[u].z * m[u].z;
-      e0 += g[u].x * vm - g[u].w * m[u].x;
-      e1 += g[u].y * vm - g[u].w * m[u].y;
-      e2 += g[u].z * vm - g[u].w * m[u].z;
-    }
-    // entries past the pipelined 512
-    const size_t row = (size_t)r * maxnb;
-    for (int k = U * 64 + lane; k < c; k += 64) {
-      const float4 v = tab[row + k];
-      const float4 mj = mu4[tj[row + k]];
-      const float vm = v.x * mj.x + v.y * mj.y + v.z * mj.z;
-      e0 += v.x * vm - v.w * mj.x;
-      e1 += v.y * vm - v.w * mj.y;
-      e2 += v.z * vm - v.w * mj.z;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      e0 += __shfl_xor(e0, off, 64);
-      e1 += __shfl_xor(e1, off, 64);
-      e2 += __shfl_xor(e2, off, 64);
-    }
-    if (lane == 0) {
-      ef[iout] = e0;
-      ef[iout + n] = e1;
-      ef[iout + 2 * n] = e2;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      g[u] = gn[u];
-      jj[u] = jn[u];
-    }
-    c = cn;
-    cn = cnn;
-  }
-}
-
-template __global__ void k_dd_tab<int>(int, int, const float4*, const int*, const int*,
-                                       const float4*, const int*, double*);
-template __global__ void k_dd_tab<unsigned short>(int, int, const float4*, const unsigned short*,
-                                                  const int*, const float4*, const int*, double*);
 
 // Fractional coordinates as 32-bit fixed point in [0, 1), rank ordered:
 // the difference of two, read as a signed integer, is the minimum-image
@@ -861,21 +742,33 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pack_u4(int n, const double* __r
   u4[rank ? rank[i] : i] = make_uint4(u[0], u[1], u[2], 0u);
 }
 
-// The SCF dipole field from the (A2, A1) table of k_dd_table<false, float2>
-// (or the static FP32 pass): r = A (u_i - u_j) 2^-32 from the fixed-point
-// positions, E_i = sum_j [A2 r (r.mu_j) - A1 mu_j].  The wave schedule and
-// the unconditional pipelined loads are those of k_dd_tab; a neighbour's
-// position and dipole are two 16-byte gathers from the same slot.
+// The SCF dipole field from the (A2, A1) table of k_dd_table (or the static
+// FP32 pass): r = A (u_i - u_j) 2^-32 from the fixed-point positions,
+// E_i = sum_j [A2 r (r.mu_j) - A1 mu_j], with A the cell of the row's
+// configuration (bc, cfg: a batch) or cell; a neighbour's position and
+// dipole are two 16-byte gathers from the same slot.  One wave per atom,
+// persistent: the waves of the grid take the rows i0 = wave, wave + waves,
+// ...; while a wave gathers and sums one row, the loads of its next row are
+// already in flight, so that the table stream does not pause.  The first
+// 512 entries of a row are pipelined (8 per lane); longer rows finish in a
+// plain loop.  The pipelined loads are issued unconditionally: indexes are
+// clamped to the row's last entry (a row without pairs holds one zero
+// entry) and to the last row, and the entries past the count are masked by
+// a zero factor, so that no branch splits the loads in flight.
 template <typename IT>
 __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_dd_tab2(
     int n, int maxnb, const float2* __restrict__ tab, const IT* __restrict__ tj,
     const int* __restrict__ tcnt, const float4* __restrict__ mu4, const uint4* __restrict__ u4,
-    Cell cell, const int* __restrict__ perm, double* __restrict__ ef) {
+    Cell cell, const BatchCfg* __restrict__ bc, const int* __restrict__ cfg,
+    const int* __restrict__ perm, double* __restrict__ ef) {
   constexpr int U = 8;
   constexpr float Q = 2.3283064365386963e-10f;  // 2^-32
-  const float A0 = (float)cell.A[0] * Q, A1c = (float)cell.A[1] * Q, A2c = (float)cell.A[2] * Q,
-              A3 = (float)cell.A[3] * Q, A4 = (float)cell.A[4] * Q, A5 = (float)cell.A[5] * Q,
-              A6 = (float)cell.A[6] * Q, A7 = (float)cell.A[7] * Q, A8 = (float)cell.A[8] * Q;
+  float Am[9];
+  auto set_cell = [&](const Cell& cl) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Am[k] = (float)cl.A[k] * Q;
+  };
+  set_cell(cell);
   const int lane = threadIdx.x & 63;
   const int wid =
       __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
@@ -900,9 +793,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
                   float& e1, float& e2) {
     const float dx = (float)(int)(ui.x - uj.x), dy = (float)(int)(ui.y - uj.y),
                 dz = (float)(int)(ui.z - uj.z);
-    const float rx = dx * A0 + dy * A3 + dz * A6;
-    const float ry = dx * A1c + dy * A4 + dz * A7;
-    const float rz = dx * A2c + dy * A5 + dz * A8;
+    const float rx = dx * Am[0] + dy * Am[3] + dz * Am[6];
+    const float ry = dx * Am[1] + dy * Am[4] + dz * Am[7];
+    const float rz = dx * Am[2] + dy * Am[5] + dz * Am[8];
     const float rm = (rx * m.x + ry * m.y + rz * m.z) * t.x;
     e0 += rx * rm - t.y * m.x;
     e1 += ry * rm - t.y * m.y;
@@ -916,6 +809,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   load(r, c, g, jj);
   for (; r < n; r += nw) {
     const int iout = perm ? perm[r] : r;
+    if (bc) set_cell(bc[cfg[iout]].cell);
     const uint4 ui = u4[r];
     float4 m[U];
     uint4 q[U];
@@ -960,14 +854,16 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 }
 
 template __global__ void k_dd_tab2<int>(int, int, const float2*, const int*, const int*,
-                                        const float4*, const uint4*, Cell, const int*, double*);
+                                        const float4*, const uint4*, Cell, const BatchCfg*,
+                                        const int*, const int*, double*);
 template __global__ void k_dd_tab2<unsigned short>(int, int, const float2*, const unsigned short*,
                                                    const int*, const float4*, const uint4*, Cell,
-                                                   const int*, double*);
+                                                   const BatchCfg*, const int*, const int*,
+                                                   double*);
 template __global__ void k_pairs_t2<false>(EsDirArgs, double*);
 template __global__ void k_pairs_t2<true>(EsDirArgs, double*);
-template __global__ void k_dd_table<false, float2>(EsDirArgs, float2*, int*, int*);
-template __global__ void k_dd_table<true, float4>(EsDirArgs, float4*, int*, int*);
+template __global__ void k_dd_table<false>(EsDirArgs, float2*, int*, int*);
+template __global__ void k_dd_table<true>(EsDirArgs, float2*, int*, int*);
 template __global__ void k_pack_mu<double4>(int, const double*, double4*);
 template __global__ void k_pack_mu<float4>(int, const double*, float4*);
 
