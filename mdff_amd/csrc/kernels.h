@@ -113,6 +113,10 @@ struct EsDirArgs {
   int* ddtj;
   int* ddcnt;
   int dd16;  // pair table: neighbour slots as 16-bit indexes (n <= 65536)
+  // pair table: only pairs of two species whose bits are set (the
+  // polarizable ones; the dipole field is needed at them only and only
+  // their dipoles are nonzero)
+  unsigned polmask;
 };
 
 struct RhoArgs {
@@ -223,6 +227,7 @@ __global__ void k_cell_bin(int n, const double4* s4, int nc0, int nc1, int nc2, 
 __global__ void k_cell_scan(int m, int* cnt, int* cur);
 __global__ void k_cell_fill(int n, const int* cellof, int* cur, int* perm);
 __global__ void k_cell_order(int m, const int* cnt, int* perm, int* rank);
+__global__ void k_pol_rows(int n, const int* perm, const int* typ, unsigned polmask, int* rows);
 __global__ void k_nlist_cells(NlCellArgs a);
 __global__ void k_perm_gather(int n, int inner, int planes, const int* idx, const double* src,
                               double* dst);
@@ -279,7 +284,8 @@ __global__ void k_pack_u4(int n, const double* s, const int* rank, uint4* u4);
 template <typename IT>
 __global__ void k_dd_tab2(int n, int maxnb, const float2* tab, const IT* tj, const int* tcnt,
                           const float4* mu4, const uint4* u4, Cell cell, const BatchCfg* bc,
-                          const int* cfg, const int* perm, double* ef);
+                          const int* cfg, const int* perm, const int* rows, int nrows,
+                          double* ef);
 // neighbour slots of the FP32 pair table fit 16 bits
 inline bool dd_idx16(long long n) { return n <= 65536; }
 // Grid of k_dd_tab2 (persistent waves): four per SIMD of the device, at most

@@ -85,6 +85,12 @@ struct mdffgpu_ctx {
   // within a cell), refreshed with the neighbour list: perm[rank] = atom
   int* d_perm = nullptr;
   int* d_rank = nullptr;
+  // FP32 SCF pair table: its species (the dipole-polarizable ones, bit per
+  // species) and the ranks of their atoms in rank order (npolrows of them,
+  // k_pol_rows at every list build)
+  unsigned polmask = 0;
+  int npolrows = 0;
+  int* d_polrows = nullptr;
   // internal numbering of the atoms, fixed at the first evaluation: species
   // by species, by cell of about 4 A (row-major) and caller index within a
   // species, so that a neighbour row meets the species in long runs and
@@ -393,8 +399,11 @@ void build_tables(mdffgpu_ctx* c) {
   c->ldip_static = 0;
   for (int k = 0; k < 48; ++k)
     if (c->P.dip[k] != 0.0) c->ldip_static = 1;
-  for (int t = 0; t < c->P.ntype; ++t)
+  c->polmask = 0;
+  for (int t = 0; t < c->P.ntype; ++t) {
     if (c->tt_h.polar[t] || c->tt_h.qpolar[t]) c->linduced = 1;
+    if (c->tt_h.polar[t]) c->polmask |= 1u << t;
+  }
   HCK(c, hipMemcpy(c->d_pt, &c->pt_h, sizeof(c->pt_h), hipMemcpyHostToDevice));
   HCK(c, hipMemcpy(c->d_tt, &c->tt_h, sizeof(c->tt_h), hipMemcpyHostToDevice));
 }
@@ -504,6 +513,7 @@ void alloc_atoms(mdffgpu_ctx* c) {
   if (c->P.scf_solver == MDFFGPU_SCF_PCG) dalloc(c, &c->d_cg, 12 * n);
   dalloc(c, &c->d_perm, n);
   dalloc(c, &c->d_rank, n);
+  dalloc(c, &c->d_polrows, n);
   dalloc(c, &c->d_cellof, n);
   dalloc(c, &c->d_mup, 3 * n);
   dalloc(c, &c->d_Estat, 3 * n);
@@ -577,14 +587,16 @@ void es_ind_dir(mdffgpu_ctx* c, const double* mu, bool packed = false) {
     const float2* tab = dd_tab(c);
     const int* tj = dd_tj(c);
     if (!packed) k_pack_mu_ranked<<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_rank, m4);
-    if (dd_idx16(n))
-      k_dd_tab2<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(
+    // the rows of the polarizable atoms only: the field elsewhere is unused
+    const int nr = c->npolrows;
+    if (nr > 0 && dd_idx16(n))
+      k_dd_tab2<<<dd_tab_blocks(nr), MDFF_BLOCK, 0, st>>>(
           n, c->maxnb, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4, dd_u4(c),
-          c->cell, nullptr, nullptr, c->d_perm, c->d_efdir);
-    else
-      k_dd_tab2<<<dd_tab_blocks(n), MDFF_BLOCK, 0, st>>>(n, c->maxnb, tab, tj, tj + cap, m4,
-                                                         dd_u4(c), c->cell, nullptr, nullptr,
-                                                         c->d_perm, c->d_efdir);
+          c->cell, nullptr, nullptr, c->d_perm, c->d_polrows, nr, c->d_efdir);
+    else if (nr > 0)
+      k_dd_tab2<<<dd_tab_blocks(nr), MDFF_BLOCK, 0, st>>>(
+          n, c->maxnb, tab, tj, tj + cap, m4, dd_u4(c), c->cell, nullptr, nullptr, c->d_perm,
+          c->d_polrows, nr, c->d_efdir);
   } else {
     EsDirArgs da;
     std::memset(&da, 0, sizeof(da));
@@ -647,6 +659,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     da.ddcnt = da.ddtj + cap;
     da.rank = c->d_rank;
     da.dd16 = dd_idx16(n);
+    da.polmask = c->polmask;
     c->t2_in_static = false;
   }
   if (mode != ES_IND) {
@@ -908,6 +921,7 @@ void build_t2(mdffgpu_ctx* c) {
   da.rank = c->d_rank;
   da.perm = c->d_perm;
   da.dd16 = dd_idx16(n);
+  da.polmask = c->polmask;
   // the FP32 table with an FP32 static real-space pass: written by that
   // pass (the SCF's first Ewald call, after this one in every evaluation)
   if (mx) {
@@ -1290,6 +1304,8 @@ void cell_sort(mdffgpu_ctx* c) {
   k_cell_scan<<<1, 1024, 0, st>>>(ncell, c->d_cstart, c->d_ccur);
   k_cell_fill<<<nblk(n), MDFF_BLOCK, 0, st>>>(n, c->d_cellof, c->d_ccur, c->d_perm);
   k_cell_order<<<nblk(ncell), MDFF_BLOCK, 0, st>>>(ncell, c->d_cstart, c->d_perm, c->d_rank);
+  if (c->linduced && scf_field_fp32(c))
+    k_pol_rows<<<1, 1024, 0, st>>>(n, c->d_perm, c->d_typ, c->polmask, c->d_polrows);
   check_launch(c, "cell sort");
 }
 
@@ -1300,6 +1316,8 @@ void load_species(mdffgpu_ctx* c) {
   const mdffgpu_params& P = c->P;
   const int n = c->n;
   HCK(c, hipMemcpy(c->d_typ, c->typ0.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  c->npolrows = 0;
+  for (int i = 0; i < n; ++i) c->npolrows += (int)((c->polmask >> c->typ0[i]) & 1u);
   std::vector<double> q(n), dip(3 * (size_t)n);
   for (int i = 0; i < n; ++i) {
     const int t = c->typ0[i];

@@ -104,6 +104,33 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_cell_order(int m, const int* __r
   for (int r = b; r < e; ++r) rank[perm[r]] = r;
 }
 
+// Ranks r whose atom perm[r] has a species with its bit in polmask, in rank
+// order: one workgroup takes the ranks in chunks of its size, and the
+// threads of a hit find its slot from the wave ballots of the chunk.
+__global__ __launch_bounds__(1024) void k_pol_rows(int n, const int* __restrict__ perm,
+                                                   const int* __restrict__ typ,
+                                                   unsigned polmask, int* __restrict__ rows) {
+  __shared__ int wsum[1024 / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = (int)(blockDim.x >> 6);
+  int base = 0;
+  for (int r0 = 0; r0 < n; r0 += blockDim.x) {
+    const int r = r0 + (int)threadIdx.x;
+    const bool hit = r < n && ((polmask >> typ[perm[r]]) & 1u);
+    const unsigned long long m = __ballot(hit);
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int off = base, tot = 0;
+    for (int k = 0; k < nw; ++k) {
+      if (k < w) off += wsum[k];
+      tot += wsum[k];
+    }
+    if (hit) rows[off + __popcll(m & ((1ull << lane) - 1ull))] = r;
+    base += tot;
+    __syncthreads();
+  }
+}
+
 // Neighbour rows from the stencil: one wave (and workgroup) per centre
 // atom, the centres in cell order.  Rows nbr[i * maxnb + k]; cnt[i] the row
 // length; the longest row above maxnb goes to overflow (the caller grows

@@ -263,12 +263,15 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
       const int tj = BATCH ? a.typ[j] : s4_type(sj);
       R rx, ry, rz;
       const R d2 = min_image_t<R>(cell, si1 - sj.x, si2 - sj.y, si3 - sj.z, rx, ry, rz);
-      // SCF pair table: the entry's slot among the in-cutoff pairs of the
-      // atom's lane group, in the group's visiting order
+      // SCF pair table: the entry's slot among the tabled pairs (inside the
+      // cutoff, both species in polmask) of the atom's lane group, in the
+      // group's visiting order
       int tpos = 0;
+      bool tin = false;
       if constexpr (TAB) {
         if (a.ddtab) {
-          const unsigned gm = (unsigned)(__ballot(!(d2 > cut2)) >> (threadIdx.x & (64 - LPA))) &
+          tin = !(d2 > cut2) && ((a.polmask >> ti) & (a.polmask >> tj) & 1u);
+          const unsigned gm = (unsigned)(__ballot(tin) >> (threadIdx.x & (64 - LPA))) &
                               (unsigned)((1ull << LPA) - 1ull);
           tpos = tbase + __popc(gm & ((1u << sub) - 1u));
           tbase += __popc(gm);
@@ -311,7 +314,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
       const R F1_dm3 = F1 * dm3;
       const R F2_dm5 = R(3) * F2 * dm5;
       if constexpr (TAB) {
-        if (a.ddtab) {
+        if (tin) {
           // (A2, A1), A2 = 3 F2 / d^5, A1 = F1 / d^3 (undamped)
           const size_t e = (size_t)i0 * a.maxnb + tpos;
           a.ddtab[e] = make_float2(F2_dm5, F1_dm3);
@@ -675,7 +678,10 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float2* __
   const float alpha = (float)alphad, alpha2 = alpha * alpha;
   const float c43a3 = (float)(4.0 / 3.0 * alphad * alphad * alphad);
   const double4 si = a.s4[i];
-  const int c = a.cnt[i];
+  // only pairs of two species in polmask (all with 0)
+  const unsigned pm = a.polmask ? a.polmask : ~0u;
+  const bool pol_i = (pm >> (BATCH ? a.typ[i] : s4_type(si))) & 1u;
+  const int c = pol_i ? a.cnt[i] : 0;
   const size_t lrow = (size_t)i * a.maxnb;  // the atom's list row
   const size_t row = (size_t)i0 * a.maxnb;  // its table row: processing order
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -690,7 +696,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_table(EsDirArgs a, float2* __
       const double4 sj = a.s4[j];
       float rx, ry, rz;
       const float d2 = min_image_t<float>(cell, si.x - sj.x, si.y - sj.y, si.z - sj.z, rx, ry, rz);
-      in = !(d2 > cut2);
+      in = !(d2 > cut2) && ((pm >> (BATCH ? a.typ[j] : s4_type(sj))) & 1u);
       if (in) {
         const float d = sqrtf(d2);
         const float dm1 = 1.f / d;
@@ -747,9 +753,12 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pack_u4(int n, const double* __r
 // E_i = sum_j [A2 r (r.mu_j) - A1 mu_j], with A the cell of the row's
 // configuration (bc, cfg: a batch) or cell; a neighbour's position and
 // dipole are two 16-byte gathers from the same slot.  One wave per atom,
-// persistent: the waves of the grid take the rows i0 = wave, wave + waves,
-// ...; while a wave gathers and sums one row, the loads of its next row are
-// already in flight, so that the table stream does not pause.  The first
+// persistent: the waves of the grid take the visits v = wave, wave + waves,
+// ... of the nrows rows listed in rows (all n rows in order when rows is
+// null; the rows of atoms outside the table's species are empty and need
+// not be listed, their field is not written); while a wave gathers and sums
+// one row, the loads of its next row are already in flight, so that the
+// table stream does not pause.  The first
 // 512 entries of a row are pipelined (8 per lane); longer rows finish in a
 // plain loop.  The pipelined loads are issued unconditionally: indexes are
 // clamped to the row's last entry (a row without pairs holds one zero
@@ -760,7 +769,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     int n, int maxnb, const float2* __restrict__ tab, const IT* __restrict__ tj,
     const int* __restrict__ tcnt, const float4* __restrict__ mu4, const uint4* __restrict__ u4,
     Cell cell, const BatchCfg* __restrict__ bc, const int* __restrict__ cfg,
-    const int* __restrict__ perm, double* __restrict__ ef) {
+    const int* __restrict__ perm, const int* __restrict__ rows, int nrows,
+    double* __restrict__ ef) {
   constexpr int U = 8;
   constexpr float Q = 2.3283064365386963e-10f;  // 2^-32
   float Am[9];
@@ -773,8 +783,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   const int wid =
       __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int nw = (int)((gridDim.x * blockDim.x) >> 6);
+  // the row of visit v (clamped to the last visit: loads past it are
+  // issued and never used)
+  auto rowof = [&](int v) { return rows ? rows[min(v, nrows - 1)] : min(v, n - 1); };
   auto load = [&](int r, int c, float2 (&g)[U], int (&jj)[U]) {
-    const size_t row = (size_t)min(r, n - 1) * maxnb;
+    const size_t row = (size_t)r * maxnb;
     const float2* trow = tab + row;
     const IT* jrow = tj + row;
     const int last = max(c - 1, 0);
@@ -803,11 +816,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   };
   float2 g[U];
   int jj[U];
-  int r = wid;
-  int c = tcnt[min(r, n - 1)];
-  int cn = tcnt[min(r + nw, n - 1)];
+  int r = rowof(wid), rn = rowof(wid + nw);
+  int c = tcnt[r];
+  int cn = tcnt[rn];
   load(r, c, g, jj);
-  for (; r < n; r += nw) {
+  for (int v = wid; v < nrows; v += nw) {
     const int iout = perm ? perm[r] : r;
     if (bc) set_cell(bc[cfg[iout]].cell);
     const uint4 ui = u4[r];
@@ -822,8 +835,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     // are loaded clamped and never used)
     float2 gn[U];
     int jn[U];
-    load(r + nw, cn, gn, jn);
-    const int cnn = tcnt[min(r + 2 * nw, n - 1)];
+    load(rn, cn, gn, jn);
+    const int rnn = rowof(v + 2 * nw);
+    const int cnn = tcnt[rnn];
     float e0 = 0.f, e1 = 0.f, e2 = 0.f;
 #pragma unroll
     for (int u = 0; u < U; ++u) pair(ui, q[u], g[u], m[u], e0, e1, e2);
@@ -848,6 +862,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
       g[u] = gn[u];
       jj[u] = jn[u];
     }
+    r = rn;
+    rn = rnn;
     c = cn;
     cn = cnn;
   }
@@ -855,11 +871,11 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 
 template __global__ void k_dd_tab2<int>(int, int, const float2*, const int*, const int*,
                                         const float4*, const uint4*, Cell, const BatchCfg*,
-                                        const int*, const int*, double*);
+                                        const int*, const int*, const int*, int, double*);
 template __global__ void k_dd_tab2<unsigned short>(int, int, const float2*, const unsigned short*,
                                                    const int*, const float4*, const uint4*, Cell,
                                                    const BatchCfg*, const int*, const int*,
-                                                   double*);
+                                                   const int*, int, double*);
 template __global__ void k_pairs_t2<false>(EsDirArgs, double*);
 template __global__ void k_pairs_t2<true>(EsDirArgs, double*);
 template __global__ void k_dd_table<false>(EsDirArgs, float2*, int*, int*);
