@@ -883,13 +883,13 @@ void es_call(mdffgpu_batch* b, int mode, const double* q, const double* mu, cons
       const float2* tab = reinterpret_cast<const float2*>(u4 + N);
       const int* tj = reinterpret_cast<const int*>(tab + cap);
       if (dd_idx16(N))
-        k_dd_tab2<<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(
+        k_dd_tab2<unsigned short, 8><<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(
             N, b->maxnb, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4, u4,
             Cell{}, b->d_bc, b->d_cfg, nullptr, nullptr, N, b->d_dir);
       else
-        k_dd_tab2<<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(N, b->maxnb, tab, tj, tj + cap, m4,
-                                                              u4, Cell{}, b->d_bc, b->d_cfg,
-                                                              nullptr, nullptr, N, b->d_dir);
+        k_dd_tab2<int, 8><<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(
+            N, b->maxnb, tab, tj, tj + cap, m4, u4, Cell{}, b->d_bc, b->d_cfg, nullptr, nullptr,
+            N, b->d_dir);
     }
   } else {
     da.task = TASK_QQ | (b->pim ? (TASK_QD | TASK_DD) : 0);

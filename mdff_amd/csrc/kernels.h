@@ -281,7 +281,7 @@ __global__ void k_pack_qmu(int n, const double* q, const double* mu, R4* qm);
 template <bool BATCH>
 __global__ void k_dd_table(EsDirArgs a, float2* tab, int* tj, int* tcnt);
 __global__ void k_pack_u4(int n, const double* s, const int* rank, uint4* u4);
-template <typename IT>
+template <typename IT, int U>
 __global__ void k_dd_tab2(int n, int maxnb, const float2* tab, const IT* tj, const int* tcnt,
                           const float4* mu4, const uint4* u4, Cell cell, const BatchCfg* bc,
                           const int* cfg, const int* perm, const int* rows, int nrows,

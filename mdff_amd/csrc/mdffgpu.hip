@@ -90,6 +90,9 @@ struct mdffgpu_ctx {
   // k_pol_rows at every list build)
   unsigned polmask = 0;
   int npolrows = 0;
+  // FP64 SCF field: the species whose field the SCF reads (dipole or
+  // quadrupole polarizable)
+  unsigned fieldmask = 0;
   int* d_polrows = nullptr;
   // internal numbering of the atoms, fixed at the first evaluation: species
   // by species, by cell of about 4 A (row-major) and caller index within a
@@ -400,8 +403,12 @@ void build_tables(mdffgpu_ctx* c) {
   for (int k = 0; k < 48; ++k)
     if (c->P.dip[k] != 0.0) c->ldip_static = 1;
   c->polmask = 0;
+  c->fieldmask = 0;
   for (int t = 0; t < c->P.ntype; ++t) {
-    if (c->tt_h.polar[t] || c->tt_h.qpolar[t]) c->linduced = 1;
+    if (c->tt_h.polar[t] || c->tt_h.qpolar[t]) {
+      c->linduced = 1;
+      c->fieldmask |= 1u << t;
+    }
     if (c->tt_h.polar[t]) c->polmask |= 1u << t;
   }
   HCK(c, hipMemcpy(c->d_pt, &c->pt_h, sizeof(c->pt_h), hipMemcpyHostToDevice));
@@ -576,6 +583,35 @@ int* dd_tj(mdffgpu_ctx* c) {
   return reinterpret_cast<int*>(dd_tab(c) + (size_t)c->maxnb * c->n);
 }
 
+// Entries per lane of k_dd_tab2's pipeline: the expected row of the table
+// (the polarizable neighbours inside the cutoff, from their density) with
+// 15 % to spare, as 4, 6 or 8.
+int dd_unroll(const mdffgpu_ctx* c) {
+  const double rc = c->P.cutlongrange;
+  const double mean = c->npolrows / c->cell.omega * (4.0 / 3.0) * MDFF_PI * rc * rc * rc;
+  const double need = 1.15 * mean / 64.0;
+  return need <= 4.0 ? 4 : need <= 6.0 ? 6 : 8;
+}
+
+template <typename IT, int U>
+void launch_dd_tab2(mdffgpu_ctx* c, const float2* tab, const IT* tj, const int* tcnt,
+                    const float4* m4) {
+  const int nr = c->npolrows;
+  k_dd_tab2<IT, U><<<dd_tab_blocks(nr), MDFF_BLOCK, 0, c->st>>>(
+      c->n, c->maxnb, tab, tj, tcnt, m4, dd_u4(c), c->cell, nullptr, nullptr, c->d_perm,
+      c->d_polrows, nr, c->d_efdir);
+}
+
+template <typename IT>
+void launch_dd_tab2(mdffgpu_ctx* c, const float2* tab, const IT* tj, const int* tcnt,
+                    const float4* m4) {
+  switch (dd_unroll(c)) {
+    case 4: launch_dd_tab2<IT, 4>(c, tab, tj, tcnt, m4); break;
+    case 6: launch_dd_tab2<IT, 6>(c, tab, tj, tcnt, m4); break;
+    default: launch_dd_tab2<IT, 8>(c, tab, tj, tcnt, m4); break;
+  }
+}
+
 void es_ind_dir(mdffgpu_ctx* c, const double* mu, bool packed = false) {
   const int n = c->n;
   hipStream_t st = c->st;
@@ -588,15 +624,10 @@ void es_ind_dir(mdffgpu_ctx* c, const double* mu, bool packed = false) {
     const int* tj = dd_tj(c);
     if (!packed) k_pack_mu_ranked<<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_rank, m4);
     // the rows of the polarizable atoms only: the field elsewhere is unused
-    const int nr = c->npolrows;
-    if (nr > 0 && dd_idx16(n))
-      k_dd_tab2<<<dd_tab_blocks(nr), MDFF_BLOCK, 0, st>>>(
-          n, c->maxnb, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4, dd_u4(c),
-          c->cell, nullptr, nullptr, c->d_perm, c->d_polrows, nr, c->d_efdir);
-    else if (nr > 0)
-      k_dd_tab2<<<dd_tab_blocks(nr), MDFF_BLOCK, 0, st>>>(
-          n, c->maxnb, tab, tj, tj + cap, m4, dd_u4(c), c->cell, nullptr, nullptr, c->d_perm,
-          c->d_polrows, nr, c->d_efdir);
+    if (c->npolrows > 0 && dd_idx16(n))
+      launch_dd_tab2(c, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4);
+    else if (c->npolrows > 0)
+      launch_dd_tab2(c, tab, tj, tj + cap, m4);
   } else {
     EsDirArgs da;
     std::memset(&da, 0, sizeof(da));
@@ -606,6 +637,8 @@ void es_ind_dir(mdffgpu_ctx* c, const double* mu, bool packed = false) {
     da.nbr = c->d_nbr;
     da.perm = c->d_perm;
     da.ef = c->d_efdir;
+    da.typ = c->d_typ;
+    da.polmask = c->fieldmask;
     k_pack_mu<double4><<<gb, MDFF_BLOCK, 0, st>>>(n, mu, c->d_mu4);
     k_dd_field<<<nblk(n * DD_LANES), MDFF_BLOCK, 0, st>>>(da, (const double*)c->d_t2, c->d_mu4);
   }
@@ -937,6 +970,8 @@ void build_t2(mdffgpu_ctx* c) {
     int* tj = dd_tj(c);
     k_dd_table<false><<<nblk(n * 64), MDFF_BLOCK, 0, c->st>>>(da, dd_tab(c), tj, tj + cap);
   } else {
+    da.typ = c->d_typ;
+    da.polmask = c->fieldmask;
     k_pairs_t2<<<nblk(n * LPA), MDFF_BLOCK, 0, c->st>>>(da, (double*)c->d_t2);
   }
   check_launch(c, "k_pairs_t2");

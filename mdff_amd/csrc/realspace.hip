@@ -510,7 +510,9 @@ template __global__ void k_es_dir<ES_FINAL, float, true>(EsDirArgs);
 // SCF, so k_pairs_t2 stores for every directed pair inside the cutoff, in the
 // row layout of nbr, the four-vector (r, 3F2/d^5) and then, in a plane of
 // its own, F1/d^3; pairs outside the cutoff get zero factors.  k_dd_field
-// applies E_i = sum_j [r (r.mu_j) 3F2/d^5 - mu_j F1/d^3].
+// applies E_i = sum_j [r (r.mu_j) 3F2/d^5 - mu_j F1/d^3].  With polmask set,
+// both run over the atoms of its species only (the field is used at the
+// polarizable atoms only; elsewhere it is not written).
 template <bool BATCH>
 __global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, double* __restrict__ tab) {
   const int n = a.n;
@@ -525,6 +527,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pairs_t2(EsDirArgs a, double* __
   const double alpha2 = alpha * alpha, alpha3 = alpha2 * alpha;
   const double piroot = sqrt(MDFF_PI);
   const double si1 = a.s[i], si2 = a.s[i + n], si3 = a.s[i + 2 * n];
+  // rows of the species in polmask only (all with 0): k_dd_field skips
+  // the others
+  if (a.polmask && !((a.polmask >> a.typ[i]) & 1u)) return;
   const int c = a.cnt[i];
   const size_t plane = (size_t)a.maxnb * n;
   for (int k = sub; k < c; k += LPA) {
@@ -611,7 +616,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_dd_field(EsDirArgs a,
   const size_t plane = (size_t)a.maxnb * n;
   const R4* t4 = reinterpret_cast<const R4*>(tab);
   const R* t1 = tab + 4 * plane;
-  const bool act = i < n;
+  const bool act = i < n && (!a.polmask || ((a.polmask >> a.typ[i]) & 1u));
   const int c = act ? a.cnt[i] : 0;
   const size_t row = (size_t)(act ? i : 0) * a.maxnb;
   constexpr int U = 2;
@@ -758,20 +763,19 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_pack_u4(int n, const double* __r
 // null; the rows of atoms outside the table's species are empty and need
 // not be listed, their field is not written); while a wave gathers and sums
 // one row, the loads of its next row are already in flight, so that the
-// table stream does not pause.  The first
-// 512 entries of a row are pipelined (8 per lane); longer rows finish in a
-// plain loop.  The pipelined loads are issued unconditionally: indexes are
+// table stream does not pause.  The first 64 U entries of a row are
+// pipelined (U per lane, chosen for the expected row length); longer rows
+// finish in a plain loop.  The pipelined loads are issued unconditionally: indexes are
 // clamped to the row's last entry (a row without pairs holds one zero
 // entry) and to the last row, and the entries past the count are masked by
 // a zero factor, so that no branch splits the loads in flight.
-template <typename IT>
+template <typename IT, int U>
 __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_dd_tab2(
     int n, int maxnb, const float2* __restrict__ tab, const IT* __restrict__ tj,
     const int* __restrict__ tcnt, const float4* __restrict__ mu4, const uint4* __restrict__ u4,
     Cell cell, const BatchCfg* __restrict__ bc, const int* __restrict__ cfg,
     const int* __restrict__ perm, const int* __restrict__ rows, int nrows,
     double* __restrict__ ef) {
-  constexpr int U = 8;
   constexpr float Q = 2.3283064365386963e-10f;  // 2^-32
   float Am[9];
   auto set_cell = [&](const Cell& cl) {
@@ -869,13 +873,17 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   }
 }
 
-template __global__ void k_dd_tab2<int>(int, int, const float2*, const int*, const int*,
-                                        const float4*, const uint4*, Cell, const BatchCfg*,
-                                        const int*, const int*, const int*, int, double*);
-template __global__ void k_dd_tab2<unsigned short>(int, int, const float2*, const unsigned short*,
-                                                   const int*, const float4*, const uint4*, Cell,
-                                                   const BatchCfg*, const int*, const int*,
-                                                   const int*, int, double*);
+#define MDFF_DD_TAB2(IT, U)                                                                  \
+  template __global__ void k_dd_tab2<IT, U>(int, int, const float2*, const IT*, const int*,        \
+                                            const float4*, const uint4*, Cell, const BatchCfg*,    \
+                                            const int*, const int*, const int*, int, double*);
+MDFF_DD_TAB2(int, 4)
+MDFF_DD_TAB2(int, 6)
+MDFF_DD_TAB2(int, 8)
+MDFF_DD_TAB2(unsigned short, 4)
+MDFF_DD_TAB2(unsigned short, 6)
+MDFF_DD_TAB2(unsigned short, 8)
+#undef MDFF_DD_TAB2
 template __global__ void k_pairs_t2<false>(EsDirArgs, double*);
 template __global__ void k_pairs_t2<true>(EsDirArgs, double*);
 template __global__ void k_dd_table<false>(EsDirArgs, float2*, int*, int*);
