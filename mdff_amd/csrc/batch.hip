@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mdffgpu.h"
@@ -657,6 +658,12 @@ struct mdffgpu_batch {
   long long epoch = 0;
   int pol_type[MDFF_NT];
   double pol_alpha[MDFF_NT];
+  // SCF pair table: the polarizable species (bit per species), the atoms of
+  // those species in index order and the entries per lane of k_dd_tab2's
+  // pipeline
+  unsigned polmask = 0;
+  int npolrows = 0, dd_u = 8;
+  int* d_polrows = nullptr;
   int upair_h[256];
   std::vector<mdff::BatchCfg> bc_h;
   mdff::BatchCfg* d_bc = nullptr;
@@ -778,6 +785,7 @@ EsDirArgs es_args(mdffgpu_batch* b, const double* q, const double* mu) {
   da.part = b->d_part;
   da.bc = b->d_bc;
   da.cfg = b->d_cfg;
+  da.polmask = b->polmask;
   return da;
 }
 
@@ -882,14 +890,26 @@ void es_call(mdffgpu_batch* b, int mode, const double* q, const double* mu, cons
       const uint4* u4 = reinterpret_cast<const uint4*>(b->d_t2);
       const float2* tab = reinterpret_cast<const float2*>(u4 + N);
       const int* tj = reinterpret_cast<const int*>(tab + cap);
-      if (dd_idx16(N))
-        k_dd_tab2<unsigned short, 8><<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(
-            N, b->maxnb, tab, reinterpret_cast<const unsigned short*>(tj), tj + cap, m4, u4,
-            Cell{}, b->d_bc, b->d_cfg, nullptr, nullptr, N, b->d_dir);
-      else
-        k_dd_tab2<int, 8><<<dd_tab_blocks(N), MDFF_BLOCK, 0, b->st>>>(
-            N, b->maxnb, tab, tj, tj + cap, m4, u4, Cell{}, b->d_bc, b->d_cfg, nullptr, nullptr,
-            N, b->d_dir);
+      // the rows of the polarizable atoms only: the field elsewhere is unused
+      auto launch = [&](auto ut, auto* tjp) {
+        constexpr int U = decltype(ut)::value;
+        using IT = typename std::remove_const<std::remove_pointer_t<decltype(tjp)>>::type;
+        k_dd_tab2<IT, U><<<dd_tab_blocks(b->npolrows), MDFF_BLOCK, 0, b->st>>>(
+            N, b->maxnb, tab, tjp, tj + cap, m4, u4, Cell{}, b->d_bc, b->d_cfg, nullptr,
+            b->d_polrows, b->npolrows, b->d_dir);
+      };
+      auto launch_u = [&](auto* tjp) {
+        if (b->dd_u == 4)
+          launch(std::integral_constant<int, 4>{}, tjp);
+        else if (b->dd_u == 6)
+          launch(std::integral_constant<int, 6>{}, tjp);
+        else
+          launch(std::integral_constant<int, 8>{}, tjp);
+      };
+      if (b->npolrows > 0 && dd_idx16(N))
+        launch_u(reinterpret_cast<const unsigned short*>(tj));
+      else if (b->npolrows > 0)
+        launch_u(tj);
     }
   } else {
     da.task = TASK_QQ | (b->pim ? (TASK_QD | TASK_DD) : 0);
@@ -1271,6 +1291,30 @@ void create_impl(const mdffgpu_params* model, int nconf, const mdffgpu_batch_con
   BCK(b, hipMemcpy(b->d_bc, b->bc_h.data(), nconf * sizeof(BatchCfg), hipMemcpyHostToDevice));
   BCK(b, hipMemcpy(b->d_cfg, cfg.data(), N * sizeof(int), hipMemcpyHostToDevice));
   BCK(b, hipMemcpy(b->d_typ, typ.data(), N * sizeof(int), hipMemcpyHostToDevice));
+  // the polarizable atoms (rows of the SCF pair table) and the pipeline
+  // depth of k_dd_tab2 for the longest expected row: the configuration with
+  // the most polarizable neighbours inside its cutoff, 15 % to spare
+  b->polmask = 0;
+  for (int t = 0; t < b->ntype; ++t)
+    if (b->ty_h.polar[t]) b->polmask |= 1u << t;
+  std::vector<int> prow;
+  std::vector<int> npc(nconf, 0);
+  for (int i = 0; i < N; ++i)
+    if ((b->polmask >> typ[i]) & 1u) {
+      prow.push_back(i);
+      ++npc[cfg[i]];
+    }
+  b->npolrows = (int)prow.size();
+  double row_max = 0.0;
+  for (int c = 0; c < nconf; ++c) {
+    const double rc3 = std::pow(b->bc_h[c].cut2, 1.5);
+    row_max = std::max(row_max, npc[c] / b->bc_h[c].cell.omega * (4.0 / 3.0) * MDFF_PI * rc3);
+  }
+  const double need = 1.15 * row_max / 64.0;
+  b->dd_u = need <= 4.0 ? 4 : need <= 6.0 ? 6 : 8;
+  balloc(b, &b->d_polrows, std::max(1, b->npolrows));
+  if (b->npolrows > 0)
+    BCK(b, hipMemcpy(b->d_polrows, prow.data(), prow.size() * sizeof(int), hipMemcpyHostToDevice));
   BCK(b, hipMemcpy(b->d_kn, kn.data(), kn.size() * sizeof(int4), hipMemcpyHostToDevice));
   BCK(b, hipMemcpy(b->d_kv, kv.data(), kv.size() * sizeof(double4), hipMemcpyHostToDevice));
   BCK(b, hipMemcpy(b->d_items, items.data(), items.size() * sizeof(int2),
