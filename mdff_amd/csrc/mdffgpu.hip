@@ -392,6 +392,8 @@ void build_kmesh(mdffgpu_ctx* c) {
   c->app_split = (c->nchunk + c->app_chunks - 1) / c->app_chunks;
   dalloc(c, &c->d_rpart, (size_t)c->rho_split * c->kpad);
   dalloc(c, &c->d_apart, (size_t)c->app_split * 9 * n);
+  // rows a partial call leaves alone hold finite values
+  HCK(c, hipMemsetAsync(c->d_apart, 0, (size_t)c->app_split * 9 * n * sizeof(double), c->st));
 }
 
 void build_tables(mdffgpu_ctx* c) {
@@ -1344,6 +1346,35 @@ void cell_sort(mdffgpu_ctx* c) {
   check_launch(c, "cell sort");
 }
 
+// Atom ranges of the mixed reciprocal space in the context's numbering: the
+// atoms of species that can carry a dipole (polarizable, a static dipole or
+// Wannier centres) and of the polarizable species (the SCF reads the field
+// there only).  The allocation-free context keeps all atoms in both.
+void mx_active(mdffgpu_ctx* c) {
+  if (!c->mx || c->noalloc) return;
+  const mdffgpu_params& P = c->P;
+  unsigned dipm = c->polmask;
+  for (int t = 0; t < P.ntype; ++t) {
+    for (int a = 0; a < 3; ++a)
+      if (P.dip[t + 16 * a] != 0.0) dipm |= 1u << t;
+    if (P.lwfc[t] > 0) dipm |= 1u << t;
+  }
+  auto range = [&](unsigned mask, int& a0, int& a1) {
+    a0 = 0;
+    a1 = -1;  // none: all atoms
+    for (int i = 0; i < c->n; ++i)
+      if ((mask >> c->typ0[i]) & 1u) {
+        if (a1 < 0) a0 = i;
+        a1 = i + 1;
+      }
+  };
+  int d0, d1, f0, f1;
+  range(dipm, d0, d1);
+  range(c->polmask, f0, f1);
+  if (!mixed::set_active(c->mx, d0, d1, f0, f1))
+    fail(c, MDFFGPU_ERR_ALLOC, "mixed reciprocal-space atom ranges");
+}
+
 // Per-atom species data of typeinfo_init (src/read_pos.f90:175-199) in the
 // context's numbering (typ0): species, charge, static dipole and, with
 // quadrupoles, the permanent quadrupole (quadia, :197).
@@ -1408,6 +1439,7 @@ void ensure_renum(mdffgpu_ctx* c, const double* rx, const double* ry, const doub
   for (int i = 0; i < n; ++i) t[i] = c->typ0[ord[i]];
   c->typ0.swap(t);
   load_species(c);
+  mx_active(c);
   c->renum = true;
   const size_t nh = (size_t)3 * K * n;
   k_perm_gather<<<(int)((nh + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0, c->st>>>(
@@ -2073,6 +2105,7 @@ void create_impl(const mdffgpu_params* params, const int32_t* itype, const doubl
     if (c->P.precision == MDFFGPU_PREC_MIXED) {
       c->mx = mixed::create(c);
       if (!c->mx) fail(c, MDFFGPU_ERR_UNSUPPORTED, "mixed-precision path unavailable");
+      mx_active(c);
     }
     // the clears of alloc_atoms are done before the context is handed out
     HCK(c, hipStreamSynchronize(c->st));

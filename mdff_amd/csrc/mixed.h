@@ -65,6 +65,12 @@ CtxView view(mdffgpu_ctx* c);
 MixedState* create(mdffgpu_ctx* c);
 void destroy(MixedState* m);
 void on_cell(MixedState* m);
+// One configuration: atoms dip0 .. dip1 - 1 hold every atom that can carry
+// a dipole (the dipole-only structure factors sum over them) and atoms
+// fld0 .. fld1 - 1 every atom whose field the static and SCF calls must
+// give (their apply rows); dip1 or fld1 < 0: all atoms.  The field of the
+// other atoms in those calls is left as the buffers held it.
+bool set_active(MixedState* m, int dip0, int dip1, int fld0, int fld1);
 // Reciprocal-space part of one multipole_ES call: writes the per-atom U, V
 // sums (apply-partials layout of k_apply64 with one split) to apart and the
 // energy / stress rows to the scalar slots.  With weights_ready (dipole

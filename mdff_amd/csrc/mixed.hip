@@ -47,6 +47,16 @@ struct MixedState {
   int CH = 0, CHp = 0;  // conjugate-paired half of the (n2, n3) columns
   int kx = 0, ky = 0, kz = 0;
   int nsplit = 0;
+  // one configuration: the atoms that can carry a dipole (the dipole-only
+  // structure factors sum over their slots, in splits of their own) and the
+  // atoms whose field the charge- and dipole-field calls must give (the
+  // rows of their apply GEMM and combine), as whole 128-slot blocks
+  // [dj0, dj1) and [fj0, fj1) of the padded axis; all slots by default
+  int dip0 = 0, dip1 = -1, fld0 = 0, fld1 = -1;
+  int dj0 = 0, dj1 = 0, fj0 = 0, fj1 = 0;
+  int2* d_split_dip = nullptr;
+  int nsplit_dip = 0, nsplit_dip_cap = 0;
+  int cg_parts = 0;  // p.Ap block partials of the last fused combine
   // phase tables, float2 (re, im)
   float2* tj1 = nullptr;  // [n1][j]   (N1 x npad)
   float2* tj2 = nullptr;  // [n2+ky][j]
@@ -1144,8 +1154,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_gprep(int N1, int N3, int C23, 
 }
 
 // ---------------------------------------------------------------------------
-// Apply GEMM: rows = atoms, K = 2 CHp (the conjugate-paired half of the
-// columns, operand of mx_gprep), NCT column tiles.  A workgroup of
+// Apply GEMM: rows = atoms (the row blocks mb0 .. mb0 + mblocks - 1), K =
+// 2 CHp (the conjugate-paired half of the columns, operand of mx_gprep), NCT
+// column tiles.  A workgroup of
 // APP_WAVES waves owns APP_WAVES*RT*16 atoms and one of S slices of K; wave
 // w owns row tiles RT*w .. RT*w+RT-1 and all column tiles.  A fragments
 // (A[i][2c] = Re P_i[c], A[i][2c+1] = Im P_i[c], P = E2 E3) are formed in
@@ -1159,15 +1170,16 @@ constexpr int APP_WAVES = 4;
 
 template <int NCT, int RT>
 __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
-    int npad, int KS, int S, int ksper, int mblocks, const int2* __restrict__ colofs,
+    int npad, int KS, int S, int ksper, int mb0, int mblocks, const int2* __restrict__ colofs,
     const float2* __restrict__ tj2, const float2* __restrict__ tj3,
     const _Float16* __restrict__ bf, const int* __restrict__ kseg, float* __restrict__ Xp) {
   __shared__ u32x4 sB[2][2 * NCT * 64];
   const int per = NXCD / S;
   const int xcd = blockIdx.x % NXCD;
   const int sl = xcd / per;
-  const int mb = (blockIdx.x / NXCD) * per + (xcd % per);
-  if (mb >= mblocks) return;  // whole block, before any barrier
+  const int mbl = (blockIdx.x / NXCD) * per + (xcd % per);
+  if (mbl >= mblocks) return;  // whole block, before any barrier
+  const int mb = mb0 + mbl;    // row blocks mb0 .. mb0 + mblocks - 1
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int kgrp = lane >> 4;
@@ -1336,7 +1348,7 @@ __global__ __launch_bounds__(64 * APP_WAVES) void mx_apply_gemm(
 constexpr int EPI_CHUNKS = 4;  // n1 ranges per slot, one wave each
 
 template <int S>
-__global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int N1, int W,
+__global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int j0, int N1, int W,
                                                             int ncolp,
                                                             const MxSeg* __restrict__ seg,
                                                             const int* __restrict__ kseg,
@@ -1351,7 +1363,7 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int 
   __shared__ double part[EPI_CHUNKS - 1][5][64];
   const int w = blockIdx.y;
   const int lane = threadIdx.x & 63, ch = threadIdx.x >> 6;
-  const int jp = blockIdx.x * 64 + lane;
+  const int jp = j0 + blockIdx.x * 64 + lane;  // slots j0 .. of the grid
   int z = 0, i = -1;
   if (jp < npad) i = slot_atom(jp, n, seg, kseg, z);
   double sr = 0, si = 0, pr = 0, pi = 0, qr = 0;
@@ -1426,7 +1438,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi(int n, int npad, int 
 // block partial of p.Ap at ca.rpart[block].  The operations and their order
 // are those of the two kernels it replaces.
 template <int S>
-__global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi_cg(int n, int npad, int N1, int ncolp,
+__global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi_cg(int n, int npad, int j0, int N1,
+                                                               int ncolp,
                                                                const float* __restrict__ Xp,
                                                                const float2* __restrict__ tj1,
                                                                const double* __restrict__ sinv,
@@ -1436,8 +1449,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_apply_epi_cg(int n, int npad, i
   __shared__ double part[EPI_CHUNKS - 1][W][64];
   __shared__ double red[4];
   const int lane = threadIdx.x & 63, ch = threadIdx.x >> 6;
-  const int jp = blockIdx.x * 64 + lane;
-  const int i = jp < n ? jp : -1;  // one configuration: slot = atom
+  const int jp = j0 + blockIdx.x * 64 + lane;  // slots j0 .. of the grid
+  const int i = jp < n ? jp : -1;              // one configuration: slot = atom
   double u[W] = {0.0, 0.0, 0.0};
   if (i >= 0) {
     const int per = (N1 + EPI_CHUNKS - 1) / EPI_CHUNKS;
@@ -1593,6 +1606,8 @@ static bool allocate(MixedState* m) {
   m->x_slices = m->app_slices;
   m->X_cap = (size_t)m->x_slices * np * APP_NCT_E * 16;
   mx_alloc(&m->X, m->X_cap);
+  // rows a partial call leaves alone hold finite values
+  if (m->X && hipMemset(m->X, 0, m->X_cap * sizeof(float)) != hipSuccess) return false;
   mx_alloc(&m->d_split, (size_t)m->nsplit);
   // per column c < CHp of the paired half: element offsets of its E2 and E3
   // rows, (n2 npad, n3 npad), with c clamped to CH - 1 (padding columns
@@ -1631,6 +1646,59 @@ static std::vector<int2> make_splits(MixedState* m) {
   return sp;
 }
 
+// The slot ranges of the dipole-only structure factors and of the field
+// calls' rows (whole 128-slot blocks of one configuration) and the splits of
+// the first, from dj0.
+static bool make_ranges(MixedState* m) {
+  auto blk = [&](int a, int b, int& j0, int& j1) {
+    if (b < 0 || !m->h_seg.empty()) {
+      j0 = 0;
+      j1 = m->npad;
+      return;
+    }
+    j0 = std::min(m->npad, a / 128 * 128);
+    j1 = std::min(m->npad, (b + 127) / 128 * 128);
+    if (j1 <= j0) j1 = std::min(m->npad, j0 + 128);
+    if (j1 <= j0) j0 = j1 - 128;
+  };
+  blk(m->dip0, m->dip1, m->dj0, m->dj1);
+  blk(m->fld0, m->fld1, m->fj0, m->fj1);
+  // as many splits as give one workgroup of mx_rho_w (ncb column blocks per
+  // split) to each CU in one round, in whole 128-slot steps
+  static int cus = 0;
+  if (cus <= 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+  }
+  const int ncb = ((m->CH + 31) / 32 + 3) / 4;
+  const int want = std::max(1, cus / std::max(1, ncb));
+  int sa = std::max(128, ((m->dj1 - m->dj0 + want - 1) / want + 127) / 128 * 128);
+  // no more splits than the partials buffer holds
+  const size_t cap_splits = m->rpart_cap / ((size_t)RHO_NRT2_MAX * 8 * 2 * m->C23p);
+  if ((size_t)((m->dj1 - m->dj0 + sa - 1) / sa) > cap_splits) sa = std::max(sa, m->split_atoms);
+  std::vector<int2> sp;
+  for (int j = m->dj0; j < m->dj1; j += sa) sp.push_back(make_int2(j, std::min(m->dj1, j + sa)));
+  if ((int)sp.size() > m->nsplit_dip_cap) {
+    mx_alloc(&m->d_split_dip, sp.size());
+    if (!m->d_split_dip) return false;
+    m->nsplit_dip_cap = (int)sp.size();
+  }
+  m->nsplit_dip = (int)sp.size();
+  return hipMemcpy(m->d_split_dip, sp.data(), sp.size() * sizeof(int2),
+                   hipMemcpyHostToDevice) == hipSuccess;
+}
+
+bool set_active(MixedState* m, int dip0, int dip1, int fld0, int fld1) {
+  m->dip0 = dip0;
+  m->dip1 = dip1;
+  m->fld0 = fld0;
+  m->fld1 = fld1;
+  return make_ranges(m);
+}
+
 static bool sizes(MixedState* m) {
   CtxView v = view(m->ctx);
   mesh(m, v.kmax);
@@ -1643,7 +1711,8 @@ static bool sizes(MixedState* m) {
   m->nsplit = (int)sp.size();
   if (!allocate(m)) return false;
   return hipMemcpy(m->d_split, sp.data(), sp.size() * sizeof(int2), hipMemcpyHostToDevice) ==
-         hipSuccess;
+             hipSuccess &&
+         make_ranges(m);
 }
 
 MixedState* create_batch(int N, const int kmax[3], const std::vector<MxSeg>& segs_in) {
@@ -1691,7 +1760,7 @@ bool set_tuning(MixedState* m, int app_slices, int split_atoms) {
     m->x_slices = app_slices;
     m->X_cap = (size_t)m->x_slices * m->npad * APP_NCT_E * 16;
     mx_alloc(&m->X, m->X_cap);
-    if (!m->X) return false;
+    if (!m->X || hipMemset(m->X, 0, m->X_cap * sizeof(float)) != hipSuccess) return false;
   }
   m->app_slices = app_slices;
   if (split_atoms == m->split_atoms) return true;
@@ -1707,6 +1776,7 @@ bool set_tuning(MixedState* m, int app_slices, int split_atoms) {
   if (hipMemcpy(m->d_split, sp.data(), sp.size() * sizeof(int2), hipMemcpyHostToDevice) !=
       hipSuccess)
     return false;
+  if (m->h_seg.empty() && !make_ranges(m)) return false;
   return m->h_seg.empty() ||
          hipMemcpy(m->d_seg, m->h_seg.data(), m->h_seg.size() * sizeof(MxSeg),
                    hipMemcpyHostToDevice) == hipSuccess;
@@ -1725,8 +1795,9 @@ MixedState* create(mdffgpu_ctx* c) {
 
 void destroy(MixedState* m) {
   if (!m) return;
-  void* p[] = {m->d_colofs, m->tj1, m->tj2, m->tj3, m->rpart, m->g, m->mq, m->gmax, m->bfrag, m->X,
-               m->Wt,    m->afrag, m->sinv,  m->d_seg, m->d_kseg, m->d_split, m->apart};
+  void* p[] = {m->d_colofs, m->tj1,   m->tj2,  m->tj3,  m->rpart,  m->g,       m->mq,
+               m->gmax,     m->bfrag, m->X,    m->Wt,   m->afrag,  m->sinv,    m->d_seg,
+               m->d_kseg,   m->d_split, m->apart, m->d_split_dip};
   for (void* x : p)
     if (x) (void)hipFree(x);
   delete m;
@@ -1736,50 +1807,58 @@ void on_cell(MixedState* m) {
   if (view(m->ctx).P->lcoulomb) sizes(m);
 }
 
+// the splits of a structure-factor call
+struct Splits {
+  const int2* d;
+  int n;
+};
+
 template <int NRT2, int RTW, bool F64>
-static void launch_rho(MixedState* m, hipStream_t st, int nrt_used) {
+static void launch_rho(MixedState* m, hipStream_t st, int nrt_used, Splits sp) {
   constexpr int CPB = 4 / ((NRT2 + RTW - 1) / RTW);
   const int ctiles = (m->CH + 15) / 16;
   const int ncb = (ctiles + CPB - 1) / CPB;
-  const int used = ncb * m->nsplit;
+  const int used = ncb * sp.n;
   const int grid = (used + NXCD - 1) / NXCD * NXCD;
   mx_rho_gemm<NRT2, RTW, F64><<<grid, 256, 0, st>>>(m->C23, m->CH, m->C23p, nrt_used, ncb, used,
-                                                    m->d_split, m->afrag, m->npad, m->N3, m->tj2,
+                                                    sp.d, m->afrag, m->npad, m->N3, m->tj2,
                                                     m->tj3, m->rpart);
 }
 
 template <int NRT2, bool F64>
-static void launch_rho_gen(MixedState* m, hipStream_t st, int nrt_used) {
+static void launch_rho_gen(MixedState* m, hipStream_t st, int nrt_used, Splits sp) {
   const int ctiles = (m->CH + 15) / 16;
   const int ncb = (ctiles + 3) / 4;
-  const int used = ncb * m->nsplit;
+  const int used = ncb * sp.n;
   const int grid = (used + NXCD - 1) / NXCD * NXCD;
-  mx_rho_gen<NRT2, F64><<<grid, 256, 0, st>>>(m->C23, m->CH, m->C23p, nrt_used, ncb, used,
-                                               m->d_split, m->afrag, m->npad, m->N3, m->tj2,
-                                               m->tj3, reinterpret_cast<RhoPart<F64>*>(m->rpart));
+  mx_rho_gen<NRT2, F64><<<grid, 256, 0, st>>>(m->C23, m->CH, m->C23p, nrt_used, ncb, used, sp.d,
+                                               m->afrag, m->npad, m->N3, m->tj2, m->tj3,
+                                               reinterpret_cast<RhoPart<F64>*>(m->rpart));
 }
 
 template <int NRT32>
-static void launch_rho_w(MixedState* m, hipStream_t st) {
+static void launch_rho_w(MixedState* m, hipStream_t st, Splits sp) {
   const int ctiles = (m->CH + 31) / 32;
   const int ncb = (ctiles + 3) / 4;
-  const int used = ncb * m->nsplit;
+  const int used = ncb * sp.n;
   const int grid = (used + NXCD - 1) / NXCD * NXCD;
-  mx_rho_w<NRT32><<<grid, 256, 0, st>>>(m->C23, m->CH, m->C23p, ncb, used, m->d_split, m->afrag,
+  mx_rho_w<NRT32><<<grid, 256, 0, st>>>(m->C23, m->CH, m->C23p, ncb, used, sp.d, m->afrag,
                                         m->npad, m->N3, m->tj2, m->tj3,
                                         reinterpret_cast<float*>(m->rpart));
 }
 
+// rows j0 .. j1 - 1 (whole row blocks)
 template <int NCT, int RT>
-static void launch_apply(MixedState* m, hipStream_t st) {
+static void launch_apply(MixedState* m, hipStream_t st, int j0, int j1) {
   const int KS = 2 * m->CHp / 32;
   const int S = m->app_slices;
   const int ksper = (KS + S - 1) / S;
   const int rows = APP_WAVES * RT * 16;
-  const int mblocks = m->npad / rows;
+  const int mb0 = j0 / rows;
+  const int mblocks = (j1 - j0) / rows;
   const int per = NXCD / S;
   const int grid = (mblocks + per - 1) / per * NXCD;
-  mx_apply_gemm<NCT, RT><<<grid, 64 * APP_WAVES, 0, st>>>(m->npad, KS, S, ksper, mblocks,
+  mx_apply_gemm<NCT, RT><<<grid, 64 * APP_WAVES, 0, st>>>(m->npad, KS, S, ksper, mb0, mblocks,
                                                          m->d_colofs, m->tj2, m->tj3, m->bfrag,
                                                          m->d_kseg, m->X);
 }
@@ -1810,6 +1889,13 @@ static bool run_kspace(MixedState* m, const KsCall& a, const double* q, const do
   if (mq_in) chg = false;
   const int n = m->N;
   const MxSeg* seg = m->d_seg;
+  // one configuration: the dipole-only structure factors sum over the slots
+  // that can carry a dipole, and a call without energy or EFG (the static
+  // and SCF field calls) gives the field at the rows that need it only
+  const Splits spl = (!seg && !chg && dip) ? Splits{m->d_split_dip, m->nsplit_dip}
+                                           : Splits{m->d_split, m->nsplit};
+  const bool rows_part = !seg && !a.energy && !efg;
+  const int rj0 = rows_part ? m->fj0 : 0, rj1 = rows_part ? m->fj1 : m->npad;
   // phase tables: positions are fixed within an evaluation, built on its
   // first reciprocal call
   if (m->tables_eval != a.epoch) {
@@ -1865,18 +1951,19 @@ static bool run_kspace(MixedState* m, const KsCall& a, const double* q, const do
     // chains, or 4 row tiles), row groups with FP64 flushes otherwise
     if (wide) {
       switch (NRT32) {
-        case 1: launch_rho_w<1>(m, st); break;
-        case 2: launch_rho_w<2>(m, st); break;
-        case 3: launch_rho_w<3>(m, st); break;
-        case 4: launch_rho_w<4>(m, st); break;
-        default: launch_rho_w<5>(m, st); break;
+        case 1: launch_rho_w<1>(m, st, spl); break;
+        case 2: launch_rho_w<2>(m, st, spl); break;
+        case 3: launch_rho_w<3>(m, st, spl); break;
+        case 4: launch_rho_w<4>(m, st, spl); break;
+        default: launch_rho_w<5>(m, st, spl); break;
       }
     } else if (NRT2l == 4)
-      chg ? launch_rho_gen<4, true>(m, st, NRT2) : launch_rho_gen<4, false>(m, st, NRT2);
+      chg ? launch_rho_gen<4, true>(m, st, NRT2, spl) : launch_rho_gen<4, false>(m, st, NRT2, spl);
     else if (NRT2l == 10)
-      chg ? launch_rho<10, 5, true>(m, st, NRT2) : launch_rho_gen<10, false>(m, st, NRT2);
+      chg ? launch_rho<10, 5, true>(m, st, NRT2, spl)
+          : launch_rho_gen<10, false>(m, st, NRT2, spl);
     else
-      launch_rho<13, 7, true>(m, st, NRT2);
+      launch_rho<13, 7, true>(m, st, NRT2, spl);
   }
   const dim3 gred((m->C23 + MDFF_BLOCK - 1) / MDFF_BLOCK, m->N1, m->nseg);
   const int gk = gred.x * gred.y;
@@ -1887,7 +1974,7 @@ static bool run_kspace(MixedState* m, const KsCall& a, const double* q, const do
     const float* rpf = reinterpret_cast<const float*>(m->rpart);
 #define MX_RED(PT, NWC, P)                                                                     \
   mx_rho_reduce<PT, NWC><<<gred, MDFF_BLOCK, 0, st>>>(                                          \
-      m->N1, m->N2, m->N3, m->C23, m->C23p, chg, dip, m->nsplit, NRT2l * 8, P, a.cell, a.alpha, \
+      m->N1, m->N2, m->N3, m->C23, m->C23p, chg, dip, spl.n, NRT2l * 8, P, a.cell, a.alpha,     \
       seg, a.energy, m->g, m->gmax, a.part, mq_in ? m->mq : nullptr, mq_save ? m->mq : nullptr)
     if (!chg && (wide || NRT2l != 13))
       MX_RED(float, 3, rpf);
@@ -1919,41 +2006,42 @@ static bool run_kspace(MixedState* m, const KsCall& a, const double* q, const do
   {
     ProfScope _ps(m->ctx, K_MX_APPLY);
     if (NCTl == APP_NCT_S)
-      launch_apply<APP_NCT_S, APP_RT_S>(m, st);
+      launch_apply<APP_NCT_S, APP_RT_S>(m, st, rj0, rj1);
     else if (NCTl == APP_NCT_F)
-      launch_apply<APP_NCT_F, APP_RT_F>(m, st);
+      launch_apply<APP_NCT_F, APP_RT_F>(m, st, rj0, rj1);
     else
-      launch_apply<APP_NCT_E, APP_RT_E>(m, st);
+      launch_apply<APP_NCT_E, APP_RT_E>(m, st, rj0, rj1);
   }
   if (cg && W == 3 && !seg) {
     // ES_IND of the conjugate-gradient SCF: fold and combine in one kernel
     ProfScope _ps(m->ctx, K_MX_COMBINE);
-    const unsigned gb = (unsigned)((m->npad + 63) / 64);
+    const unsigned gb = (unsigned)((rj1 - rj0) / 64);
     if (m->app_slices == 1)
-      mx_apply_epi_cg<1><<<gb, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, NCTl * 16, m->X, m->tj1,
-                                                    m->sinv, *cg);
+      mx_apply_epi_cg<1><<<gb, MDFF_BLOCK, 0, st>>>(n, m->npad, rj0, m->N1, NCTl * 16, m->X,
+                                                    m->tj1, m->sinv, *cg);
     else if (m->app_slices == 2)
-      mx_apply_epi_cg<2><<<gb, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, NCTl * 16, m->X, m->tj1,
-                                                    m->sinv, *cg);
+      mx_apply_epi_cg<2><<<gb, MDFF_BLOCK, 0, st>>>(n, m->npad, rj0, m->N1, NCTl * 16, m->X,
+                                                    m->tj1, m->sinv, *cg);
     else
-      mx_apply_epi_cg<4><<<gb, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, NCTl * 16, m->X, m->tj1,
-                                                    m->sinv, *cg);
+      mx_apply_epi_cg<4><<<gb, MDFF_BLOCK, 0, st>>>(n, m->npad, rj0, m->N1, NCTl * 16, m->X,
+                                                    m->tj1, m->sinv, *cg);
+    m->cg_parts = (int)gb;
     return true;
   }
   {
     ProfScope _ps(m->ctx, K_MX_COMBINE);
     // the K slices a compile-time count, so that all of a thread's loads
     // issue together
-    const dim3 ge((m->npad + 63) / 64, W);
+    const dim3 ge((rj1 - rj0) / 64, W);
     if (m->app_slices == 1)
-      mx_apply_epi<1><<<ge, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, W, NCTl * 16, seg, m->d_kseg,
-                                                 m->X, m->tj1, m->sinv, apart);
+      mx_apply_epi<1><<<ge, MDFF_BLOCK, 0, st>>>(n, m->npad, rj0, m->N1, W, NCTl * 16, seg,
+                                                 m->d_kseg, m->X, m->tj1, m->sinv, apart);
     else if (m->app_slices == 2)
-      mx_apply_epi<2><<<ge, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, W, NCTl * 16, seg, m->d_kseg,
-                                                 m->X, m->tj1, m->sinv, apart);
+      mx_apply_epi<2><<<ge, MDFF_BLOCK, 0, st>>>(n, m->npad, rj0, m->N1, W, NCTl * 16, seg,
+                                                 m->d_kseg, m->X, m->tj1, m->sinv, apart);
     else
-      mx_apply_epi<4><<<ge, MDFF_BLOCK, 0, st>>>(n, m->npad, m->N1, W, NCTl * 16, seg, m->d_kseg,
-                                                 m->X, m->tj1, m->sinv, apart);
+      mx_apply_epi<4><<<ge, MDFF_BLOCK, 0, st>>>(n, m->npad, rj0, m->N1, W, NCTl * 16, seg,
+                                                 m->d_kseg, m->X, m->tj1, m->sinv, apart);
   }
   return false;
 }
@@ -1971,7 +2059,7 @@ int kspace(MixedState* m, int mode, const double* q, const double* mu, bool chg,
   a.energy = (mode == ES_FINAL);
   a.part = v.part;
   a.sc_rec = v.sc + v.sc_rec;
-  return run_kspace(m, a, q, mu, chg, dip, efg, apart, cg) ? (m->npad + 63) / 64 : 0;
+  return run_kspace(m, a, q, mu, chg, dip, efg, apart, cg) ? m->cg_parts : 0;
 }
 
 float* dipole_weights(MixedState* m, int* npad) {
