@@ -178,9 +178,13 @@ def time_path(args, sys_, precision, frames_dev, scf=None, tuning=None, table=Tr
             flop = per_unit * n * nk
             peak, bound, pipe = FL.PEAK_FP64_TFLOPS, "valu", "fp64 vector ALU"
         elif name in ("mx_rho", "mx_apply"):
-            flop = FL.kspace_algorithmic_flops(name, n, nk, len(sampled), nsc, p.doefg) / cnt
-            executed = FL.mixed_kspace_flops(name, n, list(p.kES), len(sampled), nsc,
-                                             p.doefg) / cnt
+            # the mixed path's dipole-only and field calls run over the
+            # polarizable atoms (no species here has a static dipole)
+            npol = int(sum(1 for t in sys_.itype if p.ldip_polar[t - 1]))
+            flop = FL.kspace_algorithmic_flops(name, n, nk, len(sampled), nsc, p.doefg,
+                                               n_dip=npol, n_fld=npol) / cnt
+            executed = FL.mixed_kspace_flops(name, n, list(p.kES), len(sampled), nsc, p.doefg,
+                                             n_dip=npol, n_fld=npol) / cnt
             peak, bound = FL.PEAK_FP16_MFMA_TFLOPS, "mfma"
             pipe = "fp16 matrix core (operands split in fp16 hi + lo, 3 products)"
     if flop:

@@ -41,37 +41,51 @@ PEAK_HBM_GBS = 8000.0
 
 
 def kspace_algorithmic_flops(kernel: str, natm: int, nk: int, steps: int, n_ind_total: int,
-                             efg: bool) -> float:
+                             efg: bool, n_dip: int = None, n_fld: int = None) -> float:
     """Algorithmic flops of the reciprocal-space sums over `steps`
     evaluations with `n_ind_total` induced-dipole Ewald calls in all, from
     the per-(atom, k) weights above: per evaluation one static call
     (charges), the induced-dipole calls and one final call (charges and
     dipoles; field, and EFG / dipole forces when requested).
       rho:   RHO_CHG, RHO_DIP per SCF call, RHO_BOTH
-      apply: APPLY_FIELD, APPLY_FIELD per SCF call, APPLY_EFG (or FIELD)"""
-    nK = float(natm) * nk
+      apply: APPLY_FIELD, APPLY_FIELD per SCF call, APPLY_EFG (or FIELD)
+    With n_dip and n_fld (the mixed path), the work that path does: its
+    dipole-only structure factors (the SCF calls and the final call, whose
+    charge sums are the static call's) run over the n_dip atoms that can
+    carry a dipole, and the static and SCF field calls over the n_fld
+    polarizable atoms."""
+    if n_dip is None or n_fld is None:
+        nK = float(natm) * nk
+        if kernel in ("mx_rho", "k_rho64"):
+            w = steps * (RHO_CHG + RHO_BOTH) + n_ind_total * RHO_DIP
+        else:
+            w = steps * (APPLY_FIELD + (APPLY_EFG if efg else APPLY_FIELD)) + n_ind_total * APPLY_FIELD
+        return w * nK
     if kernel in ("mx_rho", "k_rho64"):
-        w = steps * (RHO_CHG + RHO_BOTH) + n_ind_total * RHO_DIP
-    else:
-        w = steps * (APPLY_FIELD + (APPLY_EFG if efg else APPLY_FIELD)) + n_ind_total * APPLY_FIELD
-    return w * nK
+        return float(nk) * (steps * RHO_CHG * natm + (n_ind_total + steps) * RHO_DIP * n_dip)
+    final = APPLY_EFG if efg else APPLY_FIELD
+    return float(nk) * ((steps + n_ind_total) * APPLY_FIELD * n_fld + steps * final * natm)
 
 
 def mixed_kspace_flops(kernel: str, natm: int, kmax, steps: int, n_scf_total: int,
-                       efg: bool) -> float:
+                       efg: bool, n_dip: int = None, n_fld: int = None) -> float:
     """Executed matrix-core flops of the mixed k-space GEMMs over a run of
     `steps` evaluations with `n_scf_total` SCF iterations in all: per call
     3 fp16 products of a real GEMM over the conjugate-paired half of the
     (n2, n3) columns, CH = ((2ky+1)(2kz+1) + 1) / 2.
-      mx_rho:   rows 2*NW*(kx+1) (weight, n1, re|im), cols 2*CH, K natm
-      mx_apply: rows natm, K 2*CH, cols 2*W*(kx+1)
-    NW = 1 (static call), 3 (SCF calls), 4 (final call); W = 3, or 6 in the
-    final call when EFGs or dipole forces are needed."""
+      mx_rho:   rows 2*NW*(kx+1) (weight, n1, re|im), cols 2*CH, K atoms
+      mx_apply: rows atoms, K 2*CH, cols 2*W*(kx+1)
+    NW = 1 (static call, all atoms), 3 (SCF calls and the dipole-only final
+    call, over the n_dip atoms); W = 3 (static and SCF calls, over the n_fld
+    atoms), or 6 in the final call (all atoms) when EFGs or dipole forces
+    are needed."""
     kx, ky, kz = kmax
     n1, c23 = kx + 1, (2 * ky + 1) * (2 * kz + 1)
     ch = (c23 + 1) // 2
+    nd = natm if n_dip is None else n_dip
+    nf = natm if n_fld is None else n_fld
     if kernel == "mx_rho":
-        rows = [2 * 1 * n1] * steps + [2 * 3 * n1] * n_scf_total + [2 * 4 * n1] * steps
-        return sum(3 * 2.0 * r * 2 * ch * natm for r in rows)
-    cols = [2 * 3 * n1] * (steps + n_scf_total) + [2 * 6 * n1] * steps
-    return sum(3 * 2.0 * natm * 2 * ch * c for c in cols)
+        calls = [(2 * 1 * n1, natm)] * steps + [(2 * 3 * n1, nd)] * (n_scf_total + steps)
+        return sum(3 * 2.0 * r * 2 * ch * k for r, k in calls)
+    calls = [(2 * 3 * n1, nf)] * (steps + n_scf_total) + [(2 * (6 if efg else 3) * n1, natm)] * steps
+    return sum(3 * 2.0 * rows * 2 * ch * c for c, rows in calls)
