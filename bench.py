@@ -174,13 +174,15 @@ def time_path(args, sys_, precision, frames_dev, scf=None, tuning=None, table=Tr
         avg_s = ms / cnt * 1e-3
         per_unit = FL.KERNEL_FLOPS_PER_UNIT.get(name)
         nsc = sum(nind[k] for k in sampled)
+        # the static and SCF calls' field (both paths) and the mixed path's
+        # dipole-only structure factors run over the polarizable atoms (no
+        # species here has a static dipole)
+        npol = int(sum(1 for t in sys_.itype if p.ldip_polar[t - 1]))
         if per_unit:
-            flop = per_unit * n * nk
+            # the k_apply64<field> launches are the static and SCF calls
+            flop = per_unit * (npol if name == "k_apply64<field>" else n) * nk
             peak, bound, pipe = FL.PEAK_FP64_TFLOPS, "valu", "fp64 vector ALU"
         elif name in ("mx_rho", "mx_apply"):
-            # the mixed path's dipole-only and field calls run over the
-            # polarizable atoms (no species here has a static dipole)
-            npol = int(sum(1 for t in sys_.itype if p.ldip_polar[t - 1]))
             flop = FL.kspace_algorithmic_flops(name, n, nk, len(sampled), nsc, p.doefg,
                                                n_dip=npol, n_fld=npol) / cnt
             executed = FL.mixed_kspace_flops(name, n, list(p.kES), len(sampled), nsc, p.doefg,

@@ -129,6 +129,7 @@ struct RhoArgs {
   Cell cell;
   int atoms_per_split, kpad;
   double2* part;
+  int j_begin, j_end;  // the atoms summed (splits of atoms_per_split from j_begin)
 };
 
 struct RhoRedArgs {
@@ -149,6 +150,7 @@ struct ApplyArgs {
   const double* s;
   const double4* s4;  // the same, one (s1, s2, s3, w) per atom (w: see s4_type)
   double* part;
+  int i_begin, i_end;  // the atoms whose sums are formed (others left as they are)
 };
 
 struct CombineArgs {

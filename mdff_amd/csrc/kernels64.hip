@@ -157,8 +157,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_rho64(RhoArgs a) {
 #pragma unroll
   for (int c = 0; c < KC_MAX; ++c) accR[c] = accI[c] = 0.0;
   const int n = a.n;
-  const int j0 = blockIdx.y * a.atoms_per_split;
-  const int j1 = min(n, j0 + a.atoms_per_split);
+  const int j0 = a.j_begin + blockIdx.y * a.atoms_per_split;
+  const int j1 = min(a.j_end, j0 + a.atoms_per_split);
   const double tpi = 2.0 * MDFF_PI;
   for (int tb = j0; tb < j1; tb += TA) {
     const int m = min(TA, j1 - tb);
@@ -334,8 +334,8 @@ __global__ __launch_bounds__(MDFF_BLOCK) void k_apply64(ApplyArgs a) {
   __shared__ KChunk sc[APPLY_TILE];
   __shared__ double2 sg[APPLY_TILE * KC_MAX];
   const int n = a.n;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = i < n;
+  const int i = a.i_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < a.i_end;
   const int KC = a.KC;
   const int c0 = blockIdx.y * a.chunks_per_split;
   const int c1 = min(a.nchunk, c0 + a.chunks_per_split);

@@ -93,6 +93,9 @@ struct mdffgpu_ctx {
   // FP64 SCF field: the species whose field the SCF reads (dipole or
   // quadrupole polarizable)
   unsigned fieldmask = 0;
+  // FP64 reciprocal sums: the atoms kr0 .. kr1 - 1 (context numbering) hold
+  // every atom of those species (all atoms by default)
+  int kr0 = 0, kr1 = 0;
   int* d_polrows = nullptr;
   // internal numbering of the atoms, fixed at the first evaluation: species
   // by species, by cell of about 4 A (row-major) and caller index within a
@@ -819,7 +822,11 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     ra.atoms_per_split = c->rho_atoms;
     ra.kpad = c->kpad;
     ra.part = c->d_rpart;
-    dim3 g((c->nchunk + MDFF_BLOCK - 1) / MDFF_BLOCK, c->rho_split);
+    // the SCF calls' moments sit on the atoms of the polarizable species
+    ra.j_begin = (mode == ES_IND) ? c->kr0 : 0;
+    ra.j_end = (mode == ES_IND) ? c->kr1 : n;
+    const int nsp = std::max(1, (ra.j_end - ra.j_begin + c->rho_atoms - 1) / c->rho_atoms);
+    dim3 g((c->nchunk + MDFF_BLOCK - 1) / MDFF_BLOCK, nsp);
     size_t lds = (size_t)RHO_TILE * (c->kmax[0] + c->kmax[1] + c->kmax[2] + 3) * sizeof(double2) +
                  10 * RHO_TILE * sizeof(double);
     // the quadrupoles enter rho only together with the dipoles
@@ -848,7 +855,7 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     check_launch(c, "k_rho64");
     RhoRedArgs rr;
     rr.kpad = c->kpad;
-    rr.nsplit = c->rho_split;
+    rr.nsplit = nsp;
     rr.do_energy = (mode == ES_FINAL);
     rr.part = c->d_rpart;
     rr.Ak = c->d_Ak;
@@ -874,7 +881,10 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
     aa.s = c->d_s;
     aa.s4 = c->d_s4;
     aa.part = c->d_apart;
-    dim3 ga(gb, c->app_split);
+    // the static and SCF calls give the field where the SCF reads it only
+    aa.i_begin = (mode != ES_FINAL) ? c->kr0 : 0;
+    aa.i_end = (mode != ES_FINAL) ? c->kr1 : n;
+    dim3 ga(std::max(1, nblk(aa.i_end - aa.i_begin)), c->app_split);
     if (efgv)
       {
         ProfScope _ps(c, K_APPLY_EFG);
@@ -1375,6 +1385,26 @@ void mx_active(mdffgpu_ctx* c) {
     fail(c, MDFFGPU_ERR_ALLOC, "mixed reciprocal-space atom ranges");
 }
 
+// The same for the FP64 reciprocal sums: the SCF calls' moments (induced
+// dipoles and quadrupoles) and the field the SCF reads sit on the atoms of
+// the dipole- or quadrupole-polarizable species, atoms kr0 .. kr1 - 1 (all
+// atoms in the allocation-free context or without such species).
+void k_ranges(mdffgpu_ctx* c) {
+  c->kr0 = 0;
+  c->kr1 = c->n;
+  if (c->noalloc) return;
+  int a0 = -1, a1 = -1;
+  for (int i = 0; i < c->n; ++i)
+    if ((c->fieldmask >> c->typ0[i]) & 1u) {
+      if (a0 < 0) a0 = i;
+      a1 = i + 1;
+    }
+  if (a0 >= 0) {
+    c->kr0 = a0;
+    c->kr1 = a1;
+  }
+}
+
 // Per-atom species data of typeinfo_init (src/read_pos.f90:175-199) in the
 // context's numbering (typ0): species, charge, static dipole and, with
 // quadrupoles, the permanent quadrupole (quadia, :197).
@@ -1440,6 +1470,7 @@ void ensure_renum(mdffgpu_ctx* c, const double* rx, const double* ry, const doub
   c->typ0.swap(t);
   load_species(c);
   mx_active(c);
+  k_ranges(c);
   c->renum = true;
   const size_t nh = (size_t)3 * K * n;
   k_perm_gather<<<(int)((nh + MDFF_BLOCK - 1) / MDFF_BLOCK), MDFF_BLOCK, 0, c->st>>>(
@@ -1928,6 +1959,8 @@ void efg_frame(mdffgpu_ctx* c, const mdffgpu_efg_opts& o, mdffgpu_energies* en) 
       ra.atoms_per_split = c->rho_atoms;
       ra.kpad = c->kpad;
       ra.part = c->d_rpart;
+      ra.j_begin = 0;
+      ra.j_end = n;
       dim3 g((c->nchunk + MDFF_BLOCK - 1) / MDFF_BLOCK, c->rho_split);
       size_t lds = (size_t)RHO_TILE * (c->kmax[0] + c->kmax[1] + c->kmax[2] + 3) *
                        sizeof(double2) + 4 * RHO_TILE * sizeof(double);
@@ -1965,6 +1998,8 @@ void efg_frame(mdffgpu_ctx* c, const mdffgpu_efg_opts& o, mdffgpu_energies* en) 
       aa.s = c->d_s;
       aa.s4 = c->d_s4;
       aa.part = c->d_apart;
+      aa.i_begin = 0;
+      aa.i_end = n;
       {
         ProfScope _ps(c, K_APPLY_EFG);
         k_apply64<true><<<dim3(gb, c->app_split), MDFF_BLOCK, 0, st>>>(aa);
@@ -2097,6 +2132,7 @@ void create_impl(const mdffgpu_params* params, const int32_t* itype, const doubl
     build_tables(c);
     if (c->P.lcoulomb) setup_quad_wfc(c);
     load_species(c);
+    k_ranges(c);
     update_cutoffs(c);
     if (c->list_cut > 0.0) set_list_capacity(c, initial_capacity(c));
     if (c->P.lcoulomb) build_kmesh(c);
