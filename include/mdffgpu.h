@@ -439,8 +439,10 @@ int mdffgpu_profile_only(mdffgpu_ctx* ctx, int32_t kernel);
 
 /* Launch shape of the mixed path's matrix-core reciprocal sums (picked by
  * the Python autotuner): K slices of the field apply (1, 2 or 4) and atoms
- * per structure-factor split (a multiple of 128).  MDFFGPU_ERR_UNSUPPORTED
- * on an FP64 context.  No counterpart in the reference. */
+ * per structure-factor split of the sums with charges (a multiple of 128;
+ * the dipole-only sums of the SCF and final calls size their splits to fill
+ * the device in one round).  MDFFGPU_ERR_UNSUPPORTED on an FP64 context.
+ * No counterpart in the reference. */
 int mdffgpu_set_mixed_tuning(mdffgpu_ctx* ctx, int32_t apply_slices, int32_t split_atoms);
 int mdffgpu_profile_read(const mdffgpu_ctx* ctx, int32_t kernel,
                          double* total_ms, int64_t* launches);
