@@ -117,6 +117,9 @@ struct EsDirArgs {
   // polarizable ones; the dipole field is needed at them only and only
   // their dipoles are nonzero)
   unsigned polmask;
+  // centre rows visited (the ranks listed, nrows of them), or all rows
+  const int* rows;
+  int nrows;
 };
 
 struct RhoArgs {

@@ -712,8 +712,16 @@ void es_call(mdffgpu_ctx* c, int mode, const double* q, const double* mu, int ta
   }
   if (mode == ES_STATIC) {
     ProfScope _ps(c, mx ? K_MX_DIR : K_DIR_STATIC);
+    // FP32 static pass with the FP32 SCF field: the rows of the polarizable
+    // atoms only (the static field is read there only, and their rows hold
+    // the SCF pair table)
+    const bool prow = mx && c->linduced && scf_field_fp32(c) && c->npolrows > 0;
+    if (prow) {
+      da.rows = c->d_polrows;
+      da.nrows = c->npolrows;
+    }
     if (mx)
-      k_es_dir<ES_STATIC, float><<<gbl, MDFF_BLOCK, 0, st>>>(da);
+      k_es_dir<ES_STATIC, float><<<prow ? nblk(c->npolrows * LPA) : gbl, MDFF_BLOCK, 0, st>>>(da);
     else
       k_es_dir<ES_STATIC, double><<<gbl, MDFF_BLOCK, 0, st>>>(da);
   } else if (mode == ES_IND) {

@@ -705,11 +705,12 @@ __global__ __launch_bounds__(MDFF_BLOCK) void mx_rho_afrag32(int npad, int N1, i
                                                               const float2* __restrict__ tj1,
                                                               u32x4* __restrict__ afrag,
                                                               unsigned int* __restrict__ gmax,
-                                                              int nreset) {
+                                                              int nreset, int jb0, int jb1) {
   if (blockIdx.x == 0)
     for (int t = threadIdx.x; t < nreset; t += blockDim.x) gmax[t] = 0u;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t total = (size_t)(npad / 32) * 2 * NRT32 * 64;
+  // the fragments of slots jb0 .. jb1 - 1 (multiples of 32)
+  const size_t t = (size_t)(jb0 / 32) * 2 * NRT32 * 64 + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)(jb1 / 32) * 2 * NRT32 * 64;
   if (t >= total) return;
   const int l = (int)(t & 63);
   const size_t f = t >> 6;  // (g * 2 + s) * NRT32 + rt
@@ -1922,9 +1923,14 @@ static bool run_kspace(MixedState* m, const KsCall& a, const double* q, const do
     const unsigned ga = (unsigned)((at + MDFF_BLOCK - 1) / MDFF_BLOCK);
     const int nreset = m->nseg * m->N1;
     if (wide) {
+      // the fragments of the slots the structure-factor splits cover
+      const int jb0 = (spl.d == m->d_split_dip) ? m->dj0 : 0;
+      const int jb1 = (spl.d == m->d_split_dip) ? m->dj1 : m->npad;
+      const size_t atw = (size_t)((jb1 - jb0) / 32) * NRT2l * 64;
+      const unsigned gw = (unsigned)std::max<size_t>(1, (atw + MDFF_BLOCK - 1) / MDFF_BLOCK);
 #define MX_AF32(K)                                                                           \
-  mx_rho_afrag32<K><<<ga, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag, \
-                                               m->gmax, nreset)
+  mx_rho_afrag32<K><<<gw, MDFF_BLOCK, 0, st>>>(m->npad, m->N1, NW, m->Wt, m->tj1, m->afrag, \
+                                               m->gmax, nreset, jb0, jb1)
       switch (NRT32) {
         case 1: MX_AF32(1); break;
         case 2: MX_AF32(2); break;

@@ -214,7 +214,9 @@ __global__ __launch_bounds__(MDFF_BLOCK) __attribute__((amdgpu_waves_per_eu(
   static_assert(LPA <= 32 && 64 % LPA == 0, "lane groups inside one wave");
   int tbase = 0;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i0 = tid / LPA, sub = tid % LPA;
+  const int row_v = tid / LPA, sub = tid % LPA;
+  // processing row: the v-th listed rank (rows past the list inactive) or v
+  const int i0 = a.rows ? (row_v < a.nrows ? a.rows[row_v] : n) : row_v;
   const int i = (a.perm && i0 < n) ? a.perm[i0] : i0;  // atoms in spatial order
   const int cf = (BATCH && i < n) ? a.cfg[i] : 0;
   const Cell& cell = BATCH ? a.bc[cf].cell : a.cell;
