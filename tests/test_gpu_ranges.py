@@ -69,3 +69,28 @@ def test_first_species_polarizable_mixed_vs_fp64(solver):
           f"n_scf {g['n_scf']} (fp64 {o['n_scf']})", flush=True)
     assert np.abs(o["mu"][s.itype == 2]).max() == 0.0  # O carries no dipole
     assert ef <= 1e-5 and em <= 1e-5 and eg <= 1e-5, (ef, em, eg)
+
+
+@pytest.mark.parametrize("slices", [1, 4])
+def test_apply_slice_counts_vs_fp64(slices):
+    """The other apply K-slice counts on the polarizable-first melt (their
+    row blocks cover the polarizable rows only), against FP64."""
+    from mdff_amd.field import ForceField
+    s = _si_polar(500, 1000, seed=97, rc=8.0, kmax=8)
+    s.params.conv_tol_ind = 1e-8
+    p64 = copy.deepcopy(s.params)
+    p64.precision = P.PREC_FP64
+    pmx = copy.deepcopy(s.params)
+    pmx.precision = P.PREC_MIXED
+    f64 = ForceField(p64, s.itype, s.A)
+    fmx = ForceField(pmx, s.itype, s.A)
+    try:
+        assert fmx.set_mixed_tuning(slices, 512)
+        o = f64.engforce(s.r, itime=1)
+        g = fmx.engforce(s.r, itime=1)
+    finally:
+        f64.close()
+        fmx.close()
+    ef, eg = rel_rms(g["f"], o["f"]), rel_rms(g["efg"], o["efg"])
+    print(f"apply slices {slices}: force {ef:.2e} efg {eg:.2e}", flush=True)
+    assert ef <= 1e-5 and eg <= 1e-5, (ef, eg)
